@@ -1,0 +1,259 @@
+"""Python mirror of the reconstruction entry points, over the C ABI.
+
+torch tensors are used only as device memory and for the current HIP stream;
+every computation runs in libfbp.so's HIP kernels.  Names follow the
+reference's call surface (FB/Laser.cuh:41-47, FB/PostprocessingFilters.h,
+FB/LCT.h, python/lct.py).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import (CONFOCAL, EXHAUSTIVE, LAYOUT_TIME_FASTEST, LAYOUT_TIME_MAJOR, MIS_REFERENCE,
+                   MIS_UNBIASED, STORAGE_BF16, STORAGE_F16, STORAGE_F32, FbpError, LctParams,
+                   MisParams, ReconstructionInfo, Transient, UVec3, Vec3)
+
+__all__ = [
+    "Context", "Geometry", "make_info", "CONFOCAL", "EXHAUSTIVE", "STORAGE_F32", "STORAGE_F16",
+    "STORAGE_BF16", "LAYOUT_TIME_FASTEST", "LAYOUT_TIME_MAJOR", "MIS_REFERENCE", "MIS_UNBIASED",
+    "FbpError",
+]
+
+
+def _ptr(t: torch.Tensor | None) -> int | None:
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("tensor must live on the GPU")
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return t.data_ptr()
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+@dataclass
+class Geometry:
+    """Host description of a capture, the fields NLosData::toGpu copies into
+    ReconstructionInfo (FB/NLosData.cpp:131-159)."""
+    laser_targets: np.ndarray            # (L, 3) float32
+    sensor_targets: np.ndarray           # (S, 3) float32 (== laser for confocal)
+    num_time_bins: int
+    time_step: float                     # metres per bin
+    hidden_min: tuple
+    hidden_max: tuple
+    voxel_resolution: tuple              # (X, Y, Z)
+    confocal: bool = True
+    time_offset: float = 0.0
+    discard_first_last_bounces: int = 1
+    laser_position: tuple = (0.0, 0.0, 0.0)
+    sensor_position: tuple = (0.0, 0.0, 0.0)
+    wall_normal: tuple = (0.0, 1.0, 0.0)
+    wall_size: tuple = (2.0, 2.0)
+
+
+def make_info(g: Geometry) -> ReconstructionInfo:
+    info = ReconstructionInfo()
+    f32 = np.float32
+    hmin = np.asarray(g.hidden_min, f32)
+    hmax = np.asarray(g.hidden_max, f32)
+    res = np.asarray(g.voxel_resolution, np.uint32)
+    size = (hmax - hmin).astype(f32)
+    vs = (size / res.astype(f32)).astype(f32)
+    info.sensor_position = Vec3(*map(float, np.asarray(g.sensor_position, f32)))
+    info.capture_system = CONFOCAL if g.confocal else EXHAUSTIVE
+    info.laser_position = Vec3(*map(float, np.asarray(g.laser_position, f32)))
+    info.time_offset = float(f32(g.time_offset))
+    info.num_laser_targets = len(g.laser_targets)
+    info.num_sensor_targets = len(g.sensor_targets)
+    info.num_time_bins = int(g.num_time_bins)
+    info.time_step = float(f32(g.time_step))
+    info.hidden_volume_min = Vec3(*map(float, hmin))
+    info.discard_first_last_bounces = int(g.discard_first_last_bounces)
+    info.hidden_volume_max = Vec3(*map(float, hmax))
+    info.hidden_volume_size = Vec3(*map(float, size))
+    info.hidden_volume_voxel_size = Vec3(*map(float, vs))
+    wx, wz = (f32(g.wall_size[0]), f32(g.wall_size[1]))
+    info.relay_wall_min_position = Vec3(float(-wx / f32(2)), 0.0, float(-wz / f32(2)))
+    info.relay_wall_size = Vec3(float(wx), 0.0, float(wz))
+    info.relay_wall_normal = Vec3(*map(float, np.asarray(g.wall_normal, f32)))
+    info.voxel_resolution = UVec3(*map(int, res))
+    return info
+
+
+class Context:
+    """Owns an fbp_context (scratch memory, FFT plans) on one device."""
+
+    def __init__(self, device: int | None = None):
+        self.lib = _lib.load()
+        if device is None:
+            device = torch.cuda.current_device()
+        h = C.c_void_p()
+        st = self.lib.fbp_context_create(int(device), C.byref(h))
+        if st != 0:
+            raise FbpError(st, "fbp_context_create failed")
+        self.handle = h
+        self.device = int(device)
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.fbp_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, st: int) -> None:
+        if st != 0:
+            msg = self.lib.fbp_context_last_error(self.handle).decode()
+            raise FbpError(st, msg)
+
+    # ------------------------------------------------------------ helpers
+    def _dev(self) -> torch.device:
+        return torch.device("cuda", self.device)
+
+    @staticmethod
+    def _transient(data: torch.Tensor, storage: int, layout: int,
+                   scales: torch.Tensor | None) -> Transient:
+        return Transient(_ptr(data), storage, layout, _ptr(scales))
+
+    def _volume(self, info: ReconstructionInfo, out: torch.Tensor | None) -> torch.Tensor:
+        r = info.voxel_resolution
+        shape = (r.z, r.y, r.x)
+        if out is None:
+            out = torch.empty(shape, dtype=torch.float32, device=self._dev())
+        if tuple(out.shape) != shape or out.dtype != torch.float32:
+            raise ValueError(f"volume must be float32 {shape}")
+        return out
+
+    # ------------------------------------------------------ backprojection
+    def backproject_confocal(self, info, intensity, laser_targets, *, storage=STORAGE_F32,
+                             layout=LAYOUT_TIME_FASTEST, scales=None, out=None):
+        """backprojectConfocalVoxel (FB/transient_reconstruction.cuh:93-136)."""
+        vol = self._volume(info, out)
+        tr = self._transient(intensity, storage, layout, scales)
+        self.check(self.lib.fbp_backproject_confocal_voxel(
+            self.handle, C.byref(info), C.byref(tr), _ptr(laser_targets), _ptr(vol), _stream()))
+        return vol
+
+    def backproject_exhaustive(self, info, intensity, laser_targets, sensor_targets, *,
+                               storage=STORAGE_F32, scales=None, out=None):
+        """backprojectExhaustiveVoxel (FB/transient_reconstruction.cuh:138-166)."""
+        vol = self._volume(info, out)
+        tr = self._transient(intensity, storage, LAYOUT_TIME_FASTEST, scales)
+        self.check(self.lib.fbp_backproject_exhaustive_voxel(
+            self.handle, C.byref(info), C.byref(tr), _ptr(laser_targets), _ptr(sensor_targets),
+            _ptr(vol), _stream()))
+        return vol
+
+    def backproject_depth(self, info, intensity, laser_targets, depths, sensor_targets=None,
+                          out=None):
+        """backprojectConfocal / backprojectExhaustive (depth planes)."""
+        D = int(depths.numel())
+        L = info.num_laser_targets
+        if out is None:
+            out = torch.empty((L, D), dtype=torch.float32, device=self._dev())
+        tr = self._transient(intensity, STORAGE_F32, LAYOUT_TIME_FASTEST, None)
+        if info.capture_system == CONFOCAL:
+            st = self.lib.fbp_backproject_depth_confocal(
+                self.handle, C.byref(info), C.byref(tr), _ptr(laser_targets), _ptr(depths), D,
+                _ptr(out), _stream())
+        else:
+            st = self.lib.fbp_backproject_depth_exhaustive(
+                self.handle, C.byref(info), C.byref(tr), _ptr(laser_targets),
+                _ptr(sensor_targets), _ptr(depths), D, _ptr(out), _stream())
+        self.check(st)
+        return out
+
+    def backproject_confocal_mis(self, info, intensity, laser_targets, noise, num_samples,
+                                 mode=MIS_REFERENCE, out=None):
+        """Laser::reconstructAABBConfocalMIS (FB/Laser.cu:526-611)."""
+        vol = self._volume(info, out)
+        tr = self._transient(intensity, STORAGE_F32, LAYOUT_TIME_FASTEST, None)
+        p = MisParams(int(num_samples), int(mode), _ptr(noise), int(noise.numel()))
+        self.check(self.lib.fbp_backproject_confocal_mis(
+            self.handle, C.byref(info), C.byref(tr), _ptr(laser_targets), C.byref(p), _ptr(vol),
+            _stream()))
+        return vol
+
+    def quantize(self, intensity: torch.Tensor, storage: int):
+        """Per-row scaled F16/BF16 copy of a [rows][T] float32 transient."""
+        rows = intensity.numel() // intensity.shape[-1]
+        cols = intensity.shape[-1]
+        q = torch.empty(intensity.shape, dtype=torch.int16, device=intensity.device)
+        sc = torch.empty((rows,), dtype=torch.float32, device=intensity.device)
+        self.check(self.lib.fbp_quantize_transient(self.handle, _ptr(intensity), rows, cols,
+                                                   storage, _ptr(q), _ptr(sc), _stream()))
+        return q, sc
+
+    # --------------------------------------------------------------- filters
+    def laplacian(self, volume, kernel_size=5, out=None):
+        Z, Y, X = volume.shape
+        out = torch.empty_like(volume) if out is None else out
+        self.check(self.lib.fbp_filter_laplacian(self.handle, _ptr(volume), _ptr(out),
+                                                 UVec3(X, Y, Z), kernel_size, _stream()))
+        return out
+
+    def log(self, volume, kernel_size=5, sigma=1.0, out=None):
+        Z, Y, X = volume.shape
+        out = torch.empty_like(volume) if out is None else out
+        self.check(self.lib.fbp_filter_log(self.handle, _ptr(volume), _ptr(out), UVec3(X, Y, Z),
+                                           kernel_size, float(sigma), _stream()))
+        return out
+
+    def log_fft(self, volume, sigma=1.0):
+        Z, Y, X = volume.shape
+        self.check(self.lib.fbp_filter_log_fft(self.handle, _ptr(volume), UVec3(X, Y, Z),
+                                               float(sigma), _stream()))
+        return volume
+
+    def normalize(self, v):
+        self.check(self.lib.fbp_normalize_minmax(self.handle, _ptr(v), v.numel(), _stream()))
+        return v
+
+    def minmax(self, v):
+        mn, mx = C.c_float(), C.c_float()
+        self.check(self.lib.fbp_minmax(self.handle, _ptr(v), v.numel(), C.byref(mn), C.byref(mx),
+                                       _stream()))
+        return mn.value, mx.value
+
+    def filter_h(self, data, delta_t, wl_mean, wl_sigma=0.0, border="zero"):
+        """Laser::filter_H_cuda (FB/Laser.cu:128-240), in place on data[..., T]."""
+        nt = data.shape[-1]
+        batch = data.numel() // nt
+        b = {"zero": 0, "constant": 0, "edge": 1}[border]
+        self.check(self.lib.fbp_filter_h(self.handle, _ptr(data), batch, nt, float(delta_t),
+                                         float(wl_mean), float(wl_sigma), b, _stream()))
+        return data
+
+    def lct(self, data, width, range_max, snr=0.8, is_diffuse=False, is_backprop=False,
+            z_trim=0, out=None):
+        """python/lct.py cnlos_reconstruction steps 1-4 on data[N][N][M]."""
+        N, N2, M = data.shape
+        if N != N2:
+            raise ValueError("data must be (N, N, M)")
+        p = LctParams(N, M, float(width), float(range_max), float(snr), int(is_diffuse),
+                      int(is_backprop), int(z_trim))
+        if out is None:
+            out = torch.empty((M, N, N), dtype=torch.float32, device=data.device)
+        self.check(self.lib.fbp_lct(self.handle, _ptr(data), C.byref(p), _ptr(out), _stream()))
+        return out
+
+    def render_confocal(self, scan_points, points, albedo, num_time_bins, delta, out=None):
+        L = scan_points.shape[0]
+        if out is None:
+            out = torch.empty((L, num_time_bins), dtype=torch.float32, device=scan_points.device)
+        self.check(self.lib.fbp_render_confocal(self.handle, _ptr(scan_points), L, _ptr(points),
+                                                _ptr(albedo), points.shape[0], num_time_bins,
+                                                float(delta), _ptr(out), _stream()))
+        return out

@@ -1,0 +1,376 @@
+// Confocal voxel backprojection (gather form).
+//
+// Reference: backprojectConfocalVoxel (FB/transient_reconstruction.cuh:93-136)
+// launched by Laser::reconstructAABBConfocal (FB/Laser.cu:482-498) with one
+// thread and one float atomicAdd per (voxel, scan point).
+//
+// Here one wave owns a 16^3 voxel tile (64 voxels per lane in registers) and a
+// chunk of scan points.  For every scan point the wave stages the window of
+// time bins its tile can reach into LDS (thresholded at I > 1e-5, zero outside
+// [0, T)), then every lane gathers from that window for its 64 voxels.  The
+// per-scan-point squared distances are built from per-axis exact tables in
+// glm::dot's order, so s is the reference's float value; the bin is found
+// with v_rsq and checked against the nearest integer: when the fast quotient
+// is within the proven error bound of a bin edge, the lane recomputes the
+// bin with the IEEE sequence of chain.h.  Chunk partial sums are reduced in
+// FP64 in a fixed order, so results are deterministic.
+#include "bp_common.h"
+
+namespace fbp {
+
+struct ConfocalArgs {
+    const void* data;
+    const float* scales;
+    const fbp_vec3* laser;
+    float* partial;
+    VolumeGeom g;
+    uint32_t L, T;
+    float t0, dt, inv_dt;
+    uint32_t discard;
+    float lpos[3], spos[3];
+    uint32_t chunk_len;
+    int batch;
+    int W;
+};
+
+// Per scan point parameters staged in LDS (8 words each).
+enum { P_LX = 0, P_LY, P_LZ, P_BASE, P_HME, P_COFF, P_NEAR, P_OFF };
+
+// Fast estimate of (bin, weight) from the squared distance s (pre-scaled by 4
+// when PURE).  Returns true when the estimate may round to a different bin
+// than the IEEE chain.  The fix-up pass calls this again on the same s, so
+// the fast contribution it removes is bit-identical to the one added.
+template <bool PURE, bool NEAR>
+__device__ __forceinline__ bool confocal_fast(float s, float hme, float coff, float inv_dt, int& k,
+                                              float& w) {
+    if constexpr (PURE) {
+        const float rs = __builtin_amdgcn_rsqf(s);
+        const float q = (s * rs) * inv_dt;  // ~ 2 r / dt
+        k = static_cast<int>(q);
+        const float f = __builtin_amdgcn_fractf(q);
+        w = rs * rs;                        // ~ 1 / (2r)^2
+        bool flag = fabsf(f - 0.5f) > hme;
+        if constexpr (NEAR) flag = flag || (s < 2.0e-5f);
+        return flag;
+    } else {
+        const float rs = __builtin_amdgcn_rsqf(s);
+        const float dist = fmaf(s * rs, 2.0f, coff);
+        const float q = dist * inv_dt;
+        k = static_cast<int>(q);
+        const float f = __builtin_amdgcn_fractf(q);
+        const float d2 = dist * dist;
+        w = __builtin_amdgcn_rcpf(d2);
+        bool flag = fabsf(f - 0.5f) > hme;
+        if constexpr (NEAR) flag = flag || (d2 < 2.0e-5f);
+        return flag;
+    }
+}
+
+template <bool PURE, bool NEAR>
+__device__ __forceinline__ void confocal_update_tile(float (&acc)[64], const float (&A)[4][4],
+                                                     const float (&Z2)[4], const float* win,
+                                                     int base, float hme, float coff,
+                                                     const ConfocalArgs& a, uint32_t& m0,
+                                                     uint32_t& m1) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int slot = (i * 4 + j) * 4 + m;
+                // PURE: tables are pre-scaled by 4, so s = 4 * (exact s) exactly.
+                const float s = A[i][j] + Z2[m];
+                int k;
+                float w;
+                if (__builtin_expect(confocal_fast<PURE, NEAR>(s, hme, coff, a.inv_dt, k, w), 0)) {
+                    if (slot < 32)
+                        m0 |= 1u << slot;
+                    else
+                        m1 |= 1u << (slot - 32);
+                }
+                acc[slot] = fmaf(win[base + k], w, acc[slot]);
+            }
+        }
+    }
+}
+
+template <int STORAGE, int LAYOUT, bool PURE>
+__global__ __launch_bounds__(64, 2) void confocal_voxel_kernel(ConfocalArgs a) {
+    extern __shared__ float smem[];
+    float* win = smem;
+    float* prm = smem + a.batch * a.W;
+
+    const int lane = threadIdx.x;
+    const uint32_t tile = blockIdx.x % a.g.num_tiles;
+    const uint32_t chunk = blockIdx.x / a.g.num_tiles;
+    const uint32_t tx = tile % a.g.tiles[0];
+    const uint32_t ty = (tile / a.g.tiles[0]) % a.g.tiles[1];
+    const uint32_t tz = tile / (a.g.tiles[0] * a.g.tiles[1]);
+    const uint32_t X0 = tx * a.g.tile[0], Y0 = ty * a.g.tile[1], Z0 = tz * a.g.tile[2];
+    const int lx = lane & 3, lz = (lane >> 2) & 3, ly = lane >> 4;
+
+    // World coordinates of the lane's voxel centres: vx -> world x,
+    // z -> world y (depth), vy -> world z (FB/transient_reconstruction.cuh:106).
+    float px[4], py[4], pz[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        px[i] = voxel_center(a.g.hmin[0], a.g.vs[0], X0 + lx + 4 * i);
+        py[i] = voxel_center(a.g.hmin[1], a.g.vs[1], Z0 + lz + 4 * i);
+        pz[i] = voxel_center(a.g.hmin[2], a.g.vs[2], Y0 + ly + 4 * i);
+    }
+    float lo[3], hi[3];
+    tile_box(a.g, X0, Y0, Z0, lo, hi);
+
+    float acc[64];
+#pragma unroll
+    for (int i = 0; i < 64; ++i) acc[i] = 0.f;
+    // This wave's slice of the partial buffer doubles as its correction
+    // accumulator; every element is owned by exactly one lane.
+    float* corr = a.partial + (size_t(chunk) * a.g.num_tiles + tile) * 64 * 64;
+#pragma unroll
+    for (int s = 0; s < 64; ++s) corr[s * 64 + lane] = 0.f;
+
+    const uint32_t l_begin = chunk * a.chunk_len;
+    const uint32_t l_end = min(a.L, l_begin + a.chunk_len);
+    const float tscale = PURE ? 4.0f : 1.0f;
+
+    for (uint32_t lb = l_begin; lb < l_end; lb += a.batch) {
+        const int nb = int(min(uint32_t(a.batch), l_end - lb));
+        __syncthreads();
+        if (lane < nb) {
+            const uint32_t l = lb + lane;
+            const float p[3] = {a.laser[l].x, a.laser[l].y, a.laser[l].z};
+            float off = 0.f;
+            if (a.discard == 0)
+                off = rn_add(distance3(p[0], p[1], p[2], a.lpos[0], a.lpos[1], a.lpos[2]),
+                             distance3(p[0], p[1], p[2], a.spos[0], a.spos[1], a.spos[2]));
+            const float rmin = box_min_dist(p, lo, hi);
+            const float rmax = box_max_dist(p, lo, hi);
+            const float dmin = 2.f * rmin - a.t0 + off;
+            const float dmax = 2.f * rmax - a.t0 + off;
+            const int kmin = int(floorf(dmin * a.inv_dt)) - 3;
+            // Error bound of the fast quotient against the IEEE chain: a few
+            // ulps of the largest magnitude entering dist, taken generously.
+            const float mag = 2.f * rmax + fabsf(a.t0) + fabsf(off);
+            const float eps = 16.f * 1.1920929e-7f * mag * a.inv_dt + 1.0e-6f;
+            float* P = prm + lane * 8;
+            P[P_LX] = p[0];
+            P[P_LY] = p[1];
+            P[P_LZ] = p[2];
+            P[P_BASE] = __int_as_float(lane * a.W - kmin);
+            P[P_HME] = 0.5f - eps;
+            P[P_COFF] = off - a.t0;
+            P[P_NEAR] = (dmin < 0.0045f && dmax > -0.0045f) ? 1.f : 0.f;
+            P[P_OFF] = off;
+        }
+        __syncthreads();
+        for (int jl = 0; jl < nb; ++jl) {
+            const uint32_t l = lb + jl;
+            const int kmin = jl * a.W - __float_as_int(prm[jl * 8 + P_BASE]);
+            float sc = 1.f;
+            if constexpr (STORAGE != FBP_STORAGE_F32) sc = a.scales[l];
+            for (int w = lane; w < a.W; w += 64) {
+                const int k = kmin + w;
+                float v = 0.f;
+                if (k >= 0 && k < int(a.T)) {
+                    const size_t idx = (LAYOUT == FBP_LAYOUT_TIME_FASTEST)
+                                           ? size_t(l) * a.T + uint32_t(k)
+                                           : size_t(uint32_t(k)) * a.L + l;
+                    v = load_sample<STORAGE>(a.data, idx, sc);
+                    // Quantised storage was thresholded before quantisation.
+                    if constexpr (STORAGE == FBP_STORAGE_F32) v = (v > kEps) ? v : 0.f;
+                }
+                win[jl * a.W + w] = v;
+            }
+        }
+        __syncthreads();
+        for (int jl = 0; jl < nb; ++jl) {
+            const float* P = prm + jl * 8;
+            const float lxp = rfl(P[P_LX]), lyp = rfl(P[P_LY]), lzp = rfl(P[P_LZ]);
+            const int base = rfl(__float_as_int(P[P_BASE]));
+            const float hme = rfl(P[P_HME]);
+            const float coff = rfl(P[P_COFF]);
+            const bool near = rfl(P[P_NEAR]) != 0.f;
+            const float off = rfl(P[P_OFF]);
+            float X2[4], Y2[4], Z2[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                X2[i] = sq(rn_sub(px[i], lxp)) * tscale;
+                Y2[i] = sq(rn_sub(py[i], lyp)) * tscale;
+                Z2[i] = sq(rn_sub(pz[i], lzp)) * tscale;
+            }
+            float A[4][4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) A[i][j] = rn_add(X2[i], Y2[j]);
+            uint32_t m0 = 0, m1 = 0;
+            if (near)
+                confocal_update_tile<PURE, true>(acc, A, Z2, win, base, hme, coff, a, m0, m1);
+            else
+                confocal_update_tile<PURE, false>(acc, A, Z2, win, base, hme, coff, a, m0, m1);
+            // Fix-up of the few voxels whose fast bin may be off by one: the
+            // exact contribution replaces the fast one in the lane-owned
+            // correction slot of the partial buffer.
+            if (__builtin_expect(__any(int((m0 | m1) != 0)), 0)) {
+                while (m0 | m1) {
+                    int slot;
+                    if (m0) {
+                        slot = __builtin_ctz(m0);
+                        m0 &= m0 - 1;
+                    } else {
+                        slot = 32 + __builtin_ctz(m1);
+                        m1 &= m1 - 1;
+                    }
+                    const int i = slot >> 4, j = (slot >> 2) & 3, m = slot & 3;
+                    const float pxi = i == 0 ? px[0] : i == 1 ? px[1] : i == 2 ? px[2] : px[3];
+                    const float pyj = j == 0 ? py[0] : j == 1 ? py[1] : j == 2 ? py[2] : py[3];
+                    const float pzm = m == 0 ? pz[0] : m == 1 ? pz[1] : m == 2 ? pz[2] : pz[3];
+                    const float s = rn_add(rn_add(sq(rn_sub(pxi, lxp)) * tscale,
+                                                  sq(rn_sub(pyj, lyp)) * tscale),
+                                           sq(rn_sub(pzm, lzp)) * tscale);
+                    int kf;
+                    float wf;
+                    if (near)
+                        confocal_fast<PURE, true>(s, hme, coff, a.inv_dt, kf, wf);
+                    else
+                        confocal_fast<PURE, false>(s, hme, coff, a.inv_dt, kf, wf);
+                    const float dist = confocal_dist(PURE ? s * 0.25f : s, a.t0, a.discard, off);
+                    const int ke = bin_of(dist, a.dt);
+                    const float we = safe_rcp(rn_mul(dist, dist));
+                    const float delta = win[base + ke] * we - win[base + kf] * wf;
+                    corr[slot * 64 + lane] += delta;
+                }
+            }
+        }
+    }
+
+#pragma unroll
+    for (int s = 0; s < 64; ++s) corr[s * 64 + lane] += acc[s];
+}
+
+__global__ void reduce_partials_kernel(const float* __restrict__ partial, float* __restrict__ volume,
+                                       VolumeGeom g, uint32_t num_chunks, double scale) {
+    const size_t n = size_t(g.res[0]) * g.res[1] * g.res[2];
+    const size_t v = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const uint32_t vx = uint32_t(v % g.res[0]);
+    const uint32_t vy = uint32_t((v / g.res[0]) % g.res[1]);
+    const uint32_t z = uint32_t(v / (size_t(g.res[0]) * g.res[1]));
+    const uint32_t tile =
+        ((z / g.tile[2]) * g.tiles[1] + vy / g.tile[1]) * g.tiles[0] + vx / g.tile[0];
+    const uint32_t ox = vx % g.tile[0], oy = vy % g.tile[1], oz = z % g.tile[2];
+    const uint32_t lane = (ox & 3) + 4 * (oz & 3) + 16 * (oy & 3);
+    const uint32_t slot = ((ox >> 2) * g.lt[2] + (oz >> 2)) * g.lt[1] + (oy >> 2);
+    double sum = 0.0;
+    for (uint32_t c = 0; c < num_chunks; ++c)
+        sum += double(partial[((size_t(c) * g.num_tiles + tile) * g.slots + slot) * 64 + lane]);
+    volume[v] = float(sum * scale);
+}
+
+fbp_status launch_reduce(fbp_context ctx, const float* partial, float* volume, const VolumeGeom& g,
+                         uint32_t num_chunks, double scale, hipStream_t stream) {
+    const size_t n = size_t(g.res[0]) * g.res[1] * g.res[2];
+    const unsigned blocks = unsigned((n + 255) / 256);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(blocks), dim3(256), 0, stream, partial, volume,
+                       g, num_chunks, scale);
+    FBP_CHECK_HIP(ctx, hipGetLastError());
+    return FBP_OK;
+}
+
+// Picks the number of scan-point chunks so the launch has ~16 waves per CU.
+static uint32_t pick_chunks(fbp_context ctx, uint32_t tiles, uint32_t units, uint32_t min_per_chunk) {
+    const uint32_t target = 16u * uint32_t(ctx->num_cus > 0 ? ctx->num_cus : 256);
+    uint32_t nc = (target + tiles - 1) / tiles;
+    const uint32_t cap = (units + min_per_chunk - 1) / min_per_chunk;
+    if (nc > cap) nc = cap;
+    if (nc < 1) nc = 1;
+    return nc;
+}
+
+template <int STORAGE, int LAYOUT>
+static void launch_confocal(const ConfocalArgs& a, bool pure, dim3 grid, size_t shmem,
+                            hipStream_t stream) {
+    if (pure)
+        hipLaunchKernelGGL((confocal_voxel_kernel<STORAGE, LAYOUT, true>), grid, dim3(64), shmem,
+                           stream, a);
+    else
+        hipLaunchKernelGGL((confocal_voxel_kernel<STORAGE, LAYOUT, false>), grid, dim3(64), shmem,
+                           stream, a);
+}
+
+}  // namespace fbp
+
+using namespace fbp;
+
+extern "C" fbp_status fbp_backproject_confocal_voxel(fbp_context ctx,
+                                                     const fbp_reconstruction_info* info,
+                                                     const fbp_transient* tr,
+                                                     const fbp_vec3* laser_targets, float* volume,
+                                                     void* stream) {
+    if (!ctx || !info || !tr || !tr->data || !laser_targets || !volume)
+        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "confocal: null argument");
+    if (info->num_laser_targets == 0 || info->num_time_bins == 0 || info->voxel_resolution.x == 0 ||
+        info->voxel_resolution.y == 0 || info->voxel_resolution.z == 0)
+        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "confocal: zero-sized grid");
+    if (!(info->time_step > 0.f) || !finite_info(*info))
+        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "confocal: time_step must be > 0, fields finite");
+    if (tr->storage < FBP_STORAGE_F32 || tr->storage > FBP_STORAGE_BF16)
+        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "confocal: bad storage %d", tr->storage);
+    if (tr->storage != FBP_STORAGE_F32 && !tr->scales)
+        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "confocal: F16/BF16 storage needs scales");
+    if (tr->layout != FBP_LAYOUT_TIME_FASTEST &&
+        !(tr->layout == FBP_LAYOUT_TIME_MAJOR && tr->storage == FBP_STORAGE_F32))
+        return fail(ctx, FBP_ERR_UNSUPPORTED, "confocal: time-major layout is F32 only");
+    FBP_CHECK(bind(ctx));
+
+    ConfocalArgs a{};
+    a.data = tr->data;
+    a.scales = tr->scales;
+    a.laser = laser_targets;
+    a.g = make_geom(*info, 4, 4, 4);
+    a.L = info->num_laser_targets;
+    a.T = info->num_time_bins;
+    a.t0 = info->time_offset;
+    a.dt = info->time_step;
+    a.inv_dt = 1.0f / info->time_step;
+    a.discard = info->discard_first_last_bounces;
+    a.lpos[0] = info->laser_position.x;
+    a.lpos[1] = info->laser_position.y;
+    a.lpos[2] = info->laser_position.z;
+    a.spos[0] = info->sensor_position.x;
+    a.spos[1] = info->sensor_position.y;
+    a.spos[2] = info->sensor_position.z;
+    a.W = window_bins(a.g, a.dt, 2);
+    if (size_t(a.W + 8) * 4 > 65536)
+        return fail(ctx, FBP_ERR_UNSUPPORTED,
+                    "confocal: bin window of %d bins per tile exceeds LDS (voxels too large for "
+                    "the bin width)",
+                    a.W);
+    a.batch = 16;
+    while (a.batch > 1 && size_t(a.batch) * (a.W + 8) * 4 > 32768) a.batch /= 2;
+
+    const uint32_t nc = pick_chunks(ctx, a.g.num_tiles, a.L, 64);
+    a.chunk_len = (a.L + nc - 1) / nc;
+    const uint32_t chunks = (a.L + a.chunk_len - 1) / a.chunk_len;
+    const size_t partial_bytes = size_t(chunks) * a.g.num_tiles * a.g.slots * 64 * sizeof(float);
+    FBP_CHECK(ensure_scratch(ctx, partial_bytes));
+    a.partial = static_cast<float*>(ctx->scratch);
+
+    const bool pure = (a.t0 == 0.f) && (a.discard != 0);
+    const dim3 grid(chunks * a.g.num_tiles);
+    const size_t shmem = size_t(a.batch) * (a.W + 8) * sizeof(float);
+    hipStream_t s = as_stream(stream);
+    if (tr->layout == FBP_LAYOUT_TIME_MAJOR) {
+        launch_confocal<FBP_STORAGE_F32, FBP_LAYOUT_TIME_MAJOR>(a, pure, grid, shmem, s);
+    } else if (tr->storage == FBP_STORAGE_F32) {
+        launch_confocal<FBP_STORAGE_F32, FBP_LAYOUT_TIME_FASTEST>(a, pure, grid, shmem, s);
+    } else if (tr->storage == FBP_STORAGE_F16) {
+        launch_confocal<FBP_STORAGE_F16, FBP_LAYOUT_TIME_FASTEST>(a, pure, grid, shmem, s);
+    } else {
+        launch_confocal<FBP_STORAGE_BF16, FBP_LAYOUT_TIME_FASTEST>(a, pure, grid, shmem, s);
+    }
+    FBP_CHECK_HIP(ctx, hipGetLastError());
+    return launch_reduce(ctx, a.partial, volume, a.g, chunks, 1.0, s);
+}

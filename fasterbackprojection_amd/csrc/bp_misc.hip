@@ -1,0 +1,420 @@
+// Depth-plane backprojection, MIS-sampled confocal backprojection, transient
+// quantisation and the confocal forward model.
+#include <algorithm>
+#include <vector>
+
+#include "bp_common.h"
+
+namespace fbp {
+
+// ------------------------------------------------------------ depth planes
+// Reference: backprojectConfocal / backprojectExhaustive
+// (FB/transient_reconstruction.cuh:41-91).  One thread per output (c, d)
+// gathers over all scan points / pairs in a fixed order (no atomics).  No
+// 1/r^2 weight, as in the reference.
+
+template <bool EXHAUSTIVE>
+__global__ void depth_kernel(const float* __restrict__ I, const fbp_vec3* __restrict__ laser,
+                             const fbp_vec3* __restrict__ sensor, const double* __restrict__ depths,
+                             uint32_t D, uint32_t L, uint32_t S, uint32_t T, float t0, float dt,
+                             uint32_t discard, fbp_vec3 n, fbp_vec3 lpos, fbp_vec3 spos,
+                             float* __restrict__ act) {
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= L * D) return;
+    const uint32_t c = idx / D, d = idx % D;
+    const float depth = static_cast<float>(depths[d]);
+    // checkPos = laserTargets[c] + relayWallNormal * depth (componentwise).
+    const float cx = rn_add(laser[c].x, rn_mul(n.x, depth));
+    const float cy = rn_add(laser[c].y, rn_mul(n.y, depth));
+    const float cz = rn_add(laser[c].z, rn_mul(n.z, depth));
+    float sum = 0.f;
+    if constexpr (!EXHAUSTIVE) {
+        for (uint32_t l = 0; l < L; ++l) {
+            const float lx = laser[l].x, ly = laser[l].y, lz = laser[l].z;
+            float dist = rn_sub(rn_mul(distance3(lx, ly, lz, cx, cy, cz), 2.0f), t0);
+            if (discard == 0)
+                dist = rn_add(dist, rn_add(distance3(lx, ly, lz, lpos.x, lpos.y, lpos.z),
+                                           distance3(lx, ly, lz, spos.x, spos.y, spos.z)));
+            const float q = rn_div(dist, dt);
+            if (!(q > -1.0f) || !(q < float(T))) continue;
+            const int bin = int(q);
+            const float v = I[size_t(l) * T + bin];
+            if (v > kEps) sum += v;
+        }
+    } else {
+        // Pairs in the reference's order ls = s * L + l (l = ls % L, s = ls / L).
+        for (uint32_t s = 0; s < S; ++s) {
+            const float sx = sensor[s].x, sy = sensor[s].y, sz = sensor[s].z;
+            const float ds = distance3(cx, cy, cz, sx, sy, sz);
+            for (uint32_t l = 0; l < L; ++l) {
+                const float lx = laser[l].x, ly = laser[l].y, lz = laser[l].z;
+                float dist = rn_sub(rn_add(distance3(lx, ly, lz, cx, cy, cz), ds), t0);
+                if (discard == 0)
+                    dist = rn_add(dist, rn_add(distance3(lx, ly, lz, lpos.x, lpos.y, lpos.z),
+                                               distance3(sx, sy, sz, spos.x, spos.y, spos.z)));
+                const float q = rn_div(dist, dt);
+                if (!(q > -1.0f) || !(q < float(T))) continue;
+                const int bin = int(q);
+                const float v = I[(size_t(l) * S + s) * T + bin];
+                if (v > kEps) sum += v;
+            }
+        }
+    }
+    act[idx] = sum;
+}
+
+// --------------------------------------------------------------------- MIS
+// Reference: backprojectConfocalVoxelMIS (FB/transient_reconstruction.cuh:194-260),
+// getUniformRandom / getMISIndexAlias (FB/transient_utils.cuh:18-53).  One
+// thread per voxel walks its samples s = 0..N-1 in order (the reference runs
+// one thread per (voxel, sample) with a racy +=).
+
+__device__ __forceinline__ float mis_uniform(const float* __restrict__ noise, uint32_t n,
+                                             uint32_t& state) {
+    state = state * 747796405u + 2891336453u;
+    return noise[state % n];
+}
+
+__global__ void mis_kernel(const float* __restrict__ I, const fbp_vec3* __restrict__ laser,
+                           const float* __restrict__ cdf, const uint32_t* __restrict__ alias,
+                           const float* __restrict__ prob, const float* __restrict__ noise,
+                           uint32_t noise_size, uint32_t num_samples, uint32_t mode, VolumeGeom g,
+                           uint32_t L, uint32_t T, float t0, float dt, uint32_t discard,
+                           fbp_vec3 lpos, fbp_vec3 spos, float* __restrict__ volume) {
+    const uint32_t slice = g.res[0] * g.res[1];
+    const size_t vid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (vid >= size_t(slice) * g.res[2]) return;
+    const uint32_t v = uint32_t(vid % slice), z = uint32_t(vid / slice);
+    const uint32_t vx = v % g.res[0], vy = v / g.res[0];
+    const float px = voxel_center(g.hmin[0], g.vs[0], vx);
+    const float py = voxel_center(g.hmin[1], g.vs[1], z);
+    const float pz = voxel_center(g.hmin[2], g.vs[2], vy);
+    const float invL = safe_rcp(static_cast<float>(L));
+    float sum = 0.f;
+    for (uint32_t s = 0; s < num_samples; ++s) {
+        uint32_t state = slice * z + v + s;
+        uint32_t l;
+        float pdf, gpdf, pa;
+        if (mis_uniform(noise, noise_size, state) < 0.5f) {
+            const float r1 = mis_uniform(noise, noise_size, state);
+            const float r2 = mis_uniform(noise, noise_size, state);
+            uint32_t k = static_cast<uint32_t>(r1 * static_cast<float>(L));
+            if (k >= L) k = L - 1;
+            l = (r2 < prob[k]) ? k : alias[k];
+            pdf = l > 0 ? cdf[l] - cdf[l - 1] : cdf[l];
+            gpdf = pdf + 1.0f * invL;
+            pa = pdf;
+        } else {
+            l = static_cast<uint32_t>(mis_uniform(noise, noise_size, state) *
+                                      static_cast<float>(L));
+            if (l >= L) l = L - 1;
+            pdf = 1.0f * invL;
+            pa = l > 0 ? cdf[l] - cdf[l - 1] : cdf[l];
+            gpdf = pdf + pa;
+        }
+        const float lx = laser[l].x, ly = laser[l].y, lz = laser[l].z;
+        float dist = rn_sub(rn_mul(distance3(lx, ly, lz, px, py, pz), 2.0f), t0);
+        if (discard == 0)
+            dist = rn_add(dist, rn_add(distance3(lx, ly, lz, lpos.x, lpos.y, lpos.z),
+                                       distance3(lx, ly, lz, spos.x, spos.y, spos.z)));
+        const float q = rn_div(dist, dt);
+        if (!(q > -1.0f) || !(q < float(T))) continue;
+        const int bin = int(q);
+        const float light = I[size_t(l) * T + bin];
+        if (!(light > kEps)) continue;
+        const float wd = safe_rcp(rn_mul(dist, dist));
+        if (mode == FBP_MIS_REFERENCE) {
+            const float mis_weight = rn_mul(pdf, safe_rcp(gpdf));
+            sum += rn_mul(rn_mul(mis_weight, light), wd);
+        } else {
+            // One-sample MIS over the mixture p = (p_alias + 1/L) / 2:
+            // E[f(l) / p(l)] / N = sum_l f(l) / N.
+            const float pmix = 0.5f * (pa + invL);
+            sum += (light * wd) / (pmix * float(num_samples));
+        }
+    }
+    volume[vid] = sum;
+}
+
+// Per-scan-point total intensity (I > 1e-5 samples), in FP64.
+__global__ void row_sum_kernel(const float* __restrict__ I, uint32_t L, uint32_t T,
+                               double* __restrict__ out) {
+    const uint32_t l = blockIdx.x;
+    double acc = 0.0;
+    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
+        float v = I[size_t(l) * T + t];
+        if (v > kEps) acc += v;
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[l] = red[0];
+}
+
+// ------------------------------------------------------------ quantisation
+// scale = max_t I (after zeroing I <= 1e-5); q = I / scale in F16 / BF16.
+template <int STORAGE>
+__global__ void quantize_kernel(const float* __restrict__ src, uint32_t cols, void* __restrict__ dst,
+                                float* __restrict__ scales) {
+    const size_t row = blockIdx.x;
+    const float* r = src + row * cols;
+    float mx = 0.f;
+    for (uint32_t t = threadIdx.x; t < cols; t += blockDim.x) {
+        float v = r[t];
+        v = (v > kEps) ? v : 0.f;
+        mx = fmaxf(mx, v);
+    }
+    __shared__ float red[256];
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    const float scale = red[0] > 0.f ? red[0] : 1.0f;
+    const float inv = 1.0f / scale;
+    if (threadIdx.x == 0) scales[row] = scale;
+    uint16_t* out = static_cast<uint16_t*>(dst) + row * cols;
+    for (uint32_t t = threadIdx.x; t < cols; t += blockDim.x) {
+        float v = r[t];
+        v = (v > kEps) ? v * inv : 0.f;
+        if constexpr (STORAGE == FBP_STORAGE_F16) {
+            out[t] = __half_as_ushort(__float2half_rn(v));
+        } else {
+            // round-to-nearest-even to bfloat16
+            uint32_t b = __float_as_uint(v);
+            b += 0x7fffu + ((b >> 16) & 1u);
+            out[t] = uint16_t(b >> 16);
+        }
+    }
+}
+
+// ----------------------------------------------------------- forward model
+// One workgroup per scan point (transient row): for every point p,
+// I[l][round(2 r / delta)] += albedo / r^4.  Contributions are accumulated in
+// LDS as 2^-32 fixed point in 64-bit integers, so the order in which threads
+// add them does not change the result (deterministic).
+constexpr double kRenderFix = 4294967296.0;
+
+__global__ void render_confocal_kernel(const fbp_vec3* __restrict__ scan, uint32_t num_scan,
+                                       const fbp_vec3* __restrict__ pts, const float* __restrict__ alb,
+                                       uint32_t num_points, uint32_t T, float delta,
+                                       float* __restrict__ transient) {
+    extern __shared__ unsigned long long hist[];
+    const uint32_t l = blockIdx.x;
+    if (l >= num_scan) return;
+    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) hist[t] = 0ull;
+    __syncthreads();
+    const float sx = scan[l].x, sy = scan[l].y, sz = scan[l].z;
+    for (uint32_t p = threadIdx.x; p < num_points; p += blockDim.x) {
+        const float dx = pts[p].x - sx, dy = pts[p].y - sy, dz = pts[p].z - sz;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float r = sqrtf(r2);
+        const int bin = int(rintf(2.0f * r / delta));
+        if (bin < 0 || bin >= int(T) || !(r2 > 0.f)) continue;
+        const double v = double(alb[p]) / (double(r2) * double(r2));
+        atomicAdd(&hist[bin], (unsigned long long)(v * kRenderFix + 0.5));
+    }
+    __syncthreads();
+    float* row = transient + size_t(l) * T;
+    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) row[t] = float(double(hist[t]) / kRenderFix);
+}
+
+}  // namespace fbp
+
+using namespace fbp;
+
+extern "C" {
+
+fbp_status fbp_backproject_depth_confocal(fbp_context ctx, const fbp_reconstruction_info* info,
+                                          const fbp_transient* tr, const fbp_vec3* laser_targets,
+                                          const double* depths, uint32_t num_depths,
+                                          float* activation, void* stream) {
+    if (!ctx || !info || !tr || !tr->data || !laser_targets || !depths || !activation)
+        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "depth confocal: null argument");
+    if (tr->storage != FBP_STORAGE_F32 || tr->layout != FBP_LAYOUT_TIME_FASTEST)
+        return fail(ctx, FBP_ERR_UNSUPPORTED, "depth confocal: F32 [l][t] transient only");
+    if (!num_depths || !info->num_laser_targets || !info->num_time_bins || !(info->time_step > 0.f))
+        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "depth confocal: empty grid");
+    FBP_CHECK(bind(ctx));
+    const uint32_t L = info->num_laser_targets;
+    const uint32_t n = L * num_depths;
+    hipLaunchKernelGGL(depth_kernel<false>, dim3((n + 255) / 256), dim3(256), 0, as_stream(stream),
+                       static_cast<const float*>(tr->data), laser_targets, laser_targets, depths,
+                       num_depths, L, 1u, info->num_time_bins, info->time_offset, info->time_step,
+                       info->discard_first_last_bounces, info->relay_wall_normal,
+                       info->laser_position, info->sensor_position, activation);
+    FBP_CHECK_HIP(ctx, hipGetLastError());
+    return FBP_OK;
+}
+
+fbp_status fbp_backproject_depth_exhaustive(fbp_context ctx, const fbp_reconstruction_info* info,
+                                            const fbp_transient* tr, const fbp_vec3* laser_targets,
+                                            const fbp_vec3* sensor_targets, const double* depths,
+                                            uint32_t num_depths, float* activation, void* stream) {
+    if (!ctx || !info || !tr || !tr->data || !laser_targets || !sensor_targets || !depths ||
+        !activation)
+        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "depth exhaustive: null argument");
+    if (tr->storage != FBP_STORAGE_F32)
+        return fail(ctx, FBP_ERR_UNSUPPORTED, "depth exhaustive: F32 transient only");
+    if (!num_depths || !info->num_laser_targets || !info->num_sensor_targets ||
+        !info->num_time_bins || !(info->time_step > 0.f))
+        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "depth exhaustive: empty grid");
+    FBP_CHECK(bind(ctx));
+    const uint32_t L = info->num_laser_targets;
+    const uint32_t n = L * num_depths;
+    hipLaunchKernelGGL(depth_kernel<true>, dim3((n + 255) / 256), dim3(256), 0, as_stream(stream),
+                       static_cast<const float*>(tr->data), laser_targets, sensor_targets, depths,
+                       num_depths, L, info->num_sensor_targets, info->num_time_bins,
+                       info->time_offset, info->time_step, info->discard_first_last_bounces,
+                       info->relay_wall_normal, info->laser_position, info->sensor_position,
+                       activation);
+    FBP_CHECK_HIP(ctx, hipGetLastError());
+    return FBP_OK;
+}
+
+fbp_status fbp_build_alias_tables_host(const float* cdf, uint32_t n, uint32_t* alias_table,
+                                       float* prob_table) {
+    // Laser::buildAliasTables (FB/Laser.cu:253-315): same arithmetic, same
+    // stack (pop from the back) order.
+    if (!cdf || !alias_table || !prob_table || n == 0) return FBP_ERR_INVALID_ARGUMENT;
+    std::vector<float> p(n);
+    p[0] = cdf[0];
+    for (uint32_t i = 1; i < n; ++i) p[i] = cdf[i] - cdf[i - 1];
+    std::vector<uint32_t> small_, large_;
+    float sum = 0.f;
+    for (uint32_t i = 0; i < n; ++i) {
+        alias_table[i] = 0;
+        if (p[i] < 0.f) return FBP_ERR_PROBABILITIES;
+        prob_table[i] = p[i] * float(n);
+        sum += p[i];
+        if (prob_table[i] < 1.0f)
+            small_.push_back(i);
+        else
+            large_.push_back(i);
+    }
+    if (std::fabs(sum - 1.0f) > 1e-5f) return FBP_ERR_PROBABILITIES;
+    while (!small_.empty() && !large_.empty()) {
+        const uint32_t s = small_.back();
+        small_.pop_back();
+        const uint32_t l = large_.back();
+        large_.pop_back();
+        alias_table[s] = l;
+        prob_table[l] += prob_table[s] - 1.0f;
+        if (prob_table[l] < 1.0f)
+            small_.push_back(l);
+        else
+            large_.push_back(l);
+    }
+    while (!small_.empty()) {
+        prob_table[small_.back()] = 1.0f;
+        small_.pop_back();
+    }
+    while (!large_.empty()) {
+        prob_table[large_.back()] = 1.0f;
+        large_.pop_back();
+    }
+    return FBP_OK;
+}
+
+fbp_status fbp_backproject_confocal_mis(fbp_context ctx, const fbp_reconstruction_info* info,
+                                        const fbp_transient* tr, const fbp_vec3* laser_targets,
+                                        const fbp_mis_params* params, float* volume, void* stream) {
+    if (!ctx || !info || !tr || !tr->data || !laser_targets || !params || !params->noise || !volume)
+        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "mis: null argument");
+    if (tr->storage != FBP_STORAGE_F32 || tr->layout != FBP_LAYOUT_TIME_FASTEST)
+        return fail(ctx, FBP_ERR_UNSUPPORTED, "mis: F32 [l][t] transient only");
+    if (!params->noise_size || !params->num_samples || !info->num_laser_targets ||
+        !info->num_time_bins || !(info->time_step > 0.f) || params->mode > FBP_MIS_UNBIASED)
+        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "mis: bad parameters");
+    FBP_CHECK(bind(ctx));
+    hipStream_t s = as_stream(stream);
+    const uint32_t L = info->num_laser_targets, T = info->num_time_bins;
+    // Scratch: row sums (double L) | cdf (float L) | alias (u32 L) | prob (float L)
+    const size_t bytes = size_t(L) * (8 + 4 + 4 + 4) + 256;
+    FBP_CHECK(ensure_scratch(ctx, bytes));
+    char* base = static_cast<char*>(ctx->scratch);
+    double* d_sum = reinterpret_cast<double*>(base);
+    float* d_cdf = reinterpret_cast<float*>(base + size_t(L) * 8);
+    uint32_t* d_alias = reinterpret_cast<uint32_t*>(base + size_t(L) * 12);
+    float* d_prob = reinterpret_cast<float*>(base + size_t(L) * 16);
+    hipLaunchKernelGGL(row_sum_kernel, dim3(L), dim3(256), 0, s, static_cast<const float*>(tr->data),
+                       L, T, d_sum);
+    FBP_CHECK_HIP(ctx, hipGetLastError());
+    // Spatial CDF: inclusive prefix over scan points, then min-max normalised
+    // (FB/Laser.cu:547-574, scanning the intensity rather than the volume).
+    std::vector<double> h_sum(L);
+    FBP_CHECK_HIP(ctx, hipMemcpyAsync(h_sum.data(), d_sum, size_t(L) * 8, hipMemcpyDeviceToHost, s));
+    FBP_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    std::vector<double> pre(L);
+    double run = 0.0;
+    for (uint32_t l = 0; l < L; ++l) {
+        run += h_sum[l];
+        pre[l] = run;
+    }
+    const double mn = *std::min_element(pre.begin(), pre.end());
+    const double mx = *std::max_element(pre.begin(), pre.end());
+    std::vector<float> cdf(L);
+    for (uint32_t l = 0; l < L; ++l)
+        cdf[l] = (mx > mn) ? float((pre[l] - mn) / (mx - mn)) : 0.f;
+    std::vector<uint32_t> alias(L);
+    std::vector<float> prob(L);
+    fbp_status st = fbp_build_alias_tables_host(cdf.data(), L, alias.data(), prob.data());
+    if (st != FBP_OK) return fail(ctx, st, "mis: alias tables rejected the scan-point CDF");
+    FBP_CHECK_HIP(ctx, hipMemcpyAsync(d_cdf, cdf.data(), size_t(L) * 4, hipMemcpyHostToDevice, s));
+    FBP_CHECK_HIP(ctx, hipMemcpyAsync(d_alias, alias.data(), size_t(L) * 4, hipMemcpyHostToDevice, s));
+    FBP_CHECK_HIP(ctx, hipMemcpyAsync(d_prob, prob.data(), size_t(L) * 4, hipMemcpyHostToDevice, s));
+    VolumeGeom g = make_geom(*info, 1, 1, 1);
+    const size_t nv = size_t(g.res[0]) * g.res[1] * g.res[2];
+    hipLaunchKernelGGL(mis_kernel, dim3(unsigned((nv + 255) / 256)), dim3(256), 0, s,
+                       static_cast<const float*>(tr->data), laser_targets, d_cdf, d_alias, d_prob,
+                       params->noise, params->noise_size, params->num_samples, params->mode, g, L,
+                       T, info->time_offset, info->time_step, info->discard_first_last_bounces,
+                       info->laser_position, info->sensor_position, volume);
+    FBP_CHECK_HIP(ctx, hipGetLastError());
+    // The host vectors above are consumed by the async copies: wait for them.
+    FBP_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    return FBP_OK;
+}
+
+fbp_status fbp_quantize_transient(fbp_context ctx, const float* src, uint64_t rows, uint32_t cols,
+                                  int32_t storage, void* dst, float* scales, void* stream) {
+    if (!ctx || !src || !dst || !scales || rows == 0 || cols == 0)
+        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "quantize: bad argument");
+    if (storage != FBP_STORAGE_F16 && storage != FBP_STORAGE_BF16)
+        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "quantize: storage must be F16 or BF16");
+    if (rows > 0x7fffffffull) return fail(ctx, FBP_ERR_UNSUPPORTED, "quantize: too many rows");
+    FBP_CHECK(bind(ctx));
+    if (storage == FBP_STORAGE_F16)
+        hipLaunchKernelGGL(quantize_kernel<FBP_STORAGE_F16>, dim3(unsigned(rows)), dim3(256), 0,
+                           as_stream(stream), src, cols, dst, scales);
+    else
+        hipLaunchKernelGGL(quantize_kernel<FBP_STORAGE_BF16>, dim3(unsigned(rows)), dim3(256), 0,
+                           as_stream(stream), src, cols, dst, scales);
+    FBP_CHECK_HIP(ctx, hipGetLastError());
+    return FBP_OK;
+}
+
+fbp_status fbp_render_confocal(fbp_context ctx, const fbp_vec3* scan_points, uint32_t num_scan,
+                               const fbp_vec3* points, const float* albedo, uint32_t num_points,
+                               uint32_t num_time_bins, float delta, float* transient,
+                               void* stream) {
+    if (!ctx || !scan_points || !points || !albedo || !transient || !num_scan || !num_time_bins ||
+        !(delta > 0.f))
+        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "render: bad argument");
+    FBP_CHECK(bind(ctx));
+    if (size_t(num_time_bins) * 8 > 65536)
+        return fail(ctx, FBP_ERR_UNSUPPORTED, "render: at most 8192 time bins");
+    hipLaunchKernelGGL(render_confocal_kernel, dim3(num_scan), dim3(256),
+                       size_t(num_time_bins) * 8, as_stream(stream),
+                       scan_points, num_scan, points, albedo, num_points, num_time_bins, delta,
+                       transient);
+    FBP_CHECK_HIP(ctx, hipGetLastError());
+    return FBP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,137 @@
+// Context, error reporting and scratch management of libfbp.
+// Replaces CudaHelper / CudaContext (FB/CudaHelper.cpp, FB/CudaContext.cpp:22-58).
+#include <cstdarg>
+#include <cstdlib>
+
+#include "fbp_internal.h"
+
+namespace fbp {
+
+fbp_status fail(fbp_context ctx, fbp_status st, const char* fmt, ...) {
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        ctx->last_error = buf;
+    }
+    return st;
+}
+
+fbp_status check_hip(fbp_context ctx, hipError_t e, const char* what) {
+    if (e == hipSuccess) return FBP_OK;
+    fbp_status st = (e == hipErrorOutOfMemory) ? FBP_ERR_OUT_OF_MEMORY : FBP_ERR_DEVICE;
+    return fail(ctx, st, "%s: %s", what, hipGetErrorString(e));
+}
+
+fbp_status bind(fbp_context ctx) {
+    if (!ctx) return FBP_ERR_INVALID_ARGUMENT;
+    FBP_CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    return FBP_OK;
+}
+
+static fbp_status grow(fbp_context ctx, void** ptr, size_t* have, size_t bytes) {
+    if (*have >= bytes) return FBP_OK;
+    // Round up to 64 MiB steps to avoid repeated reallocation.
+    size_t want = (bytes + (size_t(64) << 20) - 1) & ~((size_t(64) << 20) - 1);
+    if (*ptr) {
+        FBP_CHECK_HIP(ctx, hipDeviceSynchronize());
+        FBP_CHECK_HIP(ctx, hipFree(*ptr));
+        *ptr = nullptr;
+        *have = 0;
+    }
+    FBP_CHECK_HIP(ctx, hipMalloc(ptr, want));
+    *have = want;
+    return FBP_OK;
+}
+
+fbp_status ensure_scratch(fbp_context ctx, size_t bytes) {
+    return grow(ctx, &ctx->scratch, &ctx->scratch_bytes, bytes);
+}
+
+fbp_status ensure_scratch2(fbp_context ctx, size_t bytes) {
+    return grow(ctx, &ctx->scratch2, &ctx->scratch2_bytes, bytes);
+}
+
+}  // namespace fbp
+
+extern "C" {
+
+const char* fbp_status_string(int status) {
+    switch (status) {
+        case FBP_OK: return "ok";
+        case FBP_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case FBP_ERR_DEVICE: return "device (HIP runtime) error";
+        case FBP_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case FBP_ERR_FFT: return "FFT error";
+        case FBP_ERR_UNSUPPORTED: return "unsupported shape or mode";
+        case FBP_ERR_PROBABILITIES: return "invalid probabilities";
+        case FBP_ERR_IO: return "I/O error";
+        default: return "unknown status";
+    }
+}
+
+fbp_status fbp_context_create(int device, fbp_context* out) {
+    if (!out) return FBP_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return FBP_ERR_DEVICE;
+    if (device >= count) return FBP_ERR_INVALID_ARGUMENT;
+    if (device < 0) {
+        // Largest clock rate x multiprocessor count, as CudaContext::setDevice.
+        long long best = -1;
+        for (int d = 0; d < count; ++d) {
+            hipDeviceProp_t p;
+            if (hipGetDeviceProperties(&p, d) != hipSuccess) continue;
+            long long score = (long long)p.clockRate * p.multiProcessorCount;
+            if (score > best) {
+                best = score;
+                device = d;
+            }
+        }
+        if (device < 0) return FBP_ERR_DEVICE;
+    }
+    fbp_context ctx = new fbp_context_s();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess) {
+        delete ctx;
+        return FBP_ERR_DEVICE;
+    }
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device) != hipSuccess) {
+        delete ctx;
+        return FBP_ERR_DEVICE;
+    }
+    ctx->num_cus = p.multiProcessorCount;
+    if (hipHostMalloc((void**)&ctx->pinned, 64 * sizeof(float), hipHostMallocDefault) != hipSuccess) {
+        delete ctx;
+        return FBP_ERR_OUT_OF_MEMORY;
+    }
+    *out = ctx;
+    return FBP_OK;
+}
+
+fbp_status fbp_context_destroy(fbp_context ctx) {
+    if (!ctx) return FBP_ERR_INVALID_ARGUMENT;
+    hipSetDevice(ctx->device);
+    hipDeviceSynchronize();
+    fbp::fft_cache_destroy(ctx);
+    if (ctx->scratch) hipFree(ctx->scratch);
+    if (ctx->scratch2) hipFree(ctx->scratch2);
+    if (ctx->pinned) hipHostFree(ctx->pinned);
+    delete ctx;
+    return FBP_OK;
+}
+
+const char* fbp_context_last_error(fbp_context ctx) {
+    return ctx ? ctx->last_error.c_str() : "null context";
+}
+
+int fbp_context_device(fbp_context ctx) { return ctx ? ctx->device : -1; }
+
+size_t fbp_context_scratch_bytes(fbp_context ctx) {
+    return ctx ? ctx->scratch_bytes + ctx->scratch2_bytes : 0;
+}
+
+}  // extern "C"

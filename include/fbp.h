@@ -1,0 +1,248 @@
+/*
+ * fbp.h - C ABI of the MI355X (gfx950) NLOS reconstruction core.
+ *
+ * Plain C: fixed-size integers, float pointers, opaque handles.  Every entry
+ * point returns an fbp_status (0 = OK) and takes the HIP stream it runs on as
+ * a void* (NULL = the null stream).  All data pointers are DEVICE pointers
+ * unless a parameter says "host".  Nothing here allocates the caller's
+ * buffers; a context owns scratch memory (partial sums, FFT work areas,
+ * tables) and grows it on demand.  One context may be used from one host
+ * thread at a time; different contexts are independent (no global state,
+ * unlike the reference's __constant__ symbols, GpuStructs.cu:4-16).
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the reference repository; FB/ = FasterBackprojection/).
+ */
+#ifndef FBP_H
+#define FBP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+
+typedef enum fbp_status {
+    FBP_OK = 0,
+    FBP_ERR_INVALID_ARGUMENT = 1, /* null pointer, zero size, bad enum      */
+    FBP_ERR_DEVICE = 2,           /* a HIP runtime call failed              */
+    FBP_ERR_OUT_OF_MEMORY = 3,    /* scratch allocation failed              */
+    FBP_ERR_FFT = 4,              /* rocFFT plan/execute failed             */
+    FBP_ERR_UNSUPPORTED = 5,      /* shape or mode outside what is built    */
+    FBP_ERR_PROBABILITIES = 6,    /* alias table input invalid (Laser.cu:270-287) */
+    FBP_ERR_IO = 7                /* file could not be read or written      */
+} fbp_status;
+
+/* Replaces CudaHelper::checkError's exception text (FB/CudaHelper.cpp:66-69). */
+const char* fbp_status_string(int status);
+
+/* ------------------------------------------------------------- data model */
+
+typedef struct fbp_vec3 { float x, y, z; } fbp_vec3;       /* glm::vec3 layout  */
+typedef struct fbp_uvec3 { uint32_t x, y, z; } fbp_uvec3;  /* glm::uvec3 layout */
+
+enum { FBP_CONFOCAL = 0, FBP_EXHAUSTIVE = 1 };           /* FB/GpuStructs.cuh:3-6 */
+
+/* Field-for-field mirror of ReconstructionInfo (FB/GpuStructs.cuh:8-38):
+ * 148 bytes, no padding.  Volume axis convention of the reference kernels
+ * (FB/transient_reconstruction.cuh:102-106): voxel (vx, vy, z) is stored at
+ * z*X*Y + vy*X + vx and its centre is
+ *   hidden_volume_min + (vx, z, vy) * voxel_size + 0.5 * voxel_size,
+ * i.e. the slowest index walks world y (depth), the middle one world z. */
+typedef struct fbp_reconstruction_info {
+    fbp_vec3 sensor_position;
+    uint32_t capture_system;            /* FBP_CONFOCAL / FBP_EXHAUSTIVE     */
+    fbp_vec3 laser_position;
+    float time_offset;                  /* t0, metres                        */
+    uint32_t num_laser_targets;         /* L                                 */
+    uint32_t num_sensor_targets;        /* S                                 */
+    uint32_t num_time_bins;             /* T                                 */
+    float time_step;                    /* bin width in metres (Delta)       */
+    fbp_vec3 hidden_volume_min;
+    uint32_t discard_first_last_bounces;
+    fbp_vec3 hidden_volume_max;
+    fbp_vec3 hidden_volume_size;
+    fbp_vec3 hidden_volume_voxel_size;
+    fbp_vec3 relay_wall_min_position;
+    fbp_vec3 relay_wall_size;
+    fbp_vec3 relay_wall_normal;
+    fbp_uvec3 voxel_resolution;
+} fbp_reconstruction_info;
+
+/* Storage of a transient on the device. */
+enum {
+    FBP_STORAGE_F32 = 0,
+    FBP_STORAGE_F16 = 1,  /* IEEE half, value = q * scale[row]   */
+    FBP_STORAGE_BF16 = 2  /* bfloat16,  value = q * scale[row]   */
+};
+/* Element order of a confocal transient.  TIME_FASTEST = [l][t] is what the
+ * loader, filter_H and MIS produce (FB/NLosData.cpp:264, FB/Laser.cu:80-126);
+ * TIME_MAJOR = [t][l] is what getConfocalTransientIndex reads
+ * (FB/transient_utils.cuh:6-9).  Exhaustive transients are always
+ * [l][s][t] (FB/transient_utils.cuh:11-14). */
+enum { FBP_LAYOUT_TIME_FASTEST = 0, FBP_LAYOUT_TIME_MAJOR = 1 };
+
+typedef struct fbp_transient {
+    const void* data;     /* device pointer, element type given by storage    */
+    int32_t storage;      /* FBP_STORAGE_*                                    */
+    int32_t layout;       /* FBP_LAYOUT_* (confocal only)                      */
+    const float* scales;  /* one scale per time series (F16/BF16), else NULL  */
+} fbp_transient;
+
+/* --------------------------------------------------------------- context */
+
+typedef struct fbp_context_s* fbp_context;
+
+/* Replaces CudaContext::setDevice (FB/CudaContext.cpp:22-58).  device < 0
+ * picks the device with the largest clock x multiprocessor count, as the
+ * reference does. */
+fbp_status fbp_context_create(int device, fbp_context* out);
+fbp_status fbp_context_destroy(fbp_context ctx);
+/* Detail text of the last failure on this context (empty if none). */
+const char* fbp_context_last_error(fbp_context ctx);
+int fbp_context_device(fbp_context ctx);
+/* Bytes of scratch the context currently holds on its device. */
+size_t fbp_context_scratch_bytes(fbp_context ctx);
+
+/* --------------------------------------------------------- backprojection */
+
+/* Replaces backprojectConfocalVoxel + Laser::reconstructAABBConfocal
+ * (FB/transient_reconstruction.cuh:93-136, FB/Laser.cu:482-498).
+ * volume: float[Z][Y][X] (X,Y,Z = voxel_resolution), OVERWRITTEN with
+ *   sum_l [I > 1e-5] * I(l, bin) * safeRCP(dist^2).
+ * laser_targets: fbp_vec3[L].  Deterministic (no atomics): per-chunk partial
+ * sums are reduced in a fixed order in FP64. */
+fbp_status fbp_backproject_confocal_voxel(fbp_context ctx, const fbp_reconstruction_info* info,
+                                          const fbp_transient* transient,
+                                          const fbp_vec3* laser_targets, float* volume,
+                                          void* stream);
+
+/* Replaces backprojectExhaustiveVoxel + Laser::reconstructAABBExhaustive
+ * (FB/transient_reconstruction.cuh:138-166, FB/Laser.cu:500-516).  Visits every
+ * (laser, sensor) pair with l = ls / S, s = ls % S (the reference splits by L
+ * and swaps grid axes; identical when L == S and every pair is visited). */
+fbp_status fbp_backproject_exhaustive_voxel(fbp_context ctx, const fbp_reconstruction_info* info,
+                                            const fbp_transient* transient,
+                                            const fbp_vec3* laser_targets,
+                                            const fbp_vec3* sensor_targets, float* volume,
+                                            void* stream);
+
+/* Depth-plane mode: replaces backprojectConfocal / backprojectExhaustive and
+ * Laser::reconstructDepthConfocal / Exhaustive
+ * (FB/transient_reconstruction.cuh:41-91, FB/Laser.cu:379-480).
+ * depths: double[num_depths] (device).  activation: float[L][num_depths],
+ * OVERWRITTEN with the unweighted sum of I > 1e-5 samples. */
+fbp_status fbp_backproject_depth_confocal(fbp_context ctx, const fbp_reconstruction_info* info,
+                                          const fbp_transient* transient,
+                                          const fbp_vec3* laser_targets, const double* depths,
+                                          uint32_t num_depths, float* activation, void* stream);
+fbp_status fbp_backproject_depth_exhaustive(fbp_context ctx, const fbp_reconstruction_info* info,
+                                            const fbp_transient* transient,
+                                            const fbp_vec3* laser_targets,
+                                            const fbp_vec3* sensor_targets, const double* depths,
+                                            uint32_t num_depths, float* activation, void* stream);
+
+/* MIS-sampled confocal backprojection: replaces Laser::reconstructAABBConfocalMIS,
+ * buildAliasTables, spatioTemporalPrefixSum and backprojectConfocalVoxelMIS
+ * (FB/Laser.cu:253-315, 526-611; FB/transient_reconstruction.cuh:170-260;
+ * FB/transient_utils.cuh:18-53). */
+enum {
+    FBP_MIS_REFERENCE = 0, /* reference weights: w = pdf/(pdf + pdf_other), no 1/pdf */
+    FBP_MIS_UNBIASED = 1   /* one-sample MIS estimator f/(N_s (p_a + p_u)/2)         */
+};
+typedef struct fbp_mis_params {
+    uint32_t num_samples;   /* samples per voxel (reference: L / 4)            */
+    uint32_t mode;          /* FBP_MIS_*                                       */
+    const float* noise;     /* device float[noise_size] in [0,1)               */
+    uint32_t noise_size;    /* reference: number of voxels                     */
+} fbp_mis_params;
+fbp_status fbp_backproject_confocal_mis(fbp_context ctx, const fbp_reconstruction_info* info,
+                                        const fbp_transient* transient,
+                                        const fbp_vec3* laser_targets,
+                                        const fbp_mis_params* params, float* volume,
+                                        void* stream);
+/* Vose alias tables from a CDF (host arrays): Laser::buildAliasTables
+ * (FB/Laser.cu:253-315), same pop order and error checks. */
+fbp_status fbp_build_alias_tables_host(const float* cdf, uint32_t n, uint32_t* alias_table,
+                                       float* prob_table);
+
+/* Precision form: quantise a float32 transient to F16/BF16 with one scale
+ * per time series (scale = max_t I, values <= 1e-5 zeroed first).  No
+ * reference counterpart.  rows = number of time series, cols = T. */
+fbp_status fbp_quantize_transient(fbp_context ctx, const float* src, uint64_t rows, uint32_t cols,
+                                  int32_t storage, void* dst, float* scales, void* stream);
+
+/* ---------------------------------------------------------------- filters */
+
+/* Laplacian::compute + laplacianFilter (FB/PostprocessingFilters.cu:81-98,
+ * FB/transient_postprocessing.cuh:21-50): out = k^3 v - sum of in-bounds box. */
+fbp_status fbp_filter_laplacian(fbp_context ctx, const float* in, float* out, fbp_uvec3 size,
+                                int32_t kernel_size, void* stream);
+/* LoG::compute + calculateLaplacianKernel + LoGFilter
+ * (FB/PostprocessingFilters.cu:12-32, 102-175; FB/transient_postprocessing.cuh:52-83),
+ * clamp-to-edge.  Runs as three 1D passes plus a box term when the kernel is
+ * exactly separable (no Gaussian tap thresholded away), else dense. */
+fbp_status fbp_filter_log(fbp_context ctx, const float* in, float* out, fbp_uvec3 size,
+                          int32_t kernel_size, float sigma, void* stream);
+/* The k^3 LoG weights exactly as the reference builds them (host array). */
+fbp_status fbp_log_kernel_host(int32_t kernel_size, float sigma, float* weights);
+/* LoGFFT::compute (FB/PostprocessingFilters.cu:36-77): in-place, unnormalised
+ * (result scaled by X*Y*Z like the reference's C2R). */
+fbp_status fbp_filter_log_fft(fbp_context ctx, float* volume, fbp_uvec3 size, float sigma,
+                              void* stream);
+/* Laser::normalizeMatrix + normalizeReconstruction (FB/Laser.cu:317-339,
+ * FB/transient_postprocessing.cuh:10-17): v = (v - min) * safeRCP(max - min). */
+fbp_status fbp_normalize_minmax(fbp_context ctx, float* v, uint64_t n, void* stream);
+/* min and max of v written to host pointers (synchronises the stream). */
+fbp_status fbp_minmax(fbp_context ctx, const float* v, uint64_t n, float* host_min,
+                      float* host_max, void* stream);
+
+/* Laser::filter_H_cuda (FB/Laser.cu:128-240): temporal band-pass of every
+ * time series of data[batch][nt], in place on the device.  wl_sigma <= 0
+ * means wl_mean / sqrt(2).  border: 0 = zero, 1 = edge.  Like the reference
+ * the inverse transform is unnormalised (output scaled by nt_pad). */
+fbp_status fbp_filter_h(fbp_context ctx, float* data, uint64_t batch, uint32_t nt, float delta_t,
+                        float wl_mean, float wl_sigma, int32_t border, void* stream);
+/* nt_pad the reference uses for these parameters. */
+uint32_t fbp_filter_h_padded_length(uint32_t nt, float delta_t, float wl_mean, float wl_sigma);
+
+/* ------------------------------------------------------------------- LCT */
+
+/* Confocal LCT after python/lct.py:cnlos_reconstruction (:112-211), the
+ * reconstruction the reference's LCT class (FB/LCT.h:6-12) leaves empty.
+ * data: float[N][N][M] (rect_data order: x, y, t), M and N powers of two.
+ * out:  float[M][N][N] = max(Re(mtxi @ crop(ifftn(fftn(pad(mtx @ (data'*z^p))) * invpsf))), 0)
+ * with p = 4 if is_diffuse else 2.  width / range_max is the psf slope, as
+ * in define_psf(N, M, width / range_max). */
+typedef struct fbp_lct_params {
+    uint32_t n;          /* spatial resolution N                         */
+    uint32_t m;          /* temporal resolution M (after downsampling)   */
+    float width;         /* half-width of the scanned wall, metres       */
+    float range_max;     /* M * c * bin_resolution, metres               */
+    float snr;           /* Wiener SNR (invpsf = conj(F)/(|F|^2 + 1/snr)) */
+    int32_t is_diffuse;  /* z^4 instead of z^2 radiometric term          */
+    int32_t is_backprop; /* invpsf = conj(F) only                        */
+    uint32_t z_trim;     /* first z_trim bins zeroed                     */
+} fbp_lct_params;
+fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* params, float* out,
+                   void* stream);
+
+/* ------------------------------------------------------ scene forward model */
+
+/* Confocal transient of a set of Lambertian points: for every scan point l and
+ * point p, I[l][round(2 r / delta)] += albedo / r^4 (r = |l - p|).  Used to
+ * render training scenes and for the refiner's residual check.  No reference
+ * counterpart.  transient: float[L][T] time-fastest, OVERWRITTEN. */
+fbp_status fbp_render_confocal(fbp_context ctx, const fbp_vec3* scan_points, uint32_t num_scan,
+                               const fbp_vec3* points, const float* albedo, uint32_t num_points,
+                               uint32_t num_time_bins, float delta, float* transient,
+                               void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FBP_H */

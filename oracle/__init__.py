@@ -1,0 +1,193 @@
+"""CPU oracle of the NLOS reconstruction hot path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
+the checker and by bench.py's CPU-baseline leg.  The product
+(fasterbackprojection_amd) never imports it.
+
+* fbp_oracle.c (built into liboracle.so by the Makefile next to it) restates
+  the reference's backprojection, filter and filter_H formulas: bins by the
+  FP32 IEEE expression, weights and sums in FP64.
+* lct.py restates python/lct.py in NumPy/SciPy.
+
+Every function names the reference file:line it follows.  How each is
+pinned (closed forms, invariants, brute force, known answers) is in the
+tests that exercise it; the reference ships no golden vectors for this path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        src = HERE / "fbp_oracle.c"
+        if not LIB_PATH.exists() or LIB_PATH.stat().st_mtime < src.stat().st_mtime:
+            build()
+        _lib = C.CDLL(str(LIB_PATH))
+        P = C.c_void_p
+        _lib.oracle_recinfo_size.restype = C.c_int
+        _lib.oracle_confocal_bin.restype = C.c_int64
+        _lib.oracle_confocal_bin.argtypes = [P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
+        _lib.oracle_confocal_distf.restype = C.c_float
+        _lib.oracle_confocal_distf.argtypes = [P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
+        _lib.oracle_confocal_voxel.argtypes = [P, P, C.c_int, P, P]
+        _lib.oracle_exhaustive_voxel.argtypes = [P, P, P, P, P]
+        _lib.oracle_depth.argtypes = [P, P, P, P, P, C.c_uint32, C.c_int, P]
+        _lib.oracle_mis.argtypes = [P, P, P, P, P, P, P, C.c_uint32, C.c_uint32, C.c_int, P]
+        _lib.oracle_alias_tables.restype = C.c_int
+        _lib.oracle_alias_tables.argtypes = [P, C.c_uint32, P, P]
+        _lib.oracle_laplacian.argtypes = [P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int]
+        _lib.oracle_log_kernel.argtypes = [C.c_int, C.c_double, P]
+        _lib.oracle_log_filter.argtypes = [P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, P]
+        _lib.oracle_normalize.argtypes = [P, C.c_int64]
+        _lib.oracle_filter_h_nt_pad.restype = C.c_uint32
+        _lib.oracle_filter_h_nt_pad.argtypes = [C.c_uint32, C.c_float, C.c_float, C.c_float]
+        _lib.oracle_filter_h.argtypes = [P, C.c_uint64, C.c_uint32, C.c_float, C.c_float,
+                                         C.c_float, C.c_int, P]
+        assert _lib.oracle_recinfo_size() == 148
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _res(info):
+    r = info.voxel_resolution
+    return (r.z, r.y, r.x)
+
+
+def confocal_voxel(info, I, laser, layout: int = 0) -> np.ndarray:
+    """backprojectConfocalVoxel (FB/transient_reconstruction.cuh:93-136); FP64 [Z][Y][X]."""
+    I, laser = _f32(I), _f32(laser)
+    out = np.empty(_res(info), np.float64)
+    load().oracle_confocal_voxel(C.byref(info), _p(I), layout, _p(laser), _p(out))
+    return out
+
+
+def confocal_bin(info, laser, l, vx, vy, z) -> int:
+    laser = _f32(laser)
+    return int(load().oracle_confocal_bin(C.byref(info), _p(laser), l, vx, vy, z))
+
+
+def confocal_distf(info, laser, l, vx, vy, z) -> float:
+    laser = _f32(laser)
+    return float(load().oracle_confocal_distf(C.byref(info), _p(laser), l, vx, vy, z))
+
+
+def exhaustive_voxel(info, I, laser, sensor) -> np.ndarray:
+    """backprojectExhaustiveVoxel (FB/transient_reconstruction.cuh:138-166), all pairs."""
+    I, laser, sensor = _f32(I), _f32(laser), _f32(sensor)
+    out = np.empty(_res(info), np.float64)
+    load().oracle_exhaustive_voxel(C.byref(info), _p(I), _p(laser), _p(sensor), _p(out))
+    return out
+
+
+def depth(info, I, laser, depths, sensor=None, exhaustive=False) -> np.ndarray:
+    """backprojectConfocal / backprojectExhaustive (FB/transient_reconstruction.cuh:41-91)."""
+    I, laser = _f32(I), _f32(laser)
+    sensor = laser if sensor is None else _f32(sensor)
+    depths = np.ascontiguousarray(depths, np.float64)
+    out = np.empty((info.num_laser_targets, len(depths)), np.float64)
+    load().oracle_depth(C.byref(info), _p(I), _p(laser), _p(sensor), _p(depths), len(depths),
+                        int(exhaustive), _p(out))
+    return out
+
+
+def alias_tables(cdf):
+    """Laser::buildAliasTables (FB/Laser.cu:253-315).  Returns (alias, prob) or raises."""
+    cdf = _f32(cdf)
+    n = len(cdf)
+    alias = np.empty(n, np.uint32)
+    prob = np.empty(n, np.float32)
+    rc = load().oracle_alias_tables(_p(cdf), n, _p(alias), _p(prob))
+    if rc != 0:
+        raise ValueError("invalid probabilities")
+    return alias, prob
+
+
+def mis_cdf(I) -> np.ndarray:
+    """Scan-point CDF: prefix sums (FP64) of per-scan-point totals of I > 1e-5,
+    min-max normalised, as float32 (FB/Laser.cu:547-574 over the intensity)."""
+    I = np.asarray(I, np.float32)
+    tot = np.where(I > np.float32(1e-5), I, 0).astype(np.float64).sum(axis=1)
+    pre = np.cumsum(tot)
+    mn, mx = pre.min(), pre.max()
+    return ((pre - mn) / (mx - mn)).astype(np.float32) if mx > mn else np.zeros_like(pre, np.float32)
+
+
+def mis(info, I, laser, cdf, alias, prob, noise, num_samples, mode=0) -> np.ndarray:
+    """backprojectConfocalVoxelMIS (FB/transient_reconstruction.cuh:194-260)."""
+    I, laser, cdf, prob, noise = map(_f32, (I, laser, cdf, prob, noise))
+    alias = np.ascontiguousarray(alias, np.uint32)
+    out = np.empty(_res(info), np.float64)
+    load().oracle_mis(C.byref(info), _p(I), _p(laser), _p(cdf), _p(alias), _p(prob), _p(noise),
+                      len(noise), num_samples, mode, _p(out))
+    return out
+
+
+def laplacian(v, k=5) -> np.ndarray:
+    """laplacianFilter (FB/transient_postprocessing.cuh:21-50)."""
+    v = _f32(v)
+    Z, Y, X = v.shape
+    out = np.empty(v.shape, np.float64)
+    load().oracle_laplacian(_p(v), _p(out), X, Y, Z, k)
+    return out
+
+
+def log_kernel(k=5, sigma=1.0) -> np.ndarray:
+    """calculateLaplacianKernel (FB/PostprocessingFilters.cu:102-175) in FP64, [kz][ky][kx]."""
+    w = np.empty(k ** 3, np.float64)
+    load().oracle_log_kernel(k, float(sigma), _p(w))
+    return w.reshape(k, k, k)
+
+
+def log_filter(v, k=5, sigma=1.0, weights=None) -> np.ndarray:
+    """LoGFilter (FB/transient_postprocessing.cuh:52-83), clamp-to-edge."""
+    v = _f32(v)
+    Z, Y, X = v.shape
+    w = np.ascontiguousarray(log_kernel(k, sigma) if weights is None else weights, np.float64)
+    out = np.empty(v.shape, np.float64)
+    load().oracle_log_filter(_p(v), _p(out), X, Y, Z, k, _p(w))
+    return out
+
+
+def normalize(v) -> np.ndarray:
+    """normalizeReconstruction after min/max (FB/Laser.cu:317-339)."""
+    v = np.array(v, np.float64, copy=True)
+    load().oracle_normalize(_p(v), v.size)
+    return v
+
+
+def filter_h_nt_pad(nt, dt, wl_mean, wl_sigma=0.0) -> int:
+    return int(load().oracle_filter_h_nt_pad(nt, dt, wl_mean, wl_sigma))
+
+
+def filter_h(data, dt, wl_mean, wl_sigma=0.0, border=0) -> np.ndarray:
+    """Laser::filter_H_cuda (FB/Laser.cu:128-240): unnormalised, FP64, direct convolution."""
+    data = _f32(data)
+    nt = data.shape[-1]
+    batch = data.size // nt
+    out = np.empty(data.shape, np.float64)
+    load().oracle_filter_h(_p(data), batch, nt, dt, wl_mean, wl_sigma, border, _p(out))
+    return out

@@ -1,0 +1,201 @@
+"""GPU parity of the backprojection kernels against the CPU oracle.
+
+Bins are bit-exact requirements: the transients used here have values that
+vary over orders of magnitude from bin to bin, so one wrong bin in one
+(voxel, scan point) pair moves that voxel far outside the FP32 tolerance.
+Tolerances: rel-L2 <= 1e-4 against the FP64 oracle for FP32 storage,
+per-voxel relative error <= 5e-4 on voxels above 1% of the maximum."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from fasterbackprojection_amd.api import (LAYOUT_TIME_MAJOR, STORAGE_BF16, STORAGE_F16, Geometry,
+                                          make_info)
+from fasterbackprojection_amd.scene import small_confocal, wall_grid
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def assert_volume_close(gpu, ref, tol=1e-4, vox_tol=5e-4):
+    gpu = gpu.detach().cpu().numpy().astype(np.float64)
+    assert gpu.shape == ref.shape
+    e = rel_l2(gpu, ref)
+    assert e <= tol, f"rel-L2 {e:.3e}"
+    big = np.abs(ref) > 0.01 * np.abs(ref).max()
+    worst = np.max(np.abs(gpu[big] - ref[big]) / np.abs(ref[big]))
+    assert worst <= vox_tol, f"worst voxel rel err {worst:.3e}"
+
+
+def spiky(rng, shape):
+    """Transient whose adjacent bins differ by orders of magnitude."""
+    v = (10.0 ** rng.uniform(-3, 3, size=shape)).astype(np.float32)
+    v[rng.random(shape) < 0.2] = 0.0
+    return v
+
+
+def geom(n_scan=16, T=256, res=(24, 20, 28), delta=0.0096, discard=1, t0=0.0,
+         hmin=(-0.4, 0.35, -0.45), hmax=(0.45, 1.25, 0.4), wall=1.0):
+    scan = wall_grid(n_scan, wall)
+    return Geometry(laser_targets=scan, sensor_targets=scan, num_time_bins=T, time_step=delta,
+                    hidden_min=hmin, hidden_max=hmax, voxel_resolution=res,
+                    discard_first_last_bounces=discard, time_offset=t0,
+                    laser_position=(0.1, -0.8, 0.2), sensor_position=(-0.2, -0.7, 0.1),
+                    wall_size=(wall, wall))
+
+
+def cuda(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.parametrize("discard,t0", [(1, 0.0), (0, 0.0), (0, 0.45), (1, 0.3)])
+def test_confocal_spiky_parity(ctx, discard, t0):
+    """General geometry (non-multiple-of-16 volume, non-aligned voxels),
+    every bin path: pure (t0 = 0, discard) and general."""
+    g = geom(discard=discard, t0=t0)
+    info = make_info(g)
+    rng = np.random.default_rng(11 + discard)
+    I = spiky(rng, (len(g.laser_targets), g.num_time_bins))
+    ref = oracle.confocal_voxel(info, I, g.laser_targets)
+    out = ctx.backproject_confocal(info, cuda(I), cuda(g.laser_targets))
+    torch.cuda.synchronize()
+    assert_volume_close(out, ref)
+
+
+def test_confocal_small_config_parity(ctx):
+    """32x32 scan, 256 bins, 200 points, Poisson at peak 1000, 32^3 volume."""
+    cap = small_confocal(seed=1)
+    info = make_info(cap.geometry)
+    ref = oracle.confocal_voxel(info, cap.transient, cap.geometry.laser_targets)
+    out = ctx.backproject_confocal(info, cuda(cap.transient), cuda(cap.geometry.laser_targets))
+    torch.cuda.synchronize()
+    assert_volume_close(out, ref)
+
+
+def test_confocal_time_major_layout(ctx):
+    g = geom(res=(16, 16, 16))
+    info = make_info(g)
+    rng = np.random.default_rng(5)
+    I = spiky(rng, (len(g.laser_targets), g.num_time_bins))
+    ref = oracle.confocal_voxel(info, I, g.laser_targets)
+    out = ctx.backproject_confocal(info, cuda(np.ascontiguousarray(I.T)), cuda(g.laser_targets),
+                                   layout=LAYOUT_TIME_MAJOR)
+    torch.cuda.synchronize()
+    assert_volume_close(out, ref)
+
+
+def test_confocal_near_bin_edges(ctx):
+    """Scan pitch and voxel pitch chosen so many distances land within a few
+    ulps of bin edges (dt = voxel pitch): the exact fallback must fire."""
+    scan = wall_grid(16, 1.0)
+    g = Geometry(laser_targets=scan, sensor_targets=scan, num_time_bins=512, time_step=0.0625,
+                 hidden_min=(-0.5, 0.5, -0.5), hidden_max=(0.5, 1.5, 0.5), voxel_resolution=(16, 16, 16),
+                 wall_size=(1.0, 1.0))
+    info = make_info(g)
+    rng = np.random.default_rng(9)
+    I = spiky(rng, (256, 512))
+    ref = oracle.confocal_voxel(info, I, scan)
+    out = ctx.backproject_confocal(info, cuda(I), cuda(scan))
+    torch.cuda.synchronize()
+    assert_volume_close(out, ref)
+
+
+def test_confocal_deterministic(ctx):
+    cap = small_confocal(seed=2)
+    info = make_info(cap.geometry)
+    a = ctx.backproject_confocal(info, cuda(cap.transient), cuda(cap.geometry.laser_targets))
+    b = ctx.backproject_confocal(info, cuda(cap.transient), cuda(cap.geometry.laser_targets))
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("storage,tol", [(STORAGE_F16, 2e-3), (STORAGE_BF16, 8e-3)])
+def test_confocal_quantized_storage(ctx, storage, tol):
+    """Precision form: FP16/BF16 with per-scan-point max scale."""
+    cap = small_confocal(seed=3)
+    info = make_info(cap.geometry)
+    ref = oracle.confocal_voxel(info, cap.transient, cap.geometry.laser_targets)
+    I = cuda(cap.transient)
+    q, sc = ctx.quantize(I, storage)
+    out = ctx.backproject_confocal(info, q, cuda(cap.geometry.laser_targets), storage=storage,
+                                   scales=sc)
+    torch.cuda.synchronize()
+    e = rel_l2(out.cpu().numpy(), ref)
+    assert e <= tol, e
+
+
+def test_exhaustive_spiky_parity(ctx):
+    lasers = wall_grid(4, 1.0)
+    sensors = wall_grid(6, 1.0)
+    g = Geometry(laser_targets=lasers, sensor_targets=sensors, num_time_bins=200,
+                 time_step=0.0096, hidden_min=(-0.4, 0.35, -0.45), hidden_max=(0.45, 1.25, 0.4),
+                 voxel_resolution=(20, 18, 12), confocal=False, wall_size=(1.0, 1.0))
+    info = make_info(g)
+    rng = np.random.default_rng(4)
+    I = spiky(rng, (16, 36, 200))
+    ref = oracle.exhaustive_voxel(info, I, lasers, sensors)
+    out = ctx.backproject_exhaustive(info, cuda(I), cuda(lasers), cuda(sensors))
+    torch.cuda.synchronize()
+    assert_volume_close(out, ref)
+
+
+@pytest.mark.parametrize("discard,t0", [(0, 0.2)])
+def test_exhaustive_general_chain(ctx, discard, t0):
+    lasers = wall_grid(3, 1.0)
+    sensors = wall_grid(5, 1.0)
+    g = Geometry(laser_targets=lasers, sensor_targets=sensors, num_time_bins=300,
+                 time_step=0.0096, hidden_min=(-0.4, 0.35, -0.45), hidden_max=(0.45, 1.25, 0.4),
+                 voxel_resolution=(16, 16, 8), confocal=False, discard_first_last_bounces=discard,
+                 time_offset=t0, laser_position=(0.1, -0.8, 0.2), sensor_position=(-0.2, -0.7, 0.1),
+                 wall_size=(1.0, 1.0))
+    info = make_info(g)
+    rng = np.random.default_rng(8)
+    I = spiky(rng, (9, 25, 300))
+    ref = oracle.exhaustive_voxel(info, I, lasers, sensors)
+    out = ctx.backproject_exhaustive(info, cuda(I), cuda(lasers), cuda(sensors))
+    torch.cuda.synchronize()
+    assert_volume_close(out, ref)
+
+
+def test_depth_planes(ctx):
+    g = geom(n_scan=8, res=(4, 4, 4))
+    info = make_info(g)
+    rng = np.random.default_rng(2)
+    I = spiky(rng, (64, 256))
+    depths = np.linspace(0.6, 0.8, 20, endpoint=False)
+    ref = oracle.depth(info, I, g.laser_targets, depths)
+    out = ctx.backproject_depth(info, cuda(I), cuda(g.laser_targets), cuda(depths))
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=1e-5, atol=1e-6)
+
+
+def test_mis_reference_mode_matches_oracle(ctx):
+    cap = small_confocal(seed=4, res=16)
+    info = make_info(cap.geometry)
+    I = cap.transient
+    L = len(cap.geometry.laser_targets)
+    cdf = oracle.mis_cdf(I)
+    alias, prob = oracle.alias_tables(cdf)
+    rng = np.random.default_rng(0)
+    noise = rng.random(16 ** 3).astype(np.float32)
+    ref = oracle.mis(info, I, cap.geometry.laser_targets, cdf, alias, prob, noise, L // 4, 0)
+    out = ctx.backproject_confocal_mis(info, cuda(I), cuda(cap.geometry.laser_targets),
+                                       cuda(noise), L // 4, mode=0)
+    torch.cuda.synchronize()
+    assert rel_l2(out.cpu().numpy(), ref) < 1e-4
+
+
+def test_rejects_bad_arguments(ctx):
+    from fasterbackprojection_amd.api import FbpError
+    g = geom()
+    info = make_info(g)
+    info.time_step = 0.0
+    I = torch.zeros((len(g.laser_targets), g.num_time_bins), device="cuda")
+    with pytest.raises(FbpError):
+        ctx.backproject_confocal(info, I, cuda(g.laser_targets))

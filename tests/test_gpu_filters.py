@@ -1,0 +1,53 @@
+"""GPU parity of the post-processing filters against the CPU oracle."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("shape,k", [((32, 32, 32), 5), ((17, 23, 29), 5), ((12, 10, 9), 3),
+                                     ((8, 8, 8), 4)])
+def test_laplacian(ctx, shape, k):
+    rng = np.random.default_rng(0)
+    v = rng.random(shape).astype(np.float32)
+    ref = oracle.laplacian(v, k)
+    out = ctx.laplacian(torch.from_numpy(v).cuda(), k)
+    torch.cuda.synchronize()
+    assert rel_l2(out.cpu().numpy(), ref) < 1e-5
+
+
+@pytest.mark.parametrize("shape,k,sigma", [((32, 32, 32), 5, 1.0), ((19, 21, 33), 5, 1.3),
+                                           ((16, 16, 16), 7, 0.8), ((16, 16, 16), 5, 0.2),
+                                           ((10, 12, 14), 4, 1.0)])
+def test_log(ctx, shape, k, sigma):
+    """Separable path (and dense path for sigma = 0.2, where taps are
+    thresholded away, and even k) against the dense FP64 oracle."""
+    rng = np.random.default_rng(1)
+    v = rng.random(shape).astype(np.float32)
+    w = np.empty(k ** 3, np.float32)
+    from fasterbackprojection_amd import _lib
+    import ctypes
+    _lib.load().fbp_log_kernel_host(k, sigma, w.ctypes.data_as(ctypes.c_void_p))
+    ref = oracle.log_filter(v, k, sigma, weights=w.astype(np.float64).reshape(k, k, k))
+    out = ctx.log(torch.from_numpy(v).cuda(), k, sigma)
+    torch.cuda.synchronize()
+    assert rel_l2(out.cpu().numpy(), ref) < 1e-4
+
+
+def test_normalize(ctx):
+    rng = np.random.default_rng(2)
+    v = (rng.random((20, 30, 40)) * 7 - 3).astype(np.float32)
+    ref = oracle.normalize(v)
+    t = torch.from_numpy(v).cuda()
+    ctx.normalize(t)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(t.cpu().numpy(), ref, rtol=1e-6, atol=1e-7)
+    mn, mx = ctx.minmax(t)
+    assert mn == 0.0 and abs(mx - 1.0) < 1e-6

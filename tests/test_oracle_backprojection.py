@@ -1,0 +1,148 @@
+"""Pins the oracle's backprojection against mathematics and an independent
+evaluation of the reference expression (CPU only)."""
+import numpy as np
+import pytest
+
+import oracle
+from fasterbackprojection_amd.api import Geometry, make_info
+from fasterbackprojection_amd.scene import grid_coord, wall_grid
+
+
+def _geom(n_scan=8, T=128, res=(6, 5, 7), delta=0.0096, hmin=(-0.4, 0.4, -0.3),
+          hmax=(0.4, 1.2, 0.35), discard=1, t0=0.0, wall=1.0):
+    scan = wall_grid(n_scan, wall)
+    return Geometry(laser_targets=scan, sensor_targets=scan, num_time_bins=T, time_step=delta,
+                    hidden_min=hmin, hidden_max=hmax, voxel_resolution=res,
+                    discard_first_last_bounces=discard, time_offset=t0,
+                    laser_position=(0.1, -0.8, 0.2), sensor_position=(-0.2, -0.7, 0.1),
+                    wall_size=(wall, wall))
+
+
+def _centres(info):
+    f = np.float32
+    vs = info.hidden_volume_voxel_size
+    hm = info.hidden_volume_min
+    r = info.voxel_resolution
+    return (grid_coord(hm.x, vs.x, r.x), grid_coord(hm.y, vs.y, r.z), grid_coord(hm.z, vs.z, r.y))
+
+
+def test_voxel_centre_formula_matches_grid():
+    info = make_info(_geom())
+    cx, cy, cz = _centres(info)
+    laser = np.zeros((1, 3), np.float32)
+    # distance from origin to voxel (vx, vy, z) must use x <- vx, y <- z, z <- vy
+    for vx, vy, z in [(0, 0, 0), (5, 4, 6), (2, 3, 1)]:
+        d = oracle.confocal_distf(info, laser, 0, vx, vy, z)
+        r = np.sqrt(float(cx[vx]) ** 2 + float(cy[z]) ** 2 + float(cz[vy]) ** 2)
+        assert abs(d - 2 * r) < 1e-5
+
+
+@pytest.mark.parametrize("discard,t0", [(1, 0.0), (0, 0.0), (0, 0.37), (1, 0.2)])
+def test_bins_match_exact_math_away_from_edges(discard, t0):
+    """Where the real-number quotient is not within 1e-4 of an integer, the
+    oracle's bin must be its truncation (catches dropped terms, wrong signs,
+    swapped axes)."""
+    g = _geom(discard=discard, t0=t0)
+    info = make_info(g)
+    cx, cy, cz = _centres(info)
+    L = g.laser_targets.astype(np.float64)
+    Lp = np.array([0.1, -0.8, 0.2]), np.array([-0.2, -0.7, 0.1])
+    rng = np.random.default_rng(3)
+    checked = 0
+    for _ in range(400):
+        l = int(rng.integers(len(L)))
+        vx, vy, z = (int(rng.integers(6)), int(rng.integers(5)), int(rng.integers(7)))
+        p = np.array([cx[vx], cy[z], cz[vy]], np.float64)
+        dist = 2 * np.linalg.norm(p - L[l]) - np.float64(np.float32(t0))
+        if discard == 0:
+            lp = np.array([0.1, -0.8, 0.2], np.float32).astype(np.float64)
+            sp = np.array([-0.2, -0.7, 0.1], np.float32).astype(np.float64)
+            dist += np.linalg.norm(lp - L[l]) + np.linalg.norm(sp - L[l])
+        q = dist / np.float64(np.float32(0.0096))
+        b = oracle.confocal_bin(info, g.laser_targets, l, vx, vy, z)
+        if abs(q - round(q)) < 1e-4:
+            continue
+        expect = int(np.trunc(q)) if -1 < q < 128 else -1
+        assert b == expect, (l, vx, vy, z, q, b)
+        checked += 1
+    assert checked > 300
+
+
+def test_bins_match_independent_float32_evaluation():
+    """numpy float32 (IEEE, no fusion) evaluation of the reference expression,
+    including pairs whose quotient lies within a few ulps of a bin edge."""
+    g = _geom(n_scan=16, res=(16, 16, 16))
+    info = make_info(g)
+    cx, cy, cz = _centres(info)
+    f = np.float32
+    L = g.laser_targets
+    rng = np.random.default_rng(7)
+    n = 0
+    for _ in range(3000):
+        l = int(rng.integers(len(L)))
+        vx, vy, z = (int(x) for x in rng.integers(16, size=3))
+        dx = f(cx[vx]) - L[l, 0]
+        dy = f(cy[z]) - L[l, 1]
+        dz = f(cz[vy]) - L[l, 2]
+        s = f(f(dx * dx) + f(dy * dy)) + f(dz * dz)
+        d = f(np.sqrt(f(s)) * f(2.0))
+        d = f(d - f(0.0))
+        q = f(d / f(0.0096))
+        e = int(q) if (q > -1 and q < 128) else -1
+        assert oracle.confocal_bin(info, L, l, vx, vy, z) == e
+        n += 1
+    assert n == 3000
+
+
+def test_single_scatterer_peaks_at_its_voxel():
+    """Known answer: one point scatterer at a voxel centre; the backprojection
+    maximum is that voxel."""
+    g = _geom(n_scan=16, T=256, res=(8, 8, 8), hmin=(-0.4, 0.4, -0.4), hmax=(0.4, 1.2, 0.4))
+    info = make_info(g)
+    cx, cy, cz = _centres(info)
+    target = (3, 5, 4)  # (vx, vy, z)
+    p = np.array([cx[target[0]], cy[target[2]], cz[target[1]]], np.float64)
+    L = g.laser_targets.astype(np.float64)
+    I = np.zeros((len(L), 256), np.float32)
+    for l in range(len(L)):
+        r = np.linalg.norm(p - L[l])
+        I[l, int(2 * r / np.float32(0.0096))] = 1.0
+    vol = oracle.confocal_voxel(info, I, g.laser_targets)
+    z, vy, vx = np.unravel_index(np.argmax(vol), vol.shape)
+    assert (vx, vy, z) == target
+
+
+def test_time_major_layout_is_a_transpose():
+    g = _geom()
+    info = make_info(g)
+    rng = np.random.default_rng(0)
+    I = rng.random((len(g.laser_targets), 128)).astype(np.float32)
+    a = oracle.confocal_voxel(info, I, g.laser_targets, layout=0)
+    b = oracle.confocal_voxel(info, np.ascontiguousarray(I.T), g.laser_targets, layout=1)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_exhaustive_equals_confocal_when_only_diagonal_pairs_lit():
+    """With I[l][s] nonzero only for s == l and sensor grid == laser grid, the
+    exhaustive sum reduces to the confocal one (|l-p| + |p-l| = 2|l-p| in
+    float: a + a == a * 2 exactly)."""
+    g = _geom(n_scan=4, res=(4, 4, 4))
+    info = make_info(g)
+    info.capture_system = 1
+    L = len(g.laser_targets)
+    rng = np.random.default_rng(1)
+    Ic = rng.random((L, 128)).astype(np.float32)
+    Ie = np.zeros((L, L, 128), np.float32)
+    for l in range(L):
+        Ie[l, l] = Ic[l]
+    a = oracle.exhaustive_voxel(info, Ie, g.laser_targets, g.laser_targets)
+    b = oracle.confocal_voxel(info, Ic, g.laser_targets)
+    np.testing.assert_allclose(a, b, rtol=1e-12)
+
+
+def test_threshold_and_range():
+    """Samples <= 1e-5 never contribute; bins outside [0, T) never contribute."""
+    g = _geom(T=16)  # most bins fall outside
+    info = make_info(g)
+    I = np.full((len(g.laser_targets), 16), 1e-5, np.float32)
+    assert np.all(oracle.confocal_voxel(info, I, g.laser_targets) == 0)
