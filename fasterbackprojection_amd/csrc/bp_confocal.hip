@@ -40,9 +40,12 @@ enum { P_LX = 0, P_LY, P_LZ, P_BASE, P_HME, P_COFF, P_NEAR, P_OFF };
 // when PURE).  Returns true when the estimate may round to a different bin
 // than the IEEE chain.  The fix-up pass calls this again on the same s, so
 // the fast contribution it removes is bit-identical to the one added.
+// NEAR: the scan point may be (almost) on a voxel centre or dist may be near
+// zero; such lanes always take the exact path and their fast contribution is
+// zero at an in-window bin (ksafe), so no inf/NaN reaches the accumulator.
 template <bool PURE, bool NEAR>
-__device__ __forceinline__ bool confocal_fast(float s, float hme, float coff, float inv_dt, int& k,
-                                              float& w) {
+__device__ __forceinline__ bool confocal_fast(float s, float hme, float coff, float inv_dt,
+                                              int ksafe, int& k, float& w) {
     if constexpr (PURE) {
         const float rs = __builtin_amdgcn_rsqf(s);
         const float q = (s * rs) * inv_dt;  // ~ 2 r / dt
@@ -50,7 +53,13 @@ __device__ __forceinline__ bool confocal_fast(float s, float hme, float coff, fl
         const float f = __builtin_amdgcn_fractf(q);
         w = rs * rs;                        // ~ 1 / (2r)^2
         bool flag = fabsf(f - 0.5f) > hme;
-        if constexpr (NEAR) flag = flag || (s < 2.0e-5f);
+        if constexpr (NEAR) {
+            if (!(s >= 2.0e-5f)) {
+                k = ksafe;
+                w = 0.f;
+                flag = true;
+            }
+        }
         return flag;
     } else {
         const float rs = __builtin_amdgcn_rsqf(s);
@@ -61,7 +70,13 @@ __device__ __forceinline__ bool confocal_fast(float s, float hme, float coff, fl
         const float d2 = dist * dist;
         w = __builtin_amdgcn_rcpf(d2);
         bool flag = fabsf(f - 0.5f) > hme;
-        if constexpr (NEAR) flag = flag || (d2 < 2.0e-5f);
+        if constexpr (NEAR) {
+            if (!(d2 >= 2.0e-5f) || !(s >= 1.0e-6f)) {
+                k = ksafe;
+                w = 0.f;
+                flag = true;
+            }
+        }
         return flag;
     }
 }
@@ -69,7 +84,7 @@ __device__ __forceinline__ bool confocal_fast(float s, float hme, float coff, fl
 template <bool PURE, bool NEAR>
 __device__ __forceinline__ void confocal_update_tile(float (&acc)[64], const float (&A)[4][4],
                                                      const float (&Z2)[4], const float* win,
-                                                     int base, float hme, float coff,
+                                                     int base, int ksafe, float hme, float coff,
                                                      const ConfocalArgs& a, uint32_t& m0,
                                                      uint32_t& m1) {
 #pragma unroll
@@ -83,7 +98,8 @@ __device__ __forceinline__ void confocal_update_tile(float (&acc)[64], const flo
                 const float s = A[i][j] + Z2[m];
                 int k;
                 float w;
-                if (__builtin_expect(confocal_fast<PURE, NEAR>(s, hme, coff, a.inv_dt, k, w), 0)) {
+                if (__builtin_expect(
+                        confocal_fast<PURE, NEAR>(s, hme, coff, a.inv_dt, ksafe, k, w), 0)) {
                     if (slot < 32)
                         m0 |= 1u << slot;
                     else
@@ -161,7 +177,7 @@ __global__ __launch_bounds__(64, 2) void confocal_voxel_kernel(ConfocalArgs a) {
             P[P_BASE] = __int_as_float(lane * a.W - kmin);
             P[P_HME] = 0.5f - eps;
             P[P_COFF] = off - a.t0;
-            P[P_NEAR] = (dmin < 0.0045f && dmax > -0.0045f) ? 1.f : 0.f;
+            P[P_NEAR] = ((dmin < 0.0045f && dmax > -0.0045f) || rmin < 1.0e-3f) ? 1.f : 0.f;
             P[P_OFF] = off;
         }
         __syncthreads();
@@ -205,11 +221,13 @@ __global__ __launch_bounds__(64, 2) void confocal_voxel_kernel(ConfocalArgs a) {
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) A[i][j] = rn_add(X2[i], Y2[j]);
+            // Bin of window element 0 of this scan point: always in the window.
+            const int ksafe = jl * a.W - base;
             uint32_t m0 = 0, m1 = 0;
             if (near)
-                confocal_update_tile<PURE, true>(acc, A, Z2, win, base, hme, coff, a, m0, m1);
+                confocal_update_tile<PURE, true>(acc, A, Z2, win, base, ksafe, hme, coff, a, m0, m1);
             else
-                confocal_update_tile<PURE, false>(acc, A, Z2, win, base, hme, coff, a, m0, m1);
+                confocal_update_tile<PURE, false>(acc, A, Z2, win, base, ksafe, hme, coff, a, m0, m1);
             // Fix-up of the few voxels whose fast bin may be off by one: the
             // exact contribution replaces the fast one in the lane-owned
             // correction slot of the partial buffer.
@@ -233,9 +251,9 @@ __global__ __launch_bounds__(64, 2) void confocal_voxel_kernel(ConfocalArgs a) {
                     int kf;
                     float wf;
                     if (near)
-                        confocal_fast<PURE, true>(s, hme, coff, a.inv_dt, kf, wf);
+                        confocal_fast<PURE, true>(s, hme, coff, a.inv_dt, ksafe, kf, wf);
                     else
-                        confocal_fast<PURE, false>(s, hme, coff, a.inv_dt, kf, wf);
+                        confocal_fast<PURE, false>(s, hme, coff, a.inv_dt, ksafe, kf, wf);
                     const float dist = confocal_dist(PURE ? s * 0.25f : s, a.t0, a.discard, off);
                     const int ke = bin_of(dist, a.dt);
                     const float we = safe_rcp(rn_mul(dist, dist));

@@ -61,9 +61,9 @@ __device__ __forceinline__ void exhaustive_update_tile(float (&acc)[32], const f
                 const float d2 = dist * dist;
                 float w = __builtin_amdgcn_rcpf(d2);
                 bool flag = fabsf(f - 0.5f) > hme;
-                if constexpr (NEAR) flag = flag || (d2 < 2.0e-5f);
+                if constexpr (NEAR) flag = flag || !(d2 >= 2.0e-5f) || !(ss >= 1.0e-6f);
                 if (__builtin_expect(flag, 0)) {
-                    const float sl = AL[i][j] + ZL2[m];
+                    const float sl = AL[i][j] + ZL2[m];  // exact (unscaled) tables
                     const float de = exhaustive_dist(sl, ss, a.t0, a.discard, off);
                     k = bin_of(de, a.dt);
                     w = safe_rcp(rn_mul(de, de));
@@ -75,7 +75,7 @@ __device__ __forceinline__ void exhaustive_update_tile(float (&acc)[32], const f
 }
 
 template <int STORAGE>
-__global__ __launch_bounds__(64, 4) void exhaustive_voxel_kernel(ExhaustiveArgs a) {
+__global__ __launch_bounds__(64, 2) void exhaustive_voxel_kernel(ExhaustiveArgs a) {
     extern __shared__ float smem[];
     float* win = smem;
     float* prm = smem + a.batch * a.W;
@@ -136,7 +136,10 @@ __global__ __launch_bounds__(64, 4) void exhaustive_voxel_kernel(ExhaustiveArgs 
             P[Q_BASE] = __int_as_float(lane * a.W - kmin);
             P[Q_HME] = 0.5f - eps;
             P[Q_COFF] = off - a.t0;
-            P[Q_NEAR] = (dmin < 0.0045f && dmax > -0.0045f) ? 1.f : 0.f;
+            P[Q_NEAR] = ((dmin < 0.0045f && dmax > -0.0045f) || rlmin < 1.0e-3f ||
+                         rsmin < 1.0e-3f)
+                            ? 1.f
+                            : 0.f;
             P[Q_OFF] = off;
         }
         __syncthreads();
@@ -180,7 +183,7 @@ __global__ __launch_bounds__(64, 4) void exhaustive_voxel_kernel(ExhaustiveArgs 
                     for (int j = 0; j < 2; ++j)
 #pragma unroll
                         for (int m = 0; m < 4; ++m) {
-                            const float sl = AL[i][j] + ZL2[m];
+                            const float sl = AL[i][j] + ZL2[m];  // exact (unscaled) tables
                             rl[(i * 2 + j) * 4 + m] = sl * __builtin_amdgcn_rsqf(sl);
                         }
             }
