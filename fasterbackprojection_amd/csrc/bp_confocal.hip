@@ -7,12 +7,18 @@
 // Here one wave owns a 16^3 voxel tile (64 voxels per lane in registers) and a
 // chunk of scan points.  For every scan point the wave stages the window of
 // time bins its tile can reach into LDS (thresholded at I > 1e-5, zero outside
-// [0, T)), then every lane gathers from that window for its 64 voxels.  The
-// per-scan-point squared distances are built from per-axis exact tables in
-// glm::dot's order, so s is the reference's float value; the bin is found
-// with v_rsq and checked against the nearest integer: when the fast quotient
-// is within the proven error bound of a bin edge, the lane recomputes the
-// bin with the IEEE sequence of chain.h.  Chunk partial sums are reduced in
+// [0, T)), then every lane gathers from that window for its 64 voxels.
+//
+// Exact bins without per-voxel IEEE sqrt/div: the reference's bin is a
+// monotone step function of the squared distance s (sqrt, *2, -t0, +off, /dt
+// and truncation are all monotone), so with S_k = min{s : bin(s) >= k} the
+// bin is the k with S_k <= s < S_{k+1}.  A pre-pass computes S_k by binary
+// search over float bit patterns using the IEEE chain (one row when t0 = 0
+// and the first/last bounces are discarded, one row per scan point
+// otherwise).  The window stores {I[k], S[k]} pairs; the fast path takes a
+// v_rsq estimate biased low so it is k or k-1, reads entries ka and ka+1 in
+// one LDS access and picks I[ka+1] iff s >= S[ka+1].  s itself is exact:
+// per-axis tables in glm::dot's order.  Chunk partial sums are reduced in
 // FP64 in a fixed order, so results are deterministic.
 #include "bp_common.h"
 
@@ -31,62 +37,41 @@ struct ConfocalArgs {
     uint32_t chunk_len;
     int batch;
     int W;
+    const float* thr;      // S_k rows, (T + 1) floats each
+    uint32_t thr_rows;     // 1 (pure chain) or L
+    float* corr;           // near-voxel corrections, same layout as partial
 };
 
 // Per scan point parameters staged in LDS (8 words each).
-enum { P_LX = 0, P_LY, P_LZ, P_BASE, P_HME, P_COFF, P_NEAR, P_OFF };
+enum { P_LX = 0, P_LY, P_LZ, P_BASE, P_QBIAS, P_COFF, P_NEAR, P_OFF };
 
-// Fast estimate of (bin, weight) from the squared distance s (pre-scaled by 4
-// when PURE).  Returns true when the estimate may round to a different bin
-// than the IEEE chain.  The fix-up pass calls this again on the same s, so
-// the fast contribution it removes is bit-identical to the one added.
-// NEAR: the scan point may be (almost) on a voxel centre or dist may be near
-// zero; such lanes always take the exact path and their fast contribution is
-// zero at an in-window bin (ksafe), so no inf/NaN reaches the accumulator.
-template <bool PURE, bool NEAR>
-__device__ __forceinline__ bool confocal_fast(float s, float hme, float coff, float inv_dt,
-                                              int ksafe, int& k, float& w) {
+// Fast gather of one (voxel, scan point) pair.  s is exact (x4 when PURE);
+// the estimate ka is biased low so the exact bin is ka or ka + 1, decided by
+// the staged threshold S_{ka+1}.  e = {I[k], S[k]} pairs from the window.
+template <bool PURE>
+__device__ __forceinline__ float confocal_sample(float s, float cb, float coff, float qbias,
+                                                 float inv_dt, const float2* win, int base,
+                                                 float& w) {
+    const float rs = __builtin_amdgcn_rsqf(s);
+    int ka;
     if constexpr (PURE) {
-        const float rs = __builtin_amdgcn_rsqf(s);
-        const float q = (s * rs) * inv_dt;  // ~ 2 r / dt
-        k = static_cast<int>(q);
-        const float f = __builtin_amdgcn_fractf(q);
-        w = rs * rs;                        // ~ 1 / (2r)^2
-        bool flag = fabsf(f - 0.5f) > hme;
-        if constexpr (NEAR) {
-            if (!(s >= 2.0e-5f)) {
-                k = ksafe;
-                w = 0.f;
-                flag = true;
-            }
-        }
-        return flag;
+        ka = static_cast<int>((s * rs) * cb);     // ~ 2 r / dt, biased low
+        w = rs * rs;                              // ~ 1 / (2r)^2
     } else {
-        const float rs = __builtin_amdgcn_rsqf(s);
         const float dist = fmaf(s * rs, 2.0f, coff);
-        const float q = dist * inv_dt;
-        k = static_cast<int>(q);
-        const float f = __builtin_amdgcn_fractf(q);
-        const float d2 = dist * dist;
-        w = __builtin_amdgcn_rcpf(d2);
-        bool flag = fabsf(f - 0.5f) > hme;
-        if constexpr (NEAR) {
-            if (!(d2 >= 2.0e-5f) || !(s >= 1.0e-6f)) {
-                k = ksafe;
-                w = 0.f;
-                flag = true;
-            }
-        }
-        return flag;
+        ka = static_cast<int>(fmaf(dist, inv_dt, -qbias));
+        w = __builtin_amdgcn_rcpf(dist * dist);
     }
+    const float2 e0 = win[base + ka];
+    const float2 e1 = win[base + ka + 1];
+    return (s >= e1.y) ? e1.x : e0.x;
 }
 
-template <bool PURE, bool NEAR>
+template <bool PURE>
 __device__ __forceinline__ void confocal_update_tile(float (&acc)[64], const float (&A)[4][4],
-                                                     const float (&Z2)[4], const float* win,
-                                                     int base, int ksafe, float hme, float coff,
-                                                     const ConfocalArgs& a, uint32_t& m0,
-                                                     uint32_t& m1) {
+                                                     const float (&Z2)[4], const float2* win,
+                                                     int base, float cb, float coff, float qbias,
+                                                     float inv_dt) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
 #pragma unroll
@@ -96,26 +81,23 @@ __device__ __forceinline__ void confocal_update_tile(float (&acc)[64], const flo
                 const int slot = (i * 4 + j) * 4 + m;
                 // PURE: tables are pre-scaled by 4, so s = 4 * (exact s) exactly.
                 const float s = A[i][j] + Z2[m];
-                int k;
                 float w;
-                if (__builtin_expect(
-                        confocal_fast<PURE, NEAR>(s, hme, coff, a.inv_dt, ksafe, k, w), 0)) {
-                    if (slot < 32)
-                        m0 |= 1u << slot;
-                    else
-                        m1 |= 1u << (slot - 32);
-                }
-                acc[slot] = fmaf(win[base + k], w, acc[slot]);
+                const float v = confocal_sample<PURE>(s, cb, coff, qbias, inv_dt, win, base, w);
+                acc[slot] = fmaf(v, w, acc[slot]);
             }
         }
     }
 }
 
+__device__ __forceinline__ float pick4(const float (&v)[4], int i) {
+    return i == 0 ? v[0] : i == 1 ? v[1] : i == 2 ? v[2] : v[3];
+}
+
 template <int STORAGE, int LAYOUT, bool PURE>
 __global__ __launch_bounds__(64, 2) void confocal_voxel_kernel(ConfocalArgs a) {
     extern __shared__ float smem[];
-    float* win = smem;
-    float* prm = smem + a.batch * a.W;
+    float2* win = reinterpret_cast<float2*>(smem);           // batch * W pairs
+    float* prm = smem + 2 * a.batch * a.W;                    // batch * 8
 
     const int lane = threadIdx.x;
     const uint32_t tile = blockIdx.x % a.g.num_tiles;
@@ -141,15 +123,16 @@ __global__ __launch_bounds__(64, 2) void confocal_voxel_kernel(ConfocalArgs a) {
     float acc[64];
 #pragma unroll
     for (int i = 0; i < 64; ++i) acc[i] = 0.f;
-    // This wave's slice of the partial buffer doubles as its correction
-    // accumulator; every element is owned by exactly one lane.
-    float* corr = a.partial + (size_t(chunk) * a.g.num_tiles + tile) * 64 * 64;
-#pragma unroll
+    // Near-voxel corrections (rare) go to a lane-owned slice of a.corr.
+    float* corr = a.corr + (size_t(chunk) * a.g.num_tiles + tile) * 64 * 64;
+#pragma unroll 8
     for (int s = 0; s < 64; ++s) corr[s * 64 + lane] = 0.f;
 
     const uint32_t l_begin = chunk * a.chunk_len;
     const uint32_t l_end = min(a.L, l_begin + a.chunk_len);
     const float tscale = PURE ? 4.0f : 1.0f;
+    // Bias of the pure-chain estimate: 2^-18 relative, far above its error.
+    const float cb = a.inv_dt * (1.0f - 3.814697265625e-06f);
 
     for (uint32_t lb = l_begin; lb < l_end; lb += a.batch) {
         const int nb = int(min(uint32_t(a.batch), l_end - lb));
@@ -166,16 +149,16 @@ __global__ __launch_bounds__(64, 2) void confocal_voxel_kernel(ConfocalArgs a) {
             const float dmin = 2.f * rmin - a.t0 + off;
             const float dmax = 2.f * rmax - a.t0 + off;
             const int kmin = int(floorf(dmin * a.inv_dt)) - 3;
-            // Error bound of the fast quotient against the IEEE chain: a few
-            // ulps of the largest magnitude entering dist, taken generously.
+            // Low bias of the general-chain estimate (in bins): a generous
+            // multiple of the rounding error of the largest term of dist.
             const float mag = 2.f * rmax + fabsf(a.t0) + fabsf(off);
-            const float eps = 16.f * 1.1920929e-7f * mag * a.inv_dt + 1.0e-6f;
+            const float qbias = 32.f * 1.1920929e-7f * mag * a.inv_dt + 1.0e-5f;
             float* P = prm + lane * 8;
             P[P_LX] = p[0];
             P[P_LY] = p[1];
             P[P_LZ] = p[2];
             P[P_BASE] = __int_as_float(lane * a.W - kmin);
-            P[P_HME] = 0.5f - eps;
+            P[P_QBIAS] = qbias;
             P[P_COFF] = off - a.t0;
             P[P_NEAR] = ((dmin < 0.0045f && dmax > -0.0045f) || rmin < 1.0e-3f) ? 1.f : 0.f;
             P[P_OFF] = off;
@@ -186,6 +169,7 @@ __global__ __launch_bounds__(64, 2) void confocal_voxel_kernel(ConfocalArgs a) {
             const int kmin = jl * a.W - __float_as_int(prm[jl * 8 + P_BASE]);
             float sc = 1.f;
             if constexpr (STORAGE != FBP_STORAGE_F32) sc = a.scales[l];
+            const float* thr = a.thr + size_t(a.thr_rows > 1 ? l : 0) * (a.T + 1);
             for (int w = lane; w < a.W; w += 64) {
                 const int k = kmin + w;
                 float v = 0.f;
@@ -197,7 +181,10 @@ __global__ __launch_bounds__(64, 2) void confocal_voxel_kernel(ConfocalArgs a) {
                     // Quantised storage was thresholded before quantisation.
                     if constexpr (STORAGE == FBP_STORAGE_F32) v = (v > kEps) ? v : 0.f;
                 }
-                win[jl * a.W + w] = v;
+                // Threshold of bin k (only bins 0..T matter; outside, both
+                // candidate samples are zero).
+                const float t = (k < 0) ? 0.f : (k > int(a.T) ? __int_as_float(0x7f800000) : thr[k]);
+                win[jl * a.W + w] = make_float2(v, t * tscale);
             }
         }
         __syncthreads();
@@ -205,10 +192,24 @@ __global__ __launch_bounds__(64, 2) void confocal_voxel_kernel(ConfocalArgs a) {
             const float* P = prm + jl * 8;
             const float lxp = rfl(P[P_LX]), lyp = rfl(P[P_LY]), lzp = rfl(P[P_LZ]);
             const int base = rfl(__float_as_int(P[P_BASE]));
-            const float hme = rfl(P[P_HME]);
+            const float qbias = rfl(P[P_QBIAS]);
             const float coff = rfl(P[P_COFF]);
             const bool near = rfl(P[P_NEAR]) != 0.f;
-            const float off = rfl(P[P_OFF]);
+            if (__builtin_expect(near, 0)) {
+                // Scan point (almost) on a voxel centre, or dist near zero:
+                // every voxel of the tile goes through the IEEE chain and
+                // lands in the lane-owned correction slots (acc untouched).
+                const float off = rfl(P[P_OFF]);
+                for (int slot = 0; slot < 64; ++slot) {
+                    const float s = rn_add(rn_add(sq(rn_sub(pick4(px, slot >> 4), lxp)),
+                                                  sq(rn_sub(pick4(py, (slot >> 2) & 3), lyp))),
+                                           sq(rn_sub(pick4(pz, slot & 3), lzp)));
+                    const float dist = confocal_dist(s, a.t0, a.discard, off);
+                    const int ke = bin_of(dist, a.dt);
+                    corr[slot * 64 + lane] += win[base + ke].x * safe_rcp(rn_mul(dist, dist));
+                }
+                continue;
+            }
             float X2[4], Y2[4], Z2[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
@@ -221,54 +222,54 @@ __global__ __launch_bounds__(64, 2) void confocal_voxel_kernel(ConfocalArgs a) {
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) A[i][j] = rn_add(X2[i], Y2[j]);
-            // Bin of window element 0 of this scan point: always in the window.
-            const int ksafe = jl * a.W - base;
-            uint32_t m0 = 0, m1 = 0;
-            if (near)
-                confocal_update_tile<PURE, true>(acc, A, Z2, win, base, ksafe, hme, coff, a, m0, m1);
-            else
-                confocal_update_tile<PURE, false>(acc, A, Z2, win, base, ksafe, hme, coff, a, m0, m1);
-            // Fix-up of the few voxels whose fast bin may be off by one: the
-            // exact contribution replaces the fast one in the lane-owned
-            // correction slot of the partial buffer.
-            if (__builtin_expect(__any(int((m0 | m1) != 0)), 0)) {
-                while (m0 | m1) {
-                    int slot;
-                    if (m0) {
-                        slot = __builtin_ctz(m0);
-                        m0 &= m0 - 1;
-                    } else {
-                        slot = 32 + __builtin_ctz(m1);
-                        m1 &= m1 - 1;
-                    }
-                    const int i = slot >> 4, j = (slot >> 2) & 3, m = slot & 3;
-                    const float pxi = i == 0 ? px[0] : i == 1 ? px[1] : i == 2 ? px[2] : px[3];
-                    const float pyj = j == 0 ? py[0] : j == 1 ? py[1] : j == 2 ? py[2] : py[3];
-                    const float pzm = m == 0 ? pz[0] : m == 1 ? pz[1] : m == 2 ? pz[2] : pz[3];
-                    const float s = rn_add(rn_add(sq(rn_sub(pxi, lxp)) * tscale,
-                                                  sq(rn_sub(pyj, lyp)) * tscale),
-                                           sq(rn_sub(pzm, lzp)) * tscale);
-                    int kf;
-                    float wf;
-                    if (near)
-                        confocal_fast<PURE, true>(s, hme, coff, a.inv_dt, ksafe, kf, wf);
-                    else
-                        confocal_fast<PURE, false>(s, hme, coff, a.inv_dt, ksafe, kf, wf);
-                    const float dist = confocal_dist(PURE ? s * 0.25f : s, a.t0, a.discard, off);
-                    const int ke = bin_of(dist, a.dt);
-                    const float we = safe_rcp(rn_mul(dist, dist));
-                    const float delta = win[base + ke] * we - win[base + kf] * wf;
-                    corr[slot * 64 + lane] += delta;
-                }
-            }
+            confocal_update_tile<PURE>(acc, A, Z2, win, base, cb, coff, qbias, a.inv_dt);
         }
     }
 
+    float* out = a.partial + (size_t(chunk) * a.g.num_tiles + tile) * 64 * 64;
 #pragma unroll
-    for (int s = 0; s < 64; ++s) corr[s * 64 + lane] += acc[s];
+    for (int s = 0; s < 64; ++s) out[s * 64 + lane] = acc[s];
 }
 
-__global__ void reduce_partials_kernel(const float* __restrict__ partial, float* __restrict__ volume,
+// S_k = min{s >= 0 : bin(s) >= k}, k = 0..T, by binary search over the bit
+// patterns of non-negative floats (monotone in the value).  One row per scan
+// point, or a single row for the pure chain (t0 = 0, bounces discarded).
+__global__ void confocal_thresholds_kernel(const fbp_vec3* __restrict__ laser, uint32_t rows,
+                                           uint32_t T, float t0, float dt, uint32_t discard,
+                                           fbp_vec3 lpos, fbp_vec3 spos, float* __restrict__ thr) {
+    const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= size_t(rows) * (T + 1)) return;
+    const uint32_t row = uint32_t(idx / (T + 1));
+    const int k = int(idx % (T + 1));
+    float off = 0.f;
+    if (discard == 0) {
+        const float lx = laser[row].x, ly = laser[row].y, lz = laser[row].z;
+        off = rn_add(distance3(lx, ly, lz, lpos.x, lpos.y, lpos.z),
+                     distance3(lx, ly, lz, spos.x, spos.y, spos.z));
+    }
+    // bin(s) >= k ?  (bin = (int)(dist / dt), dist from the IEEE chain)
+    auto reaches = [&](uint32_t bits) {
+        const float s = __uint_as_float(bits);
+        const float q = rn_div(confocal_dist(s, t0, discard, off), dt);
+        return q >= float(k) || (k <= 0 && q > -1.0f);
+    };
+    uint32_t lo = 0u, hi = 0x7f800000u;  // +inf: never reached
+    if (reaches(0u)) {
+        hi = 0u;
+    } else {
+        while (hi - lo > 1u) {
+            const uint32_t mid = lo + (hi - lo) / 2u;
+            if (reaches(mid))
+                hi = mid;
+            else
+                lo = mid;
+        }
+    }
+    thr[idx] = __uint_as_float(hi);
+}
+
+__global__ void reduce_partials_kernel(const float* __restrict__ partial,
+                                       const float* __restrict__ corr, float* __restrict__ volume,
                                        VolumeGeom g, uint32_t num_chunks, double scale) {
     const size_t n = size_t(g.res[0]) * g.res[1] * g.res[2];
     const size_t v = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -282,17 +283,21 @@ __global__ void reduce_partials_kernel(const float* __restrict__ partial, float*
     const uint32_t lane = (ox & 3) + 4 * (oz & 3) + 16 * (oy & 3);
     const uint32_t slot = ((ox >> 2) * g.lt[2] + (oz >> 2)) * g.lt[1] + (oy >> 2);
     double sum = 0.0;
-    for (uint32_t c = 0; c < num_chunks; ++c)
-        sum += double(partial[((size_t(c) * g.num_tiles + tile) * g.slots + slot) * 64 + lane]);
+    for (uint32_t c = 0; c < num_chunks; ++c) {
+        const size_t i = ((size_t(c) * g.num_tiles + tile) * g.slots + slot) * 64 + lane;
+        sum += double(partial[i]);
+        if (corr) sum += double(corr[i]);
+    }
     volume[v] = float(sum * scale);
 }
 
-fbp_status launch_reduce(fbp_context ctx, const float* partial, float* volume, const VolumeGeom& g,
-                         uint32_t num_chunks, double scale, hipStream_t stream) {
+fbp_status launch_reduce(fbp_context ctx, const float* partial, const float* corr, float* volume,
+                         const VolumeGeom& g, uint32_t num_chunks, double scale,
+                         hipStream_t stream) {
     const size_t n = size_t(g.res[0]) * g.res[1] * g.res[2];
     const unsigned blocks = unsigned((n + 255) / 256);
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(blocks), dim3(256), 0, stream, partial, volume,
-                       g, num_chunks, scale);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(blocks), dim3(256), 0, stream, partial, corr,
+                       volume, g, num_chunks, scale);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     return FBP_OK;
 }
@@ -366,20 +371,29 @@ extern "C" fbp_status fbp_backproject_confocal_voxel(fbp_context ctx,
                     "confocal: bin window of %d bins per tile exceeds LDS (voxels too large for "
                     "the bin width)",
                     a.W);
+    // LDS per scan point: W {I, S} pairs + 8 parameters.
     a.batch = 16;
-    while (a.batch > 1 && size_t(a.batch) * (a.W + 8) * 4 > 32768) a.batch /= 2;
+    while (a.batch > 1 && size_t(a.batch) * (2 * a.W + 8) * 4 > 20480) a.batch /= 2;
 
     const uint32_t nc = pick_chunks(ctx, a.g.num_tiles, a.L, 64);
     a.chunk_len = (a.L + nc - 1) / nc;
     const uint32_t chunks = (a.L + a.chunk_len - 1) / a.chunk_len;
-    const size_t partial_bytes = size_t(chunks) * a.g.num_tiles * a.g.slots * 64 * sizeof(float);
-    FBP_CHECK(ensure_scratch(ctx, partial_bytes));
-    a.partial = static_cast<float*>(ctx->scratch);
-
     const bool pure = (a.t0 == 0.f) && (a.discard != 0);
-    const dim3 grid(chunks * a.g.num_tiles);
-    const size_t shmem = size_t(a.batch) * (a.W + 8) * sizeof(float);
+    a.thr_rows = pure ? 1u : a.L;
+    const size_t slab = size_t(chunks) * a.g.num_tiles * a.g.slots * 64;
+    const size_t thr_n = size_t(a.thr_rows) * (a.T + 1);
+    FBP_CHECK(ensure_scratch(ctx, (2 * slab + thr_n) * sizeof(float) + 256));
+    a.partial = static_cast<float*>(ctx->scratch);
+    a.corr = a.partial + slab;
+    a.thr = a.corr + slab;
+
     hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(confocal_thresholds_kernel, dim3(unsigned((thr_n + 255) / 256)), dim3(256),
+                       0, s, laser_targets, a.thr_rows, a.T, a.t0, a.dt, a.discard,
+                       info->laser_position, info->sensor_position, const_cast<float*>(a.thr));
+    FBP_CHECK_HIP(ctx, hipGetLastError());
+    const dim3 grid(chunks * a.g.num_tiles);
+    const size_t shmem = size_t(a.batch) * (2 * a.W + 8) * sizeof(float);
     if (tr->layout == FBP_LAYOUT_TIME_MAJOR) {
         launch_confocal<FBP_STORAGE_F32, FBP_LAYOUT_TIME_MAJOR>(a, pure, grid, shmem, s);
     } else if (tr->storage == FBP_STORAGE_F32) {
@@ -390,5 +404,5 @@ extern "C" fbp_status fbp_backproject_confocal_voxel(fbp_context ctx,
         launch_confocal<FBP_STORAGE_BF16, FBP_LAYOUT_TIME_FASTEST>(a, pure, grid, shmem, s);
     }
     FBP_CHECK_HIP(ctx, hipGetLastError());
-    return launch_reduce(ctx, a.partial, volume, a.g, chunks, 1.0, s);
+    return launch_reduce(ctx, a.partial, a.corr, volume, a.g, chunks, 1.0, s);
 }

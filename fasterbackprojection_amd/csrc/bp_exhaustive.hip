@@ -291,5 +291,5 @@ extern "C" fbp_status fbp_backproject_exhaustive_voxel(fbp_context ctx,
     else
         hipLaunchKernelGGL(exhaustive_voxel_kernel<FBP_STORAGE_BF16>, grid, dim3(64), shmem, s, a);
     FBP_CHECK_HIP(ctx, hipGetLastError());
-    return launch_reduce(ctx, a.partial, volume, a.g, uint32_t(chunks), 1.0, s);
+    return launch_reduce(ctx, a.partial, nullptr, volume, a.g, uint32_t(chunks), 1.0, s);
 }

@@ -22,44 +22,17 @@ namespace fbp {
 
 #define FBP_HD __host__ __device__ __forceinline__
 
-// Separately rounded IEEE operations.  On the host, x86-64 SSE scalar
-// arithmetic is IEEE round-to-nearest and the library is compiled with
-// -ffp-contract=off, so plain operators are already exact.
-FBP_HD float rn_add(float a, float b) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __fadd_rn(a, b);
-#else
-    return a + b;
-#endif
-}
-FBP_HD float rn_sub(float a, float b) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __fsub_rn(a, b);
-#else
-    return a - b;
-#endif
-}
-FBP_HD float rn_mul(float a, float b) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __fmul_rn(a, b);
-#else
-    return a * b;
-#endif
-}
-FBP_HD float rn_div(float a, float b) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __fdiv_rn(a, b);
-#else
-    return a / b;
-#endif
-}
-FBP_HD float rn_sqrt(float a) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __fsqrt_rn(a);
-#else
-    return __builtin_sqrtf(a);
-#endif
-}
+// Separately rounded IEEE operations.  The library is compiled with
+// -ffp-contract=off, so no product/sum is fused.  On the device, HIP's f32
+// division and sqrt are correctly rounded by default
+// (-fhip-fp32-correctly-rounded-divide-sqrt); __fsqrt_rn is NOT used because
+// without OCML_BASIC_ROUNDED_OPERATIONS it maps to the native (approximate)
+// square root.  On the host, x86-64 SSE scalar arithmetic is IEEE.
+FBP_HD float rn_add(float a, float b) { return a + b; }
+FBP_HD float rn_sub(float a, float b) { return a - b; }
+FBP_HD float rn_mul(float a, float b) { return a * b; }
+FBP_HD float rn_div(float a, float b) { return a / b; }
+FBP_HD float rn_sqrt(float a) { return __builtin_sqrtf(a); }
 
 // Voxel centre coordinate along one axis: (hmin + float(i) * vs) + 0.5f * vs.
 FBP_HD float voxel_center(float hmin, float vs, uint32_t i) {
