@@ -1,8 +1,380 @@
-// Frequency-domain paths: temporal band-pass filter_H, LoGFFT and LCT.
+// Frequency-domain paths on rocFFT: temporal band-pass filter_H, LoGFFT and
+// the confocal LCT.  Plans, the filter_H kernel spectrum, the LCT inverse
+// filter and resampling operators are cached on the context.
+#include <rocfft/rocfft.h>
+
+#include <cfloat>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "chain.h"
 #include "fbp_internal.h"
 
 namespace fbp {
-void fft_cache_destroy(fbp_context) {}
+
+namespace {
+
+struct FftPlan {
+    rocfft_plan plan = nullptr;
+    size_t work = 0;
+};
+
+struct Csr {  // M x M sparse operator on the device
+    int* rowptr = nullptr;
+    int* cols = nullptr;
+    float* vals = nullptr;
+};
+
+struct FftCache {
+    std::map<std::string, FftPlan> plans;
+    void* work = nullptr;
+    size_t work_bytes = 0;
+    rocfft_execution_info info = nullptr;
+    // filter_H: half spectrum of Re(K) (complex float, N/2 + 1)
+    std::string khat_key;
+    float2* khat = nullptr;
+    // LCT: half spectrum of the inverse filter (complex float), operators
+    std::string inv_key;
+    float2* invpsf = nullptr;
+    uint32_t op_m = 0;
+    Csr mtx, mtxi;
+};
+
+std::once_flag g_rocfft_once;
+rocfft_status g_rocfft_setup = rocfft_status_success;
+
+FftCache* cache_of(fbp_context ctx) {
+    if (!ctx->fft_cache) ctx->fft_cache = new FftCache();
+    return static_cast<FftCache*>(ctx->fft_cache);
+}
+
+fbp_status fft_fail(fbp_context ctx, rocfft_status st, const char* what) {
+    return fail(ctx, FBP_ERR_FFT, "%s: rocfft status %d", what, int(st));
+}
+
+fbp_status get_plan(fbp_context ctx, rocfft_transform_type type, rocfft_precision prec,
+                    std::vector<size_t> lengths, size_t batch, FftPlan** out) {
+    std::call_once(g_rocfft_once, [] { g_rocfft_setup = rocfft_setup(); });
+    if (g_rocfft_setup != rocfft_status_success) return fft_fail(ctx, g_rocfft_setup, "rocfft_setup");
+    FftCache* c = cache_of(ctx);
+    std::string key = std::to_string(int(type)) + "/" + std::to_string(int(prec)) + "/" +
+                      std::to_string(batch);
+    for (size_t l : lengths) key += "x" + std::to_string(l);
+    auto it = c->plans.find(key);
+    if (it == c->plans.end()) {
+        FftPlan p;
+        rocfft_status st = rocfft_plan_create(&p.plan, rocfft_placement_notinplace, type, prec,
+                                              lengths.size(), lengths.data(), batch, nullptr);
+        if (st != rocfft_status_success) return fft_fail(ctx, st, "rocfft_plan_create");
+        st = rocfft_plan_get_work_buffer_size(p.plan, &p.work);
+        if (st != rocfft_status_success) return fft_fail(ctx, st, "rocfft_plan_get_work_buffer_size");
+        it = c->plans.emplace(key, p).first;
+    }
+    *out = &it->second;
+    return FBP_OK;
+}
+
+fbp_status run_plan(fbp_context ctx, FftPlan* p, void* in, void* out, hipStream_t s) {
+    FftCache* c = cache_of(ctx);
+    if (!c->info) {
+        rocfft_status st = rocfft_execution_info_create(&c->info);
+        if (st != rocfft_status_success) return fft_fail(ctx, st, "rocfft_execution_info_create");
+    }
+    if (p->work > c->work_bytes) {
+        FBP_CHECK_HIP(ctx, hipStreamSynchronize(s));
+        if (c->work) FBP_CHECK_HIP(ctx, hipFree(c->work));
+        c->work = nullptr;
+        FBP_CHECK_HIP(ctx, hipMalloc(&c->work, p->work));
+        c->work_bytes = p->work;
+    }
+    rocfft_status st = rocfft_execution_info_set_stream(c->info, s);
+    if (st != rocfft_status_success) return fft_fail(ctx, st, "rocfft_execution_info_set_stream");
+    st = rocfft_execution_info_set_work_buffer(c->info, c->work, c->work_bytes);
+    if (st != rocfft_status_success) return fft_fail(ctx, st, "rocfft_execution_info_set_work_buffer");
+    void* ib[1] = {in};
+    void* ob[1] = {out};
+    st = rocfft_execute(p->plan, ib, ob, c->info);
+    if (st != rocfft_status_success) return fft_fail(ctx, st, "rocfft_execute");
+    return FBP_OK;
+}
+
+inline dim3 grid1(size_t n, unsigned bs = 256) { return dim3(unsigned((n + bs - 1) / bs)); }
+
+// ------------------------------------------------------------------ filter_H
+
+// Laser::filter_H_cuda's length rules (FB/Laser.cu:128-137), float arithmetic.
+void filter_h_lengths(uint32_t nt, float dt, float wl_mean, float& wl_sigma, uint32_t& nt_pad,
+                      uint32_t& pad) {
+    if (std::fabs(wl_sigma - 0.0f) < FLT_EPSILON) wl_sigma = wl_mean / sqrtf(2.0f);
+    int t6 = static_cast<int>(std::round(6 * wl_sigma / dt));
+    if (t6 % 2 == 1) t6 += 1;
+    nt_pad = nt + 2u * uint32_t(t6 - 1);
+    pad = (nt_pad - nt) / 2u;
+}
+
+__global__ void fh_pad_kernel(const float* __restrict__ data, float* __restrict__ real, uint64_t batch,
+                              uint32_t nt, uint32_t N, uint32_t pad, int edge) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= batch * N) return;
+    const size_t b = i / N;
+    const uint32_t t = uint32_t(i % N);
+    const float* row = data + b * nt;
+    float v;
+    if (t < pad)
+        v = edge ? row[0] : 0.f;
+    else if (t >= pad + nt)
+        v = edge ? row[nt - 1] : 0.f;
+    else
+        v = row[t - pad];
+    real[i] = v;
+}
+
+// H[b][f] *= K[f]  (half spectra; multiplyHK, FB/fourier.cuh:17-31)
+__global__ void fh_mul_kernel(float2* __restrict__ H, const float2* __restrict__ K, uint64_t batch,
+                              uint32_t nh) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= batch * nh) return;
+    const float2 h = H[i], k = K[i % nh];
+    H[i] = make_float2(h.x * k.x - h.y * k.y, h.x * k.y + h.y * k.x);
+}
+
+__global__ void fh_unpad_kernel(const float* __restrict__ real, float* __restrict__ data,
+                                uint64_t batch, uint32_t nt, uint32_t N, uint32_t pad) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= batch * nt) return;
+    const size_t b = i / nt;
+    data[i] = real[b * N + pad + (i % nt)];
+}
+
+// --------------------------------------------------------------------- LoGFFT
+// buildLoGKernel3D + multiplyKernel (FB/transient_postprocessing.cuh:97-147),
+// with x the halved (fastest) axis.
+__global__ void logfft_mul_kernel(float2* __restrict__ C, uint32_t X, uint32_t Y, uint32_t Z,
+                                  float sigma) {
+    const uint32_t xh = X / 2 + 1;
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(xh) * Y * Z) return;
+    const int x = int(i % xh), y = int((i / xh) % Y), z = int(i / (size_t(xh) * Y));
+    const float PI = 3.14159265358979323846f;
+    const float fx = (x <= int(X) / 2) ? float(x) : float(x - int(X));
+    const float fy = (y <= int(Y) / 2) ? float(y) : float(y - int(Y));
+    const float fz = (z <= int(Z) / 2) ? float(z) : float(z - int(Z));
+    const float fxn = fx / float(X), fyn = fy / float(Y), fzn = fz / float(Z);
+    const float k2 = fxn * fxn + fyn * fyn + fzn * fzn;
+    const float lap = -4.0f * PI * PI * k2;
+    const float gauss = expf(-2.0f * PI * PI * sigma * sigma * k2);
+    const float val = lap * gauss;
+    const float2 a = C[i];
+    C[i] = make_float2(a.x * val - a.y * 0.0f, a.x * 0.0f + a.y * val);
+}
+
+// ------------------------------------------------------------------------ LCT
+// python/lct.py:109-211.  Layouts: rect [x][y][t] (N, N, M); work volumes
+// [t][y][x]; padded FFT volume [2M][2N][2N] real, half spectrum
+// [2M][2N][N+1] complex.
+
+// Tiled transpose rect[x][y][t] -> d[t][y][x] with z-scaling and trimming.
+__global__ void lct_permute_kernel(const float* __restrict__ rect, float* __restrict__ d, uint32_t N,
+                                   uint32_t M, uint32_t z_trim, int power4) {
+    __shared__ float tile[32][33];
+    const uint32_t y = blockIdx.z;
+    const uint32_t t0 = blockIdx.x * 32, x0 = blockIdx.y * 32;
+    // load: threads along t (coalesced in rect)
+    for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+        const uint32_t x = x0 + r, t = t0 + threadIdx.x;
+        float v = 0.f;
+        if (x < N && t < M) v = rect[(size_t(x) * N + y) * M + t];
+        tile[r][threadIdx.x] = v;
+    }
+    __syncthreads();
+    for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+        const uint32_t t = t0 + r, x = x0 + threadIdx.x;
+        if (x < N && t < M) {
+            float v = tile[threadIdx.x][r];
+            if (t < z_trim) v = 0.f;
+            // grid_z = linspace(0, 1, M)[t]
+            const double gz = (M > 1) ? (t == M - 1 ? 1.0 : double(t) * (1.0 / double(M - 1))) : 0.0;
+            const double g2 = gz * gz;
+            v = float(double(v) * (power4 ? g2 * g2 : g2));
+            d[(size_t(t) * N + y) * N + x] = v;
+        }
+    }
+}
+
+__global__ void fill_kernel(float* __restrict__ p, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
+// out[r][p] = sum_j A[r][j] in[j][p] for pixel p < N*N; writes into the
+// padded volume (row pitch 2N, plane pitch 4N^2) or a dense [M][N][N] array.
+__global__ void lct_resample_kernel(const float* __restrict__ in, float* __restrict__ out,
+                                    const int* __restrict__ rowptr, const int* __restrict__ cols,
+                                    const float* __restrict__ vals, uint32_t N, uint32_t M,
+                                    int to_padded) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t NN = size_t(N) * N;
+    if (i >= NN * M) return;
+    const uint32_t r = uint32_t(i / NN);
+    const size_t p = i % NN;
+    float acc = 0.f;
+    for (int e = rowptr[r]; e < rowptr[r + 1]; ++e) acc = fmaf(vals[e], in[size_t(cols[e]) * NN + p], acc);
+    if (to_padded) {
+        const size_t y = p / N, x = p % N;
+        out[(size_t(r) * 2 * N + y) * 2 * N + x] = acc;
+    } else {
+        out[i] = acc;
+    }
+}
+
+__global__ void lct_filter_kernel(float2* __restrict__ C, const float2* __restrict__ inv, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float2 a = C[i], b = inv[i];
+    C[i] = make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// crop [:M, :N, :N] of the padded real volume, times 1/(8 M N^2) (ifftn's
+// normalisation), then mtxi along t and max(., 0).
+__global__ void lct_final_kernel(const float* __restrict__ pad, float* __restrict__ out,
+                                 const int* __restrict__ rowptr, const int* __restrict__ cols,
+                                 const float* __restrict__ vals, uint32_t N, uint32_t M,
+                                 float norm) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t NN = size_t(N) * N;
+    if (i >= NN * M) return;
+    const uint32_t r = uint32_t(i / NN);
+    const size_t p = i % NN;
+    const size_t y = p / N, x = p % N;
+    float acc = 0.f;
+    for (int e = rowptr[r]; e < rowptr[r + 1]; ++e)
+        acc = fmaf(vals[e], pad[(size_t(cols[e]) * 2 * N + y) * 2 * N + x], acc);
+    out[i] = fmaxf(acc * norm, 0.f);
+}
+
+// psf (define_psf, python/lct.py:295-347) in FP64: for every (y, x) column the
+// z with the smallest |(4 slope)^2 (x^2 + y^2) - z| (all ties) is 1; the
+// result is divided by sqrt(count) (the two normalisations of the reference
+// reduce to this) and rolled by (0, U, U).  Writes unnormalised 0/1 and
+// counts ones.
+__device__ __forceinline__ double linspace_at(double a, double b, uint32_t n, uint32_t i) {
+    if (n == 1) return a;
+    if (i == n - 1) return b;
+    const double step = (b - a) / double(n - 1);
+    return double(i) * step + a;
+}
+
+__global__ void psf_kernel(double* __restrict__ psf, uint32_t U, uint32_t V, double slope,
+                           unsigned long long* __restrict__ count) {
+    const uint32_t n2u = 2 * U, n2v = 2 * V;
+    const size_t col = size_t(blockIdx.x) * blockDim.x + threadIdx.x;  // (y, x) column
+    if (col >= size_t(n2u) * n2u) return;
+    const uint32_t yi = uint32_t(col / n2u), xi = uint32_t(col % n2u);
+    const double x = linspace_at(-1.0, 1.0, n2u, xi), y = linspace_at(-1.0, 1.0, n2u, yi);
+    const double s4 = (4.0 * slope) * (4.0 * slope);
+    const double c = s4 * (x * x + y * y);
+    double best = INFINITY;
+    for (uint32_t z = 0; z < n2v; ++z) best = fmin(best, fabs(c - linspace_at(0.0, 2.0, n2v, z)));
+    // rolled destination of (y, x): ((y + U) mod 2U, (x + U) mod 2U)
+    const uint32_t ry = (yi + U) % n2u, rx = (xi + U) % n2u;
+    unsigned long long ones = 0;
+    for (uint32_t z = 0; z < n2v; ++z) {
+        const bool hit = fabs(c - linspace_at(0.0, 2.0, n2v, z)) == best;
+        psf[(size_t(z) * n2u + ry) * n2u + rx] = hit ? 1.0 : 0.0;
+        ones += hit;
+    }
+    atomicAdd(count, ones);
+}
+
+__global__ void scale_d_kernel(double* __restrict__ p, size_t n,
+                               const unsigned long long* __restrict__ count) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] *= 1.0 / sqrt(double(*count));
+}
+
+// invpsf = conj(F) / (|F|^2 + 1/snr), or conj(F) (backprojection), FP64 -> c64.
+__global__ void invpsf_kernel(const double2* __restrict__ F, float2* __restrict__ inv, size_t n,
+                              double inv_snr, int backprop) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double2 f = F[i];
+    if (backprop) {
+        inv[i] = make_float2(float(f.x), float(-f.y));
+    } else {
+        const double d = f.x * f.x + f.y * f.y + inv_snr;
+        inv[i] = make_float2(float(f.x / d), float(-f.y / d));
+    }
+}
+
+// Closed form of python/lct.py:resampling_operator (:350-419):
+// mtx[r][j] = (1/M) sum_{x in [rM+1, (r+1)M], ceil(sqrt x) = j+1} 1/sqrt(x);
+// mtxi = mtx^T.  Built in FP64 on the host, stored as float CSR.
+void build_resampling(uint32_t M, std::vector<int>& rp, std::vector<int>& cl,
+                      std::vector<float>& vl, std::vector<int>& rpT, std::vector<int>& clT,
+                      std::vector<float>& vlT) {
+    std::vector<std::map<int, double>> rows(M);
+    const uint64_t MM = uint64_t(M) * M;
+    for (uint64_t x = 1; x <= MM; ++x) {
+        const uint32_t r = uint32_t((x - 1) / M);
+        int j = int(std::ceil(std::sqrt(double(x)))) - 1;
+        if (j < 0) j = 0;
+        if (j > int(M) - 1) j = int(M) - 1;
+        rows[r][j] += 1.0 / std::sqrt(double(x)) / double(M);
+    }
+    std::vector<std::map<int, double>> rowsT(M);
+    rp.assign(1, 0);
+    for (uint32_t r = 0; r < M; ++r) {
+        for (auto& kv : rows[r]) {
+            cl.push_back(kv.first);
+            vl.push_back(float(kv.second));
+            rowsT[kv.first][int(r)] = kv.second;
+        }
+        rp.push_back(int(cl.size()));
+    }
+    rpT.assign(1, 0);
+    for (uint32_t r = 0; r < M; ++r) {
+        for (auto& kv : rowsT[r]) {
+            clT.push_back(kv.first);
+            vlT.push_back(float(kv.second));
+        }
+        rpT.push_back(int(clT.size()));
+    }
+}
+
+template <typename T>
+fbp_status upload(fbp_context ctx, const std::vector<T>& v, T** dst, hipStream_t s) {
+    FBP_CHECK_HIP(ctx, hipMalloc((void**)dst, std::max<size_t>(1, v.size()) * sizeof(T)));
+    FBP_CHECK_HIP(ctx, hipMemcpyAsync(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    FBP_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    return FBP_OK;
+}
+
+void free_csr(Csr& c) {
+    if (c.rowptr) hipFree(c.rowptr);
+    if (c.cols) hipFree(c.cols);
+    if (c.vals) hipFree(c.vals);
+    c = Csr{};
+}
+
+}  // namespace
+
+void fft_cache_destroy(fbp_context ctx) {
+    if (!ctx || !ctx->fft_cache) return;
+    FftCache* c = static_cast<FftCache*>(ctx->fft_cache);
+    for (auto& kv : c->plans) rocfft_plan_destroy(kv.second.plan);
+    if (c->info) rocfft_execution_info_destroy(c->info);
+    if (c->work) hipFree(c->work);
+    if (c->khat) hipFree(c->khat);
+    if (c->invpsf) hipFree(c->invpsf);
+    free_csr(c->mtx);
+    free_csr(c->mtxi);
+    delete c;
+    ctx->fft_cache = nullptr;
+}
+
 }  // namespace fbp
 
 using namespace fbp;
@@ -10,18 +382,195 @@ using namespace fbp;
 extern "C" {
 
 uint32_t fbp_filter_h_padded_length(uint32_t nt, float delta_t, float wl_mean, float wl_sigma) {
-    (void)nt; (void)delta_t; (void)wl_mean; (void)wl_sigma;
-    return 0;
+    uint32_t N, pad;
+    filter_h_lengths(nt, delta_t, wl_mean, wl_sigma, N, pad);
+    return N;
 }
-fbp_status fbp_filter_h(fbp_context ctx, float*, uint64_t, uint32_t, float, float, float, int32_t,
-                        void*) {
-    return fail(ctx, FBP_ERR_UNSUPPORTED, "filter_h: not built yet");
+
+fbp_status fbp_filter_h(fbp_context ctx, float* data, uint64_t batch, uint32_t nt, float delta_t,
+                        float wl_mean, float wl_sigma, int32_t border, void* stream) {
+    if (!ctx || !data || batch == 0 || nt == 0 || !(delta_t > 0.f) || !(wl_mean > 0.f) ||
+        (border != 0 && border != 1))
+        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "filter_h: bad argument");
+    FBP_CHECK(bind(ctx));
+    hipStream_t s = as_stream(stream);
+    uint32_t N, pad;
+    filter_h_lengths(nt, delta_t, wl_mean, wl_sigma, N, pad);
+    const uint32_t nh = N / 2 + 1;
+    FftCache* c = cache_of(ctx);
+
+    // Half spectrum of Re(K) (the real input makes Re(IFFT(FFT(H) FFT(K)))
+    // equal to IFFT(FFT(H) FFT(Re K))).  K built as FB/Laser.cu:138-172 does.
+    char keybuf[160];
+    snprintf(keybuf, sizeof(keybuf), "%u/%a/%a/%a", N, double(delta_t), double(wl_mean),
+             double(wl_sigma));
+    if (c->khat_key != keybuf) {
+        const float t_max = delta_t * float(N - 1);
+        std::vector<float> tv(N), env(N), kre(N), shifted(N);
+        float sum = 0.0f;
+        for (uint32_t i = 0; i < N; ++i) tv[i] = float(i) * delta_t;
+        for (uint32_t i = 0; i < N; ++i) {
+            const float val = (tv[i] - t_max * 0.5f) / wl_sigma;
+            env[i] = std::exp(-(val * val) * 0.5f);
+            sum += env[i];
+        }
+        const float TWO_PI = 6.28318530717958648f;
+        for (uint32_t i = 0; i < N; ++i) {
+            const float phase = TWO_PI * tv[i] / wl_mean;
+            kre[i] = env[i] * std::cos(phase) / sum;
+        }
+        for (uint32_t i = 0; i < N; ++i) shifted[i] = kre[(i + N / 2) % N];
+        float* dk = nullptr;
+        FBP_CHECK(upload(ctx, shifted, &dk, s));
+        if (c->khat) FBP_CHECK_HIP(ctx, hipFree(c->khat));
+        c->khat = nullptr;
+        c->khat_key.clear();
+        FBP_CHECK_HIP(ctx, hipMalloc((void**)&c->khat, size_t(nh) * sizeof(float2)));
+        FftPlan* pk = nullptr;
+        FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_forward, rocfft_precision_single, {N}, 1, &pk));
+        FBP_CHECK(run_plan(ctx, pk, dk, c->khat, s));
+        FBP_CHECK_HIP(ctx, hipStreamSynchronize(s));
+        FBP_CHECK_HIP(ctx, hipFree(dk));
+        c->khat_key = keybuf;
+    }
+
+    // Process time series in chunks so the work buffers stay bounded.
+    const uint64_t chunk = std::min<uint64_t>(batch, std::max<uint64_t>(1, (uint64_t(1) << 28) / N));
+    const size_t real_n = size_t(chunk) * N, cplx_n = size_t(chunk) * nh;
+    FBP_CHECK(ensure_scratch2(ctx, real_n * sizeof(float) + cplx_n * sizeof(float2) + 256));
+    float* real = static_cast<float*>(ctx->scratch2);
+    float2* spec = reinterpret_cast<float2*>(real + ((real_n + 63) & ~size_t(63)));
+    for (uint64_t b0 = 0; b0 < batch; b0 += chunk) {
+        const uint64_t nb = std::min<uint64_t>(chunk, batch - b0);
+        float* d = data + size_t(b0) * nt;
+        hipLaunchKernelGGL(fh_pad_kernel, grid1(size_t(nb) * N), dim3(256), 0, s, d, real, nb, nt, N,
+                           pad, border);
+        FftPlan *pf = nullptr, *pi = nullptr;
+        FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_forward, rocfft_precision_single, {N}, nb, &pf));
+        FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_inverse, rocfft_precision_single, {N}, nb, &pi));
+        FBP_CHECK(run_plan(ctx, pf, real, spec, s));
+        hipLaunchKernelGGL(fh_mul_kernel, grid1(size_t(nb) * nh), dim3(256), 0, s, spec, c->khat, nb, nh);
+        FBP_CHECK(run_plan(ctx, pi, spec, real, s));
+        hipLaunchKernelGGL(fh_unpad_kernel, grid1(size_t(nb) * nt), dim3(256), 0, s, real, d, nb, nt, N,
+                           pad);
+        FBP_CHECK_HIP(ctx, hipGetLastError());
+    }
+    return FBP_OK;
 }
-fbp_status fbp_filter_log_fft(fbp_context ctx, float*, fbp_uvec3, float, void*) {
-    return fail(ctx, FBP_ERR_UNSUPPORTED, "log_fft: not built yet");
+
+fbp_status fbp_filter_log_fft(fbp_context ctx, float* volume, fbp_uvec3 size, float sigma,
+                              void* stream) {
+    if (!ctx || !volume || size.x == 0 || size.y == 0 || size.z == 0)
+        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "log_fft: bad argument");
+    FBP_CHECK(bind(ctx));
+    hipStream_t s = as_stream(stream);
+    const size_t nc = size_t(size.x / 2 + 1) * size.y * size.z;
+    FBP_CHECK(ensure_scratch2(ctx, nc * sizeof(float2)));
+    float2* spec = static_cast<float2*>(ctx->scratch2);
+    FftPlan *pf = nullptr, *pi = nullptr;
+    FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_forward, rocfft_precision_single,
+                       {size.x, size.y, size.z}, 1, &pf));
+    FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_inverse, rocfft_precision_single,
+                       {size.x, size.y, size.z}, 1, &pi));
+    FBP_CHECK(run_plan(ctx, pf, volume, spec, s));
+    hipLaunchKernelGGL(logfft_mul_kernel, grid1(nc), dim3(256), 0, s, spec, size.x, size.y, size.z,
+                       sigma);
+    FBP_CHECK_HIP(ctx, hipGetLastError());
+    return run_plan(ctx, pi, spec, volume, s);
 }
-fbp_status fbp_lct(fbp_context ctx, const float*, const fbp_lct_params*, float*, void*) {
-    return fail(ctx, FBP_ERR_UNSUPPORTED, "lct: not built yet");
+
+fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, float* out,
+                   void* stream) {
+    if (!ctx || !data || !p || !out) return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "lct: null argument");
+    const uint32_t N = p->n, M = p->m;
+    if (N == 0 || M < 2 || (N & (N - 1)) || (M & (M - 1)) || !(p->range_max > 0.f) ||
+        !(p->snr > 0.f))
+        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "lct: N, M must be powers of two, snr > 0");
+    FBP_CHECK(bind(ctx));
+    hipStream_t s = as_stream(stream);
+    FftCache* c = cache_of(ctx);
+    const size_t NN = size_t(N) * N;
+    const size_t pad_n = size_t(2 * M) * (2 * N) * (2 * N);
+    const size_t half_n = size_t(2 * M) * (2 * N) * (N + 1);
+
+    // Resampling operators (closed form, FP64 -> float CSR), cached per M.
+    if (c->op_m != M) {
+        free_csr(c->mtx);
+        free_csr(c->mtxi);
+        c->op_m = 0;
+        std::vector<int> rp, cl, rpT, clT;
+        std::vector<float> vl, vlT;
+        build_resampling(M, rp, cl, vl, rpT, clT, vlT);
+        FBP_CHECK(upload(ctx, rp, &c->mtx.rowptr, s));
+        FBP_CHECK(upload(ctx, cl, &c->mtx.cols, s));
+        FBP_CHECK(upload(ctx, vl, &c->mtx.vals, s));
+        FBP_CHECK(upload(ctx, rpT, &c->mtxi.rowptr, s));
+        FBP_CHECK(upload(ctx, clT, &c->mtxi.cols, s));
+        FBP_CHECK(upload(ctx, vlT, &c->mtxi.vals, s));
+        c->op_m = M;
+    }
+
+    // Inverse filter of the NLOS blur kernel, built in FP64 and cached.
+    const double slope = double(p->width) / double(p->range_max);
+    char keybuf[200];
+    snprintf(keybuf, sizeof(keybuf), "%u/%u/%a/%a/%d", N, M, slope, double(p->snr), p->is_backprop);
+    if (c->inv_key != keybuf) {
+        if (c->invpsf) FBP_CHECK_HIP(ctx, hipFree(c->invpsf));
+        c->invpsf = nullptr;
+        c->inv_key.clear();
+        double* psf = nullptr;
+        double2* F = nullptr;
+        unsigned long long* cnt = nullptr;
+        FBP_CHECK_HIP(ctx, hipMalloc((void**)&psf, pad_n * sizeof(double)));
+        FBP_CHECK_HIP(ctx, hipMalloc((void**)&F, half_n * sizeof(double2)));
+        FBP_CHECK_HIP(ctx, hipMalloc((void**)&cnt, sizeof(unsigned long long)));
+        FBP_CHECK_HIP(ctx, hipMemsetAsync(cnt, 0, sizeof(unsigned long long), s));
+        hipLaunchKernelGGL(psf_kernel, grid1(size_t(2 * N) * 2 * N), dim3(256), 0, s, psf, N, M, slope, cnt);
+        hipLaunchKernelGGL(scale_d_kernel, grid1(pad_n), dim3(256), 0, s, psf, pad_n, cnt);
+        FftPlan* pd = nullptr;
+        FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_forward, rocfft_precision_double,
+                           {2 * N, 2 * N, 2 * M}, 1, &pd));
+        FBP_CHECK(run_plan(ctx, pd, psf, F, s));
+        FBP_CHECK_HIP(ctx, hipMalloc((void**)&c->invpsf, half_n * sizeof(float2)));
+        hipLaunchKernelGGL(invpsf_kernel, grid1(half_n), dim3(256), 0, s, F, c->invpsf, half_n,
+                           1.0 / double(p->snr), p->is_backprop);
+        FBP_CHECK_HIP(ctx, hipGetLastError());
+        FBP_CHECK_HIP(ctx, hipStreamSynchronize(s));
+        hipFree(psf);
+        hipFree(F);
+        hipFree(cnt);
+        c->inv_key = keybuf;
+    }
+
+    // Work buffers: d [M][N][N] | padded real [2M][2N][2N] | half spectrum.
+    const size_t d_bytes = (NN * M * sizeof(float) + 255) & ~size_t(255);
+    const size_t r_bytes = (pad_n * sizeof(float) + 255) & ~size_t(255);
+    FBP_CHECK(ensure_scratch2(ctx, d_bytes + r_bytes + half_n * sizeof(float2)));
+    char* base = static_cast<char*>(ctx->scratch2);
+    float* d = reinterpret_cast<float*>(base);
+    float* padv = reinterpret_cast<float*>(base + d_bytes);
+    float2* spec = reinterpret_cast<float2*>(base + d_bytes + r_bytes);
+
+    const dim3 tgrid((M + 31) / 32, (N + 31) / 32, N);
+    hipLaunchKernelGGL(lct_permute_kernel, tgrid, dim3(32, 8), 0, s, data, d, N, M, p->z_trim,
+                       p->is_diffuse);
+    hipLaunchKernelGGL(fill_kernel, grid1(pad_n), dim3(256), 0, s, padv, pad_n);
+    hipLaunchKernelGGL(lct_resample_kernel, grid1(NN * M), dim3(256), 0, s, d, padv, c->mtx.rowptr,
+                       c->mtx.cols, c->mtx.vals, N, M, 1);
+    FBP_CHECK_HIP(ctx, hipGetLastError());
+    FftPlan *pf = nullptr, *pi = nullptr;
+    FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_forward, rocfft_precision_single,
+                       {2 * N, 2 * N, 2 * M}, 1, &pf));
+    FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_inverse, rocfft_precision_single,
+                       {2 * N, 2 * N, 2 * M}, 1, &pi));
+    FBP_CHECK(run_plan(ctx, pf, padv, spec, s));
+    hipLaunchKernelGGL(lct_filter_kernel, grid1(half_n), dim3(256), 0, s, spec, c->invpsf, half_n);
+    FBP_CHECK(run_plan(ctx, pi, spec, padv, s));
+    const float norm = float(1.0 / double(pad_n));
+    hipLaunchKernelGGL(lct_final_kernel, grid1(NN * M), dim3(256), 0, s, padv, out, c->mtxi.rowptr,
+                       c->mtxi.cols, c->mtxi.vals, N, M, norm);
+    FBP_CHECK_HIP(ctx, hipGetLastError());
+    return FBP_OK;
 }
 
 }  // extern "C"

@@ -191,3 +191,17 @@ def filter_h(data, dt, wl_mean, wl_sigma=0.0, border=0) -> np.ndarray:
     out = np.empty(data.shape, np.float64)
     load().oracle_filter_h(_p(data), batch, nt, dt, wl_mean, wl_sigma, border, _p(out))
     return out
+
+
+def log_fft(v, sigma=1.0) -> np.ndarray:
+    """LoGFFT::compute (FB/PostprocessingFilters.cu:36-77) in FP64 with NumPy's
+    FFT: R2C, times -4 pi^2 |f|^2 exp(-2 pi^2 sigma^2 |f|^2) (f in cycles per
+    sample), C2R without the 1/V normalisation (the reference keeps it off)."""
+    v = np.asarray(v, np.float64)
+    Z, Y, X = v.shape
+    fz = np.fft.fftfreq(Z)[:, None, None]
+    fy = np.fft.fftfreq(Y)[None, :, None]
+    fx = np.fft.rfftfreq(X)[None, None, :]
+    k2 = fx ** 2 + fy ** 2 + fz ** 2
+    spec = -4 * np.pi ** 2 * k2 * np.exp(-2 * np.pi ** 2 * sigma ** 2 * k2)
+    return np.fft.irfftn(np.fft.rfftn(v) * spec, s=v.shape) * v.size

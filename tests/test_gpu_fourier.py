@@ -1,0 +1,72 @@
+"""GPU parity of filter_H, LoGFFT and LCT against the CPU oracle."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from oracle import lct as olct
+from fasterbackprojection_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("nt,dt,border", [(256, 0.0096, 0), (256, 0.0096, 1), (1024, 0.0048, 0)])
+def test_filter_h(ctx, nt, dt, border):
+    rng = np.random.default_rng(nt + border)
+    data = rng.poisson(5.0, size=(7, nt)).astype(np.float32)
+    ref = oracle.filter_h(data, dt, 10 * dt, 0.0, border)
+    t = torch.from_numpy(data).cuda()
+    ctx.filter_h(t, dt, 10 * dt, 0.0, "edge" if border else "zero")
+    torch.cuda.synchronize()
+    assert rel_l2(t.cpu().numpy(), ref) < 2e-5
+    lib = _lib.load()
+    assert lib.fbp_filter_h_padded_length(nt, dt, 10 * dt, 0.0) == oracle.filter_h_nt_pad(nt, dt, 10 * dt)
+
+
+@pytest.mark.parametrize("shape,sigma", [((32, 32, 32), 1.0), ((20, 24, 16), 1.5)])
+def test_log_fft(ctx, shape, sigma):
+    rng = np.random.default_rng(0)
+    v = rng.random(shape).astype(np.float32)
+    ref = oracle.log_fft(v, sigma)
+    t = torch.from_numpy(v).cuda()
+    ctx.log_fft(t, sigma)
+    torch.cuda.synchronize()
+    assert rel_l2(t.cpu().numpy(), ref) < 1e-5
+
+
+@pytest.mark.parametrize("N,M,diffuse,backprop,z_trim", [(16, 64, False, False, 0),
+                                                         (16, 64, True, False, 5),
+                                                         (8, 32, False, True, 3),
+                                                         (32, 128, False, False, 10)])
+def test_lct_random(ctx, N, M, diffuse, backprop, z_trim):
+    rng = np.random.default_rng(N * M)
+    data = rng.random((N, N, M)).astype(np.float32)
+    width, rmax, snr = 0.5, M * 3e8 * 16e-12, 0.8
+    ref = olct.lct(data, width, rmax, snr, diffuse, backprop, z_trim)
+    out = ctx.lct(torch.from_numpy(data).cuda(), width, rmax, snr, diffuse, backprop, z_trim)
+    torch.cuda.synchronize()
+    assert rel_l2(out.cpu().numpy(), ref) < 1e-4
+
+
+def test_lct_point_scatterer(ctx):
+    N, M = 64, 256
+    width, bin_res, c = 0.5, 16e-12, 3e8
+    rmax = M * c * bin_res
+    xs = np.linspace(-width, width, N)
+    x0, y0, z0 = xs[40], xs[22], 0.7 * rmax / 2
+    data = np.zeros((N, N, M), np.float32)
+    for i in range(N):
+        for j in range(N):
+            r = np.sqrt((xs[i] - x0) ** 2 + (xs[j] - y0) ** 2 + z0 ** 2)
+            tb = int(round(2 * r / (c * bin_res)))
+            if tb < M:
+                data[i, j, tb] = 1.0
+    ref = olct.lct(data, width, rmax, 8.0)
+    out = ctx.lct(torch.from_numpy(data).cuda(), width, rmax, 8.0).cpu().numpy()
+    assert np.unravel_index(np.argmax(out), out.shape) == np.unravel_index(np.argmax(ref), ref.shape)
+    assert rel_l2(out, ref) < 1e-4
