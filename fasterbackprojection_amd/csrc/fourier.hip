@@ -92,8 +92,11 @@ fbp_status run_plan(fbp_context ctx, FftPlan* p, void* in, void* out, hipStream_
     }
     rocfft_status st = rocfft_execution_info_set_stream(c->info, s);
     if (st != rocfft_status_success) return fft_fail(ctx, st, "rocfft_execution_info_set_stream");
-    st = rocfft_execution_info_set_work_buffer(c->info, c->work, c->work_bytes);
-    if (st != rocfft_status_success) return fft_fail(ctx, st, "rocfft_execution_info_set_work_buffer");
+    if (c->work_bytes > 0) {
+        st = rocfft_execution_info_set_work_buffer(c->info, c->work, c->work_bytes);
+        if (st != rocfft_status_success)
+            return fft_fail(ctx, st, "rocfft_execution_info_set_work_buffer");
+    }
     void* ib[1] = {in};
     void* ob[1] = {out};
     st = rocfft_execute(p->plan, ib, ob, c->info);
