@@ -20,6 +20,8 @@ CSRC = PKG / "csrc"
 DROPIN = PKG / "dropin"
 BUILD = PKG / "_build"
 LIB = PKG / "libfbp.so"
+TOOLS = PKG / "tools"
+CLI = PKG / "fbp_reconstruct"
 
 ARCH = os.environ.get("FBP_OFFLOAD_ARCH", "gfx950")
 
@@ -66,7 +68,7 @@ def _compile(src: Path, force: bool) -> Path:
     if src.suffix == ".hip":
         cmd += HIP_FLAGS + ["-x", "hip"]
     else:
-        cmd += ["-x", "c++"]
+        cmd += ["-x", "c++", "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__"]
     cmd += ["-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
@@ -88,6 +90,15 @@ def build(force: bool = True, jobs: int | None = None) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
+    # Command-line driver of the drop-in classes, linked against libfbp.so.
+    cmd = ([hipcc(), "-O2", "-std=c++17", f"-I{ROOT / 'include'}", f"-I{DROPIN}",
+            "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__", "-x", "c++",
+            str(TOOLS / "fbp_reconstruct.cpp"), f"-L{PKG}", "-lfbp",
+            "-L/opt/rocm/lib", "-lamdhip64", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN",
+            "-Wl,-rpath,/opt/rocm/lib", "-o", str(CLI)])
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"fbp_reconstruct link failed\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     return LIB
 
 
