@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--res", type=int, default=128)
     ap.add_argument("--scan", type=int, default=128)
     ap.add_argument("--bins", type=int, default=1024)
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the FP16/BF16 storage leg (256^2 x 1024 -> 128^3)")
     return ap.parse_args()
 
 
@@ -138,6 +140,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(info, I, g, args.cpu_seconds)
+    precision = None if args.no_precision else precision_leg(ctx, args)
 
     out = {
         "metric": METRIC,
@@ -171,9 +174,69 @@ def main():
         },
         "breakdown_ms": {"backprojection": bp_avg, "log_and_normalize": float(np.mean(f_ms))},
         "cpu_baseline": cpu,
+        "precision": precision,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def precision_leg(ctx, args):
+    """256x256 confocal scan, 1024 bins of 16 ps, 8 seeded quads, Poisson at
+    peak 500 -> 128^3.  FP32, FP16 and BF16 storage (per-scan-point max
+    scale, Kahan accumulation, FP64 chunk reduction); errors against the FP64
+    oracle on the first depth slices."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from fasterbackprojection_amd.api import STORAGE_BF16, STORAGE_F16, STORAGE_F32, make_info
+    from fasterbackprojection_amd.scene import full_confocal_geometry, random_quads
+
+    dev = torch.device("cuda", 0)
+    g = full_confocal_geometry(256, 1024, 128)
+    info = make_info(g)
+    L = len(g.laser_targets)
+    rng = np.random.default_rng(1)
+    scene = random_quads(rng, 8, 20000)
+    laser = torch.from_numpy(g.laser_targets).to(dev)
+    pts = torch.from_numpy(scene.points.astype(np.float32)).to(dev)
+    alb = torch.from_numpy(scene.albedo.astype(np.float32)).to(dev)
+    clean = ctx.render_confocal(laser, pts, alb, 1024, g.time_step)
+    I = torch.poisson(clean * (500.0 / clean.max())).contiguous()
+    torch.cuda.synchronize()
+    nz = 2
+    sub = type(info).from_buffer_copy(info)
+    sub.voxel_resolution.z = nz
+    t = time.perf_counter()
+    ref = oracle.confocal_voxel(sub, I.cpu().numpy(), g.laser_targets)
+    log(f"precision leg: FP64 oracle on {nz} slices in {time.perf_counter() - t:.1f} s")
+    out = {"workload": "confocal 256x256x1024 (16 ps, 8 quads, Poisson peak 500) -> 128^3",
+           "oracle_slices": nz}
+    stream = torch.cuda.current_stream()
+    for name, st in (("f32", STORAGE_F32), ("f16", STORAGE_F16), ("bf16", STORAGE_BF16)):
+        if st == STORAGE_F32:
+            data, sc = I, None
+        else:
+            data, sc = ctx.quantize(I, st)
+        vol = ctx.backproject_confocal(info, data, laser, storage=st, scales=sc)
+        ms = []
+        for _ in range(3):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            ctx.backproject_confocal(info, data, laser, storage=st, scales=sc, out=vol)
+            b.record(stream)
+            b.synchronize()
+            ms.append(a.elapsed_time(b))
+        got = vol[:nz].cpu().numpy().astype(np.float64)
+        err = got - ref
+        out[name] = {"ms": float(np.mean(ms)),
+                     "voxel_updates_per_s": 128 ** 3 * L / (np.mean(ms) * 1e-3),
+                     "rel_l2": float(np.linalg.norm(err) / np.linalg.norm(ref)),
+                     "max_abs": float(np.abs(err).max()),
+                     "max_abs_rel_to_peak": float(np.abs(err).max() / np.abs(ref).max())}
+        log(f"precision {name}: {out[name]}")
+    return out
 
 
 def cpu_baseline(info, I, g, target_seconds):

@@ -67,11 +67,13 @@ __device__ __forceinline__ float confocal_sample(float s, float cb, float coff, 
     return (s >= e1.y) ? e1.x : e0.x;
 }
 
-template <bool PURE>
-__device__ __forceinline__ void confocal_update_tile(float (&acc)[64], const float (&A)[4][4],
-                                                     const float (&Z2)[4], const float2* win,
-                                                     int base, float cb, float coff, float qbias,
-                                                     float inv_dt) {
+// KAHAN: compensated summation (precision form): comp carries the low-order
+// bits lost by each add, acc + comp is written out.
+template <bool PURE, bool KAHAN>
+__device__ __forceinline__ void confocal_update_tile(float (&acc)[64], float (&comp)[64],
+                                                     const float (&A)[4][4], const float (&Z2)[4],
+                                                     const float2* win, int base, float cb,
+                                                     float coff, float qbias, float inv_dt) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
 #pragma unroll
@@ -83,7 +85,14 @@ __device__ __forceinline__ void confocal_update_tile(float (&acc)[64], const flo
                 const float s = A[i][j] + Z2[m];
                 float w;
                 const float v = confocal_sample<PURE>(s, cb, coff, qbias, inv_dt, win, base, w);
-                acc[slot] = fmaf(v, w, acc[slot]);
+                if constexpr (KAHAN) {
+                    const float y = fmaf(v, w, -comp[slot]);
+                    const float t = acc[slot] + y;
+                    comp[slot] = (t - acc[slot]) - y;
+                    acc[slot] = t;
+                } else {
+                    acc[slot] = fmaf(v, w, acc[slot]);
+                }
             }
         }
     }
@@ -94,7 +103,9 @@ __device__ __forceinline__ float pick4(const float (&v)[4], int i) {
 }
 
 template <int STORAGE, int LAYOUT, bool PURE>
-__global__ __launch_bounds__(64, 2) void confocal_voxel_kernel(ConfocalArgs a) {
+__device__ __forceinline__ void confocal_voxel_body(const ConfocalArgs& a) {
+    // Quantised (precision-form) transients accumulate with Kahan compensation.
+    constexpr bool KAHAN = STORAGE != FBP_STORAGE_F32;
     extern __shared__ float smem[];
     float2* win = reinterpret_cast<float2*>(smem);           // batch * W pairs
     float* prm = smem + 2 * a.batch * a.W;                    // batch * 8
@@ -120,9 +131,9 @@ __global__ __launch_bounds__(64, 2) void confocal_voxel_kernel(ConfocalArgs a) {
     float lo[3], hi[3];
     tile_box(a.g, X0, Y0, Z0, lo, hi);
 
-    float acc[64];
+    float acc[64], comp[64];
 #pragma unroll
-    for (int i = 0; i < 64; ++i) acc[i] = 0.f;
+    for (int i = 0; i < 64; ++i) acc[i] = comp[i] = 0.f;
     // Near-voxel corrections (rare) go to a lane-owned slice of a.corr.
     float* corr = a.corr + (size_t(chunk) * a.g.num_tiles + tile) * 64 * 64;
 #pragma unroll 8
@@ -222,13 +233,24 @@ __global__ __launch_bounds__(64, 2) void confocal_voxel_kernel(ConfocalArgs a) {
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) A[i][j] = rn_add(X2[i], Y2[j]);
-            confocal_update_tile<PURE>(acc, A, Z2, win, base, cb, coff, qbias, a.inv_dt);
+            confocal_update_tile<PURE, KAHAN>(acc, comp, A, Z2, win, base, cb, coff, qbias, a.inv_dt);
         }
     }
 
     float* out = a.partial + (size_t(chunk) * a.g.num_tiles + tile) * 64 * 64;
 #pragma unroll
-    for (int s = 0; s < 64; ++s) out[s * 64 + lane] = acc[s];
+    for (int s = 0; s < 64; ++s) out[s * 64 + lane] = KAHAN ? acc[s] - comp[s] : acc[s];
+}
+
+// FP32 storage: 2 waves per SIMD (<= 256 VGPRs).  The Kahan variants keep
+// 128 more accumulators and run at 1 wave per SIMD.
+template <int STORAGE, int LAYOUT, bool PURE>
+__global__ __launch_bounds__(64, 2) void confocal_voxel_kernel(ConfocalArgs a) {
+    confocal_voxel_body<STORAGE, LAYOUT, PURE>(a);
+}
+template <int STORAGE, int LAYOUT, bool PURE>
+__global__ __launch_bounds__(64, 1) void confocal_voxel_kahan_kernel(ConfocalArgs a) {
+    confocal_voxel_body<STORAGE, LAYOUT, PURE>(a);
 }
 
 // S_k = min{s >= 0 : bin(s) >= k}, k = 0..T, by binary search over the bit
@@ -315,12 +337,21 @@ static uint32_t pick_chunks(fbp_context ctx, uint32_t tiles, uint32_t units, uin
 template <int STORAGE, int LAYOUT>
 static void launch_confocal(const ConfocalArgs& a, bool pure, dim3 grid, size_t shmem,
                             hipStream_t stream) {
-    if (pure)
-        hipLaunchKernelGGL((confocal_voxel_kernel<STORAGE, LAYOUT, true>), grid, dim3(64), shmem,
-                           stream, a);
-    else
-        hipLaunchKernelGGL((confocal_voxel_kernel<STORAGE, LAYOUT, false>), grid, dim3(64), shmem,
-                           stream, a);
+    if constexpr (STORAGE == FBP_STORAGE_F32) {
+        if (pure)
+            hipLaunchKernelGGL((confocal_voxel_kernel<STORAGE, LAYOUT, true>), grid, dim3(64),
+                               shmem, stream, a);
+        else
+            hipLaunchKernelGGL((confocal_voxel_kernel<STORAGE, LAYOUT, false>), grid, dim3(64),
+                               shmem, stream, a);
+    } else {
+        if (pure)
+            hipLaunchKernelGGL((confocal_voxel_kahan_kernel<STORAGE, LAYOUT, true>), grid,
+                               dim3(64), shmem, stream, a);
+        else
+            hipLaunchKernelGGL((confocal_voxel_kahan_kernel<STORAGE, LAYOUT, false>), grid,
+                               dim3(64), shmem, stream, a);
+    }
 }
 
 }  // namespace fbp
