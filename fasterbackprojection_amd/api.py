@@ -257,3 +257,59 @@ class Context:
                                                 _ptr(albedo), points.shape[0], num_time_bins,
                                                 float(delta), _ptr(out), _stream()))
         return out
+
+
+class Refiner:
+    """The learned refiner (fbp_refiner_*): a 3-level 3D U-Net on S^3 volumes."""
+
+    def __init__(self, ctx: Context, size: int, seed: int = 0):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.size = int(size)
+        h = C.c_void_p()
+        ctx.check(self.lib.fbp_refiner_create(ctx.handle, self.size, int(seed), C.byref(h)))
+        self.handle = h
+        self.n_params = int(self.lib.fbp_refiner_param_count())
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.fbp_refiner_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load(self, params: np.ndarray):
+        p = np.ascontiguousarray(params, np.float32)
+        self.ctx.check(self.lib.fbp_refiner_load(self.handle, p.ctypes.data_as(C.c_void_p), p.size))
+
+    def save(self) -> np.ndarray:
+        p = np.empty(self.n_params, np.float32)
+        self.ctx.check(self.lib.fbp_refiner_save(self.handle, p.ctypes.data_as(C.c_void_p), p.size))
+        return p
+
+    def forward(self, volume: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        out = torch.empty_like(volume) if out is None else out
+        self.ctx.check(self.lib.fbp_refiner_forward(self.handle, _ptr(volume), _ptr(out), _stream()))
+        return out
+
+    def train_step(self, inputs: torch.Tensor, targets: torch.Tensor, lr: float,
+                   ssim_weight: float = 1.0, want_loss: bool = True) -> float | None:
+        batch = inputs.shape[0]
+        loss = C.c_float()
+        self.ctx.check(self.lib.fbp_refiner_train_step(
+            self.handle, _ptr(inputs), _ptr(targets), batch, float(lr), float(ssim_weight),
+            C.byref(loss) if want_loss else None, _stream()))
+        return loss.value if want_loss else None
+
+    def gradient(self, volume: torch.Tensor, target: torch.Tensor, ssim_weight: float = 1.0):
+        g = np.empty(self.n_params, np.float32)
+        loss = C.c_float()
+        torch.cuda.synchronize()
+        self.ctx.check(self.lib.fbp_refiner_gradient(self.handle, _ptr(volume), _ptr(target),
+                                                     float(ssim_weight),
+                                                     g.ctypes.data_as(C.c_void_p), C.byref(loss)))
+        return g, loss.value

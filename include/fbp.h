@@ -241,6 +241,32 @@ fbp_status fbp_render_confocal(fbp_context ctx, const fbp_vec3* scan_points, uin
                                uint32_t num_time_bins, float delta, float* transient,
                                void* stream);
 
+/* ---------------------------------------------------------- learned form */
+
+/* Learned refiner (no reference counterpart): a 3-level 3D U-Net
+ * (16/32/64 channels, 3x3x3 kernels) mapping a max-normalised MIS-sampled
+ * backprojection to the backprojection + LoG volume.  Forward convolutions
+ * with 16+ channels are BF16 MFMA implicit GEMMs with FP32 accumulation;
+ * training (L1 + SSIM loss, Adam) keeps FP32 master weights.  Volumes are
+ * float[S][S][S] on the device, S a multiple of 16. */
+typedef struct fbp_refiner_s* fbp_refiner;
+size_t fbp_refiner_param_count(void);
+fbp_status fbp_refiner_create(fbp_context ctx, uint32_t size, uint64_t seed, fbp_refiner* out);
+fbp_status fbp_refiner_destroy(fbp_refiner r);
+/* Parameters as one FP32 host array (fbp_refiner_param_count() values). */
+fbp_status fbp_refiner_load(fbp_refiner r, const float* host_params, uint64_t count);
+fbp_status fbp_refiner_save(fbp_refiner r, float* host_params, uint64_t count);
+fbp_status fbp_refiner_forward(fbp_refiner r, const float* input, float* output, void* stream);
+/* One Adam step over `batch` (input, target) volume pairs laid out back to
+ * back; loss = mean|y - t| + ssim_weight * (1 - mean SSIM(y, t)).
+ * host_loss (may be NULL, else synchronises) receives the batch mean. */
+fbp_status fbp_refiner_train_step(fbp_refiner r, const float* inputs, const float* targets,
+                                  uint32_t batch, float lr, float ssim_weight, float* host_loss,
+                                  void* stream);
+/* Gradient of the loss w.r.t. all parameters for one pair (host output). */
+fbp_status fbp_refiner_gradient(fbp_refiner r, const float* input, const float* target,
+                                float ssim_weight, float* host_grad, float* host_loss);
+
 #ifdef __cplusplus
 }
 #endif
