@@ -249,6 +249,16 @@ class Context:
         self.check(self.lib.fbp_lct(self.handle, _ptr(data), C.byref(p), _ptr(out), _stream()))
         return out
 
+    def volume_residual(self, info, volume, intensity, laser_targets, rows,
+                        threshold_fraction=0.6, blur=4) -> float:
+        """1 - cosine(measured rows, rows rendered from the volume's bright voxels)."""
+        r = C.c_double()
+        self.check(self.lib.fbp_volume_residual(
+            self.handle, C.byref(info), _ptr(volume), _ptr(intensity), _ptr(laser_targets),
+            _ptr(rows), int(rows.numel()), float(threshold_fraction), int(blur), C.byref(r),
+            _stream()))
+        return r.value
+
     def render_confocal(self, scan_points, points, albedo, num_time_bins, delta, out=None):
         L = scan_points.shape[0]
         if out is None:

@@ -1,6 +1,7 @@
 // Depth-plane backprojection, MIS-sampled confocal backprojection, transient
 // quantisation and the confocal forward model.
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "bp_common.h"
@@ -224,6 +225,73 @@ __global__ void render_confocal_kernel(const fbp_vec3* __restrict__ scan, uint32
     for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) row[t] = float(double(hist[t]) / kRenderFix);
 }
 
+// ----------------------------------------------------- forward-model residual
+// For scan point rows[r]: render the volume's voxels above `thr` (albedo
+// v - thr, at their centres) with the same 1/r^4 model, box-blur both that
+// and the measured row over +-blur bins, and accumulate <p, m>, |p|^2, |m|^2
+// in FP64 (fixed-point free: one workgroup per row writes its three sums).
+__global__ void residual_row_kernel(const float* __restrict__ volume, VolumeGeom g,
+                                    const float* __restrict__ mm, float thr_frac,
+                                    const fbp_vec3* __restrict__ scan, const uint32_t* __restrict__ rows,
+                                    const float* __restrict__ I, uint32_t T, float delta, int blur,
+                                    double* __restrict__ sums) {
+    extern __shared__ unsigned long long hist[];
+    float* pm = reinterpret_cast<float*>(hist + T);  // [2][T] blurred rows
+    const uint32_t r = blockIdx.x;
+    const uint32_t l = rows[r];
+    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) hist[t] = 0ull;
+    __syncthreads();
+    const float mn = mm[0], mx = mm[1];
+    const float thr = mn + thr_frac * (mx - mn);
+    const float sx = scan[l].x, sy = scan[l].y, sz = scan[l].z;
+    const size_t n = size_t(g.res[0]) * g.res[1] * g.res[2];
+    for (size_t v = threadIdx.x; v < n; v += blockDim.x) {
+        const float a = volume[v] - thr;
+        if (!(a > 0.f)) continue;
+        const uint32_t vx = uint32_t(v % g.res[0]), vy = uint32_t((v / g.res[0]) % g.res[1]),
+                       z = uint32_t(v / (size_t(g.res[0]) * g.res[1]));
+        const float dx = voxel_center(g.hmin[0], g.vs[0], vx) - sx;
+        const float dy = voxel_center(g.hmin[1], g.vs[1], z) - sy;
+        const float dz = voxel_center(g.hmin[2], g.vs[2], vy) - sz;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const int bin = int(rintf(2.0f * sqrtf(r2) / delta));
+        if (bin < 0 || bin >= int(T) || !(r2 > 0.f)) continue;
+        atomicAdd(&hist[bin], (unsigned long long)(double(a) / (double(r2) * r2) * kRenderFix + 0.5));
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
+        double sp = 0.0, sm = 0.0;
+        for (int k = -blur; k <= blur; ++k) {
+            const int u = int(t) + k;
+            if (u < 0 || u >= int(T)) continue;
+            sp += double(hist[u]) / kRenderFix;
+            sm += double(I[size_t(l) * T + u]);
+        }
+        pm[t] = float(sp);
+        pm[T + t] = float(sm);
+    }
+    __syncthreads();
+    __shared__ double red[3][256];
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
+        const double p = pm[t], m = pm[T + t];
+        a += p * m;
+        b += p * p;
+        c += m * m;
+    }
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = b;
+    red[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w)
+            for (int q = 0; q < 3; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; ++q) sums[size_t(r) * 3 + q] = red[q][0];
+}
+
 }  // namespace fbp
 
 using namespace fbp;
@@ -396,6 +464,47 @@ fbp_status fbp_quantize_transient(fbp_context ctx, const float* src, uint64_t ro
         hipLaunchKernelGGL(quantize_kernel<FBP_STORAGE_BF16>, dim3(unsigned(rows)), dim3(256), 0,
                            as_stream(stream), src, cols, dst, scales);
     FBP_CHECK_HIP(ctx, hipGetLastError());
+    return FBP_OK;
+}
+
+fbp_status fbp_volume_residual(fbp_context ctx, const fbp_reconstruction_info* info,
+                               const float* volume, const float* transient,
+                               const fbp_vec3* laser_targets, const uint32_t* rows,
+                               uint32_t num_rows, float threshold_fraction, int32_t blur,
+                               double* host_residual, void* stream) {
+    if (!ctx || !info || !volume || !transient || !laser_targets || !rows || !num_rows ||
+        !host_residual || blur < 0)
+        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "residual: bad argument");
+    const uint32_t T = info->num_time_bins;
+    if (size_t(T) * 16 > 65536) return fail(ctx, FBP_ERR_UNSUPPORTED, "residual: at most 4096 bins");
+    FBP_CHECK(bind(ctx));
+    hipStream_t s = as_stream(stream);
+    VolumeGeom g = make_geom(*info, 1, 1, 1);
+    const size_t nv = size_t(g.res[0]) * g.res[1] * g.res[2];
+    float mnv, mxv;
+    FBP_CHECK(fbp_minmax(ctx, volume, nv, &mnv, &mxv, stream));
+    // minmax leaves {min, max} on the device in its scratch block; keep our
+    // own copy after it for the kernel.
+    FBP_CHECK(ensure_scratch2(ctx, size_t(num_rows) * 3 * sizeof(double) + 64));
+    double* sums = static_cast<double*>(ctx->scratch2);
+    float* mm = reinterpret_cast<float*>(sums + size_t(num_rows) * 3);
+    const float hmm[2] = {mnv, mxv};
+    FBP_CHECK_HIP(ctx, hipMemcpyAsync(mm, hmm, sizeof(hmm), hipMemcpyHostToDevice, s));
+    const size_t shmem = size_t(T) * 8 + size_t(T) * 2 * 4;
+    hipLaunchKernelGGL(residual_row_kernel, dim3(num_rows), dim3(256), shmem, s, volume, g, mm,
+                       threshold_fraction, laser_targets, rows, transient, T, info->time_step, blur,
+                       sums);
+    FBP_CHECK_HIP(ctx, hipGetLastError());
+    std::vector<double> h(size_t(num_rows) * 3);
+    FBP_CHECK_HIP(ctx, hipMemcpyAsync(h.data(), sums, h.size() * 8, hipMemcpyDeviceToHost, s));
+    FBP_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (uint32_t r = 0; r < num_rows; ++r) {
+        a += h[size_t(r) * 3];
+        b += h[size_t(r) * 3 + 1];
+        c += h[size_t(r) * 3 + 2];
+    }
+    *host_residual = (b > 0.0 && c > 0.0) ? 1.0 - a / std::sqrt(b * c) : 1.0;
     return FBP_OK;
 }
 

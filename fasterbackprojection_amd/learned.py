@@ -56,16 +56,7 @@ class Pipeline:
         # residual check: every 4th scan point in x and z
         n = SCAN
         idx = np.array([iz * n + ix for iz in range(0, n, 4) for ix in range(0, n, 4)])
-        self.sub_idx = torch.from_numpy(idx).to(self.dev)
-        self.sub_scan = self.laser[self.sub_idx].contiguous()
-        c = self.info.hidden_volume_voxel_size
-        m = self.info.hidden_volume_min
-        xs = (m.x + (np.arange(SIZE) + 0.5) * c.x).astype(np.float32)
-        ys = (m.y + (np.arange(SIZE) + 0.5) * c.y).astype(np.float32)
-        zs = (m.z + (np.arange(SIZE) + 0.5) * c.z).astype(np.float32)
-        # voxel (vx, vy, z) at [z][vy][vx] -> world (x, y=depth(z), z=vy)
-        Z, Y, X = np.meshgrid(ys, zs, xs, indexing="ij")
-        self.centres = torch.from_numpy(np.stack([X, Z, Y], -1).reshape(-1, 3)).to(self.dev)
+        self.sub_idx = torch.from_numpy(idx.astype(np.int32)).to(self.dev)
 
     def render(self, seed: int, peak: float = 1000.0) -> torch.Tensor:
         rng = np.random.default_rng(seed)
@@ -94,20 +85,8 @@ class Pipeline:
                                                  out=out)
 
     def residual(self, volume: torch.Tensor, I: torch.Tensor) -> float:
-        """1 - cosine similarity between the measured transient and the one the
-        volume's positive part renders, on every 16th scan point."""
-        v = volume.reshape(-1)
-        med = v.median()
-        keep = v > med + 0.5 * (v.max() - med)
-        pts = self.centres[keep].contiguous()
-        alb = v[keep].contiguous()
-        if pts.shape[0] == 0:
-            return 1.0
-        pred = self.ctx.render_confocal(self.sub_scan, pts, alb, BINS, self.g.time_step)
-        meas = I[self.sub_idx]
-        num = float((pred * meas).sum())
-        den = float(pred.norm() * meas.norm()) + 1e-30
-        return 1.0 - num / den
+        """Forward-model residual on every 16th scan point (fbp_volume_residual)."""
+        return self.ctx.volume_residual(self.info, volume, I, self.laser, self.sub_idx)
 
 
 def rel_l2(a: torch.Tensor, b: torch.Tensor) -> float:

@@ -241,6 +241,18 @@ fbp_status fbp_render_confocal(fbp_context ctx, const fbp_vec3* scan_points, uin
                                uint32_t num_time_bins, float delta, float* transient,
                                void* stream);
 
+/* Forward-model residual of a reconstructed volume (learned form's failure
+ * check, no reference counterpart): the voxels above
+ * min + threshold_fraction * (max - min) are rendered as points (albedo =
+ * excess over the threshold, 1/r^4, bin round(2r/Delta)) at the scan points
+ * rows[0..num_rows); both that and the measured rows are box-blurred over
+ * +-blur bins; host_residual = 1 - cosine similarity.  Synchronises. */
+fbp_status fbp_volume_residual(fbp_context ctx, const fbp_reconstruction_info* info,
+                               const float* volume, const float* transient,
+                               const fbp_vec3* laser_targets, const uint32_t* rows,
+                               uint32_t num_rows, float threshold_fraction, int32_t blur,
+                               double* host_residual, void* stream);
+
 /* ---------------------------------------------------------- learned form */
 
 /* Learned refiner (no reference counterpart): a 3-level 3D U-Net
