@@ -44,6 +44,10 @@ def parse():
     ap.add_argument("--bins", type=int, default=1024)
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the FP16/BF16 storage leg (256^2 x 1024 -> 128^3)")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-vs-plain leg (64^2 x 512 -> 64^3)")
+    ap.add_argument("--no-exhaustive", action="store_true",
+                    help="skip the exhaustive leg (16^2 lasers x 64^2 sensors x 1024 -> 64^3)")
     return ap.parse_args()
 
 
@@ -141,6 +145,8 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(info, I, g, args.cpu_seconds)
     precision = None if args.no_precision else precision_leg(ctx, args)
+    learned = None if args.no_learned else learned_leg(ctx)
+    exhaustive = None if args.no_exhaustive else exhaustive_leg(ctx, args)
 
     out = {
         "metric": METRIC,
@@ -175,6 +181,8 @@ def main():
         "breakdown_ms": {"backprojection": bp_avg, "log_and_normalize": float(np.mean(f_ms))},
         "cpu_baseline": cpu,
         "precision": precision,
+        "learned": learned,
+        "exhaustive": exhaustive,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
@@ -237,6 +245,124 @@ def precision_leg(ctx, args):
                      "max_abs_rel_to_peak": float(np.abs(err).max() / np.abs(ref).max())}
         log(f"precision {name}: {out[name]}")
     return out
+
+
+def exhaustive_leg(ctx, args, reps=3):
+    """Exhaustive capture: 16x16 laser targets x 64x64 sensor targets on a
+    2.0 m wall, 1024 bins of 16 ps (1,048,576 histograms, 4 GiB FP32), 8
+    seeded random spheres (8000 samples) with 1/r^2 1/r^2 falloff and Poisson
+    noise at a peak of 1000 -> 64^3 ellipsoidal backprojection + Laplacian
+    (k = 5).  25 flops per pair-update (two distances)."""
+    import numpy as np
+    import torch
+
+    from fasterbackprojection_amd.api import make_info
+    from fasterbackprojection_amd.scene import exhaustive_geometry, random_spheres
+
+    dev = torch.device("cuda", 0)
+    g = exhaustive_geometry()
+    info = make_info(g)
+    lasers = torch.from_numpy(g.laser_targets).to(dev)
+    sensors = torch.from_numpy(g.sensor_targets).to(dev)
+    sc = random_spheres(np.random.default_rng(3), 8, 8000)
+    pts = torch.from_numpy(sc.points.astype(np.float32)).to(dev)
+    alb = torch.from_numpy(sc.albedo.astype(np.float32)).to(dev)
+    I = ctx.render_exhaustive(lasers, sensors, pts, alb, g.num_time_bins, g.time_step)
+    I.mul_(1000.0 / I.max())
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(3)
+    I = torch.poisson(I, generator=gen)
+    torch.cuda.synchronize()
+    L, S = len(g.laser_targets), len(g.sensor_targets)
+    res = g.voxel_resolution[0]
+    updates = L * S * res ** 3
+    vol = torch.empty((res,) * 3, dtype=torch.float32, device=dev)
+    filt = torch.empty_like(vol)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()
+        ms = []
+        for _ in range(reps):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            b.record(stream)
+            b.synchronize()
+            ms.append(a.elapsed_time(b))
+        return float(np.mean(ms))
+
+    bp = timed(lambda: ctx.backproject_exhaustive(info, I, lasers, sensors, out=vol))
+    lap = timed(lambda: ctx.laplacian(vol, 5, out=filt))
+    tflops = 25 * updates / (bp * 1e-3) / 1e12
+    out = {"workload": "exhaustive 16^2 lasers x 64^2 sensors x 1024 (16 ps, 4 GiB) -> 64^3 "
+                       "+ Laplacian(k=5)",
+           "pair_updates": updates, "backprojection_ms": bp, "laplacian_ms": lap,
+           "pair_updates_per_s": updates / ((bp + lap) * 1e-3),
+           "roofline": {"bound": "valu", "achieved": tflops, "peak": PEAK_FP32_TFLOPS,
+                        "unit": "TFLOP/s", "frac": tflops / PEAK_FP32_TFLOPS,
+                        "flops_per_update": 25},
+           "finite": bool(torch.isfinite(vol).all())}
+    log(f"exhaustive: {out}")
+    del I
+    torch.cuda.empty_cache()
+    return out
+
+
+def learned_leg(ctx, n_scenes=16):
+    """64x64 confocal scan, 512 bins of 16 ps, 64^3 volume: plain path (all
+    scan points + LoG + normalisation) against the learned path (MIS at 1/16
+    of the scan-point x bin pairs + refiner) on held-out seeds."""
+    import numpy as np
+    import torch
+
+    from fasterbackprojection_amd.learned import WEIGHTS, LearnedReconstructor, rel_l2
+
+    if not (WEIGHTS / "refiner_c5.f32").exists():
+        log("learned leg skipped: no trained weights")
+        return None
+    lrn = LearnedReconstructor(ctx)
+    p = lrn.p
+    stream = torch.cuda.current_stream()
+
+    def timed(fn, reps=5):
+        fn()
+        ms = []
+        for _ in range(reps):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            b.record(stream)
+            b.synchronize()
+            ms.append(a.elapsed_time(b))
+        return float(np.median(ms))
+
+    t_plain, t_learn, t_check, e_learn, e_base, fb = [], [], [], [], [], 0
+    for s in range(2_000_000, 2_000_000 + n_scenes):
+        I = p.render(s)
+        y = p.plain(I).clone()
+        t_plain.append(timed(lambda: p.plain(I)))
+        t_learn.append(timed(lambda: lrn(I, s, check=False)))
+        t0 = time.perf_counter()
+        out, fell_back, _ = lrn(I, s, check=True)
+        t_check.append((time.perf_counter() - t0) * 1e3)
+        fb += int(fell_back)
+        out, _, _ = lrn(I, s, check=False)
+        e_learn.append(rel_l2(out, y))
+        e_base.append(rel_l2(p.mis_input(I, s), y))
+    res = {"workload": "confocal 64x64x512 (16 ps, 1-6 quads/spheres) -> 64^3; "
+                       "learned = MIS (L/16 samples) + LoG + 3-level U-Net refiner",
+           "held_out_scenes": n_scenes,
+           "plain_ms": float(np.mean(t_plain)), "learned_ms": float(np.mean(t_learn)),
+           "speedup": float(np.mean(t_plain) / np.mean(t_learn)),
+           "learned_with_residual_check_ms": float(np.mean(t_check)),
+           "rel_l2_learned_vs_plain": float(np.mean(e_learn)),
+           "rel_l2_unrefined_mis_vs_plain": float(np.mean(e_base)),
+           "fallbacks": fb, "residual_threshold": lrn.threshold}
+    log(f"learned: {res}")
+    return res
 
 
 def cpu_baseline(info, I, g, target_seconds):

@@ -23,6 +23,7 @@ CONFOCAL, EXHAUSTIVE = 0, 1
 STORAGE_F32, STORAGE_F16, STORAGE_BF16 = 0, 1, 2
 LAYOUT_TIME_FASTEST, LAYOUT_TIME_MAJOR = 0, 1
 MIS_REFERENCE, MIS_UNBIASED = 0, 1
+OPTION_CONFOCAL_ALIGNED = 0
 
 
 class Vec3(C.Structure):
@@ -80,6 +81,9 @@ SIGNATURES = {
     "fbp_context_last_error": (C.c_char_p, [_CTX]),
     "fbp_context_device": (C.c_int, [_CTX]),
     "fbp_context_scratch_bytes": (C.c_size_t, [_CTX]),
+    "fbp_context_set_option": (_S, [_CTX, C.c_int32, C.c_int32]),
+    "fbp_context_get_option": (_S, [_CTX, C.c_int32, C.POINTER(C.c_int32)]),
+    "fbp_context_last_confocal_path": (_S, [_CTX, C.POINTER(C.c_int32)]),
     "fbp_backproject_confocal_voxel": (_S, [_CTX, C.POINTER(ReconstructionInfo), C.POINTER(Transient), _P, _P, _P]),
     "fbp_backproject_exhaustive_voxel": (_S, [_CTX, C.POINTER(ReconstructionInfo), C.POINTER(Transient), _P, _P, _P, _P]),
     "fbp_backproject_depth_confocal": (_S, [_CTX, C.POINTER(ReconstructionInfo), C.POINTER(Transient), _P, _P, C.c_uint32, _P, _P]),
@@ -97,7 +101,8 @@ SIGNATURES = {
     "fbp_filter_h_padded_length": (C.c_uint32, [C.c_uint32, C.c_float, C.c_float, C.c_float]),
     "fbp_lct": (_S, [_CTX, _P, C.POINTER(LctParams), _P, _P]),
     "fbp_render_confocal": (_S, [_CTX, _P, C.c_uint32, _P, _P, C.c_uint32, C.c_uint32, C.c_float, _P, _P]),
-    "fbp_volume_residual": (_S, [_CTX, C.POINTER(ReconstructionInfo), _P, _P, _P, _P, C.c_uint32, C.c_float, C.c_int32, C.POINTER(C.c_double), _P]),
+    "fbp_render_exhaustive": (_S, [_CTX, _P, C.c_uint32, _P, C.c_uint32, _P, _P, C.c_uint32, C.c_uint32, C.c_float, _P, _P]),
+    "fbp_volume_residual": (_S, [_CTX, C.POINTER(ReconstructionInfo), _P, _P, _P, _P, C.c_uint32, C.c_float, C.c_int32, C.c_int32, C.POINTER(C.c_double), _P]),
     "fbp_refiner_param_count": (C.c_size_t, []),
     "fbp_refiner_create": (_S, [_CTX, C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]),
     "fbp_refiner_destroy": (_S, [C.c_void_p]),

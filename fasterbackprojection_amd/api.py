@@ -118,6 +118,18 @@ class Context:
             msg = self.lib.fbp_context_last_error(self.handle).decode()
             raise FbpError(st, msg)
 
+    def set_aligned_path(self, enabled: bool) -> None:
+        """Allow (default) or forbid the aligned-grid confocal path."""
+        self.check(self.lib.fbp_context_set_option(self.handle, _lib.OPTION_CONFOCAL_ALIGNED,
+                                                   int(bool(enabled))))
+
+    def last_confocal_path_aligned(self) -> bool:
+        """True when the last confocal voxel backprojection took the
+        aligned-grid path (synchronises)."""
+        v = C.c_int32()
+        self.check(self.lib.fbp_context_last_confocal_path(self.handle, C.byref(v)))
+        return bool(v.value)
+
     # ------------------------------------------------------------ helpers
     def _dev(self) -> torch.device:
         return torch.device("cuda", self.device)
@@ -249,14 +261,26 @@ class Context:
         self.check(self.lib.fbp_lct(self.handle, _ptr(data), C.byref(p), _ptr(out), _stream()))
         return out
 
+    def render_exhaustive(self, lasers, sensors, points, albedo, num_time_bins, delta, out=None):
+        L, S = lasers.shape[0], sensors.shape[0]
+        if out is None:
+            out = torch.empty((L, S, num_time_bins), dtype=torch.float32, device=lasers.device)
+        self.check(self.lib.fbp_render_exhaustive(self.handle, _ptr(lasers), L, _ptr(sensors), S,
+                                                  _ptr(points), _ptr(albedo), points.shape[0],
+                                                  num_time_bins, float(delta), _ptr(out),
+                                                  _stream()))
+        return out
+
     def volume_residual(self, info, volume, intensity, laser_targets, rows,
-                        threshold_fraction=0.6, blur=4) -> float:
-        """1 - cosine(measured rows, rows rendered from the volume's bright voxels)."""
+                        threshold_fraction=0.6, blur=4, polarity=1) -> float:
+        """1 - Pearson correlation of the measured rows and the rows rendered
+        from the volume's voxels above (polarity 1) or below (polarity -1)
+        the threshold (fbp_volume_residual)."""
         r = C.c_double()
         self.check(self.lib.fbp_volume_residual(
             self.handle, C.byref(info), _ptr(volume), _ptr(intensity), _ptr(laser_targets),
-            _ptr(rows), int(rows.numel()), float(threshold_fraction), int(blur), C.byref(r),
-            _stream()))
+            _ptr(rows), int(rows.numel()), float(threshold_fraction), int(polarity), int(blur),
+            C.byref(r), _stream()))
         return r.value
 
     def render_confocal(self, scan_points, points, albedo, num_time_bins, delta, out=None):

@@ -129,8 +129,9 @@ __device__ __forceinline__ float box_max_dist(const float p[3], const float lo[3
 // Sums per-chunk partial tiles in FP64 (fixed order) and scatters them to
 // the [Z][Y][X] volume.  partial layout: [chunk][tile][slot][lane(64)].
 // corr (may be null) is a second buffer of the same layout added in.
+// skip (may be null): when *skip != 0 on the device the kernel returns.
 fbp_status launch_reduce(fbp_context ctx, const float* partial, const float* corr, float* volume,
                          const VolumeGeom& g, uint32_t num_chunks, double scale,
-                         hipStream_t stream);
+                         hipStream_t stream, const int* skip = nullptr);
 
 }  // namespace fbp

@@ -21,6 +21,7 @@
 // per-axis tables in glm::dot's order.  Chunk partial sums are reduced in
 // FP64 in a fixed order, so results are deterministic.
 #include "bp_common.h"
+#include "bp_confocal_aligned.h"
 
 namespace fbp {
 
@@ -40,6 +41,7 @@ struct ConfocalArgs {
     const float* thr;      // S_k rows, (T + 1) floats each
     uint32_t thr_rows;     // 1 (pure chain) or L
     float* corr;           // near-voxel corrections, same layout as partial
+    const int* skip;       // when non-null and *skip != 0 the aligned path ran: return
 };
 
 // Per scan point parameters staged in LDS (8 words each).
@@ -106,6 +108,7 @@ template <int STORAGE, int LAYOUT, bool PURE>
 __device__ __forceinline__ void confocal_voxel_body(const ConfocalArgs& a) {
     // Quantised (precision-form) transients accumulate with Kahan compensation.
     constexpr bool KAHAN = STORAGE != FBP_STORAGE_F32;
+    if (a.skip && *a.skip) return;
     extern __shared__ float smem[];
     float2* win = reinterpret_cast<float2*>(smem);           // batch * W pairs
     float* prm = smem + 2 * a.batch * a.W;                    // batch * 8
@@ -258,7 +261,9 @@ __global__ __launch_bounds__(64, 1) void confocal_voxel_kahan_kernel(ConfocalArg
 // point, or a single row for the pure chain (t0 = 0, bounces discarded).
 __global__ void confocal_thresholds_kernel(const fbp_vec3* __restrict__ laser, uint32_t rows,
                                            uint32_t T, float t0, float dt, uint32_t discard,
-                                           fbp_vec3 lpos, fbp_vec3 spos, float* __restrict__ thr) {
+                                           fbp_vec3 lpos, fbp_vec3 spos, float* __restrict__ thr,
+                                           const int* __restrict__ skip) {
+    if (skip && *skip) return;
     const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= size_t(rows) * (T + 1)) return;
     const uint32_t row = uint32_t(idx / (T + 1));
@@ -292,7 +297,9 @@ __global__ void confocal_thresholds_kernel(const fbp_vec3* __restrict__ laser, u
 
 __global__ void reduce_partials_kernel(const float* __restrict__ partial,
                                        const float* __restrict__ corr, float* __restrict__ volume,
-                                       VolumeGeom g, uint32_t num_chunks, double scale) {
+                                       VolumeGeom g, uint32_t num_chunks, double scale,
+                                       const int* __restrict__ skip) {
+    if (skip && *skip) return;
     const size_t n = size_t(g.res[0]) * g.res[1] * g.res[2];
     const size_t v = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (v >= n) return;
@@ -315,11 +322,11 @@ __global__ void reduce_partials_kernel(const float* __restrict__ partial,
 
 fbp_status launch_reduce(fbp_context ctx, const float* partial, const float* corr, float* volume,
                          const VolumeGeom& g, uint32_t num_chunks, double scale,
-                         hipStream_t stream) {
+                         hipStream_t stream, const int* skip) {
     const size_t n = size_t(g.res[0]) * g.res[1] * g.res[2];
     const unsigned blocks = unsigned((n + 255) / 256);
     hipLaunchKernelGGL(reduce_partials_kernel, dim3(blocks), dim3(256), 0, stream, partial, corr,
-                       volume, g, num_chunks, scale);
+                       volume, g, num_chunks, scale, skip);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     return FBP_OK;
 }
@@ -413,15 +420,29 @@ extern "C" fbp_status fbp_backproject_confocal_voxel(fbp_context ctx,
     a.thr_rows = pure ? 1u : a.L;
     const size_t slab = size_t(chunks) * a.g.num_tiles * a.g.slots * 64;
     const size_t thr_n = size_t(a.thr_rows) * (a.T + 1);
-    FBP_CHECK(ensure_scratch(ctx, (2 * slab + thr_n) * sizeof(float) + 256));
-    a.partial = static_cast<float*>(ctx->scratch);
+    // Scratch holds the general path's buffers or the aligned path's: only
+    // one of the two paths writes, so they share the space.
+    const AlignedPlan ap = plan_aligned_confocal(ctx, *info, *tr);
+    const size_t general_bytes = (2 * slab + thr_n) * sizeof(float) + 256;
+    const size_t body = std::max(general_bytes, ap.usable ? aligned_scratch_bytes(ap) : size_t(0));
+    FBP_CHECK(ensure_scratch(ctx, body));
+    if (ap.usable && !ctx->path_flag)
+        FBP_CHECK_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->path_flag), 256));
+    int* flag = ctx->path_flag;
+    char* base = static_cast<char*>(ctx->scratch);
+    a.partial = reinterpret_cast<float*>(base);
     a.corr = a.partial + slab;
     a.thr = a.corr + slab;
+    a.skip = ap.usable ? flag : nullptr;
 
     hipStream_t s = as_stream(stream);
+    ctx->path_flag_valid = ap.usable;
+    if (ap.usable)
+        FBP_CHECK(launch_aligned_confocal(ctx, ap, *info, *tr, laser_targets, volume, flag, base, s));
     hipLaunchKernelGGL(confocal_thresholds_kernel, dim3(unsigned((thr_n + 255) / 256)), dim3(256),
                        0, s, laser_targets, a.thr_rows, a.T, a.t0, a.dt, a.discard,
-                       info->laser_position, info->sensor_position, const_cast<float*>(a.thr));
+                       info->laser_position, info->sensor_position, const_cast<float*>(a.thr),
+                       a.skip);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     const dim3 grid(chunks * a.g.num_tiles);
     const size_t shmem = size_t(a.batch) * (2 * a.W + 8) * sizeof(float);
@@ -435,5 +456,5 @@ extern "C" fbp_status fbp_backproject_confocal_voxel(fbp_context ctx,
         launch_confocal<FBP_STORAGE_BF16, FBP_LAYOUT_TIME_FASTEST>(a, pure, grid, shmem, s);
     }
     FBP_CHECK_HIP(ctx, hipGetLastError());
-    return launch_reduce(ctx, a.partial, a.corr, volume, a.g, chunks, 1.0, s);
+    return launch_reduce(ctx, a.partial, a.corr, volume, a.g, chunks, 1.0, s, a.skip);
 }

@@ -225,13 +225,43 @@ __global__ void render_confocal_kernel(const fbp_vec3* __restrict__ scan, uint32
     for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) row[t] = float(double(hist[t]) / kRenderFix);
 }
 
+// One workgroup per (laser, sensor) pair: I[l][s][round((|l-p| + |p-s|) / delta)]
+// += albedo / (|l-p|^2 |p-s|^2), fixed-point LDS accumulation as above.
+__global__ void render_exhaustive_kernel(const fbp_vec3* __restrict__ lasers,
+                                         const fbp_vec3* __restrict__ sensors, uint32_t S,
+                                         const fbp_vec3* __restrict__ pts,
+                                         const float* __restrict__ alb, uint32_t num_points,
+                                         uint32_t T, float delta, float* __restrict__ transient) {
+    extern __shared__ unsigned long long hist[];
+    const uint64_t pair = blockIdx.x;
+    const uint32_t l = uint32_t(pair / S), s = uint32_t(pair % S);
+    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) hist[t] = 0ull;
+    __syncthreads();
+    const float lx = lasers[l].x, ly = lasers[l].y, lz = lasers[l].z;
+    const float sx = sensors[s].x, sy = sensors[s].y, sz = sensors[s].z;
+    for (uint32_t p = threadIdx.x; p < num_points; p += blockDim.x) {
+        const float ax = pts[p].x - lx, ay = pts[p].y - ly, az = pts[p].z - lz;
+        const float bx = pts[p].x - sx, by = pts[p].y - sy, bz = pts[p].z - sz;
+        const float ra2 = ax * ax + ay * ay + az * az, rb2 = bx * bx + by * by + bz * bz;
+        const int bin = int(rintf((sqrtf(ra2) + sqrtf(rb2)) / delta));
+        if (bin < 0 || bin >= int(T) || !(ra2 > 0.f) || !(rb2 > 0.f)) continue;
+        const double v = double(alb[p]) / (double(ra2) * double(rb2));
+        atomicAdd(&hist[bin], (unsigned long long)(v * kRenderFix + 0.5));
+    }
+    __syncthreads();
+    float* row = transient + pair * T;
+    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) row[t] = float(double(hist[t]) / kRenderFix);
+}
+
 // ----------------------------------------------------- forward-model residual
 // For scan point rows[r]: render the volume's voxels above `thr` (albedo
-// v - thr, at their centres) with the same 1/r^4 model, box-blur both that
-// and the measured row over +-blur bins, and accumulate <p, m>, |p|^2, |m|^2
-// in FP64 (fixed-point free: one workgroup per row writes its three sums).
+// v - thr, at their centres; with polarity -1 the voxels below it, albedo
+// thr - v) with the same 1/r^4 model, box-blur both that
+// and the measured row over +-blur bins, and accumulate sum p, sum m,
+// sum pm, sum p^2, sum m^2 over the bins in FP64 (one workgroup per row
+// writes its five sums).
 __global__ void residual_row_kernel(const float* __restrict__ volume, VolumeGeom g,
-                                    const float* __restrict__ mm, float thr_frac,
+                                    const float* __restrict__ mm, float thr_frac, float polarity,
                                     const fbp_vec3* __restrict__ scan, const uint32_t* __restrict__ rows,
                                     const float* __restrict__ I, uint32_t T, float delta, int blur,
                                     double* __restrict__ sums) {
@@ -246,7 +276,7 @@ __global__ void residual_row_kernel(const float* __restrict__ volume, VolumeGeom
     const float sx = scan[l].x, sy = scan[l].y, sz = scan[l].z;
     const size_t n = size_t(g.res[0]) * g.res[1] * g.res[2];
     for (size_t v = threadIdx.x; v < n; v += blockDim.x) {
-        const float a = volume[v] - thr;
+        const float a = polarity * (volume[v] - thr);
         if (!(a > 0.f)) continue;
         const uint32_t vx = uint32_t(v % g.res[0]), vy = uint32_t((v / g.res[0]) % g.res[1]),
                        z = uint32_t(v / (size_t(g.res[0]) * g.res[1]));
@@ -271,25 +301,25 @@ __global__ void residual_row_kernel(const float* __restrict__ volume, VolumeGeom
         pm[T + t] = float(sm);
     }
     __syncthreads();
-    __shared__ double red[3][256];
-    double a = 0.0, b = 0.0, c = 0.0;
+    __shared__ double red[5][256];
+    double q5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
         const double p = pm[t], m = pm[T + t];
-        a += p * m;
-        b += p * p;
-        c += m * m;
+        q5[0] += p;
+        q5[1] += m;
+        q5[2] += p * m;
+        q5[3] += p * p;
+        q5[4] += m * m;
     }
-    red[0][threadIdx.x] = a;
-    red[1][threadIdx.x] = b;
-    red[2][threadIdx.x] = c;
+    for (int q = 0; q < 5; ++q) red[q][threadIdx.x] = q5[q];
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
         if (int(threadIdx.x) < w)
-            for (int q = 0; q < 3; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+            for (int q = 0; q < 5; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int q = 0; q < 3; ++q) sums[size_t(r) * 3 + q] = red[q][0];
+        for (int q = 0; q < 5; ++q) sums[size_t(r) * 5 + q] = red[q][0];
 }
 
 }  // namespace fbp
@@ -467,13 +497,34 @@ fbp_status fbp_quantize_transient(fbp_context ctx, const float* src, uint64_t ro
     return FBP_OK;
 }
 
+fbp_status fbp_render_exhaustive(fbp_context ctx, const fbp_vec3* laser_points, uint32_t num_lasers,
+                                 const fbp_vec3* sensor_points, uint32_t num_sensors,
+                                 const fbp_vec3* points, const float* albedo, uint32_t num_points,
+                                 uint32_t num_time_bins, float delta, float* transient,
+                                 void* stream) {
+    if (!ctx || !laser_points || !sensor_points || !points || !albedo || !transient ||
+        !num_lasers || !num_sensors || !num_time_bins || !(delta > 0.f))
+        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "render_exhaustive: bad argument");
+    if (size_t(num_time_bins) * 8 > 65536)
+        return fail(ctx, FBP_ERR_UNSUPPORTED, "render_exhaustive: at most 8192 time bins");
+    FBP_CHECK(bind(ctx));
+    const uint64_t pairs = uint64_t(num_lasers) * num_sensors;
+    if (pairs >= (1ull << 31))
+        return fail(ctx, FBP_ERR_UNSUPPORTED, "render_exhaustive: at most 2^31 - 1 histograms");
+    hipLaunchKernelGGL(render_exhaustive_kernel, dim3(unsigned(pairs)), dim3(256),
+                       size_t(num_time_bins) * 8, as_stream(stream), laser_points, sensor_points,
+                       num_sensors, points, albedo, num_points, num_time_bins, delta, transient);
+    FBP_CHECK_HIP(ctx, hipGetLastError());
+    return FBP_OK;
+}
+
 fbp_status fbp_volume_residual(fbp_context ctx, const fbp_reconstruction_info* info,
                                const float* volume, const float* transient,
                                const fbp_vec3* laser_targets, const uint32_t* rows,
-                               uint32_t num_rows, float threshold_fraction, int32_t blur,
-                               double* host_residual, void* stream) {
+                               uint32_t num_rows, float threshold_fraction, int32_t polarity,
+                               int32_t blur, double* host_residual, void* stream) {
     if (!ctx || !info || !volume || !transient || !laser_targets || !rows || !num_rows ||
-        !host_residual || blur < 0)
+        !host_residual || blur < 0 || (polarity != 1 && polarity != -1))
         return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "residual: bad argument");
     const uint32_t T = info->num_time_bins;
     if (size_t(T) * 16 > 65536) return fail(ctx, FBP_ERR_UNSUPPORTED, "residual: at most 4096 bins");
@@ -485,26 +536,29 @@ fbp_status fbp_volume_residual(fbp_context ctx, const fbp_reconstruction_info* i
     FBP_CHECK(fbp_minmax(ctx, volume, nv, &mnv, &mxv, stream));
     // minmax leaves {min, max} on the device in its scratch block; keep our
     // own copy after it for the kernel.
-    FBP_CHECK(ensure_scratch2(ctx, size_t(num_rows) * 3 * sizeof(double) + 64));
+    FBP_CHECK(ensure_scratch2(ctx, size_t(num_rows) * 5 * sizeof(double) + 64));
     double* sums = static_cast<double*>(ctx->scratch2);
-    float* mm = reinterpret_cast<float*>(sums + size_t(num_rows) * 3);
+    float* mm = reinterpret_cast<float*>(sums + size_t(num_rows) * 5);
     const float hmm[2] = {mnv, mxv};
     FBP_CHECK_HIP(ctx, hipMemcpyAsync(mm, hmm, sizeof(hmm), hipMemcpyHostToDevice, s));
     const size_t shmem = size_t(T) * 8 + size_t(T) * 2 * 4;
     hipLaunchKernelGGL(residual_row_kernel, dim3(num_rows), dim3(256), shmem, s, volume, g, mm,
-                       threshold_fraction, laser_targets, rows, transient, T, info->time_step, blur,
+                       threshold_fraction, float(polarity), laser_targets, rows, transient, T, info->time_step, blur,
                        sums);
     FBP_CHECK_HIP(ctx, hipGetLastError());
-    std::vector<double> h(size_t(num_rows) * 3);
+    std::vector<double> h(size_t(num_rows) * 5);
     FBP_CHECK_HIP(ctx, hipMemcpyAsync(h.data(), sums, h.size() * 8, hipMemcpyDeviceToHost, s));
     FBP_CHECK_HIP(ctx, hipStreamSynchronize(s));
-    double a = 0.0, b = 0.0, c = 0.0;
+    // Per-row centred second moments, pooled over the rows: 1 - Pearson
+    // correlation of rendered and measured rows.
+    double cov = 0.0, vp = 0.0, vm = 0.0;
     for (uint32_t r = 0; r < num_rows; ++r) {
-        a += h[size_t(r) * 3];
-        b += h[size_t(r) * 3 + 1];
-        c += h[size_t(r) * 3 + 2];
+        const double* q = &h[size_t(r) * 5];
+        cov += q[2] - q[0] * q[1] / T;
+        vp += q[3] - q[0] * q[0] / T;
+        vm += q[4] - q[1] * q[1] / T;
     }
-    *host_residual = (b > 0.0 && c > 0.0) ? 1.0 - a / std::sqrt(b * c) : 1.0;
+    *host_residual = (vp > 0.0 && vm > 0.0) ? 1.0 - cov / std::sqrt(vp * vm) : 1.0;
     return FBP_OK;
 }
 

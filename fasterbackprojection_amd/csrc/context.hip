@@ -104,11 +104,48 @@ fbp_status fbp_context_create(int device, fbp_context* out) {
         return FBP_ERR_DEVICE;
     }
     ctx->num_cus = p.multiProcessorCount;
+    ctx->max_lds_per_block = int(p.sharedMemPerBlock);
     if (hipHostMalloc((void**)&ctx->pinned, 64 * sizeof(float), hipHostMallocDefault) != hipSuccess) {
         delete ctx;
         return FBP_ERR_OUT_OF_MEMORY;
     }
     *out = ctx;
+    return FBP_OK;
+}
+
+fbp_status fbp_context_set_option(fbp_context ctx, int32_t option, int32_t value) {
+    if (!ctx) return FBP_ERR_INVALID_ARGUMENT;
+    switch (option) {
+        case FBP_OPTION_CONFOCAL_ALIGNED:
+            if (value != 0 && value != 1)
+                return fbp::fail(ctx, FBP_ERR_INVALID_ARGUMENT, "option value must be 0 or 1");
+            ctx->disable_aligned = value ? 0 : 1;
+            return FBP_OK;
+        default:
+            return fbp::fail(ctx, FBP_ERR_INVALID_ARGUMENT, "unknown option %d", int(option));
+    }
+}
+
+fbp_status fbp_context_get_option(fbp_context ctx, int32_t option, int32_t* value) {
+    if (!ctx || !value) return FBP_ERR_INVALID_ARGUMENT;
+    switch (option) {
+        case FBP_OPTION_CONFOCAL_ALIGNED:
+            *value = ctx->disable_aligned ? 0 : 1;
+            return FBP_OK;
+        default:
+            return fbp::fail(ctx, FBP_ERR_INVALID_ARGUMENT, "unknown option %d", int(option));
+    }
+}
+
+fbp_status fbp_context_last_confocal_path(fbp_context ctx, int32_t* aligned) {
+    if (!ctx || !aligned) return FBP_ERR_INVALID_ARGUMENT;
+    FBP_CHECK(fbp::bind(ctx));
+    *aligned = 0;
+    if (!ctx->path_flag_valid || !ctx->path_flag) return FBP_OK;
+    int v = 0;
+    FBP_CHECK_HIP(ctx, hipDeviceSynchronize());
+    FBP_CHECK_HIP(ctx, hipMemcpy(&v, ctx->path_flag, sizeof(int), hipMemcpyDeviceToHost));
+    *aligned = v;
     return FBP_OK;
 }
 
@@ -119,6 +156,7 @@ fbp_status fbp_context_destroy(fbp_context ctx) {
     fbp::fft_cache_destroy(ctx);
     if (ctx->scratch) hipFree(ctx->scratch);
     if (ctx->scratch2) hipFree(ctx->scratch2);
+    if (ctx->path_flag) hipFree(ctx->path_flag);
     if (ctx->pinned) hipHostFree(ctx->pinned);
     delete ctx;
     return FBP_OK;

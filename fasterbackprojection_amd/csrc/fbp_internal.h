@@ -12,6 +12,14 @@
 struct fbp_context_s {
     int device = 0;
     int num_cus = 0;
+    int max_lds_per_block = 65536;
+    // Confocal backprojection: 0 = the aligned-grid path may run when its
+    // conditions hold (default), 1 = always the general kernel.
+    int disable_aligned = 0;
+    // Device int: path flag of the last confocal voxel call (1 = aligned
+    // path), valid when path_flag_valid is set.
+    int* path_flag = nullptr;
+    bool path_flag_valid = false;
     std::string last_error;
     // Scratch arena: grown on demand, never shrunk.  Used for per-chunk
     // partial sums, window tables, FFT buffers.

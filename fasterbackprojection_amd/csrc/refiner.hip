@@ -1,13 +1,14 @@
-// Learned refiner: a 3-level 3D U-Net (16/32/64 channels) that maps a
-// normalised MIS-sampled backprojection to the plain backprojection + LoG
-// volume (min-max normalised).  No reference counterpart.
+// Learned refiner: a 3-level 3D U-Net (16/32/64 channels) that maps the
+// LoG-filtered, normalised MIS-sampled backprojection to the plain
+// backprojection + LoG volume (min-max normalised).  No reference
+// counterpart.  x is the input divided by its largest magnitude.
 //
 //   e1 = relu(conv3(x; 1->16))            @ S        (VALU)
 //   e2 = relu(conv3(pool(e1); 16->32))    @ S/2      (MFMA)
 //   e3 = relu(conv3(pool(e2); 32->64))    @ S/4      (MFMA)
 //   d2 = relu(conv3(up(e3); 64->32)) + e2 @ S/2      (MFMA)
 //   d1 = relu(conv1(up(d2); 32->16)) + e1 @ S        (MFMA, 1x1x1)
-//   y  = conv3(d1; 16->1)                 @ S        (VALU)
+//   y  = conv3(d1; 16->1) + x             @ S        (VALU)
 //
 // Forward convolutions with 16+ input and output channels run as BF16
 // implicit GEMMs on MFMA with FP32 accumulation; activations are BF16.
@@ -476,9 +477,12 @@ fbp_status refiner_forward_impl(fbp_refiner_s* r, hipStream_t s, bool train) {
     // d1 = relu(conv1(u1; 32->16)) + e1
     hipLaunchKernelGGL((conv_mfma_fwd<1, 32, 16>), g1(V / 16 * 64), dim3(256), 0, s, r->u1,
                        Wb + woff(4), P + boff(4), r->d1, r->e1, train ? r->pre5 : nullptr, d1, 1);
-    // y = conv3(d1; 16->1)
+    // y = conv3(d1; 16->1) + x: the network predicts a correction of its
+    // (normalised) input; the input needs no gradient, so the backward pass
+    // starts from dL/dy unchanged.
     hipLaunchKernelGGL((conv_valu_fwd<16, 1, 3, bf16>), g1(V), dim3(256), 0, s, r->d1, P + woff(5),
                        P + boff(5), r->y, nullptr, d1, 0);
+    hipLaunchKernelGGL(add_into, g1(V), dim3(256), 0, s, r->y, r->x, V);
     FBP_CHECK_HIP(r->ctx, hipGetLastError());
     (void)V2;
     return FBP_OK;

@@ -5,14 +5,18 @@ residual check that falls back to the plain path.
 Workload: 64 x 64 confocal scan on a 1.0 m wall, 512 bins of 16 ps; 64^3
 volume over [-0.5, 0.5] x [0.2, 1.0] x [-0.5, 0.5].  Scenes: 1-6 random
 quads and spheres (scene.mixed_scene), 1/r^4 falloff, Poisson noise at a
-peak of 1000 counts.  Input: unbiased one-sample MIS estimate with L/16
-samples per voxel (1/16 of the scan-point x bin pairs).  Target: plain
-backprojection, LoG (sigma 1, k 5), min-max normalised.
+peak of 1000 counts.  Input: unbiased MIS estimate with L/16 samples per
+voxel (1/16 of the scan-point x bin pairs).  Target: plain
+backprojection, LoG (sigma 1, k 5), min-max normalised.  The input goes
+through the same LoG + normalisation, and the refiner predicts a correction
+of it (its last layer adds the input), so the un-refined MIS path is the
+starting point the training improves on.
 
 Every step runs in libfbp.so kernels; torch holds device memory, its RNG
 draws the Poisson counts and noise buffers.
 
     python -m fasterbackprojection_amd.learned train --out DIR [--scenes N --steps K]
+    python -m fasterbackprojection_amd.learned calibrate [--weights FILE]
 """
 from __future__ import annotations
 
@@ -53,6 +57,7 @@ class Pipeline:
         self.L = len(self.g.laser_targets)
         self.vol = torch.empty((SIZE,) * 3, dtype=torch.float32, device=self.dev)
         self.filt = torch.empty_like(self.vol)
+        self.mis = torch.empty_like(self.vol)
         # residual check: every 4th scan point in x and z
         n = SCAN
         idx = np.array([iz * n + ix for iz in range(0, n, 4) for ix in range(0, n, 4)])
@@ -77,16 +82,26 @@ class Pipeline:
         return out
 
     def mis_input(self, I: torch.Tensor, seed: int, out: torch.Tensor | None = None) -> torch.Tensor:
+        """Refiner input: unbiased MIS backprojection + LoG + min-max
+        normalisation (the same filter chain as the plain path)."""
         gen = torch.Generator(device=self.dev)
         gen.manual_seed(10_000_019 + seed)
         noise = torch.rand(SIZE ** 3, generator=gen, device=self.dev)
-        return self.ctx.backproject_confocal_mis(self.info, I, self.laser, noise,
-                                                 self.L // SAMPLE_FRACTION, mode=MIS_UNBIASED,
-                                                 out=out)
+        self.ctx.backproject_confocal_mis(self.info, I, self.laser, noise,
+                                          self.L // SAMPLE_FRACTION, mode=MIS_UNBIASED,
+                                          out=self.mis)
+        out = torch.empty_like(self.vol) if out is None else out
+        self.ctx.log(self.mis, 5, 1.0, out=out)
+        self.ctx.normalize(out)
+        return out
 
     def residual(self, volume: torch.Tensor, I: torch.Tensor) -> float:
-        """Forward-model residual on every 16th scan point (fbp_volume_residual)."""
-        return self.ctx.volume_residual(self.info, volume, I, self.laser, self.sub_idx)
+        """Forward-model residual on every 16th scan point (fbp_volume_residual):
+        the voxels below 30% of the volume's range (surfaces are the minima
+        of the LoG-filtered, normalised volume) are rendered and correlated
+        with the measured rows, both blurred over +-4 bins."""
+        return self.ctx.volume_residual(self.info, volume, I, self.laser, self.sub_idx,
+                                        threshold_fraction=0.3, blur=4, polarity=-1)
 
 
 def rel_l2(a: torch.Tensor, b: torch.Tensor) -> float:
@@ -130,13 +145,46 @@ def train(out_dir: Path, scenes: int = 4000, held_out: int = 256, steps: int = 3
     torch.cuda.synchronize()
     params = net.save()
     # held-out evaluation and residual calibration
+    train_seconds = time.time() - t0
     stats = evaluate(p, net, range(1_000_000, 1_000_000 + held_out), Xte, Yte)
+    stats["residual_flat_noise"] = flat_noise_residual(p, net)
     out_dir.mkdir(parents=True, exist_ok=True)
     params.tofile(out_dir / "refiner_c5.f32")
     meta = {"size": SIZE, "scan": SCAN, "bins": BINS, "sample_fraction": SAMPLE_FRACTION,
             "scenes": scenes, "held_out": held_out, "steps": steps, "batch": batch, "lr": lr,
-            "train_seconds": time.time() - t0, "loss_history": hist, **stats}
+            "train_seconds": train_seconds, "loss_history": hist, **stats}
     (out_dir / "refiner_c5.json").write_text(json.dumps(meta, indent=1))
+    net.close()
+    ctx.close()
+    return meta
+
+
+def flat_noise_residual(p: Pipeline, net: Refiner) -> float:
+    """Residual of an out-of-distribution capture (flat Poisson noise, mean 3
+    counts); it should lie above the calibrated threshold."""
+    gen = torch.Generator(device=p.dev)
+    gen.manual_seed(5)
+    I = torch.poisson(torch.full((SCAN * SCAN, BINS), 3.0, device=p.dev), generator=gen)
+    I = I.contiguous()
+    y = torch.empty((SIZE,) * 3, dtype=torch.float32, device=p.dev)
+    net.forward(p.mis_input(I, 0), out=y)
+    return p.residual(y, I)
+
+
+def calibrate(weights: Path, held_out: int = 256, log=print) -> dict:
+    """Re-evaluates committed weights on the held-out seeds and rewrites the
+    residual statistics and threshold in their metadata file."""
+    ctx = Context(0)
+    p = Pipeline(ctx, geometry())
+    net = Refiner(ctx, SIZE)
+    net.load(np.fromfile(weights, np.float32))
+    stats = evaluate(p, net, range(1_000_000, 1_000_000 + held_out))
+    stats["residual_flat_noise"] = flat_noise_residual(p, net)
+    meta_path = weights.with_suffix(".json")
+    meta = json.loads(meta_path.read_text()) if meta_path.exists() else {}
+    meta.update(stats)
+    meta_path.write_text(json.dumps(meta, indent=1))
+    log(f"calibrated: {stats}")
     net.close()
     ctx.close()
     return meta
@@ -146,9 +194,8 @@ def evaluate(p: Pipeline, net: Refiner, seeds, X=None, Y=None) -> dict:
     """Held-out rel-L2 of the learned output and of the un-learned method
     (MIS input + LoG + normalisation) against the plain target, and the
     residual distribution that calibrates the fallback threshold."""
-    learned, baseline, res = [], [], []
+    learned, baseline, res, res_plain = [], [], [], []
     out = torch.empty((SIZE,) * 3, dtype=torch.float32, device=p.dev)
-    tmp = torch.empty_like(out)
     for i, s in enumerate(seeds):
         if X is None:
             I = p.render(int(s))
@@ -159,15 +206,16 @@ def evaluate(p: Pipeline, net: Refiner, seeds, X=None, Y=None) -> dict:
             x, y = X[i], Y[i]
         net.forward(x, out=out)
         learned.append(rel_l2(out, y))
-        p.ctx.log(x.contiguous(), 5, 1.0, out=tmp)
-        p.ctx.normalize(tmp)
-        baseline.append(rel_l2(tmp, y))
+        baseline.append(rel_l2(x, y))
         res.append(p.residual(out, I))
+        res_plain.append(p.residual(y, I))
     res = np.array(res)
     return {"rel_l2_learned_mean": float(np.mean(learned)),
             "rel_l2_learned_median": float(np.median(learned)),
             "rel_l2_without_refiner_mean": float(np.mean(baseline)),
+            "residual_median": float(np.median(res)),
             "residual_p95": float(np.percentile(res, 95)),
+            "residual_plain_median": float(np.median(res_plain)),
             "residual_threshold": float(min(0.999, np.percentile(res, 99) * 1.05))}
 
 
@@ -185,10 +233,11 @@ class LearnedReconstructor:
         self.net = Refiner(ctx, SIZE)
         self.net.load(np.fromfile(weights, np.float32))
         self.out = torch.empty((SIZE,) * 3, dtype=torch.float32, device=self.p.dev)
+        self.x = torch.empty_like(self.out)
         self.fallbacks = 0
 
     def __call__(self, I: torch.Tensor, seed: int = 0, check: bool = True):
-        x = self.p.mis_input(I, seed)
+        x = self.p.mis_input(I, seed, out=self.x)
         self.net.forward(x, out=self.out)
         if check:
             r = self.p.residual(self.out, I)
@@ -201,7 +250,8 @@ class LearnedReconstructor:
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("cmd", choices=["train"])
+    ap.add_argument("cmd", choices=["train", "calibrate"])
+    ap.add_argument("--weights", type=Path, default=WEIGHTS / "refiner_c5.f32")
     ap.add_argument("--out", type=Path, default=WEIGHTS)
     ap.add_argument("--scenes", type=int, default=4000)
     ap.add_argument("--held-out", type=int, default=256)
@@ -209,6 +259,10 @@ def main():
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--lr", type=float, default=1e-3)
     a = ap.parse_args()
+    if a.cmd == "calibrate":
+        meta = calibrate(a.weights, a.held_out, log=lambda m: print(f"[learned] {m}", flush=True))
+        print(json.dumps({k: v for k, v in meta.items() if k != "loss_history"}))
+        return
     meta = train(a.out, a.scenes, a.held_out, a.steps, a.batch, a.lr,
                  log=lambda m: print(f"[learned] {m}", flush=True))
     print(json.dumps({k: v for k, v in meta.items() if k != "loss_history"}))

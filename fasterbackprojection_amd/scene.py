@@ -204,3 +204,16 @@ def full_confocal_geometry(n_scan: int = 128, T: int = 1024, res: int = 128) -> 
     delta = C_LIGHT * 16e-12
     return confocal_geometry(n_scan, 2.0, T, delta, (-1.0, 0.25, -1.0), (1.0, 1.75, 1.0),
                              (res,) * 3)
+
+
+def exhaustive_geometry(n_lasers: int = 16, n_sensors: int = 64, T: int = 1024,
+                        res: int = 64) -> Geometry:
+    """Exhaustive (non-confocal) capture: n_lasers^2 laser targets and
+    n_sensors^2 sensor targets on a 2.0 m wall, 1024 bins of 16 ps
+    (4.8 mm); hidden volume [-0.8,0.8] x [0.4,1.6] x [-0.8,0.8] holds the
+    random spheres of random_spheres (depth 0.6-1.4, x/z within 0.7 m)."""
+    delta = C_LIGHT * 16e-12
+    return Geometry(laser_targets=wall_grid(n_lasers, 2.0), sensor_targets=wall_grid(n_sensors, 2.0),
+                    num_time_bins=T, time_step=delta, hidden_min=(-0.8, 0.4, -0.8),
+                    hidden_max=(0.8, 1.6, 0.8), voxel_resolution=(res,) * 3, confocal=False,
+                    wall_size=(2.0, 2.0))

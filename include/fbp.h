@@ -107,6 +107,20 @@ int fbp_context_device(fbp_context ctx);
 /* Bytes of scratch the context currently holds on its device. */
 size_t fbp_context_scratch_bytes(fbp_context ctx);
 
+/* Options of a context (no reference counterpart). */
+typedef enum fbp_option {
+    /* 1 (default): fbp_backproject_confocal_voxel may take the aligned-grid
+     * path (same results; see that function); 0: always the general kernel. */
+    FBP_OPTION_CONFOCAL_ALIGNED = 0
+} fbp_option;
+fbp_status fbp_context_set_option(fbp_context ctx, int32_t option, int32_t value);
+fbp_status fbp_context_get_option(fbp_context ctx, int32_t option, int32_t* value);
+
+/* Which path the last fbp_backproject_confocal_voxel call on this context
+ * took: *aligned = 1 for the aligned-grid path, 0 for the general kernel.
+ * Synchronises the device. */
+fbp_status fbp_context_last_confocal_path(fbp_context ctx, int32_t* aligned);
+
 /* --------------------------------------------------------- backprojection */
 
 /* Replaces backprojectConfocalVoxel + Laser::reconstructAABBConfocal
@@ -241,17 +255,30 @@ fbp_status fbp_render_confocal(fbp_context ctx, const fbp_vec3* scan_points, uin
                                uint32_t num_time_bins, float delta, float* transient,
                                void* stream);
 
+/* Exhaustive transient of Lambertian points: I[l][s][round((|l-p| + |p-s|) /
+ * delta)] += albedo / (|l-p|^2 |p-s|^2).  transient: float[L][S][T],
+ * OVERWRITTEN.  No reference counterpart. */
+fbp_status fbp_render_exhaustive(fbp_context ctx, const fbp_vec3* laser_points, uint32_t num_lasers,
+                                 const fbp_vec3* sensor_points, uint32_t num_sensors,
+                                 const fbp_vec3* points, const float* albedo, uint32_t num_points,
+                                 uint32_t num_time_bins, float delta, float* transient,
+                                 void* stream);
+
 /* Forward-model residual of a reconstructed volume (learned form's failure
- * check, no reference counterpart): the voxels above
- * min + threshold_fraction * (max - min) are rendered as points (albedo =
- * excess over the threshold, 1/r^4, bin round(2r/Delta)) at the scan points
+ * check, no reference counterpart): with polarity +1 the voxels above
+ * thr = min + threshold_fraction * (max - min) are rendered as points
+ * (albedo = excess over thr), with polarity -1 the voxels below it (albedo =
+ * thr - v: surfaces are minima of a LoG-filtered volume); 1/r^4, bin
+ * round(2r/Delta), at the scan points
  * rows[0..num_rows); both that and the measured rows are box-blurred over
- * +-blur bins; host_residual = 1 - cosine similarity.  Synchronises. */
+ * +-blur bins; host_residual = 1 - Pearson correlation of the rendered and
+ * measured rows (each row centred on its own mean, second moments pooled
+ * over the rows).  1 when either side is constant.  Synchronises. */
 fbp_status fbp_volume_residual(fbp_context ctx, const fbp_reconstruction_info* info,
                                const float* volume, const float* transient,
                                const fbp_vec3* laser_targets, const uint32_t* rows,
-                               uint32_t num_rows, float threshold_fraction, int32_t blur,
-                               double* host_residual, void* stream);
+                               uint32_t num_rows, float threshold_fraction, int32_t polarity,
+                               int32_t blur, double* host_residual, void* stream);
 
 /* ---------------------------------------------------------- learned form */
 

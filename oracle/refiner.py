@@ -83,7 +83,7 @@ def forward(p, vol, emulate_bf16=False):
     d2 = q(relu(conv(u2, *mw(3))) + e2)
     u1 = up(d2)
     d1 = q(relu(conv(u1, *mw(4))) + e1)
-    return conv(d1, *L[5])[..., 0]
+    return conv(d1, *L[5])[..., 0] + x[..., 0]
 
 
 def box(a, r=2):

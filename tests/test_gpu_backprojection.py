@@ -115,6 +115,93 @@ def test_confocal_deterministic(ctx):
     assert torch.equal(a, b)
 
 
+# ------------------------------------------------------------ aligned grids
+def aligned_geom(nx, wall, X, VY, ZD, T, depth0, dpitch, t0=0.0, discard=1):
+    """Scan grid and voxel lattice coincide along world x and z (pitch
+    wall / nx, both starting at -wall / 2), so the aligned-grid path applies.
+    Voxel sizes follow the reference's size / resolution per component:
+    world x <- X, world y (depth) <- size.y / VY, world z <- size.z / ZD."""
+    pitch = wall / nx
+    scan = wall_grid(nx, wall)
+    hmin = (-wall / 2, depth0, -wall / 2)
+    hmax = (-wall / 2 + X * pitch, depth0 + VY * dpitch, -wall / 2 + ZD * pitch)
+    return Geometry(laser_targets=scan, sensor_targets=scan, num_time_bins=T, time_step=0.0096,
+                    hidden_min=hmin, hidden_max=hmax, voxel_resolution=(X, VY, ZD),
+                    discard_first_last_bounces=discard, time_offset=t0,
+                    laser_position=(0.1, -0.8, 0.2), sensor_position=(-0.2, -0.7, 0.1),
+                    wall_size=(wall, wall))
+
+
+ALIGNED = {
+    "64": dict(nx=64, wall=1.0, X=64, VY=16, ZD=8, T=256, depth0=0.25, dpitch=0.02),
+    "64_ragged_depth": dict(nx=64, wall=1.0, X=64, VY=32, ZD=5, T=256, depth0=0.05, dpitch=0.02),
+    "128": dict(nx=128, wall=2.0, X=128, VY=16, ZD=4, T=512, depth0=0.5, dpitch=0.03),
+}
+
+
+@pytest.mark.parametrize("name", sorted(ALIGNED))
+def test_confocal_aligned_path_parity(ctx, name):
+    g = aligned_geom(**ALIGNED[name])
+    info = make_info(g)
+    rng = np.random.default_rng(31)
+    I = spiky(rng, (len(g.laser_targets), g.num_time_bins))
+    ref = oracle.confocal_voxel(info, I, g.laser_targets)
+    out = ctx.backproject_confocal(info, cuda(I), cuda(g.laser_targets))
+    assert ctx.last_confocal_path_aligned()
+    assert_volume_close(out, ref)
+    again = ctx.backproject_confocal(info, cuda(I), cuda(g.laser_targets))
+    torch.cuda.synchronize()
+    assert torch.equal(out, again)                   # deterministic
+    ctx.set_aligned_path(False)
+    try:
+        gen = ctx.backproject_confocal(info, cuda(I), cuda(g.laser_targets))
+        assert not ctx.last_confocal_path_aligned()
+    finally:
+        ctx.set_aligned_path(True)
+    assert_volume_close(gen, ref)
+
+
+def test_confocal_aligned_time_major(ctx):
+    g = aligned_geom(**ALIGNED["64"])
+    info = make_info(g)
+    rng = np.random.default_rng(32)
+    I = spiky(rng, (len(g.laser_targets), g.num_time_bins))
+    ref = oracle.confocal_voxel(info, I, g.laser_targets)
+    out = ctx.backproject_confocal(info, cuda(I.T), cuda(g.laser_targets),
+                                   layout=LAYOUT_TIME_MAJOR)
+    assert ctx.last_confocal_path_aligned()
+    assert_volume_close(out, ref)
+
+
+def test_confocal_aligned_check_rejects_perturbed_grid(ctx):
+    """One scan point moved by one ulp breaks translation invariance: the
+    device-side check must send the call to the general kernel."""
+    g = aligned_geom(**ALIGNED["64"])
+    scan = g.laser_targets.copy()
+    scan[777, 0] = np.nextafter(scan[777, 0], np.float32(1.0))
+    g.laser_targets = scan
+    info = make_info(g)
+    rng = np.random.default_rng(33)
+    I = spiky(rng, (len(scan), g.num_time_bins))
+    ref = oracle.confocal_voxel(info, I, scan)
+    out = ctx.backproject_confocal(info, cuda(I), cuda(scan))
+    assert not ctx.last_confocal_path_aligned()
+    assert_volume_close(out, ref)
+
+
+def test_confocal_aligned_geometry_general_chain(ctx):
+    """t0 != 0 makes bins depend on more than the squared distance: the
+    aligned path does not apply (host-side decision)."""
+    g = aligned_geom(**ALIGNED["64"], t0=0.05)
+    info = make_info(g)
+    rng = np.random.default_rng(34)
+    I = spiky(rng, (len(g.laser_targets), g.num_time_bins))
+    ref = oracle.confocal_voxel(info, I, g.laser_targets)
+    out = ctx.backproject_confocal(info, cuda(I), cuda(g.laser_targets))
+    assert not ctx.last_confocal_path_aligned()
+    assert_volume_close(out, ref)
+
+
 @pytest.mark.parametrize("storage,tol", [(STORAGE_F16, 2e-3), (STORAGE_BF16, 8e-3)])
 def test_confocal_quantized_storage(ctx, storage, tol):
     """Precision form: FP16/BF16 with per-scan-point max scale."""

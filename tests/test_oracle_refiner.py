@@ -54,3 +54,11 @@ def test_forward_shapes_and_normalisation():
     b = orf.forward(p, 7.5 * v)                 # input is max-normalised
     assert a.shape == (16, 16, 16)
     np.testing.assert_allclose(a, b, rtol=1e-10, atol=1e-12)
+
+
+def test_zero_network_is_identity_on_normalised_input():
+    """y = net(x) + x: with every parameter zero the output is the input
+    divided by its largest magnitude."""
+    v = np.random.default_rng(6).random((16, 16, 16)) * 4 - 3
+    y = orf.forward(np.zeros(orf.param_count()), v)
+    np.testing.assert_allclose(y, v / np.abs(v).max(), rtol=0, atol=1e-15)
