@@ -31,6 +31,8 @@
 // result is deterministic.  Whether this path or the general kernel runs is
 // decided on the device by the check kernel's flag: both paths are
 // launched, each returns at once unless the flag selects it.
+#include <cstdlib>
+
 #include "bp_common.h"
 #include "bp_confocal_aligned.h"
 
@@ -70,19 +72,53 @@ __device__ __forceinline__ float z2_ref(const AlignedParams& a, const fbp_vec3* 
                     : sq(rn_sub(pz_of(a, 0), laser[size_t(m) * a.nx].z));
 }
 
-// flag = 1 iff the grid conditions hold for every scan point and pair.
+// Device-resident cache header of the offset tables: the parameters and
+// scan points they were built for, and whether the build completed.
+struct TableCache {
+    AlignedParams key;
+    int valid;
+};
+
+__device__ __forceinline__ bool same_params(const AlignedParams& x, const AlignedParams& y) {
+    bool eq = x.nx == y.nx && x.X == y.X && x.VY == y.VY && x.ZD == y.ZD && x.kq == y.kq &&
+              x.mq == y.mq && x.T == y.T && x.discard == y.discard;
+    // Bit patterns: -0 / +0 and NaNs must not compare equal by value.
+    eq = eq && __float_as_uint(x.t0) == __float_as_uint(y.t0) &&
+         __float_as_uint(x.dt) == __float_as_uint(y.dt);
+    for (int i = 0; i < 3; ++i)
+        eq = eq && __float_as_uint(x.hmin[i]) == __float_as_uint(y.hmin[i]) &&
+             __float_as_uint(x.vs[i]) == __float_as_uint(y.vs[i]);
+    return eq;
+}
+
+// flag[0] = 1 iff the grid conditions hold for every scan point and pair.
+// flag[1] = 1 iff the cached tables were built for these parameters and
+// exactly these scan points (then the table kernel is skipped); when
+// flag[0] = 1 and flag[1] = 0 the cache is marked invalid until rebuilt.
 __global__ __launch_bounds__(1024) void aligned_check_kernel(const fbp_vec3* __restrict__ laser,
-                                                             AlignedParams a, int* __restrict__ flag) {
-    __shared__ int ok;
-    if (threadIdx.x == 0) ok = 1;
+                                                             AlignedParams a, int* __restrict__ flag,
+                                                             TableCache* __restrict__ cache,
+                                                             const fbp_vec3* __restrict__ cached_laser) {
+    __shared__ int ok, same;
+    if (threadIdx.x == 0) {
+        ok = 1;
+        same = cache->valid == 1 && same_params(cache->key, a);
+    }
     __syncthreads();
-    bool good = true;
+    bool good = true, eq = true;
     const uint32_t L = a.nx * a.nx;
     const float y0 = laser[0].y;
+    const bool cmp = same != 0;
     for (uint32_t l = threadIdx.x; l < L; l += blockDim.x) {
         const uint32_t ix = l % a.nx, iz = l / a.nx;
         const fbp_vec3 p = laser[l];
         good = good && p.y == y0 && p.x == laser[ix].x && p.z == laser[size_t(iz) * a.nx].z;
+        if (cmp) {
+            const fbp_vec3 c = cached_laser[l];
+            eq = eq && __float_as_uint(p.x) == __float_as_uint(c.x) &&
+                 __float_as_uint(p.y) == __float_as_uint(c.y) &&
+                 __float_as_uint(p.z) == __float_as_uint(c.z);
+        }
     }
     for (uint32_t q = threadIdx.x; q < a.X * a.nx; q += blockDim.x) {
         const uint32_t vx = q % a.X, ix = q / a.X;
@@ -95,8 +131,13 @@ __global__ __launch_bounds__(1024) void aligned_check_kernel(const fbp_vec3* __r
         good = good && v == z2_ref(a, laser, vy > iz ? vy - iz : iz - vy);
     }
     if (!good) atomicAnd(&ok, 0);
+    if (!eq) atomicAnd(&same, 0);
     __syncthreads();
-    if (threadIdx.x == 0) *flag = ok;
+    if (threadIdx.x == 0) {
+        flag[0] = ok;
+        flag[1] = ok && same;
+        if (ok && !same) cache->valid = 0;
+    }
 }
 
 // One workgroup per depth index z: bins and weights of the quadrant
@@ -109,8 +150,14 @@ __global__ __launch_bounds__(1024) void aligned_table_kernel(const fbp_vec3* __r
                                                              AlignedParams a, const int* __restrict__ flag,
                                                              uint32_t sort_n, uint2* __restrict__ entries,
                                                              uint32_t* __restrict__ starts,
-                                                             uint32_t* __restrict__ work) {
-    if (*flag == 0) return;
+                                                             uint32_t* __restrict__ work,
+                                                             TableCache* __restrict__ cache,
+                                                             fbp_vec3* __restrict__ cached_laser) {
+    if (flag[0] == 0 || flag[1] == 1) return;
+    if (blockIdx.x == 0) {
+        for (uint32_t l = threadIdx.x; l < a.nx * a.nx; l += blockDim.x) cached_laser[l] = laser[l];
+        if (threadIdx.x == 0) cache->key = a;
+    }
     extern __shared__ uint32_t keys[];  // sort_n (power of two), then sort_n work sums
     uint32_t* wsum = keys + sort_n;
     __shared__ float X2[128], Z2[128];
@@ -306,7 +353,7 @@ __global__ __launch_bounds__(1024) void aligned_conv_kernel(ConvArgs a,
                                                             const uint2* __restrict__ entries,
                                                             const uint32_t* __restrict__ starts,
                                                             const uint32_t* __restrict__ work) {
-    static_assert(ZB == 1 || ZB == 2, "one or two depth planes per workgroup");
+    static_assert(ZB == 1 || ZB == 2 || ZB == 4, "1, 2 or 4 depth planes per workgroup");
     if (*a.flag == 0) return;
     extern __shared__ float plane[];
     const uint32_t tid = threadIdx.x;
@@ -314,7 +361,11 @@ __global__ __launch_bounds__(1024) void aligned_conv_kernel(ConvArgs a,
     const int wave = __builtin_amdgcn_readfirstlane(int(tid >> 6));
     const int lane = int(tid & 63);
     const int lx = lane & 7, ly = lane >> 3;
-    const uint32_t group = blockIdx.x / a.chunks, chunk = blockIdx.x % a.chunks;
+    // Chunk-major order: the workgroups resident at one time work on the
+    // same share of every depth's bins, so the planes they stage overlap
+    // (L2 reuse across neighbouring depths).
+    const uint32_t groups = a.ZD / ZB;
+    const uint32_t chunk = blockIdx.x / groups, group = blockIdx.x % groups;
     const uint32_t z0 = group * ZB;
     const int nx = int(a.nx);
     const int stride = int(a.stride);
@@ -362,10 +413,9 @@ __global__ __launch_bounds__(1024) void aligned_conv_kernel(ConvArgs a,
         return b;
     };
 
-    constexpr int TXC_ = TXC;
-    float acc0[TXC_], acc1[TXC_];
+    float acc0[TXC], acc1[TXC], acc2[TXC], acc3[TXC];
 #pragma unroll
-    for (int c = 0; c < TXC; ++c) acc0[c] = acc1[c] = 0.f;
+    for (int c = 0; c < TXC; ++c) acc0[c] = acc1[c] = acc2[c] = acc3[c] = 0.f;
 
     float4 stage[kStageVec];
     auto load_plane = [&](uint32_t b) {
@@ -391,11 +441,8 @@ __global__ __launch_bounds__(1024) void aligned_conv_kernel(ConvArgs a,
 
     const int y0 = 8 * wave;
     const int lane_row = (kPadRows + y0 + ly) * stride + kPadCols + lx;
-    const uint2* __restrict__ ent0 = entries + size_t(z0) * a.n;
-    const uint2* __restrict__ ent1 = entries + size_t(z0 + (ZB - 1)) * a.n;
-    const uint32_t* __restrict__ st0 = starts + size_t(z0) * srow;
-    const uint32_t* __restrict__ st1 = starts + size_t(z0 + (ZB - 1)) * srow;
-
+    auto ent_of = [&](int j) { return entries + size_t(z0 + j) * a.n; };
+    auto st_of = [&](int j) { return starts + size_t(z0 + j) * srow; };
     uint32_t b = next_bin(b_begin);
     if (prefetch && b < b_end) load_plane(b);
     while (b < b_end) {
@@ -413,25 +460,41 @@ __global__ __launch_bounds__(1024) void aligned_conv_kernel(ConvArgs a,
         __syncthreads();
         const uint32_t bn = next_bin(b + 1);
         if (prefetch && bn < b_end) load_plane(bn);
-        apply_plane<TXC>(acc0, plane, ent0, st0 + 4 * b, lane_row, y0, nx, stride);
-        if constexpr (ZB == 2) apply_plane<TXC>(acc1, plane, ent1, st1 + 4 * b, lane_row, y0, nx, stride);
+        apply_plane<TXC>(acc0, plane, ent_of(0), st_of(0) + 4 * b, lane_row, y0, nx, stride);
+        if constexpr (ZB >= 2)
+            apply_plane<TXC>(acc1, plane, ent_of(1), st_of(1) + 4 * b, lane_row, y0, nx, stride);
+        if constexpr (ZB == 4) {
+            apply_plane<TXC>(acc2, plane, ent_of(2), st_of(2) + 4 * b, lane_row, y0, nx, stride);
+            apply_plane<TXC>(acc3, plane, ent_of(3), st_of(3) + 4 * b, lane_row, y0, nx, stride);
+        }
         b = bn;
     }
 
     // Partial sums of this chunk for the ZB depth planes.
     float* out0 = a.partial + ((size_t(chunk) * a.ZD + z0) * a.VY + y0 + ly) * a.X + lx;
+    const size_t zs = size_t(a.VY) * a.X;
 #pragma unroll
-    for (int c = 0; c < TXC; ++c) out0[8 * c] = acc0[c];
-    if constexpr (ZB == 2) {
-        float* out1 = out0 + size_t(a.VY) * a.X;
-#pragma unroll
-        for (int c = 0; c < TXC; ++c) out1[8 * c] = acc1[c];
+    for (int c = 0; c < TXC; ++c) {
+        out0[8 * c] = acc0[c];
+        if constexpr (ZB >= 2) out0[zs + 8 * c] = acc1[c];
+        if constexpr (ZB == 4) {
+            out0[2 * zs + 8 * c] = acc2[c];
+            out0[3 * zs + 8 * c] = acc3[c];
+        }
     }
 }
 
+__global__ void cache_reset_kernel(TableCache* cache) {
+    if (threadIdx.x == 0) cache->valid = 0;
+}
+
 __global__ void aligned_reduce_kernel(const float* __restrict__ partial, float* __restrict__ volume,
-                                      size_t nvox, uint32_t chunks, const int* __restrict__ flag) {
+                                      size_t nvox, uint32_t chunks, const int* __restrict__ flag,
+                                      TableCache* __restrict__ cache) {
     if (*flag == 0) return;
+    // The tables (cached or rebuilt by the table kernel, which has finished)
+    // now match the key recorded in the cache header.
+    if (blockIdx.x == 0 && threadIdx.x == 0) cache->valid = 1;
     const size_t v = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (v >= nvox) return;
     double s = 0.0;
@@ -480,11 +543,16 @@ AlignedPlan plan_aligned_confocal(fbp_context ctx, const fbp_reconstruction_info
     p.n = p.kq * p.mq;
     p.sort_n = pow2_at_least(p.n);
     if (size_t(p.sort_n) * 8 + 6144 > size_t(ctx->max_lds_per_block)) return p;
-    p.zb = (ZD % 2 == 0) ? 2 : 1;
+    p.zb = 1;
     const uint32_t groups = ZD / p.zb;
-    // ~4 workgroups per CU over the whole launch (one resident per CU).
-    const uint32_t target = 4u * uint32_t(ctx->num_cus > 0 ? ctx->num_cus : 256);
+    // ~8 workgroups per CU over the whole launch (one resident per CU).
+    const uint32_t target = 8u * uint32_t(ctx->num_cus > 0 ? ctx->num_cus : 256);
     p.chunks = std::max(1u, std::min(64u, (target + groups - 1) / groups));
+#ifdef FBP_TUNING_ENV
+    if (const char* e = std::getenv("FBP_ALIGNED_ZB")) p.zb = uint32_t(std::atoi(e));
+    if (const char* e = std::getenv("FBP_ALIGNED_CHUNKS")) p.chunks = uint32_t(std::atoi(e));
+    if (ZD % p.zb) p.zb = 1;
+#endif
     const size_t nvox = size_t(X) * VY * ZD;
     p.bytes_entries = size_t(ZD) * p.n * sizeof(uint2);
     p.bytes_starts = size_t(ZD) * (4 * size_t(info.num_time_bins) + 1) * sizeof(uint32_t);
@@ -498,8 +566,7 @@ AlignedPlan plan_aligned_confocal(fbp_context ctx, const fbp_reconstruction_info
 static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 size_t aligned_scratch_bytes(const AlignedPlan& p) {
-    return align256(p.bytes_entries) + align256(p.bytes_starts) + align256(p.bytes_work) +
-           align256(p.bytes_tm) + align256(p.bytes_partial);
+    return align256(p.bytes_tm) + align256(p.bytes_partial);
 }
 
 template <int TXC, int ZB>
@@ -535,24 +602,49 @@ fbp_status launch_aligned_confocal(fbp_context ctx, const AlignedPlan& p,
     a.vs[1] = info.hidden_volume_voxel_size.y;
     a.vs[2] = info.hidden_volume_voxel_size.z;
 
+    // Offset tables live in their own allocation (kept across calls, rebuilt
+    // only when the parameters or the scan points change); the transposed
+    // transient and the partial sums use the shared scratch.
+    const size_t header = align256(sizeof(TableCache)) + align256(size_t(16384) * sizeof(fbp_vec3));
+    const size_t table_bytes = header + align256(p.bytes_entries) + align256(p.bytes_starts) +
+                               align256(p.bytes_work);
+    if (ctx->aligned_tables_bytes < table_bytes) {
+        if (ctx->aligned_tables) {
+            FBP_CHECK_HIP(ctx, hipDeviceSynchronize());
+            FBP_CHECK_HIP(ctx, hipFree(ctx->aligned_tables));
+            ctx->aligned_tables = nullptr;
+            ctx->aligned_tables_bytes = 0;
+        }
+        FBP_CHECK_HIP(ctx, hipMalloc(&ctx->aligned_tables, table_bytes));
+        ctx->aligned_tables_bytes = table_bytes;
+        // A fresh cache is invalid until a build completes.
+        hipLaunchKernelGGL(cache_reset_kernel, dim3(1), dim3(64), 0, s,
+                           static_cast<TableCache*>(ctx->aligned_tables));
+        FBP_CHECK_HIP(ctx, hipGetLastError());
+    }
+    char* tcur = static_cast<char*>(ctx->aligned_tables);
+    TableCache* cache = reinterpret_cast<TableCache*>(tcur);
+    fbp_vec3* cached_laser = reinterpret_cast<fbp_vec3*>(tcur + align256(sizeof(TableCache)));
+    tcur += header;
+    uint2* entries = reinterpret_cast<uint2*>(tcur);
+    tcur += align256(p.bytes_entries);
+    uint32_t* starts = reinterpret_cast<uint32_t*>(tcur);
+    tcur += align256(p.bytes_starts);
+    uint32_t* work = reinterpret_cast<uint32_t*>(tcur);
+
     char* cur = static_cast<char*>(base);
-    uint2* entries = reinterpret_cast<uint2*>(cur);
-    cur += align256(p.bytes_entries);
-    uint32_t* starts = reinterpret_cast<uint32_t*>(cur);
-    cur += align256(p.bytes_starts);
-    uint32_t* work = reinterpret_cast<uint32_t*>(cur);
-    cur += align256(p.bytes_work);
     float* tm = reinterpret_cast<float*>(cur);
     cur += align256(p.bytes_tm);
     float* partial = reinterpret_cast<float*>(cur);
 
-    hipLaunchKernelGGL(aligned_check_kernel, dim3(1), dim3(1024), 0, s, laser, a, flag);
+    hipLaunchKernelGGL(aligned_check_kernel, dim3(1), dim3(1024), 0, s, laser, a, flag, cache,
+                       cached_laser);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     FBP_CHECK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(aligned_table_kernel),
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            int(p.sort_n * 8)));
     hipLaunchKernelGGL(aligned_table_kernel, dim3(a.ZD), dim3(1024), size_t(p.sort_n) * 8, s, laser,
-                       a, flag, p.sort_n, entries, starts, work);
+                       a, flag, p.sort_n, entries, starts, work, cache, cached_laser);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     const uint32_t L = p.nx * p.nx;
     const float* src = static_cast<const float*>(tr.data);
@@ -581,15 +673,17 @@ fbp_status launch_aligned_confocal(fbp_context ctx, const AlignedPlan& p,
     const unsigned threads = 64u * (a.VY / 8u);
     fbp_status st;
     if (a.X == 128)
-        st = p.zb == 2 ? launch_conv<16, 2>(ctx, c, blocks, threads, p.lds_bytes, s)
-                       : launch_conv<16, 1>(ctx, c, blocks, threads, p.lds_bytes, s);
+        st = p.zb == 4   ? launch_conv<16, 4>(ctx, c, blocks, threads, p.lds_bytes, s)
+             : p.zb == 2 ? launch_conv<16, 2>(ctx, c, blocks, threads, p.lds_bytes, s)
+                         : launch_conv<16, 1>(ctx, c, blocks, threads, p.lds_bytes, s);
     else
-        st = p.zb == 2 ? launch_conv<8, 2>(ctx, c, blocks, threads, p.lds_bytes, s)
-                       : launch_conv<8, 1>(ctx, c, blocks, threads, p.lds_bytes, s);
+        st = p.zb == 4   ? launch_conv<8, 4>(ctx, c, blocks, threads, p.lds_bytes, s)
+             : p.zb == 2 ? launch_conv<8, 2>(ctx, c, blocks, threads, p.lds_bytes, s)
+                         : launch_conv<8, 1>(ctx, c, blocks, threads, p.lds_bytes, s);
     if (st != FBP_OK) return st;
     const size_t nvox = size_t(a.X) * a.VY * a.ZD;
     hipLaunchKernelGGL(aligned_reduce_kernel, dim3(unsigned((nvox + 255) / 256)), dim3(256), 0, s,
-                       partial, volume, nvox, p.chunks, flag);
+                       partial, volume, nvox, p.chunks, flag, cache);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     return FBP_OK;
 }

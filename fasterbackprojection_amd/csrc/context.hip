@@ -157,6 +157,7 @@ fbp_status fbp_context_destroy(fbp_context ctx) {
     if (ctx->scratch) hipFree(ctx->scratch);
     if (ctx->scratch2) hipFree(ctx->scratch2);
     if (ctx->path_flag) hipFree(ctx->path_flag);
+    if (ctx->aligned_tables) hipFree(ctx->aligned_tables);
     if (ctx->pinned) hipHostFree(ctx->pinned);
     delete ctx;
     return FBP_OK;

@@ -19,6 +19,9 @@ struct fbp_context_s {
     // Device int: path flag of the last confocal voxel call (1 = aligned
     // path), valid when path_flag_valid is set.
     int* path_flag = nullptr;
+    // Offset tables of the aligned confocal path, kept across calls.
+    void* aligned_tables = nullptr;
+    size_t aligned_tables_bytes = 0;
     bool path_flag_valid = false;
     std::string last_error;
     // Scratch arena: grown on demand, never shrunk.  Used for per-chunk

@@ -161,6 +161,29 @@ def test_confocal_aligned_path_parity(ctx, name):
     assert_volume_close(gen, ref)
 
 
+def test_confocal_aligned_table_cache(ctx):
+    """The offset tables are kept between calls: switching between
+    geometries (different depth range; same parameters but the wall moved
+    in depth; a perturbed grid that takes the general kernel) and back must
+    rebuild or reuse them correctly."""
+    ga = aligned_geom(**ALIGNED["64"])
+    gb = aligned_geom(**dict(ALIGNED["64"], depth0=0.4))
+    gc = aligned_geom(**ALIGNED["64"])
+    gc.laser_targets = gc.laser_targets.copy()
+    gc.laser_targets[:, 1] = np.float32(-0.05)           # wall plane moved
+    gd = aligned_geom(**ALIGNED["64"])
+    gd.laser_targets = gd.laser_targets.copy()
+    gd.laser_targets[5, 2] = np.nextafter(gd.laser_targets[5, 2], np.float32(2.0))
+    rng = np.random.default_rng(35)
+    I = spiky(rng, (64 * 64, 256))
+    for g, aligned in [(ga, True), (ga, True), (gb, True), (ga, True), (gc, True), (gd, False),
+                       (ga, True)]:
+        info = make_info(g)
+        out = ctx.backproject_confocal(info, cuda(I), cuda(g.laser_targets))
+        assert ctx.last_confocal_path_aligned() == aligned
+        assert_volume_close(out, oracle.confocal_voxel(info, I, g.laser_targets))
+
+
 def test_confocal_aligned_time_major(ctx):
     g = aligned_geom(**ALIGNED["64"])
     info = make_info(g)
