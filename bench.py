@@ -9,9 +9,11 @@ surface points, rendered with 1/r^4 falloff and Poisson noise at a peak of
 1000 counts; inputs are resident in HBM before timing starts.
 
 Prints ONE JSON line.  value = voxel-updates/s of the whole step
-(X*Y*Z*L updates per step).  roofline = the backprojection kernel's
-algorithmic FP32 rate (16 flops per voxel-update) against the MI355X FP32
-peak.  cpu_baseline = the FP64 OpenMP oracle on a bounded slab of the same
+(X*Y*Z*L updates per step).  roofline = the backprojection's algorithmic
+FP32 rate (16 flops per voxel-update) against the MI355X FP32 peak, plus
+the aligned-grid kernel's own bound (LDS bytes, 4 per update) and the HBM
+traffic per launch from the committed rocprofv3 counter profile.
+cpu_baseline = the FP64 OpenMP oracle on a bounded slab of the same
 workload, run on this host's cores.
 """
 from __future__ import annotations
@@ -28,6 +30,20 @@ sys.path.insert(0, str(ROOT))
 
 FLOPS_PER_UPDATE = 16          # confocal, FMA = 2 (reference formula)
 PEAK_FP32_TFLOPS = 157.3       # MI355X FP32 (datasheet)
+PEAK_LDS_TBPS = 78.6           # 256 CUs x 128 B/clk x 2.4 GHz
+TRAFFIC_PROFILE = ROOT / "profiles" / "confocal_128cube_traffic.json"
+
+
+def traffic_bytes(aligned):
+    """HBM bytes per launch of the dominant kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a plain copy)."""
+    try:
+        prof = json.loads(TRAFFIC_PROFILE.read_text())
+    except (OSError, ValueError):
+        return None
+    key = "aligned_conv_kernel" if aligned else "confocal_voxel_kernel"
+    k = prof.get("kernels", {}).get(key)
+    return None if k is None else k["hbm_bytes_per_launch"]
 METRIC = "voxel-updates/s + %roofline; learned vs exhaustive BP time at rel-L2<=2%"
 
 
@@ -46,6 +62,8 @@ def parse():
                     help="skip the FP16/BF16 storage leg (256^2 x 1024 -> 128^3)")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-vs-plain leg (64^2 x 512 -> 64^3)")
+    ap.add_argument("--copy-reference-mib", type=int, default=0,
+                    help="run a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--no-exhaustive", action="store_true",
                     help="skip the exhaustive leg (16^2 lasers x 64^2 sensors x 1024 -> 64^3)")
     return ap.parse_args()
@@ -117,16 +135,20 @@ def main():
     log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step (wall {wall:.2f} s)")
 
     # Dominant kernel: backprojection alone, events on the launch stream.
-    bp_ms = []
-    for _ in range(max(3, args.steps)):
-        a = torch.cuda.Event(enable_timing=True)
-        b = torch.cuda.Event(enable_timing=True)
-        a.record(stream)
-        ctx.backproject_confocal(info, I, laser, out=vol)
-        b.record(stream)
-        b.synchronize()
-        bp_ms.append(a.elapsed_time(b))
-    bp_avg = float(np.mean(bp_ms))
+    def time_bp(reps):
+        ms = []
+        for _ in range(reps):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            ctx.backproject_confocal(info, I, laser, out=vol)
+            b.record(stream)
+            b.synchronize()
+            ms.append(a.elapsed_time(b))
+        return float(np.mean(ms))
+
+    bp_avg = time_bp(max(3, args.steps))
+    aligned = ctx.last_confocal_path_aligned()
     f_ms = []
     for _ in range(max(3, args.steps)):
         a = torch.cuda.Event(enable_timing=True)
@@ -138,9 +160,21 @@ def main():
         b.synchronize()
         f_ms.append(a.elapsed_time(b))
     achieved = FLOPS_PER_UPDATE * updates / (bp_avg * 1e-3) / 1e12
-    log(f"backprojection {bp_avg:.3f} ms -> {achieved:.1f} TFLOP/s algorithmic; "
-        f"LoG+normalise {np.mean(f_ms):.3f} ms")
-
+    log(f"backprojection {bp_avg:.3f} ms ({'aligned-grid path' if aligned else 'general kernel'}) "
+        f"-> {achieved:.1f} TFLOP/s algorithmic; LoG+normalise {np.mean(f_ms):.3f} ms")
+    # The general kernel on the same input, for comparison.
+    ctx.set_aligned_path(False)
+    general_ms = time_bp(3)
+    ctx.set_aligned_path(True)
+    log(f"general kernel on the same input: {general_ms:.3f} ms")
+    if args.copy_reference_mib:
+        # Plain device copy of a known size: calibrates FETCH_SIZE / WRITE_SIZE
+        # when this run is profiled with rocprofv3 --pmc.
+        n = args.copy_reference_mib << 18
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        torch.empty_like(src).copy_(src)
+        torch.cuda.synchronize()
+        del src
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(info, I, g, args.cpu_seconds)
@@ -169,16 +203,22 @@ def main():
         },
         "roofline": {
             "bound": "valu",
-            "kernel": "confocal_voxel_kernel (+ FP64 chunk reduction)",
+            "kernel": ("aligned_conv_kernel (fbp_backproject_confocal_voxel, aligned-grid path)"
+                       if aligned else "confocal_voxel_kernel (fbp_backproject_confocal_voxel)"),
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": traffic_bytes(aligned),
             "flops_per_update": FLOPS_PER_UPDATE,
             "kernel_ms": bp_avg,
+            # The aligned path does one LDS read and one FMA per update, so
+            # its own binding resource is LDS bandwidth (4 B per update).
+            "lds": {"achieved": 4.0 * updates / (bp_avg * 1e-3) / 1e12, "peak": PEAK_LDS_TBPS,
+                    "unit": "TB/s", "frac": 4.0 * updates / (bp_avg * 1e-3) / 1e12 / PEAK_LDS_TBPS},
         },
-        "breakdown_ms": {"backprojection": bp_avg, "log_and_normalize": float(np.mean(f_ms))},
+        "breakdown_ms": {"backprojection": bp_avg, "log_and_normalize": float(np.mean(f_ms)),
+                         "backprojection_general_kernel": general_ms},
         "cpu_baseline": cpu,
         "precision": precision,
         "learned": learned,
