@@ -15,12 +15,12 @@ import torch
 
 from . import _lib
 from ._lib import (CONFOCAL, EXHAUSTIVE, LAYOUT_TIME_FASTEST, LAYOUT_TIME_MAJOR, MIS_REFERENCE,
-                   MIS_UNBIASED, STORAGE_BF16, STORAGE_F16, STORAGE_F32, FbpError, LctParams,
+                   MIS_TILED, MIS_UNBIASED, STORAGE_BF16, STORAGE_F16, STORAGE_F32, FbpError, LctParams,
                    MisParams, ReconstructionInfo, Transient, UVec3, Vec3)
 
 __all__ = [
     "Context", "Geometry", "make_info", "CONFOCAL", "EXHAUSTIVE", "STORAGE_F32", "STORAGE_F16",
-    "STORAGE_BF16", "LAYOUT_TIME_FASTEST", "LAYOUT_TIME_MAJOR", "MIS_REFERENCE", "MIS_UNBIASED",
+    "STORAGE_BF16", "LAYOUT_TIME_FASTEST", "LAYOUT_TIME_MAJOR", "MIS_REFERENCE", "MIS_UNBIASED", "MIS_TILED",
     "FbpError",
 ]
 

@@ -134,4 +134,13 @@ fbp_status launch_reduce(fbp_context ctx, const float* partial, const float* cor
                          const VolumeGeom& g, uint32_t num_chunks, double scale,
                          hipStream_t stream, const int* skip = nullptr);
 
+// Confocal voxel backprojection (bp_confocal.hip).  With samples != 0 every
+// 16^3 voxel tile visits its own scan-point list sample_l[tile][0..samples)
+// with weights sample_w[tile][..] (tile-sampled MIS); otherwise all scan
+// points with weight 1 (the aligned-grid path may then run).
+fbp_status confocal_voxel_impl(fbp_context ctx, const fbp_reconstruction_info* info,
+                               const fbp_transient* tr, const fbp_vec3* laser_targets,
+                               float* volume, void* stream, const uint32_t* sample_l,
+                               const float* sample_w, uint32_t samples);
+
 }  // namespace fbp

@@ -42,6 +42,12 @@ struct ConfocalArgs {
     uint32_t thr_rows;     // 1 (pure chain) or L
     float* corr;           // near-voxel corrections, same layout as partial
     const int* skip;       // when non-null and *skip != 0 the aligned path ran: return
+    // Tile-sampled mode (MIS): each voxel tile visits its own list of
+    // `items` scan points sample_l[tile][i] with weights sample_w[tile][i];
+    // otherwise items = L and scan point i is visited with weight 1.
+    uint32_t items;
+    const uint32_t* sample_l;
+    const float* sample_w;
 };
 
 // Per scan point parameters staged in LDS (8 words each).
@@ -143,7 +149,9 @@ __device__ __forceinline__ void confocal_voxel_body(const ConfocalArgs& a) {
     for (int s = 0; s < 64; ++s) corr[s * 64 + lane] = 0.f;
 
     const uint32_t l_begin = chunk * a.chunk_len;
-    const uint32_t l_end = min(a.L, l_begin + a.chunk_len);
+    const uint32_t l_end = min(a.items, l_begin + a.chunk_len);
+    const uint32_t* tile_l = a.sample_l ? a.sample_l + size_t(tile) * a.items : nullptr;
+    const float* tile_w = a.sample_l ? a.sample_w + size_t(tile) * a.items : nullptr;
     const float tscale = PURE ? 4.0f : 1.0f;
     // Bias of the pure-chain estimate: 2^-18 relative, far above its error.
     const float cb = a.inv_dt * (1.0f - 3.814697265625e-06f);
@@ -152,7 +160,7 @@ __device__ __forceinline__ void confocal_voxel_body(const ConfocalArgs& a) {
         const int nb = int(min(uint32_t(a.batch), l_end - lb));
         __syncthreads();
         if (lane < nb) {
-            const uint32_t l = lb + lane;
+            const uint32_t l = tile_l ? tile_l[lb + lane] : lb + lane;
             const float p[3] = {a.laser[l].x, a.laser[l].y, a.laser[l].z};
             float off = 0.f;
             if (a.discard == 0)
@@ -179,7 +187,8 @@ __device__ __forceinline__ void confocal_voxel_body(const ConfocalArgs& a) {
         }
         __syncthreads();
         for (int jl = 0; jl < nb; ++jl) {
-            const uint32_t l = lb + jl;
+            const uint32_t l = tile_l ? tile_l[lb + jl] : lb + jl;
+            const float wl = tile_l ? tile_w[lb + jl] : 1.f;
             const int kmin = jl * a.W - __float_as_int(prm[jl * 8 + P_BASE]);
             float sc = 1.f;
             if constexpr (STORAGE != FBP_STORAGE_F32) sc = a.scales[l];
@@ -194,6 +203,7 @@ __device__ __forceinline__ void confocal_voxel_body(const ConfocalArgs& a) {
                     v = load_sample<STORAGE>(a.data, idx, sc);
                     // Quantised storage was thresholded before quantisation.
                     if constexpr (STORAGE == FBP_STORAGE_F32) v = (v > kEps) ? v : 0.f;
+                    v *= wl;
                 }
                 // Threshold of bin k (only bins 0..T matter; outside, both
                 // candidate samples are zero).
@@ -365,11 +375,12 @@ static void launch_confocal(const ConfocalArgs& a, bool pure, dim3 grid, size_t 
 
 using namespace fbp;
 
-extern "C" fbp_status fbp_backproject_confocal_voxel(fbp_context ctx,
-                                                     const fbp_reconstruction_info* info,
-                                                     const fbp_transient* tr,
-                                                     const fbp_vec3* laser_targets, float* volume,
-                                                     void* stream) {
+namespace fbp {
+
+fbp_status confocal_voxel_impl(fbp_context ctx, const fbp_reconstruction_info* info,
+                               const fbp_transient* tr, const fbp_vec3* laser_targets,
+                               float* volume, void* stream, const uint32_t* sample_l,
+                               const float* sample_w, uint32_t samples) {
     if (!ctx || !info || !tr || !tr->data || !laser_targets || !volume)
         return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "confocal: null argument");
     if (info->num_laser_targets == 0 || info->num_time_bins == 0 || info->voxel_resolution.x == 0 ||
@@ -392,6 +403,9 @@ extern "C" fbp_status fbp_backproject_confocal_voxel(fbp_context ctx,
     a.laser = laser_targets;
     a.g = make_geom(*info, 4, 4, 4);
     a.L = info->num_laser_targets;
+    a.items = samples ? samples : a.L;
+    a.sample_l = samples ? sample_l : nullptr;
+    a.sample_w = samples ? sample_w : nullptr;
     a.T = info->num_time_bins;
     a.t0 = info->time_offset;
     a.dt = info->time_step;
@@ -413,16 +427,18 @@ extern "C" fbp_status fbp_backproject_confocal_voxel(fbp_context ctx,
     a.batch = 16;
     while (a.batch > 1 && size_t(a.batch) * (2 * a.W + 8) * 4 > 20480) a.batch /= 2;
 
-    const uint32_t nc = pick_chunks(ctx, a.g.num_tiles, a.L, 64);
-    a.chunk_len = (a.L + nc - 1) / nc;
-    const uint32_t chunks = (a.L + a.chunk_len - 1) / a.chunk_len;
+    // Tile-sampled lists are short: smaller chunks keep the CUs busy.
+    const uint32_t nc = pick_chunks(ctx, a.g.num_tiles, a.items, samples ? 8 : 64);
+    a.chunk_len = (a.items + nc - 1) / nc;
+    const uint32_t chunks = (a.items + a.chunk_len - 1) / a.chunk_len;
     const bool pure = (a.t0 == 0.f) && (a.discard != 0);
     a.thr_rows = pure ? 1u : a.L;
     const size_t slab = size_t(chunks) * a.g.num_tiles * a.g.slots * 64;
     const size_t thr_n = size_t(a.thr_rows) * (a.T + 1);
     // Scratch holds the general path's buffers or the aligned path's: only
     // one of the two paths writes, so they share the space.
-    const AlignedPlan ap = plan_aligned_confocal(ctx, *info, *tr);
+    AlignedPlan ap = plan_aligned_confocal(ctx, *info, *tr);
+    if (samples) ap.usable = false;
     const size_t general_bytes = (2 * slab + thr_n) * sizeof(float) + 256;
     const size_t body = std::max(general_bytes, ap.usable ? aligned_scratch_bytes(ap) : size_t(0));
     FBP_CHECK(ensure_scratch(ctx, body));
@@ -457,4 +473,14 @@ extern "C" fbp_status fbp_backproject_confocal_voxel(fbp_context ctx,
     }
     FBP_CHECK_HIP(ctx, hipGetLastError());
     return launch_reduce(ctx, a.partial, a.corr, volume, a.g, chunks, 1.0, s, a.skip);
+}
+
+}  // namespace fbp
+
+extern "C" fbp_status fbp_backproject_confocal_voxel(fbp_context ctx,
+                                                     const fbp_reconstruction_info* info,
+                                                     const fbp_transient* tr,
+                                                     const fbp_vec3* laser_targets, float* volume,
+                                                     void* stream) {
+    return confocal_voxel_impl(ctx, info, tr, laser_targets, volume, stream, nullptr, nullptr, 0);
 }

@@ -8,10 +8,11 @@
 // bounds; here every pair is visited with l = ls / S, s = ls % S (identical
 // indexing when L == S).
 //
-// One wave owns a 16 (vx) x 16 (vy) x 8 (z) tile, 32 voxels per lane, and a
-// contiguous chunk of pair indices.  Pairs are staged through LDS in batches:
-// the bin window of each pair (I[l][s][kmin .. kmin+W)) is loaded once per
-// tile.  Laser distances of the lane's voxels are cached while l is
+// A workgroup of four waves owns a 16 (vx) x 16 (vy) x 32 (z) tile (each
+// wave a z-slab of 8, 32 voxels per lane) and a contiguous chunk of pair
+// indices.  Pairs are staged through LDS in double-buffered batches: the bin
+// window of each pair (I[l][s][kmin .. kmin+W)) is loaded once per tile,
+// the next batch's windows while the current one is processed.  Laser distances of the lane's voxels are cached while l is
 // unchanged (S consecutive pairs share it); sensor distances come from exact
 // per-axis tables.  Bins use the same fast-quotient / exact-fallback scheme
 // as the confocal kernel.
@@ -38,9 +39,16 @@ struct ExhaustiveArgs {
 
 enum { Q_SX = 0, Q_SY, Q_SZ, Q_BASE, Q_HME, Q_COFF, Q_NEAR, Q_OFF };
 
+// Laser position of the cached laser distances (the exact fallback
+// recomputes the exact squared laser distance from it).
+struct LaserPos {
+    float x, y, z;
+};
+
 template <bool NEAR>
 __device__ __forceinline__ void exhaustive_update_tile(float (&acc)[32], const float (&rl)[32],
-                                                       const float (&AL)[4][2], const float (&ZL2)[4],
+                                                       const float (&px)[4], const float (&py)[2],
+                                                       const float (&pz)[4], LaserPos lp,
                                                        const float (&AS)[4][2], const float (&ZS2)[4],
                                                        const float* win, int base, float hme,
                                                        float coff, float off,
@@ -63,7 +71,9 @@ __device__ __forceinline__ void exhaustive_update_tile(float (&acc)[32], const f
                 bool flag = fabsf(f - 0.5f) > hme;
                 if constexpr (NEAR) flag = flag || !(d2 >= 2.0e-5f) || !(ss >= 1.0e-6f);
                 if (__builtin_expect(flag, 0)) {
-                    const float sl = AL[i][j] + ZL2[m];  // exact (unscaled) tables
+                    // exact squared laser distance in glm::dot's order
+                    const float sl = dot3(sq(rn_sub(px[i], lp.x)), sq(rn_sub(py[j], lp.y)),
+                                          sq(rn_sub(pz[m], lp.z)));
                     const float de = exhaustive_dist(sl, ss, a.t0, a.discard, off);
                     k = bin_of(de, a.dt);
                     w = safe_rcp(rn_mul(de, de));
@@ -74,13 +84,65 @@ __device__ __forceinline__ void exhaustive_update_tile(float (&acc)[32], const f
     }
 }
 
-template <int STORAGE>
-__global__ __launch_bounds__(64, 2) void exhaustive_voxel_kernel(ExhaustiveArgs a) {
-    extern __shared__ float smem[];
-    float* win = smem;
-    float* prm = smem + a.batch * a.W;
+// Window prefetch registers per thread: batch * W <= kPrefetch * 256.
+constexpr int kPrefetch = 16;
+constexpr int kWaves = 4;
 
-    const int lane = threadIdx.x;
+// Pair parameters of one batch slot (computed by one thread, read by all).
+struct PairParams {
+    float sx, sy, sz;
+    int base;     // slot * W - kmin: LDS index of bin k is base + k
+    float hme;    // 0.5 - fast-path tolerance (in bins)
+    float coff;   // off - t0
+    float near;   // 1: every voxel of the tile takes the exact chain
+    float off;
+    int kmin;
+};
+
+template <int STORAGE>
+__device__ __forceinline__ PairParams pair_params(const ExhaustiveArgs& a, uint64_t pair, int slot,
+                                                  const float (&lo)[3], const float (&hi)[3]) {
+    const uint32_t l = uint32_t(pair / a.S), s = uint32_t(pair % a.S);
+    const float lp[3] = {a.laser[l].x, a.laser[l].y, a.laser[l].z};
+    const float sp[3] = {a.sensor[s].x, a.sensor[s].y, a.sensor[s].z};
+    float off = 0.f;
+    if (a.discard == 0)
+        off = rn_add(distance3(lp[0], lp[1], lp[2], a.lpos[0], a.lpos[1], a.lpos[2]),
+                     distance3(sp[0], sp[1], sp[2], a.spos[0], a.spos[1], a.spos[2]));
+    const float rlmin = box_min_dist(lp, lo, hi), rlmax = box_max_dist(lp, lo, hi);
+    const float rsmin = box_min_dist(sp, lo, hi), rsmax = box_max_dist(sp, lo, hi);
+    const float dmin = rlmin + rsmin - a.t0 + off;
+    const float dmax = rlmax + rsmax - a.t0 + off;
+    PairParams P;
+    P.kmin = int(floorf(dmin * a.inv_dt)) - 3;
+    const float mag = rlmax + rsmax + fabsf(a.t0) + fabsf(off);
+    const float eps = 16.f * 1.1920929e-7f * mag * a.inv_dt + 1.0e-6f;
+    P.sx = sp[0];
+    P.sy = sp[1];
+    P.sz = sp[2];
+    P.base = slot * a.W - P.kmin;
+    P.hme = 0.5f - eps;
+    P.coff = off - a.t0;
+    P.near = ((dmin < 0.0045f && dmax > -0.0045f) || rlmin < 1.0e-3f || rsmin < 1.0e-3f) ? 1.f : 0.f;
+    P.off = off;
+    return P;
+}
+
+// Workgroup: kWaves waves own one 16 (vx) x 16 (vy) x 32 (z) tile; wave w
+// holds the z-slab 8w .. 8w+7 (32 voxels per lane) and all waves share the
+// staged bin windows of a batch of (laser, sensor) pairs.  Batches are
+// double-buffered: the windows of batch b+1 are fetched into registers while
+// batch b is processed.
+template <int STORAGE>
+__global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs a) {
+    extern __shared__ float smem[];
+    // [2][batch * W] windows, then [2][batch] PairParams
+    float* winbuf = smem;
+    PairParams* prmbuf = reinterpret_cast<PairParams*>(smem + 2 * a.batch * a.W);
+
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
     const uint32_t tile = blockIdx.x % a.g.num_tiles;
     const uint32_t chunk = blockIdx.x / a.g.num_tiles;
     const uint32_t tx = tile % a.g.tiles[0];
@@ -88,6 +150,7 @@ __global__ __launch_bounds__(64, 2) void exhaustive_voxel_kernel(ExhaustiveArgs 
     const uint32_t tz = tile / (a.g.tiles[0] * a.g.tiles[1]);
     const uint32_t X0 = tx * a.g.tile[0], Y0 = ty * a.g.tile[1], Z0 = tz * a.g.tile[2];
     const int lx = lane & 3, lz = (lane >> 2) & 3, ly = lane >> 4;
+    const uint32_t Zw = Z0 + 8u * uint32_t(wave);
 
     float px[4], py[2], pz[4];
 #pragma unroll
@@ -96,7 +159,7 @@ __global__ __launch_bounds__(64, 2) void exhaustive_voxel_kernel(ExhaustiveArgs 
         pz[i] = voxel_center(a.g.hmin[2], a.g.vs[2], Y0 + ly + 4 * i);
     }
 #pragma unroll
-    for (int j = 0; j < 2; ++j) py[j] = voxel_center(a.g.hmin[1], a.g.vs[1], Z0 + lz + 4 * j);
+    for (int j = 0; j < 2; ++j) py[j] = voxel_center(a.g.hmin[1], a.g.vs[1], Zw + lz + 4 * j);
     float lo[3], hi[3];
     tile_box(a.g, X0, Y0, Z0, lo, hi);
 
@@ -104,96 +167,95 @@ __global__ __launch_bounds__(64, 2) void exhaustive_voxel_kernel(ExhaustiveArgs 
 #pragma unroll
     for (int i = 0; i < 32; ++i) acc[i] = 0.f;
     float rl[32];
-    float AL[4][2], ZL2[4];
+    LaserPos lp{0.f, 0.f, 0.f};
     uint32_t cur_l = 0xffffffffu;
 
     const uint64_t p_begin = uint64_t(chunk) * a.chunk_len;
     const uint64_t p_end = min(a.num_pairs, p_begin + a.chunk_len);
+    const int per = a.batch * a.W;
 
-    for (uint64_t pb = p_begin; pb < p_end; pb += a.batch) {
-        const int nb = int(min(uint64_t(a.batch), p_end - pb));
-        __syncthreads();
-        if (lane < nb) {
-            const uint64_t pair = pb + lane;
-            const uint32_t l = uint32_t(pair / a.S), s = uint32_t(pair % a.S);
-            const float lp[3] = {a.laser[l].x, a.laser[l].y, a.laser[l].z};
-            const float sp[3] = {a.sensor[s].x, a.sensor[s].y, a.sensor[s].z};
-            float off = 0.f;
-            if (a.discard == 0)
-                off = rn_add(distance3(lp[0], lp[1], lp[2], a.lpos[0], a.lpos[1], a.lpos[2]),
-                             distance3(sp[0], sp[1], sp[2], a.spos[0], a.spos[1], a.spos[2]));
-            const float rlmin = box_min_dist(lp, lo, hi), rlmax = box_max_dist(lp, lo, hi);
-            const float rsmin = box_min_dist(sp, lo, hi), rsmax = box_max_dist(sp, lo, hi);
-            const float dmin = rlmin + rsmin - a.t0 + off;
-            const float dmax = rlmax + rsmax - a.t0 + off;
-            const int kmin = int(floorf(dmin * a.inv_dt)) - 3;
-            const float mag = rlmax + rsmax + fabsf(a.t0) + fabsf(off);
-            const float eps = 16.f * 1.1920929e-7f * mag * a.inv_dt + 1.0e-6f;
-            float* P = prm + lane * 8;
-            P[Q_SX] = sp[0];
-            P[Q_SY] = sp[1];
-            P[Q_SZ] = sp[2];
-            P[Q_BASE] = __int_as_float(lane * a.W - kmin);
-            P[Q_HME] = 0.5f - eps;
-            P[Q_COFF] = off - a.t0;
-            P[Q_NEAR] = ((dmin < 0.0045f && dmax > -0.0045f) || rlmin < 1.0e-3f ||
-                         rsmin < 1.0e-3f)
-                            ? 1.f
-                            : 0.f;
-            P[Q_OFF] = off;
-        }
-        __syncthreads();
-        for (int jl = 0; jl < nb; ++jl) {
-            const uint64_t pair = pb + jl;
-            const int kmin = jl * a.W - __float_as_int(prm[jl * 8 + Q_BASE]);
-            float sc = 1.f;
-            if constexpr (STORAGE != FBP_STORAGE_F32) sc = a.scales[pair];
-            for (int w = lane; w < a.W; w += 64) {
-                const int k = kmin + w;
-                float v = 0.f;
+    auto fetch = [&](uint64_t pb, int nb, const PairParams* prm, float (&reg)[kPrefetch]) {
+#pragma unroll
+        for (int i = 0; i < kPrefetch; ++i) {
+            const int f = tid + i * 256;
+            float v = 0.f;
+            if (f < nb * a.W) {
+                const int jl = f / a.W, w = f % a.W;
+                const uint64_t pair = pb + jl;
+                const int k = prm[jl].kmin + w;
                 if (k >= 0 && k < int(a.T)) {
+                    float sc = 1.f;
+                    if constexpr (STORAGE != FBP_STORAGE_F32) sc = a.scales[pair];
                     v = load_sample<STORAGE>(a.data, pair * a.T + uint32_t(k), sc);
                     if constexpr (STORAGE == FBP_STORAGE_F32) v = (v > kEps) ? v : 0.f;
                 }
-                win[jl * a.W + w] = v;
             }
+            reg[i] = v;
         }
+    };
+    auto store = [&](float* win, int nb, const float (&reg)[kPrefetch]) {
+#pragma unroll
+        for (int i = 0; i < kPrefetch; ++i) {
+            const int f = tid + i * 256;
+            if (f < nb * a.W) win[f] = reg[i];
+        }
+    };
+
+    float reg[kPrefetch];
+    // Prologue: batch 0 into buffer 0.
+    if (p_begin < p_end) {
+        const int nb = int(min(uint64_t(a.batch), p_end - p_begin));
+        if (tid < nb) prmbuf[tid] = pair_params<STORAGE>(a, p_begin + tid, tid, lo, hi);
         __syncthreads();
+        fetch(p_begin, nb, prmbuf, reg);
+        store(winbuf, nb, reg);
+        __syncthreads();
+    }
+    int cur = 0;
+    for (uint64_t pb = p_begin; pb < p_end; pb += a.batch) {
+        const int nb = int(min(uint64_t(a.batch), p_end - pb));
+        const uint64_t pn = pb + a.batch;
+        const bool has_next = pn < p_end;
+        const int nbn = has_next ? int(min(uint64_t(a.batch), p_end - pn)) : 0;
+        float* win = winbuf + cur * per;
+        const PairParams* prm = prmbuf + cur * a.batch;
+        PairParams* prm_next = prmbuf + (cur ^ 1) * a.batch;
+        if (has_next) {
+            if (tid < nbn) prm_next[tid] = pair_params<STORAGE>(a, pn + tid, tid, lo, hi);
+            __syncthreads();
+            fetch(pn, nbn, prm_next, reg);
+        }
         for (int jl = 0; jl < nb; ++jl) {
             const uint64_t pair = pb + jl;
             const uint32_t l = rfl(int(uint32_t(pair / a.S)));
             if (l != cur_l) {
                 cur_l = l;
-                const float lxp = a.laser[l].x, lyp = a.laser[l].y, lzp = a.laser[l].z;
-                float XL2[4], YL2[2];
+                lp = LaserPos{rfl(a.laser[l].x), rfl(a.laser[l].y), rfl(a.laser[l].z)};
+                float XL2[4], YL2[2], ZL2[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    XL2[i] = sq(rn_sub(px[i], lxp));
-                    ZL2[i] = sq(rn_sub(pz[i], lzp));
+                    XL2[i] = sq(rn_sub(px[i], lp.x));
+                    ZL2[i] = sq(rn_sub(pz[i], lp.z));
                 }
 #pragma unroll
-                for (int j = 0; j < 2; ++j) YL2[j] = sq(rn_sub(py[j], lyp));
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int j = 0; j < 2; ++j) AL[i][j] = rn_add(XL2[i], YL2[j]);
+                for (int j = 0; j < 2; ++j) YL2[j] = sq(rn_sub(py[j], lp.y));
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
                     for (int j = 0; j < 2; ++j)
 #pragma unroll
                         for (int m = 0; m < 4; ++m) {
-                            const float sl = AL[i][j] + ZL2[m];  // exact (unscaled) tables
+                            const float sl = dot3(XL2[i], YL2[j], ZL2[m]);
                             rl[(i * 2 + j) * 4 + m] = sl * __builtin_amdgcn_rsqf(sl);
                         }
             }
-            const float* P = prm + jl * 8;
-            const float sxp = rfl(P[Q_SX]), syp = rfl(P[Q_SY]), szp = rfl(P[Q_SZ]);
-            const int base = rfl(__float_as_int(P[Q_BASE]));
-            const float hme = rfl(P[Q_HME]);
-            const float coff = rfl(P[Q_COFF]);
-            const bool near = rfl(P[Q_NEAR]) != 0.f;
-            const float off = rfl(P[Q_OFF]);
+            const PairParams& P = prm[jl];
+            const float sxp = rfl(P.sx), syp = rfl(P.sy), szp = rfl(P.sz);
+            const int base = rfl(P.base) + cur * per;
+            const float hme = rfl(P.hme);
+            const float coff = rfl(P.coff);
+            const bool near = rfl(P.near) != 0.f;
+            const float off = rfl(P.off);
             float XS2[4], YS2[2], ZS2[4], AS[4][2];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
@@ -207,15 +269,27 @@ __global__ __launch_bounds__(64, 2) void exhaustive_voxel_kernel(ExhaustiveArgs 
 #pragma unroll
                 for (int j = 0; j < 2; ++j) AS[i][j] = rn_add(XS2[i], YS2[j]);
             if (near)
-                exhaustive_update_tile<true>(acc, rl, AL, ZL2, AS, ZS2, win, base, hme, coff, off, a);
+                exhaustive_update_tile<true>(acc, rl, px, py, pz, lp, AS, ZS2, winbuf, base, hme, coff,
+                                             off, a);
             else
-                exhaustive_update_tile<false>(acc, rl, AL, ZL2, AS, ZS2, win, base, hme, coff, off, a);
+                exhaustive_update_tile<false>(acc, rl, px, py, pz, lp, AS, ZS2, winbuf, base, hme, coff,
+                                              off, a);
         }
+        if (has_next) store(winbuf + (cur ^ 1) * per, nbn, reg);
+        __syncthreads();
+        cur ^= 1;
     }
 
-    float* out = a.partial + (size_t(chunk) * a.g.num_tiles + tile) * 32 * 64;
+    // Slots of the workgroup tile (lt = 4 x 4 x 8): wave w's z-slab holds
+    // oz >> 2 = 2w + j.
+    float* out = a.partial + (size_t(chunk) * a.g.num_tiles + tile) * a.g.slots * 64;
 #pragma unroll
-    for (int s = 0; s < 32; ++s) out[s * 64 + lane] = acc[s];
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                out[size_t((i * 8 + 2 * wave + j) * 4 + m) * 64 + lane] = acc[(i * 2 + j) * 4 + m];
 }
 
 }  // namespace fbp
@@ -249,7 +323,8 @@ extern "C" fbp_status fbp_backproject_exhaustive_voxel(fbp_context ctx,
     a.scales = tr->scales;
     a.laser = laser_targets;
     a.sensor = sensor_targets;
-    a.g = make_geom(*info, 4, 4, 2);
+    // Workgroup tiles of 16 x 16 x 32 voxels (four waves of 16 x 16 x 8).
+    a.g = make_geom(*info, 4, 4, 2 * kWaves);
     a.L = info->num_laser_targets;
     a.S = info->num_sensor_targets;
     a.T = info->num_time_bins;
@@ -265,12 +340,14 @@ extern "C" fbp_status fbp_backproject_exhaustive_voxel(fbp_context ctx,
     a.spos[2] = info->sensor_position.z;
     a.num_pairs = uint64_t(a.L) * a.S;
     a.W = window_bins(a.g, a.dt, 2);
-    if (size_t(a.W + 8) * 4 > 65536)
-        return fail(ctx, FBP_ERR_UNSUPPORTED, "exhaustive: bin window of %d bins exceeds LDS", a.W);
+    if (a.W > kPrefetch * 256)
+        return fail(ctx, FBP_ERR_UNSUPPORTED, "exhaustive: bin window of %d bins per tile too long "
+                    "(voxels too large for the bin width)", a.W);
+    // Pairs per batch: the window prefetch registers hold one batch.
     a.batch = 16;
-    while (a.batch > 1 && size_t(a.batch) * (a.W + 8) * 4 > 32768) a.batch /= 2;
-
-    const uint32_t target = 16u * uint32_t(ctx->num_cus > 0 ? ctx->num_cus : 256);
+    while (a.batch > 1 && a.batch * a.W > kPrefetch * 256) a.batch /= 2;
+    // ~8 workgroups (32 waves) per CU over the launch.
+    const uint32_t target = 8u * uint32_t(ctx->num_cus > 0 ? ctx->num_cus : 256);
     uint64_t nc = (target + a.g.num_tiles - 1) / a.g.num_tiles;
     const uint64_t cap = (a.num_pairs + 63) / 64;
     if (nc > cap) nc = cap;
@@ -282,14 +359,17 @@ extern "C" fbp_status fbp_backproject_exhaustive_voxel(fbp_context ctx,
     a.partial = static_cast<float*>(ctx->scratch);
 
     const dim3 grid(unsigned(chunks * a.g.num_tiles));
-    const size_t shmem = size_t(a.batch) * (a.W + 8) * sizeof(float);
+    const size_t shmem = 2 * (size_t(a.batch) * a.W * sizeof(float) + a.batch * sizeof(PairParams));
     hipStream_t s = as_stream(stream);
     if (tr->storage == FBP_STORAGE_F32)
-        hipLaunchKernelGGL(exhaustive_voxel_kernel<FBP_STORAGE_F32>, grid, dim3(64), shmem, s, a);
+        hipLaunchKernelGGL(exhaustive_voxel_kernel<FBP_STORAGE_F32>, grid, dim3(64 * kWaves), shmem,
+                           s, a);
     else if (tr->storage == FBP_STORAGE_F16)
-        hipLaunchKernelGGL(exhaustive_voxel_kernel<FBP_STORAGE_F16>, grid, dim3(64), shmem, s, a);
+        hipLaunchKernelGGL(exhaustive_voxel_kernel<FBP_STORAGE_F16>, grid, dim3(64 * kWaves), shmem,
+                           s, a);
     else
-        hipLaunchKernelGGL(exhaustive_voxel_kernel<FBP_STORAGE_BF16>, grid, dim3(64), shmem, s, a);
+        hipLaunchKernelGGL(exhaustive_voxel_kernel<FBP_STORAGE_BF16>, grid, dim3(64 * kWaves), shmem,
+                           s, a);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     return launch_reduce(ctx, a.partial, nullptr, volume, a.g, uint32_t(chunks), 1.0, s);
 }

@@ -137,6 +137,63 @@ __global__ void mis_kernel(const float* __restrict__ I, const fbp_vec3* __restri
     volume[vid] = sum;
 }
 
+// Inclusive FP64 prefix of the row sums (one workgroup of 1024 threads).
+__global__ __launch_bounds__(1024) void row_prefix_kernel(const double* __restrict__ sum, uint32_t L,
+                                                          double* __restrict__ cum) {
+    __shared__ double part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (L + 1023) / 1024;
+    const uint32_t b = t * per, e = min(L, b + per);
+    double local = 0.0;
+    for (uint32_t l = b; l < e; ++l) local += sum[l];
+    part[t] = local;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const double v = t >= off ? part[t - off] : 0.0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    double run = part[t] - local;
+    for (uint32_t l = b; l < e; ++l) {
+        run += sum[l];
+        cum[l] = run;
+    }
+}
+
+// Scan-point samples of every voxel tile for FBP_MIS_TILED.
+__global__ void mis_tile_samples_kernel(const double* __restrict__ sum, const double* __restrict__ cum,
+                                        uint32_t L, const float* __restrict__ noise,
+                                        uint32_t noise_size, uint32_t samples, uint32_t tiles,
+                                        uint32_t* __restrict__ sample_l, float* __restrict__ sample_w) {
+    const size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (id >= size_t(tiles) * samples) return;
+    const uint32_t tile = uint32_t(id / samples), i = uint32_t(id % samples);
+    const double total = cum[L - 1];
+    uint32_t state = tile * samples + i;
+    const float u0 = mis_uniform(noise, noise_size, state);
+    const float u1 = mis_uniform(noise, noise_size, state);
+    uint32_t l;
+    if (total > 0.0 && u0 < 0.5f) {
+        const double target = double(u1) * total;
+        uint32_t lo = 0, hi = L - 1;  // first l with cum[l] > target
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (cum[mid] > target)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        l = lo;
+    } else {
+        l = uint32_t(u1 * float(L));
+        if (l >= L) l = L - 1;
+    }
+    const double pmix = total > 0.0 ? 0.5 * (sum[l] / total + 1.0 / double(L)) : 1.0 / double(L);
+    sample_l[id] = l;
+    sample_w[id] = float(1.0 / (pmix * double(samples)));
+}
+
 // Per-scan-point total intensity (I > 1e-5 samples), in FP64.
 __global__ void row_sum_kernel(const float* __restrict__ I, uint32_t L, uint32_t T,
                                double* __restrict__ out) {
@@ -427,11 +484,31 @@ fbp_status fbp_backproject_confocal_mis(fbp_context ctx, const fbp_reconstructio
     if (tr->storage != FBP_STORAGE_F32 || tr->layout != FBP_LAYOUT_TIME_FASTEST)
         return fail(ctx, FBP_ERR_UNSUPPORTED, "mis: F32 [l][t] transient only");
     if (!params->noise_size || !params->num_samples || !info->num_laser_targets ||
-        !info->num_time_bins || !(info->time_step > 0.f) || params->mode > FBP_MIS_UNBIASED)
+        !info->num_time_bins || !(info->time_step > 0.f) || params->mode > FBP_MIS_TILED)
         return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "mis: bad parameters");
     FBP_CHECK(bind(ctx));
     hipStream_t s = as_stream(stream);
     const uint32_t L = info->num_laser_targets, T = info->num_time_bins;
+    if (params->mode == FBP_MIS_TILED) {
+        // scratch2: row sums (double L) | prefix (double L) | samples (u32, f32 per tile)
+        const uint32_t tiles = make_geom(*info, 4, 4, 4).num_tiles;
+        const size_t ns = size_t(tiles) * params->num_samples;
+        FBP_CHECK(ensure_scratch2(ctx, size_t(L) * 16 + ns * 8 + 512));
+        char* base = static_cast<char*>(ctx->scratch2);
+        double* d_sum = reinterpret_cast<double*>(base);
+        double* d_cum = d_sum + L;
+        uint32_t* d_l = reinterpret_cast<uint32_t*>(d_cum + L);
+        float* d_w = reinterpret_cast<float*>(d_l + ns);
+        hipLaunchKernelGGL(row_sum_kernel, dim3(L), dim3(256), 0, s,
+                           static_cast<const float*>(tr->data), L, T, d_sum);
+        hipLaunchKernelGGL(row_prefix_kernel, dim3(1), dim3(1024), 0, s, d_sum, L, d_cum);
+        hipLaunchKernelGGL(mis_tile_samples_kernel, dim3(unsigned((ns + 255) / 256)), dim3(256), 0, s,
+                           d_sum, d_cum, L, params->noise, params->noise_size, params->num_samples,
+                           tiles, d_l, d_w);
+        FBP_CHECK_HIP(ctx, hipGetLastError());
+        return confocal_voxel_impl(ctx, info, tr, laser_targets, volume, stream, d_l, d_w,
+                                   params->num_samples);
+    }
     // Scratch: row sums (double L) | cdf (float L) | alias (u32 L) | prob (float L)
     const size_t bytes = size_t(L) * (8 + 4 + 4 + 4) + 256;
     FBP_CHECK(ensure_scratch(ctx, bytes));

@@ -248,10 +248,29 @@ __global__ void scale_f(const float* __restrict__ a, float* __restrict__ b, size
     b[i] = m > 0.f ? a[i] / m : 0.f;
 }
 
-__global__ void absmax_kernel(const float* __restrict__ v, size_t n, float* __restrict__ mm) {
+// Largest |v|: per-block partial maxima, then one block over the partials.
+constexpr int kAbsBlocks = 256;
+__global__ void absmax_partial_kernel(const float* __restrict__ v, size_t n,
+                                      float* __restrict__ part) {
     __shared__ float s[256];
     float m = 0.f;
-    for (size_t i = threadIdx.x; i < n; i += blockDim.x) m = fmaxf(m, fabsf(v[i]));
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += size_t(gridDim.x) * blockDim.x)
+        m = fmaxf(m, fabsf(v[i]));
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) s[threadIdx.x] = fmaxf(s[threadIdx.x], s[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+__global__ void absmax_final_kernel(const float* __restrict__ part, int nparts,
+                                    float* __restrict__ mm) {
+    __shared__ float s[256];
+    float m = 0.f;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) m = fmaxf(m, part[i]);
     s[threadIdx.x] = m;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -446,7 +465,7 @@ fbp_status refiner_alloc(fbp_refiner_s* r) {
     FBP_CHECK(alloc(r, &r->gp1, V2 * 16));
     FBP_CHECK(alloc(r, &r->ge1, V * 16));
     FBP_CHECK(alloc(r, &r->tmp, V * 32));
-    FBP_CHECK(alloc(r, &r->ssim, V * 18 + 64));
+    FBP_CHECK(alloc(r, &r->ssim, V * 18 + 64 + kAbsBlocks));
     return FBP_OK;
 }
 
@@ -462,21 +481,31 @@ fbp_status refiner_forward_impl(fbp_refiner_s* r, hipStream_t s, bool train) {
     hipLaunchKernelGGL((conv_valu_fwd<1, 16, 3, float>), g1(V), dim3(256), 0, s, r->x, P + woff(0),
                        P + boff(0), train ? r->pre1 : nullptr, r->e1, d1, 1);
     hipLaunchKernelGGL(pool_fwd, g1(V2 * 16), dim3(256), 0, s, r->e1, r->p1, d2, 16);
+    // Waves of the MFMA convolutions: MT tiles of 16 voxels each.
+    auto waves = [](size_t vox, int mt) { return (vox / 16 + mt - 1) / mt * 64; };
     // e2 = relu(conv3(p1; 16->32))
-    hipLaunchKernelGGL((conv_mfma_fwd<3, 16, 32>), g1(V2 / 16 * 64), dim3(256), 0, s, r->p1,
-                       Wb + woff(1), P + boff(1), r->e2, nullptr, train ? r->pre2 : nullptr, d2, 1);
+    hipLaunchKernelGGL((conv_mfma_fwd<3, 16, 32, false, 2>), g1(waves(V2, 2)), dim3(256), 0, s,
+                       r->p1, Wb + woff(1), P + boff(1), r->e2, nullptr,
+                       train ? r->pre2 : nullptr, d2, 1);
     hipLaunchKernelGGL(pool_fwd, g1(V4 * 32), dim3(256), 0, s, r->e2, r->p2, d4, 32);
     // e3 = relu(conv3(p2; 32->64))
-    hipLaunchKernelGGL((conv_mfma_fwd<3, 32, 64>), g1(V4 / 16 * 64), dim3(256), 0, s, r->p2,
-                       Wb + woff(2), P + boff(2), r->e3, nullptr, train ? r->pre3 : nullptr, d4, 1);
-    hipLaunchKernelGGL(up_fwd, g1(V2 * 64), dim3(256), 0, s, r->e3, r->u2, d2, 64);
-    // d2 = relu(conv3(u2; 64->32)) + e2
-    hipLaunchKernelGGL((conv_mfma_fwd<3, 64, 32>), g1(V2 / 16 * 64), dim3(256), 0, s, r->u2,
-                       Wb + woff(3), P + boff(3), r->d2, r->e2, train ? r->pre4 : nullptr, d2, 1);
-    hipLaunchKernelGGL(up_fwd, g1(V * 32), dim3(256), 0, s, r->d2, r->u1, d1, 32);
-    // d1 = relu(conv1(u1; 32->16)) + e1
-    hipLaunchKernelGGL((conv_mfma_fwd<1, 32, 16>), g1(V / 16 * 64), dim3(256), 0, s, r->u1,
-                       Wb + woff(4), P + boff(4), r->d1, r->e1, train ? r->pre5 : nullptr, d1, 1);
+    hipLaunchKernelGGL((conv_mfma_fwd<3, 32, 64, false, 1>), g1(waves(V4, 1)), dim3(256), 0, s,
+                       r->p2, Wb + woff(2), P + boff(2), r->e3, nullptr,
+                       train ? r->pre3 : nullptr, d4, 1);
+    // d2 = relu(conv3(up(e3); 64->32)) + e2  (upsampling fused into the input reads)
+    hipLaunchKernelGGL((conv_mfma_fwd<3, 64, 32, true, 2>), g1(waves(V2, 2)), dim3(256), 0, s,
+                       r->e3, Wb + woff(3), P + boff(3), r->d2, r->e2,
+                       train ? r->pre4 : nullptr, d2, 1);
+    // d1 = relu(conv1(up(d2); 32->16)) + e1
+    hipLaunchKernelGGL((conv_mfma_fwd<1, 32, 16, true, 2>), g1(waves(V, 2)), dim3(256), 0, s,
+                       r->d2, Wb + woff(4), P + boff(4), r->d1, r->e1,
+                       train ? r->pre5 : nullptr, d1, 1);
+    if (train) {
+        // The backward pass takes the upsampled tensors as inputs of its
+        // weight gradients.
+        hipLaunchKernelGGL(up_fwd, g1(V2 * 64), dim3(256), 0, s, r->e3, r->u2, d2, 64);
+        hipLaunchKernelGGL(up_fwd, g1(V * 32), dim3(256), 0, s, r->d2, r->u1, d1, 32);
+    }
     // y = conv3(d1; 16->1) + x: the network predicts a correction of its
     // (normalised) input; the input needs no gradient, so the backward pass
     // starts from dL/dy unchanged.
@@ -577,7 +606,9 @@ fbp_status refiner_backward(fbp_refiner_s* r, hipStream_t s) {
 fbp_status normalize_input(fbp_refiner_s* r, const float* in, hipStream_t s) {
     const size_t V = size_t(r->S) * r->S * r->S;
     float* mm = r->ssim + 18 * V + 8;
-    hipLaunchKernelGGL(absmax_kernel, dim3(1), dim3(256), 0, s, in, V, mm);
+    float* part = r->ssim + 18 * V + 64;
+    hipLaunchKernelGGL(absmax_partial_kernel, dim3(kAbsBlocks), dim3(256), 0, s, in, V, part);
+    hipLaunchKernelGGL(absmax_final_kernel, dim3(1), dim3(256), 0, s, part, kAbsBlocks, mm);
     hipLaunchKernelGGL(scale_f, g1(V), dim3(256), 0, s, in, r->x, V, mm);
     FBP_CHECK_HIP(r->ctx, hipGetLastError());
     return FBP_OK;

@@ -36,10 +36,12 @@ __device__ __forceinline__ void unflat(size_t v, const Dims& d, int& z, int& y, 
 // Forward convolution, Cin and Cout multiples of 16, kernel KS^3 (1 or 3),
 // zero padding KS/2, BF16 in, FP32 accumulation on MFMA 16x16x16, bias,
 // optional ReLU, then optional skip addition; BF16 out.  Weights
-// [tap][Cout][Cin] BF16.  One wave: 16 consecutive voxels along x times all
-// Cout.  Lane l: row (voxel) l % 16, k-group l / 16 (4 input channels).
-// Needs the voxel count to be a multiple of 16.
-template <int KS, int CIN, int COUT>
+// [tap][Cout][Cin] BF16.  One wave: MT tiles of 16 consecutive voxels times
+// all Cout; every weight fragment it loads feeds the MT tiles.  Lane l: row
+// (voxel) l % 16, k-group l / 16 (4 input channels).  UP: the input is the
+// 2x coarser grid, read at the halved coordinates (fused nearest-neighbour
+// upsampling).  Needs the voxel count to be a multiple of 16.
+template <int KS, int CIN, int COUT, bool UP = false, int MT = 1>
 __global__ __launch_bounds__(256) void conv_mfma_fwd(const bf16* __restrict__ x,
                                                      const bf16* __restrict__ w,
                                                      const float* __restrict__ bias,
@@ -50,35 +52,56 @@ __global__ __launch_bounds__(256) void conv_mfma_fwd(const bf16* __restrict__ x,
     constexpr int NT = COUT / 16;
     constexpr int R = KS / 2;
     const int lane = threadIdx.x & 63;
-    const size_t tile = (size_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const size_t wave = (size_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
     const size_t tiles = d.vox() / 16;
-    if (tile >= tiles) return;
-    const size_t v0 = tile * 16;
+    const size_t t0 = wave * MT;
+    if (t0 >= tiles) return;
     const int row = lane & 15, kg = lane >> 4;
-    // The lane's own voxel (a tile may span rows when W < 16).
-    int z, yy, xr;
-    unflat(v0 + row, d, z, yy, xr);
-    f4 acc[NT];
+    const Dims din = UP ? Dims{d.D / 2, d.H / 2, d.W / 2} : d;
+    // The lane's own voxel in each tile (a tile may span rows when W < 16).
+    int z[MT], yy[MT], xr[MT];
+    bool live[MT];
 #pragma unroll
-    for (int n = 0; n < NT; ++n) acc[n] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int m = 0; m < MT; ++m) {
+        live[m] = t0 + m < tiles;
+        unflat((t0 + m) * 16 + row, d, z[m], yy[m], xr[m]);
+    }
+    f4 acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) acc[m][n] = f4{0.f, 0.f, 0.f, 0.f};
     for (int dz = -R; dz <= R; ++dz) {
-        const int zz = z + dz;
         for (int dy = -R; dy <= R; ++dy) {
-            const int yv = yy + dy;
             for (int dx = -R; dx <= R; ++dx) {
-                const int xv = xr + dx;
                 const int tap = ((dz + R) * KS + (dy + R)) * KS + (dx + R);
-                const bool in = zz >= 0 && zz < d.D && yv >= 0 && yv < d.H && xv >= 0 && xv < d.W;
-                const size_t src = ((size_t(zz) * d.H + yv) * d.W + xv) * CIN;
+                size_t src[MT];
+                bool in[MT];
+#pragma unroll
+                for (int m = 0; m < MT; ++m) {
+                    const int zz = z[m] + dz, yv = yy[m] + dy, xv = xr[m] + dx;
+                    in[m] = live[m] && zz >= 0 && zz < d.D && yv >= 0 && yv < d.H && xv >= 0 &&
+                            xv < d.W;
+                    const int sz = UP ? zz >> 1 : zz, sy = UP ? yv >> 1 : yv,
+                              sx = UP ? xv >> 1 : xv;
+                    src[m] = ((size_t(sz) * din.H + sy) * din.W + sx) * CIN;
+                }
 #pragma unroll
                 for (int c0 = 0; c0 < CIN; c0 += 16) {
-                    s4 a = s4{0, 0, 0, 0};
-                    if (in) a = *reinterpret_cast<const s4*>(x + src + c0 + 4 * kg);
+                    s4 a[MT];
+#pragma unroll
+                    for (int m = 0; m < MT; ++m) {
+                        a[m] = s4{0, 0, 0, 0};
+                        if (in[m]) a[m] = *reinterpret_cast<const s4*>(x + src[m] + c0 + 4 * kg);
+                    }
 #pragma unroll
                     for (int n = 0; n < NT; ++n) {
                         const s4 b = *reinterpret_cast<const s4*>(
                             w + (size_t(tap) * COUT + n * 16 + row) * CIN + c0 + 4 * kg);
-                        acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[n], 0, 0, 0);
+#pragma unroll
+                        for (int m = 0; m < MT; ++m)
+                            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], b, acc[m][n],
+                                                                                  0, 0, 0);
                     }
                 }
             }
@@ -86,16 +109,21 @@ __global__ __launch_bounds__(256) void conv_mfma_fwd(const bf16* __restrict__ x,
     }
     // D[i][j]: lane holds rows 4*kg + e, column row (= output channel offset)
 #pragma unroll
-    for (int n = 0; n < NT; ++n) {
-        const int co = n * 16 + row;
+    for (int m = 0; m < MT; ++m) {
+        if (!live[m]) continue;
+        const size_t v0 = (t0 + m) * 16;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const size_t v = v0 + 4 * kg + e;
-            float val = acc[n][e] + bias[co];
-            if (pre_relu) pre_relu[v * COUT + co] = val;
-            if (relu) val = fmaxf(val, 0.f);
-            if (skip) val += bf2f(skip[v * COUT + co]);
-            y[v * COUT + co] = f2bf(val);
+        for (int n = 0; n < NT; ++n) {
+            const int co = n * 16 + row;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const size_t v = v0 + 4 * kg + e;
+                float val = acc[m][n][e] + bias[co];
+                if (pre_relu) pre_relu[v * COUT + co] = val;
+                if (relu) val = fmaxf(val, 0.f);
+                if (skip) val += bf2f(skip[v * COUT + co]);
+                y[v * COUT + co] = f2bf(val);
+            }
         }
     }
 }

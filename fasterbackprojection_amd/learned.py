@@ -6,7 +6,7 @@ Workload: 64 x 64 confocal scan on a 1.0 m wall, 512 bins of 16 ps; 64^3
 volume over [-0.5, 0.5] x [0.2, 1.0] x [-0.5, 0.5].  Scenes: 1-6 random
 quads and spheres (scene.mixed_scene), 1/r^4 falloff, Poisson noise at a
 peak of 1000 counts.  Input: unbiased MIS estimate with L/16 samples per
-voxel (1/16 of the scan-point x bin pairs).  Target: plain
+voxel, shared by the voxels of each 16^3 tile (FBP_MIS_TILED) (1/16 of the scan-point x bin pairs).  Target: plain
 backprojection, LoG (sigma 1, k 5), min-max normalised.  The input goes
 through the same LoG + normalisation, and the refiner predicts a correction
 of it (its last layer adds the input), so the un-refined MIS path is the
@@ -29,7 +29,7 @@ from pathlib import Path
 import numpy as np
 import torch
 
-from .api import MIS_UNBIASED, Context, Geometry, Refiner, make_info
+from .api import MIS_TILED, Context, Geometry, Refiner, make_info
 from .scene import C_LIGHT, confocal_geometry, mixed_scene
 
 WEIGHTS = Path(__file__).resolve().parent / "weights"
@@ -88,7 +88,7 @@ class Pipeline:
         gen.manual_seed(10_000_019 + seed)
         noise = torch.rand(SIZE ** 3, generator=gen, device=self.dev)
         self.ctx.backproject_confocal_mis(self.info, I, self.laser, noise,
-                                          self.L // SAMPLE_FRACTION, mode=MIS_UNBIASED,
+                                          self.L // SAMPLE_FRACTION, mode=MIS_TILED,
                                           out=self.mis)
         out = torch.empty_like(self.vol) if out is None else out
         self.ctx.log(self.mis, 5, 1.0, out=out)

@@ -165,7 +165,18 @@ fbp_status fbp_backproject_depth_exhaustive(fbp_context ctx, const fbp_reconstru
  * FB/transient_utils.cuh:18-53). */
 enum {
     FBP_MIS_REFERENCE = 0, /* reference weights: w = pdf/(pdf + pdf_other), no 1/pdf */
-    FBP_MIS_UNBIASED = 1   /* one-sample MIS estimator f/(N_s (p_a + p_u)/2)         */
+    FBP_MIS_UNBIASED = 1,  /* one-sample MIS estimator f/(N_s (p_a + p_u)/2)         */
+    /* Tile-shared samples (no reference counterpart): the voxels of each
+     * 16x16x16 tile (tiles in the order of the voxel kernels) share
+     * num_samples scan points, drawn with probability 1/2 from
+     * p_a(l) = R(l) / sum R (R = FP64 sum of the samples of row l above 1e-5)
+     * and 1/2 uniformly, each weighted 1 / (num_samples * (p_a(l) + 1/L) / 2):
+     * unbiased.  Sample i of tile t: LCG state = t * num_samples + i, two
+     * draws (branch, position) from noise; the importance position is the
+     * first l whose inclusive prefix of R exceeds u * sum R.  The tables stay
+     * on the device (no host round trip); contributions use the exact bins
+     * and weights of fbp_backproject_confocal_voxel. */
+    FBP_MIS_TILED = 2
 };
 typedef struct fbp_mis_params {
     uint32_t num_samples;   /* samples per voxel (reference: L / 4)            */

@@ -146,6 +146,62 @@ def mis(info, I, laser, cdf, alias, prob, noise, num_samples, mode=0) -> np.ndar
     return out
 
 
+def mis_tiled_samples(I, noise, num_samples, tiles):
+    """Scan-point samples of the tile-shared MIS mode (no reference
+    counterpart; restatement of fbp_mis_params FBP_MIS_TILED): per tile t and
+    sample i, LCG state t*S + i (s = s*747796405 + 2891336453 mod 2^32,
+    u = noise[s % n]); u0 < 1/2 picks the first l whose inclusive FP64 prefix
+    of the row sums R exceeds u1 * sum R, else l = int(u1 * L); weight
+    1 / (S * (R_l / sum R + 1/L) / 2).  Returns (l[tiles][S], w[tiles][S])."""
+    I = np.asarray(I, np.float32)
+    noise = np.asarray(noise, np.float32)
+    L = I.shape[0]
+    R = np.where(I > np.float32(1e-5), I, 0).astype(np.float64).sum(axis=1)
+    cum = np.cumsum(R)
+    total = cum[-1]
+    n = len(noise)
+    ls = np.empty((tiles, num_samples), np.int64)
+    ws = np.empty((tiles, num_samples), np.float64)
+    for t in range(tiles):
+        for i in range(num_samples):
+            state = (t * num_samples + i) & 0xFFFFFFFF
+            state = (state * 747796405 + 2891336453) & 0xFFFFFFFF
+            u0 = noise[state % n]
+            state = (state * 747796405 + 2891336453) & 0xFFFFFFFF
+            u1 = noise[state % n]
+            if total > 0 and u0 < np.float32(0.5):
+                l = int(np.searchsorted(cum, float(u1) * total, side="right"))
+                l = min(l, L - 1)
+            else:
+                l = min(int(np.float32(u1) * np.float32(L)), L - 1)
+            pmix = 0.5 * (R[l] / total + 1.0 / L) if total > 0 else 1.0 / L
+            ls[t, i] = l
+            ws[t, i] = np.float32(1.0 / (pmix * num_samples))
+    return ls, ws
+
+
+def mis_tiled(info, I, laser, noise, num_samples) -> np.ndarray:
+    """Tile-shared MIS backprojection: every 16^3 tile (vx, vy, z; x fastest)
+    gathers only its sampled scan points, each with its weight, through the
+    confocal_voxel restatement (the per-tile weighted transient sum_i w_i I[l_i]
+    thresholded first).  FP64 [Z][Y][X]."""
+    I = np.asarray(I, np.float32)
+    Z, Y, X = _res(info)
+    tx, ty, tz = -(-X // 16), -(-Y // 16), -(-Z // 16)
+    ls, ws = mis_tiled_samples(I, noise, num_samples, tx * ty * tz)
+    Ithr = np.where(I > np.float32(1e-5), I, np.float32(0))
+    out = np.zeros((Z, Y, X))
+    for t in range(tx * ty * tz):
+        bx, by, bz = t % tx, (t // tx) % ty, t // (tx * ty)
+        wl = np.zeros(I.shape[0])
+        np.add.at(wl, ls[t], ws[t])
+        It = (Ithr.astype(np.float64) * wl[:, None]).astype(np.float32)
+        vol = confocal_voxel(info, It, laser)
+        sl = (slice(16 * bz, 16 * bz + 16), slice(16 * by, 16 * by + 16), slice(16 * bx, 16 * bx + 16))
+        out[sl] = vol[sl]
+    return out
+
+
 def laplacian(v, k=5) -> np.ndarray:
     """laplacianFilter (FB/transient_postprocessing.cuh:21-50)."""
     v = _f32(v)

@@ -10,8 +10,8 @@ import pytest
 import torch
 
 import oracle
-from fasterbackprojection_amd.api import (LAYOUT_TIME_MAJOR, STORAGE_BF16, STORAGE_F16, Geometry,
-                                          make_info)
+from fasterbackprojection_amd.api import (LAYOUT_TIME_MAJOR, MIS_TILED, STORAGE_BF16, STORAGE_F16,
+                                          Geometry, make_info)
 from fasterbackprojection_amd.scene import small_confocal, wall_grid
 
 pytestmark = pytest.mark.gpu
@@ -299,6 +299,48 @@ def test_mis_reference_mode_matches_oracle(ctx):
                                        cuda(noise), L // 4, mode=0)
     torch.cuda.synchronize()
     assert rel_l2(out.cpu().numpy(), ref) < 1e-4
+
+
+def test_mis_tiled_matches_oracle(ctx):
+    """Tile-shared MIS: same samples and weights as the restatement, exact
+    bins; values either 0 or >= 1e-2 so that weighting cannot cross the
+    1e-5 threshold."""
+    g = geom(n_scan=16, T=256, res=(32, 32, 20))
+    info = make_info(g)
+    rng = np.random.default_rng(41)
+    I = (10.0 ** rng.uniform(-2, 3, size=(256, 256))).astype(np.float32)
+    I[rng.random(I.shape) < 0.3] = 0.0
+    noise = rng.random(4096).astype(np.float32)
+    ref = oracle.mis_tiled(info, I, g.laser_targets, noise, 48)
+    out = ctx.backproject_confocal_mis(info, cuda(I), cuda(g.laser_targets), cuda(noise), 48,
+                                       mode=MIS_TILED)
+    torch.cuda.synchronize()
+    assert_volume_close(out, ref)
+
+
+def test_mis_tiled_is_unbiased(ctx):
+    """The mean over independent noise tables converges to the plain
+    backprojection: its error must be within 4 standard errors estimated
+    from the draws themselves, and shrink with the number of draws."""
+    cap = small_confocal(seed=5, n_scan=16, T=256, res=16)
+    info = make_info(cap.geometry)
+    I = cuda(cap.transient)
+    laser = cuda(cap.geometry.laser_targets)
+    plain = ctx.backproject_confocal(info, I, laser).double()
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(3)
+    draws = []
+    for _ in range(256):
+        noise = torch.rand(4096, generator=gen, device="cuda")
+        draws.append(ctx.backproject_confocal_mis(info, I, laser, noise, 32, mode=MIS_TILED).double())
+    D = torch.stack(draws)
+    mean = D.mean(0)
+    se = D.std(0) / np.sqrt(len(draws))
+    err = float((mean - plain).norm() / plain.norm())
+    bound = float(4 * se.norm() / plain.norm())
+    assert err < bound, (err, bound)
+    err16 = float((D[:16].mean(0) - plain).norm() / plain.norm())
+    assert err < err16
 
 
 def test_rejects_bad_arguments(ctx):
