@@ -279,6 +279,7 @@ def precision_leg(ctx, args):
         got = vol[:nz].cpu().numpy().astype(np.float64)
         err = got - ref
         out[name] = {"ms": float(np.mean(ms)),
+                     "path": "aligned" if ctx.last_confocal_path_aligned() else "general",
                      "voxel_updates_per_s": 128 ** 3 * L / (np.mean(ms) * 1e-3),
                      "rel_l2": float(np.linalg.norm(err) / np.linalg.norm(ref)),
                      "max_abs": float(np.abs(err).max()),

@@ -8,26 +8,30 @@ namespace fbp {
 struct AlignedPlan {
     bool usable;            // host-side conditions hold (the grid check runs on the device)
     uint32_t nx;            // scan grid edge
+    uint32_t R;             // scan points per voxel pitch (1 or 2)
+    uint32_t pn;            // phase plane edge nx / R
+    uint32_t classes;       // offset sign classes (1 for R = 1, 4 for R = 2)
     uint32_t stride;        // LDS row stride (floats)
     size_t lds_bytes;       // dynamic LDS of the convolution kernel
     uint32_t kq, mq, n;     // quadrant extents, n = kq * mq
     uint32_t sort_n;        // n rounded up to a power of two
-    uint32_t zb;            // depth planes per workgroup
-    uint32_t chunks;        // bin chunks per depth group
+    uint32_t chunks;        // bin chunks per (depth, phase)
     size_t bytes_entries, bytes_starts, bytes_work, bytes_tm, bytes_partial;
 };
 
-// Host-side eligibility and sizes: pure chain, F32 storage, square scan grid
-// of 64^2 or 128^2 points with the voxel lattice on the scan lattice along
-// x (X = nx), a multiple of 16 voxels (at most nx) along y, and enough LDS
-// per workgroup.
+// Host-side eligibility and sizes: pure chain; F32 storage (either layout)
+// or F16/BF16 time-fastest; a square scan grid of nx^2 points with X = 64 or
+// 128 voxels along x and nx = X or 2 X (the voxel lattice on every first or
+// second scan point: checked on the device); a multiple of 16 voxels (at
+// most X) along y; enough LDS per workgroup.
 AlignedPlan plan_aligned_confocal(fbp_context ctx, const fbp_reconstruction_info& info,
                                   const fbp_transient& tr);
 
 size_t aligned_scratch_bytes(const AlignedPlan& p);
 
-// Launches the grid check (writes *flag), the offset table, the transpose
-// (time-fastest input) and the convolution + FP64 reduction into volume.
+// Launches the grid check (writes *flag), the offset tables, the reorder
+// into phase planes (unless the input is F32 time-major with R = 1) and the
+// convolution + FP64 reduction into volume.
 // Every kernel after the check returns at once when *flag == 0.
 fbp_status launch_aligned_confocal(fbp_context ctx, const AlignedPlan& p,
                                    const fbp_reconstruction_info& info, const fbp_transient& tr,

@@ -5,32 +5,40 @@
 // sum over scan points l of I'[l][bin(v, l)] * safeRCP(dist^2), with
 // I' = I thresholded at EPS and the bin/dist of the IEEE chain in chain.h.
 //
-// When the scan points form an nx x nx grid on a plane of constant world y
-// and the voxel centres along world x and z sit on the same lattice, the
-// rounded per-axis squares fl(fl(p - l)^2) depend only on |vx - ix| and
-// |vy - iz|.  A device-side check proves that for every (voxel, scan point)
-// pair before anything else runs; then bin and weight are functions of the
-// offset (k, m) = (vx - ix, vy - iz) and of the depth index z only, and the
-// depth plane z of the volume is
-//   vol_z[vy][vx] = sum_b sum_{(k,m) in ring(z,b)} w(k,m,z) * P_b[vy-m][vx-k]
-// where P_b is the plane of bin b of the transient (time-major) and
-// ring(z, b) = {(k, m) : bin(k, m, z) = b}.  Each workgroup keeps whole
-// depth planes of the volume in registers (8x8 voxel tiles, one voxel per
-// lane per tile; a wave owns two tile rows), stages one transient plane at
-// a time in LDS with zero borders (32 columns, 8 rows), and for every ring
-// offset does one LDS read and one FMA per voxel: no per-pair distance,
-// square root or division.  The offsets of a depth are enumerated once per
-// call by a table kernel: quadrant k, m >= 0 (the four sign combinations
-// share bin and weight), sorted by (bin, k / 32, index).  Within a k / 32
-// bucket the column groups (4 tiles = 32 voxels) that can meet the plane
-// are known at compile time, so the inner bodies are straight-line code;
-// tile rows that miss the plane read the wave's own (in-bounds) position
-// with weight 0.
+// The scan points form an nx x nx grid on a plane of constant world y, and
+// the voxel lattice along world x and z has R = 1 or 2 scan pitches.  Split
+// the scan grid into R^2 phase planes (ix = R j + px, iz = R i + pz), each a
+// pn x pn grid (pn = nx / R) with the voxel pitch.  When the rounded
+// per-axis squares fl(fl(p - l)^2) of phase 0 depend only on the signed
+// offset n = vx - j (and phase 1 is its mirror, n -> -n; for R = 1 the
+// square is even in n), bin and weight of a (voxel, scan point) pair are
+// functions of the offset and the depth index z only.  A check kernel
+// proves this in FP32 for every pair before anything else runs.  The depth
+// plane z of the volume is then, summed over the phases p,
+//   vol_z[vy][vx] = sum_p sum_b sum_{(n,m) in ring_p(z,b)} w * P_{p,b}[vy-m][vx-n]
+// where P_{p,b} is phase plane p of bin b of the transient and
+// ring_p(z, b) = {(n, m) : bin = b}.
 //
-// Partial sums per bin chunk are reduced in FP64 in a fixed order, so the
-// result is deterministic.  Whether this path or the general kernel runs is
-// decided on the device by the check kernel's flag: both paths are
-// launched, each returns at once unless the flag selects it.
+// Offsets are enumerated once per geometry by a table kernel: classes of
+// sign (sn, sm) of phase-0 offsets (one class for R = 1, four for R = 2),
+// each over the quadrant (|n|, |m|), sorted by (bin, |n| / 32, index).  An
+// entry applies to four (plane, signed offset) combinations: the four sign
+// combinations on the one plane (R = 1), or one signed offset on each of
+// the four phase planes (R = 2: flipping the phase mirrors the offset).
+//
+// Each workgroup keeps one depth plane of the volume in registers (16
+// waves; wave w owns voxel rows 8w..8w+7 as 8x8 tiles, one voxel per lane
+// per tile), stages one transient phase plane at a time in LDS with zero
+// borders (32 columns, 8 rows), and for every applied offset does one LDS
+// read and one FMA per voxel.  Within an |n| / 32 bucket the column groups
+// (4 tiles = 32 voxels) that can meet the plane are compile-time constants.
+//
+// Partial sums per (phase, bin chunk) are reduced in FP64 in a fixed order:
+// deterministic.  F16/BF16 storage is dequantised when the transient is
+// reordered into phase planes; its sums are compensated (Kahan) across
+// planes.  Whether this path or the general kernel runs is decided on the
+// device by the check kernel's flag: both paths are launched, each returns
+// at once unless the flag selects it.
 #include <cstdlib>
 
 #include "bp_common.h"
@@ -43,13 +51,16 @@ namespace {
 constexpr int kPadRows = 9;    // 1 + 8 zero rows above the data, 8 below
 constexpr int kPadCols = 32;   // zero columns on each side (groups of 4 tiles)
 constexpr int kKeyBits = 14;   // quadrant index bits in the sort key
-constexpr int kBucketBits = 2; // k / 32 bits in the sort key (above the index)
+constexpr int kBucketBits = 2; // |n| / 32 bits in the sort key (above the index)
 constexpr int kBinShift = kKeyBits + kBucketBits;
 
 struct AlignedParams {
     uint32_t nx;            // scan grid edge, L = nx * nx (l = iz * nx + ix)
+    uint32_t R;             // scan points per voxel pitch (1 or 2)
+    uint32_t pn;            // phase plane edge nx / R (= X)
+    uint32_t classes;       // 1 (R = 1) or 4 (R = 2)
     uint32_t X, VY, ZD;     // voxels along vx (world x), vy (world z), z (world y)
-    uint32_t kq, mq;        // quadrant extents max(X, nx), max(VY, nx)
+    uint32_t kq, mq;        // quadrant extents (= pn)
     uint32_t T;
     float t0, dt;
     uint32_t discard;
@@ -63,13 +74,15 @@ __device__ __forceinline__ float pz_of(const AlignedParams& a, uint32_t vy) {
     return voxel_center(a.hmin[2], a.vs[2], vy);
 }
 
-// fl(fl(px - lx)^2) for |vx - ix| = k, from a representative pair.
-__device__ __forceinline__ float x2_ref(const AlignedParams& a, const fbp_vec3* laser, uint32_t k) {
-    return k < a.X ? sq(rn_sub(px_of(a, k), laser[0].x)) : sq(rn_sub(px_of(a, 0), laser[k].x));
+// fl(fl(px - lx)^2) of phase 0 for the signed offset n = vx - j
+// (ix = R j), from a representative pair.
+__device__ __forceinline__ float x2_signed(const AlignedParams& a, const fbp_vec3* laser, int n) {
+    return n >= 0 ? sq(rn_sub(px_of(a, uint32_t(n)), laser[0].x))
+                  : sq(rn_sub(px_of(a, 0), laser[size_t(-n) * a.R].x));
 }
-__device__ __forceinline__ float z2_ref(const AlignedParams& a, const fbp_vec3* laser, uint32_t m) {
-    return m < a.VY ? sq(rn_sub(pz_of(a, m), laser[0].z))
-                    : sq(rn_sub(pz_of(a, 0), laser[size_t(m) * a.nx].z));
+__device__ __forceinline__ float z2_signed(const AlignedParams& a, const fbp_vec3* laser, int m) {
+    return m >= 0 ? sq(rn_sub(pz_of(a, uint32_t(m)), laser[0].z))
+                  : sq(rn_sub(pz_of(a, 0), laser[size_t(-m) * a.R * a.nx].z));
 }
 
 // Device-resident cache header of the offset tables: the parameters and
@@ -80,8 +93,9 @@ struct TableCache {
 };
 
 __device__ __forceinline__ bool same_params(const AlignedParams& x, const AlignedParams& y) {
-    bool eq = x.nx == y.nx && x.X == y.X && x.VY == y.VY && x.ZD == y.ZD && x.kq == y.kq &&
-              x.mq == y.mq && x.T == y.T && x.discard == y.discard;
+    bool eq = x.nx == y.nx && x.R == y.R && x.pn == y.pn && x.classes == y.classes &&
+              x.X == y.X && x.VY == y.VY && x.ZD == y.ZD && x.kq == y.kq && x.mq == y.mq &&
+              x.T == y.T && x.discard == y.discard;
     // Bit patterns: -0 / +0 and NaNs must not compare equal by value.
     eq = eq && __float_as_uint(x.t0) == __float_as_uint(y.t0) &&
          __float_as_uint(x.dt) == __float_as_uint(y.dt);
@@ -120,15 +134,21 @@ __global__ __launch_bounds__(1024) void aligned_check_kernel(const fbp_vec3* __r
                  __float_as_uint(p.z) == __float_as_uint(c.z);
         }
     }
+    // Every pair (voxel column, scan column): phase 0 reads the table at n,
+    // phase 1 (R = 2) at -n; R = 1 needs an even table.
     for (uint32_t q = threadIdx.x; q < a.X * a.nx; q += blockDim.x) {
         const uint32_t vx = q % a.X, ix = q / a.X;
+        const int n = int(vx) - int(ix / a.R);
         const float v = sq(rn_sub(px_of(a, vx), laser[ix].x));
-        good = good && v == x2_ref(a, laser, vx > ix ? vx - ix : ix - vx);
+        good = good && v == x2_signed(a, laser, (ix % a.R) ? -n : n);
+        if (a.R == 1 && n > 0) good = good && v == x2_signed(a, laser, -n);
     }
     for (uint32_t q = threadIdx.x; q < a.VY * a.nx; q += blockDim.x) {
         const uint32_t vy = q % a.VY, iz = q / a.VY;
+        const int m = int(vy) - int(iz / a.R);
         const float v = sq(rn_sub(pz_of(a, vy), laser[size_t(iz) * a.nx].z));
-        good = good && v == z2_ref(a, laser, vy > iz ? vy - iz : iz - vy);
+        good = good && v == z2_signed(a, laser, (iz % a.R) ? -m : m);
+        if (a.R == 1 && m > 0) good = good && v == z2_signed(a, laser, -m);
     }
     if (!good) atomicAnd(&ok, 0);
     if (!eq) atomicAnd(&same, 0);
@@ -140,12 +160,13 @@ __global__ __launch_bounds__(1024) void aligned_check_kernel(const fbp_vec3* __r
     }
 }
 
-// One workgroup per depth index z: bins and weights of the quadrant
-// offsets (k, m) in [0, kq) x [0, mq), sorted by (bin, k / 32, m * kq + k),
-// plus the first entry of every (bin, bucket): starts[z][4 b + q], and
-// starts[z][4 T] = number of offsets whose bin lies inside [0, T).
-// work[z][b] = the convolution work (column groups x sign combinations) of
-// the entries with bin < b, for balancing the bin chunks.
+// One workgroup per (depth index z, sign class c): bins and weights of the
+// class's quadrant offsets (|n|, |m|) in [0, kq) x [0, mq) (signs of class c;
+// the negative side of a class excludes 0), sorted by (bin, |n| / 32,
+// |m| * kq + |n|), plus the first entry of every (bin, bucket):
+// starts[z][c][4 b + q], and starts[z][c][4 T] = number of offsets whose bin
+// lies inside [0, T).  work[z][c][b] = the convolution work (column groups x
+// applications) of the entries with bin < b, for balancing the bin chunks.
 __global__ __launch_bounds__(1024) void aligned_table_kernel(const fbp_vec3* __restrict__ laser,
                                                              AlignedParams a, const int* __restrict__ flag,
                                                              uint32_t sort_n, uint2* __restrict__ entries,
@@ -154,7 +175,8 @@ __global__ __launch_bounds__(1024) void aligned_table_kernel(const fbp_vec3* __r
                                                              TableCache* __restrict__ cache,
                                                              fbp_vec3* __restrict__ cached_laser) {
     if (flag[0] == 0 || flag[1] == 1) return;
-    if (blockIdx.x == 0) {
+    const uint32_t z = blockIdx.x, c = blockIdx.y;
+    if (z == 0 && c == 0) {
         for (uint32_t l = threadIdx.x; l < a.nx * a.nx; l += blockDim.x) cached_laser[l] = laser[l];
         if (threadIdx.x == 0) cache->key = a;
     }
@@ -162,20 +184,22 @@ __global__ __launch_bounds__(1024) void aligned_table_kernel(const fbp_vec3* __r
     uint32_t* wsum = keys + sort_n;
     __shared__ float X2[128], Z2[128];
     __shared__ uint32_t part[1024];
-    const uint32_t z = blockIdx.x;
     const uint32_t tid = threadIdx.x;
-    for (uint32_t k = tid; k < a.kq; k += blockDim.x) X2[k] = x2_ref(a, laser, k);
-    for (uint32_t m = tid; m < a.mq; m += blockDim.x) Z2[m] = z2_ref(a, laser, m);
+    const int sn = (c & 1u) ? -1 : 1, smg = (c & 2u) ? -1 : 1;
+    for (uint32_t k = tid; k < a.kq; k += blockDim.x) X2[k] = x2_signed(a, laser, sn * int(k));
+    for (uint32_t m = tid; m < a.mq; m += blockDim.x) Z2[m] = z2_signed(a, laser, smg * int(m));
     const float Y2 = sq(rn_sub(voxel_center(a.hmin[1], a.vs[1], z), laser[0].y));
     __syncthreads();
     const uint32_t n = a.kq * a.mq;
     for (uint32_t e = tid; e < sort_n; e += blockDim.x) {
         uint32_t key = 0xffffffffu;
         if (e < n) {
-            const float s = dot3(X2[e % a.kq], Y2, Z2[e / a.kq]);
+            const uint32_t k = e % a.kq, m = e / a.kq;
+            const bool skip = (sn < 0 && k == 0) || (smg < 0 && m == 0);
+            const float s = dot3(X2[k], Y2, Z2[m]);
             const int bin = bin_of(confocal_dist(s, a.t0, a.discard, 0.f), a.dt);
-            if (bin >= 0 && bin < int(a.T))
-                key = (uint32_t(bin) << kBinShift) | ((e % a.kq) / 32u) << kKeyBits | e;
+            if (!skip && bin >= 0 && bin < int(a.T))
+                key = (uint32_t(bin) << kBinShift) | (k / 32u) << kKeyBits | e;
         }
         keys[e] = key;
     }
@@ -198,7 +222,8 @@ __global__ __launch_bounds__(1024) void aligned_table_kernel(const fbp_vec3* __r
         }
     }
     const uint32_t groups = a.kq / 32u;
-    uint2* out = entries + size_t(z) * n;
+    const size_t zc = size_t(z) * a.classes + c;
+    uint2* out = entries + zc * n;
     for (uint32_t i = tid; i < sort_n; i += blockDim.x) {
         const uint32_t key = keys[i];
         uint32_t wk = 0;
@@ -210,7 +235,10 @@ __global__ __launch_bounds__(1024) void aligned_table_kernel(const fbp_vec3* __r
                 const float s = dot3(X2[k], Y2, Z2[m]);
                 const float dist = confocal_dist(s, a.t0, a.discard, 0.f);
                 ent = make_uint2(k | (m << 16), __float_as_uint(safe_rcp(rn_mul(dist, dist))));
-                wk = (groups - k / 32u) * (k > 0 ? 2u : 1u) * (m > 0 ? 2u : 1u);
+                // R = 1: up to four sign combinations on the one plane; R = 2:
+                // one application per phase plane (separate workgroups).
+                wk = (groups - k / 32u) *
+                     (a.classes == 1 ? (k > 0 ? 2u : 1u) * (m > 0 ? 2u : 1u) : 1u);
             }
             out[i] = ent;
         }
@@ -250,38 +278,63 @@ __global__ __launch_bounds__(1024) void aligned_table_kernel(const fbp_vec3* __r
             else
                 hi = mid;
         }
-        starts[size_t(z) * (4 * a.T + 1) + i] = lo;
-        if ((i & 3u) == 0) work[size_t(z) * (a.T + 1) + (i >> 2)] = lo == 0 ? 0u : wsum[lo - 1];
+        starts[zc * (4 * a.T + 1) + i] = lo;
+        if ((i & 3u) == 0) work[zc * (a.T + 1) + (i >> 2)] = lo == 0 ? 0u : wsum[lo - 1];
     }
 }
 
-// [L][T] -> [T][L] through 64x64 LDS tiles.
-__global__ __launch_bounds__(256) void transpose_lt_kernel(const float* __restrict__ in,
-                                                           float* __restrict__ out, uint32_t L,
-                                                           uint32_t T, const int* __restrict__ flag) {
+// Transient -> phase planes [T][R^2][pn][pn] (FP32, dequantised; F32 input
+// unthresholded: the convolution kernel thresholds at staging).
+// Time-fastest input [L][T] through 64x64 LDS tiles.
+template <int STORAGE>
+__global__ __launch_bounds__(256) void phase_planes_lt_kernel(const void* __restrict__ in,
+                                                              const float* __restrict__ scales,
+                                                              float* __restrict__ out, uint32_t nx,
+                                                              uint32_t R, uint32_t T,
+                                                              const int* __restrict__ flag) {
     if (*flag == 0) return;
     __shared__ float tile[64][65];
+    const uint32_t L = nx * nx, pn = nx / R;
     const uint32_t t0 = blockIdx.x * 64, l0 = blockIdx.y * 64;
     const uint32_t tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     for (uint32_t r = ty; r < 64; r += 4) {
         const uint32_t l = l0 + r, t = t0 + tx;
-        tile[r][tx] = (l < L && t < T) ? in[size_t(l) * T + t] : 0.f;
+        float v = 0.f;
+        if (l < L && t < T) v = load_sample<STORAGE>(in, size_t(l) * T + t, scales ? scales[l] : 1.f);
+        tile[r][tx] = v;
     }
     __syncthreads();
     for (uint32_t r = ty; r < 64; r += 4) {
         const uint32_t t = t0 + r, l = l0 + tx;
-        if (l < L && t < T) out[size_t(t) * L + l] = tile[tx][r];
+        if (l < L && t < T) {
+            const uint32_t ix = l % nx, iz = l / nx;
+            const uint32_t ph = (iz % R) * R + (ix % R);
+            out[((size_t(t) * R * R + ph) * pn + iz / R) * pn + ix / R] = tile[tx][r];
+        }
     }
+}
+
+// Time-major FP32 input [T][L] -> phase planes (R = 2 only; R = 1 uses the
+// input as is).
+__global__ void phase_planes_tl_kernel(const float* __restrict__ in, float* __restrict__ out,
+                                       uint32_t nx, uint32_t R, uint32_t T,
+                                       const int* __restrict__ flag) {
+    if (*flag == 0) return;
+    const uint32_t L = nx * nx, pn = nx / R;
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(L) * T) return;
+    const uint32_t t = uint32_t(i / L), l = uint32_t(i % L);
+    const uint32_t ix = l % nx, iz = l / nx;
+    const uint32_t ph = (iz % R) * R + (ix % R);
+    out[((size_t(t) * R * R + ph) * pn + iz / R) * pn + ix / R] = in[i];
 }
 
 struct ConvArgs {
     const int* flag;
-    const float* tm;          // transient, [T][L] (time-major)
-    const uint2* entries;     // [ZD][n]
-    const uint32_t* starts;   // [ZD][4 T + 1]
-    const uint32_t* work;     // [ZD][T + 1]
-    float* partial;           // [chunks][ZD][VY][X]
-    uint32_t nx, X, VY, ZD, T, n, chunks, stride;
+    const float* tm;          // phase planes [T][R^2][pn][pn]
+    float* partial;           // [R^2 * chunks][ZD][VY][X]
+    uint32_t pn, R, classes, X, VY, ZD, T, n, chunks, stride;
+    int threshold;            // 1: zero samples <= 1e-5 at staging (F32 input)
 };
 
 // Prefetch registers for one transient plane: 4 float4 per thread (planes
@@ -301,61 +354,107 @@ __device__ __forceinline__ void apply_offset(float (&acc)[TXC], const float* pla
     for (int c = C0; c < C1; ++c) acc[c] = fmaf(w, v[c], acc[c]);
 }
 
-// The entries of one k / 32 bucket Q for one depth: every sign combination
-// of every entry.  For +k the column groups Q..G-1 can meet the plane, for
-// -k the groups 0..G-1-Q (X = nx = 32 G); a group that meets the plane only
-// in part reads zeros from the border.
-template <int TXC, int Q>
-__device__ __forceinline__ void apply_bucket(float (&acc)[TXC], const float* plane,
-                                             const uint2* __restrict__ ent, uint32_t e,
-                                             uint32_t e_end, int lane_row, int y0, int nx,
-                                             int stride) {
-    constexpr int G = TXC / 4;
-    for (; e < e_end; ++e) {
-        const uint2 q = ent[e];
-        const uint32_t km = uint32_t(rfl(int(q.x)));
-        const float w = rfl(__uint_as_float(q.y));
-        const int k = int(km & 0xffffu), m = int(km >> 16);
-        for (int ms = 0; ms < 2; ++ms) {
-            if (ms == 1 && m == 0) break;
-            const int sm = ms ? -m : m;
-            // Does the wave's tile row (voxel rows y0 .. y0+7) meet the plane?
-            if (y0 + 7 - sm < 0 || y0 - sm >= nx) continue;
-            const int a = lane_row - sm * stride;
-            apply_offset<TXC, Q, G - 1>(acc, plane, a - k, w);
-            if (k > 0) apply_offset<TXC, 0, G - 1 - Q>(acc, plane, a + k, w);
+struct EntryRange {
+    const uint2* __restrict__ ent;
+    uint32_t e, e_end;
+};
+
+// Entries loaded per scalar load batch (the table allocation is padded so
+// that a batch may read past the last entry).
+constexpr int kEntryBatch = 2;
+
+// Calls body(k, m, w) for every entry of r in order.  Entries are loaded in
+// batches, so one load latency covers kEntryBatch entries.
+template <typename Body>
+__device__ __forceinline__ void for_each_entry(EntryRange r, Body&& body) {
+    for (uint32_t e = r.e; e < r.e_end; e += kEntryBatch) {
+        uint2 q[kEntryBatch];
+#pragma unroll
+        for (int i = 0; i < kEntryBatch; ++i) q[i] = r.ent[e + i];
+#pragma unroll
+        for (int i = 0; i < kEntryBatch; ++i) {
+            if (i > 0 && e + i >= r.e_end) break;
+            const uint32_t km = uint32_t(rfl(int(q[i].x)));
+            body(int(km & 0xffffu), int(km >> 16), rfl(__uint_as_float(q[i].y)));
         }
     }
 }
 
-template <int TXC>
-__device__ __forceinline__ void apply_plane(float (&acc)[TXC], const float* plane,
+// R = 1: the entries of one |n| / 32 bucket Q, every sign combination of
+// every entry.  For +n the column groups Q..G-1 can meet the plane, for -n
+// the groups 0..G-1-Q (X = pn = 32 G); a group that meets the plane only in
+// part reads zeros from the border.
+template <int TXC, int Q>
+__device__ __forceinline__ void apply_bucket_all_signs(float (&acc)[TXC], const float* plane,
+                                                       EntryRange r, int lane_row, int y0, int pn,
+                                                       int stride) {
+    constexpr int G = TXC / 4;
+    for_each_entry(r, [&](int k, int m, float w) {
+        for (int ms = 0; ms < 2; ++ms) {
+            if (ms == 1 && m == 0) break;
+            const int sm = ms ? -m : m;
+            // Does the wave's tile row (voxel rows y0 .. y0+7) meet the plane?
+            if (y0 + 7 - sm < 0 || y0 - sm >= pn) continue;
+            const int a = lane_row - sm * stride;
+            apply_offset<TXC, Q, G - 1>(acc, plane, a - k, w);
+            if (k > 0) apply_offset<TXC, 0, G - 1 - Q>(acc, plane, a + k, w);
+        }
+    });
+}
+
+// R = 2: the entries of one bucket of one sign class, each applied once to
+// this workgroup's phase plane with the signed offset (NEG ? -|n| : |n|,
+// msign * |m|).
+template <int TXC, int Q, bool NEG>
+__device__ __forceinline__ void apply_bucket_one_sign(float (&acc)[TXC], const float* plane,
+                                                      EntryRange r, int msign, int lane_row,
+                                                      int y0, int pn, int stride) {
+    constexpr int G = TXC / 4;
+    for_each_entry(r, [&](int k, int m, float w) {
+        const int sm = msign * m;
+        if (y0 + 7 - sm < 0 || y0 - sm >= pn) return;
+        const int a = lane_row - sm * stride;
+        if constexpr (NEG)
+            apply_offset<TXC, 0, G - 1 - Q>(acc, plane, a + k, w);
+        else
+            apply_offset<TXC, Q, G - 1>(acc, plane, a - k, w);
+    });
+}
+
+// R = 2: the buckets of the class whose entries land on this phase plane
+// with x sign NEG, m sign msign.
+template <int TXC, bool NEG>
+__device__ __forceinline__ void apply_class(float (&acc)[TXC], const float* plane,
                                             const uint2* __restrict__ ent,
-                                            const uint32_t* __restrict__ sb, int lane_row, int y0,
-                                            int nx, int stride) {
+                                            const uint32_t* __restrict__ sb, int msign,
+                                            int lane_row, int y0, int pn, int stride) {
     constexpr int G = TXC / 4;
     uint32_t bq[5];
 #pragma unroll
     for (int q = 0; q <= 4; ++q) bq[q] = uint32_t(rfl(int(sb[q])));
-    apply_bucket<TXC, 0>(acc, plane, ent, bq[0], bq[1], lane_row, y0, nx, stride);
-    apply_bucket<TXC, 1>(acc, plane, ent, bq[1], bq[2], lane_row, y0, nx, stride);
+    apply_bucket_one_sign<TXC, 0, NEG>(acc, plane, {ent, bq[0], bq[1]}, msign, lane_row, y0, pn,
+                                       stride);
+    apply_bucket_one_sign<TXC, 1, NEG>(acc, plane, {ent, bq[1], bq[2]}, msign, lane_row, y0, pn,
+                                       stride);
     if constexpr (G == 4) {
-        apply_bucket<TXC, 2>(acc, plane, ent, bq[2], bq[3], lane_row, y0, nx, stride);
-        apply_bucket<TXC, 3>(acc, plane, ent, bq[3], bq[4], lane_row, y0, nx, stride);
+        apply_bucket_one_sign<TXC, 2, NEG>(acc, plane, {ent, bq[2], bq[3]}, msign, lane_row, y0,
+                                           pn, stride);
+        apply_bucket_one_sign<TXC, 3, NEG>(acc, plane, {ent, bq[3], bq[4]}, msign, lane_row, y0,
+                                           pn, stride);
     }
 }
 
-// Workgroup: ZB consecutive depth planes x one chunk of bins.  Wave w owns
-// voxel rows 8w .. 8w+7 of those planes (TXC tiles of 8x8, one voxel per
-// lane per tile and depth).
-template <int TXC, int ZB>
+// Workgroup: one depth plane x one (phase, bin chunk).  Wave w owns voxel
+// rows 8w .. 8w+7 (TXC tiles of 8x8, one voxel per lane per tile).  KAHAN:
+// per-plane partial sums are added into a compensated total.
+template <int TXC, bool KAHAN>
 __global__ __launch_bounds__(1024) void aligned_conv_kernel(ConvArgs a,
                                                             const uint2* __restrict__ entries,
                                                             const uint32_t* __restrict__ starts,
                                                             const uint32_t* __restrict__ work) {
-    static_assert(ZB == 1 || ZB == 2 || ZB == 4, "1, 2 or 4 depth planes per workgroup");
     if (*a.flag == 0) return;
     extern __shared__ float plane[];
+    constexpr int G = TXC / 4;
     const uint32_t tid = threadIdx.x;
     const uint32_t nthreads = blockDim.x;
     const int wave = __builtin_amdgcn_readfirstlane(int(tid >> 6));
@@ -364,47 +463,49 @@ __global__ __launch_bounds__(1024) void aligned_conv_kernel(ConvArgs a,
     // Chunk-major order: the workgroups resident at one time work on the
     // same share of every depth's bins, so the planes they stage overlap
     // (L2 reuse across neighbouring depths).
-    const uint32_t groups = a.ZD / ZB;
-    const uint32_t chunk = blockIdx.x / groups, group = blockIdx.x % groups;
-    const uint32_t z0 = group * ZB;
-    const int nx = int(a.nx);
+    const uint32_t phases = a.R * a.R;
+    // (Divisions run on the vector unit: read the uniform results back into
+    // scalar registers so that table addresses stay scalar.)
+    const uint32_t z = uint32_t(rfl(int(blockIdx.x % a.ZD)));
+    const uint32_t cp = uint32_t(rfl(int(blockIdx.x / a.ZD)));  // chunk * phases + phase
+    const uint32_t chunk = uint32_t(rfl(int(cp / phases))), phase = cp - chunk * phases;
+    const int pn = int(a.pn);
     const int stride = int(a.stride);
-    const uint32_t L = a.nx * a.nx;
-    const uint32_t rows = a.nx + 2 * kPadRows - 1;
-    const uint32_t srow = 4 * a.T + 1;  // starts per depth
-    const uint32_t wrow = a.T + 1;      // work prefix per depth
+    const uint32_t P = a.pn * a.pn;
+    const uint32_t rows = a.pn + 2 * kPadRows - 1;
+    const uint32_t srow = 4 * a.T + 1;  // starts per (depth, class)
+    const uint32_t wrow = a.T + 1;      // work prefix per (depth, class)
+    const size_t zc0 = size_t(z) * a.classes;
 
     // Zero the whole plane buffer once: the borders stay zero.
     for (uint32_t i = tid; i < rows * a.stride; i += nthreads) plane[i] = 0.f;
 
-    // Bin range of this chunk: equal shares of the group's convolution work.
-    auto group_work = [&](uint32_t b) {
+    // Bin range of this chunk: equal shares of the depth's convolution work.
+    auto depth_work = [&](uint32_t b) {
         uint32_t w = 0;
-#pragma unroll
-        for (int j = 0; j < ZB; ++j) w += work[size_t(z0 + j) * wrow + b];
+        for (uint32_t c = 0; c < a.classes; ++c) w += work[(zc0 + c) * wrow + b];
         return w;
     };
     auto first_bin_at = [&](uint64_t target) {
-        uint32_t lo = 0, hi = a.T;  // smallest b with group_work(b) >= target
+        uint32_t lo = 0, hi = a.T;  // smallest b with depth_work(b) >= target
         while (lo < hi) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (group_work(mid) < target)
+            if (depth_work(mid) < target)
                 lo = mid + 1;
             else
                 hi = mid;
         }
         return lo;
     };
-    const uint64_t W = group_work(a.T);
+    const uint64_t W = depth_work(a.T);
     const uint32_t b_begin = chunk == 0 ? 0u : first_bin_at(W * chunk / a.chunks);
     const uint32_t b_end = chunk + 1 == a.chunks ? a.T : first_bin_at(W * (chunk + 1) / a.chunks);
 
     auto nonempty = [&](uint32_t b) {
         bool any = false;
-#pragma unroll
-        for (int j = 0; j < ZB; ++j) {
-            const uint32_t* sj = starts + size_t(z0 + j) * srow;
-            any = any || (sj[4 * b + 4] > sj[4 * b]);
+        for (uint32_t c = 0; c < a.classes; ++c) {
+            const uint32_t* sc = starts + (zc0 + c) * srow;
+            any = any || (sc[4 * b + 4] > sc[4 * b]);
         }
         return any;
     };
@@ -413,36 +514,39 @@ __global__ __launch_bounds__(1024) void aligned_conv_kernel(ConvArgs a,
         return b;
     };
 
-    float acc0[TXC], acc1[TXC], acc2[TXC], acc3[TXC];
+    float acc[TXC], sum[TXC], comp[TXC];
 #pragma unroll
-    for (int c = 0; c < TXC; ++c) acc0[c] = acc1[c] = acc2[c] = acc3[c] = 0.f;
+    for (int c = 0; c < TXC; ++c) acc[c] = sum[c] = comp[c] = 0.f;
 
     float4 stage[kStageVec];
     auto load_plane = [&](uint32_t b) {
-        const float4* src = reinterpret_cast<const float4*>(a.tm + size_t(b) * L);
+        const float4* src = reinterpret_cast<const float4*>(a.tm + (size_t(b) * phases + phase) * P);
 #pragma unroll
         for (int i = 0; i < kStageVec; ++i) {
             const uint32_t f = (tid + uint32_t(i) * nthreads) * 4;
-            stage[i] = f < L ? src[f / 4] : make_float4(0.f, 0.f, 0.f, 0.f);
+            stage[i] = f < P ? src[f / 4] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
     };
     auto thr = [](float v) { return v > kEps ? v : 0.f; };
     auto store4 = [&](uint32_t f, float4 v) {
-        const uint32_t iz = f / a.nx, ix = f % a.nx;
-        v.x = thr(v.x);
-        v.y = thr(v.y);
-        v.z = thr(v.z);
-        v.w = thr(v.w);
+        const uint32_t iz = f / a.pn, ix = f % a.pn;
+        if (a.threshold) {
+            v.x = thr(v.x);
+            v.y = thr(v.y);
+            v.z = thr(v.z);
+            v.w = thr(v.w);
+        }
         *reinterpret_cast<float4*>(plane + (kPadRows + iz) * a.stride + kPadCols + ix) = v;
     };
     // Planes that fit the prefetch registers are loaded while the previous
     // plane is processed; larger ones (few threads) are copied directly.
-    const bool prefetch = L <= 4u * kStageVec * nthreads;
+    const bool prefetch = P <= 4u * kStageVec * nthreads;
 
     const int y0 = 8 * wave;
     const int lane_row = (kPadRows + y0 + ly) * stride + kPadCols + lx;
-    auto ent_of = [&](int j) { return entries + size_t(z0 + j) * a.n; };
-    auto st_of = [&](int j) { return starts + size_t(z0 + j) * srow; };
+    // R = 2: the phase of this plane along x and z (1 mirrors the offsets).
+    const int pz = phases > 1 ? int(phase >> 1) : 0, px = int(phase) - pz * int(a.R);
+
     uint32_t b = next_bin(b_begin);
     if (prefetch && b < b_end) load_plane(b);
     while (b < b_end) {
@@ -451,37 +555,69 @@ __global__ __launch_bounds__(1024) void aligned_conv_kernel(ConvArgs a,
 #pragma unroll
             for (int i = 0; i < kStageVec; ++i) {
                 const uint32_t f = (tid + uint32_t(i) * nthreads) * 4;
-                if (f < L) store4(f, stage[i]);
+                if (f < P) store4(f, stage[i]);
             }
         } else {
-            const float4* src = reinterpret_cast<const float4*>(a.tm + size_t(b) * L);
-            for (uint32_t f = tid * 4; f < L; f += 4 * nthreads) store4(f, src[f / 4]);
+            const float4* src =
+                reinterpret_cast<const float4*>(a.tm + (size_t(b) * phases + phase) * P);
+            for (uint32_t f = tid * 4; f < P; f += 4 * nthreads) store4(f, src[f / 4]);
         }
         __syncthreads();
         const uint32_t bn = next_bin(b + 1);
         if (prefetch && bn < b_end) load_plane(bn);
-        apply_plane<TXC>(acc0, plane, ent_of(0), st_of(0) + 4 * b, lane_row, y0, nx, stride);
-        if constexpr (ZB >= 2)
-            apply_plane<TXC>(acc1, plane, ent_of(1), st_of(1) + 4 * b, lane_row, y0, nx, stride);
-        if constexpr (ZB == 4) {
-            apply_plane<TXC>(acc2, plane, ent_of(2), st_of(2) + 4 * b, lane_row, y0, nx, stride);
-            apply_plane<TXC>(acc3, plane, ent_of(3), st_of(3) + 4 * b, lane_row, y0, nx, stride);
+        if (a.classes == 1) {
+            const uint32_t* sb = starts + zc0 * srow + 4 * b;
+            const uint2* ent = entries + zc0 * a.n;
+            uint32_t bq[5];
+#pragma unroll
+            for (int q = 0; q <= 4; ++q) bq[q] = uint32_t(rfl(int(sb[q])));
+            apply_bucket_all_signs<TXC, 0>(acc, plane, {ent, bq[0], bq[1]}, lane_row, y0, pn,
+                                           stride);
+            apply_bucket_all_signs<TXC, 1>(acc, plane, {ent, bq[1], bq[2]}, lane_row, y0, pn,
+                                           stride);
+            if constexpr (G == 4) {
+                apply_bucket_all_signs<TXC, 2>(acc, plane, {ent, bq[2], bq[3]}, lane_row, y0, pn,
+                                               stride);
+                apply_bucket_all_signs<TXC, 3>(acc, plane, {ent, bq[3], bq[4]}, lane_row, y0, pn,
+                                               stride);
+            }
+        } else {
+            // The same sequence of loops on every phase plane: (m sign, n
+            // sign) of the applied offset in a fixed order; the class that
+            // holds those entries is the phase-0 class with the signs the
+            // phase flips (px, pz).
+#pragma unroll
+            for (int zs = 0; zs < 2; ++zs) {
+                const uint32_t cz = uint32_t(zs) ^ uint32_t(pz);
+                const uint32_t cpos = (uint32_t(px) | (cz << 1));
+                const uint32_t cneg = (uint32_t(1 - px) | (cz << 1));
+                const int msign = zs ? -1 : 1;
+                apply_class<TXC, false>(acc, plane, entries + (zc0 + cpos) * a.n,
+                                        starts + (zc0 + cpos) * srow + 4 * b, msign, lane_row, y0,
+                                        pn, stride);
+                apply_class<TXC, true>(acc, plane, entries + (zc0 + cneg) * a.n,
+                                       starts + (zc0 + cneg) * srow + 4 * b, msign, lane_row, y0,
+                                       pn, stride);
+            }
+        }
+        if constexpr (KAHAN) {
+            // Compensated total of the per-plane partial sums.
+#pragma unroll
+            for (int c = 0; c < TXC; ++c) {
+                const float y = acc[c] - comp[c];
+                const float t = sum[c] + y;
+                comp[c] = (t - sum[c]) - y;
+                sum[c] = t;
+                acc[c] = 0.f;
+            }
         }
         b = bn;
     }
 
-    // Partial sums of this chunk for the ZB depth planes.
-    float* out0 = a.partial + ((size_t(chunk) * a.ZD + z0) * a.VY + y0 + ly) * a.X + lx;
-    const size_t zs = size_t(a.VY) * a.X;
+    // Partial sums of this (phase, chunk) for the depth plane.
+    float* out = a.partial + ((size_t(cp) * a.ZD + z) * a.VY + y0 + ly) * a.X + lx;
 #pragma unroll
-    for (int c = 0; c < TXC; ++c) {
-        out0[8 * c] = acc0[c];
-        if constexpr (ZB >= 2) out0[zs + 8 * c] = acc1[c];
-        if constexpr (ZB == 4) {
-            out0[2 * zs + 8 * c] = acc2[c];
-            out0[3 * zs + 8 * c] = acc3[c];
-        }
-    }
+    for (int c = 0; c < TXC; ++c) out[8 * c] = KAHAN ? sum[c] - comp[c] : acc[c];
 }
 
 __global__ void cache_reset_kernel(TableCache* cache) {
@@ -489,7 +625,7 @@ __global__ void cache_reset_kernel(TableCache* cache) {
 }
 
 __global__ void aligned_reduce_kernel(const float* __restrict__ partial, float* __restrict__ volume,
-                                      size_t nvox, uint32_t chunks, const int* __restrict__ flag,
+                                      size_t nvox, uint32_t parts, const int* __restrict__ flag,
                                       TableCache* __restrict__ cache) {
     if (*flag == 0) return;
     // The tables (cached or rebuilt by the table kernel, which has finished)
@@ -498,12 +634,12 @@ __global__ void aligned_reduce_kernel(const float* __restrict__ partial, float* 
     const size_t v = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (v >= nvox) return;
     double s = 0.0;
-    for (uint32_t c = 0; c < chunks; ++c) s += double(partial[size_t(c) * nvox + v]);
+    for (uint32_t c = 0; c < parts; ++c) s += double(partial[size_t(c) * nvox + v]);
     volume[v] = float(s);
 }
 
-uint32_t lds_stride(uint32_t nx) {
-    uint32_t s = nx + 2 * kPadCols;
+uint32_t lds_stride(uint32_t pn) {
+    uint32_t s = pn + 2 * kPadCols;
     while (s % 64 != 8 && s % 64 != 24 && s % 64 != 40 && s % 64 != 56) ++s;
     return s;
 }
@@ -522,60 +658,66 @@ AlignedPlan plan_aligned_confocal(fbp_context ctx, const fbp_reconstruction_info
     p.usable = false;
     if (ctx->disable_aligned) return p;
     const bool pure = info.time_offset == 0.f && info.discard_first_last_bounces != 0;
-    if (!pure || tr.storage != FBP_STORAGE_F32) return p;
+    if (!pure) return p;
+    // F32 in either layout; F16/BF16 time-fastest with per-series scales.
+    if (tr.storage != FBP_STORAGE_F32 && tr.layout != FBP_LAYOUT_TIME_FASTEST) return p;
     const uint32_t L = info.num_laser_targets;
-    uint32_t nx = uint32_t(std::lround(std::sqrt(double(L))));
-    if (nx * nx != L || nx < 8 || nx > 128 || nx % 4 != 0) return p;
+    const uint32_t nx = uint32_t(std::lround(std::sqrt(double(L))));
+    if (nx * nx != L || nx < 16) return p;
     const uint32_t X = info.voxel_resolution.x, VY = info.voxel_resolution.y,
                    ZD = info.voxel_resolution.z;
-    // X = 8 * TXC with TXC in {8, 16}; VY a multiple of 16 up to 128.
-    // X = nx = 32 * (TXC / 4) with TXC in {8, 16}; VY a multiple of 16, at
-    // most nx.
-    if (!(X == 128 || X == 64) || X != nx || VY % 16 != 0 || VY == 0 || VY > nx || ZD == 0)
-        return p;
+    // Phase planes of pn = X = 32 * (TXC / 4) points, TXC in {8, 16}; R = nx / X
+    // scan points per voxel pitch; VY a multiple of 16, at most pn.
+    if (!(X == 128 || X == 64) || nx % X != 0) return p;
+    const uint32_t R = nx / X;
+    if (R != 1 && R != 2) return p;
+    if (VY % 16 != 0 || VY == 0 || VY > X || ZD == 0) return p;
     if (info.num_time_bins == 0 || info.num_time_bins >= (1u << 16)) return p;
     p.nx = nx;
-    p.stride = lds_stride(nx);
-    p.lds_bytes = size_t(nx + 2 * kPadRows - 1) * p.stride * sizeof(float);
+    p.R = R;
+    p.pn = X;
+    p.classes = R == 1 ? 1 : 4;
+    p.stride = lds_stride(p.pn);
+    p.lds_bytes = size_t(p.pn + 2 * kPadRows - 1) * p.stride * sizeof(float);
     if (p.lds_bytes > size_t(ctx->max_lds_per_block)) return p;
-    p.kq = std::max(X, nx);
-    p.mq = std::max(VY, nx);
+    p.kq = p.pn;
+    p.mq = p.pn;
     p.n = p.kq * p.mq;
     p.sort_n = pow2_at_least(p.n);
     if (size_t(p.sort_n) * 8 + 6144 > size_t(ctx->max_lds_per_block)) return p;
-    p.zb = 1;
-    const uint32_t groups = ZD / p.zb;
     // ~8 workgroups per CU over the whole launch (one resident per CU).
     const uint32_t target = 8u * uint32_t(ctx->num_cus > 0 ? ctx->num_cus : 256);
-    p.chunks = std::max(1u, std::min(64u, (target + groups - 1) / groups));
-#ifdef FBP_TUNING_ENV
-    if (const char* e = std::getenv("FBP_ALIGNED_ZB")) p.zb = uint32_t(std::atoi(e));
-    if (const char* e = std::getenv("FBP_ALIGNED_CHUNKS")) p.chunks = uint32_t(std::atoi(e));
-    if (ZD % p.zb) p.zb = 1;
-#endif
+    const uint32_t per_chunk = ZD * R * R;
+    p.chunks = std::max(1u, std::min(64u, (target + per_chunk - 1) / per_chunk));
     const size_t nvox = size_t(X) * VY * ZD;
-    p.bytes_entries = size_t(ZD) * p.n * sizeof(uint2);
-    p.bytes_starts = size_t(ZD) * (4 * size_t(info.num_time_bins) + 1) * sizeof(uint32_t);
-    p.bytes_work = size_t(ZD) * (size_t(info.num_time_bins) + 1) * sizeof(uint32_t);
-    p.bytes_tm = tr.layout == FBP_LAYOUT_TIME_MAJOR ? 0 : size_t(L) * info.num_time_bins * 4;
-    p.bytes_partial = size_t(p.chunks) * nvox * sizeof(float);
+    p.bytes_entries = size_t(ZD) * p.classes * p.n * sizeof(uint2);
+    p.bytes_starts = size_t(ZD) * p.classes * (4 * size_t(info.num_time_bins) + 1) * sizeof(uint32_t);
+    p.bytes_work = size_t(ZD) * p.classes * (size_t(info.num_time_bins) + 1) * sizeof(uint32_t);
+    const bool direct = R == 1 && tr.storage == FBP_STORAGE_F32 && tr.layout == FBP_LAYOUT_TIME_MAJOR;
+    p.bytes_tm = direct ? 0 : size_t(L) * info.num_time_bins * 4;
+    p.bytes_partial = size_t(R) * R * p.chunks * nvox * sizeof(float);
     p.usable = true;
     return p;
 }
 
 static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
+// Readable bytes after the last table entry (batched entry loads).
+static constexpr size_t kEntryPad = 256;
+static_assert(kEntryPad >= (kEntryBatch - 1) * sizeof(uint2), "entry padding");
+
 size_t aligned_scratch_bytes(const AlignedPlan& p) {
     return align256(p.bytes_tm) + align256(p.bytes_partial);
 }
 
-template <int TXC, int ZB>
-static fbp_status launch_conv(fbp_context ctx, const ConvArgs& c, unsigned blocks, unsigned threads,
-                              size_t lds, hipStream_t s) {
-    auto kern = aligned_conv_kernel<TXC, ZB>;
+template <int TXC, bool KAHAN>
+static fbp_status launch_conv(fbp_context ctx, const ConvArgs& c, const uint2* entries,
+                              const uint32_t* starts, const uint32_t* work, unsigned blocks,
+                              unsigned threads, size_t lds, hipStream_t s) {
+    auto kern = aligned_conv_kernel<TXC, KAHAN>;
     FBP_CHECK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, s, c, c.entries, c.starts, c.work);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, s, c, entries, starts, work);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     return FBP_OK;
 }
@@ -586,6 +728,9 @@ fbp_status launch_aligned_confocal(fbp_context ctx, const AlignedPlan& p,
                                    hipStream_t s) {
     AlignedParams a{};
     a.nx = p.nx;
+    a.R = p.R;
+    a.pn = p.pn;
+    a.classes = p.classes;
     a.X = info.voxel_resolution.x;
     a.VY = info.voxel_resolution.y;
     a.ZD = info.voxel_resolution.z;
@@ -603,11 +748,11 @@ fbp_status launch_aligned_confocal(fbp_context ctx, const AlignedPlan& p,
     a.vs[2] = info.hidden_volume_voxel_size.z;
 
     // Offset tables live in their own allocation (kept across calls, rebuilt
-    // only when the parameters or the scan points change); the transposed
-    // transient and the partial sums use the shared scratch.
-    const size_t header = align256(sizeof(TableCache)) + align256(size_t(16384) * sizeof(fbp_vec3));
-    const size_t table_bytes = header + align256(p.bytes_entries) + align256(p.bytes_starts) +
-                               align256(p.bytes_work);
+    // only when the parameters or the scan points change); the phase planes
+    // and the partial sums use the shared scratch.
+    const size_t header = align256(sizeof(TableCache)) + align256(size_t(65536) * sizeof(fbp_vec3));
+    const size_t table_bytes = header + align256(p.bytes_entries + kEntryPad) +
+                               align256(p.bytes_starts) + align256(p.bytes_work);
     if (ctx->aligned_tables_bytes < table_bytes) {
         if (ctx->aligned_tables) {
             FBP_CHECK_HIP(ctx, hipDeviceSynchronize());
@@ -627,7 +772,7 @@ fbp_status launch_aligned_confocal(fbp_context ctx, const AlignedPlan& p,
     fbp_vec3* cached_laser = reinterpret_cast<fbp_vec3*>(tcur + align256(sizeof(TableCache)));
     tcur += header;
     uint2* entries = reinterpret_cast<uint2*>(tcur);
-    tcur += align256(p.bytes_entries);
+    tcur += align256(p.bytes_entries + kEntryPad);
     uint32_t* starts = reinterpret_cast<uint32_t*>(tcur);
     tcur += align256(p.bytes_starts);
     uint32_t* work = reinterpret_cast<uint32_t*>(tcur);
@@ -643,25 +788,38 @@ fbp_status launch_aligned_confocal(fbp_context ctx, const AlignedPlan& p,
     FBP_CHECK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(aligned_table_kernel),
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            int(p.sort_n * 8)));
-    hipLaunchKernelGGL(aligned_table_kernel, dim3(a.ZD), dim3(1024), size_t(p.sort_n) * 8, s, laser,
-                       a, flag, p.sort_n, entries, starts, work, cache, cached_laser);
+    hipLaunchKernelGGL(aligned_table_kernel, dim3(a.ZD, p.classes), dim3(1024), size_t(p.sort_n) * 8,
+                       s, laser, a, flag, p.sort_n, entries, starts, work, cache, cached_laser);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     const uint32_t L = p.nx * p.nx;
-    const float* src = static_cast<const float*>(tr.data);
-    if (tr.layout != FBP_LAYOUT_TIME_MAJOR) {
-        hipLaunchKernelGGL(transpose_lt_kernel, dim3((a.T + 63) / 64, (L + 63) / 64), dim3(256), 0,
-                           s, src, tm, L, a.T, flag);
+    const float* planes = static_cast<const float*>(tr.data);
+    if (p.bytes_tm) {
+        if (tr.layout == FBP_LAYOUT_TIME_MAJOR) {
+            const size_t n = size_t(L) * a.T;
+            hipLaunchKernelGGL(phase_planes_tl_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0,
+                               s, static_cast<const float*>(tr.data), tm, p.nx, p.R, a.T, flag);
+        } else {
+            const dim3 grid((a.T + 63) / 64, (L + 63) / 64);
+            if (tr.storage == FBP_STORAGE_F32)
+                hipLaunchKernelGGL(phase_planes_lt_kernel<FBP_STORAGE_F32>, grid, dim3(256), 0, s,
+                                   tr.data, nullptr, tm, p.nx, p.R, a.T, flag);
+            else if (tr.storage == FBP_STORAGE_F16)
+                hipLaunchKernelGGL(phase_planes_lt_kernel<FBP_STORAGE_F16>, grid, dim3(256), 0, s,
+                                   tr.data, tr.scales, tm, p.nx, p.R, a.T, flag);
+            else
+                hipLaunchKernelGGL(phase_planes_lt_kernel<FBP_STORAGE_BF16>, grid, dim3(256), 0, s,
+                                   tr.data, tr.scales, tm, p.nx, p.R, a.T, flag);
+        }
         FBP_CHECK_HIP(ctx, hipGetLastError());
-        src = tm;
+        planes = tm;
     }
     ConvArgs c{};
     c.flag = flag;
-    c.tm = src;
-    c.entries = entries;
-    c.starts = starts;
-    c.work = work;
+    c.tm = planes;
     c.partial = partial;
-    c.nx = p.nx;
+    c.pn = p.pn;
+    c.R = p.R;
+    c.classes = p.classes;
     c.X = a.X;
     c.VY = a.VY;
     c.ZD = a.ZD;
@@ -669,21 +827,26 @@ fbp_status launch_aligned_confocal(fbp_context ctx, const AlignedPlan& p,
     c.n = p.n;
     c.chunks = p.chunks;
     c.stride = p.stride;
-    const unsigned blocks = (a.ZD / p.zb) * p.chunks;
+    // Quantised storage was thresholded before quantisation.
+    c.threshold = tr.storage == FBP_STORAGE_F32 ? 1 : 0;
+    const bool kahan = tr.storage != FBP_STORAGE_F32;
+    const unsigned blocks = a.ZD * p.chunks * p.R * p.R;
     const unsigned threads = 64u * (a.VY / 8u);
     fbp_status st;
     if (a.X == 128)
-        st = p.zb == 4   ? launch_conv<16, 4>(ctx, c, blocks, threads, p.lds_bytes, s)
-             : p.zb == 2 ? launch_conv<16, 2>(ctx, c, blocks, threads, p.lds_bytes, s)
-                         : launch_conv<16, 1>(ctx, c, blocks, threads, p.lds_bytes, s);
+        st = kahan ? launch_conv<16, true>(ctx, c, entries, starts, work, blocks, threads,
+                                           p.lds_bytes, s)
+                   : launch_conv<16, false>(ctx, c, entries, starts, work, blocks, threads,
+                                            p.lds_bytes, s);
     else
-        st = p.zb == 4   ? launch_conv<8, 4>(ctx, c, blocks, threads, p.lds_bytes, s)
-             : p.zb == 2 ? launch_conv<8, 2>(ctx, c, blocks, threads, p.lds_bytes, s)
-                         : launch_conv<8, 1>(ctx, c, blocks, threads, p.lds_bytes, s);
+        st = kahan ? launch_conv<8, true>(ctx, c, entries, starts, work, blocks, threads,
+                                          p.lds_bytes, s)
+                   : launch_conv<8, false>(ctx, c, entries, starts, work, blocks, threads,
+                                           p.lds_bytes, s);
     if (st != FBP_OK) return st;
     const size_t nvox = size_t(a.X) * a.VY * a.ZD;
     hipLaunchKernelGGL(aligned_reduce_kernel, dim3(unsigned((nvox + 255) / 256)), dim3(256), 0, s,
-                       partial, volume, nvox, p.chunks, flag, cache);
+                       partial, volume, nvox, p.chunks * p.R * p.R, flag, cache);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     return FBP_OK;
 }

@@ -116,12 +116,14 @@ def test_confocal_deterministic(ctx):
 
 
 # ------------------------------------------------------------ aligned grids
-def aligned_geom(nx, wall, X, VY, ZD, T, depth0, dpitch, t0=0.0, discard=1):
-    """Scan grid and voxel lattice coincide along world x and z (pitch
-    wall / nx, both starting at -wall / 2), so the aligned-grid path applies.
-    Voxel sizes follow the reference's size / resolution per component:
-    world x <- X, world y (depth) <- size.y / VY, world z <- size.z / ZD."""
-    pitch = wall / nx
+def aligned_geom(nx, wall, X, VY, ZD, T, depth0, dpitch, t0=0.0, discard=1, R=1):
+    """Scan grid of pitch wall / nx; voxel lattice of pitch R * wall / nx
+    along world x and z, both starting at -wall / 2, so the aligned-grid
+    path applies (R = 2: every voxel centre lies between two scan points,
+    and odd scan columns mirror even ones).  Voxel sizes follow the
+    reference's size / resolution per component: world x <- X, world y
+    (depth) <- size.y / VY, world z <- size.z / ZD."""
+    pitch = R * wall / nx
     scan = wall_grid(nx, wall)
     hmin = (-wall / 2, depth0, -wall / 2)
     hmax = (-wall / 2 + X * pitch, depth0 + VY * dpitch, -wall / 2 + ZD * pitch)
@@ -136,6 +138,11 @@ ALIGNED = {
     "64": dict(nx=64, wall=1.0, X=64, VY=16, ZD=8, T=256, depth0=0.25, dpitch=0.02),
     "64_ragged_depth": dict(nx=64, wall=1.0, X=64, VY=32, ZD=5, T=256, depth0=0.05, dpitch=0.02),
     "128": dict(nx=128, wall=2.0, X=128, VY=16, ZD=4, T=512, depth0=0.5, dpitch=0.03),
+    # Two scan points per voxel pitch: four phase planes per bin.
+    "r2_64": dict(nx=128, wall=1.0, R=2, X=64, VY=16, ZD=6, T=256, depth0=0.25, dpitch=0.02),
+    "r2_64_full_rows": dict(nx=128, wall=1.0, R=2, X=64, VY=64, ZD=2, T=256, depth0=0.05,
+                            dpitch=0.01),
+    "r2_128": dict(nx=256, wall=2.0, R=2, X=128, VY=16, ZD=3, T=512, depth0=0.5, dpitch=0.03),
 }
 
 
@@ -218,6 +225,72 @@ def test_confocal_aligned_geometry_general_chain(ctx):
     g = aligned_geom(**ALIGNED["64"], t0=0.05)
     info = make_info(g)
     rng = np.random.default_rng(34)
+    I = spiky(rng, (len(g.laser_targets), g.num_time_bins))
+    ref = oracle.confocal_voxel(info, I, g.laser_targets)
+    out = ctx.backproject_confocal(info, cuda(I), cuda(g.laser_targets))
+    assert not ctx.last_confocal_path_aligned()
+    assert_volume_close(out, ref)
+
+
+def dequantize(q, sc, storage):
+    """The FP32 samples the kernels read from F16/BF16 storage: the stored
+    value widened to FP32 times the row's FP32 scale (one rounding)."""
+    q = q.cpu().numpy()
+    if storage == STORAGE_F16:
+        v = q.view(np.float16).astype(np.float32)
+    else:
+        v = (q.view(np.uint16).astype(np.uint32) << 16).view(np.float32)
+    return (v * sc.cpu().numpy()[:, None]).astype(np.float32)
+
+
+@pytest.mark.parametrize("name", ["64", "r2_64"])
+@pytest.mark.parametrize("storage", [STORAGE_F16, STORAGE_BF16])
+def test_confocal_aligned_quantized(ctx, name, storage):
+    """F16/BF16 storage on the aligned path (dequantised into phase planes,
+    Kahan-compensated across planes): within FP32 tolerance of the FP64
+    oracle run on the dequantised samples, and of the general kernel on
+    the same storage."""
+    g = aligned_geom(**ALIGNED[name])
+    info = make_info(g)
+    rng = np.random.default_rng(36)
+    I = spiky(rng, (len(g.laser_targets), g.num_time_bins))
+    I[I <= 1e-5] = 0.0            # quantised storage holds thresholded samples
+    q, sc = ctx.quantize(cuda(I), storage)
+    ref = oracle.confocal_voxel(info, dequantize(q, sc, storage), g.laser_targets)
+    out = ctx.backproject_confocal(info, q, cuda(g.laser_targets), storage=storage, scales=sc)
+    assert ctx.last_confocal_path_aligned()
+    assert_volume_close(out, ref)
+    ctx.set_aligned_path(False)
+    try:
+        gen = ctx.backproject_confocal(info, q, cuda(g.laser_targets), storage=storage, scales=sc)
+        assert not ctx.last_confocal_path_aligned()
+    finally:
+        ctx.set_aligned_path(True)
+    assert_volume_close(gen, ref)
+
+
+def test_confocal_aligned_r2_time_major(ctx):
+    g = aligned_geom(**ALIGNED["r2_64"])
+    info = make_info(g)
+    rng = np.random.default_rng(37)
+    I = spiky(rng, (len(g.laser_targets), g.num_time_bins))
+    ref = oracle.confocal_voxel(info, I, g.laser_targets)
+    out = ctx.backproject_confocal(info, cuda(I.T), cuda(g.laser_targets),
+                                   layout=LAYOUT_TIME_MAJOR)
+    assert ctx.last_confocal_path_aligned()
+    assert_volume_close(out, ref)
+
+
+def test_confocal_aligned_r2_rejects_unmirrored_grid(ctx):
+    """Shifting the voxel lattice by a quarter of its pitch puts voxel
+    centres on even scan columns: odd columns no longer mirror even ones,
+    and the device-side check must send the call to the general kernel."""
+    g = aligned_geom(**ALIGNED["r2_64"])
+    vs = (g.hidden_max[0] - g.hidden_min[0]) / 64
+    g.hidden_min = (g.hidden_min[0] - vs / 4, g.hidden_min[1], g.hidden_min[2])
+    g.hidden_max = (g.hidden_max[0] - vs / 4, g.hidden_max[1], g.hidden_max[2])
+    info = make_info(g)
+    rng = np.random.default_rng(38)
     I = spiky(rng, (len(g.laser_targets), g.num_time_bins))
     ref = oracle.confocal_voxel(info, I, g.laser_targets)
     out = ctx.backproject_confocal(info, cuda(I), cuda(g.laser_targets))
