@@ -354,6 +354,25 @@ __device__ __forceinline__ void apply_offset(float (&acc)[TXC], const float* pla
     for (int c = C0; c < C1; ++c) acc[c] = fmaf(w, v[c], acc[c]);
 }
 
+// Two offsets in one straight-line body (all reads issued before the
+// FMAs): a1 over tile columns of groups G1LO..G1HI with weight w1, then a2
+// over G2LO..G2HI with w2.  Per column the FMAs keep the order (a1, a2).
+template <int TXC, int G1LO, int G1HI, int G2LO, int G2HI>
+__device__ __forceinline__ void apply_two(float (&acc)[TXC], const float* plane, int a1, float w1,
+                                          int a2, float w2) {
+    constexpr int C10 = 4 * G1LO, C11 = 4 * G1HI + 4, C20 = 4 * G2LO, C21 = 4 * G2HI + 4;
+    float v1[TXC], v2[TXC];
+#pragma unroll
+    for (int c = C10; c < C11; ++c) v1[c] = plane[a1 + 8 * c];
+#pragma unroll
+    for (int c = C20; c < C21; ++c) v2[c] = plane[a2 + 8 * c];
+#pragma unroll
+    for (int c = 0; c < TXC; ++c) {
+        if (c >= C10 && c < C11) acc[c] = fmaf(w1, v1[c], acc[c]);
+        if (c >= C20 && c < C21) acc[c] = fmaf(w2, v2[c], acc[c]);
+    }
+}
+
 struct EntryRange {
     const uint2* __restrict__ ent;
     uint32_t e, e_end;
@@ -362,23 +381,36 @@ struct EntryRange {
 // Entries loaded per scalar load batch (the table allocation is padded so
 // that a batch may read past the last entry).
 constexpr int kEntryBatch = 2;
+static_assert(kEntryBatch >= 2, "phase-plane walk reads entry pairs");
 
-// Calls body(k, m, w) for every entry of r in order.  Entries are loaded in
-// batches, so one load latency covers kEntryBatch entries.
+// Calls body(k, m, w) for the entries of r in order while |m| <= mmax.
+// Within a (bin, bucket) the entries are sorted by |m| (then |n|), so the
+// first entry with |m| > mmax ends the walk: no later entry can meet the
+// wave's rows.  Entries are loaded in batches, so one load latency covers
+// kEntryBatch entries.
 template <typename Body>
-__device__ __forceinline__ void for_each_entry(EntryRange r, Body&& body) {
+__device__ __forceinline__ void for_each_entry(EntryRange r, int mmax, Body&& body) {
     for (uint32_t e = r.e; e < r.e_end; e += kEntryBatch) {
         uint2 q[kEntryBatch];
 #pragma unroll
         for (int i = 0; i < kEntryBatch; ++i) q[i] = r.ent[e + i];
 #pragma unroll
         for (int i = 0; i < kEntryBatch; ++i) {
-            if (i > 0 && e + i >= r.e_end) break;
+            if (i > 0 && e + i >= r.e_end) return;
             const uint32_t km = uint32_t(rfl(int(q[i].x)));
-            body(int(km & 0xffffu), int(km >> 16), rfl(__uint_as_float(q[i].y)));
+            const int m = int(km >> 16);
+            if (m > mmax) return;
+            body(int(km & 0xffffu), m, rfl(__uint_as_float(q[i].y)));
         }
     }
 }
+
+// Largest |m| for which the offset +|m| (mpos) or -|m| (mneg) moves a plane
+// row into the wave's voxel rows y0 .. y0+7: vy - m in [0, pn) for some
+// vy in that range.
+struct RowReach {
+    int mpos, mneg;
+};
 
 // R = 1: the entries of one |n| / 32 bucket Q, every sign combination of
 // every entry.  For +n the column groups Q..G-1 can meet the plane, for -n
@@ -386,34 +418,31 @@ __device__ __forceinline__ void for_each_entry(EntryRange r, Body&& body) {
 // part reads zeros from the border.
 template <int TXC, int Q>
 __device__ __forceinline__ void apply_bucket_all_signs(float (&acc)[TXC], const float* plane,
-                                                       EntryRange r, int lane_row, int y0, int pn,
+                                                       EntryRange r, int lane_row, RowReach rr,
                                                        int stride) {
     constexpr int G = TXC / 4;
-    for_each_entry(r, [&](int k, int m, float w) {
-        for (int ms = 0; ms < 2; ++ms) {
-            if (ms == 1 && m == 0) break;
-            const int sm = ms ? -m : m;
-            // Does the wave's tile row (voxel rows y0 .. y0+7) meet the plane?
-            if (y0 + 7 - sm < 0 || y0 - sm >= pn) continue;
-            const int a = lane_row - sm * stride;
-            apply_offset<TXC, Q, G - 1>(acc, plane, a - k, w);
-            if (k > 0) apply_offset<TXC, 0, G - 1 - Q>(acc, plane, a + k, w);
+    for_each_entry(r, max(rr.mpos, rr.mneg), [&](int k, int m, float w) {
+        // +k and -k in one body; for k = 0 the second read is weighted 0
+        // (adding +0 leaves every sum unchanged).
+        const float wneg = k > 0 ? w : 0.f;
+        if (m <= rr.mpos) {
+            const int a = lane_row - m * stride;
+            apply_two<TXC, Q, G - 1, 0, G - 1 - Q>(acc, plane, a - k, w, a + k, wneg);
+        }
+        if (m > 0 && m <= rr.mneg) {
+            const int a = lane_row + m * stride;
+            apply_two<TXC, Q, G - 1, 0, G - 1 - Q>(acc, plane, a - k, w, a + k, wneg);
         }
     });
 }
 
-// R = 2: the entries of one bucket of one sign class, each applied once to
-// this workgroup's phase plane with the signed offset (NEG ? -|n| : |n|,
-// msign * |m|).
 template <int TXC, int Q, bool NEG>
 __device__ __forceinline__ void apply_bucket_one_sign(float (&acc)[TXC], const float* plane,
-                                                      EntryRange r, int msign, int lane_row,
-                                                      int y0, int pn, int stride) {
+                                                      EntryRange r, int msign, int mmax,
+                                                      int lane_row, int stride) {
     constexpr int G = TXC / 4;
-    for_each_entry(r, [&](int k, int m, float w) {
-        const int sm = msign * m;
-        if (y0 + 7 - sm < 0 || y0 - sm >= pn) return;
-        const int a = lane_row - sm * stride;
+    for_each_entry(r, mmax, [&](int k, int m, float w) {
+        const int a = lane_row - msign * m * stride;
         if constexpr (NEG)
             apply_offset<TXC, 0, G - 1 - Q>(acc, plane, a + k, w);
         else
@@ -426,21 +455,21 @@ __device__ __forceinline__ void apply_bucket_one_sign(float (&acc)[TXC], const f
 template <int TXC, bool NEG>
 __device__ __forceinline__ void apply_class(float (&acc)[TXC], const float* plane,
                                             const uint2* __restrict__ ent,
-                                            const uint32_t* __restrict__ sb, int msign,
-                                            int lane_row, int y0, int pn, int stride) {
+                                            const uint32_t* __restrict__ sb, int msign, int mmax,
+                                            int lane_row, int stride) {
     constexpr int G = TXC / 4;
     uint32_t bq[5];
 #pragma unroll
     for (int q = 0; q <= 4; ++q) bq[q] = uint32_t(rfl(int(sb[q])));
-    apply_bucket_one_sign<TXC, 0, NEG>(acc, plane, {ent, bq[0], bq[1]}, msign, lane_row, y0, pn,
+    apply_bucket_one_sign<TXC, 0, NEG>(acc, plane, {ent, bq[0], bq[1]}, msign, mmax, lane_row,
                                        stride);
-    apply_bucket_one_sign<TXC, 1, NEG>(acc, plane, {ent, bq[1], bq[2]}, msign, lane_row, y0, pn,
+    apply_bucket_one_sign<TXC, 1, NEG>(acc, plane, {ent, bq[1], bq[2]}, msign, mmax, lane_row,
                                        stride);
     if constexpr (G == 4) {
-        apply_bucket_one_sign<TXC, 2, NEG>(acc, plane, {ent, bq[2], bq[3]}, msign, lane_row, y0,
-                                           pn, stride);
-        apply_bucket_one_sign<TXC, 3, NEG>(acc, plane, {ent, bq[3], bq[4]}, msign, lane_row, y0,
-                                           pn, stride);
+        apply_bucket_one_sign<TXC, 2, NEG>(acc, plane, {ent, bq[2], bq[3]}, msign, mmax, lane_row,
+                                           stride);
+        apply_bucket_one_sign<TXC, 3, NEG>(acc, plane, {ent, bq[3], bq[4]}, msign, mmax, lane_row,
+                                           stride);
     }
 }
 
@@ -544,6 +573,7 @@ __global__ __launch_bounds__(1024) void aligned_conv_kernel(ConvArgs a,
 
     const int y0 = 8 * wave;
     const int lane_row = (kPadRows + y0 + ly) * stride + kPadCols + lx;
+    const RowReach reach{y0 + 7, pn - 1 - y0};
     // R = 2: the phase of this plane along x and z (1 mirrors the offsets).
     const int pz = phases > 1 ? int(phase >> 1) : 0, px = int(phase) - pz * int(a.R);
 
@@ -571,14 +601,14 @@ __global__ __launch_bounds__(1024) void aligned_conv_kernel(ConvArgs a,
             uint32_t bq[5];
 #pragma unroll
             for (int q = 0; q <= 4; ++q) bq[q] = uint32_t(rfl(int(sb[q])));
-            apply_bucket_all_signs<TXC, 0>(acc, plane, {ent, bq[0], bq[1]}, lane_row, y0, pn,
+            apply_bucket_all_signs<TXC, 0>(acc, plane, {ent, bq[0], bq[1]}, lane_row, reach,
                                            stride);
-            apply_bucket_all_signs<TXC, 1>(acc, plane, {ent, bq[1], bq[2]}, lane_row, y0, pn,
+            apply_bucket_all_signs<TXC, 1>(acc, plane, {ent, bq[1], bq[2]}, lane_row, reach,
                                            stride);
             if constexpr (G == 4) {
-                apply_bucket_all_signs<TXC, 2>(acc, plane, {ent, bq[2], bq[3]}, lane_row, y0, pn,
+                apply_bucket_all_signs<TXC, 2>(acc, plane, {ent, bq[2], bq[3]}, lane_row, reach,
                                                stride);
-                apply_bucket_all_signs<TXC, 3>(acc, plane, {ent, bq[3], bq[4]}, lane_row, y0, pn,
+                apply_bucket_all_signs<TXC, 3>(acc, plane, {ent, bq[3], bq[4]}, lane_row, reach,
                                                stride);
             }
         } else {
@@ -592,12 +622,13 @@ __global__ __launch_bounds__(1024) void aligned_conv_kernel(ConvArgs a,
                 const uint32_t cpos = (uint32_t(px) | (cz << 1));
                 const uint32_t cneg = (uint32_t(1 - px) | (cz << 1));
                 const int msign = zs ? -1 : 1;
+                const int mmax = zs ? reach.mneg : reach.mpos;
                 apply_class<TXC, false>(acc, plane, entries + (zc0 + cpos) * a.n,
-                                        starts + (zc0 + cpos) * srow + 4 * b, msign, lane_row, y0,
-                                        pn, stride);
+                                        starts + (zc0 + cpos) * srow + 4 * b, msign, mmax,
+                                        lane_row, stride);
                 apply_class<TXC, true>(acc, plane, entries + (zc0 + cneg) * a.n,
-                                       starts + (zc0 + cneg) * srow + 4 * b, msign, lane_row, y0,
-                                       pn, stride);
+                                       starts + (zc0 + cneg) * srow + 4 * b, msign, mmax, lane_row,
+                                       stride);
             }
         }
         if constexpr (KAHAN) {

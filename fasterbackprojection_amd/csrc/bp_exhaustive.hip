@@ -13,9 +13,15 @@
 // indices.  Pairs are staged through LDS in double-buffered batches: the bin
 // window of each pair (I[l][s][kmin .. kmin+W)) is loaded once per tile,
 // the next batch's windows while the current one is processed.  Laser distances of the lane's voxels are cached while l is
-// unchanged (S consecutive pairs share it); sensor distances come from exact
-// per-axis tables.  Bins use the same fast-quotient / exact-fallback scheme
-// as the confocal kernel.
+// unchanged (S consecutive pairs share it) in bin units (rl / dt).  The
+// fast estimate of the bin quotient is q = rl/dt + sqrt(ss/dt^2) with the
+// squared sensor distance from per-pair axis tables scaled by 1/dt^2, and
+// the weight is 1/q^2 (dt^2 / dist^2; the reduction multiplies by 1/dt^2 in
+// FP64).  Lanes whose q lies within the estimate's error bound of a bin
+// boundary K recompute the distance with the reference's IEEE chain and
+// compare it with the boundary table tk[K] = min{d : (int)(d / dt) >= K}
+// (no division).  Pairs whose tile may come within a few millimetres of
+// the wall or of zero distance take the whole exact chain.
 #include "bp_common.h"
 
 namespace fbp {
@@ -25,19 +31,19 @@ struct ExhaustiveArgs {
     const float* scales;
     const fbp_vec3* laser;
     const fbp_vec3* sensor;
+    const float* thresholds;  // tk[0..T]: bin boundaries in distance (or null)
     float* partial;
     VolumeGeom g;
     uint32_t L, S, T;
-    float t0, dt, inv_dt;
+    float t0, dt, inv_dt, inv_dt2, dt2;
     uint32_t discard;
     float lpos[3], spos[3];
     uint64_t num_pairs;
     uint64_t chunk_len;
     int batch;
     int W;
+    float inv_W;
 };
-
-enum { Q_SX = 0, Q_SY, Q_SZ, Q_BASE, Q_HME, Q_COFF, Q_NEAR, Q_OFF };
 
 // Laser position of the cached laser distances (the exact fallback
 // recomputes the exact squared laser distance from it).
@@ -45,13 +51,27 @@ struct LaserPos {
     float x, y, z;
 };
 
-template <bool NEAR>
-__device__ __forceinline__ void exhaustive_update_tile(float (&acc)[32], const float (&rl)[32],
+// Exact IEEE chain (reference order) for the voxel (x, y, z) and sensor s:
+// the squared laser and sensor distances and the time-of-flight distance.
+__device__ __forceinline__ float exact_dist(float x, float y, float z, LaserPos lp, float sx,
+                                            float sy, float sz, const ExhaustiveArgs& a,
+                                            float off) {
+    const float sl = dot3(sq(rn_sub(x, lp.x)), sq(rn_sub(y, lp.y)), sq(rn_sub(z, lp.z)));
+    const float ss = dot3(sq(rn_sub(x, sx)), sq(rn_sub(y, sy)), sq(rn_sub(z, sz)));
+    return exhaustive_dist(sl, ss, a.t0, a.discard, off);
+}
+
+// One pair applied to the lane's 32 voxels.  rq: laser distances / dt;
+// AQ, ZQ: the pair's per-axis squared sensor distances (x+y, z) / dt^2.
+// NEAR: every voxel takes the whole exact chain (bin and weight).
+template <bool NEAR, bool COFF>
+__device__ __forceinline__ void exhaustive_update_tile(float (&acc)[32], const float (&rq)[32],
                                                        const float (&px)[4], const float (&py)[2],
                                                        const float (&pz)[4], LaserPos lp,
-                                                       const float (&AS)[4][2], const float (&ZS2)[4],
-                                                       const float* win, int base, float hme,
-                                                       float coff, float off,
+                                                       const float (&AQ)[4][2], const float (&ZQ)[4],
+                                                       const float* win, const float* tk,
+                                                       int base, float hme, float coffq,
+                                                       float off, float sx, float sy, float sz,
                                                        const ExhaustiveArgs& a) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -60,23 +80,28 @@ __device__ __forceinline__ void exhaustive_update_tile(float (&acc)[32], const f
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
                 const int slot = (i * 2 + j) * 4 + m;
-                const float ss = AS[i][j] + ZS2[m];
-                const float rs = ss * __builtin_amdgcn_rsqf(ss);
-                const float dist = (rl[slot] + rs) + coff;
-                const float q = dist * a.inv_dt;
+                const float s2 = AQ[i][j] + ZQ[m];
+                float q = fmaf(s2, __builtin_amdgcn_rsqf(s2), rq[slot]);
+                if constexpr (COFF) q = q + coffq;
                 int k = static_cast<int>(q);
-                const float f = __builtin_amdgcn_fractf(q);
-                const float d2 = dist * dist;
-                float w = __builtin_amdgcn_rcpf(d2);
-                bool flag = fabsf(f - 0.5f) > hme;
-                if constexpr (NEAR) flag = flag || !(d2 >= 2.0e-5f) || !(ss >= 1.0e-6f);
-                if (__builtin_expect(flag, 0)) {
-                    // exact squared laser distance in glm::dot's order
-                    const float sl = dot3(sq(rn_sub(px[i], lp.x)), sq(rn_sub(py[j], lp.y)),
-                                          sq(rn_sub(pz[m], lp.z)));
-                    const float de = exhaustive_dist(sl, ss, a.t0, a.discard, off);
+                float w = __builtin_amdgcn_rcpf(q * q);
+                if constexpr (NEAR) {
+                    const float de = exact_dist(px[i], py[j], pz[m], lp, sx, sy, sz, a, off);
                     k = bin_of(de, a.dt);
-                    w = safe_rcp(rn_mul(de, de));
+                    w = safe_rcp(rn_mul(de, de)) * a.dt2;
+                } else {
+                    const float f = __builtin_amdgcn_fractf(q);
+                    if (__builtin_expect(fabsf(f - 0.5f) > hme, 0)) {
+                        // q is within the error bound of the boundary K: the
+                        // exact bin is K - 1 or K.
+                        const float de = exact_dist(px[i], py[j], pz[m], lp, sx, sy, sz, a, off);
+                        if (tk) {
+                            const int K = min(static_cast<int>(q + 0.5f), int(a.T));
+                            k = de >= tk[K] ? K : K - 1;
+                        } else {
+                            k = bin_of(de, a.dt);
+                        }
+                    }
                 }
                 acc[slot] = fmaf(win[base + k], w, acc[slot]);
             }
@@ -84,16 +109,36 @@ __device__ __forceinline__ void exhaustive_update_tile(float (&acc)[32], const f
     }
 }
 
+// Bin boundaries in distance: tk[K] = the smallest d >= 0 with
+// (int)(d / dt) >= K (IEEE division), K = 0..T, by bisection over the
+// ordered bit patterns of non-negative floats.
+__global__ void exhaustive_thresholds_kernel(float dt, uint32_t T, float* __restrict__ tk) {
+    const uint32_t K = blockIdx.x * blockDim.x + threadIdx.x;
+    if (K > T) return;
+    uint32_t lo = 0, hi = __float_as_uint(rn_mul(float(T + 2), dt) * 2.f);
+    // invariant: bin(hi) >= K; find the smallest such bit pattern
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (bin_of(__uint_as_float(mid), dt) >= int(K))
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    tk[K] = __uint_as_float(lo);
+}
+
 // Window prefetch registers per thread: batch * W <= kPrefetch * 256.
 constexpr int kPrefetch = 16;
 constexpr int kWaves = 4;
+// Bin boundary tables up to this many bins live in LDS.
+constexpr uint32_t kMaxTableBins = 8192;
 
 // Pair parameters of one batch slot (computed by one thread, read by all).
 struct PairParams {
     float sx, sy, sz;
     int base;     // slot * W - kmin: LDS index of bin k is base + k
     float hme;    // 0.5 - fast-path tolerance (in bins)
-    float coff;   // off - t0
+    float coffq;  // (off - t0) / dt
     float near;   // 1: every voxel of the tile takes the exact chain
     float off;
     int kmin;
@@ -115,14 +160,17 @@ __device__ __forceinline__ PairParams pair_params(const ExhaustiveArgs& a, uint6
     const float dmax = rlmax + rsmax - a.t0 + off;
     PairParams P;
     P.kmin = int(floorf(dmin * a.inv_dt)) - 3;
+    // Error bound of the fast quotient against fl(dist / dt), in bins: with
+    // 1-ulp v_rsq the pure chain stays within 4.75 * 2^-23 * q and the
+    // offset terms add 3 * 2^-23 * mag / dt; 10 * 2^-23 * mag / dt covers both.
     const float mag = rlmax + rsmax + fabsf(a.t0) + fabsf(off);
-    const float eps = 16.f * 1.1920929e-7f * mag * a.inv_dt + 1.0e-6f;
+    const float eps = 10.f * 1.1920929e-7f * mag * a.inv_dt + 1.0e-6f;
     P.sx = sp[0];
     P.sy = sp[1];
     P.sz = sp[2];
     P.base = slot * a.W - P.kmin;
     P.hme = 0.5f - eps;
-    P.coff = off - a.t0;
+    P.coffq = (off - a.t0) * a.inv_dt;
     P.near = ((dmin < 0.0045f && dmax > -0.0045f) || rlmin < 1.0e-3f || rsmin < 1.0e-3f) ? 1.f : 0.f;
     P.off = off;
     return P;
@@ -133,12 +181,16 @@ __device__ __forceinline__ PairParams pair_params(const ExhaustiveArgs& a, uint6
 // staged bin windows of a batch of (laser, sensor) pairs.  Batches are
 // double-buffered: the windows of batch b+1 are fetched into registers while
 // batch b is processed.
-template <int STORAGE>
+// COFF: the chain has a time offset or per-pair path offsets (q needs the
+// pair's (off - t0) / dt).
+template <int STORAGE, bool COFF>
 __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs a) {
     extern __shared__ float smem[];
-    // [2][batch * W] windows, then [2][batch] PairParams
+    // [2][batch * W] windows, [2][batch] PairParams, then the bin boundaries
+    // tk[0..T] when a.thresholds is set
     float* winbuf = smem;
     PairParams* prmbuf = reinterpret_cast<PairParams*>(smem + 2 * a.batch * a.W);
+    float* tk = a.thresholds ? reinterpret_cast<float*>(prmbuf + 2 * a.batch) : nullptr;
 
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -166,7 +218,7 @@ __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs
     float acc[32];
 #pragma unroll
     for (int i = 0; i < 32; ++i) acc[i] = 0.f;
-    float rl[32];
+    float rq[32];
     LaserPos lp{0.f, 0.f, 0.f};
     uint32_t cur_l = 0xffffffffu;
 
@@ -180,7 +232,8 @@ __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs
             const int f = tid + i * 256;
             float v = 0.f;
             if (f < nb * a.W) {
-                const int jl = f / a.W, w = f % a.W;
+                // f / W through the reciprocal: exact for f < 2^22.
+                const int jl = int((float(f) + 0.5f) * a.inv_W), w = f - jl * a.W;
                 const uint64_t pair = pb + jl;
                 const int k = prm[jl].kmin + w;
                 if (k >= 0 && k < int(a.T)) {
@@ -202,6 +255,8 @@ __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs
     };
 
     float reg[kPrefetch];
+    if (tk)
+        for (uint32_t i = uint32_t(tid); i <= a.T; i += 256) tk[i] = a.thresholds[i];
     // Prologue: batch 0 into buffer 0.
     if (p_begin < p_end) {
         const int nb = int(min(uint64_t(a.batch), p_end - p_begin));
@@ -246,34 +301,36 @@ __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs
 #pragma unroll
                         for (int m = 0; m < 4; ++m) {
                             const float sl = dot3(XL2[i], YL2[j], ZL2[m]);
-                            rl[(i * 2 + j) * 4 + m] = sl * __builtin_amdgcn_rsqf(sl);
+                            rq[(i * 2 + j) * 4 + m] =
+                                sl * __builtin_amdgcn_rsqf(sl) * a.inv_dt;
                         }
             }
             const PairParams& P = prm[jl];
             const float sxp = rfl(P.sx), syp = rfl(P.sy), szp = rfl(P.sz);
             const int base = rfl(P.base) + cur * per;
             const float hme = rfl(P.hme);
-            const float coff = rfl(P.coff);
+            const float coffq = rfl(P.coffq);
             const bool near = rfl(P.near) != 0.f;
             const float off = rfl(P.off);
-            float XS2[4], YS2[2], ZS2[4], AS[4][2];
+            // Per-axis squared sensor distances / dt^2 (x + y, z).
+            float XS2[4], YS2[2], ZQ[4], AQ[4][2];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 XS2[i] = sq(rn_sub(px[i], sxp));
-                ZS2[i] = sq(rn_sub(pz[i], szp));
+                ZQ[i] = sq(rn_sub(pz[i], szp)) * a.inv_dt2;
             }
 #pragma unroll
             for (int j = 0; j < 2; ++j) YS2[j] = sq(rn_sub(py[j], syp));
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j) AS[i][j] = rn_add(XS2[i], YS2[j]);
+                for (int j = 0; j < 2; ++j) AQ[i][j] = (XS2[i] + YS2[j]) * a.inv_dt2;
             if (near)
-                exhaustive_update_tile<true>(acc, rl, px, py, pz, lp, AS, ZS2, winbuf, base, hme, coff,
-                                             off, a);
+                exhaustive_update_tile<true, COFF>(acc, rq, px, py, pz, lp, AQ, ZQ, winbuf, tk, base,
+                                                   hme, coffq, off, sxp, syp, szp, a);
             else
-                exhaustive_update_tile<false>(acc, rl, px, py, pz, lp, AS, ZS2, winbuf, base, hme, coff,
-                                              off, a);
+                exhaustive_update_tile<false, COFF>(acc, rq, px, py, pz, lp, AQ, ZQ, winbuf, tk, base,
+                                                    hme, coffq, off, sxp, syp, szp, a);
         }
         if (has_next) store(winbuf + (cur ^ 1) * per, nbn, reg);
         __syncthreads();
@@ -331,6 +388,8 @@ extern "C" fbp_status fbp_backproject_exhaustive_voxel(fbp_context ctx,
     a.t0 = info->time_offset;
     a.dt = info->time_step;
     a.inv_dt = 1.0f / info->time_step;
+    a.inv_dt2 = a.inv_dt * a.inv_dt;
+    a.dt2 = info->time_step * info->time_step;
     a.discard = info->discard_first_last_bounces;
     a.lpos[0] = info->laser_position.x;
     a.lpos[1] = info->laser_position.y;
@@ -346,6 +405,7 @@ extern "C" fbp_status fbp_backproject_exhaustive_voxel(fbp_context ctx,
     // Pairs per batch: the window prefetch registers hold one batch.
     a.batch = 16;
     while (a.batch > 1 && a.batch * a.W > kPrefetch * 256) a.batch /= 2;
+    a.inv_W = 1.0f / float(a.W);
     // ~8 workgroups (32 waves) per CU over the launch.
     const uint32_t target = 8u * uint32_t(ctx->num_cus > 0 ? ctx->num_cus : 256);
     uint64_t nc = (target + a.g.num_tiles - 1) / a.g.num_tiles;
@@ -355,21 +415,38 @@ extern "C" fbp_status fbp_backproject_exhaustive_voxel(fbp_context ctx,
     a.chunk_len = (a.num_pairs + nc - 1) / nc;
     const uint64_t chunks = (a.num_pairs + a.chunk_len - 1) / a.chunk_len;
     const size_t partial_bytes = size_t(chunks) * a.g.num_tiles * a.g.slots * 64 * sizeof(float);
-    FBP_CHECK(ensure_scratch(ctx, partial_bytes));
+    // Bin boundary table (in LDS) for T < kMaxTableBins; larger T divides.
+    const bool table = a.T < kMaxTableBins;
+    const size_t table_off = (partial_bytes + 255) & ~size_t(255);
+    FBP_CHECK(ensure_scratch(ctx, table_off + (table ? (size_t(a.T) + 1) * sizeof(float) : 0)));
     a.partial = static_cast<float*>(ctx->scratch);
+    hipStream_t s = as_stream(stream);
+    if (table) {
+        float* tk = reinterpret_cast<float*>(static_cast<char*>(ctx->scratch) + table_off);
+        hipLaunchKernelGGL(exhaustive_thresholds_kernel, dim3((a.T + 1 + 255) / 256), dim3(256), 0,
+                           s, a.dt, a.T, tk);
+        FBP_CHECK_HIP(ctx, hipGetLastError());
+        a.thresholds = tk;
+    }
 
     const dim3 grid(unsigned(chunks * a.g.num_tiles));
-    const size_t shmem = 2 * (size_t(a.batch) * a.W * sizeof(float) + a.batch * sizeof(PairParams));
-    hipStream_t s = as_stream(stream);
+    const size_t shmem = 2 * (size_t(a.batch) * a.W * sizeof(float) + a.batch * sizeof(PairParams)) +
+                         (table ? (size_t(a.T) + 1) * sizeof(float) : 0);
+    const bool coff = !(a.t0 == 0.f && a.discard != 0);
+    auto launch = [&](auto kern) {
+        hipLaunchKernelGGL(kern, grid, dim3(64 * kWaves), shmem, s, a);
+    };
     if (tr->storage == FBP_STORAGE_F32)
-        hipLaunchKernelGGL(exhaustive_voxel_kernel<FBP_STORAGE_F32>, grid, dim3(64 * kWaves), shmem,
-                           s, a);
+        coff ? launch(exhaustive_voxel_kernel<FBP_STORAGE_F32, true>)
+             : launch(exhaustive_voxel_kernel<FBP_STORAGE_F32, false>);
     else if (tr->storage == FBP_STORAGE_F16)
-        hipLaunchKernelGGL(exhaustive_voxel_kernel<FBP_STORAGE_F16>, grid, dim3(64 * kWaves), shmem,
-                           s, a);
+        coff ? launch(exhaustive_voxel_kernel<FBP_STORAGE_F16, true>)
+             : launch(exhaustive_voxel_kernel<FBP_STORAGE_F16, false>);
     else
-        hipLaunchKernelGGL(exhaustive_voxel_kernel<FBP_STORAGE_BF16>, grid, dim3(64 * kWaves), shmem,
-                           s, a);
+        coff ? launch(exhaustive_voxel_kernel<FBP_STORAGE_BF16, true>)
+             : launch(exhaustive_voxel_kernel<FBP_STORAGE_BF16, false>);
     FBP_CHECK_HIP(ctx, hipGetLastError());
-    return launch_reduce(ctx, a.partial, nullptr, volume, a.g, uint32_t(chunks), 1.0, s);
+    // The fast weights carry dt^2 (1 / q^2): scale by 1 / dt^2 in FP64.
+    return launch_reduce(ctx, a.partial, nullptr, volume, a.g, uint32_t(chunks),
+                         1.0 / (double(a.dt) * double(a.dt)), s);
 }
