@@ -107,57 +107,57 @@ __device__ __forceinline__ bool same_params(const AlignedParams& x, const Aligne
 
 // flag[0] = 1 iff the grid conditions hold for every scan point and pair.
 // flag[1] = 1 iff the cached tables were built for these parameters and
-// exactly these scan points (then the table kernel is skipped); when
-// flag[0] = 1 and flag[1] = 0 the cache is marked invalid until rebuilt.
-__global__ __launch_bounds__(1024) void aligned_check_kernel(const fbp_vec3* __restrict__ laser,
-                                                             AlignedParams a, int* __restrict__ flag,
-                                                             TableCache* __restrict__ cache,
-                                                             const fbp_vec3* __restrict__ cached_laser) {
-    __shared__ int ok, same;
+// exactly these scan points (then the table kernel is skipped).  The init
+// kernel sets both from the cache header; the check kernel's workgroups
+// clear them for any failing item.
+__global__ void aligned_check_init_kernel(AlignedParams a, int* __restrict__ flag,
+                                          const TableCache* __restrict__ cache) {
     if (threadIdx.x == 0) {
-        ok = 1;
-        same = cache->valid == 1 && same_params(cache->key, a);
+        flag[0] = 1;
+        flag[1] = cache->valid == 1 && same_params(cache->key, a);
     }
-    __syncthreads();
-    bool good = true, eq = true;
-    const uint32_t L = a.nx * a.nx;
+}
+
+// Items: scan points l < L (on the grid; equal to the cached copy), then
+// (voxel column, scan column) pairs, then (voxel row, scan row) pairs.
+__global__ __launch_bounds__(256) void aligned_check_kernel(const fbp_vec3* __restrict__ laser,
+                                                            AlignedParams a, int* __restrict__ flag,
+                                                            const fbp_vec3* __restrict__ cached_laser) {
+    const uint32_t L = a.nx * a.nx, NX = a.X * a.nx, NZ = a.VY * a.nx;
     const float y0 = laser[0].y;
-    const bool cmp = same != 0;
-    for (uint32_t l = threadIdx.x; l < L; l += blockDim.x) {
-        const uint32_t ix = l % a.nx, iz = l / a.nx;
-        const fbp_vec3 p = laser[l];
-        good = good && p.y == y0 && p.x == laser[ix].x && p.z == laser[size_t(iz) * a.nx].z;
-        if (cmp) {
-            const fbp_vec3 c = cached_laser[l];
-            eq = eq && __float_as_uint(p.x) == __float_as_uint(c.x) &&
-                 __float_as_uint(p.y) == __float_as_uint(c.y) &&
-                 __float_as_uint(p.z) == __float_as_uint(c.z);
+    const bool cmp = flag[1] != 0;
+    bool good = true, eq = true;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < L + NX + NZ;
+         t += gridDim.x * blockDim.x) {
+        if (t < L) {
+            const uint32_t ix = t % a.nx, iz = t / a.nx;
+            const fbp_vec3 p = laser[t];
+            good = good && p.y == y0 && p.x == laser[ix].x && p.z == laser[size_t(iz) * a.nx].z;
+            if (cmp) {
+                const fbp_vec3 c = cached_laser[t];
+                eq = eq && __float_as_uint(p.x) == __float_as_uint(c.x) &&
+                     __float_as_uint(p.y) == __float_as_uint(c.y) &&
+                     __float_as_uint(p.z) == __float_as_uint(c.z);
+            }
+        } else if (t < L + NX) {
+            // Every pair (voxel column, scan column): phase 0 reads the table
+            // at n, phase 1 (R = 2) at -n; R = 1 needs an even table.
+            const uint32_t q = t - L, vx = q % a.X, ix = q / a.X;
+            const int n = int(vx) - int(ix / a.R);
+            const float v = sq(rn_sub(px_of(a, vx), laser[ix].x));
+            good = good && v == x2_signed(a, laser, (ix % a.R) ? -n : n);
+            if (a.R == 1 && n > 0) good = good && v == x2_signed(a, laser, -n);
+        } else {
+            const uint32_t q = t - L - NX, vy = q % a.VY, iz = q / a.VY;
+            const int m = int(vy) - int(iz / a.R);
+            const float v = sq(rn_sub(pz_of(a, vy), laser[size_t(iz) * a.nx].z));
+            good = good && v == z2_signed(a, laser, (iz % a.R) ? -m : m);
+            if (a.R == 1 && m > 0) good = good && v == z2_signed(a, laser, -m);
         }
     }
-    // Every pair (voxel column, scan column): phase 0 reads the table at n,
-    // phase 1 (R = 2) at -n; R = 1 needs an even table.
-    for (uint32_t q = threadIdx.x; q < a.X * a.nx; q += blockDim.x) {
-        const uint32_t vx = q % a.X, ix = q / a.X;
-        const int n = int(vx) - int(ix / a.R);
-        const float v = sq(rn_sub(px_of(a, vx), laser[ix].x));
-        good = good && v == x2_signed(a, laser, (ix % a.R) ? -n : n);
-        if (a.R == 1 && n > 0) good = good && v == x2_signed(a, laser, -n);
-    }
-    for (uint32_t q = threadIdx.x; q < a.VY * a.nx; q += blockDim.x) {
-        const uint32_t vy = q % a.VY, iz = q / a.VY;
-        const int m = int(vy) - int(iz / a.R);
-        const float v = sq(rn_sub(pz_of(a, vy), laser[size_t(iz) * a.nx].z));
-        good = good && v == z2_signed(a, laser, (iz % a.R) ? -m : m);
-        if (a.R == 1 && m > 0) good = good && v == z2_signed(a, laser, -m);
-    }
-    if (!good) atomicAnd(&ok, 0);
-    if (!eq) atomicAnd(&same, 0);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        flag[0] = ok;
-        flag[1] = ok && same;
-        if (ok && !same) cache->valid = 0;
-    }
+    // One atomic per wave that found a failing item.
+    if (__any(!good) && (threadIdx.x & 63) == 0) atomicAnd(&flag[0], 0);
+    if (__any(!eq) && (threadIdx.x & 63) == 0) atomicAnd(&flag[1], 0);
 }
 
 // One workgroup per (depth index z, sign class c): bins and weights of the
@@ -177,6 +177,9 @@ __global__ __launch_bounds__(1024) void aligned_table_kernel(const fbp_vec3* __r
     if (flag[0] == 0 || flag[1] == 1) return;
     const uint32_t z = blockIdx.x, c = blockIdx.y;
     if (z == 0 && c == 0) {
+        // Rebuild: the cache stays invalid until the reduction kernel (after
+        // every table workgroup has finished) marks it valid.
+        if (threadIdx.x == 0) cache->valid = 0;
         for (uint32_t l = threadIdx.x; l < a.nx * a.nx; l += blockDim.x) cached_laser[l] = laser[l];
         if (threadIdx.x == 0) cache->key = a;
     }
@@ -813,9 +816,15 @@ fbp_status launch_aligned_confocal(fbp_context ctx, const AlignedPlan& p,
     cur += align256(p.bytes_tm);
     float* partial = reinterpret_cast<float*>(cur);
 
-    hipLaunchKernelGGL(aligned_check_kernel, dim3(1), dim3(1024), 0, s, laser, a, flag, cache,
-                       cached_laser);
+    hipLaunchKernelGGL(aligned_check_init_kernel, dim3(1), dim3(64), 0, s, a, flag, cache);
     FBP_CHECK_HIP(ctx, hipGetLastError());
+    {
+        const uint32_t items = p.nx * p.nx + (a.X + a.VY) * p.nx;
+        const unsigned blocks = std::min(256u, (items + 255) / 256);
+        hipLaunchKernelGGL(aligned_check_kernel, dim3(blocks), dim3(256), 0, s, laser, a, flag,
+                           cached_laser);
+        FBP_CHECK_HIP(ctx, hipGetLastError());
+    }
     FBP_CHECK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(aligned_table_kernel),
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            int(p.sort_n * 8)));
