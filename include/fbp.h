@@ -128,7 +128,16 @@ fbp_status fbp_context_last_confocal_path(fbp_context ctx, int32_t* aligned);
  * volume: float[Z][Y][X] (X,Y,Z = voxel_resolution), OVERWRITTEN with
  *   sum_l [I > 1e-5] * I(l, bin) * safeRCP(dist^2).
  * laser_targets: fbp_vec3[L].  Deterministic (no atomics): per-chunk partial
- * sums are reduced in a fixed order in FP64. */
+ * sums are reduced in a fixed order in FP64.
+ * Two implementations, chosen on the device per call: the aligned-grid path
+ * (a ring convolution over transient planes) runs when t0 = 0 with bounces
+ * discarded, the scan points form an n x n grid on a plane of constant y,
+ * X = 64 or 128 voxels along x with n = X or 2X, Y a multiple of 16 (at most
+ * X), storage F32 (either layout) or F16/BF16 time-fastest, and a device
+ * check proves in FP32 that the per-axis squared distances depend only on
+ * the voxel-to-scan-point offset (mirrored for odd scan columns when
+ * n = 2X); otherwise the general kernel runs.  Both give the bins of the
+ * IEEE chain; results agree within the FP32 tolerance. */
 fbp_status fbp_backproject_confocal_voxel(fbp_context ctx, const fbp_reconstruction_info* info,
                                           const fbp_transient* transient,
                                           const fbp_vec3* laser_targets, float* volume,
