@@ -24,6 +24,7 @@ STORAGE_F32, STORAGE_F16, STORAGE_BF16 = 0, 1, 2
 LAYOUT_TIME_FASTEST, LAYOUT_TIME_MAJOR = 0, 1
 MIS_REFERENCE, MIS_UNBIASED, MIS_TILED = 0, 1, 2
 OPTION_CONFOCAL_ALIGNED = 0
+OPTION_VOXEL_SCATTERING = 1
 
 
 class Vec3(C.Structure):

@@ -123,6 +123,19 @@ class Context:
         self.check(self.lib.fbp_context_set_option(self.handle, _lib.OPTION_CONFOCAL_ALIGNED,
                                                    int(bool(enabled))))
 
+    def set_voxel_scattering(self, enabled: bool) -> None:
+        """Voxel-scattering variant of the confocal voxel and MIS
+        backprojections (the reference's ACCUMULATE_VOXEL_SCATTERING switch,
+        FB/transient_reconstruction.cuh:112-126, 235-250): every bin a voxel
+        reaches adds its sample.  Off by default."""
+        self.check(self.lib.fbp_context_set_option(self.handle, _lib.OPTION_VOXEL_SCATTERING,
+                                                   int(bool(enabled))))
+
+    def option(self, option: int) -> int:
+        v = C.c_int32()
+        self.check(self.lib.fbp_context_get_option(self.handle, option, C.byref(v)))
+        return int(v.value)
+
     def last_confocal_path_aligned(self) -> bool:
         """True when the last confocal voxel backprojection took the
         aligned-grid path (synchronises)."""

@@ -143,4 +143,11 @@ fbp_status confocal_voxel_impl(fbp_context ctx, const fbp_reconstruction_info* i
                                float* volume, void* stream, const uint32_t* sample_l,
                                const float* sample_w, uint32_t samples);
 
+// Voxel-scattering variant of the confocal voxel backprojection
+// (bp_scatter.hip); the same sample-list convention as confocal_voxel_impl.
+fbp_status confocal_scatter_impl(fbp_context ctx, const fbp_reconstruction_info* info,
+                                 const fbp_transient* tr, const fbp_vec3* laser_targets,
+                                 float* volume, hipStream_t s, const uint32_t* sample_l,
+                                 const float* sample_w, uint32_t samples);
+
 }  // namespace fbp

@@ -396,6 +396,11 @@ fbp_status confocal_voxel_impl(fbp_context ctx, const fbp_reconstruction_info* i
         !(tr->layout == FBP_LAYOUT_TIME_MAJOR && tr->storage == FBP_STORAGE_F32))
         return fail(ctx, FBP_ERR_UNSUPPORTED, "confocal: time-major layout is F32 only");
     FBP_CHECK(bind(ctx));
+    if (ctx->voxel_scattering) {
+        ctx->path_flag_valid = false;
+        return confocal_scatter_impl(ctx, info, tr, laser_targets, volume, as_stream(stream),
+                                     sample_l, sample_w, samples);
+    }
 
     ConfocalArgs a{};
     a.data = tr->data;

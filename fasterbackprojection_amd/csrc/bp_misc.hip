@@ -81,7 +81,8 @@ __global__ void mis_kernel(const float* __restrict__ I, const fbp_vec3* __restri
                            const float* __restrict__ prob, const float* __restrict__ noise,
                            uint32_t noise_size, uint32_t num_samples, uint32_t mode, VolumeGeom g,
                            uint32_t L, uint32_t T, float t0, float dt, uint32_t discard,
-                           fbp_vec3 lpos, fbp_vec3 spos, float* __restrict__ volume) {
+                           fbp_vec3 lpos, fbp_vec3 spos, uint32_t scatter, float ext,
+                           float* __restrict__ volume) {
     const uint32_t slice = g.res[0] * g.res[1];
     const size_t vid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (vid >= size_t(slice) * g.res[2]) return;
@@ -118,21 +119,33 @@ __global__ void mis_kernel(const float* __restrict__ I, const fbp_vec3* __restri
         if (discard == 0)
             dist = rn_add(dist, rn_add(distance3(lx, ly, lz, lpos.x, lpos.y, lpos.z),
                                        distance3(lx, ly, lz, spos.x, spos.y, spos.z)));
+        const float wd = safe_rcp(rn_mul(dist, dist));
+        // Weight of one sample of l: reference MIS weight, or the one-sample
+        // estimator over the mixture p = (p_alias + 1/L) / 2 (unbiased:
+        // E[f(l) / p(l)] / N = sum_l f(l) / N).
+        auto add = [&](float light) {
+            if (mode == FBP_MIS_REFERENCE) {
+                const float mis_weight = rn_mul(pdf, safe_rcp(gpdf));
+                sum += rn_mul(rn_mul(mis_weight, light), wd);
+            } else {
+                const float pmix = 0.5f * (pa + invL);
+                sum += (light * wd) / (pmix * float(num_samples));
+            }
+        };
+        if (scatter) {
+            // Voxel-scattering walk (FB/transient_reconstruction.cuh:235-250).
+            scatter_walk(dist, ext, dt, T, [&](int bin) {
+                const float light = I[size_t(l) * T + uint32_t(bin)];
+                if (light > kEps) add(light);
+            });
+            continue;
+        }
         const float q = rn_div(dist, dt);
         if (!(q > -1.0f) || !(q < float(T))) continue;
         const int bin = int(q);
         const float light = I[size_t(l) * T + bin];
         if (!(light > kEps)) continue;
-        const float wd = safe_rcp(rn_mul(dist, dist));
-        if (mode == FBP_MIS_REFERENCE) {
-            const float mis_weight = rn_mul(pdf, safe_rcp(gpdf));
-            sum += rn_mul(rn_mul(mis_weight, light), wd);
-        } else {
-            // One-sample MIS over the mixture p = (p_alias + 1/L) / 2:
-            // E[f(l) / p(l)] / N = sum_l f(l) / N.
-            const float pmix = 0.5f * (pa + invL);
-            sum += (light * wd) / (pmix * float(num_samples));
-        }
+        add(light);
     }
     volume[vid] = sum;
 }
@@ -549,7 +562,12 @@ fbp_status fbp_backproject_confocal_mis(fbp_context ctx, const fbp_reconstructio
                        static_cast<const float*>(tr->data), laser_targets, d_cdf, d_alias, d_prob,
                        params->noise, params->noise_size, params->num_samples, params->mode, g, L,
                        T, info->time_offset, info->time_step, info->discard_first_last_bounces,
-                       info->laser_position, info->sensor_position, volume);
+                       info->laser_position, info->sensor_position,
+                       uint32_t(ctx->voxel_scattering),
+                       voxel_extent(info->hidden_volume_voxel_size.x,
+                                    info->hidden_volume_voxel_size.y,
+                                    info->hidden_volume_voxel_size.z),
+                       volume);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     // The host vectors above are consumed by the async copies: wait for them.
     FBP_CHECK_HIP(ctx, hipStreamSynchronize(s));

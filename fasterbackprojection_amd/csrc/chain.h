@@ -72,6 +72,32 @@ FBP_HD int bin_of(float dist, float dt) {
     return static_cast<int>(q);
 }
 
+// Half the voxel diagonal, glm::length(voxel size) / 2
+// (FB/transient_reconstruction.cuh:113).
+FBP_HD float voxel_extent(float vx, float vy, float vz) {
+    return rn_div(rn_sqrt(dot3(sq(vx), sq(vy), sq(vz))), 2.0f);
+}
+
+// Bins visited by the voxel-scattering walk of one (voxel, scan point) pair
+// (FB/transient_reconstruction.cuh:114-125, 235-250): d = max(0, dist - e),
+// then d += (Delta + EPS) while d < dist + e, calling f(bin) with
+// bin = (int)(d / Delta); the walk ends at the first bin outside [0, T)
+// (d >= 0, so only bin >= T can occur).  At most T + 2 steps: a step below
+// the spacing of floats at d would not advance the walk.
+template <class F>
+FBP_HD void scatter_walk(float dist, float ext, float dt, uint32_t T, F&& f) {
+    const float lo = rn_sub(dist, ext);
+    float d = 0.0f < lo ? lo : 0.0f;
+    const float hi = rn_add(dist, ext);
+    const float step = rn_add(dt, 0.00001f);
+    for (uint32_t n = 0; n < T + 2u && d < hi; ++n) {
+        const float q = rn_div(d, dt);
+        if (!(q < static_cast<float>(T))) return;
+        f(static_cast<int>(q));
+        d = rn_add(d, step);
+    }
+}
+
 // safeRCP (FB/math.cuh:23-29).
 FBP_HD float safe_rcp(float x) {
     if (x > 0.00001f || x < -0.00001f) return rn_div(1.0f, x);

@@ -121,6 +121,11 @@ fbp_status fbp_context_set_option(fbp_context ctx, int32_t option, int32_t value
                 return fbp::fail(ctx, FBP_ERR_INVALID_ARGUMENT, "option value must be 0 or 1");
             ctx->disable_aligned = value ? 0 : 1;
             return FBP_OK;
+        case FBP_OPTION_VOXEL_SCATTERING:
+            if (value != 0 && value != 1)
+                return fbp::fail(ctx, FBP_ERR_INVALID_ARGUMENT, "option value must be 0 or 1");
+            ctx->voxel_scattering = value;
+            return FBP_OK;
         default:
             return fbp::fail(ctx, FBP_ERR_INVALID_ARGUMENT, "unknown option %d", int(option));
     }
@@ -131,6 +136,9 @@ fbp_status fbp_context_get_option(fbp_context ctx, int32_t option, int32_t* valu
     switch (option) {
         case FBP_OPTION_CONFOCAL_ALIGNED:
             *value = ctx->disable_aligned ? 0 : 1;
+            return FBP_OK;
+        case FBP_OPTION_VOXEL_SCATTERING:
+            *value = ctx->voxel_scattering;
             return FBP_OK;
         default:
             return fbp::fail(ctx, FBP_ERR_INVALID_ARGUMENT, "unknown option %d", int(option));

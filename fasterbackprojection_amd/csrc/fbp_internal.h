@@ -16,6 +16,9 @@ struct fbp_context_s {
     // Confocal backprojection: 0 = the aligned-grid path may run when its
     // conditions hold (default), 1 = always the general kernel.
     int disable_aligned = 0;
+    // 1: confocal voxel and MIS backprojection take the voxel-scattering
+    // walk over every bin a voxel reaches (reference compile-time switch).
+    int voxel_scattering = 0;
     // Device int: path flag of the last confocal voxel call (1 = aligned
     // path), valid when path_flag_valid is set.
     int* path_flag = nullptr;

@@ -111,7 +111,16 @@ size_t fbp_context_scratch_bytes(fbp_context ctx);
 typedef enum fbp_option {
     /* 1 (default): fbp_backproject_confocal_voxel may take the aligned-grid
      * path (same results; see that function); 0: always the general kernel. */
-    FBP_OPTION_CONFOCAL_ALIGNED = 0
+    FBP_OPTION_CONFOCAL_ALIGNED = 0,
+    /* 0 (default): one bin per (voxel, scan point).  1: the reference's
+     * ACCUMULATE_VOXEL_SCATTERING variant (FB/transient_reconstruction.cuh:
+     * 112-126, 235-250): fbp_backproject_confocal_voxel and
+     * fbp_backproject_confocal_mis add, for each pair, the samples of every
+     * bin visited by d = max(0, dist - e), d += (Delta + 1e-5) while
+     * d < dist + e (e = half the voxel diagonal), each weighted by
+     * safeRCP(dist^2); the walk ends at the first bin >= T.  Runs a
+     * dedicated kernel (not the aligned-grid path). */
+    FBP_OPTION_VOXEL_SCATTERING = 1
 } fbp_option;
 fbp_status fbp_context_set_option(fbp_context ctx, int32_t option, int32_t value);
 fbp_status fbp_context_get_option(fbp_context ctx, int32_t option, int32_t* value);

@@ -48,7 +48,12 @@ def load() -> C.CDLL:
         _lib.oracle_confocal_voxel.argtypes = [P, P, C.c_int, P, P]
         _lib.oracle_exhaustive_voxel.argtypes = [P, P, P, P, P]
         _lib.oracle_depth.argtypes = [P, P, P, P, P, C.c_uint32, C.c_int, P]
-        _lib.oracle_mis.argtypes = [P, P, P, P, P, P, P, C.c_uint32, C.c_uint32, C.c_int, P]
+        _lib.oracle_mis.argtypes = [P, P, P, P, P, P, P, C.c_uint32, C.c_uint32, C.c_int,
+                                    C.c_int, P]
+        _lib.oracle_confocal_voxel_scatter.argtypes = [P, P, C.c_int, P, P]
+        _lib.oracle_scatter_bins.restype = C.c_int
+        _lib.oracle_scatter_bins.argtypes = [P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                             P]
         _lib.oracle_alias_tables.restype = C.c_int
         _lib.oracle_alias_tables.argtypes = [P, C.c_uint32, P, P]
         _lib.oracle_laplacian.argtypes = [P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int]
@@ -136,14 +141,32 @@ def mis_cdf(I) -> np.ndarray:
     return ((pre - mn) / (mx - mn)).astype(np.float32) if mx > mn else np.zeros_like(pre, np.float32)
 
 
-def mis(info, I, laser, cdf, alias, prob, noise, num_samples, mode=0) -> np.ndarray:
-    """backprojectConfocalVoxelMIS (FB/transient_reconstruction.cuh:194-260)."""
+def mis(info, I, laser, cdf, alias, prob, noise, num_samples, mode=0, scatter=False) -> np.ndarray:
+    """backprojectConfocalVoxelMIS (FB/transient_reconstruction.cuh:194-260);
+    scatter: its ACCUMULATE_VOXEL_SCATTERING branch (:235-250)."""
     I, laser, cdf, prob, noise = map(_f32, (I, laser, cdf, prob, noise))
     alias = np.ascontiguousarray(alias, np.uint32)
     out = np.empty(_res(info), np.float64)
     load().oracle_mis(C.byref(info), _p(I), _p(laser), _p(cdf), _p(alias), _p(prob), _p(noise),
-                      len(noise), num_samples, mode, _p(out))
+                      len(noise), num_samples, mode, int(bool(scatter)), _p(out))
     return out
+
+
+def confocal_voxel_scatter(info, I, laser, layout: int = 0) -> np.ndarray:
+    """ACCUMULATE_VOXEL_SCATTERING branch of backprojectConfocalVoxel
+    (FB/transient_reconstruction.cuh:112-126); FP64 [Z][Y][X]."""
+    I, laser = _f32(I), _f32(laser)
+    out = np.empty(_res(info), np.float64)
+    load().oracle_confocal_voxel_scatter(C.byref(info), _p(I), layout, _p(laser), _p(out))
+    return out
+
+
+def scatter_bins(info, laser, l, vx, vy, z) -> list:
+    """Bins visited by the voxel-scattering walk of one (voxel, scan point)."""
+    laser = _f32(laser)
+    buf = np.empty(int(info.num_time_bins) + 2, np.int64)
+    n = load().oracle_scatter_bins(C.byref(info), _p(laser), l, vx, vy, z, _p(buf))
+    return buf[:n].tolist()
 
 
 def mis_tiled_samples(I, noise, num_samples, tiles):
@@ -180,11 +203,12 @@ def mis_tiled_samples(I, noise, num_samples, tiles):
     return ls, ws
 
 
-def mis_tiled(info, I, laser, noise, num_samples) -> np.ndarray:
+def mis_tiled(info, I, laser, noise, num_samples, scatter=False) -> np.ndarray:
     """Tile-shared MIS backprojection: every 16^3 tile (vx, vy, z; x fastest)
     gathers only its sampled scan points, each with its weight, through the
     confocal_voxel restatement (the per-tile weighted transient sum_i w_i I[l_i]
-    thresholded first).  FP64 [Z][Y][X]."""
+    thresholded first; the voxel-scattering restatement when scatter).
+    FP64 [Z][Y][X]."""
     I = np.asarray(I, np.float32)
     Z, Y, X = _res(info)
     tx, ty, tz = -(-X // 16), -(-Y // 16), -(-Z // 16)
@@ -196,7 +220,7 @@ def mis_tiled(info, I, laser, noise, num_samples) -> np.ndarray:
         wl = np.zeros(I.shape[0])
         np.add.at(wl, ls[t], ws[t])
         It = (Ithr.astype(np.float64) * wl[:, None]).astype(np.float32)
-        vol = confocal_voxel(info, It, laser)
+        vol = (confocal_voxel_scatter if scatter else confocal_voxel)(info, It, laser)
         sl = (slice(16 * bz, 16 * bz + 16), slice(16 * by, 16 * by + 16), slice(16 * bx, 16 * bx + 16))
         out[sl] = vol[sl]
     return out

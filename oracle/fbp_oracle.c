@@ -152,6 +152,71 @@ void oracle_confocal_voxel(const RecInfo* in, const float* I, int layout, const 
     }
 }
 
+/* ------------------------------------------- voxel-scattering variant
+ * The ACCUMULATE_VOXEL_SCATTERING branch of backprojectConfocalVoxel
+ * (FB/transient_reconstruction.cuh:112-126): e = length(voxel size) / 2,
+ * d = max(0, dist - e); while d < dist + e: bin = (int)(d / Delta), a bin
+ * outside [0, T) ends the pair, else add I(l, bin) * safeRCP(dist^2) when
+ * I > EPS; d += Delta + EPS.  At most T + 2 steps (a walk whose step falls
+ * below the float spacing at d would not advance).  Returns the number of
+ * bins written to bins[]. */
+static float voxel_extent(const RecInfo* in) {
+    vec3 v = in->hidden_volume_voxel_size;
+    float x2 = v.x * v.x, y2 = v.y * v.y, z2 = v.z * v.z;
+    float s = (x2 + y2) + z2;
+    return sqrtf(s) / 2.0f;
+}
+
+static int scatter_bins(float dist, float ext, float dt, uint32_t T, int64_t* bins) {
+    float lo = dist - ext;
+    float d = (0.0f < lo) ? lo : 0.0f;
+    float hi = dist + ext;
+    float step = dt + OR_EPS;
+    int n = 0;
+    for (uint32_t k = 0; k < T + 2u && d < hi; ++k) {
+        float q = d / dt;
+        if (!(q < (float)T)) break;
+        bins[n++] = (int64_t)(int)q;
+        d = d + step;
+    }
+    return n;
+}
+
+/* Bins of the walk of one pair (exposed for tests); returns the count. */
+int oracle_scatter_bins(const RecInfo* in, const vec3* laser, uint32_t l, uint32_t vx,
+                        uint32_t vy, uint32_t z, int64_t* bins) {
+    float d = confocal_dist(in, laser[l], voxel_pos(in, vx, vy, z));
+    return scatter_bins(d, voxel_extent(in), in->time_step, in->num_time_bins, bins);
+}
+
+void oracle_confocal_voxel_scatter(const RecInfo* in, const float* I, int layout,
+                                   const vec3* laser, double* vol) {
+    const uint32_t X = in->voxel_resolution.x, Y = in->voxel_resolution.y,
+                   Z = in->voxel_resolution.z;
+    const uint32_t L = in->num_laser_targets, T = in->num_time_bins;
+    const float ext = voxel_extent(in);
+    const int64_t n = (int64_t)X * Y * Z;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t v = 0; v < n; ++v) {
+        uint32_t vx = (uint32_t)(v % X), vy = (uint32_t)((v / X) % Y), z = (uint32_t)(v / ((int64_t)X * Y));
+        vec3 p = voxel_pos(in, vx, vy, z);
+        int64_t* bins = (int64_t*)malloc(sizeof(int64_t) * (T + 2));
+        double acc = 0.0;
+        for (uint32_t l = 0; l < L; ++l) {
+            float d = confocal_dist(in, laser[l], p);
+            int nb = scatter_bins(d, ext, in->time_step, T, bins);
+            double w = safe_rcp64((double)(d * d));
+            for (int j = 0; j < nb; ++j) {
+                size_t idx = layout == 0 ? (size_t)l * T + (size_t)bins[j] : (size_t)bins[j] * L + l;
+                float light = I[idx];
+                if (light > OR_EPS) acc += (double)light * w;
+            }
+        }
+        free(bins);
+        vol[v] = acc;
+    }
+}
+
 /* ------------------------------------------------- exhaustive backprojection
  * backprojectExhaustiveVoxel (FB/transient_reconstruction.cuh:138-166) with
  * every pair visited, l = ls / S, s = ls % S. */
@@ -240,17 +305,19 @@ static float mis_uniform(const float* noise, uint32_t n, uint32_t* state) {
 
 void oracle_mis(const RecInfo* in, const float* I, const vec3* laser, const float* cdf,
                 const uint32_t* alias, const float* prob, const float* noise, uint32_t noise_size,
-                uint32_t num_samples, int mode, double* vol) {
+                uint32_t num_samples, int mode, int scatter, double* vol) {
     const uint32_t X = in->voxel_resolution.x, Y = in->voxel_resolution.y,
                    Z = in->voxel_resolution.z;
     const uint32_t L = in->num_laser_targets, T = in->num_time_bins;
     const uint32_t slice = X * Y;
     const float invL = 1.0f / (float)L;
+    const float ext = voxel_extent(in);
 #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t vid = 0; vid < (int64_t)slice * Z; ++vid) {
         uint32_t v = (uint32_t)(vid % slice), z = (uint32_t)(vid / slice);
         uint32_t vx = v % X, vy = v / X;
         vec3 p = voxel_pos(in, vx, vy, z);
+        int64_t* bins = scatter ? (int64_t*)malloc(sizeof(int64_t) * (T + 2)) : NULL;
         double acc = 0.0;
         for (uint32_t s = 0; s < num_samples; ++s) {
             uint32_t state = slice * z + v + s;
@@ -273,19 +340,28 @@ void oracle_mis(const RecInfo* in, const float* I, const vec3* laser, const floa
                 gpdf = pdf + pa;
             }
             float d = confocal_dist(in, laser[l], p);
+            double wd = safe_rcp64((double)(d * d));
+            double sw;
+            if (mode == 0)
+                sw = (double)pdf * safe_rcp64((double)gpdf);
+            else
+                sw = 1.0 / (0.5 * ((double)pa + (double)invL) * (double)num_samples);
+            if (scatter) {
+                /* ACCUMULATE_VOXEL_SCATTERING branch (:235-250). */
+                int nb = scatter_bins(d, ext, in->time_step, T, bins);
+                for (int j = 0; j < nb; ++j) {
+                    float light = I[(size_t)l * T + (size_t)bins[j]];
+                    if (light > OR_EPS) acc += sw * (double)light * wd;
+                }
+                continue;
+            }
             int64_t b = bin_of(d, in->time_step, T);
             if (b < 0) continue;
             float light = I[(size_t)l * T + (size_t)b];
             if (!(light > OR_EPS)) continue;
-            double wd = safe_rcp64((double)(d * d));
-            if (mode == 0) {
-                double mw = (double)pdf * safe_rcp64((double)gpdf);
-                acc += mw * (double)light * wd;
-            } else {
-                double pmix = 0.5 * ((double)pa + (double)invL);
-                acc += (double)light * wd / (pmix * (double)num_samples);
-            }
+            acc += sw * (double)light * wd;
         }
+        free(bins);
         vol[vid] = acc;
     }
 }

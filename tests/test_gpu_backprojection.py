@@ -424,3 +424,90 @@ def test_rejects_bad_arguments(ctx):
     I = torch.zeros((len(g.laser_targets), g.num_time_bins), device="cuda")
     with pytest.raises(FbpError):
         ctx.backproject_confocal(info, I, cuda(g.laser_targets))
+
+
+# ------------------------------------------------ voxel-scattering variant
+@pytest.fixture
+def scatter_ctx(ctx):
+    ctx.set_voxel_scattering(True)
+    assert ctx.option(1) == 1
+    try:
+        yield ctx
+    finally:
+        ctx.set_voxel_scattering(False)
+
+
+@pytest.mark.parametrize("discard,t0", [(1, 0.0), (0, 0.3)])
+def test_confocal_scatter_matches_oracle(scatter_ctx, discard, t0):
+    """Every bin of each voxel's walk (walks that leave [0, T) included:
+    distances here exceed T * Delta), spiky values so that one wrong bin
+    fails the tolerance."""
+    g = geom(discard=discard, t0=t0)
+    info = make_info(g)
+    rng = np.random.default_rng(51)
+    I = spiky(rng, (len(g.laser_targets), g.num_time_bins))
+    ref = oracle.confocal_voxel_scatter(info, I, g.laser_targets)
+    out = scatter_ctx.backproject_confocal(info, cuda(I), cuda(g.laser_targets))
+    assert not scatter_ctx.last_confocal_path_aligned()
+    assert_volume_close(out, ref)
+
+
+def test_confocal_scatter_time_major_and_aligned_grid(scatter_ctx):
+    """An aligned-grid geometry still takes the scattering kernel; the
+    time-major layout reads the same samples."""
+    g = aligned_geom(**ALIGNED["64"])
+    info = make_info(g)
+    rng = np.random.default_rng(52)
+    I = spiky(rng, (len(g.laser_targets), g.num_time_bins))
+    ref = oracle.confocal_voxel_scatter(info, I, g.laser_targets)
+    out = scatter_ctx.backproject_confocal(info, cuda(I), cuda(g.laser_targets))
+    assert not scatter_ctx.last_confocal_path_aligned()
+    assert_volume_close(out, ref)
+    out_tm = scatter_ctx.backproject_confocal(info, cuda(I.T), cuda(g.laser_targets),
+                                              layout=LAYOUT_TIME_MAJOR)
+    assert_volume_close(out_tm, ref)
+
+
+def test_confocal_scatter_quantized(scatter_ctx):
+    g = geom()
+    info = make_info(g)
+    rng = np.random.default_rng(53)
+    I = spiky(rng, (len(g.laser_targets), g.num_time_bins))
+    I[I <= 1e-5] = 0.0
+    q, sc = scatter_ctx.quantize(cuda(I), STORAGE_F16)
+    ref = oracle.confocal_voxel_scatter(info, dequantize(q, sc, STORAGE_F16), g.laser_targets)
+    out = scatter_ctx.backproject_confocal(info, q, cuda(g.laser_targets), storage=STORAGE_F16,
+                                           scales=sc)
+    assert_volume_close(out, ref)
+
+
+def test_mis_reference_scatter_matches_oracle(scatter_ctx):
+    cap = small_confocal(seed=6, res=16)
+    info = make_info(cap.geometry)
+    I = cap.transient
+    L = len(cap.geometry.laser_targets)
+    cdf = oracle.mis_cdf(I)
+    alias, prob = oracle.alias_tables(cdf)
+    noise = np.random.default_rng(1).random(16 ** 3).astype(np.float32)
+    ref = oracle.mis(info, I, cap.geometry.laser_targets, cdf, alias, prob, noise, L // 4, 0,
+                     scatter=True)
+    plain = oracle.mis(info, I, cap.geometry.laser_targets, cdf, alias, prob, noise, L // 4, 0)
+    out = scatter_ctx.backproject_confocal_mis(info, cuda(I), cuda(cap.geometry.laser_targets),
+                                               cuda(noise), L // 4, mode=0)
+    torch.cuda.synchronize()
+    assert rel_l2(out.cpu().numpy(), ref) < 1e-4
+    assert rel_l2(plain, ref) > 0.1       # the walk adds several bins per sample
+
+
+def test_mis_tiled_scatter_matches_oracle(scatter_ctx):
+    g = geom(n_scan=16, T=256, res=(32, 32, 20))
+    info = make_info(g)
+    rng = np.random.default_rng(54)
+    I = (10.0 ** rng.uniform(-2, 3, size=(256, 256))).astype(np.float32)
+    I[rng.random(I.shape) < 0.3] = 0.0
+    noise = rng.random(4096).astype(np.float32)
+    ref = oracle.mis_tiled(info, I, g.laser_targets, noise, 48, scatter=True)
+    out = scatter_ctx.backproject_confocal_mis(info, cuda(I), cuda(g.laser_targets), cuda(noise),
+                                               48, mode=MIS_TILED)
+    torch.cuda.synchronize()
+    assert_volume_close(out, ref)

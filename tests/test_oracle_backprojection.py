@@ -146,3 +146,110 @@ def test_threshold_and_range():
     info = make_info(g)
     I = np.full((len(g.laser_targets), 16), 1e-5, np.float32)
     assert np.all(oracle.confocal_voxel(info, I, g.laser_targets) == 0)
+
+
+# ------------------------------------------------ voxel-scattering variant
+def _real_dist(info, g, l, vx, vy, z, t0=0.0, discard=1):
+    cx, cy, cz = _centres(info)
+    p = np.array([cx[vx], cy[z], cz[vy]], np.float64)
+    L = g.laser_targets.astype(np.float64)
+    d = 2 * np.linalg.norm(p - L[l]) - np.float64(np.float32(t0))
+    if discard == 0:
+        lp = np.array([0.1, -0.8, 0.2], np.float32).astype(np.float64)
+        sp = np.array([-0.2, -0.7, 0.1], np.float32).astype(np.float64)
+        d += np.linalg.norm(lp - L[l]) + np.linalg.norm(sp - L[l])
+    return d
+
+
+def _extent(info):
+    vs = info.hidden_volume_voxel_size
+    return 0.5 * np.sqrt(float(vs.x) ** 2 + float(vs.y) ** 2 + float(vs.z) ** 2)
+
+
+@pytest.mark.parametrize("discard,t0", [(1, 0.0), (0, 0.37)])
+def test_scatter_walk_covers_the_voxel_extent(discard, t0):
+    """The walk starts at the bin of max(0, dist - e), steps by Delta + 1e-5
+    while below dist + e, with e = |voxel size| / 2 (anisotropic voxels here,
+    so a wrong extent shows): in real arithmetic the bins are strictly
+    increasing, the first is floor(max(0, dist - e) / Delta) and the count is
+    ceil((dist + e - start) / (Delta + 1e-5)) unless a bin >= T ends it."""
+    g = _geom(res=(6, 5, 7), hmin=(-0.4, 0.4, -0.3), hmax=(0.5, 1.2, 0.35), T=160,
+              discard=discard, t0=t0)
+    info = make_info(g)
+    e = _extent(info)
+    dt, T = float(np.float32(info.time_step)), info.num_time_bins
+    step = dt + 1e-5
+    rng = np.random.default_rng(9)
+    checked = 0
+    for _ in range(300):
+        l = int(rng.integers(len(g.laser_targets)))
+        vx, vy, z = int(rng.integers(6)), int(rng.integers(5)), int(rng.integers(7))
+        d = _real_dist(info, g, l, vx, vy, z, t0, discard)
+        start = max(0.0, d - e)
+        n_real = int(np.ceil((d + e - start) / step))
+        q0 = start / dt
+        # skip cases within float rounding of a bin edge or of the count edge
+        frac_edges = [(start + k * step) / dt for k in range(n_real + 1)]
+        if any(abs(f - round(f)) < 1e-4 for f in frac_edges) or \
+                abs((d + e - start) / step - round((d + e - start) / step)) < 1e-4:
+            continue
+        bins = oracle.scatter_bins(info, g.laser_targets, l, vx, vy, z)
+        expect = [int(np.floor((start + k * step) / dt)) for k in range(n_real)]
+        expect = [b for b in expect if b < T][:len(expect)]
+        if any(b >= T for b in [int(np.floor((start + k * step) / dt)) for k in range(n_real)]):
+            cut = next(k for k in range(n_real) if int(np.floor((start + k * step) / dt)) >= T)
+            expect = expect[:cut]
+        assert bins == expect, (l, vx, vy, z, bins, expect, q0)
+        checked += 1
+    assert checked > 200
+
+
+def test_scatter_constant_transient_counts_steps():
+    """A transient constant in time: each pair adds c * safeRCP(dist^2) once
+    per walk step, so the scattering volume divided by the single-bin volume
+    is a weighted mean of the step counts, which lie within one of
+    2e / (Delta + 1e-5)."""
+    g = _geom(n_scan=6, T=1024, res=(5, 4, 6))
+    info = make_info(g)
+    I = np.full((len(g.laser_targets), 1024), 2.5, np.float32)
+    one = oracle.confocal_voxel(info, I, g.laser_targets)
+    sc = oracle.confocal_voxel_scatter(info, I, g.laser_targets)
+    k = 2 * _extent(info) / (float(np.float32(info.time_step)) + 1e-5)
+    ratio = sc / one
+    assert np.all(ratio >= np.floor(k) - 1e-9) and np.all(ratio <= np.ceil(k) + 1 + 1e-9), \
+        (ratio.min(), ratio.max(), k)
+
+
+def test_scatter_impulse_reaches_the_voxels_of_its_bin():
+    """One nonzero sample (l0, k0): a voxel receives it exactly when its walk
+    [max(0, dist - e), dist + e) meets bin k0, i.e. when some step point
+    start + j (Delta + 1e-5) falls in [k0 Delta, (k0 + 1) Delta) (real
+    arithmetic, voxels within 1e-4 bins of an edge skipped); its value is
+    then I * 1/dist^2."""
+    g = _geom(n_scan=6, T=200, res=(8, 7, 9), hmin=(-0.4, 0.3, -0.4), hmax=(0.4, 1.0, 0.4))
+    info = make_info(g)
+    e = _extent(info)
+    dt = float(np.float32(info.time_step))
+    step = dt + 1e-5
+    l0 = 14
+    d_all = np.array([[[_real_dist(info, g, l0, vx, vy, z) for vx in range(8)] for vy in range(7)]
+                      for z in range(9)])
+    k0 = int(np.median(d_all) / dt)
+    I = np.zeros((len(g.laser_targets), 200), np.float32)
+    I[l0, k0] = 3.0
+    vol = oracle.confocal_voxel_scatter(info, I, g.laser_targets)
+    checked = hit = 0
+    for (z, vy, vx), d in np.ndenumerate(d_all):
+        start = max(0.0, d - e)
+        pts = [start + j * step for j in range(int(np.ceil((d + e - start) / step)))]
+        qs = [p / dt for p in pts]
+        if any(abs(q - round(q)) < 1e-4 for q in qs):
+            continue
+        reach = any(int(np.floor(q)) == k0 for q in qs)
+        if reach:
+            assert vol[z, vy, vx] == pytest.approx(3.0 / d ** 2, rel=1e-5)
+            hit += 1
+        else:
+            assert vol[z, vy, vx] == 0.0
+        checked += 1
+    assert checked > 400 and hit > 10, (checked, hit)
