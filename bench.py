@@ -31,6 +31,7 @@ sys.path.insert(0, str(ROOT))
 FLOPS_PER_UPDATE = 16          # confocal, FMA = 2 (reference formula)
 PEAK_FP32_TFLOPS = 157.3       # MI355X FP32 (datasheet)
 PEAK_LDS_TBPS = 78.6           # 256 CUs x 128 B/clk x 2.4 GHz
+PEAK_HBM_GBPS = 8000.0         # MI355X HBM3E (datasheet)
 TRAFFIC_PROFILE = ROOT / "profiles" / "confocal_128cube_traffic.json"
 
 
@@ -64,6 +65,8 @@ def parse():
                     help="skip the learned-vs-plain leg (64^2 x 512 -> 64^3)")
     ap.add_argument("--copy-reference-mib", type=int, default=0,
                     help="run a plain device copy of this many MiB (counter calibration)")
+    ap.add_argument("--no-lct", action="store_true",
+                    help="skip the LCT leg (128^2 x 1024 padded to 256x256x2048)")
     ap.add_argument("--no-exhaustive", action="store_true",
                     help="skip the exhaustive leg (16^2 lasers x 64^2 sensors x 1024 -> 64^3)")
     return ap.parse_args()
@@ -181,6 +184,7 @@ def main():
     precision = None if args.no_precision else precision_leg(ctx, args)
     learned = None if args.no_learned else learned_leg(ctx)
     exhaustive = None if args.no_exhaustive else exhaustive_leg(ctx, args)
+    lct = None if args.no_lct else lct_leg(ctx, args)
 
     out = {
         "metric": METRIC,
@@ -223,6 +227,7 @@ def main():
         "precision": precision,
         "learned": learned,
         "exhaustive": exhaustive,
+        "lct": lct,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
@@ -286,6 +291,84 @@ def precision_leg(ctx, args):
                      "max_abs_rel_to_peak": float(np.abs(err).max() / np.abs(ref).max())}
         log(f"precision {name}: {out[name]}")
     return out
+
+
+def lct_leg(ctx, args, reps=5):
+    """LCT of the full confocal capture (128x128 scan, 1024 bins of 16 ps on a
+    2 m wall) zero-padded to 256x256x2048, Wiener SNR 8 (python/lct.py
+    semantics).  HBM-bound: algorithmic bytes = 2 FFTs x (read + write) x
+    8 |P| + filter read 8 |P| for the padded complex volume |P| (5.4 GB).
+    CPU baseline: the NumPy restatement of python/lct.py (oracle/lct.py) on
+    the small configuration (32x32 scan, 256 bins of 32 ps, pad 64x64x512),
+    with the GPU timed on the same input."""
+    import numpy as np
+    import torch
+
+    from fasterbackprojection_amd.scene import (C_LIGHT, full_confocal_geometry, random_quads,
+                                                small_confocal)
+
+    dev = torch.device("cuda", 0)
+    g = full_confocal_geometry(128, 1024, 128)
+    sc = random_quads(np.random.default_rng(2), 4, 20000)
+    laser = torch.from_numpy(g.laser_targets).to(dev)
+    clean = ctx.render_confocal(laser, torch.from_numpy(sc.points.astype(np.float32)).to(dev),
+                                torch.from_numpy(sc.albedo.astype(np.float32)).to(dev), 1024,
+                                g.time_step)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(2)
+    I = torch.poisson(clean * (1000.0 / clean.max()), generator=gen)
+    # scan order l = iz * n + ix -> data[iz][ix][t] = (N, N, M)
+    data = I.reshape(128, 128, 1024).contiguous()
+    width, rmax = 1.0, 1024 * float(g.time_step)
+    out = ctx.lct(data, width, rmax, 8.0)
+    stream = torch.cuda.current_stream()
+    ms = []
+    for _ in range(reps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        ctx.lct(data, width, rmax, 8.0, out=out)
+        b.record(stream)
+        b.synchronize()
+        ms.append(a.elapsed_time(b))
+    P = 256 * 256 * 2048
+    alg = 5 * 8 * P
+    t = float(np.median(ms))
+    res = {"workload": "LCT 128x128x1024 (16 ps, 2 m wall) padded to 256x256x2048, snr 8",
+           "ms": t, "roofline": {"bound": "hbm", "achieved": alg / (t * 1e-3) / 1e9,
+                                 "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                                 "frac": alg / (t * 1e-3) / 1e9 / PEAK_HBM_GBPS,
+                                 "algorithmic_bytes": alg},
+           "finite": bool(torch.isfinite(out).all())}
+    # small configuration: GPU vs the CPU restatement, same input
+    if not args.no_cpu_baseline:
+        from oracle import lct as olct
+        cap = small_confocal(seed=0)
+        d1 = cap.transient.reshape(32, 32, 256).astype(np.float32)
+        w1, r1 = 0.5, 256 * C_LIGHT * 32e-12
+        t0 = time.perf_counter()
+        ref = olct.lct(d1, w1, r1, 8.0)
+        cpu_s = time.perf_counter() - t0
+        g1 = ctx.lct(torch.from_numpy(d1).to(dev), w1, r1, 8.0)
+        torch.cuda.synchronize()
+        ms1 = []
+        for _ in range(reps):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            ctx.lct(torch.from_numpy(d1).to(dev), w1, r1, 8.0, out=g1)
+            b.record(stream)
+            b.synchronize()
+            ms1.append(a.elapsed_time(b))
+        res["small"] = {"workload": "LCT 32x32x256 (32 ps, 1 m wall) padded to 64x64x512",
+                        "gpu_ms": float(np.median(ms1)), "cpu_s": cpu_s, "cpu_cores": 1,
+                        "cpu_kind": "port (NumPy restatement of python/lct.py)",
+                        "rel_l2_gpu_vs_cpu": float(np.linalg.norm(g1.cpu().numpy() - ref) /
+                                                   np.linalg.norm(ref))}
+    log(f"lct: {res}")
+    del I, data, out
+    torch.cuda.empty_cache()
+    return res
 
 
 def exhaustive_leg(ctx, args, reps=3):

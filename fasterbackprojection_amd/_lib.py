@@ -103,6 +103,8 @@ SIGNATURES = {
     "fbp_lct": (_S, [_CTX, _P, C.POINTER(LctParams), _P, _P]),
     "fbp_render_confocal": (_S, [_CTX, _P, C.c_uint32, _P, _P, C.c_uint32, C.c_uint32, C.c_float, _P, _P]),
     "fbp_render_exhaustive": (_S, [_CTX, _P, C.c_uint32, _P, C.c_uint32, _P, _P, C.c_uint32, C.c_uint32, C.c_float, _P, _P]),
+    "fbp_backproject_confocal_points": (_S, [_CTX, C.POINTER(ReconstructionInfo), C.POINTER(Transient), _P, _P, C.c_uint32, _P, _P]),
+    "fbp_learned_calibrate": (_S, [_CTX, C.POINTER(ReconstructionInfo), C.POINTER(Transient), _P, _P, _P, C.c_int32, C.c_float, C.c_uint32, _P]),
     "fbp_volume_residual": (_S, [_CTX, C.POINTER(ReconstructionInfo), _P, _P, _P, _P, C.c_uint32, C.c_float, C.c_int32, C.c_int32, C.POINTER(C.c_double), _P]),
     "fbp_refiner_param_count": (C.c_size_t, []),
     "fbp_refiner_create": (_S, [_CTX, C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]),

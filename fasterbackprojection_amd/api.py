@@ -211,6 +211,29 @@ class Context:
             _stream()))
         return vol
 
+    def backproject_points(self, info, intensity, laser_targets, voxels, out=None):
+        """Exact confocal voxel backprojection at the listed voxel indices
+        (fbp_backproject_confocal_points); float64 values."""
+        n = int(voxels.numel())
+        if out is None:
+            out = torch.empty((n,), dtype=torch.float64, device=intensity.device)
+        tr = self._transient(intensity, STORAGE_F32, LAYOUT_TIME_FASTEST, None)
+        self.check(self.lib.fbp_backproject_confocal_points(
+            self.handle, C.byref(info), C.byref(tr), _ptr(laser_targets), _ptr(voxels), n,
+            _ptr(out), _stream()))
+        return out
+
+    def learned_calibrate(self, info, intensity, laser_targets, volume, reference, extremes=16,
+                          kernel_size=5, sigma=1.0):
+        """Global calibration of a refined, normalised LoG volume
+        (fbp_learned_calibrate), in place: reference is an independent
+        unbiased estimate of the LoG in physical units."""
+        tr = self._transient(intensity, STORAGE_F32, LAYOUT_TIME_FASTEST, None)
+        self.check(self.lib.fbp_learned_calibrate(
+            self.handle, C.byref(info), C.byref(tr), _ptr(laser_targets), _ptr(volume),
+            _ptr(reference), int(kernel_size), float(sigma), int(extremes), _stream()))
+        return volume
+
     def quantize(self, intensity: torch.Tensor, storage: int):
         """Per-row scaled F16/BF16 copy of a [rows][T] float32 transient."""
         rows = intensity.numel() // intensity.shape[-1]

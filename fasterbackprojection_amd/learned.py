@@ -12,6 +12,14 @@ through the same LoG + normalisation, and the refiner predicts a correction
 of it (its last layer adds the input), so the un-refined MIS path is the
 starting point the training improves on.
 
+The refined volume matches the target up to a global affine map (the
+normalisation of the noisy input is set by its extremes, which the
+convolutions cannot see); fbp_learned_calibrate fixes that map: a second,
+independent MIS draw (+ LoG) regressed on the refined volume over all
+voxels gives the LoG in physical units, and exact LoG values at the refined
+volume's CALIB_EXTREMES lowest and highest voxels (exact backprojections of
+their 5^3 neighbourhoods) give the true range.
+
 Every step runs in libfbp.so kernels; torch holds device memory, its RNG
 draws the Poisson counts and noise buffers.
 
@@ -37,6 +45,7 @@ SIZE = 64
 SCAN = 64
 BINS = 512
 SAMPLE_FRACTION = 16          # MIS samples per voxel = L / SAMPLE_FRACTION
+CALIB_EXTREMES = 16           # exact LoG at the k lowest and k highest voxels
 
 
 def geometry() -> Geometry:
@@ -58,6 +67,8 @@ class Pipeline:
         self.vol = torch.empty((SIZE,) * 3, dtype=torch.float32, device=self.dev)
         self.filt = torch.empty_like(self.vol)
         self.mis = torch.empty_like(self.vol)
+        self.mis2 = torch.empty_like(self.vol)
+        self.ref = torch.empty_like(self.vol)
         # residual check: every 4th scan point in x and z
         n = SCAN
         idx = np.array([iz * n + ix for iz in range(0, n, 4) for ix in range(0, n, 4)])
@@ -94,6 +105,19 @@ class Pipeline:
         self.ctx.log(self.mis, 5, 1.0, out=out)
         self.ctx.normalize(out)
         return out
+
+    def calibrate(self, volume: torch.Tensor, I: torch.Tensor, seed: int) -> torch.Tensor:
+        """Global calibration of a refined volume (fbp_learned_calibrate), in
+        place, against the LoG of a second MIS draw (independent noise)."""
+        gen = torch.Generator(device=self.dev)
+        gen.manual_seed(20_000_033 + seed)
+        noise = torch.rand(SIZE ** 3, generator=gen, device=self.dev)
+        self.ctx.backproject_confocal_mis(self.info, I, self.laser, noise,
+                                          self.L // SAMPLE_FRACTION, mode=MIS_TILED,
+                                          out=self.mis2)
+        self.ctx.log(self.mis2, 5, 1.0, out=self.ref)
+        return self.ctx.learned_calibrate(self.info, I, self.laser, volume, self.ref,
+                                          CALIB_EXTREMES, 5, 1.0)
 
     def residual(self, volume: torch.Tensor, I: torch.Tensor) -> float:
         """Forward-model residual on every 16th scan point (fbp_volume_residual):
@@ -168,6 +192,7 @@ def flat_noise_residual(p: Pipeline, net: Refiner) -> float:
     I = I.contiguous()
     y = torch.empty((SIZE,) * 3, dtype=torch.float32, device=p.dev)
     net.forward(p.mis_input(I, 0), out=y)
+    p.calibrate(y, I, 0)
     return p.residual(y, I)
 
 
@@ -205,6 +230,7 @@ def evaluate(p: Pipeline, net: Refiner, seeds, X=None, Y=None) -> dict:
             I = p.render(int(s))
             x, y = X[i], Y[i]
         net.forward(x, out=out)
+        p.calibrate(out, I, int(s))
         learned.append(rel_l2(out, y))
         baseline.append(rel_l2(x, y))
         res.append(p.residual(out, I))
@@ -239,6 +265,7 @@ class LearnedReconstructor:
     def __call__(self, I: torch.Tensor, seed: int = 0, check: bool = True):
         x = self.p.mis_input(I, seed, out=self.x)
         self.net.forward(x, out=self.out)
+        self.p.calibrate(self.out, I, seed)
         if check:
             r = self.p.residual(self.out, I)
             if r > self.threshold:

@@ -273,6 +273,31 @@ typedef struct fbp_lct_params {
 fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* params, float* out,
                    void* stream);
 
+/* Confocal voxel backprojection evaluated only at the listed voxels (linear
+ * indices z*X*Y + vy*X + vx): values[i] = sum_l [I > 1e-5] I(l, bin) *
+ * safeRCP(dist^2) for voxel voxels[i], with the reference's IEEE chain per
+ * pair (FB/transient_reconstruction.cuh:93-136) and an FP64 sum.  F32
+ * time-fastest transients.  values: double[count] (device). */
+fbp_status fbp_backproject_confocal_points(fbp_context ctx, const fbp_reconstruction_info* info,
+                                           const fbp_transient* transient,
+                                           const fbp_vec3* laser_targets, const uint32_t* voxels,
+                                           uint32_t count, double* values, void* stream);
+
+/* Learned form (no reference counterpart): maps a refined, min-max
+ * normalised LoG volume (volume, float[Z][Y][X], IN/OUT) onto the exactly
+ * normalised LoG of the plain backprojection.  reference (float[Z][Y][X]):
+ * an unbiased estimate of the LoG volume in physical units whose noise is
+ * independent of volume's (e.g. the LoG of a second MIS draw); its
+ * least-squares fit on volume over all voxels gives the LoG as a v + b.
+ * The true range is the exact LoG (kernel_size 5, sigma) at the `extremes`
+ * (1..32) lowest and highest voxels of volume, from exact backprojections of
+ * their 5^3 clamp-to-edge neighbourhoods.  Output (a v + b - min) *
+ * safeRCP(max - min).  F32 time-fastest transients. */
+fbp_status fbp_learned_calibrate(fbp_context ctx, const fbp_reconstruction_info* info,
+                                 const fbp_transient* transient, const fbp_vec3* laser_targets,
+                                 float* volume, const float* reference, int32_t kernel_size,
+                                 float sigma, uint32_t extremes, void* stream);
+
 /* ------------------------------------------------------ scene forward model */
 
 /* Confocal transient of a set of Lambertian points: for every scan point l and

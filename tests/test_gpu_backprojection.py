@@ -511,3 +511,17 @@ def test_mis_tiled_scatter_matches_oracle(scatter_ctx):
                                                48, mode=MIS_TILED)
     torch.cuda.synchronize()
     assert_volume_close(out, ref)
+
+
+# --------------------------------------------------- exact backprojection at points
+@pytest.mark.parametrize("discard,t0", [(1, 0.0), (0, 0.3)])
+def test_backproject_points_matches_oracle(ctx, discard, t0):
+    g = geom(discard=discard, t0=t0)
+    info = make_info(g)
+    rng = np.random.default_rng(61)
+    I = spiky(rng, (len(g.laser_targets), g.num_time_bins))
+    ref = oracle.confocal_voxel(info, I, g.laser_targets).reshape(-1)
+    idx = rng.choice(ref.size, 300, replace=False).astype(np.int32)
+    out = ctx.backproject_points(info, cuda(I), cuda(g.laser_targets), cuda(idx))
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(out.cpu().numpy(), ref[idx], rtol=2e-6, atol=0)
