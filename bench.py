@@ -67,6 +67,8 @@ def parse():
                     help="run a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--no-lct", action="store_true",
                     help="skip the LCT leg (128^2 x 1024 padded to 256x256x2048)")
+    ap.add_argument("--no-depth", action="store_true",
+                    help="skip the depth-plane mode + slice images leg")
     ap.add_argument("--no-exhaustive", action="store_true",
                     help="skip the exhaustive leg (16^2 lasers x 64^2 sensors x 1024 -> 64^3)")
     return ap.parse_args()
@@ -185,6 +187,7 @@ def main():
     learned = None if args.no_learned else learned_leg(ctx)
     exhaustive = None if args.no_exhaustive else exhaustive_leg(ctx, args)
     lct = None if args.no_lct else lct_leg(ctx, args)
+    depth = None if args.no_depth else depth_leg(ctx, args)
 
     out = {
         "metric": METRIC,
@@ -228,6 +231,7 @@ def main():
         "learned": learned,
         "exhaustive": exhaustive,
         "lct": lct,
+        "depth_planes": depth,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
@@ -367,6 +371,103 @@ def lct_leg(ctx, args, reps=5):
                                                    np.linalg.norm(ref))}
     log(f"lct: {res}")
     del I, data, out
+    torch.cuda.empty_cache()
+    return res
+
+
+def depth_leg(ctx, args, reps=3):
+    """Depth-plane mode on the full confocal capture (128x128 scan, 1024 bins
+    of 16 ps): 200 planes at focus +- 0.1 (FB/Laser.cu:365-377), unweighted
+    sums (aligned grid: the ring convolution of the voxel path with weight 1;
+    the gather kernel timed beside it; priced like the voxel path, 16 flops
+    per (pixel, scan point, depth) update against FP32 VALU peak), min-max
+    normalise, then the per-depth RGBA images enlarged to 256x256
+    (slice_bytes_kernel + slice_rgba_kernel, HBM-bound: activation read
+    4 B + byte planes written and read 2 B per (pixel, depth), 4 B per output
+    pixel written).  CPU: the oracle's depth gather on the first 2 planes."""
+    import numpy as np
+    import torch
+
+    from fasterbackprojection_amd.api import make_info
+    from fasterbackprojection_amd.scene import full_confocal_geometry, random_quads
+
+    dev = torch.device("cuda", 0)
+    g = full_confocal_geometry(128, 1024, 128)
+    info = make_info(g)
+    sc = random_quads(np.random.default_rng(3), 4, 20000)
+    laser = torch.from_numpy(g.laser_targets).to(dev)
+    clean = ctx.render_confocal(laser, torch.from_numpy(sc.points.astype(np.float32)).to(dev),
+                                torch.from_numpy(sc.albedo.astype(np.float32)).to(dev), 1024,
+                                g.time_step)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(3)
+    I = torch.poisson(clean * (1000.0 / clean.max()), generator=gen).contiguous()
+    L, D = 128 * 128, 200
+    focus = abs((np.float32(g.hidden_min[1]) + np.float32(g.hidden_max[1])) * np.float32(0.5))
+    lo = float(np.float32(focus - np.float32(0.1)))
+    hi = float(np.float32(focus + np.float32(0.1)))
+    depths = torch.tensor(lo + (hi - lo) / D * np.arange(D), dtype=torch.float64, device=dev)
+    act = ctx.backproject_depth(info, I, laser, depths)
+    ctx.normalize(act)
+    img = ctx.depth_slices_rgba(act, 128, 128, 256, 256)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        ms = []
+        for _ in range(reps):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            b.record(stream)
+            b.synchronize()
+            ms.append(a.elapsed_time(b))
+        return float(np.median(ms))
+
+    t_bp = timed(lambda: ctx.backproject_depth(info, I, laser, depths, out=act))
+    aligned = ctx.last_confocal_path_aligned()
+    ctx.set_aligned_path(False)
+    try:
+        t_gather = timed(lambda: ctx.backproject_depth(info, I, laser, depths, out=act))
+    finally:
+        ctx.set_aligned_path(True)
+    ctx.backproject_depth(info, I, laser, depths, out=act)
+    ctx.normalize(act)
+    t_norm = timed(lambda: ctx.normalize(act))
+    t_img = timed(lambda: ctx.depth_slices_rgba(act, 128, 128, 256, 256, out=img))
+    updates = L * L * D
+    achieved = updates * FLOPS_PER_UPDATE / (t_bp * 1e-3) / 1e12
+    img_bytes = L * D * (4 + 2) + D * 256 * 256 * 4
+    img_gbps = img_bytes / (t_img * 1e-3) / 1e9
+    res = {"workload": "depth planes: confocal 128x128x1024 (16 ps), 200 depths at focus +- 0.1 "
+                       "-> activation [16384][200] + 200 RGBA images 256x256",
+           "ms": {"depth_backprojection": t_bp, "normalize": t_norm, "slice_images": t_img,
+                  "depth_backprojection_gather_kernel": t_gather},
+           "path": "aligned ring convolution" if aligned else "gather kernel",
+           "updates": updates,
+           "roofline_depth_backprojection": {"bound": "valu", "achieved": achieved,
+                                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                                     "frac": achieved / PEAK_FP32_TFLOPS},
+           "roofline_slice_images": {"bound": "hbm", "achieved": img_gbps, "peak": PEAK_HBM_GBPS,
+                                     "unit": "GB/s", "frac": img_gbps / PEAK_HBM_GBPS,
+                                     "algorithmic_bytes": img_bytes}}
+    if not args.no_cpu_baseline:
+        import oracle
+        Ih = I.cpu().numpy()
+        d2 = depths[:2].cpu().numpy()
+        t0 = time.perf_counter()
+        ref = oracle.depth(info, Ih, g.laser_targets, d2)
+        cpu_s = time.perf_counter() - t0
+        raw = ctx.backproject_depth(info, I, laser, depths[:2].contiguous())
+        got = raw.cpu().numpy().astype(np.float64)
+        res["cpu"] = {"sample": "oracle depth gather, first 2 of the 200 planes",
+                      "updates_per_s": L * L * 2 / cpu_s,
+                      "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                      "kind": "port",
+                      "rel_l2_gpu_vs_cpu": float(np.linalg.norm(got - ref) / np.linalg.norm(ref))}
+    log(f"depth: {res}")
+    del I, act, img
     torch.cuda.empty_cache()
     return res
 

@@ -89,6 +89,8 @@ SIGNATURES = {
     "fbp_backproject_exhaustive_voxel": (_S, [_CTX, C.POINTER(ReconstructionInfo), C.POINTER(Transient), _P, _P, _P, _P]),
     "fbp_backproject_depth_confocal": (_S, [_CTX, C.POINTER(ReconstructionInfo), C.POINTER(Transient), _P, _P, C.c_uint32, _P, _P]),
     "fbp_backproject_depth_exhaustive": (_S, [_CTX, C.POINTER(ReconstructionInfo), C.POINTER(Transient), _P, _P, _P, C.c_uint32, _P, _P]),
+    "fbp_depth_slices_rgba": (_S, [_CTX, _P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P]),
+    "fbp_write_png_rgba": (_S, [C.c_char_p, _P, C.c_uint32, C.c_uint32]),
     "fbp_backproject_confocal_mis": (_S, [_CTX, C.POINTER(ReconstructionInfo), C.POINTER(Transient), _P, C.POINTER(MisParams), _P, _P]),
     "fbp_build_alias_tables_host": (_S, [_P, C.c_uint32, _P, _P]),
     "fbp_quantize_transient": (_S, [_CTX, _P, C.c_uint64, C.c_uint32, C.c_int32, _P, _P, _P]),

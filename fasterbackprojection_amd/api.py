@@ -200,6 +200,23 @@ class Context:
         self.check(st)
         return out
 
+    def depth_slices_rgba(self, activation, width, height, out_width=None, out_height=None,
+                          out=None):
+        """Per-depth RGBA slice images of TransientImage::save
+        (FB/TransientImage.cpp:27-45, FB/Image.h:45-177): activation
+        [width*height][D] -> uint8 [D][out_height][out_width][4]."""
+        if activation.dim() != 2 or activation.shape[0] != width * height:
+            raise ValueError("activation must be [width*height][num_depths]")
+        D = int(activation.shape[1])
+        ow = int(out_width or width)
+        oh = int(out_height or height)
+        if out is None:
+            out = torch.empty((D, oh, ow, 4), dtype=torch.uint8, device=self._dev())
+        self.check(self.lib.fbp_depth_slices_rgba(
+            self.handle, _ptr(activation.contiguous()), int(width), int(height), D, ow, oh,
+            _ptr(out), _stream()))
+        return out
+
     def backproject_confocal_mis(self, info, intensity, laser_targets, noise, num_samples,
                                  mode=MIS_REFERENCE, out=None):
         """Laser::reconstructAABBConfocalMIS (FB/Laser.cu:526-611)."""

@@ -85,7 +85,7 @@ def build(force: bool = True, jobs: int | None = None) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     cmd = ([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp)]
            + [str(o) for o in objs]
-           + ["-L/opt/rocm/lib", "-lrocfft", "-Wl,-rpath,/opt/rocm/lib"])
+           + ["-L/opt/rocm/lib", "-lrocfft", "-lz", "-Wl,-rpath,/opt/rocm/lib"])
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")

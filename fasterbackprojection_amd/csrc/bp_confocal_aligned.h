@@ -27,6 +27,14 @@ struct AlignedPlan {
 AlignedPlan plan_aligned_confocal(fbp_context ctx, const fbp_reconstruction_info& info,
                                   const fbp_transient& tr);
 
+// Depth-plane mode (backprojectConfocal, FB/transient_reconstruction.cuh:41-63):
+// pure chain, F32 time-fastest input, a 64^2 or 128^2 scan grid, and a wall
+// normal with zero x and z (the planes move the scan points along y).  The
+// outputs are the scan points themselves at every depth (X = VY = nx,
+// ZD = num_depths), weight 1.
+AlignedPlan plan_aligned_depth(fbp_context ctx, const fbp_reconstruction_info& info,
+                               const fbp_transient& tr, uint32_t num_depths);
+
 size_t aligned_scratch_bytes(const AlignedPlan& p);
 
 // Launches the grid check (writes *flag), the offset tables, the reorder
@@ -37,5 +45,12 @@ fbp_status launch_aligned_confocal(fbp_context ctx, const AlignedPlan& p,
                                    const fbp_reconstruction_info& info, const fbp_transient& tr,
                                    const fbp_vec3* laser, float* volume, int* flag, void* base,
                                    hipStream_t s);
+
+// The same for the depth-plane activation [nx^2][num_depths] (depths: device
+// doubles).  The tables are rebuilt on every call.
+fbp_status launch_aligned_depth(fbp_context ctx, const AlignedPlan& p,
+                                const fbp_reconstruction_info& info, const fbp_transient& tr,
+                                const fbp_vec3* laser, const double* depths, uint32_t num_depths,
+                                float* activation, int* flag, void* base, hipStream_t s);
 
 }  // namespace fbp

@@ -65,24 +65,41 @@ struct AlignedParams {
     float t0, dt;
     uint32_t discard;
     float hmin[3], vs[3];
+    // Depth-plane mode (backprojectConfocal): the "voxels" are the scan
+    // points moved along the wall normal (0, ny, 0) by depths[z]; weight 1.
+    uint32_t depth_mode;
+    float ny;
+    const double* depths;
 };
 
-__device__ __forceinline__ float px_of(const AlignedParams& a, uint32_t vx) {
-    return voxel_center(a.hmin[0], a.vs[0], vx);
+// World x of voxel column vx and world z of voxel row vy.  Depth-plane mode:
+// checkPos = laser[c] + n * depth with n.x = n.z = 0, so x and z are the
+// scan point's own (adding a zero leaves x unchanged up to the sign of a
+// zero, which squaring removes).
+__device__ __forceinline__ float px_of(const AlignedParams& a, const fbp_vec3* laser, uint32_t vx) {
+    return a.depth_mode ? laser[vx].x : voxel_center(a.hmin[0], a.vs[0], vx);
 }
-__device__ __forceinline__ float pz_of(const AlignedParams& a, uint32_t vy) {
-    return voxel_center(a.hmin[2], a.vs[2], vy);
+__device__ __forceinline__ float pz_of(const AlignedParams& a, const fbp_vec3* laser, uint32_t vy) {
+    return a.depth_mode ? laser[size_t(vy) * a.nx].z : voxel_center(a.hmin[2], a.vs[2], vy);
+}
+
+// fl(fl(py - ly)^2) of depth index z (every scan point has the y of laser[0]).
+__device__ __forceinline__ float y2_of(const AlignedParams& a, const fbp_vec3* laser, uint32_t z) {
+    const float ly = laser[0].y;
+    const float py = a.depth_mode ? rn_add(ly, rn_mul(a.ny, static_cast<float>(a.depths[z])))
+                                  : voxel_center(a.hmin[1], a.vs[1], z);
+    return sq(rn_sub(py, ly));
 }
 
 // fl(fl(px - lx)^2) of phase 0 for the signed offset n = vx - j
 // (ix = R j), from a representative pair.
 __device__ __forceinline__ float x2_signed(const AlignedParams& a, const fbp_vec3* laser, int n) {
-    return n >= 0 ? sq(rn_sub(px_of(a, uint32_t(n)), laser[0].x))
-                  : sq(rn_sub(px_of(a, 0), laser[size_t(-n) * a.R].x));
+    return n >= 0 ? sq(rn_sub(px_of(a, laser, uint32_t(n)), laser[0].x))
+                  : sq(rn_sub(px_of(a, laser, 0), laser[size_t(-n) * a.R].x));
 }
 __device__ __forceinline__ float z2_signed(const AlignedParams& a, const fbp_vec3* laser, int m) {
-    return m >= 0 ? sq(rn_sub(pz_of(a, uint32_t(m)), laser[0].z))
-                  : sq(rn_sub(pz_of(a, 0), laser[size_t(-m) * a.R * a.nx].z));
+    return m >= 0 ? sq(rn_sub(pz_of(a, laser, uint32_t(m)), laser[0].z))
+                  : sq(rn_sub(pz_of(a, laser, 0), laser[size_t(-m) * a.R * a.nx].z));
 }
 
 // Device-resident cache header of the offset tables: the parameters and
@@ -95,7 +112,7 @@ struct TableCache {
 __device__ __forceinline__ bool same_params(const AlignedParams& x, const AlignedParams& y) {
     bool eq = x.nx == y.nx && x.R == y.R && x.pn == y.pn && x.classes == y.classes &&
               x.X == y.X && x.VY == y.VY && x.ZD == y.ZD && x.kq == y.kq && x.mq == y.mq &&
-              x.T == y.T && x.discard == y.discard;
+              x.T == y.T && x.discard == y.discard && x.depth_mode == y.depth_mode;
     // Bit patterns: -0 / +0 and NaNs must not compare equal by value.
     eq = eq && __float_as_uint(x.t0) == __float_as_uint(y.t0) &&
          __float_as_uint(x.dt) == __float_as_uint(y.dt);
@@ -107,14 +124,15 @@ __device__ __forceinline__ bool same_params(const AlignedParams& x, const Aligne
 
 // flag[0] = 1 iff the grid conditions hold for every scan point and pair.
 // flag[1] = 1 iff the cached tables were built for these parameters and
-// exactly these scan points (then the table kernel is skipped).  The init
-// kernel sets both from the cache header; the check kernel's workgroups
-// clear them for any failing item.
+// exactly these scan points (then the table kernel is skipped; never in
+// depth-plane mode, whose depths live in device memory).  The init kernel
+// sets both from the cache header; the check kernel's workgroups clear them
+// for any failing item.
 __global__ void aligned_check_init_kernel(AlignedParams a, int* __restrict__ flag,
                                           const TableCache* __restrict__ cache) {
     if (threadIdx.x == 0) {
         flag[0] = 1;
-        flag[1] = cache->valid == 1 && same_params(cache->key, a);
+        flag[1] = !a.depth_mode && cache->valid == 1 && same_params(cache->key, a);
     }
 }
 
@@ -144,13 +162,13 @@ __global__ __launch_bounds__(256) void aligned_check_kernel(const fbp_vec3* __re
             // at n, phase 1 (R = 2) at -n; R = 1 needs an even table.
             const uint32_t q = t - L, vx = q % a.X, ix = q / a.X;
             const int n = int(vx) - int(ix / a.R);
-            const float v = sq(rn_sub(px_of(a, vx), laser[ix].x));
+            const float v = sq(rn_sub(px_of(a, laser, vx), laser[ix].x));
             good = good && v == x2_signed(a, laser, (ix % a.R) ? -n : n);
             if (a.R == 1 && n > 0) good = good && v == x2_signed(a, laser, -n);
         } else {
             const uint32_t q = t - L - NX, vy = q % a.VY, iz = q / a.VY;
             const int m = int(vy) - int(iz / a.R);
-            const float v = sq(rn_sub(pz_of(a, vy), laser[size_t(iz) * a.nx].z));
+            const float v = sq(rn_sub(pz_of(a, laser, vy), laser[size_t(iz) * a.nx].z));
             good = good && v == z2_signed(a, laser, (iz % a.R) ? -m : m);
             if (a.R == 1 && m > 0) good = good && v == z2_signed(a, laser, -m);
         }
@@ -191,7 +209,7 @@ __global__ __launch_bounds__(1024) void aligned_table_kernel(const fbp_vec3* __r
     const int sn = (c & 1u) ? -1 : 1, smg = (c & 2u) ? -1 : 1;
     for (uint32_t k = tid; k < a.kq; k += blockDim.x) X2[k] = x2_signed(a, laser, sn * int(k));
     for (uint32_t m = tid; m < a.mq; m += blockDim.x) Z2[m] = z2_signed(a, laser, smg * int(m));
-    const float Y2 = sq(rn_sub(voxel_center(a.hmin[1], a.vs[1], z), laser[0].y));
+    const float Y2 = y2_of(a, laser, z);
     __syncthreads();
     const uint32_t n = a.kq * a.mq;
     for (uint32_t e = tid; e < sort_n; e += blockDim.x) {
@@ -237,7 +255,8 @@ __global__ __launch_bounds__(1024) void aligned_table_kernel(const fbp_vec3* __r
                 const uint32_t k = e % a.kq, m = e / a.kq;
                 const float s = dot3(X2[k], Y2, Z2[m]);
                 const float dist = confocal_dist(s, a.t0, a.discard, 0.f);
-                ent = make_uint2(k | (m << 16), __float_as_uint(safe_rcp(rn_mul(dist, dist))));
+                const float w = a.depth_mode ? 1.0f : safe_rcp(rn_mul(dist, dist));
+                ent = make_uint2(k | (m << 16), __float_as_uint(w));
                 // R = 1: up to four sign combinations on the one plane; R = 2:
                 // one application per phase plane (separate workgroups).
                 wk = (groups - k / 32u) *
@@ -658,9 +677,12 @@ __global__ void cache_reset_kernel(TableCache* cache) {
     if (threadIdx.x == 0) cache->valid = 0;
 }
 
+// planes == 0: volume[v] (v = (z * VY + vy) * X + vx); else the depth-plane
+// activation [VY * X][planes]: out[(v % plane) * planes + v / plane].
 __global__ void aligned_reduce_kernel(const float* __restrict__ partial, float* __restrict__ volume,
                                       size_t nvox, uint32_t parts, const int* __restrict__ flag,
-                                      TableCache* __restrict__ cache) {
+                                      TableCache* __restrict__ cache, uint32_t plane,
+                                      uint32_t planes) {
     if (*flag == 0) return;
     // The tables (cached or rebuilt by the table kernel, which has finished)
     // now match the key recorded in the cache header.
@@ -669,7 +691,8 @@ __global__ void aligned_reduce_kernel(const float* __restrict__ partial, float* 
     if (v >= nvox) return;
     double s = 0.0;
     for (uint32_t c = 0; c < parts; ++c) s += double(partial[size_t(c) * nvox + v]);
-    volume[v] = float(s);
+    const size_t o = planes ? (v % plane) * planes + v / plane : v;
+    volume[o] = float(s);
 }
 
 uint32_t lds_stride(uint32_t pn) {
@@ -686,27 +709,16 @@ uint32_t pow2_at_least(uint32_t n) {
 
 }  // namespace
 
-AlignedPlan plan_aligned_confocal(fbp_context ctx, const fbp_reconstruction_info& info,
-                                  const fbp_transient& tr) {
+namespace {
+
+// Sizes of the plan for an nx^2 scan grid, X x VY x ZD outputs and T bins
+// (conditions on X, VY and T checked by the caller).
+AlignedPlan size_plan(fbp_context ctx, const fbp_transient& tr, uint32_t nx, uint32_t X,
+                      uint32_t VY, uint32_t ZD, uint32_t T) {
     AlignedPlan p{};
     p.usable = false;
-    if (ctx->disable_aligned) return p;
-    const bool pure = info.time_offset == 0.f && info.discard_first_last_bounces != 0;
-    if (!pure) return p;
-    // F32 in either layout; F16/BF16 time-fastest with per-series scales.
-    if (tr.storage != FBP_STORAGE_F32 && tr.layout != FBP_LAYOUT_TIME_FASTEST) return p;
-    const uint32_t L = info.num_laser_targets;
-    const uint32_t nx = uint32_t(std::lround(std::sqrt(double(L))));
-    if (nx * nx != L || nx < 16) return p;
-    const uint32_t X = info.voxel_resolution.x, VY = info.voxel_resolution.y,
-                   ZD = info.voxel_resolution.z;
-    // Phase planes of pn = X = 32 * (TXC / 4) points, TXC in {8, 16}; R = nx / X
-    // scan points per voxel pitch; VY a multiple of 16, at most pn.
-    if (!(X == 128 || X == 64) || nx % X != 0) return p;
+    const uint32_t L = nx * nx;
     const uint32_t R = nx / X;
-    if (R != 1 && R != 2) return p;
-    if (VY % 16 != 0 || VY == 0 || VY > X || ZD == 0) return p;
-    if (info.num_time_bins == 0 || info.num_time_bins >= (1u << 16)) return p;
     p.nx = nx;
     p.R = R;
     p.pn = X;
@@ -725,13 +737,59 @@ AlignedPlan plan_aligned_confocal(fbp_context ctx, const fbp_reconstruction_info
     p.chunks = std::max(1u, std::min(64u, (target + per_chunk - 1) / per_chunk));
     const size_t nvox = size_t(X) * VY * ZD;
     p.bytes_entries = size_t(ZD) * p.classes * p.n * sizeof(uint2);
-    p.bytes_starts = size_t(ZD) * p.classes * (4 * size_t(info.num_time_bins) + 1) * sizeof(uint32_t);
-    p.bytes_work = size_t(ZD) * p.classes * (size_t(info.num_time_bins) + 1) * sizeof(uint32_t);
+    p.bytes_starts = size_t(ZD) * p.classes * (4 * size_t(T) + 1) * sizeof(uint32_t);
+    p.bytes_work = size_t(ZD) * p.classes * (size_t(T) + 1) * sizeof(uint32_t);
     const bool direct = R == 1 && tr.storage == FBP_STORAGE_F32 && tr.layout == FBP_LAYOUT_TIME_MAJOR;
-    p.bytes_tm = direct ? 0 : size_t(L) * info.num_time_bins * 4;
+    p.bytes_tm = direct ? 0 : size_t(L) * T * 4;
     p.bytes_partial = size_t(R) * R * p.chunks * nvox * sizeof(float);
     p.usable = true;
     return p;
+}
+
+// nx when L is a square nx^2 with nx >= 16, else 0.
+uint32_t square_edge(uint32_t L) {
+    const uint32_t nx = uint32_t(std::lround(std::sqrt(double(L))));
+    return (nx * nx == L && nx >= 16) ? nx : 0u;
+}
+
+}  // namespace
+
+AlignedPlan plan_aligned_confocal(fbp_context ctx, const fbp_reconstruction_info& info,
+                                  const fbp_transient& tr) {
+    AlignedPlan none{};
+    none.usable = false;
+    if (ctx->disable_aligned) return none;
+    const bool pure = info.time_offset == 0.f && info.discard_first_last_bounces != 0;
+    if (!pure) return none;
+    // F32 in either layout; F16/BF16 time-fastest with per-series scales.
+    if (tr.storage != FBP_STORAGE_F32 && tr.layout != FBP_LAYOUT_TIME_FASTEST) return none;
+    const uint32_t nx = square_edge(info.num_laser_targets);
+    if (nx == 0) return none;
+    const uint32_t X = info.voxel_resolution.x, VY = info.voxel_resolution.y,
+                   ZD = info.voxel_resolution.z;
+    // Phase planes of pn = X = 32 * (TXC / 4) points, TXC in {8, 16}; R = nx / X
+    // scan points per voxel pitch; VY a multiple of 16, at most pn.
+    if (!(X == 128 || X == 64) || nx % X != 0) return none;
+    const uint32_t R = nx / X;
+    if (R != 1 && R != 2) return none;
+    if (VY % 16 != 0 || VY == 0 || VY > X || ZD == 0) return none;
+    if (info.num_time_bins == 0 || info.num_time_bins >= (1u << 16)) return none;
+    return size_plan(ctx, tr, nx, X, VY, ZD, info.num_time_bins);
+}
+
+AlignedPlan plan_aligned_depth(fbp_context ctx, const fbp_reconstruction_info& info,
+                               const fbp_transient& tr, uint32_t num_depths) {
+    AlignedPlan none{};
+    none.usable = false;
+    if (ctx->disable_aligned) return none;
+    const bool pure = info.time_offset == 0.f && info.discard_first_last_bounces != 0;
+    if (!pure || tr.storage != FBP_STORAGE_F32 || tr.layout != FBP_LAYOUT_TIME_FASTEST) return none;
+    // The wall normal moves the scan points along y only.
+    if (info.relay_wall_normal.x != 0.f || info.relay_wall_normal.z != 0.f) return none;
+    const uint32_t nx = square_edge(info.num_laser_targets);
+    if (!(nx == 64 || nx == 128) || num_depths == 0) return none;
+    if (info.num_time_bins == 0 || info.num_time_bins >= (1u << 16)) return none;
+    return size_plan(ctx, tr, nx, nx, nx, num_depths, info.num_time_bins);
 }
 
 static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
@@ -756,30 +814,11 @@ static fbp_status launch_conv(fbp_context ctx, const ConvArgs& c, const uint2* e
     return FBP_OK;
 }
 
-fbp_status launch_aligned_confocal(fbp_context ctx, const AlignedPlan& p,
-                                   const fbp_reconstruction_info& info, const fbp_transient& tr,
-                                   const fbp_vec3* laser, float* volume, int* flag, void* base,
-                                   hipStream_t s) {
-    AlignedParams a{};
-    a.nx = p.nx;
-    a.R = p.R;
-    a.pn = p.pn;
-    a.classes = p.classes;
-    a.X = info.voxel_resolution.x;
-    a.VY = info.voxel_resolution.y;
-    a.ZD = info.voxel_resolution.z;
-    a.kq = p.kq;
-    a.mq = p.mq;
-    a.T = info.num_time_bins;
-    a.t0 = info.time_offset;
-    a.dt = info.time_step;
-    a.discard = info.discard_first_last_bounces;
-    a.hmin[0] = info.hidden_volume_min.x;
-    a.hmin[1] = info.hidden_volume_min.y;
-    a.hmin[2] = info.hidden_volume_min.z;
-    a.vs[0] = info.hidden_volume_voxel_size.x;
-    a.vs[1] = info.hidden_volume_voxel_size.y;
-    a.vs[2] = info.hidden_volume_voxel_size.z;
+// Check, tables, phase planes, convolution and reduction for the prepared
+// parameters; depth_planes != 0 writes the depth-plane activation layout.
+static fbp_status launch_aligned(fbp_context ctx, const AlignedPlan& p, const AlignedParams& a,
+                                 const fbp_transient& tr, const fbp_vec3* laser, float* volume,
+                                 int* flag, void* base, hipStream_t s, uint32_t depth_planes) {
 
     // Offset tables live in their own allocation (kept across calls, rebuilt
     // only when the parameters or the scan points change); the phase planes
@@ -886,9 +925,56 @@ fbp_status launch_aligned_confocal(fbp_context ctx, const AlignedPlan& p,
     if (st != FBP_OK) return st;
     const size_t nvox = size_t(a.X) * a.VY * a.ZD;
     hipLaunchKernelGGL(aligned_reduce_kernel, dim3(unsigned((nvox + 255) / 256)), dim3(256), 0, s,
-                       partial, volume, nvox, p.chunks * p.R * p.R, flag, cache);
+                       partial, volume, nvox, p.chunks * p.R * p.R, flag, cache, a.X * a.VY,
+                       depth_planes);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     return FBP_OK;
+}
+
+static AlignedParams params_of(const AlignedPlan& p, const fbp_reconstruction_info& info) {
+    AlignedParams a{};
+    a.nx = p.nx;
+    a.R = p.R;
+    a.pn = p.pn;
+    a.classes = p.classes;
+    a.kq = p.kq;
+    a.mq = p.mq;
+    a.T = info.num_time_bins;
+    a.t0 = info.time_offset;
+    a.dt = info.time_step;
+    a.discard = info.discard_first_last_bounces;
+    return a;
+}
+
+fbp_status launch_aligned_confocal(fbp_context ctx, const AlignedPlan& p,
+                                   const fbp_reconstruction_info& info, const fbp_transient& tr,
+                                   const fbp_vec3* laser, float* volume, int* flag, void* base,
+                                   hipStream_t s) {
+    AlignedParams a = params_of(p, info);
+    a.X = info.voxel_resolution.x;
+    a.VY = info.voxel_resolution.y;
+    a.ZD = info.voxel_resolution.z;
+    a.hmin[0] = info.hidden_volume_min.x;
+    a.hmin[1] = info.hidden_volume_min.y;
+    a.hmin[2] = info.hidden_volume_min.z;
+    a.vs[0] = info.hidden_volume_voxel_size.x;
+    a.vs[1] = info.hidden_volume_voxel_size.y;
+    a.vs[2] = info.hidden_volume_voxel_size.z;
+    return launch_aligned(ctx, p, a, tr, laser, volume, flag, base, s, 0);
+}
+
+fbp_status launch_aligned_depth(fbp_context ctx, const AlignedPlan& p,
+                                const fbp_reconstruction_info& info, const fbp_transient& tr,
+                                const fbp_vec3* laser, const double* depths, uint32_t num_depths,
+                                float* activation, int* flag, void* base, hipStream_t s) {
+    AlignedParams a = params_of(p, info);
+    a.X = p.nx;
+    a.VY = p.nx;
+    a.ZD = num_depths;
+    a.depth_mode = 1;
+    a.ny = info.relay_wall_normal.y;
+    a.depths = depths;
+    return launch_aligned(ctx, p, a, tr, laser, activation, flag, base, s, num_depths);
 }
 
 }  // namespace fbp

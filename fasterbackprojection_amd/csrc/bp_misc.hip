@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "bp_common.h"
+#include "bp_confocal_aligned.h"
 
 namespace fbp {
 
@@ -12,14 +13,20 @@ namespace fbp {
 // Reference: backprojectConfocal / backprojectExhaustive
 // (FB/transient_reconstruction.cuh:41-91).  One thread per output (c, d)
 // gathers over all scan points / pairs in a fixed order (no atomics).  No
-// 1/r^2 weight, as in the reference.
+// 1/r^2 weight, as in the reference.  Confocal captures on an aligned 64^2
+// or 128^2 scan grid with the wall normal along y take the ring-convolution
+// path instead (bp_confocal_aligned.hip, depth-plane mode): there the bin of
+// a (pixel, scan point) pair depends only on their grid offset and the
+// depth.
 
 template <bool EXHAUSTIVE>
 __global__ void depth_kernel(const float* __restrict__ I, const fbp_vec3* __restrict__ laser,
                              const fbp_vec3* __restrict__ sensor, const double* __restrict__ depths,
                              uint32_t D, uint32_t L, uint32_t S, uint32_t T, float t0, float dt,
                              uint32_t discard, fbp_vec3 n, fbp_vec3 lpos, fbp_vec3 spos,
-                             float* __restrict__ act) {
+                             float* __restrict__ act, const int* __restrict__ skip) {
+    // The aligned-grid path wrote the activation (its check passed).
+    if (skip && *skip != 0) return;
     const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= L * D) return;
     const uint32_t c = idx / D, d = idx % D;
@@ -411,11 +418,25 @@ fbp_status fbp_backproject_depth_confocal(fbp_context ctx, const fbp_reconstruct
     FBP_CHECK(bind(ctx));
     const uint32_t L = info->num_laser_targets;
     const uint32_t n = L * num_depths;
-    hipLaunchKernelGGL(depth_kernel<false>, dim3((n + 255) / 256), dim3(256), 0, as_stream(stream),
+    hipStream_t s = as_stream(stream);
+    // Aligned scan grid: ring convolution per depth plane (decided on the
+    // device by the grid check; the gather below returns at once then).
+    const AlignedPlan ap = plan_aligned_depth(ctx, *info, *tr, num_depths);
+    const int* skip = nullptr;
+    ctx->path_flag_valid = ap.usable;
+    if (ap.usable) {
+        FBP_CHECK(ensure_scratch(ctx, aligned_scratch_bytes(ap)));
+        if (!ctx->path_flag)
+            FBP_CHECK_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->path_flag), 256));
+        FBP_CHECK(launch_aligned_depth(ctx, ap, *info, *tr, laser_targets, depths, num_depths,
+                                       activation, ctx->path_flag, ctx->scratch, s));
+        skip = ctx->path_flag;
+    }
+    hipLaunchKernelGGL(depth_kernel<false>, dim3((n + 255) / 256), dim3(256), 0, s,
                        static_cast<const float*>(tr->data), laser_targets, laser_targets, depths,
                        num_depths, L, 1u, info->num_time_bins, info->time_offset, info->time_step,
                        info->discard_first_last_bounces, info->relay_wall_normal,
-                       info->laser_position, info->sensor_position, activation);
+                       info->laser_position, info->sensor_position, activation, skip);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     return FBP_OK;
 }
@@ -440,7 +461,7 @@ fbp_status fbp_backproject_depth_exhaustive(fbp_context ctx, const fbp_reconstru
                        num_depths, L, info->num_sensor_targets, info->num_time_bins,
                        info->time_offset, info->time_step, info->discard_first_last_bounces,
                        info->relay_wall_normal, info->laser_position, info->sensor_position,
-                       activation);
+                       activation, nullptr);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     return FBP_OK;
 }

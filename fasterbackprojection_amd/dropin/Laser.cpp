@@ -1,11 +1,15 @@
 #include "Laser.h"
 
+#include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <cmath>
 #include <filesystem>
 #include <fstream>
 #include <iostream>
 #include <random>
 #include <stdexcept>
+#include <thread>
 
 #include "Runtime.h"
 
@@ -110,7 +114,7 @@ void Laser::reconstructShapeDepths(const ReconstructionInfo& recInfo,
                                    const ReconstructionBuffers& recBuffers,
                                    const TransientParameters& transientParams) {
     // FB/Laser.cu:365-480: planes at focus +- 0.1, unweighted sums, min-max
-    // normalised.  The reference writes PNG slices; the activation is kept in
+    // normalised, one PNG image per depth.  The activation is also kept in
     // memory and written raw to <outputFolder>depths.raw ([L][D] float32).
     fbp_context ctx = fbp::Runtime::instance().context();
     const float focus = recInfo.getFocusDepth();
@@ -144,11 +148,50 @@ void Laser::reconstructShapeDepths(const ReconstructionInfo& recInfo,
     std::cout << "Reconstruction finished in " << _elapsedMs << " milliseconds.\n";
     _activation.resize(n);
     fbp::download(_activation.data(), act, n * sizeof(float));
+    std::filesystem::create_directories(transientParams._outputFolder);
+    {
+        std::ofstream f(transientParams._outputFolder + "depths.raw", std::ios::binary);
+        if (f) f.write(reinterpret_cast<const char*>(_activation.data()), std::streamsize(n * 4));
+    }
+    // One image per depth, <outputFolder>transient_<d>.png (FB/Laser.cu:410-424,
+    // 463-476): res = floor(sqrt(S)) pixels a side, enlarged to
+    // res * (res < 256 ? 256 / res : 1) (confocal) or 4 res (exhaustive).
+    uint32_t res = uint32_t(std::sqrt(double(recInfo._numSensorTargets)));
+    while (uint64_t(res) * res > recInfo._numSensorTargets) --res;
+    if (res == 0 || uint64_t(res) * res > recInfo._numLaserTargets) {
+        fbp::deviceFree(dDepths);
+        fbp::deviceFree(act);
+        throw std::runtime_error("Laser: depth images need at least one sensor target per pixel");
+    }
+    const bool confocal = recInfo._captureSystem == CaptureSystem::Confocal;
+    const uint32_t side = confocal ? res * (res < 256u ? 256u / res : 1u) : res * 4u;
+    const size_t imageBytes = size_t(side) * side * 4;
+    uint8_t* dRgba = static_cast<uint8_t*>(fbp::deviceAlloc(imageBytes * D));
+    st = fbp_depth_slices_rgba(ctx, act, res, res, D, side, side, dRgba, nullptr);
+    std::vector<uint8_t> rgba;
+    if (st == FBP_OK) {
+        rgba.resize(imageBytes * D);
+        fbp::download(rgba.data(), dRgba, rgba.size());
+    }
+    fbp::deviceFree(dRgba);
     fbp::deviceFree(dDepths);
     fbp::deviceFree(act);
-    std::filesystem::create_directories(transientParams._outputFolder);
-    std::ofstream f(transientParams._outputFolder + "depths.raw", std::ios::binary);
-    if (f) f.write(reinterpret_cast<const char*>(_activation.data()), std::streamsize(n * 4));
+    fbp::check(st, "reconstructShapeDepths: slice images");
+    std::vector<std::thread> workers;
+    std::atomic<uint32_t> next{0}, failed{0};
+    const unsigned nw = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    for (unsigned w = 0; w < nw; ++w)
+        workers.emplace_back([&] {
+            for (uint32_t d = next++; d < D; d = next++) {
+                const std::string name =
+                    transientParams._outputFolder + "transient_" + std::to_string(d) + ".png";
+                if (fbp_write_png_rgba(name.c_str(), rgba.data() + d * imageBytes, side, side) !=
+                    FBP_OK)
+                    ++failed;
+            }
+        });
+    for (auto& t : workers) t.join();
+    if (failed) throw std::runtime_error("Laser: could not write the depth images");
 }
 
 void Laser::reconstructAABBConfocalMIS(float* volume, const ReconstructionInfo& recInfo) const {

@@ -125,9 +125,10 @@ typedef enum fbp_option {
 fbp_status fbp_context_set_option(fbp_context ctx, int32_t option, int32_t value);
 fbp_status fbp_context_get_option(fbp_context ctx, int32_t option, int32_t* value);
 
-/* Which path the last fbp_backproject_confocal_voxel call on this context
- * took: *aligned = 1 for the aligned-grid path, 0 for the general kernel.
- * Synchronises the device. */
+/* Which path the last fbp_backproject_confocal_voxel or
+ * fbp_backproject_depth_confocal call on this context took: *aligned = 1 for
+ * the aligned-grid path, 0 for the general (gather) kernel.  Synchronises
+ * the device. */
 fbp_status fbp_context_last_confocal_path(fbp_context ctx, int32_t* aligned);
 
 /* --------------------------------------------------------- backprojection */
@@ -166,7 +167,10 @@ fbp_status fbp_backproject_exhaustive_voxel(fbp_context ctx, const fbp_reconstru
  * Laser::reconstructDepthConfocal / Exhaustive
  * (FB/transient_reconstruction.cuh:41-91, FB/Laser.cu:379-480).
  * depths: double[num_depths] (device).  activation: float[L][num_depths],
- * OVERWRITTEN with the unweighted sum of I > 1e-5 samples. */
+ * OVERWRITTEN with the unweighted sum of I > 1e-5 samples.  Confocal calls on
+ * a 64^2 or 128^2 scan grid with the wall normal along y, t0 = 0 and the
+ * first/last bounces discarded run the ring-convolution path when the
+ * device-side grid check passes (same sums, FP32 partials reduced in FP64). */
 fbp_status fbp_backproject_depth_confocal(fbp_context ctx, const fbp_reconstruction_info* info,
                                           const fbp_transient* transient,
                                           const fbp_vec3* laser_targets, const double* depths,
@@ -176,6 +180,30 @@ fbp_status fbp_backproject_depth_exhaustive(fbp_context ctx, const fbp_reconstru
                                             const fbp_vec3* laser_targets,
                                             const fbp_vec3* sensor_targets, const double* depths,
                                             uint32_t num_depths, float* activation, void* stream);
+
+/* Depth-plane slice images: replaces the per-depth image output of
+ * Laser::reconstructDepthConfocal / Exhaustive (FB/Laser.cu:410-424, 463-476)
+ * made by TransientImage::save (FB/TransientImage.cpp:27-45) with
+ * Image<uchar>::flipImageVertically / resize / interpolate
+ * (FB/Image.h:45-67, 93-137, 161-177).
+ * activation: float[width * height][num_depths] (device, depth fastest, as
+ * written by fbp_backproject_depth_*; normally min-max normalised).
+ * rgba: uint8[num_depths][out_height][out_width][4] (device, 4-byte aligned):
+ * gray = clamp(uint(a * 255), 0, 255) (negative and NaN -> 0), alpha 255,
+ * rows flipped, then, when the output size differs, the reference's FP32
+ * bilinear resampling of all four channels with ratio (W-1)/(W'-1).  The
+ * reference's sizes: confocal W' = W * (W < 256 ? 256 / W : 1), exhaustive
+ * W' = 4 W.  Edges at most 65535 (16-bit in the reference). */
+fbp_status fbp_depth_slices_rgba(fbp_context ctx, const float* activation, uint32_t width,
+                                 uint32_t height, uint32_t num_depths, uint32_t out_width,
+                                 uint32_t out_height, uint8_t* rgba, void* stream);
+
+/* PNG file (8-bit RGBA, zlib-compressed) of one host image, rows top to
+ * bottom: the SOIL_save_image(..., SOIL_SAVE_TYPE_PNG, w, h, 4, ...) call of
+ * TransientImage::save (FB/TransientImage.cpp:44).  The encoded bytes differ
+ * from SOIL's encoder; the decoded pixels are the input bytes. */
+fbp_status fbp_write_png_rgba(const char* path, const uint8_t* rgba, uint32_t width,
+                              uint32_t height);
 
 /* MIS-sampled confocal backprojection: replaces Laser::reconstructAABBConfocalMIS,
  * buildAliasTables, spatioTemporalPrefixSum and backprojectConfocalVoxelMIS

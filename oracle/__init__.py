@@ -56,6 +56,8 @@ def load() -> C.CDLL:
                                              P]
         _lib.oracle_alias_tables.restype = C.c_int
         _lib.oracle_alias_tables.argtypes = [P, C.c_uint32, P, P]
+        _lib.oracle_depth_slices_rgba.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32,
+                                                  C.c_uint32, C.c_uint32, P]
         _lib.oracle_laplacian.argtypes = [P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int]
         _lib.oracle_log_kernel.argtypes = [C.c_int, C.c_double, P]
         _lib.oracle_log_filter.argtypes = [P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, P]
@@ -223,6 +225,18 @@ def mis_tiled(info, I, laser, noise, num_samples, scatter=False) -> np.ndarray:
         vol = (confocal_voxel_scatter if scatter else confocal_voxel)(info, It, laser)
         sl = (slice(16 * bz, 16 * bz + 16), slice(16 * by, 16 * by + 16), slice(16 * bx, 16 * bx + 16))
         out[sl] = vol[sl]
+    return out
+
+
+def depth_slices_rgba(act, width, height, out_width=None, out_height=None) -> np.ndarray:
+    """TransientImage::save images of every depth plane
+    (FB/TransientImage.cpp:27-45, FB/Image.h:45-177): act [W*H][D] float32 ->
+    uint8 [D][OH][OW][4]."""
+    act = _f32(act)
+    D = act.shape[1]
+    ow, oh = int(out_width or width), int(out_height or height)
+    out = np.empty((D, oh, ow, 4), np.uint8)
+    load().oracle_depth_slices_rgba(_p(act), width, height, D, ow, oh, _p(out))
     return out
 
 

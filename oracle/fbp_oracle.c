@@ -545,3 +545,65 @@ void oracle_filter_h(const float* data, uint64_t batch, uint32_t nt, float dt, f
     }
     free(kr); free(ki); free(er);
 }
+
+/* ---- Depth-plane slice images (TransientImage::save) ---------------------
+ * FB/TransientImage.cpp:27-45 with Image<uchar> (FB/Image.h:45-67, 93-137,
+ * 161-177), in the reference's order: build the W x H RGBA image of plane d
+ * (gray = clamp(uint(a[p * D + d] * 255), 0, 255), alpha 255), flip it
+ * vertically, then, when (OW, OH) differs, resize it bilinearly channel by
+ * channel with the FP32 expression of Image::interpolate.  Negative and NaN
+ * activations (undefined in the reference's cast) give 0.
+ * out: uint8[D][OH][OW][4]. */
+static unsigned char slice_gray(float v) {
+    float s = v * 255.0f;
+    if (!(s > 0.0f)) return 0;
+    if (s >= 4294967040.0f) return 255;
+    uint32_t u = (uint32_t)s;
+    return (unsigned char)(u > 255u ? 255u : u);
+}
+
+void oracle_depth_slices_rgba(const float* act, uint32_t W, uint32_t H, uint32_t D, uint32_t OW,
+                              uint32_t OH, unsigned char* out) {
+    const size_t P = (size_t)W * H;
+    unsigned char* img = (unsigned char*)malloc(P * 4);
+    unsigned char* row = (unsigned char*)malloc((size_t)W * 4);
+    for (uint32_t d = 0; d < D; ++d) {
+        for (size_t p = 0; p < P; ++p) {
+            unsigned char g = slice_gray(act[p * D + d]);
+            img[p * 4 + 0] = img[p * 4 + 1] = img[p * 4 + 2] = g;
+            img[p * 4 + 3] = 255;
+        }
+        /* flipImageVertically: swap row i with row H-1-i. */
+        for (uint32_t i = 0; i < H / 2; ++i) {
+            memcpy(row, img + (size_t)i * W * 4, (size_t)W * 4);
+            memcpy(img + (size_t)i * W * 4, img + (size_t)(H - 1 - i) * W * 4, (size_t)W * 4);
+            memcpy(img + (size_t)(H - 1 - i) * W * 4, row, (size_t)W * 4);
+        }
+        unsigned char* o = out + (size_t)d * OW * OH * 4;
+        if (OW == W && OH == H) {
+            memcpy(o, img, P * 4);
+            continue;
+        }
+        const float xRatio = (float)(W - 1) / (float)(OW - 1);
+        const float yRatio = (float)(H - 1) / (float)(OH - 1);
+        for (uint32_t y = 0; y < OH; ++y)
+            for (uint32_t x = 0; x < OW; ++x) {
+                const float xs = (float)(int)x * xRatio, ys = (float)(int)y * yRatio;
+                const int x0 = (int)xs, y0 = (int)ys;
+                const int x1 = x0 + 1 < (int)W - 1 ? x0 + 1 : (int)W - 1;
+                const int y1 = y0 + 1 < (int)H - 1 ? y0 + 1 : (int)H - 1;
+                const float dx = xs - (float)x0, dy = ys - (float)y0;
+                for (int c = 0; c < 4; ++c) {
+                    const float p00 = img[((size_t)y0 * W + x0) * 4 + c];
+                    const float p01 = img[((size_t)y0 * W + x1) * 4 + c];
+                    const float p10 = img[((size_t)y1 * W + x0) * 4 + c];
+                    const float p11 = img[((size_t)y1 * W + x1) * 4 + c];
+                    const float v = (1.0f - dx) * (1.0f - dy) * p00 + dx * (1.0f - dy) * p01 +
+                                    (1.0f - dx) * dy * p10 + dx * dy * p11;
+                    o[((size_t)y * OW + x) * 4 + c] = (unsigned char)v;
+                }
+            }
+    }
+    free(row);
+    free(img);
+}

@@ -358,6 +358,72 @@ def test_depth_planes(ctx):
     np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=1e-5, atol=1e-6)
 
 
+DEPTH_ALIGNED = {
+    "64": dict(nx=64, wall=1.0, T=256, lo=0.3, hi=0.5, D=20),
+    "128": dict(nx=128, wall=2.0, T=512, lo=0.6, hi=0.9, D=4),
+    "64_one_depth": dict(nx=64, wall=1.0, T=256, lo=0.45, hi=0.46, D=1),
+}
+
+
+def depth_geom(nx, wall, T, **_):
+    scan = wall_grid(nx, wall)
+    return Geometry(laser_targets=scan, sensor_targets=scan, num_time_bins=T, time_step=0.0096,
+                    hidden_min=(-0.5, 0.2, -0.5), hidden_max=(0.5, 1.0, 0.5),
+                    voxel_resolution=(16, 16, 16), discard_first_last_bounces=1,
+                    laser_position=(0.1, -0.8, 0.2), sensor_position=(-0.2, -0.7, 0.1),
+                    wall_size=(wall, wall))
+
+
+@pytest.mark.parametrize("name", sorted(DEPTH_ALIGNED))
+def test_depth_planes_aligned_path(ctx, name):
+    """Depth-plane mode on a 64^2 / 128^2 scan grid with the wall normal
+    along y: the ring-convolution path (weight 1, pixels = scan points)
+    against the oracle's gather, and the general kernel on the same input."""
+    cfg = DEPTH_ALIGNED[name]
+    g = depth_geom(**cfg)
+    info = make_info(g)
+    rng = np.random.default_rng(41)
+    I = spiky(rng, (len(g.laser_targets), g.num_time_bins))
+    depths = np.linspace(cfg["lo"], cfg["hi"], cfg["D"], endpoint=False)
+    ref = oracle.depth(info, I, g.laser_targets, depths)
+    out = ctx.backproject_depth(info, cuda(I), cuda(g.laser_targets), cuda(depths))
+    assert ctx.last_confocal_path_aligned()
+    assert_volume_close(out, ref)
+    ctx.set_aligned_path(False)
+    try:
+        gen = ctx.backproject_depth(info, cuda(I), cuda(g.laser_targets), cuda(depths))
+        assert not ctx.last_confocal_path_aligned()
+    finally:
+        ctx.set_aligned_path(True)
+    assert_volume_close(gen, ref)
+
+
+def test_depth_planes_aligned_check_and_host_conditions(ctx):
+    """A scan point moved by one ulp (device check) or a wall normal with an
+    x component (host check) sends the depth planes to the gather kernel;
+    a voxel backprojection between depth calls rebuilds its own tables."""
+    cfg = DEPTH_ALIGNED["64"]
+    depths = np.linspace(cfg["lo"], cfg["hi"], 8, endpoint=False)
+    rng = np.random.default_rng(42)
+    g = depth_geom(**cfg)
+    I = spiky(rng, (len(g.laser_targets), g.num_time_bins))
+    moved = depth_geom(**cfg)
+    moved.laser_targets = moved.laser_targets.copy()
+    moved.laser_targets[300, 0] = np.nextafter(moved.laser_targets[300, 0], np.float32(1.0))
+    tilted = depth_geom(**cfg)
+    tilted.wall_normal = (0.0, 0.8, 0.6)
+    vg = aligned_geom(**ALIGNED["64"])
+    for gg, aligned in [(g, True), (moved, False), (tilted, False), (g, True)]:
+        info = make_info(gg)
+        out = ctx.backproject_depth(info, cuda(I), cuda(gg.laser_targets), cuda(depths))
+        assert ctx.last_confocal_path_aligned() == aligned
+        assert_volume_close(out, oracle.depth(info, I, gg.laser_targets, depths))
+        vinfo = make_info(vg)
+        vol = ctx.backproject_confocal(vinfo, cuda(I), cuda(vg.laser_targets))
+        assert ctx.last_confocal_path_aligned()
+        assert_volume_close(vol, oracle.confocal_voxel(vinfo, I, vg.laser_targets))
+
+
 def test_mis_reference_mode_matches_oracle(ctx):
     cap = small_confocal(seed=4, res=16)
     info = make_info(cap.geometry)

@@ -12,6 +12,7 @@ from fasterbackprojection_amd import nlos_io
 from fasterbackprojection_amd.api import make_info
 from fasterbackprojection_amd.build import CLI
 from fasterbackprojection_amd.scene import small_confocal
+from pngio import read_rgba
 
 pytestmark = pytest.mark.gpu
 
@@ -64,6 +65,13 @@ def test_reconstruct_shape_depths(capture, tmp_path):
     depths = lo + (hi - lo) / 50 * np.arange(50)
     ref = oracle.normalize(oracle.depth(info, cap.transient, cap.geometry.laser_targets, depths))
     assert rel_l2(got, ref) < 1e-5
+    # One PNG per depth (FB/Laser.cu:410-424): 16 x 16 pixels enlarged to
+    # 16 * (256 / 16) = 256 a side, bytes as TransientImage::save makes them
+    # from the same activation.
+    slices = oracle.depth_slices_rgba(got, 16, 16, 256, 256)
+    for d in (0, 17, 49):
+        assert (read_rgba(tmp_path / f"transient_{d}.png") == slices[d]).all(), d
+    assert not (tmp_path / "transient_50.png").exists()
 
 
 def test_lct_class(capture, tmp_path):
