@@ -32,6 +32,7 @@ FLOPS_PER_UPDATE = 16          # confocal, FMA = 2 (reference formula)
 PEAK_FP32_TFLOPS = 157.3       # MI355X FP32 (datasheet)
 PEAK_LDS_TBPS = 78.6           # 256 CUs x 128 B/clk x 2.4 GHz
 PEAK_HBM_GBPS = 8000.0         # MI355X HBM3E (datasheet)
+VALU_LANE_SLOTS = 39.3e12      # 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (non-packed issue)
 TRAFFIC_PROFILE = ROOT / "profiles" / "confocal_128cube_traffic.json"
 
 
@@ -172,6 +173,28 @@ def main():
     general_ms = time_bp(3)
     ctx.set_aligned_path(True)
     log(f"general kernel on the same input: {general_ms:.3f} ms")
+    # End to end from host memory: pinned transient -> device, one step,
+    # normalised volume -> pinned host buffer.
+    I_host = torch.empty(I.shape, dtype=I.dtype, pin_memory=True)
+    I_host.copy_(I)
+    vol_host = torch.empty(filt.shape, dtype=filt.dtype, pin_memory=True)
+    I_dev = torch.empty_like(I)
+    e2e = []
+    for _ in range(3):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        I_dev.copy_(I_host, non_blocking=True)
+        ctx.backproject_confocal(info, I_dev, laser, out=vol)
+        ctx.log(vol, 5, 1.0, out=filt)
+        ctx.normalize(filt)
+        vol_host.copy_(filt, non_blocking=True)
+        b.record(stream)
+        b.synchronize()
+        e2e.append(a.elapsed_time(b))
+    e2e_ms = float(np.median(e2e))
+    del I_host, vol_host, I_dev
+    log(f"end to end with host transfers: {e2e_ms:.3f} ms")
     if args.copy_reference_mib:
         # Plain device copy of a known size: calibrates FETCH_SIZE / WRITE_SIZE
         # when this run is profiled with rocprofv3 --pmc.
@@ -225,7 +248,21 @@ def main():
                     "unit": "TB/s", "frac": 4.0 * updates / (bp_avg * 1e-3) / 1e12 / PEAK_LDS_TBPS},
         },
         "breakdown_ms": {"backprojection": bp_avg, "log_and_normalize": float(np.mean(f_ms)),
-                         "backprojection_general_kernel": general_ms},
+                         "backprojection_general_kernel": general_ms,
+                         "end_to_end_with_h2d_d2h": e2e_ms},
+        # Issue-slot view (non-packed FP32: 39.3e12 lane-slots/s; a
+        # transcendental takes 4 slots).  Aligned path: 1.27 FMA slots and
+        # 1.27 LDS reads per useful update (LDS, at 2 clocks per wave-read
+        # per CU, binds).  General kernel: 13.9 slots per update counted in
+        # its ISA (10.9 VALU + v_rsq).
+        "issue_slot_diagnostic": {
+            "aligned_conv_kernel": {
+                "lds_bound_updates_per_s": PEAK_LDS_TBPS * 1e12 / 4.0 / 1.27,
+                "frac": updates / (bp_avg * 1e-3) / (PEAK_LDS_TBPS * 1e12 / 4.0 / 1.27)},
+            "confocal_voxel_kernel": {
+                "valu_slots_per_update": 13.9,
+                "bound_updates_per_s": VALU_LANE_SLOTS / 13.9,
+                "frac": updates / (general_ms * 1e-3) / (VALU_LANE_SLOTS / 13.9)}},
         "cpu_baseline": cpu,
         "precision": precision,
         "learned": learned,
@@ -266,9 +303,14 @@ def precision_leg(ctx, args):
     sub.voxel_resolution.z = nz
     t = time.perf_counter()
     ref = oracle.confocal_voxel(sub, I.cpu().numpy(), g.laser_targets)
-    log(f"precision leg: FP64 oracle on {nz} slices in {time.perf_counter() - t:.1f} s")
+    cpu_s = time.perf_counter() - t
+    log(f"precision leg: FP64 oracle on {nz} slices in {cpu_s:.1f} s")
     out = {"workload": "confocal 256x256x1024 (16 ps, 8 quads, Poisson peak 500) -> 128^3",
-           "oracle_slices": nz}
+           "oracle_slices": nz,
+           "cpu": {"sample": f"FP64 OpenMP oracle on the first {nz} of 128 depth slices",
+                   "voxel_updates_per_s": 128 * 128 * nz * L / cpu_s,
+                   "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                   "kind": "port"}}
     stream = torch.cuda.current_stream()
     for name, st in (("f32", STORAGE_F32), ("f16", STORAGE_F16), ("bf16", STORAGE_BF16)):
         if st == STORAGE_F32:
@@ -520,7 +562,27 @@ def exhaustive_leg(ctx, args, reps=3):
 
     bp = timed(lambda: ctx.backproject_exhaustive(info, I, lasers, sensors, out=vol))
     lap = timed(lambda: ctx.laplacian(vol, 5, out=filt))
+    cpu = None
+    if not args.no_cpu_baseline:
+        # The oracle on the first depth slice (64x64 voxels x all pairs).
+        import oracle
+        sub = type(info).from_buffer_copy(info)
+        sub.voxel_resolution.z = 1
+        Ih = I.cpu().numpy()
+        t0 = time.perf_counter()
+        ref = oracle.exhaustive_voxel(sub, Ih, g.laser_targets, g.sensor_targets)
+        cpu_s = time.perf_counter() - t0
+        del Ih
+        got = vol[:1].cpu().numpy().astype(np.float64)
+        cpu = {"sample": "FP64 OpenMP oracle on the first of 64 depth slices (64x64 voxels x "
+                         "all 1,048,576 pairs)",
+               "pair_updates_per_s": L * S * res * res / cpu_s,
+               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+               "kind": "port",
+               "rel_l2_gpu_vs_cpu": float(np.linalg.norm(got - ref) / np.linalg.norm(ref))}
     tflops = 25 * updates / (bp * 1e-3) / 1e12
+    # Fast path per pair-update: 11 VALU + v_rsq + v_rcp = 17 issue slots.
+    slot_bound = VALU_LANE_SLOTS / 17.0
     out = {"workload": "exhaustive 16^2 lasers x 64^2 sensors x 1024 (16 ps, 4 GiB) -> 64^3 "
                        "+ Laplacian(k=5)",
            "pair_updates": updates, "backprojection_ms": bp, "laplacian_ms": lap,
@@ -528,7 +590,10 @@ def exhaustive_leg(ctx, args, reps=3):
            "roofline": {"bound": "valu", "achieved": tflops, "peak": PEAK_FP32_TFLOPS,
                         "unit": "TFLOP/s", "frac": tflops / PEAK_FP32_TFLOPS,
                         "flops_per_update": 25},
-           "finite": bool(torch.isfinite(vol).all())}
+           "issue_slot_diagnostic": {"valu_slots_per_update": 17, "bound_updates_per_s": slot_bound,
+                                     "frac": updates / (bp * 1e-3) / slot_bound},
+           "finite": bool(torch.isfinite(vol).all()),
+           "cpu": cpu}
     log(f"exhaustive: {out}")
     del I
     torch.cuda.empty_cache()
@@ -590,6 +655,16 @@ def learned_leg(ctx, n_scenes=16):
     return res
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(info, I, g, target_seconds):
     """FP64 OpenMP oracle on the first z-slices of the same volume."""
     import numpy as np
@@ -610,10 +685,24 @@ def cpu_baseline(info, I, g, target_seconds):
     t1 = run(1)
     nz = int(max(1, min(info.voxel_resolution.z, target_seconds / max(t1, 1e-3))))
     t = run(nz) if nz > 1 else t1
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cores = oracle.max_threads()
     log(f"cpu oracle: {nz} slices in {t:.2f} s on {cores} threads")
+    # One core on a smaller slab (a quarter of one slice's rows).
+    oracle.set_threads(1)
+    try:
+        sub.voxel_resolution.z = 1
+        sub.voxel_resolution.y = max(1, Y // 4)
+        t0 = time.perf_counter()
+        oracle.confocal_voxel(sub, host, laser)
+        t_one = time.perf_counter() - t0
+    finally:
+        oracle.set_threads(cores)
+    one_core = X * max(1, Y // 4) * L / t_one
+    log(f"cpu oracle, 1 thread: {one_core:.3e} voxel-updates/s")
     return {
         "value": X * Y * nz * L / t,
+        "value_one_core": one_core,
+        "cpu_model": cpu_model(),
         "unit": "voxel-updates/s",
         "cores": cores,
         "kind": "port",

@@ -56,6 +56,8 @@ def load() -> C.CDLL:
                                              P]
         _lib.oracle_alias_tables.restype = C.c_int
         _lib.oracle_alias_tables.argtypes = [P, C.c_uint32, P, P]
+        _lib.oracle_set_threads.argtypes = [C.c_int]
+        _lib.oracle_max_threads.restype = C.c_int
         _lib.oracle_depth_slices_rgba.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32,
                                                   C.c_uint32, C.c_uint32, P]
         _lib.oracle_laplacian.argtypes = [P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int]
@@ -226,6 +228,15 @@ def mis_tiled(info, I, laser, noise, num_samples, scatter=False) -> np.ndarray:
         sl = (slice(16 * bz, 16 * bz + 16), slice(16 * by, 16 * by + 16), slice(16 * bx, 16 * bx + 16))
         out[sl] = vol[sl]
     return out
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the oracle's loops."""
+    load().oracle_set_threads(int(n))
+
+
+def max_threads() -> int:
+    return int(load().oracle_max_threads())
 
 
 def depth_slices_rgba(act, width, height, out_width=None, out_height=None) -> np.ndarray:

@@ -20,6 +20,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <omp.h>
+
 typedef struct { float x, y, z; } vec3;
 typedef struct { uint32_t x, y, z; } uvec3;
 
@@ -607,3 +609,8 @@ void oracle_depth_slices_rgba(const float* act, uint32_t W, uint32_t H, uint32_t
     free(row);
     free(img);
 }
+
+/* Threads of the OpenMP loops (the CPU baseline is timed at all cores and
+ * at one). */
+void oracle_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int oracle_max_threads(void) { return omp_get_max_threads(); }
