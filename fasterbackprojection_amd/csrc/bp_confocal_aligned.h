@@ -47,7 +47,8 @@ fbp_status launch_aligned_confocal(fbp_context ctx, const AlignedPlan& p,
                                    hipStream_t s);
 
 // The same for the depth-plane activation [nx^2][num_depths] (depths: device
-// doubles).  The tables are rebuilt on every call.
+// doubles).  The tables are kept while the parameters, the scan points and
+// the depth values stay the same (compared on the device).
 fbp_status launch_aligned_depth(fbp_context ctx, const AlignedPlan& p,
                                 const fbp_reconstruction_info& info, const fbp_transient& tr,
                                 const fbp_vec3* laser, const double* depths, uint32_t num_depths,

@@ -113,6 +113,7 @@ __device__ __forceinline__ bool same_params(const AlignedParams& x, const Aligne
     bool eq = x.nx == y.nx && x.R == y.R && x.pn == y.pn && x.classes == y.classes &&
               x.X == y.X && x.VY == y.VY && x.ZD == y.ZD && x.kq == y.kq && x.mq == y.mq &&
               x.T == y.T && x.discard == y.discard && x.depth_mode == y.depth_mode;
+    eq = eq && __float_as_uint(x.ny) == __float_as_uint(y.ny);
     // Bit patterns: -0 / +0 and NaNs must not compare equal by value.
     eq = eq && __float_as_uint(x.t0) == __float_as_uint(y.t0) &&
          __float_as_uint(x.dt) == __float_as_uint(y.dt);
@@ -124,28 +125,30 @@ __device__ __forceinline__ bool same_params(const AlignedParams& x, const Aligne
 
 // flag[0] = 1 iff the grid conditions hold for every scan point and pair.
 // flag[1] = 1 iff the cached tables were built for these parameters and
-// exactly these scan points (then the table kernel is skipped; never in
-// depth-plane mode, whose depths live in device memory).  The init kernel
-// sets both from the cache header; the check kernel's workgroups clear them
-// for any failing item.
+// exactly these scan points (and, in depth-plane mode, these depths): then
+// the table kernel is skipped.  The init kernel sets both from the cache
+// header; the check kernel's workgroups clear them for any failing item.
 __global__ void aligned_check_init_kernel(AlignedParams a, int* __restrict__ flag,
                                           const TableCache* __restrict__ cache) {
     if (threadIdx.x == 0) {
         flag[0] = 1;
-        flag[1] = !a.depth_mode && cache->valid == 1 && same_params(cache->key, a);
+        flag[1] = cache->valid == 1 && same_params(cache->key, a);
     }
 }
 
 // Items: scan points l < L (on the grid; equal to the cached copy), then
-// (voxel column, scan column) pairs, then (voxel row, scan row) pairs.
+// (voxel column, scan column) pairs, then (voxel row, scan row) pairs, then
+// in depth-plane mode the depths (equal to the cached copy).
 __global__ __launch_bounds__(256) void aligned_check_kernel(const fbp_vec3* __restrict__ laser,
                                                             AlignedParams a, int* __restrict__ flag,
-                                                            const fbp_vec3* __restrict__ cached_laser) {
+                                                            const fbp_vec3* __restrict__ cached_laser,
+                                                            const double* __restrict__ cached_depths) {
     const uint32_t L = a.nx * a.nx, NX = a.X * a.nx, NZ = a.VY * a.nx;
+    const uint32_t ND = a.depth_mode ? a.ZD : 0u;
     const float y0 = laser[0].y;
     const bool cmp = flag[1] != 0;
     bool good = true, eq = true;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < L + NX + NZ;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < L + NX + NZ + ND;
          t += gridDim.x * blockDim.x) {
         if (t < L) {
             const uint32_t ix = t % a.nx, iz = t / a.nx;
@@ -165,12 +168,15 @@ __global__ __launch_bounds__(256) void aligned_check_kernel(const fbp_vec3* __re
             const float v = sq(rn_sub(px_of(a, laser, vx), laser[ix].x));
             good = good && v == x2_signed(a, laser, (ix % a.R) ? -n : n);
             if (a.R == 1 && n > 0) good = good && v == x2_signed(a, laser, -n);
-        } else {
+        } else if (t < L + NX + NZ) {
             const uint32_t q = t - L - NX, vy = q % a.VY, iz = q / a.VY;
             const int m = int(vy) - int(iz / a.R);
             const float v = sq(rn_sub(pz_of(a, laser, vy), laser[size_t(iz) * a.nx].z));
             good = good && v == z2_signed(a, laser, (iz % a.R) ? -m : m);
             if (a.R == 1 && m > 0) good = good && v == z2_signed(a, laser, -m);
+        } else if (cmp) {
+            const uint32_t z = t - L - NX - NZ;
+            eq = eq && __double_as_longlong(a.depths[z]) == __double_as_longlong(cached_depths[z]);
         }
     }
     // One atomic per wave that found a failing item.
@@ -191,7 +197,8 @@ __global__ __launch_bounds__(1024) void aligned_table_kernel(const fbp_vec3* __r
                                                              uint32_t* __restrict__ starts,
                                                              uint32_t* __restrict__ work,
                                                              TableCache* __restrict__ cache,
-                                                             fbp_vec3* __restrict__ cached_laser) {
+                                                             fbp_vec3* __restrict__ cached_laser,
+                                                             double* __restrict__ cached_depths) {
     if (flag[0] == 0 || flag[1] == 1) return;
     const uint32_t z = blockIdx.x, c = blockIdx.y;
     if (z == 0 && c == 0) {
@@ -199,6 +206,8 @@ __global__ __launch_bounds__(1024) void aligned_table_kernel(const fbp_vec3* __r
         // every table workgroup has finished) marks it valid.
         if (threadIdx.x == 0) cache->valid = 0;
         for (uint32_t l = threadIdx.x; l < a.nx * a.nx; l += blockDim.x) cached_laser[l] = laser[l];
+        if (a.depth_mode)
+            for (uint32_t d = threadIdx.x; d < a.ZD; d += blockDim.x) cached_depths[d] = a.depths[d];
         if (threadIdx.x == 0) cache->key = a;
     }
     extern __shared__ uint32_t keys[];  // sort_n (power of two), then sort_n work sums
@@ -824,8 +833,10 @@ static fbp_status launch_aligned(fbp_context ctx, const AlignedPlan& p, const Al
     // only when the parameters or the scan points change); the phase planes
     // and the partial sums use the shared scratch.
     const size_t header = align256(sizeof(TableCache)) + align256(size_t(65536) * sizeof(fbp_vec3));
+    const size_t depth_bytes = a.depth_mode ? size_t(a.ZD) * sizeof(double) : 0;
     const size_t table_bytes = header + align256(p.bytes_entries + kEntryPad) +
-                               align256(p.bytes_starts) + align256(p.bytes_work);
+                               align256(p.bytes_starts) + align256(p.bytes_work) +
+                               align256(depth_bytes);
     if (ctx->aligned_tables_bytes < table_bytes) {
         if (ctx->aligned_tables) {
             FBP_CHECK_HIP(ctx, hipDeviceSynchronize());
@@ -849,6 +860,9 @@ static fbp_status launch_aligned(fbp_context ctx, const AlignedPlan& p, const Al
     uint32_t* starts = reinterpret_cast<uint32_t*>(tcur);
     tcur += align256(p.bytes_starts);
     uint32_t* work = reinterpret_cast<uint32_t*>(tcur);
+    tcur += align256(p.bytes_work);
+    // Depth-plane mode: the depths the tables were built for.
+    double* cached_depths = reinterpret_cast<double*>(tcur);
 
     char* cur = static_cast<char*>(base);
     float* tm = reinterpret_cast<float*>(cur);
@@ -858,17 +872,18 @@ static fbp_status launch_aligned(fbp_context ctx, const AlignedPlan& p, const Al
     hipLaunchKernelGGL(aligned_check_init_kernel, dim3(1), dim3(64), 0, s, a, flag, cache);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     {
-        const uint32_t items = p.nx * p.nx + (a.X + a.VY) * p.nx;
+        const uint32_t items = p.nx * p.nx + (a.X + a.VY) * p.nx + (a.depth_mode ? a.ZD : 0u);
         const unsigned blocks = std::min(256u, (items + 255) / 256);
         hipLaunchKernelGGL(aligned_check_kernel, dim3(blocks), dim3(256), 0, s, laser, a, flag,
-                           cached_laser);
+                           cached_laser, cached_depths);
         FBP_CHECK_HIP(ctx, hipGetLastError());
     }
     FBP_CHECK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(aligned_table_kernel),
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            int(p.sort_n * 8)));
     hipLaunchKernelGGL(aligned_table_kernel, dim3(a.ZD, p.classes), dim3(1024), size_t(p.sort_n) * 8,
-                       s, laser, a, flag, p.sort_n, entries, starts, work, cache, cached_laser);
+                       s, laser, a, flag, p.sort_n, entries, starts, work, cache, cached_laser,
+                       cached_depths);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     const uint32_t L = p.nx * p.nx;
     const float* planes = static_cast<const float*>(tr.data);

@@ -424,6 +424,31 @@ def test_depth_planes_aligned_check_and_host_conditions(ctx):
         assert_volume_close(vol, oracle.confocal_voxel(vinfo, I, vg.laser_targets))
 
 
+def test_depth_planes_aligned_table_cache(ctx):
+    """Depth-plane tables are reused for the same depths and rebuilt when a
+    depth value changes (by one ulp), when the number of depths changes, and
+    after a voxel backprojection used the tables."""
+    cfg = DEPTH_ALIGNED["64"]
+    g = depth_geom(**cfg)
+    info = make_info(g)
+    rng = np.random.default_rng(43)
+    I = spiky(rng, (len(g.laser_targets), g.num_time_bins))
+    d1 = np.linspace(cfg["lo"], cfg["hi"], 6, endpoint=False)
+    d2 = d1.copy()
+    d2[3] = float(np.nextafter(np.float32(d1[3]), np.float32(1.0)))  # next float depth
+    d3 = d1[:5].copy()
+    vg = aligned_geom(**ALIGNED["64"])
+    vinfo = make_info(vg)
+    for depths in (d1, d1, d2, d1, d3, "voxel", d3):
+        if isinstance(depths, str):
+            vol = ctx.backproject_confocal(vinfo, cuda(I), cuda(vg.laser_targets))
+            assert_volume_close(vol, oracle.confocal_voxel(vinfo, I, vg.laser_targets))
+            continue
+        out = ctx.backproject_depth(info, cuda(I), cuda(g.laser_targets), cuda(depths))
+        assert ctx.last_confocal_path_aligned()
+        assert_volume_close(out, oracle.depth(info, I, g.laser_targets, depths))
+
+
 def test_mis_reference_mode_matches_oracle(ctx):
     cap = small_confocal(seed=4, res=16)
     info = make_info(cap.geometry)
