@@ -41,6 +41,9 @@ struct FftCache {
     float2* invpsf = nullptr;
     uint32_t op_m = 0;
     Csr mtx, mtxi;
+    // LCT z-pass twiddles exp(-2 pi i k / (2M)), k < M
+    uint32_t tw_m = 0;
+    float2* tw = nullptr;
 };
 
 std::once_flag g_rocfft_once;
@@ -233,6 +236,108 @@ __global__ void lct_resample_kernel(const float* __restrict__ in, float* __restr
     }
 }
 
+// Padded planes z < M of the resampled volume: pad[z][y][x] (2N x 2N) =
+// sum_j A[z][j] in[j][y][x] for y, x < N, zero elsewhere (no separate fill).
+__global__ void lct_resample_planes_kernel(const float* __restrict__ in, float* __restrict__ pad,
+                                           const int* __restrict__ rowptr,
+                                           const int* __restrict__ cols,
+                                           const float* __restrict__ vals, uint32_t N, uint32_t M) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t P = size_t(4) * N * N;
+    if (i >= P * M) return;
+    const uint32_t z = uint32_t(i / P);
+    const uint32_t q = uint32_t(i % P), y = q / (2 * N), x = q % (2 * N);
+    float acc = 0.f;
+    if (y < N && x < N) {
+        const size_t NN = size_t(N) * N, p = size_t(y) * N + x;
+        for (int e = rowptr[z]; e < rowptr[z + 1]; ++e)
+            acc = fmaf(vals[e], in[size_t(cols[e]) * NN + p], acc);
+    }
+    pad[i] = acc;
+}
+
+// ----------------------------------------------------------- LCT z-pass
+// For every (ky, kx) column of the half spectrum S[z][ky][kx] of the
+// planes z < M (2D transforms of the padded planes), in one workgroup:
+//   X[k] = sum_{z < M} S[z] e^{-2 pi i z k / 2M}    (the upper M planes of
+//          the padded volume are zero: pruned input)
+//   Y[k] = X[k] * inv[k][ky][kx]                     (Wiener / inverse filter)
+//   y[z] = sum_k Y[k] e^{+2 pi i z k / 2M}, z < M    (crop: only z < M kept)
+// written back to S in place.  kLctCols adjacent kx columns per workgroup
+// (64-byte rows per plane); the 2M x kLctCols block lives in LDS.  Forward:
+// radix-2 decimation in time from bit-reversed input; inverse: decimation
+// in frequency to bit-reversed output, read back through the bit reversal.
+constexpr int kLctCols = 8;
+constexpr int kLctThreads = 512;
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ uint32_t bitrev(uint32_t i, uint32_t lg) {
+    return __brev(i) >> (32u - lg);
+}
+
+__global__ __launch_bounds__(kLctThreads) void lct_zpass_kernel(float2* __restrict__ S,
+                                                                const float2* __restrict__ inv,
+                                                                const float2* __restrict__ tw,
+                                                                uint32_t N, uint32_t M,
+                                                                uint32_t lg) {
+    extern __shared__ float2 lds[];
+    float2* twl = lds;           // M twiddles
+    float2* buf = lds + M;       // [2M][kLctCols]
+    const uint32_t tid = threadIdx.x;
+    const uint32_t K = 2 * M, H = N + 1, Y2 = 2 * N;
+    const uint32_t kx0 = blockIdx.x * kLctCols, ky = blockIdx.y;
+    for (uint32_t k = tid; k < M; k += kLctThreads) twl[k] = tw[k];
+    for (uint32_t idx = tid; idx < K * kLctCols; idx += kLctThreads) {
+        const uint32_t i = idx / kLctCols, c = idx % kLctCols;
+        const uint32_t z = bitrev(i, lg), kx = kx0 + c;
+        float2 v = make_float2(0.f, 0.f);
+        if (z < M && kx < H) v = S[(size_t(z) * Y2 + ky) * H + kx];
+        buf[idx] = v;
+    }
+    // Forward, decimation in time: butterflies of span h = 2^s.
+    for (uint32_t s = 0; s < lg; ++s) {
+        __syncthreads();
+        for (uint32_t b = tid; b < M * kLctCols; b += kLctThreads) {
+            const uint32_t j = b / kLctCols, c = b % kLctCols;
+            const uint32_t pos = j & ((1u << s) - 1u);
+            const uint32_t i0 = ((j >> s) << (s + 1)) | pos, i1 = i0 + (1u << s);
+            const float2 w = twl[pos << (lg - 1u - s)];
+            const float2 u = buf[i0 * kLctCols + c];
+            const float2 v = cmul(buf[i1 * kLctCols + c], w);
+            buf[i0 * kLctCols + c] = make_float2(u.x + v.x, u.y + v.y);
+            buf[i1 * kLctCols + c] = make_float2(u.x - v.x, u.y - v.y);
+        }
+    }
+    __syncthreads();
+    for (uint32_t idx = tid; idx < K * kLctCols; idx += kLctThreads) {
+        const uint32_t k = idx / kLctCols, kx = kx0 + idx % kLctCols;
+        if (kx < H) buf[idx] = cmul(buf[idx], inv[(size_t(k) * Y2 + ky) * H + kx]);
+    }
+    // Inverse, decimation in frequency with conjugate twiddles.
+    for (uint32_t s = lg; s-- > 0;) {
+        __syncthreads();
+        for (uint32_t b = tid; b < M * kLctCols; b += kLctThreads) {
+            const uint32_t j = b / kLctCols, c = b % kLctCols;
+            const uint32_t pos = j & ((1u << s) - 1u);
+            const uint32_t i0 = ((j >> s) << (s + 1)) | pos, i1 = i0 + (1u << s);
+            const float2 t = twl[pos << (lg - 1u - s)];
+            const float2 w = make_float2(t.x, -t.y);
+            const float2 u = buf[i0 * kLctCols + c];
+            const float2 v = buf[i1 * kLctCols + c];
+            buf[i0 * kLctCols + c] = make_float2(u.x + v.x, u.y + v.y);
+            buf[i1 * kLctCols + c] = cmul(make_float2(u.x - v.x, u.y - v.y), w);
+        }
+    }
+    __syncthreads();
+    for (uint32_t idx = tid; idx < M * kLctCols; idx += kLctThreads) {
+        const uint32_t z = idx / kLctCols, c = idx % kLctCols, kx = kx0 + c;
+        if (kx < H) S[(size_t(z) * Y2 + ky) * H + kx] = buf[bitrev(z, lg) * kLctCols + c];
+    }
+}
+
 __global__ void lct_filter_kernel(float2* __restrict__ C, const float2* __restrict__ inv, size_t n) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -372,6 +477,7 @@ void fft_cache_destroy(fbp_context ctx) {
     if (c->work) hipFree(c->work);
     if (c->khat) hipFree(c->khat);
     if (c->invpsf) hipFree(c->invpsf);
+    if (c->tw) hipFree(c->tw);
     free_csr(c->mtx);
     free_csr(c->mtxi);
     delete c;
@@ -545,6 +651,58 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
         c->inv_key = keybuf;
     }
 
+    const dim3 tgrid((M + 31) / 32, (N + 31) / 32, N);
+    const float norm = float(1.0 / double(pad_n));
+    // Pruned path: the padded volume's upper M planes are zero and only its
+    // first M output planes are kept, so the 2D transforms run on M planes
+    // and one kernel does the z transforms and the filter in LDS.
+    const uint32_t lg = 1u + uint32_t(std::lround(std::log2(double(M))));
+    const size_t zpass_lds = (size_t(2 * M) * kLctCols + M) * sizeof(float2);
+    if (zpass_lds <= size_t(ctx->max_lds_per_block)) {
+        if (c->tw_m != M) {
+            std::vector<float2> tw(M);
+            for (uint32_t k = 0; k < M; ++k) {
+                const double a = -2.0 * M_PI * double(k) / double(2 * M);
+                tw[k] = make_float2(float(std::cos(a)), float(std::sin(a)));
+            }
+            if (c->tw) FBP_CHECK_HIP(ctx, hipFree(c->tw));
+            c->tw = nullptr;
+            c->tw_m = 0;
+            FBP_CHECK(upload(ctx, tw, &c->tw, s));
+            c->tw_m = M;
+        }
+        const size_t plane = size_t(2 * N) * (2 * N), hplane = size_t(2 * N) * (N + 1);
+        const size_t d_bytes = (NN * M * sizeof(float) + 255) & ~size_t(255);
+        const size_t r_bytes = (plane * M * sizeof(float) + 255) & ~size_t(255);
+        FBP_CHECK(ensure_scratch2(ctx, d_bytes + r_bytes + hplane * M * sizeof(float2)));
+        char* base = static_cast<char*>(ctx->scratch2);
+        float* d = reinterpret_cast<float*>(base);
+        float* planes = reinterpret_cast<float*>(base + d_bytes);
+        float2* spec = reinterpret_cast<float2*>(base + d_bytes + r_bytes);
+        hipLaunchKernelGGL(lct_permute_kernel, tgrid, dim3(32, 8), 0, s, data, d, N, M, p->z_trim,
+                           p->is_diffuse);
+        hipLaunchKernelGGL(lct_resample_planes_kernel, grid1(plane * M), dim3(256), 0, s, d, planes,
+                           c->mtx.rowptr, c->mtx.cols, c->mtx.vals, N, M);
+        FBP_CHECK_HIP(ctx, hipGetLastError());
+        FftPlan *pf = nullptr, *pi = nullptr;
+        FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_forward, rocfft_precision_single,
+                           {2 * N, 2 * N}, M, &pf));
+        FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_inverse, rocfft_precision_single,
+                           {2 * N, 2 * N}, M, &pi));
+        FBP_CHECK(run_plan(ctx, pf, planes, spec, s));
+        FBP_CHECK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(lct_zpass_kernel),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               int(zpass_lds)));
+        hipLaunchKernelGGL(lct_zpass_kernel, dim3((N + 1 + kLctCols - 1) / kLctCols, 2 * N),
+                           dim3(kLctThreads), zpass_lds, s, spec, c->invpsf, c->tw, N, M, lg);
+        FBP_CHECK_HIP(ctx, hipGetLastError());
+        FBP_CHECK(run_plan(ctx, pi, spec, planes, s));
+        hipLaunchKernelGGL(lct_final_kernel, grid1(NN * M), dim3(256), 0, s, planes, out,
+                           c->mtxi.rowptr, c->mtxi.cols, c->mtxi.vals, N, M, norm);
+        FBP_CHECK_HIP(ctx, hipGetLastError());
+        return FBP_OK;
+    }
+
     // Work buffers: d [M][N][N] | padded real [2M][2N][2N] | half spectrum.
     const size_t d_bytes = (NN * M * sizeof(float) + 255) & ~size_t(255);
     const size_t r_bytes = (pad_n * sizeof(float) + 255) & ~size_t(255);
@@ -554,7 +712,6 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
     float* padv = reinterpret_cast<float*>(base + d_bytes);
     float2* spec = reinterpret_cast<float2*>(base + d_bytes + r_bytes);
 
-    const dim3 tgrid((M + 31) / 32, (N + 31) / 32, N);
     hipLaunchKernelGGL(lct_permute_kernel, tgrid, dim3(32, 8), 0, s, data, d, N, M, p->z_trim,
                        p->is_diffuse);
     hipLaunchKernelGGL(fill_kernel, grid1(pad_n), dim3(256), 0, s, padv, pad_n);
@@ -569,7 +726,6 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
     FBP_CHECK(run_plan(ctx, pf, padv, spec, s));
     hipLaunchKernelGGL(lct_filter_kernel, grid1(half_n), dim3(256), 0, s, spec, c->invpsf, half_n);
     FBP_CHECK(run_plan(ctx, pi, spec, padv, s));
-    const float norm = float(1.0 / double(pad_n));
     hipLaunchKernelGGL(lct_final_kernel, grid1(NN * M), dim3(256), 0, s, padv, out, c->mtxi.rowptr,
                        c->mtxi.cols, c->mtxi.vals, N, M, norm);
     FBP_CHECK_HIP(ctx, hipGetLastError());

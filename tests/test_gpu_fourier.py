@@ -42,7 +42,9 @@ def test_log_fft(ctx, shape, sigma):
 @pytest.mark.parametrize("N,M,diffuse,backprop,z_trim", [(16, 64, False, False, 0),
                                                          (16, 64, True, False, 5),
                                                          (8, 32, False, True, 3),
-                                                         (32, 128, False, False, 10)])
+                                                         (32, 128, False, False, 10),
+                                                         # 2M > 2048: the full 3D transform path
+                                                         (4, 2048, False, False, 0)])
 def test_lct_random(ctx, N, M, diffuse, backprop, z_trim):
     rng = np.random.default_rng(N * M)
     data = rng.random((N, N, M)).astype(np.float32)

@@ -11,6 +11,7 @@ import pytest
 import torch
 
 import oracle
+from oracle import lct as olct
 from fasterbackprojection_amd.api import STORAGE_BF16, STORAGE_F16, make_info
 from fasterbackprojection_amd.scene import (exhaustive_geometry, full_confocal_geometry,
                                             random_quads, random_spheres)
@@ -148,3 +149,14 @@ def test_c3_full_exhaustive_matches_oracle_slice(ctx):
                                   g.sensor_targets)
     assert rel_l2(vol[:1].cpu().numpy(), ref) <= 1e-4
     assert bool(torch.isfinite(vol).all())
+
+
+def test_lct_full_c2_matches_restatement(ctx, c2):
+    """LCT of the full capture (128 x 128 x 1024 padded to 256 x 256 x 2048,
+    snr 8) against the NumPy/SciPy restatement of python/lct.py."""
+    g, info, laser, I = c2
+    data = I.reshape(128, 128, 1024).contiguous()
+    width, rmax = 1.0, 1024 * float(g.time_step)
+    out = ctx.lct(data, width, rmax, 8.0).cpu().numpy()
+    ref = olct.lct(data.cpu().numpy(), width, rmax, 8.0)
+    assert rel_l2(out, ref) <= 1e-4
