@@ -338,6 +338,182 @@ __global__ __launch_bounds__(kLctThreads) void lct_zpass_kernel(float2* __restri
     }
 }
 
+// 2M = 2048: the same z-pass with register DFTs.  Index digits (forward):
+//   n = a + 128 b, k = k0 + 16 k'   (radix 16 over b; twiddle W^{a k0})
+//   a = a' + 8 c,  k' = k1 + 16 k2  (radix 16 over c; twiddle W_128^{a' k1})
+//   radix 8 over a'                 -> X[k0 + 16 k1 + 256 k2]
+// and (inverse, digits of the held k first):
+//   k = kl + 256 k2, z = z0 + 8 z'  (radix 8 over k2; twiddle W^{-kl z0})
+//   kl = l0 + 16 l1, z' = z1 + 16 z2 (radix 16 over l1; twiddle W_256^{-l0 z1})
+//   radix 16 over l0                -> y[z0 + 8 z1 + 128 z2], z2 < 8 kept.
+// 64 threads per column; four LDS round trips (the first pass reads global
+// memory, the last writes it).  LDS index pos * 8 + col + 4 (pos / 8): the
+// padding spreads every pass's 64-lane pattern evenly over the banks.
+constexpr float kC16[16] = {1.0f, 0.92387953251128674f, 0.70710678118654752f,
+                            0.38268343236508977f, 0.0f, -0.38268343236508977f,
+                            -0.70710678118654752f, -0.92387953251128674f, -1.0f,
+                            -0.92387953251128674f, -0.70710678118654752f,
+                            -0.38268343236508977f, 0.0f, 0.38268343236508977f,
+                            0.70710678118654752f, 0.92387953251128674f};
+
+// W_16^m = cos(2 pi m / 16) - i sin(2 pi m / 16) (conjugated when INV).
+template <bool INV>
+__device__ __forceinline__ float2 w16(int m) {
+    const float c = kC16[m & 15], sn = kC16[(m + 12) & 15];  // sin = cos(x - pi/2)
+    return make_float2(c, INV ? sn : -sn);
+}
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+template <int LG>
+constexpr int brev_c(int i) {
+    int r = 0;
+    for (int b = 0; b < LG; ++b) r |= ((i >> b) & 1) << (LG - 1 - b);
+    return r;
+}
+
+// In-register DFT of N = 8 or 16 points, natural order in and out.
+template <int N, bool INV>
+__device__ __forceinline__ void dft_reg(float2 (&v)[N]) {
+    constexpr int LG = N == 8 ? 3 : 4;
+    float2 t[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) t[i] = v[brev_c<LG>(i)];
+#pragma unroll
+    for (int s = 0; s < LG; ++s) {
+        const int h = 1 << s;
+#pragma unroll
+        for (int j = 0; j < N / 2; ++j) {
+            const int pos = j & (h - 1);
+            const int i0 = ((j >> s) << (s + 1)) | pos, i1 = i0 + h;
+            const float2 b = pos == 0 ? t[i1] : cmul(t[i1], w16<INV>(pos * (16 / (2 * h))));
+            t[i1] = csub(t[i0], b);
+            t[i0] = cadd(t[i0], b);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = t[i];
+}
+
+// W_2048^m from the table of W_2048^k, k < 1024 (W^{k + 1024} = -W^k).
+template <bool INV>
+__device__ __forceinline__ float2 w2048(const float2* twl, uint32_t m) {
+    const float2 t = twl[m & 1023u];
+    const float sg = (m & 1024u) ? -1.f : 1.f;
+    return make_float2(sg * t.x, INV ? -sg * t.y : sg * t.y);
+}
+
+__global__ __launch_bounds__(kLctThreads) void lct_zpass2048_kernel(float2* __restrict__ S,
+                                                                    const float2* __restrict__ inv,
+                                                                    const float2* __restrict__ tw,
+                                                                    uint32_t N) {
+    extern __shared__ float2 lds[];
+    float2* twl = lds;          // 1024 twiddles
+    float2* buf = lds + 1024;   // [2048][kLctCols] + padding
+    const uint32_t tid = threadIdx.x;
+    const uint32_t col = tid % kLctCols, u = tid / kLctCols;  // u in [0, 64)
+    const uint32_t H = N + 1, Y2 = 2 * N;
+    const uint32_t kx0 = blockIdx.x * kLctCols, ky = blockIdx.y, kx = kx0 + col;
+    const bool live = kx < H;
+    auto at = [&](uint32_t pos) -> float2& { return buf[pos * kLctCols + col + 4u * (pos >> 3)]; };
+    for (uint32_t k = tid; k < 1024; k += kLctThreads) twl[k] = tw[k];
+    __syncthreads();
+
+    // Forward pass 1: a in {u, u + 64}; x[a + 128 b] = 0 for b >= 8.
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t a = u + 64u * h;
+        float2 v[16];
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+            v[b] = make_float2(0.f, 0.f);
+            if (b < 8 && live) v[b] = S[(size_t(a + 128u * b) * Y2 + ky) * H + kx];
+        }
+        dft_reg<16, false>(v);
+#pragma unroll
+        for (int k0 = 0; k0 < 16; ++k0)
+            at(k0 * 128u + a) = k0 == 0 ? v[0] : cmul(v[k0], w2048<false>(twl, a * k0));
+    }
+    __syncthreads();
+    // Forward pass 2: q = k0 * 8 + a' in {u, u + 64}.
+    float2 r2[2][16];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t q = u + 64u * h, k0 = q / 8u, ap = q % 8u;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) r2[h][c] = at(k0 * 128u + ap + 8u * c);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t q = u + 64u * h, k0 = q / 8u, ap = q % 8u;
+        dft_reg<16, false>(r2[h]);
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1)
+            at(k0 * 128u + k1 * 8u + ap) =
+                k1 == 0 ? r2[h][0] : cmul(r2[h][k1], w2048<false>(twl, 16u * ap * k1));
+    }
+    __syncthreads();
+    // Forward pass 3 (radix 8 over a'), filter, inverse pass 1 (radix 8
+    // over k2): r = k0 * 16 + k1 in {u, u + 64, u + 128, u + 192}.
+    float2 r3[4][8];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        const uint32_t r = u + 64u * h, k0 = r / 16u, k1 = r % 16u;
+#pragma unroll
+        for (int ap = 0; ap < 8; ++ap) r3[h][ap] = at(k0 * 128u + k1 * 8u + ap);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        const uint32_t r = u + 64u * h, k0 = r / 16u, k1 = r % 16u, kl = k0 + 16u * k1;
+        dft_reg<8, false>(r3[h]);
+#pragma unroll
+        for (int k2 = 0; k2 < 8; ++k2) {
+            const uint32_t k = kl + 256u * k2;
+            const float2 f = live ? inv[(size_t(k) * Y2 + ky) * H + kx] : make_float2(0.f, 0.f);
+            r3[h][k2] = cmul(r3[h][k2], f);
+        }
+        dft_reg<8, true>(r3[h]);
+#pragma unroll
+        for (int z0 = 0; z0 < 8; ++z0)
+            at(z0 * 256u + kl) = z0 == 0 ? r3[h][0] : cmul(r3[h][z0], w2048<true>(twl, kl * z0));
+    }
+    __syncthreads();
+    // Inverse pass 2: q = z0 * 16 + l0 in {u, u + 64}.
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t q = u + 64u * h, z0 = q / 16u, l0 = q % 16u;
+#pragma unroll
+        for (int l1 = 0; l1 < 16; ++l1) r2[h][l1] = at(z0 * 256u + l0 + 16u * l1);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t q = u + 64u * h, z0 = q / 16u, l0 = q % 16u;
+        dft_reg<16, true>(r2[h]);
+#pragma unroll
+        for (int z1 = 0; z1 < 16; ++z1)
+            at(z0 * 256u + z1 * 16u + l0) =
+                z1 == 0 ? r2[h][0] : cmul(r2[h][z1], w2048<true>(twl, 8u * l0 * z1));
+    }
+    __syncthreads();
+    // Inverse pass 3 (radix 16 over l0): q = z0 * 16 + z1 in {u, u + 64};
+    // outputs z = z0 + 8 z1 + 128 z2 for z2 < 8 (z < 1024).
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t q = u + 64u * h, z0 = q / 16u, z1 = q % 16u;
+#pragma unroll
+        for (int l0 = 0; l0 < 16; ++l0) r2[h][l0] = at(z0 * 256u + z1 * 16u + l0);
+        dft_reg<16, true>(r2[h]);
+        if (live)
+#pragma unroll
+            for (int z2 = 0; z2 < 8; ++z2)
+                S[(size_t(z0 + 8u * z1 + 128u * z2) * Y2 + ky) * H + kx] = r2[h][z2];
+    }
+}
+
 __global__ void lct_filter_kernel(float2* __restrict__ C, const float2* __restrict__ inv, size_t n) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -693,8 +869,18 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
         FBP_CHECK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(lct_zpass_kernel),
                                                hipFuncAttributeMaxDynamicSharedMemorySize,
                                                int(zpass_lds)));
-        hipLaunchKernelGGL(lct_zpass_kernel, dim3((N + 1 + kLctCols - 1) / kLctCols, 2 * N),
-                           dim3(kLctThreads), zpass_lds, s, spec, c->invpsf, c->tw, N, M, lg);
+        const dim3 zgrid((N + 1 + kLctCols - 1) / kLctCols, 2 * N);
+        const size_t lds2048 = (size_t(2048) * kLctCols + 4 * 256 + 1024) * sizeof(float2);
+        if (M == 1024 && lds2048 <= size_t(ctx->max_lds_per_block)) {
+            FBP_CHECK_HIP(ctx, hipFuncSetAttribute(
+                                   reinterpret_cast<const void*>(lct_zpass2048_kernel),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(lds2048)));
+            hipLaunchKernelGGL(lct_zpass2048_kernel, zgrid, dim3(kLctThreads), lds2048, s, spec,
+                               c->invpsf, c->tw, N);
+        } else {
+            hipLaunchKernelGGL(lct_zpass_kernel, zgrid, dim3(kLctThreads), zpass_lds, s, spec,
+                               c->invpsf, c->tw, N, M, lg);
+        }
         FBP_CHECK_HIP(ctx, hipGetLastError());
         FBP_CHECK(run_plan(ctx, pi, spec, planes, s));
         hipLaunchKernelGGL(lct_final_kernel, grid1(NN * M), dim3(256), 0, s, planes, out,
