@@ -418,6 +418,17 @@ __global__ __launch_bounds__(kLctThreads) void lct_zpass2048_kernel(float2* __re
     const bool live = kx < H;
     auto at = [&](uint32_t pos) -> float2& { return buf[pos * kLctCols + col + 4u * (pos >> 3)]; };
     for (uint32_t k = tid; k < 1024; k += kLctThreads) twl[k] = tw[k];
+    // The filter values this thread applies in pass 3, loaded now so that
+    // their latency overlaps the forward passes.
+    float2 fil[4][8];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        const uint32_t r = u + 64u * h, kl = r / 16u + 16u * (r % 16u);
+#pragma unroll
+        for (int k2 = 0; k2 < 8; ++k2)
+            fil[h][k2] = live ? inv[(size_t(kl + 256u * k2) * Y2 + ky) * H + kx]
+                              : make_float2(0.f, 0.f);
+    }
     __syncthreads();
 
     // Forward pass 1: a in {u, u + 64}; x[a + 128 b] = 0 for b >= 8.
@@ -470,11 +481,7 @@ __global__ __launch_bounds__(kLctThreads) void lct_zpass2048_kernel(float2* __re
         const uint32_t r = u + 64u * h, k0 = r / 16u, k1 = r % 16u, kl = k0 + 16u * k1;
         dft_reg<8, false>(r3[h]);
 #pragma unroll
-        for (int k2 = 0; k2 < 8; ++k2) {
-            const uint32_t k = kl + 256u * k2;
-            const float2 f = live ? inv[(size_t(k) * Y2 + ky) * H + kx] : make_float2(0.f, 0.f);
-            r3[h][k2] = cmul(r3[h][k2], f);
-        }
+        for (int k2 = 0; k2 < 8; ++k2) r3[h][k2] = cmul(r3[h][k2], fil[h][k2]);
         dft_reg<8, true>(r3[h]);
 #pragma unroll
         for (int z0 = 0; z0 < 8; ++z0)
