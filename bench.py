@@ -562,6 +562,18 @@ def exhaustive_leg(ctx, args, reps=3):
 
     bp = timed(lambda: ctx.backproject_exhaustive(info, I, lasers, sensors, out=vol))
     lap = timed(lambda: ctx.laplacian(vol, 5, out=filt))
+    # Depth-plane mode of the same capture (backprojectExhaustive): 200
+    # planes at focus +- 0.1 over the laser-grid pixels, every pair.
+    focus = abs((np.float32(g.hidden_min[1]) + np.float32(g.hidden_max[1])) * np.float32(0.5))
+    lo_d = float(np.float32(focus - np.float32(0.1)))
+    hi_d = float(np.float32(focus + np.float32(0.1)))
+    depths = torch.tensor(lo_d + (hi_d - lo_d) / 200 * np.arange(200), dtype=torch.float64,
+                          device=dev)
+    act = torch.empty((L, 200), dtype=torch.float32, device=dev)
+    t_depth = timed(lambda: ctx.backproject_depth(info, I, lasers, depths, sensor_targets=sensors,
+                                                  out=act))
+    depth_updates = L * 200 * L * S
+    depth_tflops = 25 * depth_updates / (t_depth * 1e-3) / 1e12
     cpu = None
     if not args.no_cpu_baseline:
         # The oracle on the first depth slice (64x64 voxels x all pairs).
@@ -592,6 +604,12 @@ def exhaustive_leg(ctx, args, reps=3):
                         "flops_per_update": 25},
            "issue_slot_diagnostic": {"valu_slots_per_update": 17, "bound_updates_per_s": slot_bound,
                                      "frac": updates / (bp * 1e-3) / slot_bound},
+           "depth_planes": {"workload": "200 depth planes at focus +- 0.1 over the 16x16 laser "
+                                        "pixels, all 1,048,576 pairs",
+                            "ms": t_depth, "pair_updates": depth_updates,
+                            "roofline": {"bound": "valu", "achieved": depth_tflops,
+                                         "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                                         "frac": depth_tflops / PEAK_FP32_TFLOPS}},
            "finite": bool(torch.isfinite(vol).all()),
            "cpu": cpu}
     log(f"exhaustive: {out}")

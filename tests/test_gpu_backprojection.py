@@ -358,6 +358,29 @@ def test_depth_planes(ctx):
     np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=1e-5, atol=1e-6)
 
 
+@pytest.mark.parametrize("L,S,D", [(16, 64, 5), (10, 73, 3), (4, 9, 40)])
+def test_depth_planes_exhaustive(ctx, L, S, D):
+    """Exhaustive depth planes (backprojectExhaustive): pair chunks split
+    across threads (ragged last chunk, chunks crossing sensor rows) and
+    reduced in FP64, against the oracle."""
+    g = geom(n_scan=8, T=256)
+    rng = np.random.default_rng(L * 100 + S)
+    lasers = wall_grid(8, 1.0)[rng.choice(64, L, replace=False)].astype(np.float32)
+    sensors = (wall_grid(16, 1.0)[rng.choice(256, S, replace=False)]).astype(np.float32)
+    g = Geometry(laser_targets=lasers, sensor_targets=sensors, num_time_bins=256,
+                 time_step=0.0096, hidden_min=g.hidden_min, hidden_max=g.hidden_max,
+                 voxel_resolution=g.voxel_resolution, confocal=False, discard_first_last_bounces=0,
+                 laser_position=(0.1, -0.8, 0.2), sensor_position=(-0.2, -0.7, 0.1),
+                 wall_size=(1.0, 1.0))
+    info = make_info(g)
+    I = spiky(rng, (L, S, 256))
+    depths = np.linspace(0.5, 0.9, D, endpoint=False)
+    ref = oracle.depth(info, I, lasers, depths, sensor=sensors, exhaustive=True)
+    out = ctx.backproject_depth(info, cuda(I), cuda(lasers), cuda(depths), sensor_targets=cuda(sensors))
+    torch.cuda.synchronize()
+    assert rel_l2(out.cpu().numpy(), ref) <= 1e-5
+
+
 DEPTH_ALIGNED = {
     "64": dict(nx=64, wall=1.0, T=256, lo=0.3, hi=0.5, D=20),
     "128": dict(nx=128, wall=2.0, T=512, lo=0.6, hi=0.9, D=4),
