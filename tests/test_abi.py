@@ -91,3 +91,20 @@ def test_alias_tables_reject_invalid():
     assert lib.fbp_build_alias_tables_host(short.ctypes.data_as(C.c_void_p), 2,
                                            a.ctypes.data_as(C.c_void_p),
                                            pr.ctypes.data_as(C.c_void_p)) == 6
+
+
+def test_entry_points_reject_null_context_without_gpu():
+    """Every device entry point checks its arguments before touching the
+    device: a null context (or null buffers) is FBP_ERR_INVALID_ARGUMENT (1)."""
+    lib = _lib.load()
+    n = None
+    assert lib.fbp_backproject_exhaustive_voxel(n, n, n, n, n, n, n) == 1
+    assert lib.fbp_backproject_depth_confocal(n, n, n, n, n, 1, n, n) == 1
+    assert lib.fbp_backproject_depth_exhaustive(n, n, n, n, n, n, 1, n, n) == 1
+    assert lib.fbp_backproject_confocal_mis(n, n, n, n, n, n, n) == 1
+    assert lib.fbp_depth_slices_rgba(n, n, 4, 4, 1, 4, 4, n, n) == 1
+    assert lib.fbp_filter_laplacian(n, n, n, _lib.UVec3(4, 4, 4), 5, n) == 1
+    assert lib.fbp_normalize_minmax(n, n, 16, n) == 1
+    assert lib.fbp_lct(n, n, n, n, n) == 1
+    assert lib.fbp_backproject_confocal_points(n, n, n, n, n, 1, n, n) == 1
+    assert lib.fbp_render_confocal(n, n, 1, n, n, 1, 16, 0.01, n, n) == 1
