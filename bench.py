@@ -648,10 +648,19 @@ def learned_leg(ctx, n_scenes=16):
         return float(np.median(ms))
 
     t_plain, t_learn, t_check, e_learn, e_base, fb = [], [], [], [], [], 0
+    t_general = []
     for s in range(2_000_000, 2_000_000 + n_scenes):
         I = p.render(s)
         y = p.plain(I).clone()
         t_plain.append(timed(lambda: p.plain(I)))
+        if s < 2_000_000 + 4:
+            # The plain path on the general kernel: what a capture whose scan
+            # grid is not aligned with the voxel lattice gets.
+            ctx.set_aligned_path(False)
+            try:
+                t_general.append(timed(lambda: p.plain(I)))
+            finally:
+                ctx.set_aligned_path(True)
         t_learn.append(timed(lambda: lrn(I, s, check=False)))
         t0 = time.perf_counter()
         out, fell_back, _ = lrn(I, s, check=True)
@@ -665,6 +674,8 @@ def learned_leg(ctx, n_scenes=16):
            "held_out_scenes": n_scenes,
            "plain_ms": float(np.mean(t_plain)), "learned_ms": float(np.mean(t_learn)),
            "speedup": float(np.mean(t_plain) / np.mean(t_learn)),
+           "plain_general_kernel_ms": float(np.mean(t_general)),
+           "speedup_vs_plain_general_kernel": float(np.mean(t_general) / np.mean(t_learn)),
            "learned_with_residual_check_ms": float(np.mean(t_check)),
            "rel_l2_learned_vs_plain": float(np.mean(e_learn)),
            "rel_l2_unrefined_mis_vs_plain": float(np.mean(e_base)),

@@ -310,30 +310,35 @@ __global__ void reduce_partials_kernel(const float* __restrict__ partial,
                                        VolumeGeom g, uint32_t num_chunks, double scale,
                                        const int* __restrict__ skip) {
     if (skip && *skip) return;
-    const size_t n = size_t(g.res[0]) * g.res[1] * g.res[2];
-    const size_t v = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (v >= n) return;
-    const uint32_t vx = uint32_t(v % g.res[0]);
-    const uint32_t vy = uint32_t((v / g.res[0]) % g.res[1]);
-    const uint32_t z = uint32_t(v / (size_t(g.res[0]) * g.res[1]));
-    const uint32_t tile =
-        ((z / g.tile[2]) * g.tiles[1] + vy / g.tile[1]) * g.tiles[0] + vx / g.tile[0];
-    const uint32_t ox = vx % g.tile[0], oy = vy % g.tile[1], oz = z % g.tile[2];
-    const uint32_t lane = (ox & 3) + 4 * (oz & 3) + 16 * (oy & 3);
-    const uint32_t slot = ((ox >> 2) * g.lt[2] + (oz >> 2)) * g.lt[1] + (oy >> 2);
+    // One thread per (tile, slot, lane) entry of a chunk's partial block, so
+    // the chunk reads are contiguous; the voxel it holds is written once.
+    const size_t per = size_t(g.num_tiles) * g.slots * 64;
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= per) return;
+    const uint32_t tile = uint32_t(i / (size_t(g.slots) * 64));
+    const uint32_t r = uint32_t(i % (size_t(g.slots) * 64));
+    const uint32_t slot = r / 64, lane = r % 64;
+    const uint32_t tx = tile % g.tiles[0], ty = (tile / g.tiles[0]) % g.tiles[1];
+    const uint32_t tz = tile / (g.tiles[0] * g.tiles[1]);
+    // Inverse of lane = (ox & 3) + 4 (oz & 3) + 16 (oy & 3) and
+    // slot = ((ox >> 2) lt_z + (oz >> 2)) lt_y + (oy >> 2).
+    const uint32_t ox = (lane & 3u) + 4u * (slot / (g.lt[2] * g.lt[1]));
+    const uint32_t oz = ((lane >> 2) & 3u) + 4u * ((slot / g.lt[1]) % g.lt[2]);
+    const uint32_t oy = (lane >> 4) + 4u * (slot % g.lt[1]);
+    const uint32_t vx = tx * g.tile[0] + ox, vy = ty * g.tile[1] + oy, z = tz * g.tile[2] + oz;
+    if (vx >= g.res[0] || vy >= g.res[1] || z >= g.res[2]) return;
     double sum = 0.0;
     for (uint32_t c = 0; c < num_chunks; ++c) {
-        const size_t i = ((size_t(c) * g.num_tiles + tile) * g.slots + slot) * 64 + lane;
-        sum += double(partial[i]);
-        if (corr) sum += double(corr[i]);
+        sum += double(partial[size_t(c) * per + i]);
+        if (corr) sum += double(corr[size_t(c) * per + i]);
     }
-    volume[v] = float(sum * scale);
+    volume[(size_t(z) * g.res[1] + vy) * g.res[0] + vx] = float(sum * scale);
 }
 
 fbp_status launch_reduce(fbp_context ctx, const float* partial, const float* corr, float* volume,
                          const VolumeGeom& g, uint32_t num_chunks, double scale,
                          hipStream_t stream, const int* skip) {
-    const size_t n = size_t(g.res[0]) * g.res[1] * g.res[2];
+    const size_t n = size_t(g.num_tiles) * g.slots * 64;
     const unsigned blocks = unsigned((n + 255) / 256);
     hipLaunchKernelGGL(reduce_partials_kernel, dim3(blocks), dim3(256), 0, stream, partial, corr,
                        volume, g, num_chunks, scale, skip);
