@@ -17,11 +17,12 @@
 // fast estimate of the bin quotient is q = rl/dt + sqrt(ss/dt^2) with the
 // squared sensor distance from per-pair axis tables scaled by 1/dt^2, and
 // the weight is 1/q^2 (dt^2 / dist^2; the reduction multiplies by 1/dt^2 in
-// FP64).  Lanes whose q lies within the estimate's error bound of a bin
-// boundary K recompute the distance with the reference's IEEE chain and
-// compare it with the boundary table tk[K] = min{d : (int)(d / dt) >= K}
-// (no division).  Pairs whose tile may come within a few millimetres of
-// the wall or of zero distance take the whole exact chain.
+// FP64).  Voxels whose q lies within the estimate's error bound of a bin
+// boundary add nothing in the fast loop (a select, no branch); after the
+// pair's 32 updates, the lanes that flagged any take the reference's IEEE
+// chain (bin and weight) for those voxels into per-thread LDS correction
+// slots, behind one wave-uniform test.  Pairs whose tile may come within a
+// few millimetres of the wall or of zero distance take the whole exact chain.
 #include "bp_common.h"
 
 namespace fbp {
@@ -31,7 +32,6 @@ struct ExhaustiveArgs {
     const float* scales;
     const fbp_vec3* laser;
     const fbp_vec3* sensor;
-    const float* thresholds;  // tk[0..T]: bin boundaries in distance (or null)
     float* partial;
     VolumeGeom g;
     uint32_t L, S, T;
@@ -64,15 +64,20 @@ __device__ __forceinline__ float exact_dist(float x, float y, float z, LaserPos 
 // One pair applied to the lane's 32 voxels.  rq: laser distances / dt;
 // AQ, ZQ: the pair's per-axis squared sensor distances (x+y, z) / dt^2.
 // NEAR: every voxel takes the whole exact chain (bin and weight).
+__device__ __forceinline__ float pick4f(const float (&v)[4], int i) {
+    return i == 0 ? v[0] : i == 1 ? v[1] : i == 2 ? v[2] : v[3];
+}
+
 template <bool NEAR, bool COFF>
 __device__ __forceinline__ void exhaustive_update_tile(float (&acc)[32], const float (&rq)[32],
                                                        const float (&px)[4], const float (&py)[2],
                                                        const float (&pz)[4], LaserPos lp,
                                                        const float (&AQ)[4][2], const float (&ZQ)[4],
-                                                       const float* win, const float* tk,
+                                                       const float* win,
                                                        int base, float hme, float coffq,
                                                        float off, float sx, float sy, float sz,
-                                                       const ExhaustiveArgs& a) {
+                                                       const ExhaustiveArgs& a, float* corr) {
+    uint32_t band_mask = 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
 #pragma unroll
@@ -89,49 +94,40 @@ __device__ __forceinline__ void exhaustive_update_tile(float (&acc)[32], const f
                     const float de = exact_dist(px[i], py[j], pz[m], lp, sx, sy, sz, a, off);
                     k = bin_of(de, a.dt);
                     w = safe_rcp(rn_mul(de, de)) * a.dt2;
+                    acc[slot] = fmaf(win[base + k], w, acc[slot]);
                 } else {
+                    // q within the error bound of a bin boundary: left out
+                    // here, added exactly below.
                     const float f = __builtin_amdgcn_fractf(q);
-                    if (__builtin_expect(fabsf(f - 0.5f) > hme, 0)) {
-                        // q is within the error bound of the boundary K: the
-                        // exact bin is K - 1 or K.
-                        const float de = exact_dist(px[i], py[j], pz[m], lp, sx, sy, sz, a, off);
-                        if (tk) {
-                            const int K = min(static_cast<int>(q + 0.5f), int(a.T));
-                            k = de >= tk[K] ? K : K - 1;
-                        } else {
-                            k = bin_of(de, a.dt);
-                        }
-                    }
+                    const bool band = fabsf(f - 0.5f) > hme;
+                    band_mask |= uint32_t(band) << slot;
+                    const float v = win[base + k];
+                    acc[slot] = fmaf(band ? 0.f : v, w, acc[slot]);
                 }
-                acc[slot] = fmaf(win[base + k], w, acc[slot]);
+            }
+        }
+    }
+    if constexpr (!NEAR) {
+        if (__any(band_mask != 0)) {
+            // The voxels next to a boundary take the exact chain (bin and
+            // weight) into the lane's correction slots.
+            while (band_mask) {
+                const int slot = __builtin_ctz(band_mask);
+                band_mask &= band_mask - 1u;
+                const int i = slot >> 3, j = (slot >> 2) & 1, m = slot & 3;
+                const float de = exact_dist(pick4f(px, i), j ? py[1] : py[0], pick4f(pz, m), lp,
+                                            sx, sy, sz, a, off);
+                const int k = bin_of(de, a.dt);
+                if (k >= 0 && k < int(a.T))
+                    corr[slot * 256] += win[base + k] * (safe_rcp(rn_mul(de, de)) * a.dt2);
             }
         }
     }
 }
 
-// Bin boundaries in distance: tk[K] = the smallest d >= 0 with
-// (int)(d / dt) >= K (IEEE division), K = 0..T, by bisection over the
-// ordered bit patterns of non-negative floats.
-__global__ void exhaustive_thresholds_kernel(float dt, uint32_t T, float* __restrict__ tk) {
-    const uint32_t K = blockIdx.x * blockDim.x + threadIdx.x;
-    if (K > T) return;
-    uint32_t lo = 0, hi = __float_as_uint(rn_mul(float(T + 2), dt) * 2.f);
-    // invariant: bin(hi) >= K; find the smallest such bit pattern
-    while (lo < hi) {
-        const uint32_t mid = lo + (hi - lo) / 2;
-        if (bin_of(__uint_as_float(mid), dt) >= int(K))
-            hi = mid;
-        else
-            lo = mid + 1;
-    }
-    tk[K] = __uint_as_float(lo);
-}
-
 // Window prefetch registers per thread: batch * W <= kPrefetch * 256.
 constexpr int kPrefetch = 16;
 constexpr int kWaves = 4;
-// Bin boundary tables up to this many bins live in LDS.
-constexpr uint32_t kMaxTableBins = 8192;
 
 // Pair parameters of one batch slot (computed by one thread, read by all).
 struct PairParams {
@@ -186,11 +182,12 @@ __device__ __forceinline__ PairParams pair_params(const ExhaustiveArgs& a, uint6
 template <int STORAGE, bool COFF>
 __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs a) {
     extern __shared__ float smem[];
-    // [2][batch * W] windows, [2][batch] PairParams, then the bin boundaries
-    // tk[0..T] when a.thresholds is set
+    // [2][batch * W] windows, [2][batch] PairParams, then the correction
+    // slots of the boundary voxels [32][256 threads] (thread-fastest: no
+    // bank conflicts).
     float* winbuf = smem;
     PairParams* prmbuf = reinterpret_cast<PairParams*>(smem + 2 * a.batch * a.W);
-    float* tk = a.thresholds ? reinterpret_cast<float*>(prmbuf + 2 * a.batch) : nullptr;
+    float* corrl = reinterpret_cast<float*>(prmbuf + 2 * a.batch) + threadIdx.x;
 
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -217,7 +214,10 @@ __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs
 
     float acc[32];
 #pragma unroll
-    for (int i = 0; i < 32; ++i) acc[i] = 0.f;
+    for (int i = 0; i < 32; ++i) {
+        acc[i] = 0.f;
+        corrl[i * 256] = 0.f;
+    }
     float rq[32];
     LaserPos lp{0.f, 0.f, 0.f};
     uint32_t cur_l = 0xffffffffu;
@@ -255,8 +255,6 @@ __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs
     };
 
     float reg[kPrefetch];
-    if (tk)
-        for (uint32_t i = uint32_t(tid); i <= a.T; i += 256) tk[i] = a.thresholds[i];
     // Prologue: batch 0 into buffer 0.
     if (p_begin < p_end) {
         const int nb = int(min(uint64_t(a.batch), p_end - p_begin));
@@ -326,11 +324,11 @@ __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs
 #pragma unroll
                 for (int j = 0; j < 2; ++j) AQ[i][j] = (XS2[i] + YS2[j]) * a.inv_dt2;
             if (near)
-                exhaustive_update_tile<true, COFF>(acc, rq, px, py, pz, lp, AQ, ZQ, winbuf, tk, base,
-                                                   hme, coffq, off, sxp, syp, szp, a);
+                exhaustive_update_tile<true, COFF>(acc, rq, px, py, pz, lp, AQ, ZQ, winbuf, base,
+                                                   hme, coffq, off, sxp, syp, szp, a, corrl);
             else
-                exhaustive_update_tile<false, COFF>(acc, rq, px, py, pz, lp, AQ, ZQ, winbuf, tk, base,
-                                                    hme, coffq, off, sxp, syp, szp, a);
+                exhaustive_update_tile<false, COFF>(acc, rq, px, py, pz, lp, AQ, ZQ, winbuf, base,
+                                                    hme, coffq, off, sxp, syp, szp, a, corrl);
         }
         if (has_next) store(winbuf + (cur ^ 1) * per, nbn, reg);
         __syncthreads();
@@ -346,7 +344,8 @@ __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int m = 0; m < 4; ++m)
-                out[size_t((i * 8 + 2 * wave + j) * 4 + m) * 64 + lane] = acc[(i * 2 + j) * 4 + m];
+                out[size_t((i * 8 + 2 * wave + j) * 4 + m) * 64 + lane] =
+                    acc[(i * 2 + j) * 4 + m] + corrl[((i * 2 + j) * 4 + m) * 256];
 }
 
 }  // namespace fbp
@@ -415,36 +414,30 @@ extern "C" fbp_status fbp_backproject_exhaustive_voxel(fbp_context ctx,
     a.chunk_len = (a.num_pairs + nc - 1) / nc;
     const uint64_t chunks = (a.num_pairs + a.chunk_len - 1) / a.chunk_len;
     const size_t partial_bytes = size_t(chunks) * a.g.num_tiles * a.g.slots * 64 * sizeof(float);
-    // Bin boundary table (in LDS) for T < kMaxTableBins; larger T divides.
-    const bool table = a.T < kMaxTableBins;
-    const size_t table_off = (partial_bytes + 255) & ~size_t(255);
-    FBP_CHECK(ensure_scratch(ctx, table_off + (table ? (size_t(a.T) + 1) * sizeof(float) : 0)));
+    FBP_CHECK(ensure_scratch(ctx, partial_bytes));
     a.partial = static_cast<float*>(ctx->scratch);
     hipStream_t s = as_stream(stream);
-    if (table) {
-        float* tk = reinterpret_cast<float*>(static_cast<char*>(ctx->scratch) + table_off);
-        hipLaunchKernelGGL(exhaustive_thresholds_kernel, dim3((a.T + 1 + 255) / 256), dim3(256), 0,
-                           s, a.dt, a.T, tk);
-        FBP_CHECK_HIP(ctx, hipGetLastError());
-        a.thresholds = tk;
-    }
 
     const dim3 grid(unsigned(chunks * a.g.num_tiles));
     const size_t shmem = 2 * (size_t(a.batch) * a.W * sizeof(float) + a.batch * sizeof(PairParams)) +
-                         (table ? (size_t(a.T) + 1) * sizeof(float) : 0);
+                         size_t(32) * 64 * kWaves * sizeof(float);
+    // At most 2 * 16 KiB of windows + 32 KiB of correction slots (+ pair
+    // parameters): beyond the default 64 KiB for the longest windows.
+    if (shmem > size_t(ctx->max_lds_per_block))
+        return fail(ctx, FBP_ERR_UNSUPPORTED, "exhaustive: %zu B of LDS per workgroup needed", shmem);
     const bool coff = !(a.t0 == 0.f && a.discard != 0);
-    auto launch = [&](auto kern) {
-        hipLaunchKernelGGL(kern, grid, dim3(64 * kWaves), shmem, s, a);
-    };
-    if (tr->storage == FBP_STORAGE_F32)
-        coff ? launch(exhaustive_voxel_kernel<FBP_STORAGE_F32, true>)
-             : launch(exhaustive_voxel_kernel<FBP_STORAGE_F32, false>);
-    else if (tr->storage == FBP_STORAGE_F16)
-        coff ? launch(exhaustive_voxel_kernel<FBP_STORAGE_F16, true>)
-             : launch(exhaustive_voxel_kernel<FBP_STORAGE_F16, false>);
-    else
-        coff ? launch(exhaustive_voxel_kernel<FBP_STORAGE_BF16, true>)
-             : launch(exhaustive_voxel_kernel<FBP_STORAGE_BF16, false>);
+    void (*kern)(ExhaustiveArgs) =
+        tr->storage == FBP_STORAGE_F32
+            ? (coff ? exhaustive_voxel_kernel<FBP_STORAGE_F32, true>
+                    : exhaustive_voxel_kernel<FBP_STORAGE_F32, false>)
+        : tr->storage == FBP_STORAGE_F16
+            ? (coff ? exhaustive_voxel_kernel<FBP_STORAGE_F16, true>
+                    : exhaustive_voxel_kernel<FBP_STORAGE_F16, false>)
+            : (coff ? exhaustive_voxel_kernel<FBP_STORAGE_BF16, true>
+                    : exhaustive_voxel_kernel<FBP_STORAGE_BF16, false>);
+    FBP_CHECK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
+    hipLaunchKernelGGL(kern, grid, dim3(64 * kWaves), shmem, s, a);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     // The fast weights carry dt^2 (1 / q^2): scale by 1 / dt^2 in FP64.
     return launch_reduce(ctx, a.partial, nullptr, volume, a.g, uint32_t(chunks),
