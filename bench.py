@@ -573,9 +573,23 @@ def exhaustive_leg(ctx, args, reps=3):
     t_depth = timed(lambda: ctx.backproject_depth(info, I, lasers, depths, sensor_targets=sensors,
                                                   out=act))
     depth_updates = L * 200 * L * S
-    depth_tflops = 25 * depth_updates / (t_depth * 1e-3) / 1e12
+    # backprojectExhaustive per (pixel, depth, pair): |p - l| (3 sub, 3 mul,
+    # 2 add, sqrt) + |p - s| (cached per sensor) - t0, / dt, + sample = 13.
+    depth_tflops = 13 * depth_updates / (t_depth * 1e-3) / 1e12
+    # Fast path per update (counted in the kernel's ISA): 16 VALU + v_rsq
+    # (4 slots) = 19 issue slots; boundary points add the IEEE chain.
+    depth_slot_bound = VALU_LANE_SLOTS / 19.0
+    depth_check = None
     cpu = None
     if not args.no_cpu_baseline:
+        import oracle
+        cols = [3, 151]
+        ref_d = oracle.depth(info, I.cpu().numpy(), g.laser_targets,
+                             depths.cpu().numpy()[cols], sensor=g.sensor_targets, exhaustive=True)
+        got_d = act[:, cols].cpu().numpy().astype(np.float64)
+        err_d = np.abs(got_d - ref_d) / np.maximum(np.abs(ref_d), 1e-3 * np.abs(ref_d).max())
+        depth_check = {"sample": "oracle on planes 3 and 151 (all pixels, all pairs)",
+                       "max_rel_err": float(err_d.max())}
         # The oracle on the first depth slice (64x64 voxels x all pairs).
         import oracle
         sub = type(info).from_buffer_copy(info)
@@ -610,7 +624,13 @@ def exhaustive_leg(ctx, args, reps=3):
                             "ms": t_depth, "pair_updates": depth_updates,
                             "roofline": {"bound": "valu", "achieved": depth_tflops,
                                          "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                                         "frac": depth_tflops / PEAK_FP32_TFLOPS}},
+                                         "frac": depth_tflops / PEAK_FP32_TFLOPS,
+                                         "flops_per_update": 13},
+                            "issue_slot_diagnostic": {
+                                "valu_slots_per_update": 19,
+                                "bound_updates_per_s": depth_slot_bound,
+                                "frac": depth_updates / (t_depth * 1e-3) / depth_slot_bound},
+                            "oracle_check": depth_check},
            "finite": bool(torch.isfinite(vol).all()),
            "cpu": cpu}
     log(f"exhaustive: {out}")

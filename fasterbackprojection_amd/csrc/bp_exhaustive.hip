@@ -226,24 +226,25 @@ __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs
     const uint64_t p_end = min(a.num_pairs, p_begin + a.chunk_len);
     const int per = a.batch * a.W;
 
+    // Every register's load is issued (unused ones read element 0), so the
+    // loads of a batch are in flight together and waited for once, at the
+    // store.
     auto fetch = [&](uint64_t pb, int nb, const PairParams* prm, float (&reg)[kPrefetch]) {
 #pragma unroll
         for (int i = 0; i < kPrefetch; ++i) {
             const int f = tid + i * 256;
-            float v = 0.f;
-            if (f < nb * a.W) {
-                // f / W through the reciprocal: exact for f < 2^22.
-                const int jl = int((float(f) + 0.5f) * a.inv_W), w = f - jl * a.W;
-                const uint64_t pair = pb + jl;
-                const int k = prm[jl].kmin + w;
-                if (k >= 0 && k < int(a.T)) {
-                    float sc = 1.f;
-                    if constexpr (STORAGE != FBP_STORAGE_F32) sc = a.scales[pair];
-                    v = load_sample<STORAGE>(a.data, pair * a.T + uint32_t(k), sc);
-                    if constexpr (STORAGE == FBP_STORAGE_F32) v = (v > kEps) ? v : 0.f;
-                }
-            }
-            reg[i] = v;
+            const bool in = f < nb * a.W;
+            // f / W through the reciprocal: exact for f < 2^22.
+            const int jl = in ? int((float(f) + 0.5f) * a.inv_W) : 0;
+            const int w = f - jl * a.W;
+            const uint64_t pair = pb + jl;
+            const int k = prm[jl].kmin + w;
+            const bool ok = in && k >= 0 && k < int(a.T);
+            float sc = 1.f;
+            if constexpr (STORAGE != FBP_STORAGE_F32) sc = a.scales[ok ? pair : 0];
+            float v = load_sample<STORAGE>(a.data, ok ? pair * a.T + uint32_t(k) : 0, sc);
+            if constexpr (STORAGE == FBP_STORAGE_F32) v = (v > kEps) ? v : 0.f;
+            reg[i] = ok ? v : 0.f;
         }
     };
     auto store = [&](float* win, int nb, const float (&reg)[kPrefetch]) {

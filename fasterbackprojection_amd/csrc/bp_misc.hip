@@ -10,10 +10,10 @@
 namespace fbp {
 
 // ------------------------------------------------------------ depth planes
-// Reference: backprojectConfocal / backprojectExhaustive
-// (FB/transient_reconstruction.cuh:41-91).  One thread per output (c, d)
-// gathers over all scan points / pairs in a fixed order (no atomics).  No
-// 1/r^2 weight, as in the reference.  Confocal captures on an aligned 64^2
+// Reference: backprojectConfocal (FB/transient_reconstruction.cuh:41-64).
+// One thread per output (c, d) gathers over all scan points in a fixed
+// order (no atomics).  No 1/r^2 weight, as in the reference.  The exhaustive
+// form is in bp_depth_exhaustive.hip.  Confocal captures on an aligned 64^2
 // or 128^2 scan grid with the wall normal along y take the ring-convolution
 // path instead (bp_confocal_aligned.hip, depth-plane mode): there the bin of
 // a (pixel, scan point) pair depends only on their grid offset and the
@@ -47,66 +47,6 @@ __global__ void depth_kernel(const float* __restrict__ I, const fbp_vec3* __rest
         if (v > kEps) sum += v;
     }
     act[idx] = sum;
-}
-
-// Exhaustive depth planes: one thread per (pair chunk, pixel c, depth d) sums
-// the pairs ls = s * L + l of its chunk (the reference's pair numbering,
-// l = ls % L, s = ls / L) into partial[chunk][c * D + d]; the chunks are
-// added in FP64 in a fixed order by depth_reduce_kernel.
-__global__ void depth_exhaustive_kernel(const float* __restrict__ I,
-                                        const fbp_vec3* __restrict__ laser,
-                                        const fbp_vec3* __restrict__ sensor,
-                                        const double* __restrict__ depths, uint32_t D, uint32_t L,
-                                        uint32_t S, uint32_t T, float t0, float dt,
-                                        uint32_t discard, fbp_vec3 n, fbp_vec3 lpos,
-                                        fbp_vec3 spos, uint64_t chunk_len,
-                                        float* __restrict__ partial) {
-    const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const size_t per = size_t(L) * D;
-    const uint64_t pairs = uint64_t(L) * S;
-    const size_t chunk = idx / per, o = idx % per;
-    const uint64_t p0 = uint64_t(chunk) * chunk_len;
-    if (p0 >= pairs) return;
-    const uint64_t p1 = min(pairs, p0 + chunk_len);
-    const uint32_t c = uint32_t(o / D), d = uint32_t(o % D);
-    const float depth = static_cast<float>(depths[d]);
-    const float cx = rn_add(laser[c].x, rn_mul(n.x, depth));
-    const float cy = rn_add(laser[c].y, rn_mul(n.y, depth));
-    const float cz = rn_add(laser[c].z, rn_mul(n.z, depth));
-    float sum = 0.f;
-    uint32_t s = uint32_t(p0 / L), l = uint32_t(p0 % L);
-    float sx = sensor[s].x, sy = sensor[s].y, sz = sensor[s].z;
-    float ds = distance3(cx, cy, cz, sx, sy, sz);
-    for (uint64_t ls = p0; ls < p1; ++ls) {
-        const float lx = laser[l].x, ly = laser[l].y, lz = laser[l].z;
-        float dist = rn_sub(rn_add(distance3(lx, ly, lz, cx, cy, cz), ds), t0);
-        if (discard == 0)
-            dist = rn_add(dist, rn_add(distance3(lx, ly, lz, lpos.x, lpos.y, lpos.z),
-                                       distance3(sx, sy, sz, spos.x, spos.y, spos.z)));
-        const float q = rn_div(dist, dt);
-        if (q > -1.0f && q < float(T)) {
-            const float v = I[(size_t(l) * S + s) * T + int(q)];
-            if (v > kEps) sum += v;
-        }
-        if (++l == L && ls + 1 < p1) {
-            l = 0;
-            ++s;
-            sx = sensor[s].x;
-            sy = sensor[s].y;
-            sz = sensor[s].z;
-            ds = distance3(cx, cy, cz, sx, sy, sz);
-        }
-    }
-    partial[idx] = sum;
-}
-
-__global__ void depth_reduce_kernel(const float* __restrict__ partial, uint32_t chunks, size_t n,
-                                    float* __restrict__ act) {
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double s = 0.0;
-    for (uint32_t c = 0; c < chunks; ++c) s += double(partial[size_t(c) * n + i]);
-    act[i] = float(s);
 }
 
 // --------------------------------------------------------------------- MIS
@@ -475,45 +415,6 @@ fbp_status fbp_backproject_depth_confocal(fbp_context ctx, const fbp_reconstruct
                        info->num_time_bins, info->time_offset, info->time_step,
                        info->discard_first_last_bounces, info->relay_wall_normal,
                        info->laser_position, info->sensor_position, activation, skip);
-    FBP_CHECK_HIP(ctx, hipGetLastError());
-    return FBP_OK;
-}
-
-fbp_status fbp_backproject_depth_exhaustive(fbp_context ctx, const fbp_reconstruction_info* info,
-                                            const fbp_transient* tr, const fbp_vec3* laser_targets,
-                                            const fbp_vec3* sensor_targets, const double* depths,
-                                            uint32_t num_depths, float* activation, void* stream) {
-    if (!ctx || !info || !tr || !tr->data || !laser_targets || !sensor_targets || !depths ||
-        !activation)
-        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "depth exhaustive: null argument");
-    if (tr->storage != FBP_STORAGE_F32)
-        return fail(ctx, FBP_ERR_UNSUPPORTED, "depth exhaustive: F32 transient only");
-    if (!num_depths || !info->num_laser_targets || !info->num_sensor_targets ||
-        !info->num_time_bins || !(info->time_step > 0.f))
-        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "depth exhaustive: empty grid");
-    FBP_CHECK(bind(ctx));
-    hipStream_t s = as_stream(stream);
-    const uint32_t L = info->num_laser_targets, S = info->num_sensor_targets;
-    const size_t n = size_t(L) * num_depths;
-    const uint64_t pairs = uint64_t(L) * S;
-    // Enough (chunk, pixel, depth) threads to fill the device: about
-    // 32 waves per CU, chunks of at least 256 pairs.
-    const uint64_t want = 32ull * 64 * uint64_t(ctx->num_cus > 0 ? ctx->num_cus : 256);
-    uint64_t chunks = std::max<uint64_t>(1, std::min<uint64_t>((want + n - 1) / n,
-                                                               (pairs + 255) / 256));
-    const uint64_t chunk_len = (pairs + chunks - 1) / chunks;
-    chunks = (pairs + chunk_len - 1) / chunk_len;
-    FBP_CHECK(ensure_scratch(ctx, size_t(chunks) * n * sizeof(float)));
-    float* partial = static_cast<float*>(ctx->scratch);
-    const size_t threads = size_t(chunks) * n;
-    hipLaunchKernelGGL(depth_exhaustive_kernel, dim3(unsigned((threads + 255) / 256)), dim3(256),
-                       0, s, static_cast<const float*>(tr->data), laser_targets, sensor_targets,
-                       depths, num_depths, L, S, info->num_time_bins, info->time_offset,
-                       info->time_step, info->discard_first_last_bounces, info->relay_wall_normal,
-                       info->laser_position, info->sensor_position, chunk_len, partial);
-    FBP_CHECK_HIP(ctx, hipGetLastError());
-    hipLaunchKernelGGL(depth_reduce_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s,
-                       partial, uint32_t(chunks), n, activation);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     return FBP_OK;
 }

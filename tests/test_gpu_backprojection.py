@@ -381,6 +381,48 @@ def test_depth_planes_exhaustive(ctx, L, S, D):
     assert rel_l2(out.cpu().numpy(), ref) <= 1e-5
 
 
+# Exhaustive depth-plane cases by kernel path: the staged-window fast path
+# with and without the time/path offsets, pairs whose points coincide with a
+# laser target (depth 0: the IEEE chain for every point), many workgroups
+# (64 pixels x 50 depths), and windows too wide for the staging buffers
+# (0.5 mm bins: the per-pair gather).
+DEPTH_EXHAUSTIVE = {
+    "pure": dict(L=16, S=64, D=5, discard=1, t0=0.0, dt=0.0096, T=256, lo=0.5, hi=0.9),
+    "t0": dict(L=16, S=64, D=5, discard=1, t0=0.05, dt=0.0096, T=256, lo=0.5, hi=0.9),
+    "offsets_t0": dict(L=12, S=50, D=7, discard=0, t0=0.3, dt=0.0096, T=512, lo=0.4, hi=0.8),
+    "near_wall": dict(L=16, S=64, D=6, discard=1, t0=0.0, dt=0.0096, T=256, lo=0.0, hi=0.3),
+    "many_blocks": dict(L=64, S=64, D=50, discard=1, t0=0.0, dt=0.0096, T=256, lo=0.5, hi=0.9),
+    "wide_window": dict(L=16, S=64, D=5, discard=1, t0=0.0, dt=0.0005, T=6000, lo=0.5, hi=0.9),
+}
+
+
+@pytest.mark.parametrize("name", sorted(DEPTH_EXHAUSTIVE))
+def test_depth_planes_exhaustive_paths(ctx, name):
+    """Every output against the oracle: a wrong bin for one (pixel, depth,
+    pair) moves that output by far more than the tolerance (spiky data)."""
+    p = DEPTH_EXHAUSTIVE[name]
+    L, S, D, T = p["L"], p["S"], p["D"], p["T"]
+    rng = np.random.default_rng(L * 1000 + S + D)
+    lasers = wall_grid(8, 1.0)[rng.choice(64, L, replace=False)].astype(np.float32)
+    sensors = wall_grid(16, 1.0)[rng.choice(256, S, replace=False)].astype(np.float32)
+    g = Geometry(laser_targets=lasers, sensor_targets=sensors, num_time_bins=T,
+                 time_step=p["dt"], hidden_min=(-0.5, 0.2, -0.5), hidden_max=(0.5, 1.0, 0.5),
+                 voxel_resolution=(8, 8, 8), confocal=False,
+                 discard_first_last_bounces=p["discard"], time_offset=p["t0"],
+                 laser_position=(0.1, -0.8, 0.2), sensor_position=(-0.2, -0.7, 0.1),
+                 wall_size=(1.0, 1.0))
+    info = make_info(g)
+    I = spiky(rng, (L, S, T))
+    depths = np.linspace(p["lo"], p["hi"], D, endpoint=False)
+    ref = oracle.depth(info, I, lasers, depths, sensor=sensors, exhaustive=True)
+    out = ctx.backproject_depth(info, cuda(I), cuda(lasers), cuda(depths),
+                                sensor_targets=cuda(sensors)).cpu().numpy().astype(np.float64)
+    assert out.shape == ref.shape
+    # FP32 partial sums: ~1e-6 relative per output.
+    err = np.abs(out - ref) / np.maximum(np.abs(ref), 1e-3 * np.abs(ref).max())
+    assert err.max() <= 2e-5, (name, np.unravel_index(np.argmax(err), err.shape))
+
+
 DEPTH_ALIGNED = {
     "64": dict(nx=64, wall=1.0, T=256, lo=0.3, hi=0.5, D=20),
     "128": dict(nx=128, wall=2.0, T=512, lo=0.6, hi=0.9, D=4),

@@ -151,6 +151,39 @@ def test_c3_full_exhaustive_matches_oracle_slice(ctx):
     assert bool(torch.isfinite(vol).all())
 
 
+def test_c3_full_depth_planes_match_oracle(ctx):
+    """200 depth planes of the 16^2 x 64^2 x 1024 capture (staged-window
+    kernel, row-wrapping workgroups included) against the oracle on two of
+    the planes, output by output: one wrong bin moves an output far beyond
+    the tolerance."""
+    g = exhaustive_geometry()
+    info = make_info(g)
+    dev = torch.device("cuda", 0)
+    lasers = torch.from_numpy(g.laser_targets).to(dev)
+    sensors = torch.from_numpy(g.sensor_targets).to(dev)
+    sc = random_spheres(np.random.default_rng(14), 8, 8000)
+    I = ctx.render_exhaustive(lasers, sensors, torch.from_numpy(sc.points.astype(np.float32)).to(dev),
+                              torch.from_numpy(sc.albedo.astype(np.float32)).to(dev),
+                              g.num_time_bins, g.time_step)
+    I.mul_(1000.0 / I.max())
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(14)
+    I = torch.poisson(I, generator=gen)
+    focus = abs((np.float32(g.hidden_min[1]) + np.float32(g.hidden_max[1])) * np.float32(0.5))
+    lo = float(np.float32(focus - np.float32(0.1)))
+    hi = float(np.float32(focus + np.float32(0.1)))
+    depths = lo + (hi - lo) / 200 * np.arange(200)
+    act = ctx.backproject_depth(info, I, lasers, torch.from_numpy(depths).to(dev),
+                                sensor_targets=sensors).cpu().numpy().astype(np.float64)
+    cols = [3, 151]
+    ref = oracle.depth(info, I.cpu().numpy(), g.laser_targets, depths[cols],
+                       sensor=g.sensor_targets, exhaustive=True)
+    got = act[:, cols]
+    err = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-3 * np.abs(ref).max())
+    assert err.max() <= 1e-5
+    assert np.isfinite(act).all()
+
+
 def test_lct_full_c2_matches_restatement(ctx, c2):
     """LCT of the full capture (128 x 128 x 1024 padded to 256 x 256 x 2048,
     snr 8) against the NumPy/SciPy restatement of python/lct.py."""
