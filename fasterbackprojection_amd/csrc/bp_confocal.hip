@@ -38,6 +38,7 @@ struct ConfocalArgs {
     uint32_t chunk_len;
     int batch;
     int W;
+    float inv_W;
     const float* thr;      // S_k rows, (T + 1) floats each
     uint32_t thr_rows;     // 1 (pure chain) or L
     float* corr;           // near-voxel corrections, same layout as partial
@@ -50,8 +51,12 @@ struct ConfocalArgs {
     const float* sample_w;
 };
 
-// Per scan point parameters staged in LDS (8 words each).
-enum { P_LX = 0, P_LY, P_LZ, P_BASE, P_QBIAS, P_COFF, P_NEAR, P_OFF };
+// Per scan point parameters staged in LDS (kPrmWords words each): position,
+// window base, estimate bias, offsets, near flag, scan point index and
+// sample weight.
+enum { P_LX = 0, P_LY, P_LZ, P_BASE, P_QBIAS, P_COFF, P_NEAR, P_OFF, P_L, P_WL, kPrmWords };
+// Window elements per lane whose loads are issued together.
+constexpr int kStage = 8;
 
 // Fast gather of one (voxel, scan point) pair.  s is exact (x4 when PURE);
 // the estimate ka is biased low so the exact bin is ka or ka + 1, decided by
@@ -117,7 +122,7 @@ __device__ __forceinline__ void confocal_voxel_body(const ConfocalArgs& a) {
     if (a.skip && *a.skip) return;
     extern __shared__ float smem[];
     float2* win = reinterpret_cast<float2*>(smem);           // batch * W pairs
-    float* prm = smem + 2 * a.batch * a.W;                    // batch * 8
+    float* prm = smem + 2 * a.batch * a.W;                    // batch * kPrmWords
 
     const int lane = threadIdx.x;
     const uint32_t tile = blockIdx.x % a.g.num_tiles;
@@ -175,7 +180,7 @@ __device__ __forceinline__ void confocal_voxel_body(const ConfocalArgs& a) {
             // multiple of the rounding error of the largest term of dist.
             const float mag = 2.f * rmax + fabsf(a.t0) + fabsf(off);
             const float qbias = 32.f * 1.1920929e-7f * mag * a.inv_dt + 1.0e-5f;
-            float* P = prm + lane * 8;
+            float* P = prm + lane * kPrmWords;
             P[P_LX] = p[0];
             P[P_LY] = p[1];
             P[P_LZ] = p[2];
@@ -184,36 +189,52 @@ __device__ __forceinline__ void confocal_voxel_body(const ConfocalArgs& a) {
             P[P_COFF] = off - a.t0;
             P[P_NEAR] = ((dmin < 0.0045f && dmax > -0.0045f) || rmin < 1.0e-3f) ? 1.f : 0.f;
             P[P_OFF] = off;
+            P[P_L] = __uint_as_float(l);
+            P[P_WL] = tile_l ? tile_w[lb + lane] : 1.f;
         }
         __syncthreads();
-        for (int jl = 0; jl < nb; ++jl) {
-            const uint32_t l = tile_l ? tile_l[lb + jl] : lb + jl;
-            const float wl = tile_l ? tile_w[lb + jl] : 1.f;
-            const int kmin = jl * a.W - __float_as_int(prm[jl * 8 + P_BASE]);
-            float sc = 1.f;
-            if constexpr (STORAGE != FBP_STORAGE_F32) sc = a.scales[l];
-            const float* thr = a.thr + size_t(a.thr_rows > 1 ? l : 0) * (a.T + 1);
-            for (int w = lane; w < a.W; w += 64) {
-                const int k = kmin + w;
-                float v = 0.f;
-                if (k >= 0 && k < int(a.T)) {
-                    const size_t idx = (LAYOUT == FBP_LAYOUT_TIME_FASTEST)
-                                           ? size_t(l) * a.T + uint32_t(k)
-                                           : size_t(uint32_t(k)) * a.L + l;
-                    v = load_sample<STORAGE>(a.data, idx, sc);
-                    // Quantised storage was thresholded before quantisation.
-                    if constexpr (STORAGE == FBP_STORAGE_F32) v = (v > kEps) ? v : 0.f;
-                    v *= wl;
-                }
+        // Window staging: element f = jl * W + w of the batch.  Each lane
+        // issues the loads of kStage elements (the sample and the threshold
+        // of bin kmin + w; unused ones read element 0) before writing any.
+        const int total = nb * a.W;
+        for (int f0 = 0; f0 < total; f0 += 64 * kStage) {
+            float v[kStage], t[kStage];
+#pragma unroll
+            for (int u = 0; u < kStage; ++u) {
+                const int f = f0 + u * 64 + lane;
+                const bool in = f < total;
+                // f / W through the reciprocal: exact for f < 2^22.
+                const int jl = in ? int((float(f) + 0.5f) * a.inv_W) : 0;
+                const int w = f - jl * a.W;
+                const float* P = prm + jl * kPrmWords;
+                const uint32_t l = __float_as_uint(P[P_L]);
+                const int k = jl * a.W - __float_as_int(P[P_BASE]) + w;
+                const bool ok = in && k >= 0 && k < int(a.T);
+                const size_t idx = !ok ? 0
+                                   : (LAYOUT == FBP_LAYOUT_TIME_FASTEST)
+                                       ? size_t(l) * a.T + uint32_t(k)
+                                       : size_t(uint32_t(k)) * a.L + l;
+                float sc = 1.f;
+                if constexpr (STORAGE != FBP_STORAGE_F32) sc = a.scales[ok ? l : 0];
+                float x = load_sample<STORAGE>(a.data, idx, sc);
+                // Quantised storage was thresholded before quantisation.
+                if constexpr (STORAGE == FBP_STORAGE_F32) x = (x > kEps) ? x : 0.f;
+                v[u] = ok ? x * P[P_WL] : 0.f;
                 // Threshold of bin k (only bins 0..T matter; outside, both
                 // candidate samples are zero).
-                const float t = (k < 0) ? 0.f : (k > int(a.T) ? __int_as_float(0x7f800000) : thr[k]);
-                win[jl * a.W + w] = make_float2(v, t * tscale);
+                const float* thr = a.thr + size_t(a.thr_rows > 1 ? l : 0) * (a.T + 1);
+                const float th = thr[min(max(k, 0), int(a.T))];
+                t[u] = (k < 0) ? 0.f : (k > int(a.T) ? __int_as_float(0x7f800000) : th);
+            }
+#pragma unroll
+            for (int u = 0; u < kStage; ++u) {
+                const int f = f0 + u * 64 + lane;
+                if (f < total) win[f] = make_float2(v[u], t[u] * tscale);
             }
         }
         __syncthreads();
         for (int jl = 0; jl < nb; ++jl) {
-            const float* P = prm + jl * 8;
+            const float* P = prm + jl * kPrmWords;
             const float lxp = rfl(P[P_LX]), lyp = rfl(P[P_LY]), lzp = rfl(P[P_LZ]);
             const int base = rfl(__float_as_int(P[P_BASE]));
             const float qbias = rfl(P[P_QBIAS]);
@@ -433,9 +454,10 @@ fbp_status confocal_voxel_impl(fbp_context ctx, const fbp_reconstruction_info* i
                     "confocal: bin window of %d bins per tile exceeds LDS (voxels too large for "
                     "the bin width)",
                     a.W);
-    // LDS per scan point: W {I, S} pairs + 8 parameters.
+    // LDS per scan point: W {I, S} pairs + the parameters.
     a.batch = 16;
-    while (a.batch > 1 && size_t(a.batch) * (2 * a.W + 8) * 4 > 20480) a.batch /= 2;
+    while (a.batch > 1 && size_t(a.batch) * (2 * a.W + kPrmWords) * 4 > 20480) a.batch /= 2;
+    a.inv_W = 1.0f / float(a.W);
 
     // Tile-sampled lists are short: smaller chunks keep the CUs busy.
     const uint32_t nc = pick_chunks(ctx, a.g.num_tiles, a.items, samples ? 8 : 64);
@@ -471,7 +493,7 @@ fbp_status confocal_voxel_impl(fbp_context ctx, const fbp_reconstruction_info* i
                        a.skip);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     const dim3 grid(chunks * a.g.num_tiles);
-    const size_t shmem = size_t(a.batch) * (2 * a.W + 8) * sizeof(float);
+    const size_t shmem = size_t(a.batch) * (2 * a.W + kPrmWords) * sizeof(float);
     if (tr->layout == FBP_LAYOUT_TIME_MAJOR) {
         launch_confocal<FBP_STORAGE_F32, FBP_LAYOUT_TIME_MAJOR>(a, pure, grid, shmem, s);
     } else if (tr->storage == FBP_STORAGE_F32) {

@@ -269,6 +269,9 @@ __global__ void lct_resample_planes_kernel(const float* __restrict__ in, float* 
 // in frequency to bit-reversed output, read back through the bit reversal.
 constexpr int kLctCols = 8;
 constexpr int kLctThreads = 512;
+// The 2048-point z-pass: 1024 threads, kZU = 128 rows of kLctCols columns.
+constexpr int kZ2048Threads = 1024;
+constexpr int kZU = kZ2048Threads / kLctCols;
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -404,7 +407,7 @@ __device__ __forceinline__ float2 w2048(const float2* twl, uint32_t m) {
     return make_float2(sg * t.x, INV ? -sg * t.y : sg * t.y);
 }
 
-__global__ __launch_bounds__(kLctThreads) void lct_zpass2048_kernel(float2* __restrict__ S,
+__global__ __launch_bounds__(kZ2048Threads) void lct_zpass2048_kernel(float2* __restrict__ S,
                                                                     const float2* __restrict__ inv,
                                                                     const float2* __restrict__ tw,
                                                                     uint32_t N) {
@@ -412,18 +415,18 @@ __global__ __launch_bounds__(kLctThreads) void lct_zpass2048_kernel(float2* __re
     float2* twl = lds;          // 1024 twiddles
     float2* buf = lds + 1024;   // [2048][kLctCols] + padding
     const uint32_t tid = threadIdx.x;
-    const uint32_t col = tid % kLctCols, u = tid / kLctCols;  // u in [0, 64)
+    const uint32_t col = tid % kLctCols, u = tid / kLctCols;  // u in [0, kZU)
     const uint32_t H = N + 1, Y2 = 2 * N;
     const uint32_t kx0 = blockIdx.x * kLctCols, ky = blockIdx.y, kx = kx0 + col;
     const bool live = kx < H;
     auto at = [&](uint32_t pos) -> float2& { return buf[pos * kLctCols + col + 4u * (pos >> 3)]; };
-    for (uint32_t k = tid; k < 1024; k += kLctThreads) twl[k] = tw[k];
+    for (uint32_t k = tid; k < 1024; k += kZ2048Threads) twl[k] = tw[k];
     // The filter values this thread applies in pass 3, loaded now so that
     // their latency overlaps the forward passes.
-    float2 fil[4][8];
+    float2 fil[256 / kZU][8];
 #pragma unroll
-    for (int h = 0; h < 4; ++h) {
-        const uint32_t r = u + 64u * h, kl = r / 16u + 16u * (r % 16u);
+    for (int h = 0; h < 256 / kZU; ++h) {
+        const uint32_t r = u + uint32_t(kZU) * h, kl = r / 16u + 16u * (r % 16u);
 #pragma unroll
         for (int k2 = 0; k2 < 8; ++k2)
             fil[h][k2] = live ? inv[(size_t(kl + 256u * k2) * Y2 + ky) * H + kx]
@@ -431,10 +434,10 @@ __global__ __launch_bounds__(kLctThreads) void lct_zpass2048_kernel(float2* __re
     }
     __syncthreads();
 
-    // Forward pass 1: a in {u, u + 64}; x[a + 128 b] = 0 for b >= 8.
+    // Forward pass 1: a = u; x[a + 128 b] = 0 for b >= 8.
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const uint32_t a = u + 64u * h;
+    for (int h = 0; h < 128 / kZU; ++h) {
+        const uint32_t a = u + uint32_t(kZU) * h;
         float2 v[16];
 #pragma unroll
         for (int b = 0; b < 16; ++b) {
@@ -447,18 +450,18 @@ __global__ __launch_bounds__(kLctThreads) void lct_zpass2048_kernel(float2* __re
             at(k0 * 128u + a) = k0 == 0 ? v[0] : cmul(v[k0], w2048<false>(twl, a * k0));
     }
     __syncthreads();
-    // Forward pass 2: q = k0 * 8 + a' in {u, u + 64}.
-    float2 r2[2][16];
+    // Forward pass 2: q = k0 * 8 + a' = u.
+    float2 r2[128 / kZU][16];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const uint32_t q = u + 64u * h, k0 = q / 8u, ap = q % 8u;
+    for (int h = 0; h < 128 / kZU; ++h) {
+        const uint32_t q = u + uint32_t(kZU) * h, k0 = q / 8u, ap = q % 8u;
 #pragma unroll
         for (int c = 0; c < 16; ++c) r2[h][c] = at(k0 * 128u + ap + 8u * c);
     }
     __syncthreads();
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const uint32_t q = u + 64u * h, k0 = q / 8u, ap = q % 8u;
+    for (int h = 0; h < 128 / kZU; ++h) {
+        const uint32_t q = u + uint32_t(kZU) * h, k0 = q / 8u, ap = q % 8u;
         dft_reg<16, false>(r2[h]);
 #pragma unroll
         for (int k1 = 0; k1 < 16; ++k1)
@@ -467,18 +470,18 @@ __global__ __launch_bounds__(kLctThreads) void lct_zpass2048_kernel(float2* __re
     }
     __syncthreads();
     // Forward pass 3 (radix 8 over a'), filter, inverse pass 1 (radix 8
-    // over k2): r = k0 * 16 + k1 in {u, u + 64, u + 128, u + 192}.
-    float2 r3[4][8];
+    // over k2): r = k0 * 16 + k1 in {u, u + kZU}.
+    float2 r3[256 / kZU][8];
 #pragma unroll
-    for (int h = 0; h < 4; ++h) {
-        const uint32_t r = u + 64u * h, k0 = r / 16u, k1 = r % 16u;
+    for (int h = 0; h < 256 / kZU; ++h) {
+        const uint32_t r = u + uint32_t(kZU) * h, k0 = r / 16u, k1 = r % 16u;
 #pragma unroll
         for (int ap = 0; ap < 8; ++ap) r3[h][ap] = at(k0 * 128u + k1 * 8u + ap);
     }
     __syncthreads();
 #pragma unroll
-    for (int h = 0; h < 4; ++h) {
-        const uint32_t r = u + 64u * h, k0 = r / 16u, k1 = r % 16u, kl = k0 + 16u * k1;
+    for (int h = 0; h < 256 / kZU; ++h) {
+        const uint32_t r = u + uint32_t(kZU) * h, k0 = r / 16u, k1 = r % 16u, kl = k0 + 16u * k1;
         dft_reg<8, false>(r3[h]);
 #pragma unroll
         for (int k2 = 0; k2 < 8; ++k2) r3[h][k2] = cmul(r3[h][k2], fil[h][k2]);
@@ -488,17 +491,17 @@ __global__ __launch_bounds__(kLctThreads) void lct_zpass2048_kernel(float2* __re
             at(z0 * 256u + kl) = z0 == 0 ? r3[h][0] : cmul(r3[h][z0], w2048<true>(twl, kl * z0));
     }
     __syncthreads();
-    // Inverse pass 2: q = z0 * 16 + l0 in {u, u + 64}.
+    // Inverse pass 2: q = z0 * 16 + l0 = u.
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const uint32_t q = u + 64u * h, z0 = q / 16u, l0 = q % 16u;
+    for (int h = 0; h < 128 / kZU; ++h) {
+        const uint32_t q = u + uint32_t(kZU) * h, z0 = q / 16u, l0 = q % 16u;
 #pragma unroll
         for (int l1 = 0; l1 < 16; ++l1) r2[h][l1] = at(z0 * 256u + l0 + 16u * l1);
     }
     __syncthreads();
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const uint32_t q = u + 64u * h, z0 = q / 16u, l0 = q % 16u;
+    for (int h = 0; h < 128 / kZU; ++h) {
+        const uint32_t q = u + uint32_t(kZU) * h, z0 = q / 16u, l0 = q % 16u;
         dft_reg<16, true>(r2[h]);
 #pragma unroll
         for (int z1 = 0; z1 < 16; ++z1)
@@ -506,11 +509,11 @@ __global__ __launch_bounds__(kLctThreads) void lct_zpass2048_kernel(float2* __re
                 z1 == 0 ? r2[h][0] : cmul(r2[h][z1], w2048<true>(twl, 8u * l0 * z1));
     }
     __syncthreads();
-    // Inverse pass 3 (radix 16 over l0): q = z0 * 16 + z1 in {u, u + 64};
+    // Inverse pass 3 (radix 16 over l0): q = z0 * 16 + z1 = u;
     // outputs z = z0 + 8 z1 + 128 z2 for z2 < 8 (z < 1024).
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const uint32_t q = u + 64u * h, z0 = q / 16u, z1 = q % 16u;
+    for (int h = 0; h < 128 / kZU; ++h) {
+        const uint32_t q = u + uint32_t(kZU) * h, z0 = q / 16u, z1 = q % 16u;
 #pragma unroll
         for (int l0 = 0; l0 < 16; ++l0) r2[h][l0] = at(z0 * 256u + z1 * 16u + l0);
         dft_reg<16, true>(r2[h]);
@@ -882,7 +885,7 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
             FBP_CHECK_HIP(ctx, hipFuncSetAttribute(
                                    reinterpret_cast<const void*>(lct_zpass2048_kernel),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(lds2048)));
-            hipLaunchKernelGGL(lct_zpass2048_kernel, zgrid, dim3(kLctThreads), lds2048, s, spec,
+            hipLaunchKernelGGL(lct_zpass2048_kernel, zgrid, dim3(kZ2048Threads), lds2048, s, spec,
                                c->invpsf, c->tw, N);
         } else {
             hipLaunchKernelGGL(lct_zpass_kernel, zgrid, dim3(kLctThreads), zpass_lds, s, spec,
