@@ -393,6 +393,9 @@ DEPTH_EXHAUSTIVE = {
     "near_wall": dict(L=16, S=64, D=6, discard=1, t0=0.0, dt=0.0096, T=256, lo=0.0, hi=0.3),
     "many_blocks": dict(L=64, S=64, D=50, discard=1, t0=0.0, dt=0.0096, T=256, lo=0.5, hi=0.9),
     "wide_window": dict(L=16, S=64, D=5, discard=1, t0=0.0, dt=0.0005, T=6000, lo=0.5, hi=0.9),
+    # 12 pairs: a single staged batch; 24 pairs: two (the second ragged).
+    "one_batch": dict(L=3, S=4, D=7, discard=1, t0=0.0, dt=0.0096, T=256, lo=0.5, hi=0.9),
+    "two_batches": dict(L=4, S=6, D=9, discard=0, t0=0.1, dt=0.0096, T=256, lo=0.5, hi=0.9),
 }
 
 
