@@ -32,6 +32,7 @@ FLOPS_PER_UPDATE = 16          # confocal, FMA = 2 (reference formula)
 PEAK_FP32_TFLOPS = 157.3       # MI355X FP32 (datasheet)
 PEAK_LDS_TBPS = 78.6           # 256 CUs x 128 B/clk x 2.4 GHz
 PEAK_HBM_GBPS = 8000.0         # MI355X HBM3E (datasheet)
+PEAK_BF16_TFLOPS = 2500.0      # MI355X dense BF16 MFMA (datasheet, no sparsity)
 VALU_LANE_SLOTS = 39.3e12      # 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (non-packed issue)
 TRAFFIC_PROFILE = ROOT / "profiles" / "confocal_128cube_traffic.json"
 
@@ -70,6 +71,8 @@ def parse():
                     help="skip the LCT leg (128^2 x 1024 padded to 256x256x2048)")
     ap.add_argument("--no-depth", action="store_true",
                     help="skip the depth-plane mode + slice images leg")
+    ap.add_argument("--no-filters", action="store_true",
+                    help="skip the filter leg (Laplacian, LoG, normalise, LoGFFT, filter_H)")
     ap.add_argument("--no-exhaustive", action="store_true",
                     help="skip the exhaustive leg (16^2 lasers x 64^2 sensors x 1024 -> 64^3)")
     return ap.parse_args()
@@ -211,6 +214,7 @@ def main():
     exhaustive = None if args.no_exhaustive else exhaustive_leg(ctx, args)
     lct = None if args.no_lct else lct_leg(ctx, args)
     depth = None if args.no_depth else depth_leg(ctx, args)
+    filters = None if args.no_filters else filters_leg(ctx, args)
 
     out = {
         "metric": METRIC,
@@ -269,9 +273,79 @@ def main():
         "exhaustive": exhaustive,
         "lct": lct,
         "depth_planes": depth,
+        "filters": filters,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def filters_leg(ctx, args, reps=5):
+    """The post-filters on a 128^3 volume (Laplacian k=5, separable LoG
+    sigma=1 k=5, min-max normalise, LoGFFT sigma=1) and filter_H on the
+    128^2 x 1024 transient (wl = 10 bins), each against the HBM roofline
+    with its algorithmic bytes: read V*4 + write V*4 per filter; normalise
+    reads twice (min/max, apply) and writes once; LoGFFT two FFTs of
+    (read + write) plus the spectrum multiply over the half spectrum;
+    filter_H two FFTs x (read + write) x 8 bytes x L x nt_pad."""
+    import numpy as np
+    import torch
+
+    dev = torch.device("cuda", 0)
+    n = 128
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(9)
+    vol = torch.rand((n, n, n), device=dev, generator=gen)
+    out = torch.empty_like(vol)
+    V = n ** 3
+
+    def timed(fn, setup=None):
+        fn()
+        torch.cuda.synchronize()
+        ms = []
+        for _ in range(reps):
+            if setup:
+                setup()
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            b.synchronize()
+            ms.append(a.elapsed_time(b))
+        return float(np.median(ms))
+
+    def entry(ms, nbytes):
+        gbps = nbytes / (ms * 1e-3) / 1e9
+        return {"ms": ms, "roofline": {"bound": "hbm", "achieved": gbps, "peak": PEAK_HBM_GBPS,
+                                       "unit": "GB/s", "frac": gbps / PEAK_HBM_GBPS,
+                                       "algorithmic_bytes": nbytes}}
+
+    res = {"workload": "128^3 volume (Laplacian, LoG, normalise, LoGFFT); "
+                       "128^2 x 1024 transient at 16 ps (filter_H, wl = 10 bins)"}
+    res["laplacian_k5"] = entry(timed(lambda: ctx.laplacian(vol, 5, out=out)), 8 * V)
+    res["log_k5_sigma1"] = entry(timed(lambda: ctx.log(vol, 5, 1.0, out=out)), 8 * V)
+    work = vol.clone()
+    res["normalize"] = entry(timed(lambda: ctx.normalize(work),
+                                   setup=lambda: work.copy_(vol)), 12 * V)
+    half = n * n * (n // 2 + 1)
+    res["log_fft_sigma1"] = entry(timed(lambda: ctx.log_fft(work),
+                                        setup=lambda: work.copy_(vol)),
+                                  2 * (4 * V + 8 * half) + 16 * half)
+    L, T, dt = 128 * 128, 1024, 0.0048
+    I = torch.rand((L, T), device=dev, generator=gen)
+    Iw = I.clone()
+    wl = 10 * dt
+    sigma = wl / np.sqrt(2.0)
+    t6 = int(round(6 * sigma / dt))
+    t6 += t6 % 2
+    nt_pad = T + 2 * (t6 - 1)
+    res["filter_h"] = entry(timed(lambda: ctx.filter_h(Iw, dt, wl), setup=lambda: Iw.copy_(I)),
+                            2 * 8 * L * nt_pad * 2)
+    res["filter_h"]["nt_pad"] = nt_pad
+    log(f"filters: {res}")
+    del vol, out, work, I, Iw
+    torch.cuda.empty_cache()
+    return res
 
 
 def precision_leg(ctx, args):
@@ -701,6 +775,23 @@ def learned_leg(ctx, n_scenes=16):
            "rel_l2_learned_vs_plain": float(np.mean(e_learn)),
            "rel_l2_unrefined_mis_vs_plain": float(np.mean(e_base)),
            "fallbacks": fb, "residual_threshold": lrn.threshold}
+    # The refiner's forward pass alone against the dense BF16 MFMA peak:
+    # flops = sum over layers of 2 * V_out * k^3 * Cin * Cout at 64^3
+    # (16/32/64 channels; the 1->16 and 16->1 layers run on VALU).
+    S = 64
+    layers = [(S, 3, 1, 16), (S // 2, 3, 16, 32), (S // 4, 3, 32, 64), (S // 2, 3, 64, 32),
+              (S, 1, 32, 16), (S, 3, 16, 1)]
+    flops = sum(2 * r ** 3 * k ** 3 * ci * co for r, k, ci, co in layers)
+    gen = torch.Generator(device=torch.device("cuda", 0))
+    gen.manual_seed(3)
+    x = torch.rand((S, S, S), device=torch.device("cuda", 0), generator=gen)
+    yb = torch.empty_like(x)
+    t_ref = timed(lambda: lrn.net.forward(x, out=yb))
+    tf = flops / (t_ref * 1e-3) / 1e12
+    res["refiner_forward"] = {"ms": t_ref, "flops": flops,
+                              "roofline": {"bound": "mfma", "achieved": tf,
+                                           "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                                           "frac": tf / PEAK_BF16_TFLOPS}}
     log(f"learned: {res}")
     return res
 

@@ -2,8 +2,9 @@
 //
 // Laplacian (FB/PostprocessingFilters.cu:81-98, FB/transient_postprocessing.cuh:21-50):
 //   out = k^3 v - sum_{k^3 box, in-bounds only} v
-// computed as three 1D box passes (zero outside the volume) fused with the
-// final subtraction.
+// computed as 1D box sums along x, y and z (zero outside the volume) and the
+// final subtraction, in one pass over the volume (boxes up to 15 wide;
+// wider boxes take three passes through scratch).
 //
 // LoG (FB/PostprocessingFilters.cu:12-32, 102-175; transient_postprocessing.cuh:52-83):
 // the reference convolves with a dense k^3 kernel, clamp-to-edge per axis.
@@ -11,9 +12,11 @@
 // normalised Gaussian, so when no Gaussian tap is thresholded away it
 // factors exactly into rank-1 terms:
 //   LoG(v) = (Hx Gy Gz + Gx Hy Gz + Gx Gy Hz)(v) / s^4 - 3/s^2 Gx Gy Gz (v) - c Bx By Bz (v)
-// with g = e/sum(e), h = x^2 g, b = 1 (clamp-to-edge 1D passes).  That is
-// three HBM passes instead of k^3 gathers per voxel; otherwise the dense path
-// runs.
+// with g = e/sum(e), h = x^2 g, b = 1 (clamp-to-edge 1D passes), evaluated
+// in one pass over the volume (x and y passes in LDS per z-plane, the z pass
+// over a register ring of planes) instead of k^3 gathers per voxel;
+// otherwise the dense path runs.
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <vector>
@@ -95,76 +98,6 @@ __global__ void laplacian_final_kernel(const float* __restrict__ vol, const floa
     out[v] = k3 * vol[v] - s;
 }
 
-// ---- LoG, separable form.
-// Pass X: in -> (Gx v, Hx v, Bx v)
-__global__ void log_pass_x(const float* __restrict__ in, float* __restrict__ g, float* __restrict__ h,
-                           float* __restrict__ b, AxisInfo a, Taps t) {
-    const size_t n = size_t(a.X) * a.Y * a.Z;
-    const size_t v = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (v >= n) return;
-    uint32_t x, y, z;
-    decompose(v, a, x, y, z);
-    float sg = 0.f, sh = 0.f, sb = 0.f;
-    for (int d = -t.half; d <= t.half; ++d) {
-        const int cc = min(max(int(x) + d, 0), int(a.X) - 1);
-        const float val = in[shifted<0>(x, y, z, cc, a)];
-        sg = fmaf(t.g[d + t.half], val, sg);
-        sh = fmaf(t.h[d + t.half], val, sh);
-        sb += val;
-    }
-    g[v] = sg;
-    h[v] = sh;
-    b[v] = sb;
-}
-
-// Pass Y: (gx, hx, bx) -> (Gy gx, Gy hx, Hy gx, By bx)
-__global__ void log_pass_y(const float* __restrict__ gx, const float* __restrict__ hx,
-                           const float* __restrict__ bx, float* __restrict__ gg,
-                           float* __restrict__ hg, float* __restrict__ gh, float* __restrict__ bb,
-                           AxisInfo a, Taps t) {
-    const size_t n = size_t(a.X) * a.Y * a.Z;
-    const size_t v = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (v >= n) return;
-    uint32_t x, y, z;
-    decompose(v, a, x, y, z);
-    float s_gg = 0.f, s_hg = 0.f, s_gh = 0.f, s_bb = 0.f;
-    for (int d = -t.half; d <= t.half; ++d) {
-        const int cc = min(max(int(y) + d, 0), int(a.Y) - 1);
-        const size_t i = shifted<1>(x, y, z, cc, a);
-        const float vg = gx[i], vh = hx[i];
-        s_gg = fmaf(t.g[d + t.half], vg, s_gg);
-        s_hg = fmaf(t.g[d + t.half], vh, s_hg);
-        s_gh = fmaf(t.h[d + t.half], vg, s_gh);
-        s_bb += bx[i];
-    }
-    gg[v] = s_gg;
-    hg[v] = s_hg;
-    gh[v] = s_gh;
-    bb[v] = s_bb;
-}
-
-// Pass Z: out = (Gz hg + Gz gh + Hz gg) * a4 - Gz gg * a2 - c * Bz bb
-__global__ void log_pass_z(const float* __restrict__ gg, const float* __restrict__ hg,
-                           const float* __restrict__ gh, const float* __restrict__ bb,
-                           float* __restrict__ out, AxisInfo a, Taps t, float a4, float a2,
-                           float c) {
-    const size_t n = size_t(a.X) * a.Y * a.Z;
-    const size_t v = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (v >= n) return;
-    uint32_t x, y, z;
-    decompose(v, a, x, y, z);
-    float s_r2 = 0.f, s_g = 0.f, s_b = 0.f;
-    for (int d = -t.half; d <= t.half; ++d) {
-        const int cc = min(max(int(z) + d, 0), int(a.Z) - 1);
-        const size_t i = shifted<2>(x, y, z, cc, a);
-        const float gz = t.g[d + t.half], hz = t.h[d + t.half];
-        s_r2 = fmaf(gz, hg[i] + gh[i], fmaf(hz, gg[i], s_r2));
-        s_g = fmaf(gz, gg[i], s_g);
-        s_b += bb[i];
-    }
-    out[v] = (s_r2 * a4 - s_g * a2) - c * s_b;
-}
-
 // ---- LoG, dense form (kernel with thresholded taps): the reference's loop.
 __global__ void log_dense_kernel(const float* __restrict__ in, float* __restrict__ out,
                                  const float* __restrict__ w, AxisInfo a, int k, int half) {
@@ -190,6 +123,195 @@ __global__ void log_dense_kernel(const float* __restrict__ in, float* __restrict
 
 // ---- min / max and normalisation (deterministic two-stage reductions).
 constexpr int kRedBlocks = 1024;
+
+// ---- Fused forms: one workgroup owns a 16 x 16 (x, y) tile and walks a
+// chunk of z-planes.  Each input plane is read once into LDS with its x/y
+// halo; the x and y passes run in LDS, the z pass over a register ring of
+// the last 2H + 1 planes' y-pass results.  Every output takes the same
+// operations in the same order as the three-pass kernels above (bit-equal
+// results) with one read of the volume (plus halos from L2) and one write.
+constexpr int kFT = 16;  // tile edge (x and y); 256 threads
+
+template <int H>
+__global__ __launch_bounds__(256) void log_fused_kernel(const float* __restrict__ in,
+                                                        float* __restrict__ out, AxisInfo a, Taps t,
+                                                        float a4, float a2, float c, int zchunk) {
+    constexpr int K = 2 * H + 1, PW = kFT + 2 * H, PH = kFT + 2 * H;
+    __shared__ float pl[PH][PW + 1];
+    __shared__ float sg[PH][kFT], sh[PH][kFT], sb[PH][kFT];
+    const int tid = threadIdx.x, tx = tid & (kFT - 1), ty = tid / kFT;
+    const int X = int(a.X), Y = int(a.Y), Z = int(a.Z);
+    const int x0 = int(blockIdx.x) * kFT, y0 = int(blockIdx.y) * kFT;
+    const int zb = int(blockIdx.z) * zchunk, ze = min(Z, zb + zchunk);
+    float tg[K], th[K];
+#pragma unroll
+    for (int d = 0; d < K; ++d) {
+        tg[d] = t.g[d];
+        th[d] = t.h[d];
+    }
+    float rgg[K], rhg[K], rgh[K], rbb[K];
+#pragma unroll
+    for (int d = 0; d < K; ++d) rgg[d] = rhg[d] = rgh[d] = rbb[d] = 0.f;
+    // Plane clamp(z) with clamp-to-edge x/y halo, loaded into registers one
+    // plane ahead of its use.
+    constexpr int NPF = (PH * PW + 255) / 256;
+    float pf[NPF];
+    auto load = [&](int z) {
+        const int zc = min(max(z, 0), Z - 1);
+#pragma unroll
+        for (int u = 0; u < NPF; ++u) {
+            const int i = min(tid + u * 256, PH * PW - 1);
+            const int r = i / PW, q = i - r * PW;
+            const int gy = min(max(y0 + r - H, 0), Y - 1), gx = min(max(x0 + q - H, 0), X - 1);
+            pf[u] = in[(size_t(zc) * Y + gy) * X + gx];
+        }
+    };
+    load(zb - H);
+    for (int zi = zb - H; zi < ze + H; ++zi) {
+#pragma unroll
+        for (int u = 0; u < NPF; ++u) {
+            const int i = tid + u * 256;
+            if (i < PH * PW) pl[i / PW][i % PW] = pf[u];
+        }
+        __syncthreads();
+        if (zi + 1 < ze + H) load(zi + 1);
+        // Pass X over every halo row.
+        for (int i = tid; i < PH * kFT; i += 256) {
+            const int r = i / kFT, q = i - r * kFT;
+            float g = 0.f, h = 0.f, b = 0.f;
+#pragma unroll
+            for (int d = 0; d < K; ++d) {
+                const float v = pl[r][q + d];
+                g = fmaf(tg[d], v, g);
+                h = fmaf(th[d], v, h);
+                b += v;
+            }
+            sg[r][q] = g;
+            sh[r][q] = h;
+            sb[r][q] = b;
+        }
+        __syncthreads();
+        // Pass Y for this thread's (x, y), pushed into the ring.
+        float gg = 0.f, hg = 0.f, gh = 0.f, bb = 0.f;
+#pragma unroll
+        for (int d = 0; d < K; ++d) {
+            const float vg = sg[ty + d][tx], vh = sh[ty + d][tx];
+            gg = fmaf(tg[d], vg, gg);
+            hg = fmaf(tg[d], vh, hg);
+            gh = fmaf(th[d], vg, gh);
+            bb += sb[ty + d][tx];
+        }
+#pragma unroll
+        for (int d = 0; d + 1 < K; ++d) {
+            rgg[d] = rgg[d + 1];
+            rhg[d] = rhg[d + 1];
+            rgh[d] = rgh[d + 1];
+            rbb[d] = rbb[d + 1];
+        }
+        rgg[K - 1] = gg;
+        rhg[K - 1] = hg;
+        rgh[K - 1] = gh;
+        rbb[K - 1] = bb;
+        // Pass Z: the ring holds planes zo - H .. zo + H.
+        const int zo = zi - H;
+        const int ox = x0 + tx, oy = y0 + ty;
+        if (zo >= zb && ox < X && oy < Y) {
+            float s_r2 = 0.f, s_g = 0.f, s_b = 0.f;
+#pragma unroll
+            for (int d = 0; d < K; ++d) {
+                s_r2 = fmaf(tg[d], rhg[d] + rgh[d], fmaf(th[d], rgg[d], s_r2));
+                s_g = fmaf(tg[d], rgg[d], s_g);
+                s_b += rbb[d];
+            }
+            out[(size_t(zo) * Y + oy) * X + ox] = (s_r2 * a4 - s_g * a2) - c * s_b;
+        }
+        __syncthreads();
+    }
+}
+
+// Laplacian, zero outside the volume: k^3 v - (box sum), box sums by
+// x, then y, then z in order (adding the zeros of the padding leaves each
+// partial sum unchanged).
+template <int H>
+__global__ __launch_bounds__(256) void laplacian_fused_kernel(const float* __restrict__ in,
+                                                              float* __restrict__ out, AxisInfo a,
+                                                              float k3, int zchunk) {
+    constexpr int K = 2 * H + 1, PW = kFT + 2 * H, PH = kFT + 2 * H;
+    __shared__ float pl[PH][PW + 1];
+    __shared__ float sb[PH][kFT];
+    const int tid = threadIdx.x, tx = tid & (kFT - 1), ty = tid / kFT;
+    const int X = int(a.X), Y = int(a.Y), Z = int(a.Z);
+    const int x0 = int(blockIdx.x) * kFT, y0 = int(blockIdx.y) * kFT;
+    const int zb = int(blockIdx.z) * zchunk, ze = min(Z, zb + zchunk);
+    float rb[K], rv[K];
+#pragma unroll
+    for (int d = 0; d < K; ++d) rb[d] = rv[d] = 0.f;
+    // Plane z (zero outside the volume), loaded into registers one plane
+    // ahead of its use; every load is issued (outside ones read element 0).
+    constexpr int NPF = (PH * PW + 255) / 256;
+    float pf[NPF];
+    auto load = [&](int z) {
+        const bool zin = z >= 0 && z < Z;
+#pragma unroll
+        for (int u = 0; u < NPF; ++u) {
+            const int i = min(tid + u * 256, PH * PW - 1);
+            const int r = i / PW, q = i - r * PW;
+            const int gy = y0 + r - H, gx = x0 + q - H;
+            const bool ok = zin && gy >= 0 && gy < Y && gx >= 0 && gx < X;
+            const float v = in[ok ? (size_t(z) * Y + gy) * X + gx : 0];
+            pf[u] = ok ? v : 0.f;
+        }
+    };
+    load(zb - H);
+    for (int zi = zb - H; zi < ze + H; ++zi) {
+#pragma unroll
+        for (int u = 0; u < NPF; ++u) {
+            const int i = tid + u * 256;
+            if (i < PH * PW) pl[i / PW][i % PW] = pf[u];
+        }
+        __syncthreads();
+        if (zi + 1 < ze + H) load(zi + 1);
+        for (int i = tid; i < PH * kFT; i += 256) {
+            const int r = i / kFT, q = i - r * kFT;
+            float b = 0.f;
+#pragma unroll
+            for (int d = 0; d < K; ++d) b += pl[r][q + d];
+            sb[r][q] = b;
+        }
+        __syncthreads();
+        float bxy = 0.f;
+#pragma unroll
+        for (int d = 0; d < K; ++d) bxy += sb[ty + d][tx];
+#pragma unroll
+        for (int d = 0; d + 1 < K; ++d) {
+            rb[d] = rb[d + 1];
+            rv[d] = rv[d + 1];
+        }
+        rb[K - 1] = bxy;
+        rv[K - 1] = pl[ty + H][tx + H];
+        const int zo = zi - H;
+        const int ox = x0 + tx, oy = y0 + ty;
+        if (zo >= zb && ox < X && oy < Y) {
+            float sz = 0.f;
+#pragma unroll
+            for (int d = 0; d < K; ++d) sz += rb[d];
+            out[(size_t(zo) * Y + oy) * X + ox] = k3 * rv[H] - sz;
+        }
+        __syncthreads();
+    }
+}
+
+// Grid of the fused kernels: z chunks so that about 4 workgroups per CU
+// run, each chunk at least 2H + 2 planes long (the halo planes are read
+// again by the neighbouring chunk).
+inline dim3 fused_grid(fbp_context ctx, const AxisInfo& a, int H, int& zchunk) {
+    const uint32_t tiles = ((a.X + kFT - 1) / kFT) * ((a.Y + kFT - 1) / kFT);
+    const uint32_t want = 4u * uint32_t(ctx->num_cus > 0 ? ctx->num_cus : 256);
+    const uint32_t nz = std::max<uint32_t>(1, (want + tiles - 1) / tiles);
+    zchunk = std::max<int>(2 * H + 2, int((a.Z + nz - 1) / nz));
+    return dim3((a.X + kFT - 1) / kFT, (a.Y + kFT - 1) / kFT,
+                unsigned((a.Z + uint32_t(zchunk) - 1) / uint32_t(zchunk)));
+}
 
 __global__ void minmax_partial_kernel(const float* __restrict__ v, size_t n,
                                       float* __restrict__ part) {
@@ -317,14 +439,30 @@ fbp_status fbp_filter_laplacian(fbp_context ctx, const float* in, float* out, fb
     FBP_CHECK(bind(ctx));
     const AxisInfo a{size.x, size.y, size.z};
     const size_t n = size_t(size.x) * size.y * size.z;
-    FBP_CHECK(ensure_scratch(ctx, 2 * n * sizeof(float)));
-    float* t0 = static_cast<float*>(ctx->scratch);
-    float* t1 = t0 + n;
     // halfFilterSize = floor(filterSize / 2); multiplier filterSize^3 in float.
     const float fk = float(kernel_size);
     const int half = int(std::floor(fk / 2.0f));
     const float k3 = fk * fk * fk;
     hipStream_t s = as_stream(stream);
+    if (half <= 7) {
+        int zchunk = 0;
+        const dim3 grid = fused_grid(ctx, a, half, zchunk);
+        switch (half) {
+#define FBP_LAP(HH)                                                                              \
+    case HH:                                                                                     \
+        hipLaunchKernelGGL(laplacian_fused_kernel<HH>, grid, dim3(256), 0, s, in, out, a, k3,   \
+                           zchunk);                                                              \
+        break;
+            FBP_LAP(0) FBP_LAP(1) FBP_LAP(2) FBP_LAP(3) FBP_LAP(4) FBP_LAP(5) FBP_LAP(6) FBP_LAP(7)
+#undef FBP_LAP
+        }
+        FBP_CHECK_HIP(ctx, hipGetLastError());
+        return FBP_OK;
+    }
+    // Boxes wider than 15: three passes through scratch.
+    FBP_CHECK(ensure_scratch(ctx, 2 * n * sizeof(float)));
+    float* t0 = static_cast<float*>(ctx->scratch);
+    float* t1 = t0 + n;
     hipLaunchKernelGGL(box_pass_kernel<0>, grid_for(n), dim3(256), 0, s, in, t0, a, half);
     hipLaunchKernelGGL(box_pass_kernel<1>, grid_for(n), dim3(256), 0, s, t0, t1, a, half);
     hipLaunchKernelGGL(laplacian_final_kernel, grid_for(n), dim3(256), 0, s, in, t1, out, a, half, k3);
@@ -368,16 +506,19 @@ fbp_status fbp_filter_log(fbp_context ctx, const float* in, float* out, fbp_uvec
         for (int d = -half; d <= half; ++d) m2 += double(d) * d * e[d + half] / se;
         const double s2 = double(std2);
         const float c = float(3.0 * (m2 - s2) / (s2 * s2 * double(k) * k * k));
-        FBP_CHECK(ensure_scratch(ctx, 7 * n * sizeof(float)));
-        float* buf = static_cast<float*>(ctx->scratch);
-        float *gx = buf, *hx = buf + n, *bx = buf + 2 * n;
-        float *gg = buf + 3 * n, *hg = buf + 4 * n, *gh = buf + 5 * n, *bb = buf + 6 * n;
-        hipLaunchKernelGGL(log_pass_x, grid_for(n), dim3(256), 0, s, in, gx, hx, bx, a, t);
-        hipLaunchKernelGGL(log_pass_y, grid_for(n), dim3(256), 0, s, gx, hx, bx, gg, hg, gh, bb, a, t);
         const float a4 = float(1.0 / (double(std2) * std2));
         const float a2 = float(3.0 / double(std2));
-        hipLaunchKernelGGL(log_pass_z, grid_for(n), dim3(256), 0, s, gg, hg, gh, bb, out, a, t, a4,
-                           a2, c);
+        int zchunk = 0;
+        const dim3 grid = fused_grid(ctx, a, half, zchunk);
+        switch (half) {  // k <= kMaxTaps = 15: half <= 7
+#define FBP_LOG(HH)                                                                              \
+    case HH:                                                                                     \
+        hipLaunchKernelGGL(log_fused_kernel<HH>, grid, dim3(256), 0, s, in, out, a, t, a4, a2, c, \
+                           zchunk);                                                              \
+        break;
+            FBP_LOG(0) FBP_LOG(1) FBP_LOG(2) FBP_LOG(3) FBP_LOG(4) FBP_LOG(5) FBP_LOG(6) FBP_LOG(7)
+#undef FBP_LOG
+        }
     } else {
         std::vector<float> w(size_t(k) * k * k);
         fbp_log_kernel_host(k, sigma, w.data());
