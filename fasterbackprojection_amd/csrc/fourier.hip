@@ -121,15 +121,37 @@ void filter_h_lengths(uint32_t nt, float dt, float wl_mean, float& wl_sigma, uin
     pad = (nt_pad - nt) / 2u;
 }
 
+// Transform length for the N-point circular convolution of filter_H.  N
+// itself when it factors into 2, 3, 5, 7, 11 and 13 (rocFFT's radices);
+// otherwise (N = 1106 = 2 * 7 * 79 for 1024 bins) the smallest such M >=
+// N + R: K's Gaussian envelope (sigma = wl_sigma) is below 1e-30 of its peak
+// beyond R = 2 t6 bins, so on the cropped samples the M-point convolution
+// with K's taps |lag| <= R equals the reference's N-point circular one to
+// FP32 rounding, and no Bluestein transform is needed.
+bool smooth_length(uint32_t n) {
+    for (uint32_t p : {2u, 3u, 5u, 7u, 11u, 13u})
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+uint32_t filter_h_transform_length(uint32_t N, uint32_t R) {
+    if (smooth_length(N)) return N;
+    uint32_t M = N + R;
+    while (!smooth_length(M)) ++M;
+    return M;
+}
+
+// Rows of M >= N floats: the padded series in [0, N), zeros after it.
 __global__ void fh_pad_kernel(const float* __restrict__ data, float* __restrict__ real, uint64_t batch,
-                              uint32_t nt, uint32_t N, uint32_t pad, int edge) {
+                              uint32_t nt, uint32_t N, uint32_t M, uint32_t pad, int edge) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= batch * N) return;
-    const size_t b = i / N;
-    const uint32_t t = uint32_t(i % N);
+    if (i >= batch * M) return;
+    const size_t b = i / M;
+    const uint32_t t = uint32_t(i % M);
     const float* row = data + b * nt;
     float v;
-    if (t < pad)
+    if (t >= N)
+        v = 0.f;
+    else if (t < pad)
         v = edge ? row[0] : 0.f;
     else if (t >= pad + nt)
         v = edge ? row[nt - 1] : 0.f;
@@ -147,12 +169,16 @@ __global__ void fh_mul_kernel(float2* __restrict__ H, const float2* __restrict__
     H[i] = make_float2(h.x * k.x - h.y * k.y, h.x * k.y + h.y * k.x);
 }
 
+// scale: N / M when the transform is longer than the padded series (1 when
+// M = N: the reference's unnormalised N-point inverse).
 __global__ void fh_unpad_kernel(const float* __restrict__ real, float* __restrict__ data,
-                                uint64_t batch, uint32_t nt, uint32_t N, uint32_t pad) {
+                                uint64_t batch, uint32_t nt, uint32_t M, uint32_t pad,
+                                float scale) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= batch * nt) return;
     const size_t b = i / nt;
-    data[i] = real[b * N + pad + (i % nt)];
+    const float v = real[b * M + pad + (i % nt)];
+    data[i] = scale == 1.0f ? v : v * scale;
 }
 
 // --------------------------------------------------------------------- LoGFFT
@@ -691,13 +717,16 @@ fbp_status fbp_filter_h(fbp_context ctx, float* data, uint64_t batch, uint32_t n
     hipStream_t s = as_stream(stream);
     uint32_t N, pad;
     filter_h_lengths(nt, delta_t, wl_mean, wl_sigma, N, pad);
-    const uint32_t nh = N / 2 + 1;
+    // Tap radius kept when the transform is longer than N: 2 t6 (pad = t6 - 1).
+    const uint32_t R = std::min<uint32_t>(2 * (pad + 1), N / 2 - 1);
+    const uint32_t M = filter_h_transform_length(N, R);
+    const uint32_t nh = M / 2 + 1;
     FftCache* c = cache_of(ctx);
 
     // Half spectrum of Re(K) (the real input makes Re(IFFT(FFT(H) FFT(K)))
     // equal to IFFT(FFT(H) FFT(Re K))).  K built as FB/Laser.cu:138-172 does.
     char keybuf[160];
-    snprintf(keybuf, sizeof(keybuf), "%u/%a/%a/%a", N, double(delta_t), double(wl_mean),
+    snprintf(keybuf, sizeof(keybuf), "%u/%u/%a/%a/%a", N, M, double(delta_t), double(wl_mean),
              double(wl_sigma));
     if (c->khat_key != keybuf) {
         const float t_max = delta_t * float(N - 1);
@@ -715,6 +744,13 @@ fbp_status fbp_filter_h(fbp_context ctx, float* data, uint64_t batch, uint32_t n
             kre[i] = env[i] * std::cos(phase) / sum;
         }
         for (uint32_t i = 0; i < N; ++i) shifted[i] = kre[(i + N / 2) % N];
+        if (M != N) {
+            // Lags 0..R and -R..-1 of the N-periodic kernel on the M circle.
+            std::vector<float> km(M, 0.0f);
+            for (uint32_t j = 0; j <= R; ++j) km[j] = shifted[j];
+            for (uint32_t j = 1; j <= R; ++j) km[M - j] = shifted[N - j];
+            shifted.swap(km);
+        }
         float* dk = nullptr;
         FBP_CHECK(upload(ctx, shifted, &dk, s));
         if (c->khat) FBP_CHECK_HIP(ctx, hipFree(c->khat));
@@ -722,7 +758,7 @@ fbp_status fbp_filter_h(fbp_context ctx, float* data, uint64_t batch, uint32_t n
         c->khat_key.clear();
         FBP_CHECK_HIP(ctx, hipMalloc((void**)&c->khat, size_t(nh) * sizeof(float2)));
         FftPlan* pk = nullptr;
-        FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_forward, rocfft_precision_single, {N}, 1, &pk));
+        FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_forward, rocfft_precision_single, {M}, 1, &pk));
         FBP_CHECK(run_plan(ctx, pk, dk, c->khat, s));
         FBP_CHECK_HIP(ctx, hipStreamSynchronize(s));
         FBP_CHECK_HIP(ctx, hipFree(dk));
@@ -730,24 +766,24 @@ fbp_status fbp_filter_h(fbp_context ctx, float* data, uint64_t batch, uint32_t n
     }
 
     // Process time series in chunks so the work buffers stay bounded.
-    const uint64_t chunk = std::min<uint64_t>(batch, std::max<uint64_t>(1, (uint64_t(1) << 28) / N));
-    const size_t real_n = size_t(chunk) * N, cplx_n = size_t(chunk) * nh;
+    const uint64_t chunk = std::min<uint64_t>(batch, std::max<uint64_t>(1, (uint64_t(1) << 28) / M));
+    const size_t real_n = size_t(chunk) * M, cplx_n = size_t(chunk) * nh;
     FBP_CHECK(ensure_scratch2(ctx, real_n * sizeof(float) + cplx_n * sizeof(float2) + 256));
     float* real = static_cast<float*>(ctx->scratch2);
     float2* spec = reinterpret_cast<float2*>(real + ((real_n + 63) & ~size_t(63)));
     for (uint64_t b0 = 0; b0 < batch; b0 += chunk) {
         const uint64_t nb = std::min<uint64_t>(chunk, batch - b0);
         float* d = data + size_t(b0) * nt;
-        hipLaunchKernelGGL(fh_pad_kernel, grid1(size_t(nb) * N), dim3(256), 0, s, d, real, nb, nt, N,
-                           pad, border);
+        hipLaunchKernelGGL(fh_pad_kernel, grid1(size_t(nb) * M), dim3(256), 0, s, d, real, nb, nt, N,
+                           M, pad, border);
         FftPlan *pf = nullptr, *pi = nullptr;
-        FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_forward, rocfft_precision_single, {N}, nb, &pf));
-        FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_inverse, rocfft_precision_single, {N}, nb, &pi));
+        FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_forward, rocfft_precision_single, {M}, nb, &pf));
+        FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_inverse, rocfft_precision_single, {M}, nb, &pi));
         FBP_CHECK(run_plan(ctx, pf, real, spec, s));
         hipLaunchKernelGGL(fh_mul_kernel, grid1(size_t(nb) * nh), dim3(256), 0, s, spec, c->khat, nb, nh);
         FBP_CHECK(run_plan(ctx, pi, spec, real, s));
-        hipLaunchKernelGGL(fh_unpad_kernel, grid1(size_t(nb) * nt), dim3(256), 0, s, real, d, nb, nt, N,
-                           pad);
+        hipLaunchKernelGGL(fh_unpad_kernel, grid1(size_t(nb) * nt), dim3(256), 0, s, real, d, nb, nt, M,
+                           pad, M == N ? 1.0f : float(double(N) / double(M)));
         FBP_CHECK_HIP(ctx, hipGetLastError());
     }
     return FBP_OK;
