@@ -128,8 +128,9 @@ constexpr int kRedBlocks = 1024;
 // chunk of z-planes.  Each input plane is read once into LDS with its x/y
 // halo; the x and y passes run in LDS, the z pass over a register ring of
 // the last 2H + 1 planes' y-pass results.  Every output takes the same
-// operations in the same order as the three-pass kernels above (bit-equal
-// results) with one read of the volume (plus halos from L2) and one write.
+// operations in the same order as a separable evaluation in three passes
+// over the whole volume (x, then y, then z), with one read of the volume
+// (plus halos from L2) and one write instead of the intermediate volumes.
 constexpr int kFT = 16;  // tile edge (x and y); 256 threads
 
 template <int H>

@@ -254,8 +254,9 @@ fbp_status fbp_filter_laplacian(fbp_context ctx, const float* in, float* out, fb
                                 int32_t kernel_size, void* stream);
 /* LoG::compute + calculateLaplacianKernel + LoGFilter
  * (FB/PostprocessingFilters.cu:12-32, 102-175; FB/transient_postprocessing.cuh:52-83),
- * clamp-to-edge.  Runs as three 1D passes plus a box term when the kernel is
- * exactly separable (no Gaussian tap thresholded away), else dense. */
+ * clamp-to-edge.  Evaluated in separable form (1D passes plus a box term,
+ * fused into one pass over the volume) when the kernel is exactly separable
+ * (no Gaussian tap thresholded away, odd size), else as the dense k^3 sum. */
 fbp_status fbp_filter_log(fbp_context ctx, const float* in, float* out, fbp_uvec3 size,
                           int32_t kernel_size, float sigma, void* stream);
 /* The k^3 LoG weights exactly as the reference builds them (host array). */
