@@ -71,6 +71,8 @@ def parse():
                     help="skip the LCT leg (128^2 x 1024 padded to 256x256x2048)")
     ap.add_argument("--no-depth", action="store_true",
                     help="skip the depth-plane mode + slice images leg")
+    ap.add_argument("--no-mis", action="store_true",
+                    help="skip the MIS measurement on the main capture")
     ap.add_argument("--no-filters", action="store_true",
                     help="skip the filter leg (Laplacian, LoG, normalise, LoGFFT, filter_H)")
     ap.add_argument("--no-exhaustive", action="store_true",
@@ -176,6 +178,7 @@ def main():
     general_ms = time_bp(3)
     ctx.set_aligned_path(True)
     log(f"general kernel on the same input: {general_ms:.3f} ms")
+    mis = None if args.no_mis else mis_measure(ctx, info, I, laser, vol, stream)
     # End to end from host memory: pinned transient -> device, one step,
     # normalised volume -> pinned host buffer.
     I_host = torch.empty(I.shape, dtype=I.dtype, pin_memory=True)
@@ -274,9 +277,54 @@ def main():
         "lct": lct,
         "depth_planes": depth,
         "filters": filters,
+        "mis": mis,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def mis_measure(ctx, info, I, laser, vol, stream, reps=3):
+    """MIS-sampled backprojection of the main capture (128^2 x 1024 -> 128^3):
+    REFERENCE mode (the reference's per-voxel sampling and weights, L/16
+    samples per voxel) and the tile-shared mode (L/16 samples per 16^3 tile
+    through the general kernel).  Per sample the confocal chain (16 flops)
+    is evaluated; the VALU roofline is priced on that count."""
+    import numpy as np
+    import torch
+
+    from fasterbackprojection_amd.api import MIS_REFERENCE, MIS_TILED
+
+    dev = torch.device("cuda", 0)
+    L = info.num_laser_targets
+    V = info.voxel_resolution.x * info.voxel_resolution.y * info.voxel_resolution.z
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(21)
+    noise = torch.rand(1 << 20, device=dev, generator=gen)
+    n = L // 16
+    res = {"workload": "128^2 x 1024 confocal -> 128^3, L/16 = %d samples per voxel (reference "
+                       "mode) or per 16^3 tile (tile-shared mode)" % n}
+    for name, mode in (("reference_mode", MIS_REFERENCE), ("tile_shared_mode", MIS_TILED)):
+        ctx.backproject_confocal_mis(info, I, laser, noise, n, mode=mode, out=vol)
+        torch.cuda.synchronize()
+        ms = []
+        for _ in range(reps):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            ctx.backproject_confocal_mis(info, I, laser, noise, n, mode=mode, out=vol)
+            b.record(stream)
+            b.synchronize()
+            ms.append(a.elapsed_time(b))
+        t = float(np.median(ms))
+        samples = V * n
+        tf = FLOPS_PER_UPDATE * samples / (t * 1e-3) / 1e12
+        res[name] = {"ms": t, "sample_updates": samples,
+                     "sample_updates_per_s": samples / (t * 1e-3),
+                     "roofline": {"bound": "valu", "achieved": tf, "peak": PEAK_FP32_TFLOPS,
+                                  "unit": "TFLOP/s", "frac": tf / PEAK_FP32_TFLOPS,
+                                  "flops_per_update": FLOPS_PER_UPDATE}}
+    log(f"mis: {res}")
+    return res
 
 
 def filters_leg(ctx, args, reps=5):

@@ -78,59 +78,85 @@ __global__ void mis_kernel(const float* __restrict__ I, const fbp_vec3* __restri
     const float pz = voxel_center(g.hmin[2], g.vs[2], vy);
     const float invL = safe_rcp(static_cast<float>(L));
     float sum = 0.f;
-    for (uint32_t s = 0; s < num_samples; ++s) {
-        uint32_t state = slice * z + v + s;
-        uint32_t l;
-        float pdf, gpdf, pa;
-        if (mis_uniform(noise, noise_size, state) < 0.5f) {
-            const float r1 = mis_uniform(noise, noise_size, state);
-            const float r2 = mis_uniform(noise, noise_size, state);
-            uint32_t k = static_cast<uint32_t>(r1 * static_cast<float>(L));
-            if (k >= L) k = L - 1;
-            l = (r2 < prob[k]) ? k : alias[k];
-            pdf = l > 0 ? cdf[l] - cdf[l - 1] : cdf[l];
-            gpdf = pdf + 1.0f * invL;
-            pa = pdf;
+    // Weight of one sample of l: reference MIS weight, or the one-sample
+    // estimator over the mixture p = (p_alias + 1/L) / 2 (unbiased:
+    // E[f(l) / p(l)] / N = sum_l f(l) / N).
+    auto add = [&](float light, float wd, float pdf, float gpdf, float pa) {
+        if (mode == FBP_MIS_REFERENCE) {
+            const float mis_weight = rn_mul(pdf, safe_rcp(gpdf));
+            sum += rn_mul(rn_mul(mis_weight, light), wd);
         } else {
-            l = static_cast<uint32_t>(mis_uniform(noise, noise_size, state) *
-                                      static_cast<float>(L));
-            if (l >= L) l = L - 1;
-            pdf = 1.0f * invL;
-            pa = l > 0 ? cdf[l] - cdf[l - 1] : cdf[l];
-            gpdf = pdf + pa;
+            const float pmix = 0.5f * (pa + invL);
+            sum += (light * wd) / (pmix * float(num_samples));
         }
+    };
+    // Sample s: draws r0 (strategy), r1 (uniform index k), r2 (alias
+    // acceptance, used by the alias strategy only) from state
+    // slice * z + v + s; l = alias-table pick or k.
+    auto pick = [&](uint32_t s, uint32_t& l, float& pdf, float& gpdf, float& pa) {
+        uint32_t state = slice * z + v + s;
+        const float r0 = mis_uniform(noise, noise_size, state);
+        const float r1 = mis_uniform(noise, noise_size, state);
+        const float r2 = mis_uniform(noise, noise_size, state);
+        uint32_t k = static_cast<uint32_t>(r1 * static_cast<float>(L));
+        if (k >= L) k = L - 1;
+        const bool by_alias = r0 < 0.5f;
+        const float pk = prob[k];
+        const uint32_t ak = alias[k];
+        l = by_alias ? ((r2 < pk) ? k : ak) : k;
+        const float cl = cdf[l], cm = cdf[l > 0 ? l - 1 : 0];
+        const float pl = l > 0 ? cl - cm : cl;
+        pdf = by_alias ? pl : 1.0f * invL;
+        pa = pl;
+        gpdf = by_alias ? pdf + 1.0f * invL : pdf + pa;
+    };
+    auto path = [&](uint32_t l) {
         const float lx = laser[l].x, ly = laser[l].y, lz = laser[l].z;
         float dist = rn_sub(rn_mul(distance3(lx, ly, lz, px, py, pz), 2.0f), t0);
         if (discard == 0)
             dist = rn_add(dist, rn_add(distance3(lx, ly, lz, lpos.x, lpos.y, lpos.z),
                                        distance3(lx, ly, lz, spos.x, spos.y, spos.z)));
-        const float wd = safe_rcp(rn_mul(dist, dist));
-        // Weight of one sample of l: reference MIS weight, or the one-sample
-        // estimator over the mixture p = (p_alias + 1/L) / 2 (unbiased:
-        // E[f(l) / p(l)] / N = sum_l f(l) / N).
-        auto add = [&](float light) {
-            if (mode == FBP_MIS_REFERENCE) {
-                const float mis_weight = rn_mul(pdf, safe_rcp(gpdf));
-                sum += rn_mul(rn_mul(mis_weight, light), wd);
-            } else {
-                const float pmix = 0.5f * (pa + invL);
-                sum += (light * wd) / (pmix * float(num_samples));
-            }
-        };
-        if (scatter) {
-            // Voxel-scattering walk (FB/transient_reconstruction.cuh:235-250).
+        return dist;
+    };
+    if (scatter) {
+        // Voxel-scattering walk (FB/transient_reconstruction.cuh:235-250).
+        for (uint32_t s = 0; s < num_samples; ++s) {
+            uint32_t l;
+            float pdf, gpdf, pa;
+            pick(s, l, pdf, gpdf, pa);
+            const float dist = path(l);
+            const float wd = safe_rcp(rn_mul(dist, dist));
             scatter_walk(dist, ext, dt, T, [&](int bin) {
                 const float light = I[size_t(l) * T + uint32_t(bin)];
-                if (light > kEps) add(light);
+                if (light > kEps) add(light, wd, pdf, gpdf, pa);
             });
-            continue;
         }
-        const float q = rn_div(dist, dt);
-        if (!(q > -1.0f) || !(q < float(T))) continue;
-        const int bin = int(q);
-        const float light = I[size_t(l) * T + bin];
-        if (!(light > kEps)) continue;
-        add(light);
+        volume[vid] = sum;
+        return;
+    }
+    // Samples in groups of kMisGroup: their loads are independent (each has
+    // its own noise state), so the group's chains overlap; every load is
+    // issued (indices clamped in range) and the contributions are added in
+    // sample order.
+    constexpr int kMisGroup = 8;
+    for (uint32_t s0 = 0; s0 < num_samples; s0 += kMisGroup) {
+        uint32_t lq[kMisGroup];
+        float pdf[kMisGroup], gpdf[kMisGroup], pa[kMisGroup], wd[kMisGroup], light[kMisGroup];
+        bool ok[kMisGroup];
+#pragma unroll
+        for (int u = 0; u < kMisGroup; ++u) pick(s0 + u, lq[u], pdf[u], gpdf[u], pa[u]);
+#pragma unroll
+        for (int u = 0; u < kMisGroup; ++u) {
+            const float dist = path(lq[u]);
+            wd[u] = safe_rcp(rn_mul(dist, dist));
+            const float q = rn_div(dist, dt);
+            ok[u] = (s0 + u < num_samples) && (q > -1.0f) && (q < float(T));
+            const int bin = ok[u] ? int(q) : 0;
+            light[u] = I[size_t(ok[u] ? lq[u] : 0) * T + uint32_t(bin)];
+        }
+#pragma unroll
+        for (int u = 0; u < kMisGroup; ++u)
+            if (ok[u] && light[u] > kEps) add(light[u], wd[u], pdf[u], gpdf[u], pa[u]);
     }
     volume[vid] = sum;
 }
