@@ -112,13 +112,21 @@ inline dim3 grid1(size_t n, unsigned bs = 256) { return dim3(unsigned((n + bs - 
 // ------------------------------------------------------------------ filter_H
 
 // Laser::filter_H_cuda's length rules (FB/Laser.cu:128-137), float arithmetic.
-void filter_h_lengths(uint32_t nt, float dt, float wl_mean, float& wl_sigma, uint32_t& nt_pad,
+// Returns false when the rule gives no padding (t6 < 2: six sigmas shorter
+// than one bin, which would make nt_pad < nt) or a padded length beyond
+// 2^24 samples.
+bool filter_h_lengths(uint32_t nt, float dt, float wl_mean, float& wl_sigma, uint32_t& nt_pad,
                       uint32_t& pad) {
     if (std::fabs(wl_sigma - 0.0f) < FLT_EPSILON) wl_sigma = wl_mean / sqrtf(2.0f);
+    const double t6f = std::round(6.0 * double(wl_sigma) / double(dt));
+    nt_pad = pad = 0;
+    if (!(t6f >= 2.0) || t6f > double(1 << 24)) return false;
     int t6 = static_cast<int>(std::round(6 * wl_sigma / dt));
     if (t6 % 2 == 1) t6 += 1;
+    if (t6 < 2 || uint64_t(nt) + 2u * uint64_t(t6 - 1) > (uint64_t(1) << 24)) return false;
     nt_pad = nt + 2u * uint32_t(t6 - 1);
     pad = (nt_pad - nt) / 2u;
+    return true;
 }
 
 // Transform length for the N-point circular convolution of filter_H.  N
@@ -704,7 +712,9 @@ extern "C" {
 
 uint32_t fbp_filter_h_padded_length(uint32_t nt, float delta_t, float wl_mean, float wl_sigma) {
     uint32_t N, pad;
-    filter_h_lengths(nt, delta_t, wl_mean, wl_sigma, N, pad);
+    if (!(delta_t > 0.f) || !(wl_mean > 0.f) ||
+        !filter_h_lengths(nt, delta_t, wl_mean, wl_sigma, N, pad))
+        return 0;
     return N;
 }
 
@@ -713,10 +723,13 @@ fbp_status fbp_filter_h(fbp_context ctx, float* data, uint64_t batch, uint32_t n
     if (!ctx || !data || batch == 0 || nt == 0 || !(delta_t > 0.f) || !(wl_mean > 0.f) ||
         (border != 0 && border != 1))
         return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "filter_h: bad argument");
+    uint32_t N, pad;
+    if (!filter_h_lengths(nt, delta_t, wl_mean, wl_sigma, N, pad))
+        return fail(ctx, FBP_ERR_INVALID_ARGUMENT,
+                    "filter_h: 6 sigma = %g is shorter than two bins of %g (no padding) or the "
+                    "padded length exceeds 2^24", double(6 * wl_sigma), double(delta_t));
     FBP_CHECK(bind(ctx));
     hipStream_t s = as_stream(stream);
-    uint32_t N, pad;
-    filter_h_lengths(nt, delta_t, wl_mean, wl_sigma, N, pad);
     // Tap radius kept when the transform is longer than N: 2 t6 (pad = t6 - 1).
     const uint32_t R = std::min<uint32_t>(2 * (pad + 1), N / 2 - 1);
     const uint32_t M = filter_h_transform_length(N, R);

@@ -278,7 +278,9 @@ fbp_status fbp_minmax(fbp_context ctx, const float* v, uint64_t n, float* host_m
  * the inverse transform is unnormalised (output scaled by nt_pad). */
 fbp_status fbp_filter_h(fbp_context ctx, float* data, uint64_t batch, uint32_t nt, float delta_t,
                         float wl_mean, float wl_sigma, int32_t border, void* stream);
-/* nt_pad the reference uses for these parameters. */
+/* nt_pad the reference uses for these parameters; 0 when the rule gives
+ * no padding (round(6 sigma / delta_t) < 2, i.e. a wavelength shorter than
+ * about 2.4 bins: fbp_filter_h rejects it) or more than 2^24 samples. */
 uint32_t fbp_filter_h_padded_length(uint32_t nt, float delta_t, float wl_mean, float wl_sigma);
 
 /* ------------------------------------------------------------------- LCT */

@@ -108,3 +108,14 @@ def test_entry_points_reject_null_context_without_gpu():
     assert lib.fbp_lct(n, n, n, n, n) == 1
     assert lib.fbp_backproject_confocal_points(n, n, n, n, n, 1, n, n) == 1
     assert lib.fbp_render_confocal(n, n, 1, n, n, 1, 16, 0.01, n, n) == 1
+
+
+def test_filter_h_padded_length_rule_and_rejections():
+    """nt_pad = nt + 2 (t6 - 1) with t6 = round(6 sigma / dt) made even
+    (42 -> 82 extra bins for wl = 10 bins); wavelengths whose six sigmas
+    round below two bins give no padding and are rejected (0)."""
+    lib = _lib.load()
+    dt = 0.0048
+    assert lib.fbp_filter_h_padded_length(1024, dt, 10 * dt, 0.0) == 1106
+    assert lib.fbp_filter_h_padded_length(1024, dt, 0.1 * dt, 0.0) == 0
+    assert lib.fbp_filter_h_padded_length(1024, dt, 10 * dt, -1.0) == 0
