@@ -419,6 +419,8 @@ fbp_status fbp_backproject_depth_confocal(fbp_context ctx, const fbp_reconstruct
         return fail(ctx, FBP_ERR_UNSUPPORTED, "depth confocal: F32 [l][t] transient only");
     if (!num_depths || !info->num_laser_targets || !info->num_time_bins || !(info->time_step > 0.f))
         return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "depth confocal: empty grid");
+    if (uint64_t(info->num_laser_targets) * num_depths > 0xffffffffull)
+        return fail(ctx, FBP_ERR_UNSUPPORTED, "depth confocal: more than 2^32 - 1 outputs");
     FBP_CHECK(bind(ctx));
     const uint32_t L = info->num_laser_targets;
     const uint32_t n = L * num_depths;
