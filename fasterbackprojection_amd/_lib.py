@@ -67,7 +67,7 @@ class MisParams(C.Structure):
 class LctParams(C.Structure):
     _fields_ = [("n", C.c_uint32), ("m", C.c_uint32), ("width", C.c_float),
                 ("range_max", C.c_float), ("snr", C.c_float), ("is_diffuse", C.c_int32),
-                ("is_backprop", C.c_int32), ("z_trim", C.c_uint32)]
+                ("is_backprop", C.c_int32), ("z_trim", C.c_uint32), ("downsample", C.c_uint32)]
 
 
 _P = C.c_void_p

@@ -39,6 +39,49 @@ def _stream() -> int:
     return torch.cuda.current_stream().cuda_stream
 
 
+_TWO_BYTE = (torch.int16, torch.uint16, torch.float16, torch.bfloat16)
+
+
+def _need(t, name: str, dtypes, numel: int | None = None, shape: tuple | None = None,
+          min_numel: int | None = None) -> None:
+    """Raises ValueError unless tensor t has one of dtypes and the given
+    element count / shape.  Runs before any pointer reaches the C ABI, so a
+    short or mistyped buffer is refused on the host instead of being read or
+    written past its end by a kernel."""
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name} must be a torch tensor")
+    dtypes = dtypes if isinstance(dtypes, tuple) else (dtypes,)
+    if t.dtype not in dtypes:
+        raise ValueError(f"{name} must be {' or '.join(map(str, dtypes))}, got {t.dtype}")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} must have shape {tuple(shape)}, got {tuple(t.shape)}")
+    if numel is not None and t.numel() != numel:
+        raise ValueError(f"{name} must hold {numel} elements, got {t.numel()}")
+    if min_numel is not None and t.numel() < min_numel:
+        raise ValueError(f"{name} must hold at least {min_numel} elements, got {t.numel()}")
+
+
+def _need_points(t, name: str, n: int) -> None:
+    """float32 (n, 3) points (fbp_vec3[n])."""
+    _need(t, name, torch.float32)
+    if t.numel() != 3 * n or t.shape[-1] != 3:
+        raise ValueError(f"{name} must be float32 ({n}, 3), got {tuple(t.shape)}")
+
+
+def _need_transient(info, intensity, storage: int, scales, exhaustive: bool) -> None:
+    """The transient holds L*T (confocal) or L*S*T (exhaustive) samples of the
+    storage type; F16/BF16 carry one float32 scale per time series."""
+    rows = info.num_laser_targets * (info.num_sensor_targets if exhaustive else 1)
+    n = rows * info.num_time_bins
+    if storage == STORAGE_F32:
+        _need(intensity, "intensity", torch.float32, numel=n)
+    elif storage in (STORAGE_F16, STORAGE_BF16):
+        _need(intensity, "intensity", _TWO_BYTE, numel=n)
+        _need(scales, "scales", torch.float32, numel=rows)
+    else:
+        raise ValueError(f"unknown storage {storage}")
+
+
 @dataclass
 class Geometry:
     """Host description of a capture, the fields NLosData::toGpu copies into
@@ -156,15 +199,16 @@ class Context:
         r = info.voxel_resolution
         shape = (r.z, r.y, r.x)
         if out is None:
-            out = torch.empty(shape, dtype=torch.float32, device=self._dev())
-        if tuple(out.shape) != shape or out.dtype != torch.float32:
-            raise ValueError(f"volume must be float32 {shape}")
+            return torch.empty(shape, dtype=torch.float32, device=self._dev())
+        _need(out, "out", torch.float32, shape=shape)
         return out
 
     # ------------------------------------------------------ backprojection
     def backproject_confocal(self, info, intensity, laser_targets, *, storage=STORAGE_F32,
                              layout=LAYOUT_TIME_FASTEST, scales=None, out=None):
         """backprojectConfocalVoxel (FB/transient_reconstruction.cuh:93-136)."""
+        _need_transient(info, intensity, storage, scales, exhaustive=False)
+        _need_points(laser_targets, "laser_targets", info.num_laser_targets)
         vol = self._volume(info, out)
         tr = self._transient(intensity, storage, layout, scales)
         self.check(self.lib.fbp_backproject_confocal_voxel(
@@ -174,6 +218,9 @@ class Context:
     def backproject_exhaustive(self, info, intensity, laser_targets, sensor_targets, *,
                                storage=STORAGE_F32, scales=None, out=None):
         """backprojectExhaustiveVoxel (FB/transient_reconstruction.cuh:138-166)."""
+        _need_transient(info, intensity, storage, scales, exhaustive=True)
+        _need_points(laser_targets, "laser_targets", info.num_laser_targets)
+        _need_points(sensor_targets, "sensor_targets", info.num_sensor_targets)
         vol = self._volume(info, out)
         tr = self._transient(intensity, storage, LAYOUT_TIME_FASTEST, scales)
         self.check(self.lib.fbp_backproject_exhaustive_voxel(
@@ -184,10 +231,18 @@ class Context:
     def backproject_depth(self, info, intensity, laser_targets, depths, sensor_targets=None,
                           out=None):
         """backprojectConfocal / backprojectExhaustive (depth planes)."""
+        _need(depths, "depths", torch.float64)
         D = int(depths.numel())
         L = info.num_laser_targets
+        exhaustive = info.capture_system != CONFOCAL
+        _need_transient(info, intensity, STORAGE_F32, None, exhaustive=exhaustive)
+        _need_points(laser_targets, "laser_targets", L)
+        if exhaustive:
+            _need_points(sensor_targets, "sensor_targets", info.num_sensor_targets)
         if out is None:
             out = torch.empty((L, D), dtype=torch.float32, device=self._dev())
+        else:
+            _need(out, "out", torch.float32, shape=(L, D))
         tr = self._transient(intensity, STORAGE_F32, LAYOUT_TIME_FASTEST, None)
         if info.capture_system == CONFOCAL:
             st = self.lib.fbp_backproject_depth_confocal(
@@ -205,6 +260,7 @@ class Context:
         """Per-depth RGBA slice images of TransientImage::save
         (FB/TransientImage.cpp:27-45, FB/Image.h:45-177): activation
         [width*height][D] -> uint8 [D][out_height][out_width][4]."""
+        _need(activation, "activation", torch.float32)
         if activation.dim() != 2 or activation.shape[0] != width * height:
             raise ValueError("activation must be [width*height][num_depths]")
         D = int(activation.shape[1])
@@ -212,6 +268,8 @@ class Context:
         oh = int(out_height or height)
         if out is None:
             out = torch.empty((D, oh, ow, 4), dtype=torch.uint8, device=self._dev())
+        else:
+            _need(out, "out", torch.uint8, shape=(D, oh, ow, 4))
         self.check(self.lib.fbp_depth_slices_rgba(
             self.handle, _ptr(activation.contiguous()), int(width), int(height), D, ow, oh,
             _ptr(out), _stream()))
@@ -220,6 +278,9 @@ class Context:
     def backproject_confocal_mis(self, info, intensity, laser_targets, noise, num_samples,
                                  mode=MIS_REFERENCE, out=None):
         """Laser::reconstructAABBConfocalMIS (FB/Laser.cu:526-611)."""
+        _need_transient(info, intensity, STORAGE_F32, None, exhaustive=False)
+        _need_points(laser_targets, "laser_targets", info.num_laser_targets)
+        _need(noise, "noise", torch.float32, min_numel=1)
         vol = self._volume(info, out)
         tr = self._transient(intensity, STORAGE_F32, LAYOUT_TIME_FASTEST, None)
         p = MisParams(int(num_samples), int(mode), _ptr(noise), int(noise.numel()))
@@ -231,9 +292,14 @@ class Context:
     def backproject_points(self, info, intensity, laser_targets, voxels, out=None):
         """Exact confocal voxel backprojection at the listed voxel indices
         (fbp_backproject_confocal_points); float64 values."""
+        _need_transient(info, intensity, STORAGE_F32, None, exhaustive=False)
+        _need_points(laser_targets, "laser_targets", info.num_laser_targets)
+        _need(voxels, "voxels", (torch.int32, torch.uint32))
         n = int(voxels.numel())
         if out is None:
             out = torch.empty((n,), dtype=torch.float64, device=intensity.device)
+        else:
+            _need(out, "out", torch.float64, numel=n)
         tr = self._transient(intensity, STORAGE_F32, LAYOUT_TIME_FASTEST, None)
         self.check(self.lib.fbp_backproject_confocal_points(
             self.handle, C.byref(info), C.byref(tr), _ptr(laser_targets), _ptr(voxels), n,
@@ -245,6 +311,11 @@ class Context:
         """Global calibration of a refined, normalised LoG volume
         (fbp_learned_calibrate), in place: reference is an independent
         unbiased estimate of the LoG in physical units."""
+        _need_transient(info, intensity, STORAGE_F32, None, exhaustive=False)
+        _need_points(laser_targets, "laser_targets", info.num_laser_targets)
+        r = info.voxel_resolution
+        _need(volume, "volume", torch.float32, shape=(r.z, r.y, r.x))
+        _need(reference, "reference", torch.float32, shape=(r.z, r.y, r.x))
         tr = self._transient(intensity, STORAGE_F32, LAYOUT_TIME_FASTEST, None)
         self.check(self.lib.fbp_learned_calibrate(
             self.handle, C.byref(info), C.byref(tr), _ptr(laser_targets), _ptr(volume),
@@ -253,6 +324,7 @@ class Context:
 
     def quantize(self, intensity: torch.Tensor, storage: int):
         """Per-row scaled F16/BF16 copy of a [rows][T] float32 transient."""
+        _need(intensity, "intensity", torch.float32, min_numel=1)
         rows = intensity.numel() // intensity.shape[-1]
         cols = intensity.shape[-1]
         q = torch.empty(intensity.shape, dtype=torch.int16, device=intensity.device)
@@ -262,31 +334,44 @@ class Context:
         return q, sc
 
     # --------------------------------------------------------------- filters
+    @staticmethod
+    def _filter_io(volume, out):
+        _need(volume, "volume", torch.float32)
+        if volume.dim() != 3:
+            raise ValueError("volume must be [Z][Y][X]")
+        if out is None:
+            return torch.empty_like(volume)
+        _need(out, "out", torch.float32, shape=tuple(volume.shape))
+        return out
+
     def laplacian(self, volume, kernel_size=5, out=None):
+        out = self._filter_io(volume, out)
         Z, Y, X = volume.shape
-        out = torch.empty_like(volume) if out is None else out
         self.check(self.lib.fbp_filter_laplacian(self.handle, _ptr(volume), _ptr(out),
                                                  UVec3(X, Y, Z), kernel_size, _stream()))
         return out
 
     def log(self, volume, kernel_size=5, sigma=1.0, out=None):
+        out = self._filter_io(volume, out)
         Z, Y, X = volume.shape
-        out = torch.empty_like(volume) if out is None else out
         self.check(self.lib.fbp_filter_log(self.handle, _ptr(volume), _ptr(out), UVec3(X, Y, Z),
                                            kernel_size, float(sigma), _stream()))
         return out
 
     def log_fft(self, volume, sigma=1.0):
+        self._filter_io(volume, volume)
         Z, Y, X = volume.shape
         self.check(self.lib.fbp_filter_log_fft(self.handle, _ptr(volume), UVec3(X, Y, Z),
                                                float(sigma), _stream()))
         return volume
 
     def normalize(self, v):
+        _need(v, "v", torch.float32, min_numel=1)
         self.check(self.lib.fbp_normalize_minmax(self.handle, _ptr(v), v.numel(), _stream()))
         return v
 
     def minmax(self, v):
+        _need(v, "v", torch.float32, min_numel=1)
         mn, mx = C.c_float(), C.c_float()
         self.check(self.lib.fbp_minmax(self.handle, _ptr(v), v.numel(), C.byref(mn), C.byref(mx),
                                        _stream()))
@@ -294,6 +379,7 @@ class Context:
 
     def filter_h(self, data, delta_t, wl_mean, wl_sigma=0.0, border="zero"):
         """Laser::filter_H_cuda (FB/Laser.cu:128-240), in place on data[..., T]."""
+        _need(data, "data", torch.float32, min_numel=1)
         nt = data.shape[-1]
         batch = data.numel() // nt
         b = {"zero": 0, "constant": 0, "edge": 1}[border]
@@ -302,22 +388,38 @@ class Context:
         return data
 
     def lct(self, data, width, range_max, snr=0.8, is_diffuse=False, is_backprop=False,
-            z_trim=0, out=None):
-        """python/lct.py cnlos_reconstruction steps 1-4 on data[N][N][M]."""
-        N, N2, M = data.shape
-        if N != N2:
-            raise ValueError("data must be (N, N, M)")
-        p = LctParams(N, M, float(width), float(range_max), float(snr), int(is_diffuse),
-                      int(is_backprop), int(z_trim))
+            z_trim=0, out=None, downsample=0):
+        """python/lct.py cnlos_reconstruction (:95-211) on data[N][N][m]:
+        `downsample` (K) rounds of pairwise time sums, then steps 1-4; returns
+        [m >> K][N][N].  range_max and z_trim are in the undownsampled units,
+        as in lct.py."""
+        _need(data, "data", torch.float32)
+        if data.dim() != 3 or data.shape[0] != data.shape[1]:
+            raise ValueError("data must be (N, N, m)")
+        N, _, m = data.shape
+        K = int(downsample)
+        if not 0 <= K <= 16 or (m >> K) < 2:
+            raise ValueError("downsample must be in [0, 16] with m >> downsample >= 2")
+        M = m >> K
+        p = LctParams(N, m, float(width), float(range_max), float(snr), int(is_diffuse),
+                      int(is_backprop), int(z_trim), K)
         if out is None:
             out = torch.empty((M, N, N), dtype=torch.float32, device=data.device)
+        else:
+            _need(out, "out", torch.float32, shape=(M, N, N))
         self.check(self.lib.fbp_lct(self.handle, _ptr(data), C.byref(p), _ptr(out), _stream()))
         return out
 
     def render_exhaustive(self, lasers, sensors, points, albedo, num_time_bins, delta, out=None):
         L, S = lasers.shape[0], sensors.shape[0]
+        _need_points(lasers, "lasers", L)
+        _need_points(sensors, "sensors", S)
+        _need_points(points, "points", points.shape[0])
+        _need(albedo, "albedo", torch.float32, numel=points.shape[0])
         if out is None:
             out = torch.empty((L, S, num_time_bins), dtype=torch.float32, device=lasers.device)
+        else:
+            _need(out, "out", torch.float32, numel=L * S * num_time_bins)
         self.check(self.lib.fbp_render_exhaustive(self.handle, _ptr(lasers), L, _ptr(sensors), S,
                                                   _ptr(points), _ptr(albedo), points.shape[0],
                                                   num_time_bins, float(delta), _ptr(out),
@@ -329,6 +431,11 @@ class Context:
         """1 - Pearson correlation of the measured rows and the rows rendered
         from the volume's voxels above (polarity 1) or below (polarity -1)
         the threshold (fbp_volume_residual)."""
+        res = info.voxel_resolution
+        _need(volume, "volume", torch.float32, shape=(res.z, res.y, res.x))
+        _need_transient(info, intensity, STORAGE_F32, None, exhaustive=False)
+        _need_points(laser_targets, "laser_targets", info.num_laser_targets)
+        _need(rows, "rows", (torch.int32, torch.uint32))
         r = C.c_double()
         self.check(self.lib.fbp_volume_residual(
             self.handle, C.byref(info), _ptr(volume), _ptr(intensity), _ptr(laser_targets),
@@ -338,8 +445,13 @@ class Context:
 
     def render_confocal(self, scan_points, points, albedo, num_time_bins, delta, out=None):
         L = scan_points.shape[0]
+        _need_points(scan_points, "scan_points", L)
+        _need_points(points, "points", points.shape[0])
+        _need(albedo, "albedo", torch.float32, numel=points.shape[0])
         if out is None:
             out = torch.empty((L, num_time_bins), dtype=torch.float32, device=scan_points.device)
+        else:
+            _need(out, "out", torch.float32, numel=L * num_time_bins)
         self.check(self.lib.fbp_render_confocal(self.handle, _ptr(scan_points), L, _ptr(points),
                                                 _ptr(albedo), points.shape[0], num_time_bins,
                                                 float(delta), _ptr(out), _stream()))
@@ -378,7 +490,15 @@ class Refiner:
         self.ctx.check(self.lib.fbp_refiner_save(self.handle, p.ctypes.data_as(C.c_void_p), p.size))
         return p
 
+    def _need_volume(self, t, name, batch=None):
+        S = self.size
+        shape = (S, S, S) if batch is None else None
+        _need(t, name, torch.float32, shape=shape, numel=None if batch is None else batch * S ** 3)
+
     def forward(self, volume: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        self._need_volume(volume, "volume")
+        if out is not None:
+            self._need_volume(out, "out")
         out = torch.empty_like(volume) if out is None else out
         self.ctx.check(self.lib.fbp_refiner_forward(self.handle, _ptr(volume), _ptr(out), _stream()))
         return out
@@ -386,6 +506,8 @@ class Refiner:
     def train_step(self, inputs: torch.Tensor, targets: torch.Tensor, lr: float,
                    ssim_weight: float = 1.0, want_loss: bool = True) -> float | None:
         batch = inputs.shape[0]
+        self._need_volume(inputs, "inputs", batch)
+        self._need_volume(targets, "targets", batch)
         loss = C.c_float()
         self.ctx.check(self.lib.fbp_refiner_train_step(
             self.handle, _ptr(inputs), _ptr(targets), batch, float(lr), float(ssim_weight),
@@ -393,6 +515,8 @@ class Refiner:
         return loss.value if want_loss else None
 
     def gradient(self, volume: torch.Tensor, target: torch.Tensor, ssim_weight: float = 1.0):
+        self._need_volume(volume, "volume")
+        self._need_volume(target, "target")
         g = np.empty(self.n_params, np.float32)
         loss = C.c_float()
         torch.cuda.synchronize()

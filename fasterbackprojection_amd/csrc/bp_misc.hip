@@ -438,7 +438,7 @@ fbp_status fbp_backproject_depth_confocal(fbp_context ctx, const fbp_reconstruct
                                        activation, ctx->path_flag, ctx->scratch, s));
         skip = ctx->path_flag;
     }
-    hipLaunchKernelGGL(depth_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
+    hipLaunchKernelGGL(depth_kernel, dim3(unsigned((uint64_t(n) + 255) / 256)), dim3(256), 0, s,
                        static_cast<const float*>(tr->data), laser_targets, depths, num_depths, L,
                        info->num_time_bins, info->time_offset, info->time_step,
                        info->discard_first_last_bounces, info->relay_wall_normal,

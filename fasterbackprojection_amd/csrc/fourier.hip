@@ -20,7 +20,14 @@ namespace {
 struct FftPlan {
     rocfft_plan plan = nullptr;
     size_t work = 0;
+    uint64_t last_use = 0;
 };
+
+// Plans kept per context; beyond this the least recently used one is
+// destroyed (after a device synchronisation, so no transform of it is still
+// queued), bounding the device memory plans hold for callers that vary
+// their batch sizes.
+constexpr size_t kMaxPlans = 24;
 
 struct Csr {  // M x M sparse operator on the device
     int* rowptr = nullptr;
@@ -30,6 +37,7 @@ struct Csr {  // M x M sparse operator on the device
 
 struct FftCache {
     std::map<std::string, FftPlan> plans;
+    uint64_t plan_clock = 0;
     void* work = nullptr;
     size_t work_bytes = 0;
     rocfft_execution_info info = nullptr;
@@ -68,6 +76,14 @@ fbp_status get_plan(fbp_context ctx, rocfft_transform_type type, rocfft_precisio
     for (size_t l : lengths) key += "x" + std::to_string(l);
     auto it = c->plans.find(key);
     if (it == c->plans.end()) {
+        if (c->plans.size() >= kMaxPlans) {
+            auto lru = c->plans.begin();
+            for (auto j = c->plans.begin(); j != c->plans.end(); ++j)
+                if (j->second.last_use < lru->second.last_use) lru = j;
+            FBP_CHECK_HIP(ctx, hipDeviceSynchronize());
+            rocfft_plan_destroy(lru->second.plan);
+            c->plans.erase(lru);
+        }
         FftPlan p;
         rocfft_status st = rocfft_plan_create(&p.plan, rocfft_placement_notinplace, type, prec,
                                               lengths.size(), lengths.data(), batch, nullptr);
@@ -76,6 +92,7 @@ fbp_status get_plan(fbp_context ctx, rocfft_transform_type type, rocfft_precisio
         if (st != rocfft_status_success) return fft_fail(ctx, st, "rocfft_plan_get_work_buffer_size");
         it = c->plans.emplace(key, p).first;
     }
+    it->second.last_use = ++c->plan_clock;
     *out = &it->second;
     return FBP_OK;
 }
@@ -111,16 +128,18 @@ inline dim3 grid1(size_t n, unsigned bs = 256) { return dim3(unsigned((n + bs - 
 
 // ------------------------------------------------------------------ filter_H
 
-// Laser::filter_H_cuda's length rules (FB/Laser.cu:128-137), float arithmetic.
-// Returns false when the rule gives no padding (t6 < 2: six sigmas shorter
-// than one bin, which would make nt_pad < nt) or a padded length beyond
-// 2^24 samples.
+// Laser::filter_H_cuda's length rules (FB/Laser.cu:128-137), float arithmetic:
+// t6 = round(6 sigma / dt), bumped to even (so 1 becomes 2: nt_pad = nt + 2,
+// one bin of padding each side).  Returns false when t6 is still below 2
+// after the bump (t6 <= 0: no padding, nt_pad <= nt) or the padded length
+// exceeds 2^24 samples.
 bool filter_h_lengths(uint32_t nt, float dt, float wl_mean, float& wl_sigma, uint32_t& nt_pad,
                       uint32_t& pad) {
     if (std::fabs(wl_sigma - 0.0f) < FLT_EPSILON) wl_sigma = wl_mean / sqrtf(2.0f);
     const double t6f = std::round(6.0 * double(wl_sigma) / double(dt));
     nt_pad = pad = 0;
-    if (!(t6f >= 2.0) || t6f > double(1 << 24)) return false;
+    // NaN, negative and huge ratios end here, before the int conversion.
+    if (!(t6f >= 0.0) || t6f > double(1 << 24)) return false;
     int t6 = static_cast<int>(std::round(6 * wl_sigma / dt));
     if (t6 % 2 == 1) t6 += 1;
     if (t6 < 2 || uint64_t(nt) + 2u * uint64_t(t6 - 1) > (uint64_t(1) << 24)) return false;
@@ -216,17 +235,37 @@ __global__ void logfft_mul_kernel(float2* __restrict__ C, uint32_t X, uint32_t Y
 // [t][y][x]; padded FFT volume [2M][2N][2N] real, half spectrum
 // [2M][2N][N+1] complex.
 
-// Tiled transpose rect[x][y][t] -> d[t][y][x] with z-scaling and trimming.
+// Sum of the 2^K values src[0 .. 2^K) as the balanced tree that K rounds of
+// python/lct.py:103 (rect[:, :, 0::2] + rect[:, :, 1::2]) build: left
+// subtree + right subtree at every level.  The stack holds the pending left
+// subtree of each level (a binary carry over the element index).
+__device__ inline float lct_tree_sum(const float* __restrict__ src, uint32_t K) {
+    if (K == 0) return src[0];
+    float stack[17];
+    const uint32_t n = 1u << K;
+    for (uint32_t i = 0; i < n; ++i) {
+        float v = src[i];
+        uint32_t lv = 0;
+        for (; (i >> lv) & 1u; ++lv) v = stack[lv] + v;
+        stack[lv] = v;
+    }
+    return stack[K];
+}
+
+// Tiled transpose rect[x][y][t_in] -> d[t][y][x] (t_in = t * 2^K .. (t+1) *
+// 2^K - 1 summed by lct_tree_sum) with z-scaling and trimming; M is the
+// downsampled bin count.
 __global__ void lct_permute_kernel(const float* __restrict__ rect, float* __restrict__ d, uint32_t N,
-                                   uint32_t M, uint32_t z_trim, int power4) {
+                                   uint32_t M, uint32_t K, uint32_t z_trim, int power4) {
     __shared__ float tile[32][33];
     const uint32_t y = blockIdx.z;
     const uint32_t t0 = blockIdx.x * 32, x0 = blockIdx.y * 32;
-    // load: threads along t (coalesced in rect)
+    const size_t m_in = size_t(M) << K;
+    // load: threads along t (coalesced in rect for K = 0)
     for (int r = threadIdx.y; r < 32; r += blockDim.y) {
         const uint32_t x = x0 + r, t = t0 + threadIdx.x;
         float v = 0.f;
-        if (x < N && t < M) v = rect[(size_t(x) * N + y) * M + t];
+        if (x < N && t < M) v = lct_tree_sum(rect + (size_t(x) * N + y) * m_in + (size_t(t) << K), K);
         tile[r][threadIdx.x] = v;
     }
     __syncthreads();
@@ -726,7 +765,7 @@ fbp_status fbp_filter_h(fbp_context ctx, float* data, uint64_t batch, uint32_t n
     uint32_t N, pad;
     if (!filter_h_lengths(nt, delta_t, wl_mean, wl_sigma, N, pad))
         return fail(ctx, FBP_ERR_INVALID_ARGUMENT,
-                    "filter_h: 6 sigma = %g is shorter than two bins of %g (no padding) or the "
+                    "filter_h: 6 sigma = %g rounds to no bins of %g (no padding) or the "
                     "padded length exceeds 2^24", double(6 * wl_sigma), double(delta_t));
     FBP_CHECK(bind(ctx));
     hipStream_t s = as_stream(stream);
@@ -784,14 +823,17 @@ fbp_status fbp_filter_h(fbp_context ctx, float* data, uint64_t batch, uint32_t n
     FBP_CHECK(ensure_scratch2(ctx, real_n * sizeof(float) + cplx_n * sizeof(float2) + 256));
     float* real = static_cast<float*>(ctx->scratch2);
     float2* spec = reinterpret_cast<float2*>(real + ((real_n + 63) & ~size_t(63)));
+    // Every chunk, the ragged last one included, runs the chunk-sized plans:
+    // rows nb..chunk-1 of the work buffer then hold stale rows whose
+    // (independent) transforms nobody reads.
+    FftPlan *pf = nullptr, *pi = nullptr;
+    FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_forward, rocfft_precision_single, {M}, chunk, &pf));
+    FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_inverse, rocfft_precision_single, {M}, chunk, &pi));
     for (uint64_t b0 = 0; b0 < batch; b0 += chunk) {
         const uint64_t nb = std::min<uint64_t>(chunk, batch - b0);
         float* d = data + size_t(b0) * nt;
         hipLaunchKernelGGL(fh_pad_kernel, grid1(size_t(nb) * M), dim3(256), 0, s, d, real, nb, nt, N,
                            M, pad, border);
-        FftPlan *pf = nullptr, *pi = nullptr;
-        FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_forward, rocfft_precision_single, {M}, nb, &pf));
-        FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_inverse, rocfft_precision_single, {M}, nb, &pi));
         FBP_CHECK(run_plan(ctx, pf, real, spec, s));
         hipLaunchKernelGGL(fh_mul_kernel, grid1(size_t(nb) * nh), dim3(256), 0, s, spec, c->khat, nb, nh);
         FBP_CHECK(run_plan(ctx, pi, spec, real, s));
@@ -826,10 +868,19 @@ fbp_status fbp_filter_log_fft(fbp_context ctx, float* volume, fbp_uvec3 size, fl
 fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, float* out,
                    void* stream) {
     if (!ctx || !data || !p || !out) return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "lct: null argument");
-    const uint32_t N = p->n, M = p->m;
-    if (N == 0 || M < 2 || (N & (N - 1)) || (M & (M - 1)) || !(p->range_max > 0.f) ||
-        !(p->snr > 0.f))
-        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "lct: N, M must be powers of two, snr > 0");
+    const uint32_t N = p->n, K = p->downsample;
+    if (K > 16 || N == 0 || (N & (N - 1)) || p->m == 0 || (p->m & (p->m - 1)) || (p->m >> K) < 2 ||
+        !(p->range_max > 0.f) || !(p->snr > 0.f))
+        return fail(ctx, FBP_ERR_INVALID_ARGUMENT,
+                    "lct: N, m must be powers of two with m >> downsample >= 2, downsample <= 16, snr > 0");
+    const uint32_t M = p->m >> K;
+    // z_trim = round(z_trim / 2) per downsampling round (python/lct.py:104),
+    // Python's round: halves go to the even neighbour.
+    uint32_t z_trim = p->z_trim;
+    for (uint32_t k = 0; k < K; ++k) {
+        const uint32_t q = z_trim >> 1;
+        z_trim = (z_trim & 1u) ? q + (q & 1u) : q;
+    }
     FBP_CHECK(bind(ctx));
     hipStream_t s = as_stream(stream);
     FftCache* c = cache_of(ctx);
@@ -914,7 +965,7 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
         float* d = reinterpret_cast<float*>(base);
         float* planes = reinterpret_cast<float*>(base + d_bytes);
         float2* spec = reinterpret_cast<float2*>(base + d_bytes + r_bytes);
-        hipLaunchKernelGGL(lct_permute_kernel, tgrid, dim3(32, 8), 0, s, data, d, N, M, p->z_trim,
+        hipLaunchKernelGGL(lct_permute_kernel, tgrid, dim3(32, 8), 0, s, data, d, N, M, K, z_trim,
                            p->is_diffuse);
         hipLaunchKernelGGL(lct_resample_planes_kernel, grid1(plane * M), dim3(256), 0, s, d, planes,
                            c->mtx.rowptr, c->mtx.cols, c->mtx.vals, N, M);
@@ -957,7 +1008,7 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
     float* padv = reinterpret_cast<float*>(base + d_bytes);
     float2* spec = reinterpret_cast<float2*>(base + d_bytes + r_bytes);
 
-    hipLaunchKernelGGL(lct_permute_kernel, tgrid, dim3(32, 8), 0, s, data, d, N, M, p->z_trim,
+    hipLaunchKernelGGL(lct_permute_kernel, tgrid, dim3(32, 8), 0, s, data, d, N, M, K, z_trim,
                        p->is_diffuse);
     hipLaunchKernelGGL(fill_kernel, grid1(pad_n), dim3(256), 0, s, padv, pad_n);
     hipLaunchKernelGGL(lct_resample_kernel, grid1(NN * M), dim3(256), 0, s, d, padv, c->mtx.rowptr,

@@ -12,18 +12,21 @@ LCT::LCT(const ReconstructionInfo& recInfo) {
 
 void LCT::reconstructAABB(const ReconstructionInfo& recInfo, const ReconstructionBuffers& recBuffers,
                           const TransientParameters& transientParams) {
-    const uint32_t N = _size.x, M = recInfo._numTimeBins;
+    const uint32_t N = _size.x, T = recInfo._numTimeBins, K = transientParams._lctDownsample;
     if (size_t(N) * N != recInfo._numLaserTargets || recInfo._captureSystem != CaptureSystem::Confocal)
         throw std::runtime_error("LCT: needs a square confocal scan grid");
+    if (K > 16 || (T >> K) < 2) throw std::runtime_error("LCT: downsampling leaves fewer than 2 bins");
+    const uint32_t M = T >> K;
     fbp_lct_params p{};
     p.n = N;
-    p.m = M;
+    p.m = T;
     p.width = recInfo._relayWallSize.x * 0.5f;
-    p.range_max = float(M) * recInfo._timeStep;  // M * c * bin_resolution
+    p.range_max = float(T) * recInfo._timeStep;  // M * c * bin_resolution before downsampling
     p.snr = transientParams._lctSnr;
     p.is_diffuse = transientParams._lctDiffuse ? 1 : 0;
     p.is_backprop = 0;
     p.z_trim = transientParams._lctZTrim;
+    p.downsample = K;
     const size_t n = size_t(N) * N * M;
     float* out = static_cast<float*>(fbp::deviceAlloc(n * sizeof(float)));
     fbp_status st = fbp_lct(fbp::Runtime::instance().context(), recBuffers._intensity, &p, out, nullptr);

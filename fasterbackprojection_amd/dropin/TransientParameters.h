@@ -27,6 +27,10 @@ public:
     float _lctSnr;
     bool _lctDiffuse;
     uint32_t _lctZTrim;
+    // K: time bins summed pairwise K times before the LCT (python/lct.py:31,
+    // 98-105).  lct.py sets 1 for its 4 ps captures; 0 keeps the capture's
+    // own bins, as _lctZTrim = 0 keeps all of them.
+    uint32_t _lctDownsample;
 
     TransientParameters()
         : _useFourierFilter(true),
@@ -40,5 +44,6 @@ public:
           _saveReconstructedBoundingBox(false),
           _lctSnr(0.8f),
           _lctDiffuse(false),
-          _lctZTrim(0) {}
+          _lctZTrim(0),
+          _lctDownsample(0) {}
 };

@@ -278,28 +278,36 @@ fbp_status fbp_minmax(fbp_context ctx, const float* v, uint64_t n, float* host_m
  * the inverse transform is unnormalised (output scaled by nt_pad). */
 fbp_status fbp_filter_h(fbp_context ctx, float* data, uint64_t batch, uint32_t nt, float delta_t,
                         float wl_mean, float wl_sigma, int32_t border, void* stream);
-/* nt_pad the reference uses for these parameters; 0 when the rule gives
- * no padding (round(6 sigma / delta_t) < 2, i.e. a wavelength shorter than
- * about 2.4 bins: fbp_filter_h rejects it) or more than 2^24 samples. */
+/* nt_pad the reference uses for these parameters (t6 = round(6 sigma /
+ * delta_t) made even, nt_pad = nt + 2 (t6 - 1)); 0 when the rule gives no
+ * padding (t6 = 0, i.e. six sigmas below half a bin: fbp_filter_h rejects
+ * it) or more than 2^24 samples. */
 uint32_t fbp_filter_h_padded_length(uint32_t nt, float delta_t, float wl_mean, float wl_sigma);
 
 /* ------------------------------------------------------------------- LCT */
 
-/* Confocal LCT after python/lct.py:cnlos_reconstruction (:112-211), the
+/* Confocal LCT after python/lct.py:cnlos_reconstruction (:95-211), the
  * reconstruction the reference's LCT class (FB/LCT.h:6-12) leaves empty.
- * data: float[N][N][M] (rect_data order: x, y, t), M and N powers of two.
+ * data: float[N][N][m] (rect_data order: x, y, t), m and N powers of two.
+ * Downsampling (:98-105): K = downsample rounds, each summing adjacent time
+ * bins pairwise (rect[:, :, 0::2] + rect[:, :, 1::2], so the 2^K bins of one
+ * output bin are added as a balanced tree in that order) and halving z_trim
+ * with Python's round (half to even); M = m >> K.  Then the first z_trim
+ * bins are zeroed (:109) and
  * out:  float[M][N][N] = max(Re(mtxi @ crop(ifftn(fftn(pad(mtx @ (data'*z^p))) * invpsf))), 0)
  * with p = 4 if is_diffuse else 2.  width / range_max is the psf slope, as
- * in define_psf(N, M, width / range_max). */
+ * in define_psf(N, M, width / range_max); range_max is the pre-downsampling
+ * m * c * bin_resolution, as lct.py computes it before the loop (:95). */
 typedef struct fbp_lct_params {
     uint32_t n;          /* spatial resolution N                         */
-    uint32_t m;          /* temporal resolution M (after downsampling)   */
+    uint32_t m;          /* time bins of data (before downsampling)      */
     float width;         /* half-width of the scanned wall, metres       */
-    float range_max;     /* M * c * bin_resolution, metres               */
+    float range_max;     /* m * c * bin_resolution, metres               */
     float snr;           /* Wiener SNR (invpsf = conj(F)/(|F|^2 + 1/snr)) */
     int32_t is_diffuse;  /* z^4 instead of z^2 radiometric term          */
     int32_t is_backprop; /* invpsf = conj(F) only                        */
-    uint32_t z_trim;     /* first z_trim bins zeroed                     */
+    uint32_t z_trim;     /* bins zeroed, counted before downsampling     */
+    uint32_t downsample; /* K (lct.py:31), 0..16; m >> K >= 2            */
 } fbp_lct_params;
 fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* params, float* out,
                    void* stream);

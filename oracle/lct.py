@@ -65,11 +65,26 @@ def inverse_filter(N: int, M: int, width: float, range_max: float, snr: float,
     return np.conj(f) / (np.abs(f) ** 2 + 1.0 / snr)
 
 
+def downsample(rect: np.ndarray, z_trim: int, K: int):
+    """python/lct.py:98-105: K rounds of M //= 2, rect = rect[:, :, 0::2] +
+    rect[:, :, 1::2] and z_trim = round(z_trim / 2) (Python's round, halves
+    to even).  Returns (rect, z_trim)."""
+    rect = np.asarray(rect)
+    for _ in range(K):
+        rect = rect[:, :, 0::2] + rect[:, :, 1::2]
+        z_trim = round(z_trim / 2)
+    return rect, z_trim
+
+
 def lct(data: np.ndarray, width: float, range_max: float, snr: float = 0.8,
         is_diffuse: bool = False, is_backprop: bool = False, z_trim: int = 0,
-        invpsf: np.ndarray | None = None, workers: int = -1) -> np.ndarray:
-    """python/lct.py:109-211 on rect_data (N, N, M) (already downsampled).
-    Returns vol (M, N, N) = max(Re(mtxi @ crop(ifftn(fftn(pad) * invpsf))), 0)."""
+        invpsf: np.ndarray | None = None, workers: int = -1, K: int = 0) -> np.ndarray:
+    """python/lct.py:98-211 on rect_data (N, N, m): downsampled K times
+    (downsample(); range_max stays the caller's pre-downsampling value, as
+    lct.py:95 computes it before the loop), then steps 1-4.
+    Returns vol (M, N, N) = max(Re(mtxi @ crop(ifftn(fftn(pad) * invpsf))), 0),
+    M = m / 2^K."""
+    data, z_trim = downsample(np.asarray(data, np.float64), z_trim, K)
     data = np.array(data, np.float64, copy=True)
     N, _, M = data.shape
     data[:, :, :z_trim] = 0

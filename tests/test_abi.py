@@ -5,6 +5,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
+import oracle
 from fasterbackprojection_amd import _lib
 
 
@@ -112,10 +113,15 @@ def test_entry_points_reject_null_context_without_gpu():
 
 def test_filter_h_padded_length_rule_and_rejections():
     """nt_pad = nt + 2 (t6 - 1) with t6 = round(6 sigma / dt) made even
-    (42 -> 82 extra bins for wl = 10 bins); wavelengths whose six sigmas
-    round below two bins give no padding and are rejected (0)."""
+    (42 -> 82 extra bins for wl = 10 bins; 1 -> 2 gives nt + 2); six sigmas
+    that round to 0 bins give no padding and are rejected (0)."""
     lib = _lib.load()
     dt = 0.0048
     assert lib.fbp_filter_h_padded_length(1024, dt, 10 * dt, 0.0) == 1106
+    for ratio in (0.6, 1.0, 1.4):   # round(6 sigma / dt) = 1
+        sigma = ratio * dt / 6
+        assert lib.fbp_filter_h_padded_length(1024, dt, 10 * dt, sigma) == 1026, ratio
+        assert oracle.filter_h_nt_pad(1024, dt, 10 * dt, sigma) == 1026, ratio
+    assert lib.fbp_filter_h_padded_length(1024, dt, 10 * dt, 0.4 * dt / 6) == 0
     assert lib.fbp_filter_h_padded_length(1024, dt, 0.1 * dt, 0.0) == 0
     assert lib.fbp_filter_h_padded_length(1024, dt, 10 * dt, -1.0) == 0

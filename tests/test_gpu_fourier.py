@@ -15,17 +15,22 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-@pytest.mark.parametrize("nt,dt,border", [(256, 0.0096, 0), (256, 0.0096, 1), (1024, 0.0048, 0)])
-def test_filter_h(ctx, nt, dt, border):
+@pytest.mark.parametrize("nt,dt,border,sigma_bins", [(256, 0.0096, 0, 0.0), (256, 0.0096, 1, 0.0),
+                                                     (1024, 0.0048, 0, 0.0),
+                                                     # round(6 sigma / dt) = 1, bumped to 2
+                                                     (256, 0.0096, 1, 1.0 / 6)])
+def test_filter_h(ctx, nt, dt, border, sigma_bins):
     rng = np.random.default_rng(nt + border)
     data = rng.poisson(5.0, size=(7, nt)).astype(np.float32)
-    ref = oracle.filter_h(data, dt, 10 * dt, 0.0, border)
+    sigma = sigma_bins * dt
+    ref = oracle.filter_h(data, dt, 10 * dt, sigma, border)
     t = torch.from_numpy(data).cuda()
-    ctx.filter_h(t, dt, 10 * dt, 0.0, "edge" if border else "zero")
+    ctx.filter_h(t, dt, 10 * dt, sigma, "edge" if border else "zero")
     torch.cuda.synchronize()
     assert rel_l2(t.cpu().numpy(), ref) < 2e-5
     lib = _lib.load()
-    assert lib.fbp_filter_h_padded_length(nt, dt, 10 * dt, 0.0) == oracle.filter_h_nt_pad(nt, dt, 10 * dt)
+    assert lib.fbp_filter_h_padded_length(nt, dt, 10 * dt, sigma) == \
+        oracle.filter_h_nt_pad(nt, dt, 10 * dt, sigma)
 
 
 @pytest.mark.parametrize("shape,sigma", [((32, 32, 32), 1.0), ((20, 24, 16), 1.5)])
@@ -53,6 +58,25 @@ def test_lct_random(ctx, N, M, diffuse, backprop, z_trim):
     out = ctx.lct(torch.from_numpy(data).cuda(), width, rmax, snr, diffuse, backprop, z_trim)
     torch.cuda.synchronize()
     assert rel_l2(out.cpu().numpy(), ref) < 1e-4
+
+
+@pytest.mark.parametrize("K,z_trim,M", [(1, 7, 128), (2, 9, 128), (1, 0, 2048)])
+def test_lct_downsample(ctx, K, z_trim, M):
+    """Downsampling by K rounds of pairwise time sums (python/lct.py:98-105)
+    before the LCT: range_max stays the undownsampled value and z_trim is
+    halved with Python's round (7 -> 4, 9 -> 4 -> 2).  The last case ends on
+    M = 1024 bins, the specialised z-pass."""
+    N = 16 if M < 2048 else 8
+    rng = np.random.default_rng(K * 100 + z_trim)
+    data = rng.random((N, N, M)).astype(np.float32)
+    width, rmax, snr = 0.5, M * 3e8 * 8e-12, 0.8
+    ref = olct.lct(data, width, rmax, snr, z_trim=z_trim, K=K)
+    out = ctx.lct(torch.from_numpy(data).cuda(), width, rmax, snr, z_trim=z_trim, downsample=K)
+    torch.cuda.synchronize()
+    assert out.shape == (M >> K, N, N)
+    assert rel_l2(out.cpu().numpy(), ref) < 1e-4
+    with pytest.raises(ValueError):
+        ctx.lct(torch.from_numpy(data).cuda(), width, rmax, snr, downsample=20)
 
 
 def test_lct_point_scatterer(ctx):

@@ -123,3 +123,42 @@ def test_log_fft_known_answers():
     f = k / N
     lam = -4 * np.pi ** 2 * f ** 2 * np.exp(-2 * np.pi ** 2 * s ** 2 * f ** 2)
     np.testing.assert_allclose(oracle.log_fft(v, s), v * lam * v.size, rtol=1e-9, atol=1e-9)
+
+
+def test_lct_downsample_known_answers():
+    """python/lct.py:98-105: each round sums bins (2t, 2t+1) and halves
+    z_trim with Python's round, which takes halves to the even neighbour
+    (600 -> 300 -> 150 -> 75 -> 38; 5 -> 2; 7 -> 4; 3 -> 2 -> 1)."""
+    rect = np.arange(2 * 3 * 8, dtype=np.float64).reshape(2, 3, 8)
+    r1, z1 = olct.downsample(rect, 5, 1)
+    np.testing.assert_array_equal(r1, rect[:, :, 0::2] + rect[:, :, 1::2])
+    assert r1[1, 2, 3] == rect[1, 2, 6] + rect[1, 2, 7] and z1 == 2
+    r2, z2 = olct.downsample(rect, 3, 2)
+    np.testing.assert_array_equal(r2, rect.reshape(2, 3, 2, 4).sum(-1))
+    assert z2 == 1
+    assert olct.downsample(rect, 7, 1)[1] == 4
+    assert olct.downsample(np.zeros((1, 1, 64)), 600, 4)[1] == 38
+    assert olct.downsample(rect, 600, 0)[1] == 600 and olct.downsample(rect, 600, 0)[0] is rect
+
+
+def test_lct_downsample_point_scatterer_keeps_its_depth():
+    """A point scatterer rendered at 8 ps and downsampled once reconstructs
+    at its (x, y) pixel and at the depth index of the 16 ps grid,
+    z / (range / 2) * (M/2 - 1) (N = 64, as in the undownsampled case)."""
+    N, M = 64, 512
+    width, bin_res, c = 0.5, 8e-12, 3e8
+    rng_max = M * c * bin_res
+    xs = np.linspace(-width, width, N)
+    x0, y0, z0 = xs[40], xs[22], 0.7 * rng_max / 2
+    data = np.zeros((N, N, M))
+    for i in range(N):
+        for j in range(N):
+            r = np.sqrt((xs[i] - x0) ** 2 + (xs[j] - y0) ** 2 + z0 ** 2)
+            t = int(round(2 * r / (c * bin_res)))
+            if t < M:
+                data[i, j, t] = 1.0
+    vol = olct.lct(data, width, rng_max, snr=8.0, K=1)
+    assert vol.shape == (M // 2, N, N)
+    t, y, x = np.unravel_index(np.argmax(vol), vol.shape)
+    assert abs(int(x) - 40) <= 1 and abs(int(y) - 22) <= 1
+    assert abs(t - z0 / (rng_max / 2) * (M // 2 - 1)) <= 2
