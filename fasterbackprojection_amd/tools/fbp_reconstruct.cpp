@@ -5,7 +5,7 @@
 //   fbp_reconstruct info <capture.nlos>
 //   fbp_reconstruct aabb <capture.nlos> <out/> <res> <filter 0..3> <fourier 0|1> <discard 0|1>
 //   fbp_reconstruct depths <capture.nlos> <out/> <num_depths> <fourier 0|1> <discard 0|1>
-//   fbp_reconstruct lct <capture.nlos> <out/> <snr> <z_trim>
+//   fbp_reconstruct lct <capture.nlos> <out/> <snr> <z_trim> [downsample K]
 //   fbp_reconstruct mis <capture.nlos> <out/> <res> <discard 0|1>
 #include <cstdio>
 #include <cstdlib>
@@ -74,6 +74,7 @@ int main(int argc, char** argv) {
         if (mode == "lct" && argc >= 6) {
             p._lctSnr = float(std::atof(argv[4]));
             p._lctZTrim = uint32_t(std::atoi(argv[5]));
+            if (argc >= 7) p._lctDownsample = uint32_t(std::atoi(argv[6]));
             ReconstructionInfo info;
             ReconstructionBuffers buf;
             data.toGpu(info, buf, p);

@@ -48,6 +48,10 @@ def load() -> C.CDLL:
         _lib.oracle_confocal_voxel.argtypes = [P, P, C.c_int, P, P]
         _lib.oracle_exhaustive_voxel.argtypes = [P, P, P, P, P]
         _lib.oracle_depth.argtypes = [P, P, P, P, P, C.c_uint32, C.c_int, P]
+        _lib.oracle_confocal_bins.argtypes = [P, P, C.c_uint32, P, P]
+        _lib.oracle_exhaustive_bins.argtypes = [P, P, P, C.c_uint32, C.c_uint32, P, P]
+        _lib.oracle_depth_bins.argtypes = [P, P, P, P, C.c_uint32, C.c_int, C.c_uint32,
+                                           C.c_uint32, P]
         _lib.oracle_mis.argtypes = [P, P, P, P, P, P, P, C.c_uint32, C.c_uint32, C.c_int,
                                     C.c_int, P]
         _lib.oracle_confocal_voxel_scatter.argtypes = [P, P, C.c_int, P, P]
@@ -102,6 +106,39 @@ def confocal_bin(info, laser, l, vx, vy, z) -> int:
 def confocal_distf(info, laser, l, vx, vy, z) -> float:
     laser = _f32(laser)
     return float(load().oracle_confocal_distf(C.byref(info), _p(laser), l, vx, vy, z))
+
+
+def confocal_bins(info, laser, l):
+    """Bin (-1 outside [0, T)) and FP32 dist of every voxel for scan point l
+    (FB/transient_reconstruction.cuh:102-130): int64 and float32 [Z][Y][X]."""
+    laser = _f32(laser)
+    bins = np.empty(_res(info), np.int64)
+    dist = np.empty(_res(info), np.float32)
+    load().oracle_confocal_bins(C.byref(info), _p(laser), int(l), _p(bins), _p(dist))
+    return bins, dist
+
+
+def exhaustive_bins(info, laser, sensor, l, s):
+    """Bin and FP32 dist of every voxel for the pair (l, s)
+    (FB/transient_reconstruction.cuh:150-161)."""
+    laser, sensor = _f32(laser), _f32(sensor)
+    bins = np.empty(_res(info), np.int64)
+    dist = np.empty(_res(info), np.float32)
+    load().oracle_exhaustive_bins(C.byref(info), _p(laser), _p(sensor), int(l), int(s), _p(bins),
+                                  _p(dist))
+    return bins, dist
+
+
+def depth_bins(info, laser, depths, l, sensor=None, s=0, exhaustive=False):
+    """Depth-plane bins [c][d] of one lit scan point l (or pair (l, s))
+    (FB/transient_reconstruction.cuh:41-91); -1 outside [0, T)."""
+    laser = _f32(laser)
+    sensor = laser if sensor is None else _f32(sensor)
+    depths = np.ascontiguousarray(depths, np.float64)
+    bins = np.empty((info.num_laser_targets, len(depths)), np.int64)
+    load().oracle_depth_bins(C.byref(info), _p(laser), _p(sensor), _p(depths), len(depths),
+                             int(exhaustive), int(l), int(s), _p(bins))
+    return bins
 
 
 def exhaustive_voxel(info, I, laser, sensor) -> np.ndarray:

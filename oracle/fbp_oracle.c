@@ -248,21 +248,81 @@ void oracle_exhaustive_voxel(const RecInfo* in, const float* I, const vec3* lase
     }
 }
 
+/* Bins (-1 when outside [0, T)) and FP32 dist of every voxel for one scan
+ * point (confocal) or one (laser, sensor) pair (exhaustive); [Z][Y][X]. */
+void oracle_confocal_bins(const RecInfo* in, const vec3* laser, uint32_t l, int64_t* bins,
+                          float* dist) {
+    const uint32_t X = in->voxel_resolution.x, Y = in->voxel_resolution.y,
+                   Z = in->voxel_resolution.z;
+    for (int64_t v = 0; v < (int64_t)X * Y * Z; ++v) {
+        uint32_t vx = (uint32_t)(v % X), vy = (uint32_t)((v / X) % Y), z = (uint32_t)(v / ((int64_t)X * Y));
+        float d = confocal_dist(in, laser[l], voxel_pos(in, vx, vy, z));
+        dist[v] = d;
+        bins[v] = bin_of(d, in->time_step, in->num_time_bins);
+    }
+}
+
+void oracle_exhaustive_bins(const RecInfo* in, const vec3* laser, const vec3* sensor, uint32_t l,
+                            uint32_t s, int64_t* bins, float* dist) {
+    const uint32_t X = in->voxel_resolution.x, Y = in->voxel_resolution.y,
+                   Z = in->voxel_resolution.z;
+    for (int64_t v = 0; v < (int64_t)X * Y * Z; ++v) {
+        uint32_t vx = (uint32_t)(v % X), vy = (uint32_t)((v / X) % Y), z = (uint32_t)(v / ((int64_t)X * Y));
+        float d = exhaustive_dist(in, laser[l], sensor[s], voxel_pos(in, vx, vy, z));
+        dist[v] = d;
+        bins[v] = bin_of(d, in->time_step, in->num_time_bins);
+    }
+}
+
 /* ------------------------------------------------------------ depth planes
  * backprojectConfocal / backprojectExhaustive (FB/transient_reconstruction.cuh:41-91).
  * act: double[L][D]. */
+static vec3 depth_point(const RecInfo* in, const vec3* laser, uint32_t c, double depth_d) {
+    const vec3 nrm = in->relay_wall_normal;
+    float depth = (float)depth_d;
+    vec3 cp;
+    cp.x = laser[c].x + nrm.x * depth;
+    cp.y = laser[c].y + nrm.y * depth;
+    cp.z = laser[c].z + nrm.z * depth;
+    return cp;
+}
+
+/* distance(lPos, checkPos) + distance(checkPos, sPos) - t0 (+ offsets),
+ * FB/transient_reconstruction.cuh:25-27, 82-84. */
+static float depth_exhaustive_dist(const RecInfo* in, vec3 l, vec3 s, vec3 cp) {
+    float a = glm_distance(l, cp);
+    float b2 = glm_distance(cp, s);
+    float t = a + b2;
+    t = t - in->time_offset;
+    if (in->discard_first_last_bounces == 0) {
+        float o = glm_distance(l, in->laser_position) + glm_distance(s, in->sensor_position);
+        t = t + o;
+    }
+    return t;
+}
+
+/* Bins (-1 outside [0, T)) of the depth-plane mode for one lit scan point l
+ * (confocal) or one lit pair (l, s) (exhaustive): bins[c][d]. */
+void oracle_depth_bins(const RecInfo* in, const vec3* laser, const vec3* sensor,
+                       const double* depths, uint32_t D, int exhaustive, uint32_t l, uint32_t s,
+                       int64_t* bins) {
+    const uint32_t L = in->num_laser_targets;
+    for (int64_t idx = 0; idx < (int64_t)L * D; ++idx) {
+        uint32_t c = (uint32_t)(idx / D), d = (uint32_t)(idx % D);
+        vec3 cp = depth_point(in, laser, c, depths[d]);
+        float t = exhaustive ? depth_exhaustive_dist(in, laser[l], sensor[s], cp)
+                             : confocal_dist(in, laser[l], cp);
+        bins[idx] = bin_of(t, in->time_step, in->num_time_bins);
+    }
+}
+
 void oracle_depth(const RecInfo* in, const float* I, const vec3* laser, const vec3* sensor,
                   const double* depths, uint32_t D, int exhaustive, double* act) {
     const uint32_t L = in->num_laser_targets, S = in->num_sensor_targets, T = in->num_time_bins;
-    const vec3 nrm = in->relay_wall_normal;
 #pragma omp parallel for schedule(dynamic, 4)
     for (int64_t idx = 0; idx < (int64_t)L * D; ++idx) {
         uint32_t c = (uint32_t)(idx / D), d = (uint32_t)(idx % D);
-        float depth = (float)depths[d];
-        vec3 cp;
-        cp.x = laser[c].x + nrm.x * depth;
-        cp.y = laser[c].y + nrm.y * depth;
-        cp.z = laser[c].z + nrm.z * depth;
+        vec3 cp = depth_point(in, laser, c, depths[d]);
         double acc = 0.0;
         if (!exhaustive) {
             for (uint32_t l = 0; l < L; ++l) {
@@ -275,16 +335,7 @@ void oracle_depth(const RecInfo* in, const float* I, const vec3* laser, const ve
         } else {
             for (uint32_t s = 0; s < S; ++s)
                 for (uint32_t l = 0; l < L; ++l) {
-                    /* distance(lPos, checkPos) + distance(checkPos, sPos) */
-                    float a = glm_distance(laser[l], cp);
-                    float b2 = glm_distance(cp, sensor[s]);
-                    float t = a + b2;
-                    t = t - in->time_offset;
-                    if (in->discard_first_last_bounces == 0) {
-                        float o = glm_distance(laser[l], in->laser_position) +
-                                  glm_distance(sensor[s], in->sensor_position);
-                        t = t + o;
-                    }
+                    float t = depth_exhaustive_dist(in, laser[l], sensor[s], cp);
                     int64_t b = bin_of(t, in->time_step, T);
                     if (b < 0) continue;
                     float light = I[((size_t)l * S + s) * T + (size_t)b];

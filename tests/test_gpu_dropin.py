@@ -9,9 +9,10 @@ import pytest
 import oracle
 from oracle import lct as olct
 from fasterbackprojection_amd import nlos_io
-from fasterbackprojection_amd.api import make_info
+from fasterbackprojection_amd.api import Geometry, make_info
 from fasterbackprojection_amd.build import CLI
-from fasterbackprojection_amd.scene import small_confocal
+from fasterbackprojection_amd.scene import (poisson, random_points, render_exhaustive,
+                                            small_confocal, wall_grid)
 from pngio import read_rgba
 
 pytestmark = pytest.mark.gpu
@@ -54,6 +55,41 @@ def test_reconstruct_shape_aabb(capture, tmp_path, filt, fourier):
     assert rel_l2(got, ref) < 2e-4
 
 
+@pytest.fixture(scope="module")
+def exhaustive_capture(tmp_path_factory):
+    """Exhaustive capture: 2x2 lasers and 4x4 sensors on a 1 m wall, 256 bins
+    of 9.6 mm, 40 points, Poisson at peak 1000; stored as [1][L][1][S][T]."""
+    rng = np.random.default_rng(9)
+    lasers, sensors = wall_grid(2, 1.0), wall_grid(4, 1.0)
+    T, delta = 256, 0.0096
+    g = Geometry(laser_targets=lasers, sensor_targets=sensors, num_time_bins=T, time_step=delta,
+                 hidden_min=(-0.4, 0.4, -0.4), hidden_max=(0.4, 1.2, 0.4),
+                 voxel_resolution=(8, 8, 8), confocal=False, wall_size=(1.0, 1.0))
+    tr = poisson(render_exhaustive(random_points(rng, 40), lasers, sensors, T, delta), 1000.0, rng)
+    p = tmp_path_factory.mktemp("exh") / "exh.nlos"
+    nlos_io.write(p, nlos_io.from_geometry(g, tr))
+    return g, tr, p
+
+
+@pytest.mark.parametrize("filt,fourier", [(1, 0), (1, 1), (0, 0)])
+def test_reconstruct_shape_aabb_exhaustive(exhaustive_capture, tmp_path, filt, fourier):
+    """Laser::reconstructShape on an exhaustive capture (FB/Laser.cu:341-363,
+    500-516): filter_H over every (laser, sensor) series, exhaustive voxel
+    backprojection, Laplacian, min-max normalisation."""
+    g, I, path = exhaustive_capture
+    run("aabb", path, tmp_path, 8, filt, fourier, 1, 1)
+    got = np.fromfile(tmp_path / "aabb.cube", np.float32).reshape(8, 8, 8)
+    if fourier:
+        dt = np.float32(g.time_step)
+        I = oracle.filter_h(I.reshape(-1, I.shape[-1]), dt, dt * np.float32(10)).astype(
+            np.float32).reshape(I.shape)
+    vol = oracle.exhaustive_voxel(make_info(g), I, g.laser_targets, g.sensor_targets)
+    if filt == 1:
+        vol = oracle.laplacian(vol.astype(np.float32), 5)
+    ref = oracle.normalize(vol)
+    assert rel_l2(got, ref) < 2e-4
+
+
 def test_reconstruct_shape_depths(capture, tmp_path):
     cap, path = capture
     run("depths", path, tmp_path, 50, 0, 1)
@@ -74,14 +110,18 @@ def test_reconstruct_shape_depths(capture, tmp_path):
     assert not (tmp_path / "transient_50.png").exists()
 
 
-def test_lct_class(capture, tmp_path):
+@pytest.mark.parametrize("K", [0, 1, 2])
+def test_lct_class(capture, tmp_path, K):
+    """LCT::reconstructAABB with TransientParameters::_lctDownsample = K:
+    range_max from the capture's own bins, z_trim halved per round."""
     cap, path = capture
-    run("lct", path, tmp_path, 0.8, 3)
-    got = np.fromfile(tmp_path / "lct.raw", np.float32).reshape(256, 16, 16)
+    run("lct", path, tmp_path, 0.8, 3, K)
+    M = 256 >> K
+    got = np.fromfile(tmp_path / "lct.raw", np.float32).reshape(M, 16, 16)
     g = cap.geometry
     rect = cap.transient.reshape(16, 16, 256)
     ref = olct.lct(rect, float(np.float32(g.wall_size[0]) * np.float32(0.5)),
-                   float(np.float32(256) * np.float32(g.time_step)), 0.8, z_trim=3)
+                   float(np.float32(256) * np.float32(g.time_step)), 0.8, z_trim=3, K=K)
     assert rel_l2(got, ref) < 1e-4
 
 

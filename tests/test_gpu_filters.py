@@ -41,6 +41,29 @@ def test_log(ctx, shape, k, sigma):
     assert rel_l2(out.cpu().numpy(), ref) < 1e-4
 
 
+def test_filters_at_the_timed_sizes(ctx):
+    """The sizes the benchmark times: the separable LoG (k = 5, sigma = 1)
+    on a 128^3 volume, whose fused kernel walks several z-chunks, and the
+    Laplacian (k = 5) on 64^3, each against the FP64 oracle."""
+    rng = np.random.default_rng(4)
+    v = (rng.random((128, 128, 128)) ** 3).astype(np.float32)
+    w = np.empty(125, np.float32)
+    from fasterbackprojection_amd import _lib
+    import ctypes
+    _lib.load().fbp_log_kernel_host(5, 1.0, w.ctypes.data_as(ctypes.c_void_p))
+    ref = oracle.log_filter(v, 5, 1.0, weights=w.astype(np.float64).reshape(5, 5, 5))
+    out = ctx.log(torch.from_numpy(v).cuda(), 5, 1.0).cpu().numpy()
+    assert rel_l2(out, ref) < 1e-4
+    # a plane in each z-chunk boundary region and the borders, voxel by voxel
+    for z in (0, 1, 2, 31, 32, 33, 63, 64, 95, 96, 126, 127):
+        np.testing.assert_allclose(out[z], ref[z], rtol=0, atol=2e-5 * np.abs(ref).max())
+    v64 = rng.random((64, 64, 64)).astype(np.float32)
+    ref64 = oracle.laplacian(v64, 5)
+    out64 = ctx.laplacian(torch.from_numpy(v64).cuda(), 5).cpu().numpy()
+    assert rel_l2(out64, ref64) < 1e-5
+    np.testing.assert_allclose(out64, ref64, rtol=0, atol=1e-4 * np.abs(ref64).max())
+
+
 def test_normalize(ctx):
     rng = np.random.default_rng(2)
     v = (rng.random((20, 30, 40)) * 7 - 3).astype(np.float32)

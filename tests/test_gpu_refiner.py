@@ -24,8 +24,9 @@ def _params(seed, scale=0.15):
     return p
 
 
-@pytest.mark.parametrize("S", [16, 32])
+@pytest.mark.parametrize("S", [16, 32, 64])
 def test_forward_matches_oracle(ctx, S):
+    """S = 64 is the learned form's volume (the size the benchmark times)."""
     p = _params(S)
     ref = Refiner(ctx, S)
     ref.load(p)
