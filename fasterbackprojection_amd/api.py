@@ -228,6 +228,26 @@ class Context:
             _ptr(vol), _stream()))
         return vol
 
+    def backproject_exhaustive_noalloc(self, info, intensity, laser_targets, sensor_targets,
+                                       out, *, storage=STORAGE_F32, scales=None):
+        """fbp_backproject_exhaustive_voxel_noalloc: the exhaustive
+        backprojection with its partial sums in the library's fixed per-GPU
+        array; `out` must be given (nothing is allocated)."""
+        _need_transient(info, intensity, storage, scales, exhaustive=True)
+        _need_points(laser_targets, "laser_targets", info.num_laser_targets)
+        _need_points(sensor_targets, "sensor_targets", info.num_sensor_targets)
+        r = info.voxel_resolution
+        _need(out, "out", torch.float32, shape=(r.z, r.y, r.x))
+        tr = self._transient(intensity, storage, LAYOUT_TIME_FASTEST, scales)
+        self.check(self.lib.fbp_backproject_exhaustive_voxel_noalloc(
+            self.handle, C.byref(info), C.byref(tr), _ptr(laser_targets), _ptr(sensor_targets),
+            _ptr(out), _stream()))
+        return out
+
+    def scratch_bytes(self) -> int:
+        """Device bytes the context's scratch arenas hold (fbp_context_scratch_bytes)."""
+        return int(self.lib.fbp_context_scratch_bytes(self.handle))
+
     def backproject_depth(self, info, intensity, laser_targets, depths, sensor_targets=None,
                           out=None):
         """backprojectConfocal / backprojectExhaustive (depth planes)."""

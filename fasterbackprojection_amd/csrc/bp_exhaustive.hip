@@ -23,9 +23,18 @@
 // chain (bin and weight) for those voxels into per-thread LDS correction
 // slots, behind one wave-uniform test.  Pairs whose tile may come within a
 // few millimetres of the wall or of zero distance take the whole exact chain.
+#include <mutex>
+
 #include "bp_common.h"
 
 namespace fbp {
+
+// Partial sums of fbp_backproject_exhaustive_voxel_noalloc: one array per
+// GPU (each device holds its own copy of this module's variable until the
+// process ends), shared by every no-alloc call on that device.  Host code
+// never uses the name itself for device memory: its per-device address comes
+// from hipGetSymbolAddress (noalloc_scratch_address below).
+static __device__ __align__(256) float scratch[FBP_NOALLOC_SCRATCH_BYTES / sizeof(float)];
 
 struct ExhaustiveArgs {
     const void* data;
@@ -349,16 +358,39 @@ __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs
                     acc[(i * 2 + j) * 4 + m] + corrl[((i * 2 + j) * 4 + m) * 256];
 }
 
-}  // namespace fbp
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kMaxDevices] = {};
+}  // namespace
 
-using namespace fbp;
+// Device address of `scratch` on the current device (ctx->device), looked up
+// once per device and kept; called by fbp_context_create with the device
+// current.  The module variable exists separately on every device, so the
+// address is per device.
+fbp_status noalloc_scratch_address(fbp_context ctx, void** out) {
+    *out = nullptr;
+    if (ctx->device < 0 || ctx->device >= kMaxDevices)
+        return fail(ctx, FBP_ERR_UNSUPPORTED, "no-alloc scratch: device %d beyond %d", ctx->device,
+                    kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[ctx->device]) {
+        void* p = nullptr;
+        FBP_CHECK_HIP(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_noalloc_ptr[ctx->device] = p;
+    }
+    *out = g_noalloc_ptr[ctx->device];
+    return FBP_OK;
+}
 
-extern "C" fbp_status fbp_backproject_exhaustive_voxel(fbp_context ctx,
-                                                       const fbp_reconstruction_info* info,
-                                                       const fbp_transient* tr,
-                                                       const fbp_vec3* laser_targets,
-                                                       const fbp_vec3* sensor_targets,
-                                                       float* volume, void* stream) {
+// Shared body of the two exhaustive entry points.  fixed_partial == nullptr:
+// partial sums in the context's scratch arena (grown on demand).  Otherwise
+// they go to fixed_partial (fixed_floats floats) and the number of pair
+// chunks is capped so they fit; nothing is allocated.
+static fbp_status exhaustive_voxel_impl(fbp_context ctx, const fbp_reconstruction_info* info,
+                                        const fbp_transient* tr, const fbp_vec3* laser_targets,
+                                        const fbp_vec3* sensor_targets, float* volume,
+                                        void* stream, float* fixed_partial, size_t fixed_floats) {
     if (!ctx || !info || !tr || !tr->data || !laser_targets || !sensor_targets || !volume)
         return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "exhaustive: null argument");
     if (info->num_laser_targets == 0 || info->num_sensor_targets == 0 ||
@@ -411,12 +443,25 @@ extern "C" fbp_status fbp_backproject_exhaustive_voxel(fbp_context ctx,
     uint64_t nc = (target + a.g.num_tiles - 1) / a.g.num_tiles;
     const uint64_t cap = (a.num_pairs + 63) / 64;
     if (nc > cap) nc = cap;
+    const size_t chunk_floats = size_t(a.g.num_tiles) * a.g.slots * 64;
+    if (fixed_partial) {
+        const uint64_t fit = fixed_floats / chunk_floats;
+        if (fit == 0)
+            return fail(ctx, FBP_ERR_UNSUPPORTED,
+                        "exhaustive no-alloc: %zu voxels (tile-padded) need %zu bytes of partial "
+                        "sums, beyond the %zu-byte scratch",
+                        chunk_floats, chunk_floats * sizeof(float), fixed_floats * sizeof(float));
+        if (nc > fit) nc = fit;
+    }
     if (nc < 1) nc = 1;
     a.chunk_len = (a.num_pairs + nc - 1) / nc;
     const uint64_t chunks = (a.num_pairs + a.chunk_len - 1) / a.chunk_len;
-    const size_t partial_bytes = size_t(chunks) * a.g.num_tiles * a.g.slots * 64 * sizeof(float);
-    FBP_CHECK(ensure_scratch(ctx, partial_bytes));
-    a.partial = static_cast<float*>(ctx->scratch);
+    if (fixed_partial) {
+        a.partial = fixed_partial;
+    } else {
+        FBP_CHECK(ensure_scratch(ctx, size_t(chunks) * chunk_floats * sizeof(float)));
+        a.partial = static_cast<float*>(ctx->scratch);
+    }
     hipStream_t s = as_stream(stream);
 
     const dim3 grid(unsigned(chunks * a.g.num_tiles));
@@ -443,4 +488,35 @@ extern "C" fbp_status fbp_backproject_exhaustive_voxel(fbp_context ctx,
     // The fast weights carry dt^2 (1 / q^2): scale by 1 / dt^2 in FP64.
     return launch_reduce(ctx, a.partial, nullptr, volume, a.g, uint32_t(chunks),
                          1.0 / (double(a.dt) * double(a.dt)), s);
+}
+
+}  // namespace fbp
+
+using namespace fbp;
+
+extern "C" fbp_status fbp_backproject_exhaustive_voxel(fbp_context ctx,
+                                                       const fbp_reconstruction_info* info,
+                                                       const fbp_transient* tr,
+                                                       const fbp_vec3* laser_targets,
+                                                       const fbp_vec3* sensor_targets,
+                                                       float* volume, void* stream) {
+    return exhaustive_voxel_impl(ctx, info, tr, laser_targets, sensor_targets, volume, stream,
+                                 nullptr, 0);
+}
+
+extern "C" fbp_status fbp_backproject_exhaustive_voxel_noalloc(fbp_context ctx,
+                                                               const fbp_reconstruction_info* info,
+                                                               const fbp_transient* tr,
+                                                               const fbp_vec3* laser_targets,
+                                                               const fbp_vec3* sensor_targets,
+                                                               float* volume, void* stream) {
+    if (!ctx) return FBP_ERR_INVALID_ARGUMENT;
+    if (!ctx->noalloc_scratch)
+        return fail(ctx, FBP_ERR_DEVICE, "exhaustive no-alloc: the scratch address of device %d "
+                    "could not be resolved at context creation", ctx->device);
+    // Every partial-sum slot the reduction reads is written by this call's
+    // kernel first, so the array needs no clearing between calls.
+    return exhaustive_voxel_impl(ctx, info, tr, laser_targets, sensor_targets, volume, stream,
+                                 static_cast<float*>(ctx->noalloc_scratch),
+                                 FBP_NOALLOC_SCRATCH_BYTES / sizeof(float));
 }

@@ -109,6 +109,11 @@ fbp_status fbp_context_create(int device, fbp_context* out) {
         delete ctx;
         return FBP_ERR_OUT_OF_MEMORY;
     }
+    // Library set-up for the no-alloc entry point, with the device current:
+    // the per-device address of its scratch array.  A failure here only
+    // disables that entry point (it reports the recorded message).
+    if (fbp::noalloc_scratch_address(ctx, &ctx->noalloc_scratch) != FBP_OK)
+        ctx->noalloc_scratch = nullptr;
     *out = ctx;
     return FBP_OK;
 }

@@ -38,6 +38,10 @@ struct fbp_context_s {
     float* pinned = nullptr;
     // Cached rocFFT plans and the LCT inverse-psf spectrum (owned by fourier.hip).
     void* fft_cache = nullptr;
+    // Device address, on this context's device, of the module-scope partial
+    // sum array of fbp_backproject_exhaustive_voxel_noalloc (resolved at
+    // context creation; nullptr when that failed).
+    void* noalloc_scratch = nullptr;
 };
 
 namespace fbp {
@@ -52,6 +56,10 @@ fbp_status check_hip(fbp_context ctx, hipError_t e, const char* what);
 // device only when it has to grow.
 fbp_status ensure_scratch(fbp_context ctx, size_t bytes);
 fbp_status ensure_scratch2(fbp_context ctx, size_t bytes);
+
+// Device address of the no-alloc scratch array (bp_exhaustive.hip) on
+// ctx->device, which must be current; looked up once per device.
+fbp_status noalloc_scratch_address(fbp_context ctx, void** out);
 
 // Set the context's device current for this host thread.
 fbp_status bind(fbp_context ctx);

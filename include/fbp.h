@@ -163,6 +163,27 @@ fbp_status fbp_backproject_exhaustive_voxel(fbp_context ctx, const fbp_reconstru
                                             const fbp_vec3* sensor_targets, float* volume,
                                             void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): fbp_backproject_exhaustive_voxel with the same
+ * arguments and results, whose per-chunk partial sums live in a fixed
+ * module-scope device array of FBP_NOALLOC_SCRATCH_BYTES per GPU (held from
+ * fbp_context_create until the process ends) instead of the context's
+ * growing arena.  Calls no hipMalloc/hipFree or allocating library routine.
+ * The pair chunks are capped so their partial sums fit: volumes whose
+ * 16 x 16 x 32-tile-padded voxel count times 4 bytes exceeds the array
+ * (more than 8 Mi padded voxels, e.g. above 128 x 128 x 512) are refused
+ * with FBP_ERR_UNSUPPORTED.  Every call on one GPU shares that GPU's array,
+ * whatever the context: calls that overlap on one GPU (other streams,
+ * other threads) race and must be serialised by the caller. */
+#define FBP_NOALLOC_SCRATCH_BYTES (32u << 20)
+fbp_status fbp_backproject_exhaustive_voxel_noalloc(fbp_context ctx,
+                                                    const fbp_reconstruction_info* info,
+                                                    const fbp_transient* transient,
+                                                    const fbp_vec3* laser_targets,
+                                                    const fbp_vec3* sensor_targets, float* volume,
+                                                    void* stream);
+
 /* Depth-plane mode: replaces backprojectConfocal / backprojectExhaustive and
  * Laser::reconstructDepthConfocal / Exhaustive
  * (FB/transient_reconstruction.cuh:41-91, FB/Laser.cu:379-480).
