@@ -70,22 +70,29 @@ __device__ __forceinline__ float exact_dist(float x, float y, float z, LaserPos 
     return exhaustive_dist(sl, ss, a.t0, a.discard, off);
 }
 
-// One pair applied to the lane's 32 voxels.  rq: laser distances / dt;
-// AQ, ZQ: the pair's per-axis squared sensor distances (x+y, z) / dt^2.
-// NEAR: every voxel takes the whole exact chain (bin and weight).
 __device__ __forceinline__ float pick4f(const float (&v)[4], int i) {
     return i == 0 ? v[0] : i == 1 ? v[1] : i == 2 ? v[2] : v[3];
 }
 
-template <bool NEAR, bool COFF>
-__device__ __forceinline__ void exhaustive_update_tile(float (&acc)[32], const float (&rq)[32],
-                                                       const float (&px)[4], const float (&py)[2],
-                                                       const float (&pz)[4], LaserPos lp,
-                                                       const float (&AQ)[4][2], const float (&ZQ)[4],
-                                                       const float* win,
-                                                       int base, float hme, float coffq,
-                                                       float off, float sx, float sy, float sz,
-                                                       const ExhaustiveArgs& a, float* corr) {
+// Window entry of bin k.  Linear-weight pairs (LIN, bins >= kLinMinBin
+// only): {I / m^2, -2 I / m^3} with m = k + 1/2, so I / q^2 for q = m + d is
+// a + b d up to 3 d^2 / m^2 <= 0.75 / m^2 relative (<= 3e-5 at m >= 160).
+// Other pairs: {I, 0} and the weight is 1 / q^2.  Bins in units of dt: the
+// reduction scales by 1 / dt^2.
+constexpr int kLinMinBin = 160;
+
+// Fast update of one pair on the lane's 32 voxels.  rq: laser distances /
+// dt; AQ, ZQ: the pair's per-axis squared sensor distances (x + y, z) /
+// dt^2.  q = rl/dt + sqrt(ss/dt^2) (+ (off - t0)/dt), bin (int)q.  A voxel
+// whose q lies within the estimate's error bound of a bin boundary still
+// adds its fast contribution but sets its bit in the returned mask (bit
+// 31 - slot); exhaustive_fixup later replaces that contribution with the
+// reference's exact chain.  No select and no division per update.
+template <bool LIN, bool COFF>
+__device__ __forceinline__ uint32_t exhaustive_fast(float (&acc)[32], const float (&rq)[32],
+                                                    const float (&AQ)[4][2], const float (&ZQ)[4],
+                                                    const float2* win, int base, float hme,
+                                                    float coffq) {
     uint32_t band_mask = 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -97,41 +104,73 @@ __device__ __forceinline__ void exhaustive_update_tile(float (&acc)[32], const f
                 const float s2 = AQ[i][j] + ZQ[m];
                 float q = fmaf(s2, __builtin_amdgcn_rsqf(s2), rq[slot]);
                 if constexpr (COFF) q = q + coffq;
-                int k = static_cast<int>(q);
-                float w = __builtin_amdgcn_rcpf(q * q);
-                if constexpr (NEAR) {
-                    const float de = exact_dist(px[i], py[j], pz[m], lp, sx, sy, sz, a, off);
-                    k = bin_of(de, a.dt);
-                    w = safe_rcp(rn_mul(de, de)) * a.dt2;
-                    acc[slot] = fmaf(win[base + k], w, acc[slot]);
-                } else {
-                    // q within the error bound of a bin boundary: left out
-                    // here, added exactly below.
-                    const float f = __builtin_amdgcn_fractf(q);
-                    const bool band = fabsf(f - 0.5f) > hme;
-                    band_mask |= uint32_t(band) << slot;
-                    const float v = win[base + k];
-                    acc[slot] = fmaf(band ? 0.f : v, w, acc[slot]);
-                }
+                const int k = static_cast<int>(q);
+                const float d = __builtin_amdgcn_fractf(q) - 0.5f;
+                band_mask = band_mask + band_mask + (fabsf(d) > hme ? 1u : 0u);
+                const float2 e = win[base + k];
+                if constexpr (LIN)
+                    acc[slot] += fmaf(d, e.y, e.x);
+                else
+                    acc[slot] += e.x * __builtin_amdgcn_rcpf(q * q);
             }
         }
     }
-    if constexpr (!NEAR) {
-        if (__any(band_mask != 0)) {
-            // The voxels next to a boundary take the exact chain (bin and
-            // weight) into the lane's correction slots.
-            while (band_mask) {
-                const int slot = __builtin_ctz(band_mask);
-                band_mask &= band_mask - 1u;
-                const int i = slot >> 3, j = (slot >> 2) & 1, m = slot & 3;
-                const float de = exact_dist(pick4f(px, i), j ? py[1] : py[0], pick4f(pz, m), lp,
-                                            sx, sy, sz, a, off);
+    return band_mask;
+}
+
+// Pairs near the wall or near zero distance: every voxel takes the
+// reference's exact chain (bin by IEEE division, weight safeRCP(dist^2)).
+// Window entries {I, 0}.
+__device__ __forceinline__ void exhaustive_near(float (&acc)[32], const float (&px)[4],
+                                                const float (&py)[2], const float (&pz)[4],
+                                                LaserPos lp, const float2* win, int base,
+                                                float off, float sx, float sy, float sz,
+                                                const ExhaustiveArgs& a) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const float de = exact_dist(px[i], py[j], pz[m], lp, sx, sy, sz, a, off);
                 const int k = bin_of(de, a.dt);
-                if (k >= 0 && k < int(a.T))
-                    corr[slot * 256] += win[base + k] * (safe_rcp(rn_mul(de, de)) * a.dt2);
+                const float w = safe_rcp(rn_mul(de, de)) * a.dt2;
+                acc[(i * 2 + j) * 4 + m] = fmaf(win[base + k].x, w, acc[(i * 2 + j) * 4 + m]);
             }
+}
+
+// Replaces the fast contribution of one flagged (voxel, pair) by the exact
+// one: recomputes the fast path's q, bin and contribution with the same
+// operations (so the same value) and the reference's IEEE chain, and adds
+// the difference to the voxel's correction slot.
+template <bool COFF>
+__device__ __forceinline__ void exhaustive_fixup(int slot, float x, float y, float z, LaserPos lp,
+                                                 const float2* win, int base, float coffq,
+                                                 float off, float sx, float sy, float sz, bool lin,
+                                                 const ExhaustiveArgs& a, float* corr) {
+    const float sl = dot3(sq(rn_sub(x, lp.x)), sq(rn_sub(y, lp.y)), sq(rn_sub(z, lp.z)));
+    const float rqv = sl * __builtin_amdgcn_rsqf(sl) * a.inv_dt;
+    const float aq = (sq(rn_sub(x, sx)) + sq(rn_sub(y, sy))) * a.inv_dt2;
+    const float zq = sq(rn_sub(z, sz)) * a.inv_dt2;
+    const float s2 = aq + zq;
+    float q = fmaf(s2, __builtin_amdgcn_rsqf(s2), rqv);
+    if constexpr (COFF) q = q + coffq;
+    const int kf = static_cast<int>(q);
+    const float d = __builtin_amdgcn_fractf(q) - 0.5f;
+    const float2 ef = win[base + kf];
+    const float fast = lin ? fmaf(d, ef.y, ef.x) : ef.x * __builtin_amdgcn_rcpf(q * q);
+    const float de = exact_dist(x, y, z, lp, sx, sy, sz, a, off);
+    const int ke = bin_of(de, a.dt);
+    float exact = 0.f;
+    if (ke >= 0 && ke < int(a.T)) {
+        float I = win[base + ke].x;
+        if (lin) {
+            const float mk = float(ke) + 0.5f;
+            I = I * (mk * mk);
         }
+        exact = I * (safe_rcp(rn_mul(de, de)) * a.dt2);
     }
+    corr[slot * 256] += exact - fast;
 }
 
 // Window prefetch registers per thread: batch * W <= kPrefetch * 256.
@@ -147,9 +186,11 @@ struct PairParams {
     float near;   // 1: every voxel of the tile takes the exact chain
     float off;
     int kmin;
+    int lin;      // 1: window entries carry linear weights (kmin >= kLinMinBin)
+    uint32_t l;   // laser index of the pair
 };
 
-template <int STORAGE>
+template <int STORAGE, bool LINK>
 __device__ __forceinline__ PairParams pair_params(const ExhaustiveArgs& a, uint64_t pair, int slot,
                                                   const float (&lo)[3], const float (&hi)[3]) {
     const uint32_t l = uint32_t(pair / a.S), s = uint32_t(pair % a.S);
@@ -178,6 +219,8 @@ __device__ __forceinline__ PairParams pair_params(const ExhaustiveArgs& a, uint6
     P.coffq = (off - a.t0) * a.inv_dt;
     P.near = ((dmin < 0.0045f && dmax > -0.0045f) || rlmin < 1.0e-3f || rsmin < 1.0e-3f) ? 1.f : 0.f;
     P.off = off;
+    P.lin = (LINK && P.near == 0.f && P.kmin >= kLinMinBin) ? 1 : 0;
+    P.l = l;
     return P;
 }
 
@@ -187,16 +230,21 @@ __device__ __forceinline__ PairParams pair_params(const ExhaustiveArgs& a, uint6
 // double-buffered: the windows of batch b+1 are fetched into registers while
 // batch b is processed.
 // COFF: the chain has a time offset or per-pair path offsets (q needs the
-// pair's (off - t0) / dt).
-template <int STORAGE, bool COFF>
+// pair's (off - t0) / dt).  LINK: the fast body uses linear-weight windows;
+// pairs that do not qualify (bins below kLinMinBin) take the exact body.
+// Without LINK the fast body divides (rcp) per update.  One fast body per
+// kernel keeps its q computation next to its window reads.
+template <int STORAGE, bool COFF, bool LINK>
 __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs a) {
     extern __shared__ float smem[];
-    // [2][batch * W] windows, [2][batch] PairParams, then the correction
-    // slots of the boundary voxels [32][256 threads] (thread-fastest: no
-    // bank conflicts).
-    float* winbuf = smem;
-    PairParams* prmbuf = reinterpret_cast<PairParams*>(smem + 2 * a.batch * a.W);
+    // [2][batch * W] window entries, [2][batch] PairParams, the correction
+    // slots of the boundary voxels [32][256 threads] and the band masks of
+    // the batch's pairs [batch][256 threads] (thread-fastest: no bank
+    // conflicts).
+    float2* winbuf = reinterpret_cast<float2*>(smem);
+    PairParams* prmbuf = reinterpret_cast<PairParams*>(winbuf + 2 * a.batch * a.W);
     float* corrl = reinterpret_cast<float*>(prmbuf + 2 * a.batch) + threadIdx.x;
+    uint32_t* maskl = reinterpret_cast<uint32_t*>(corrl - threadIdx.x + 32 * 256) + threadIdx.x;
 
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -230,6 +278,7 @@ __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs
     float rq[32];
     LaserPos lp{0.f, 0.f, 0.f};
     uint32_t cur_l = 0xffffffffu;
+    uint32_t pending = 0;   // pairs of the current batch with flagged voxels (bit jl)
 
     const uint64_t p_begin = uint64_t(chunk) * a.chunk_len;
     const uint64_t p_end = min(a.num_pairs, p_begin + a.chunk_len);
@@ -256,11 +305,24 @@ __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs
             reg[i] = ok ? v : 0.f;
         }
     };
-    auto store = [&](float* win, int nb, const float (&reg)[kPrefetch]) {
+    // Window entries from the fetched samples: {I / m^2, -2 I / m^3} (m = k
+    // + 1/2) for linear-weight pairs, {I, 0} otherwise.
+    auto store = [&](float2* win, const PairParams* prm, int nb, const float (&reg)[kPrefetch]) {
 #pragma unroll
         for (int i = 0; i < kPrefetch; ++i) {
             const int f = tid + i * 256;
-            if (f < nb * a.W) win[f] = reg[i];
+            if (f < nb * a.W) {
+                const int jl = int((float(f) + 0.5f) * a.inv_W);
+                const float v = reg[i];
+                float2 e = make_float2(v, 0.f);
+                if (prm[jl].lin) {
+                    const float mk = float(prm[jl].kmin + (f - jl * a.W)) + 0.5f;
+                    const float r = __builtin_amdgcn_rcpf(mk);
+                    const float wa = v * (r * r);
+                    e = make_float2(wa, -2.f * wa * r);
+                }
+                win[f] = e;
+            }
         }
     };
 
@@ -268,10 +330,10 @@ __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs
     // Prologue: batch 0 into buffer 0.
     if (p_begin < p_end) {
         const int nb = int(min(uint64_t(a.batch), p_end - p_begin));
-        if (tid < nb) prmbuf[tid] = pair_params<STORAGE>(a, p_begin + tid, tid, lo, hi);
+        if (tid < nb) prmbuf[tid] = pair_params<STORAGE, LINK>(a, p_begin + tid, tid, lo, hi);
         __syncthreads();
         fetch(p_begin, nb, prmbuf, reg);
-        store(winbuf, nb, reg);
+        store(winbuf, prmbuf, nb, reg);
         __syncthreads();
     }
     int cur = 0;
@@ -280,11 +342,10 @@ __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs
         const uint64_t pn = pb + a.batch;
         const bool has_next = pn < p_end;
         const int nbn = has_next ? int(min(uint64_t(a.batch), p_end - pn)) : 0;
-        float* win = winbuf + cur * per;
         const PairParams* prm = prmbuf + cur * a.batch;
         PairParams* prm_next = prmbuf + (cur ^ 1) * a.batch;
         if (has_next) {
-            if (tid < nbn) prm_next[tid] = pair_params<STORAGE>(a, pn + tid, tid, lo, hi);
+            if (tid < nbn) prm_next[tid] = pair_params<STORAGE, LINK>(a, pn + tid, tid, lo, hi);
             __syncthreads();
             fetch(pn, nbn, prm_next, reg);
         }
@@ -333,14 +394,39 @@ __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int j = 0; j < 2; ++j) AQ[i][j] = (XS2[i] + YS2[j]) * a.inv_dt2;
-            if (near)
-                exhaustive_update_tile<true, COFF>(acc, rq, px, py, pz, lp, AQ, ZQ, winbuf, base,
-                                                   hme, coffq, off, sxp, syp, szp, a, corrl);
+            uint32_t mk = 0;
+            if (near || (LINK && rfl(P.lin) == 0))
+                exhaustive_near(acc, px, py, pz, lp, winbuf, base, off, sxp, syp, szp, a);
             else
-                exhaustive_update_tile<false, COFF>(acc, rq, px, py, pz, lp, AQ, ZQ, winbuf, base,
-                                                    hme, coffq, off, sxp, syp, szp, a, corrl);
+                mk = exhaustive_fast<LINK, COFF>(acc, rq, AQ, ZQ, winbuf, base, hme, coffq);
+            maskl[jl * 256] = mk;
+            pending |= (mk != 0u ? 1u : 0u) << jl;
         }
-        if (has_next) store(winbuf + (cur ^ 1) * per, nbn, reg);
+        // The next batch's windows go to the other buffer (its previous batch
+        // was finished by every thread before this batch began), which frees
+        // the prefetch registers before the boundary voxels are handled.
+        if (has_next) store(winbuf + (cur ^ 1) * per, prm_next, nbn, reg);
+        // Deferred boundary voxels of this batch: each lane replaces the fast
+        // contributions of its own flagged voxels, pair by pair in order
+        // (deterministic), while this batch's windows are still staged.
+        if (__any(pending != 0u)) {
+            while (pending) {
+                const int jl = __builtin_ctz(pending);
+                pending &= pending - 1u;
+                uint32_t m = maskl[jl * 256];
+                const PairParams& P = prm[jl];
+                const LaserPos lq{a.laser[P.l].x, a.laser[P.l].y, a.laser[P.l].z};
+                const int base = P.base + cur * per;
+                while (m) {
+                    const int slot = 31 - __builtin_ctz(m);
+                    m &= m - 1u;
+                    const int i = slot >> 3, j = (slot >> 2) & 1, mm = slot & 3;
+                    exhaustive_fixup<COFF>(slot, pick4f(px, i), j ? py[1] : py[0], pick4f(pz, mm),
+                                           lq, winbuf, base, P.coffq, P.off, P.sx, P.sy, P.sz,
+                                           P.lin != 0, a, corrl);
+                }
+            }
+        }
         __syncthreads();
         cur ^= 1;
     }
@@ -381,6 +467,47 @@ fbp_status noalloc_scratch_address(fbp_context ctx, void** out) {
     }
     *out = g_noalloc_ptr[ctx->device];
     return FBP_OK;
+}
+
+template <int STORAGE>
+static void (*pick_kernel_s(bool coff, bool link))(ExhaustiveArgs) {
+    if (coff) return link ? exhaustive_voxel_kernel<STORAGE, true, true>
+                          : exhaustive_voxel_kernel<STORAGE, true, false>;
+    return link ? exhaustive_voxel_kernel<STORAGE, false, true>
+                : exhaustive_voxel_kernel<STORAGE, false, false>;
+}
+
+static void (*pick_kernel(int storage, bool coff, bool link))(ExhaustiveArgs) {
+    return storage == FBP_STORAGE_F32   ? pick_kernel_s<FBP_STORAGE_F32>(coff, link)
+           : storage == FBP_STORAGE_F16 ? pick_kernel_s<FBP_STORAGE_F16>(coff, link)
+                                        : pick_kernel_s<FBP_STORAGE_BF16>(coff, link);
+}
+
+// Host-side choice of the linear-weight kernel: when the scan targets lie on
+// the relay wall plane, every leg is at least the hidden box's distance from
+// that plane, so every pair's first window bin is at least
+// floor((2 gap - t0) / dt) - 3 (the bounce offsets only add).  If that bound
+// reaches kLinMinBin, (almost) every pair qualifies for linear weights.  The
+// choice is performance only: pairs that do not qualify take the exact body.
+static bool linear_weights_apply(const fbp_reconstruction_info& in) {
+    const double n[3] = {in.relay_wall_normal.x, in.relay_wall_normal.y, in.relay_wall_normal.z};
+    const double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (!(nn > 0.0)) return false;
+    const double w0[3] = {in.relay_wall_min_position.x, in.relay_wall_min_position.y,
+                          in.relay_wall_min_position.z};
+    const double lo[3] = {in.hidden_volume_min.x, in.hidden_volume_min.y, in.hidden_volume_min.z};
+    const double hi[3] = {in.hidden_volume_max.x, in.hidden_volume_max.y, in.hidden_volume_max.z};
+    // Signed distance of the box from the plane along the normal: its
+    // smallest corner distance (negative when the box crosses the plane).
+    double gap = 1e30;
+    for (int c = 0; c < 8; ++c) {
+        double dot = 0.0;
+        for (int k = 0; k < 3; ++k) dot += ((((c >> k) & 1) ? hi[k] : lo[k]) - w0[k]) * n[k];
+        gap = std::min(gap, dot / nn);
+    }
+    if (!(gap > 0.0)) return false;
+    const double kmin = std::floor((2.0 * gap - double(in.time_offset)) / double(in.time_step)) - 3.0;
+    return kmin >= double(kLinMinBin);
 }
 
 // Shared body of the two exhaustive entry points.  fixed_partial == nullptr:
@@ -434,9 +561,16 @@ static fbp_status exhaustive_voxel_impl(fbp_context ctx, const fbp_reconstructio
     if (a.W > kPrefetch * 256)
         return fail(ctx, FBP_ERR_UNSUPPORTED, "exhaustive: bin window of %d bins per tile too long "
                     "(voxels too large for the bin width)", a.W);
-    // Pairs per batch: the window prefetch registers hold one batch.
+    // Pairs per batch: the window prefetch registers hold one batch, and the
+    // LDS of two workgroups per CU (2 x 80 KiB) holds their windows, pair
+    // parameters, correction slots and band masks when the window allows.
+    auto lds_bytes = [&](int b) {
+        return 2 * (size_t(b) * a.W * sizeof(float2) + b * sizeof(PairParams)) +
+               size_t(32) * 64 * kWaves * sizeof(float) + size_t(b) * 64 * kWaves * sizeof(uint32_t);
+    };
     a.batch = 16;
-    while (a.batch > 1 && a.batch * a.W > kPrefetch * 256) a.batch /= 2;
+    while (a.batch > 1 && (a.batch * a.W > kPrefetch * 256 || lds_bytes(a.batch) > (80u << 10)))
+        --a.batch;
     a.inv_W = 1.0f / float(a.W);
     // ~8 workgroups (32 waves) per CU over the launch.
     const uint32_t target = 8u * uint32_t(ctx->num_cus > 0 ? ctx->num_cus : 256);
@@ -465,22 +599,14 @@ static fbp_status exhaustive_voxel_impl(fbp_context ctx, const fbp_reconstructio
     hipStream_t s = as_stream(stream);
 
     const dim3 grid(unsigned(chunks * a.g.num_tiles));
-    const size_t shmem = 2 * (size_t(a.batch) * a.W * sizeof(float) + a.batch * sizeof(PairParams)) +
-                         size_t(32) * 64 * kWaves * sizeof(float);
-    // At most 2 * 16 KiB of windows + 32 KiB of correction slots (+ pair
-    // parameters): beyond the default 64 KiB for the longest windows.
+    const size_t shmem = lds_bytes(a.batch);
+    // Up to 2 x 32 KiB of window entries + 32 KiB of correction slots (+ pair
+    // parameters and masks): beyond the default 64 KiB for the longest windows.
     if (shmem > size_t(ctx->max_lds_per_block))
         return fail(ctx, FBP_ERR_UNSUPPORTED, "exhaustive: %zu B of LDS per workgroup needed", shmem);
     const bool coff = !(a.t0 == 0.f && a.discard != 0);
-    void (*kern)(ExhaustiveArgs) =
-        tr->storage == FBP_STORAGE_F32
-            ? (coff ? exhaustive_voxel_kernel<FBP_STORAGE_F32, true>
-                    : exhaustive_voxel_kernel<FBP_STORAGE_F32, false>)
-        : tr->storage == FBP_STORAGE_F16
-            ? (coff ? exhaustive_voxel_kernel<FBP_STORAGE_F16, true>
-                    : exhaustive_voxel_kernel<FBP_STORAGE_F16, false>)
-            : (coff ? exhaustive_voxel_kernel<FBP_STORAGE_BF16, true>
-                    : exhaustive_voxel_kernel<FBP_STORAGE_BF16, false>);
+    const bool link = linear_weights_apply(*info);
+    void (*kern)(ExhaustiveArgs) = pick_kernel(tr->storage, coff, link);
     FBP_CHECK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
     hipLaunchKernelGGL(kern, grid, dim3(64 * kWaves), shmem, s, a);

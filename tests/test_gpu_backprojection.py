@@ -328,6 +328,30 @@ def test_exhaustive_spiky_parity(ctx):
     assert_volume_close(out, ref)
 
 
+@pytest.mark.parametrize("discard,t0", [(1, 0.0), (0, 0.1)])
+def test_exhaustive_linear_weight_kernel(ctx, discard, t0):
+    """Hidden box >= 0.5 m from the wall at 4.8 mm bins: every pair's bins
+    are >= 160, so the linear-weight kernel runs (window entries I/m^2 and
+    -2I/m^3, boundary voxels replaced by the exact chain at the end of each
+    batch).  Spiky data: a wrong bin or weight moves a voxel far out of the
+    tolerance."""
+    lasers = wall_grid(4, 1.0)
+    sensors = wall_grid(6, 1.0)
+    g = Geometry(laser_targets=lasers, sensor_targets=sensors, num_time_bins=1024,
+                 time_step=0.0048, hidden_min=(-0.4, 0.5, -0.45), hidden_max=(0.45, 1.4, 0.4),
+                 voxel_resolution=(20, 18, 40), confocal=False,
+                 discard_first_last_bounces=discard, time_offset=t0,
+                 laser_position=(0.1, -0.8, 0.2), sensor_position=(-0.2, -0.7, 0.1),
+                 wall_size=(1.0, 1.0))
+    info = make_info(g)
+    rng = np.random.default_rng(14)
+    I = spiky(rng, (16, 36, 1024))
+    ref = oracle.exhaustive_voxel(info, I, lasers, sensors)
+    out = ctx.backproject_exhaustive(info, cuda(I), cuda(lasers), cuda(sensors))
+    torch.cuda.synchronize()
+    assert_volume_close(out, ref)
+
+
 @pytest.mark.parametrize("discard,t0", [(0, 0.2)])
 def test_exhaustive_general_chain(ctx, discard, t0):
     lasers = wall_grid(3, 1.0)

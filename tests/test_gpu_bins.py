@@ -146,15 +146,19 @@ def test_confocal_aligned_path_bins_exact_quantized(ctx, storage):
                         storage=storage)
 
 
-@pytest.mark.parametrize("discard,t0", [(1, 0.0), (0, 0.29)])
-def test_exhaustive_bins_exact(ctx, discard, t0):
+@pytest.mark.parametrize("discard,t0,dt,depth0", [(1, 0.0, 0.0096, 0.35), (0, 0.29, 0.0096, 0.35),
+                                                  # bins >= 160: linear-weight kernel
+                                                  (1, 0.0, 0.0048, 0.5), (0, 0.1, 0.0048, 0.5)])
+def test_exhaustive_bins_exact(ctx, discard, t0, dt, depth0):
     """Every (voxel, laser, sensor) triple with 4 lasers x 9 sensors on
-    different grids, one lit pair per call (index l*S*T + s*T + t)."""
+    different grids, one lit pair per call (index l*S*T + s*T + t).  With
+    linear weights vol * dist^2 = (bin + 1)(1 - e), e <= 3e-5, which still
+    rounds to bin + 1 for bins below 10^4."""
     lasers = wall_grid(2, 0.5)
     sensors = wall_grid(3, 1.0)
-    T = 1024
-    g = Geometry(laser_targets=lasers, sensor_targets=sensors, num_time_bins=T, time_step=0.0096,
-                 hidden_min=(-0.4, 0.35, -0.45), hidden_max=(0.45, 1.25, 0.4),
+    T = 1024 if dt > 0.005 else 2048
+    g = Geometry(laser_targets=lasers, sensor_targets=sensors, num_time_bins=T, time_step=dt,
+                 hidden_min=(-0.4, depth0, -0.45), hidden_max=(0.45, depth0 + 0.9, 0.4),
                  voxel_resolution=(8, 6, 5), confocal=False, discard_first_last_bounces=discard,
                  time_offset=t0, laser_position=(0.1, -0.8, 0.2),
                  sensor_position=(-0.2, -0.7, 0.1), wall_size=(1.0, 1.0))
