@@ -350,8 +350,8 @@ __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs
             fetch(pn, nbn, prm_next, reg);
         }
         for (int jl = 0; jl < nb; ++jl) {
-            const uint64_t pair = pb + jl;
-            const uint32_t l = rfl(int(uint32_t(pair / a.S)));
+            const PairParams& P = prm[jl];
+            const uint32_t l = uint32_t(rfl(int(P.l)));
             if (l != cur_l) {
                 cur_l = l;
                 lp = LaserPos{rfl(a.laser[l].x), rfl(a.laser[l].y), rfl(a.laser[l].z)};
@@ -374,7 +374,6 @@ __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs
                                 sl * __builtin_amdgcn_rsqf(sl) * a.inv_dt;
                         }
             }
-            const PairParams& P = prm[jl];
             const float sxp = rfl(P.sx), syp = rfl(P.sy), szp = rfl(P.sz);
             const int base = rfl(P.base) + cur * per;
             const float hme = rfl(P.hme);
