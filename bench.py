@@ -209,8 +209,24 @@ def main():
         torch.empty_like(src).copy_(src)
         torch.cuda.synchronize()
         del src
+    # The aligned path's first call on a fresh context builds its offset
+    # tables (geometry only, kept across calls): cold and warm call.
+    ctx_cold = Context(0)
+    cold = []
+    for _ in range(2):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        ctx_cold.backproject_confocal(info, I, laser, out=vol)
+        b.record(stream)
+        b.synchronize()
+        cold.append(a.elapsed_time(b))
+    ctx_cold.close()
+    log(f"fresh context: first call {cold[0]:.3f} ms (table build), second {cold[1]:.3f} ms")
     cpu = None
     if not args.no_cpu_baseline:
+        import oracle
+        log(f"cpu legs use the oracle built for this host: {oracle.use_native()}")
         cpu = cpu_baseline(info, I, g, args.cpu_seconds)
     precision = None if args.no_precision else precision_leg(ctx, args)
     learned = None if args.no_learned else learned_leg(ctx)
@@ -256,6 +272,8 @@ def main():
         },
         "breakdown_ms": {"backprojection": bp_avg, "log_and_normalize": float(np.mean(f_ms)),
                          "backprojection_general_kernel": general_ms,
+                         "backprojection_fresh_context_first_call": cold[0],
+                         "backprojection_fresh_context_second_call": cold[1],
                          "end_to_end_with_h2d_d2h": e2e_ms},
         # Issue-slot view (non-packed FP32: 39.3e12 lane-slots/s; a
         # transcendental takes 4 slots).  Aligned path: 1.27 FMA slots and
@@ -466,9 +484,12 @@ def lct_leg(ctx, args, reps=5):
     2 m wall) zero-padded to 256x256x2048, Wiener SNR 8 (python/lct.py
     semantics).  HBM-bound: algorithmic bytes = 2 FFTs x (read + write) x
     8 |P| + filter read 8 |P| for the padded complex volume |P| (5.4 GB).
-    CPU baseline: the NumPy restatement of python/lct.py (oracle/lct.py) on
-    the small configuration (32x32 scan, 256 bins of 32 ps, pad 64x64x512),
-    with the GPU timed on the same input."""
+    CPU baseline: the NumPy/SciPy restatement of python/lct.py
+    (oracle/lct.py, scipy.fft with the host's worker threads) on the same
+    capture and on the small configuration (32x32 scan, 256 bins of 32 ps,
+    pad 64x64x512), with the inverse filter built once beforehand on both
+    sides (the GPU caches it): the full size is timed once (it takes seconds
+    and ~10 GB), the small one as the median of 3 after a warm-up run."""
     import numpy as np
     import torch
 
@@ -508,15 +529,37 @@ def lct_leg(ctx, args, reps=5):
                                  "frac": alg / (t * 1e-3) / 1e9 / PEAK_HBM_GBPS,
                                  "algorithmic_bytes": alg},
            "finite": bool(torch.isfinite(out).all())}
-    # small configuration: GPU vs the CPU restatement, same input
     if not args.no_cpu_baseline:
         from oracle import lct as olct
+        workers = cpu_workers()
+        # full size: the same capture, inverse filter prebuilt
+        data_h = data.cpu().numpy()
+        t0 = time.perf_counter()
+        inv_full = olct.inverse_filter(128, 1024, width, rmax, 8.0, workers=workers)
+        filt_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ref_full = olct.lct(data_h, width, rmax, 8.0, invpsf=inv_full, workers=workers)
+        full_s = time.perf_counter() - t0
+        res["cpu"] = {"sample": "the same 128x128x1024 capture, one run (inverse filter "
+                                "prebuilt, as the GPU caches it)",
+                      "s": full_s, "inverse_filter_build_s": filt_s, "cores": workers,
+                      "kind": "port (NumPy/SciPy restatement of python/lct.py)",
+                      "rel_l2_gpu_vs_cpu": float(np.linalg.norm(out.cpu().numpy() - ref_full) /
+                                                 np.linalg.norm(ref_full))}
+        log(f"lct cpu (full size, {workers} workers): {full_s:.2f} s")
+        del data_h, inv_full, ref_full
+        # small configuration: GPU vs the CPU restatement, same input
         cap = small_confocal(seed=0)
         d1 = cap.transient.reshape(32, 32, 256).astype(np.float32)
         w1, r1 = 0.5, 256 * C_LIGHT * 32e-12
-        t0 = time.perf_counter()
-        ref = olct.lct(d1, w1, r1, 8.0)
-        cpu_s = time.perf_counter() - t0
+        inv1 = olct.inverse_filter(32, 256, w1, r1, 8.0, workers=workers)
+        ref = olct.lct(d1, w1, r1, 8.0, invpsf=inv1, workers=workers)   # warm-up
+        cpu_runs = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            ref = olct.lct(d1, w1, r1, 8.0, invpsf=inv1, workers=workers)
+            cpu_runs.append(time.perf_counter() - t0)
+        cpu_s = float(np.median(cpu_runs))
         g1 = ctx.lct(torch.from_numpy(d1).to(dev), w1, r1, 8.0)
         torch.cuda.synchronize()
         ms1 = []
@@ -529,8 +572,8 @@ def lct_leg(ctx, args, reps=5):
             b.synchronize()
             ms1.append(a.elapsed_time(b))
         res["small"] = {"workload": "LCT 32x32x256 (32 ps, 1 m wall) padded to 64x64x512",
-                        "gpu_ms": float(np.median(ms1)), "cpu_s": cpu_s, "cpu_cores": 1,
-                        "cpu_kind": "port (NumPy restatement of python/lct.py)",
+                        "gpu_ms": float(np.median(ms1)), "cpu_s": cpu_s, "cpu_cores": workers,
+                        "cpu_kind": "port (NumPy/SciPy restatement of python/lct.py), median of 3",
                         "rel_l2_gpu_vs_cpu": float(np.linalg.norm(g1.cpu().numpy() - ref) /
                                                    np.linalg.norm(ref))}
     log(f"lct: {res}")
@@ -854,8 +897,21 @@ def cpu_model():
     return None
 
 
+def cpu_workers():
+    """Host threads a CPU leg uses: OMP_NUM_THREADS when set (the GPU box
+    sets it to its share of cores), else the cores this process may run on."""
+    import os
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
 def cpu_baseline(info, I, g, target_seconds):
-    """FP64 OpenMP oracle on the first z-slices of the same volume."""
+    """FP64 OpenMP oracle (its -O3 -march=native build for this host) on the
+    first z-slices of the same volume: one slice as warm-up and calibration,
+    then the median of 3 timed runs sized to take about target_seconds
+    together; one core: the median of 3 runs on a quarter slice."""
     import numpy as np
 
     import oracle
@@ -872,18 +928,22 @@ def cpu_baseline(info, I, g, target_seconds):
         return time.perf_counter() - t
 
     t1 = run(1)
-    nz = int(max(1, min(info.voxel_resolution.z, target_seconds / max(t1, 1e-3))))
-    t = run(nz) if nz > 1 else t1
+    nz = int(max(1, min(info.voxel_resolution.z, target_seconds / 3 / max(t1, 1e-3))))
+    times = [run(nz) for _ in range(3)]
+    t = float(np.median(times))
     cores = oracle.max_threads()
-    log(f"cpu oracle: {nz} slices in {t:.2f} s on {cores} threads")
+    log(f"cpu oracle: {nz} slices in {t:.2f} s (median of 3) on {cores} threads")
     # One core on a smaller slab (a quarter of one slice's rows).
     oracle.set_threads(1)
     try:
         sub.voxel_resolution.z = 1
         sub.voxel_resolution.y = max(1, Y // 4)
-        t0 = time.perf_counter()
-        oracle.confocal_voxel(sub, host, laser)
-        t_one = time.perf_counter() - t0
+        t_ones = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            oracle.confocal_voxel(sub, host, laser)
+            t_ones.append(time.perf_counter() - t0)
+        t_one = float(np.median(t_ones))
     finally:
         oracle.set_threads(cores)
     one_core = X * max(1, Y // 4) * L / t_one
@@ -895,8 +955,10 @@ def cpu_baseline(info, I, g, target_seconds):
         "unit": "voxel-updates/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"FP64 OpenMP oracle (oracle/fbp_oracle.c) on the first {nz} of "
-                  f"{info.voxel_resolution.z} depth slices ({X}x{Y}x{nz} voxels x {L} scan points)",
+        "sample": f"FP64 OpenMP oracle (oracle/fbp_oracle.c, -O3 -march=native) on the first "
+                  f"{nz} of {info.voxel_resolution.z} depth slices ({X}x{Y}x{nz} voxels x {L} "
+                  f"scan points), median of 3 runs after a warm-up slice",
+        "runs_s": times,
     }
 
 

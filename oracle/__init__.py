@@ -25,20 +25,39 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle.so"
 
 _lib = None
+_lib_path = LIB_PATH
 
 
 def build() -> Path:
-    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    subprocess.run(["make", "-s", "-C", str(HERE), "liboracle.so"], check=True)
     return LIB_PATH
+
+
+def use_native() -> Path:
+    """Switches every later oracle call to an -O3 -march=native build made
+    now, for the host this process runs on, in a fresh temporary directory
+    (never shipped with the tree).  It computes the same values as the
+    portable build (no fast-math, no contraction); for CPU baseline timing."""
+    import shutil
+    import tempfile
+    global _lib, _lib_path
+    out = Path(tempfile.mkdtemp(prefix="oracle_native_"))
+    shutil.copy(HERE / "fbp_oracle.c", out / "fbp_oracle.c")
+    shutil.copy(HERE / "Makefile", out / "Makefile")
+    subprocess.run(["make", "-s", "-C", str(out), "liboracle_native.so"], check=True)
+    _lib, _lib_path = None, out / "liboracle_native.so"
+    load()
+    return _lib_path
 
 
 def load() -> C.CDLL:
     global _lib
     if _lib is None:
         src = HERE / "fbp_oracle.c"
-        if not LIB_PATH.exists() or LIB_PATH.stat().st_mtime < src.stat().st_mtime:
+        if _lib_path == LIB_PATH and (not LIB_PATH.exists() or
+                                      LIB_PATH.stat().st_mtime < src.stat().st_mtime):
             build()
-        _lib = C.CDLL(str(LIB_PATH))
+        _lib = C.CDLL(str(_lib_path))
         P = C.c_void_p
         _lib.oracle_recinfo_size.restype = C.c_int
         _lib.oracle_confocal_bin.restype = C.c_int64

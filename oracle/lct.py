@@ -56,10 +56,10 @@ def resampling_operator(M: int):
 
 
 def inverse_filter(N: int, M: int, width: float, range_max: float, snr: float,
-                   is_backprop: bool = False) -> np.ndarray:
+                   is_backprop: bool = False, workers: int = -1) -> np.ndarray:
     """python/lct.py:112-120: invpsf = conj(F) / (|F|^2 + 1/snr), F = fftn(psf)."""
     psf = define_psf(N, M, width / range_max)
-    f = scipy.fft.fftn(psf, workers=-1)
+    f = scipy.fft.fftn(psf, workers=workers)
     if is_backprop:
         return np.conj(f)
     return np.conj(f) / (np.abs(f) ** 2 + 1.0 / snr)
@@ -89,7 +89,7 @@ def lct(data: np.ndarray, width: float, range_max: float, snr: float = 0.8,
     N, _, M = data.shape
     data[:, :, :z_trim] = 0
     if invpsf is None:
-        invpsf = inverse_filter(N, M, width, range_max, snr, is_backprop)
+        invpsf = inverse_filter(N, M, width, range_max, snr, is_backprop, workers)
     mtx, mtxi = resampling_operator(M)
     d = np.transpose(data, (2, 1, 0))                     # (M, N, N)
     gz = np.linspace(0, 1, M)[:, None, None]
