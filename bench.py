@@ -34,6 +34,13 @@ PEAK_LDS_TBPS = 78.6           # 256 CUs x 128 B/clk x 2.4 GHz
 PEAK_HBM_GBPS = 8000.0         # MI355X HBM3E (datasheet)
 PEAK_BF16_TFLOPS = 2500.0      # MI355X dense BF16 MFMA (datasheet, no sparsity)
 VALU_LANE_SLOTS = 39.3e12      # 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (non-packed issue)
+# Issue slots per update: (VALU instructions - transcendentals) + 2.37 x
+# transcendentals, from the counter profile of the kernels
+# (profiles/counters_confocal_exhaustive_lct_bench.md; 2.37 = measured
+# v_fma / v_rsq throughput ratio on this GPU).
+GENERAL_SLOTS = 14.9           # confocal_voxel_kernel
+EXH_SLOTS = 19.8               # exhaustive_voxel_kernel (linear-weight kernel)
+DEPTH_EXH_SLOTS = 17.4         # depth_exhaustive_window_kernel fast path, ISA count
 TRAFFIC_PROFILE = ROOT / "profiles" / "confocal_128cube_traffic.json"
 
 
@@ -276,18 +283,18 @@ def main():
                          "backprojection_fresh_context_second_call": cold[1],
                          "end_to_end_with_h2d_d2h": e2e_ms},
         # Issue-slot view (non-packed FP32: 39.3e12 lane-slots/s; a
-        # transcendental takes 4 slots).  Aligned path: 1.27 FMA slots and
-        # 1.27 LDS reads per useful update (LDS, at 2 clocks per wave-read
-        # per CU, binds).  General kernel: 13.9 slots per update counted in
-        # its ISA (10.9 VALU + v_rsq).
+        # transcendental takes 2.37 slots, measured).  Aligned path: 1.27 FMA
+        # slots and 1.27 LDS reads per useful update (LDS, at 2 clocks per
+        # wave-read per CU, binds).  General kernel: 14.9 slots per update
+        # measured by counters (13.5 VALU instructions, 1.0 of them v_rsq).
         "issue_slot_diagnostic": {
             "aligned_conv_kernel": {
                 "lds_bound_updates_per_s": PEAK_LDS_TBPS * 1e12 / 4.0 / 1.27,
                 "frac": updates / (bp_avg * 1e-3) / (PEAK_LDS_TBPS * 1e12 / 4.0 / 1.27)},
             "confocal_voxel_kernel": {
-                "valu_slots_per_update": 13.9,
-                "bound_updates_per_s": VALU_LANE_SLOTS / 13.9,
-                "frac": updates / (general_ms * 1e-3) / (VALU_LANE_SLOTS / 13.9)}},
+                "valu_slots_per_update": GENERAL_SLOTS,
+                "bound_updates_per_s": VALU_LANE_SLOTS / GENERAL_SLOTS,
+                "frac": updates / (general_ms * 1e-3) / (VALU_LANE_SLOTS / GENERAL_SLOTS)}},
         "cpu_baseline": cpu,
         "precision": precision,
         "learned": learned,
@@ -742,8 +749,9 @@ def exhaustive_leg(ctx, args, reps=3):
     # 2 add, sqrt) + |p - s| (cached per sensor) - t0, / dt, + sample = 13.
     depth_tflops = 13 * depth_updates / (t_depth * 1e-3) / 1e12
     # Fast path per update (counted in the kernel's ISA): 16 VALU + v_rsq
-    # (4 slots) = 19 issue slots; boundary points add the IEEE chain.
-    depth_slot_bound = VALU_LANE_SLOTS / 19.0
+    # (2.37 slots at the measured transcendental rate) = 17.4 issue slots;
+    # boundary points add the IEEE chain.
+    depth_slot_bound = VALU_LANE_SLOTS / DEPTH_EXH_SLOTS
     depth_check = None
     cpu = None
     if not args.no_cpu_baseline:
@@ -772,9 +780,10 @@ def exhaustive_leg(ctx, args, reps=3):
                "kind": "port",
                "rel_l2_gpu_vs_cpu": float(np.linalg.norm(got - ref) / np.linalg.norm(ref))}
     tflops = 25 * updates / (bp * 1e-3) / 1e12
-    # Fast path per pair-update (counted in the kernel's ISA): 11.5 VALU +
-    # v_rsq + v_rcp (4 slots each) = 19.5 issue slots.
-    slot_bound = VALU_LANE_SLOTS / 19.5
+    # Issue slots per pair-update measured by counters over the whole kernel
+    # (profiles/counters_confocal_exhaustive_lct_bench.md): 18.3 VALU
+    # instructions, 1.12 of them transcendental at 2.37 slots.
+    slot_bound = VALU_LANE_SLOTS / EXH_SLOTS
     out = {"workload": "exhaustive 16^2 lasers x 64^2 sensors x 1024 (16 ps, 4 GiB) -> 64^3 "
                        "+ Laplacian(k=5)",
            "pair_updates": updates, "backprojection_ms": bp, "laplacian_ms": lap,
@@ -782,7 +791,7 @@ def exhaustive_leg(ctx, args, reps=3):
            "roofline": {"bound": "valu", "achieved": tflops, "peak": PEAK_FP32_TFLOPS,
                         "unit": "TFLOP/s", "frac": tflops / PEAK_FP32_TFLOPS,
                         "flops_per_update": 25},
-           "issue_slot_diagnostic": {"valu_slots_per_update": 19.5, "bound_updates_per_s": slot_bound,
+           "issue_slot_diagnostic": {"valu_slots_per_update": EXH_SLOTS, "bound_updates_per_s": slot_bound,
                                      "frac": updates / (bp * 1e-3) / slot_bound},
            "depth_planes": {"workload": "200 depth planes at focus +- 0.1 over the 16x16 laser "
                                         "pixels, all 1,048,576 pairs",
@@ -792,7 +801,7 @@ def exhaustive_leg(ctx, args, reps=3):
                                          "frac": depth_tflops / PEAK_FP32_TFLOPS,
                                          "flops_per_update": 13},
                             "issue_slot_diagnostic": {
-                                "valu_slots_per_update": 19,
+                                "valu_slots_per_update": DEPTH_EXH_SLOTS,
                                 "bound_updates_per_s": depth_slot_bound,
                                 "frac": depth_updates / (t_depth * 1e-3) / depth_slot_bound},
                             "oracle_check": depth_check},
