@@ -61,7 +61,7 @@ class Transient(C.Structure):
 
 class MisParams(C.Structure):
     _fields_ = [("num_samples", C.c_uint32), ("mode", C.c_uint32), ("noise", C.c_void_p),
-                ("noise_size", C.c_uint32)]
+                ("noise_size", C.c_uint32), ("seed", C.c_uint64)]
 
 
 class LctParams(C.Structure):
@@ -100,6 +100,7 @@ SIGNATURES = {
     "fbp_log_kernel_host": (_S, [C.c_int32, C.c_float, _P]),
     "fbp_filter_log_fft": (_S, [_CTX, _P, UVec3, C.c_float, _P]),
     "fbp_normalize_minmax": (_S, [_CTX, _P, C.c_uint64, _P]),
+    "fbp_filter_log_normalize": (_S, [_CTX, _P, _P, UVec3, C.c_int32, C.c_float, _P]),
     "fbp_minmax": (_S, [_CTX, _P, C.c_uint64, C.POINTER(C.c_float), C.POINTER(C.c_float), _P]),
     "fbp_filter_h": (_S, [_CTX, _P, C.c_uint64, C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_int32, _P]),
     "fbp_filter_h_padded_length": (C.c_uint32, [C.c_uint32, C.c_float, C.c_float, C.c_float]),
@@ -115,6 +116,7 @@ SIGNATURES = {
     "fbp_refiner_load": (_S, [C.c_void_p, _P, C.c_uint64]),
     "fbp_refiner_save": (_S, [C.c_void_p, _P, C.c_uint64]),
     "fbp_refiner_forward": (_S, [C.c_void_p, _P, _P, _P]),
+    "fbp_learned_reconstruct": (_S, [C.c_void_p, C.POINTER(ReconstructionInfo), C.POINTER(Transient), _P, C.c_uint32, C.c_uint64, C.c_int32, C.c_float, C.c_uint32, _P, _P]),
     "fbp_refiner_train_step": (_S, [C.c_void_p, _P, _P, C.c_uint32, C.c_float, C.c_float, C.POINTER(C.c_float), _P]),
     "fbp_refiner_gradient": (_S, [C.c_void_p, _P, _P, C.c_float, _P, C.POINTER(C.c_float)]),
 }

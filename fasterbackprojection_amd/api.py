@@ -296,14 +296,26 @@ class Context:
         return out
 
     def backproject_confocal_mis(self, info, intensity, laser_targets, noise, num_samples,
-                                 mode=MIS_REFERENCE, out=None):
-        """Laser::reconstructAABBConfocalMIS (FB/Laser.cu:526-611)."""
+                                 mode=MIS_REFERENCE, out=None, seed=None, noise_size=None):
+        """Laser::reconstructAABBConfocalMIS (FB/Laser.cu:526-611).  With
+        noise=None (MIS_TILED only) the uniforms come from the seeded counter
+        hash over a virtual noise array of noise_size entries (default: the
+        number of voxels)."""
         _need_transient(info, intensity, STORAGE_F32, None, exhaustive=False)
         _need_points(laser_targets, "laser_targets", info.num_laser_targets)
-        _need(noise, "noise", torch.float32, min_numel=1)
+        if noise is None:
+            if mode != MIS_TILED or seed is None:
+                raise ValueError("noise=None needs mode=MIS_TILED and a seed")
+            r = info.voxel_resolution
+            n = int(noise_size) if noise_size is not None else r.x * r.y * r.z
+            if not 0 < n < 2 ** 32:
+                raise ValueError("noise_size out of range")
+            p = MisParams(int(num_samples), int(mode), None, n, int(seed) & (2 ** 64 - 1))
+        else:
+            _need(noise, "noise", torch.float32, min_numel=1)
+            p = MisParams(int(num_samples), int(mode), _ptr(noise), int(noise.numel()), 0)
         vol = self._volume(info, out)
         tr = self._transient(intensity, STORAGE_F32, LAYOUT_TIME_FASTEST, None)
-        p = MisParams(int(num_samples), int(mode), _ptr(noise), int(noise.numel()))
         self.check(self.lib.fbp_backproject_confocal_mis(
             self.handle, C.byref(info), C.byref(tr), _ptr(laser_targets), C.byref(p), _ptr(vol),
             _stream()))
@@ -376,6 +388,16 @@ class Context:
         Z, Y, X = volume.shape
         self.check(self.lib.fbp_filter_log(self.handle, _ptr(volume), _ptr(out), UVec3(X, Y, Z),
                                            kernel_size, float(sigma), _stream()))
+        return out
+
+    def log_normalize(self, volume, kernel_size=5, sigma=1.0, out=None):
+        """LoG then min-max normalisation (fbp_filter_log_normalize): the
+        results of log() followed by normalize() in two launches."""
+        out = self._filter_io(volume, out)
+        Z, Y, X = volume.shape
+        self.check(self.lib.fbp_filter_log_normalize(self.handle, _ptr(volume), _ptr(out),
+                                                     UVec3(X, Y, Z), kernel_size, float(sigma),
+                                                     _stream()))
         return out
 
     def log_fft(self, volume, sigma=1.0):
@@ -521,6 +543,27 @@ class Refiner:
             self._need_volume(out, "out")
         out = torch.empty_like(volume) if out is None else out
         self.ctx.check(self.lib.fbp_refiner_forward(self.handle, _ptr(volume), _ptr(out), _stream()))
+        return out
+
+    def reconstruct(self, info, intensity, laser_targets, num_samples, seed, out=None,
+                    kernel_size=5, sigma=1.0, extremes=16):
+        """Learned form in one call (fbp_learned_reconstruct): seeded
+        tile-shared MIS, LoG, refiner on its min-max normalisation,
+        calibration against that LoG."""
+        _need_transient(info, intensity, STORAGE_F32, None, exhaustive=False)
+        _need_points(laser_targets, "laser_targets", info.num_laser_targets)
+        r = info.voxel_resolution
+        if (r.x, r.y, r.z) != (self.size,) * 3:
+            raise ValueError(f"voxel grid must be {self.size}^3")
+        if out is not None:
+            self._need_volume(out, "out")
+        out = torch.empty((self.size,) * 3, dtype=torch.float32,
+                          device=intensity.device) if out is None else out
+        tr = self.ctx._transient(intensity, STORAGE_F32, LAYOUT_TIME_FASTEST, None)
+        self.ctx.check(self.lib.fbp_learned_reconstruct(
+            self.handle, C.byref(info), C.byref(tr), _ptr(laser_targets), int(num_samples),
+            int(seed) & (2 ** 64 - 1), int(kernel_size), float(sigma), int(extremes), _ptr(out),
+            _stream()))
         return out
 
     def train_step(self, inputs: torch.Tensor, targets: torch.Tensor, lr: float,

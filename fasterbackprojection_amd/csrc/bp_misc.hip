@@ -185,18 +185,42 @@ __global__ __launch_bounds__(1024) void row_prefix_kernel(const double* __restri
     }
 }
 
+// Entry p of the virtual noise array of a seeded draw: a uniform in [0, 1)
+// with 24 random bits from a counter hash of (seed, p) (the "lowbias32"
+// integer mix, three rounds).
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+__device__ __forceinline__ float seeded_noise(uint64_t seed, uint32_t p) {
+    const uint32_t key = mix32(uint32_t(seed) ^ mix32(uint32_t(seed >> 32) + 0x9e3779b9u));
+    return float(mix32(p ^ key) >> 8) * (1.0f / 16777216.0f);
+}
+
+// mis_uniform over noise, or over the seeded virtual array when noise is null.
+__device__ __forceinline__ float mis_uniform_or_seeded(const float* __restrict__ noise, uint32_t n,
+                                                       uint64_t seed, uint32_t& state) {
+    state = state * 747796405u + 2891336453u;
+    return noise ? noise[state % n] : seeded_noise(seed, state % n);
+}
+
 // Scan-point samples of every voxel tile for FBP_MIS_TILED.
 __global__ void mis_tile_samples_kernel(const double* __restrict__ sum, const double* __restrict__ cum,
                                         uint32_t L, const float* __restrict__ noise,
-                                        uint32_t noise_size, uint32_t samples, uint32_t tiles,
-                                        uint32_t* __restrict__ sample_l, float* __restrict__ sample_w) {
+                                        uint32_t noise_size, uint64_t seed, uint32_t samples,
+                                        uint32_t tiles, uint32_t* __restrict__ sample_l,
+                                        float* __restrict__ sample_w) {
     const size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (id >= size_t(tiles) * samples) return;
     const uint32_t tile = uint32_t(id / samples), i = uint32_t(id % samples);
     const double total = cum[L - 1];
     uint32_t state = tile * samples + i;
-    const float u0 = mis_uniform(noise, noise_size, state);
-    const float u1 = mis_uniform(noise, noise_size, state);
+    const float u0 = mis_uniform_or_seeded(noise, noise_size, seed, state);
+    const float u1 = mis_uniform_or_seeded(noise, noise_size, seed, state);
     uint32_t l;
     if (total > 0.0 && u0 < 0.5f) {
         const double target = double(u1) * total;
@@ -494,7 +518,8 @@ fbp_status fbp_build_alias_tables_host(const float* cdf, uint32_t n, uint32_t* a
 fbp_status fbp_backproject_confocal_mis(fbp_context ctx, const fbp_reconstruction_info* info,
                                         const fbp_transient* tr, const fbp_vec3* laser_targets,
                                         const fbp_mis_params* params, float* volume, void* stream) {
-    if (!ctx || !info || !tr || !tr->data || !laser_targets || !params || !params->noise || !volume)
+    if (!ctx || !info || !tr || !tr->data || !laser_targets || !params || !volume ||
+        (!params->noise && params->mode != FBP_MIS_TILED))
         return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "mis: null argument");
     if (tr->storage != FBP_STORAGE_F32 || tr->layout != FBP_LAYOUT_TIME_FASTEST)
         return fail(ctx, FBP_ERR_UNSUPPORTED, "mis: F32 [l][t] transient only");
@@ -518,8 +543,8 @@ fbp_status fbp_backproject_confocal_mis(fbp_context ctx, const fbp_reconstructio
                            static_cast<const float*>(tr->data), L, T, d_sum);
         hipLaunchKernelGGL(row_prefix_kernel, dim3(1), dim3(1024), 0, s, d_sum, L, d_cum);
         hipLaunchKernelGGL(mis_tile_samples_kernel, dim3(unsigned((ns + 255) / 256)), dim3(256), 0, s,
-                           d_sum, d_cum, L, params->noise, params->noise_size, params->num_samples,
-                           tiles, d_l, d_w);
+                           d_sum, d_cum, L, params->noise, params->noise_size, params->seed,
+                           params->num_samples, tiles, d_l, d_w);
         FBP_CHECK_HIP(ctx, hipGetLastError());
         return confocal_voxel_impl(ctx, info, tr, laser_targets, volume, stream, d_l, d_w,
                                    params->num_samples);

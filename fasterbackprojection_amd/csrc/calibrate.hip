@@ -16,11 +16,15 @@
 // out of the fit.  (2) The true range: the exact LoG at the k lowest and k
 // highest voxels of the refined volume (exact backprojections of their 5^3
 // clamp-to-edge neighbourhoods).  Output: (a v + b - min) / (max - min).
-// No host round trip.
+// No host round trip; four launches: one pass over the volume (the fit's
+// sums and each block's extreme candidates), one block selecting the k of
+// each side, the exact neighbourhoods, and the affine map (each block
+// solving the small fit itself).
 #include <algorithm>
 #include <vector>
 
 #include "bp_common.h"
+#include "topk.h"
 
 namespace fbp {
 
@@ -75,89 +79,152 @@ __global__ __launch_bounds__(256) void bp_points_kernel(PointArgs a, const uint3
     if (lane == 0) out[p] = acc;
 }
 
-// The k smallest (blockIdx.y = 0) or largest (1) (value, index) pairs of
-// each wave's segment of 64 * PER_LANE entries, by k rounds of a wave-wide
-// arg-min (ties by index).  idx == nullptr: the indices are the positions
-// in the input.  Entries beyond n never win while n >= k.
-template <int PER_LANE>
-__global__ __launch_bounds__(256) void wave_extremes_kernel(const float* __restrict__ in_lo,
-                                                            const uint32_t* __restrict__ idx_lo,
-                                                            const float* __restrict__ in_hi,
-                                                            const uint32_t* __restrict__ idx_hi,
-                                                            uint32_t n, uint32_t k,
-                                                            float* __restrict__ lo_v,
-                                                            uint32_t* __restrict__ lo_i,
-                                                            float* __restrict__ hi_v,
-                                                            uint32_t* __restrict__ hi_i) {
-    const bool high = blockIdx.y == 1;
-    const float* in = high ? in_hi : in_lo;
-    const uint32_t* in_idx = high ? idx_hi : idx_lo;
-    float* out_v = high ? hi_v : lo_v;
-    uint32_t* out_i = high ? hi_i : lo_i;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t seg = wave * (64u * PER_LANE);
-    if (seg >= n) return;  // whole wave
-    // The largest are the smallest of -value: one arg-min loop serves both.
-    const float sgn = high ? -1.f : 1.f;
-    float v[PER_LANE];
-    uint32_t id[PER_LANE];
+// ---- extremes: (value, index) pairs as 64-bit keys whose ascending order
+// is value ascending (lows) or descending (highs), ties by smaller index.
+constexpr int kScanChunk = 512;     // voxels per block_topk round (2 per thread)
+constexpr uint32_t kMaxScanBlocks = 1024;
+constexpr uint64_t kPadKey = kTopkPad;
+
+__device__ __forceinline__ uint32_t ordered_bits(float v) {
+    const uint32_t u = __float_as_uint(v);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ uint64_t key_low(float v, uint32_t i) {
+    return (uint64_t(ordered_bits(v)) << 32) | i;
+}
+__device__ __forceinline__ uint64_t key_high(float v, uint32_t i) {
+    return (uint64_t(~ordered_bits(v)) << 32) | i;
+}
+
+// One pass over the volume.  Block b owns voxels [b seg, (b + 1) seg):
+// (1) least-squares sums of reference r on volume v in FP64,
+// sums[b][0..4] = (n, sum v, sum v^2, sum r, sum v r); (2) its k lowest and
+// k highest keys, cand[0 or 1][b][0..k), over 512-voxel chunks merged with
+// the best k so far.
+__global__ __launch_bounds__(256) void calib_scan_kernel(
+    const float* __restrict__ v, const float* __restrict__ r, uint32_t n, uint32_t seg,
+    uint32_t k, uint64_t* __restrict__ cand, double* __restrict__ sums) {
+    __shared__ uint64_t blo[kMaxExtremes], bhi[kMaxExtremes];
+    __shared__ double red[5][256];
+    const uint32_t t = threadIdx.x, b = blockIdx.x, nb = gridDim.x;
+    const uint32_t begin = b * seg, end = min(n, begin + seg);
+    double acc[5] = {0, 0, 0, 0, 0};
+    bool first = true;
+    for (uint32_t c0 = begin; c0 < end; c0 += kScanChunk) {
+        uint64_t kl[2], kh[2];
 #pragma unroll
-    for (int j = 0; j < PER_LANE; ++j) {
-        const uint32_t g = seg + uint32_t(j) * 64 + lane;
-        const bool ok = g < n;
-        v[j] = ok ? sgn * in[g] : __builtin_inff();
-        id[j] = ok ? (in_idx ? in_idx[g] : g) : 0xffffffffu;
+        for (int j = 0; j < 2; ++j) {
+            const uint32_t i = c0 + t + 256u * uint32_t(j);
+            kl[j] = kh[j] = kPadKey;
+            if (i < end) {
+                const float x = v[i];
+                const double xd = x, yd = r[i];
+                acc[0] += 1.0;
+                acc[1] += xd;
+                acc[2] += xd * xd;
+                acc[3] += yd;
+                acc[4] += xd * yd;
+                kl[j] = key_low(x, i);
+                kh[j] = key_high(x, i);
+            }
+        }
+        block_topk<2, kMaxExtremes>(kl, k, blo, !first, blo);
+        block_topk<2, kMaxExtremes>(kh, k, bhi, !first, bhi);
+        first = false;
     }
-    auto better = [](float a, uint32_t ia, float b, uint32_t ib) {
-        return a < b || (a == b && ia < ib);
-    };
-    for (uint32_t r = 0; r < k; ++r) {
-        float bv = v[0];
-        uint32_t bi = id[0];
+    if (t < k) {
+        cand[size_t(b) * k + t] = blo[t];
+        cand[(size_t(nb) + b) * k + t] = bhi[t];
+    }
+    for (int q = 0; q < 5; ++q) red[q][t] = acc[q];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (int(t) < w)
+            for (int q = 0; q < 5; ++q) red[q][t] += red[q][t + w];
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int q = 0; q < 5; ++q) sums[size_t(b) * 5 + q] = red[q][0];
+}
+
+// One block after the scan over nb blocks: the fit sums reduced in a fixed
+// order into fit[0..4], and the global k of each side: the k blocks with
+// the smallest first keys hold all of them (a block outside those has k
+// smaller keys before its first), so the k smallest of their k x k keys
+// are the answer.  sel[0..k): lowest voxels, sel[k..2k): highest.
+__global__ __launch_bounds__(256) void calib_select_kernel(const uint64_t* __restrict__ cand,
+                                                           const double* __restrict__ sums,
+                                                           uint32_t nb, uint32_t seg, uint32_t k,
+                                                           uint32_t* __restrict__ sel,
+                                                           double* __restrict__ fit) {
+    __shared__ double red[5][256];
+    __shared__ uint64_t sl[kMaxExtremes], sh[kMaxExtremes];
+    __shared__ uint32_t blo[kMaxExtremes], bhi[kMaxExtremes];
+    const uint32_t t = threadIdx.x;
+    // Fit sums: thread t adds blocks t, t + 256, ... in order, then the tree.
+    double s5[5] = {0, 0, 0, 0, 0};
+    for (uint32_t i = t; i < nb; i += 256)
+        for (int q = 0; q < 5; ++q) s5[q] += sums[size_t(i) * 5 + q];
+    for (int q = 0; q < 5; ++q) red[q][t] = s5[q];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (int(t) < w)
+            for (int q = 0; q < 5; ++q) red[q][t] += red[q][t + w];
+        __syncthreads();
+    }
+    if (t < 5) fit[t] = red[t][0];
+    // The first key of every block (nb <= 1024: 4 per thread).
+    uint64_t a[4], c[4];
 #pragma unroll
-        for (int j = 1; j < PER_LANE; ++j)
-            if (better(v[j], id[j], bv, bi)) {
-                bv = v[j];
-                bi = id[j];
-            }
-        for (int off = 32; off > 0; off >>= 1) {
-            const float ov = __shfl_xor(bv, off, 64);
-            const uint32_t oi = __shfl_xor(bi, off, 64);
-            if (better(ov, oi, bv, bi)) {
-                bv = ov;
-                bi = oi;
-            }
-        }
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t i = t + 256u * uint32_t(j);
+        a[j] = i < nb ? cand[size_t(i) * k] : kPadKey;
+        c[j] = i < nb ? cand[(size_t(nb) + i) * k] : kPadKey;
+    }
+    block_topk<4, kMaxExtremes>(a, k, nullptr, false, sl);
+    block_topk<4, kMaxExtremes>(c, k, nullptr, false, sh);
+    // (fewer than k blocks: the pads select no block)
+    constexpr uint32_t kNoBlock = 0xffffffffu;
+    if (t < k) {
+        blo[t] = sl[t] == kPadKey ? kNoBlock : uint32_t(sl[t] & 0xffffffffu) / seg;
+        bhi[t] = sh[t] == kPadKey ? kNoBlock : uint32_t(sh[t] & 0xffffffffu) / seg;
+    }
+    __syncthreads();
+    // Their k x k keys (k <= 32: 4 per thread).
 #pragma unroll
-        for (int j = 0; j < PER_LANE; ++j)
-            if (id[j] == bi && bi != 0xffffffffu) {
-                v[j] = __builtin_inff();
-                id[j] = 0xffffffffu;
-            }
-        if (lane == 0) {
-            out_v[size_t(wave) * k + r] = sgn * bv;
-            out_i[size_t(wave) * k + r] = bi;
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t i = t + 256u * uint32_t(j);
+        a[j] = c[j] = kPadKey;
+        if (i < k * k) {
+            const uint32_t bb = i / k, m = i % k;
+            if (blo[bb] != kNoBlock) a[j] = cand[size_t(blo[bb]) * k + m];
+            if (bhi[bb] != kNoBlock) c[j] = cand[(size_t(nb) + bhi[bb]) * k + m];
         }
+    }
+    block_topk<4, kMaxExtremes>(a, k, nullptr, false, sl);
+    block_topk<4, kMaxExtremes>(c, k, nullptr, false, sh);
+    if (t < k) {
+        sel[t] = uint32_t(sl[t] & 0xffffffffu);
+        sel[k + t] = uint32_t(sh[t] & 0xffffffffu);
     }
 }
 
-// Exact LoG contribution of one chunk of scan points at extreme candidate ci
-// (ci < k: lows, else highs): block (ci, chunk); thread t < 125 backprojects
-// neighbour t = (dz, dy, dx) of the candidate (clamped to the volume) over
-// the chunk.  All threads read the same scan point's row at nearby bins
-// (coalesced).  partial[ci * chunks + chunk] = sum_t w[t] bp_t (FP64, fixed
-// order).
+// Exact LoG contribution of one chunk of scan points at candidate ci
+// (ci < k: lows, else highs): block (ci, chunk); thread t < 125
+// backprojects neighbour t = (dz, dy, dx) of the candidate (clamped to the
+// volume) over the chunk, eight scan points per round with every load of
+// the round issued first.  All threads read the same scan point's row at
+// nearby bins (coalesced).  partial[ci * chunks + chunk] = sum_t w[t] bp_t
+// (FP64, fixed order).
 __global__ __launch_bounds__(128) void calib_neigh_kernel(PointArgs a, Weights w,
-                                                          const uint32_t* __restrict__ lo,
-                                                          const uint32_t* __restrict__ hi, uint32_t k,
-                                                          uint32_t chunk_len,
+                                                          const uint32_t* __restrict__ sel,
+                                                          uint32_t k, uint32_t chunk_len,
                                                           double* __restrict__ partial) {
     __shared__ double red[128];
     const uint32_t ci = blockIdx.x, chunk = blockIdx.y, t = threadIdx.x;
     double acc = 0.0;
     if (t < uint32_t(kTaps)) {
-        const uint32_t c = ci < k ? lo[ci] : hi[ci - k];
+        const uint32_t c = sel[ci];
         const int x = int(c % a.X), y = int((c / a.X) % a.Y), z = int(c / (a.X * a.Y));
         const int nx = min(max(x + int(t % 5) - 2, 0), int(a.X) - 1);
         const int ny = min(max(y + int((t / 5) % 5) - 2, 0), int(a.Y) - 1);
@@ -166,7 +233,29 @@ __global__ __launch_bounds__(128) void calib_neigh_kernel(PointArgs a, Weights w
         const float py = voxel_center(a.hmin[1], a.vs[1], uint32_t(nz));
         const float pz = voxel_center(a.hmin[2], a.vs[2], uint32_t(ny));
         const uint32_t l0 = chunk * chunk_len, l1 = min(a.L, l0 + chunk_len);
-        for (uint32_t l = l0; l < l1; ++l) acc += point_sample(a, l, px, py, pz);
+        // point_sample's arithmetic, branch-free so the round's loads overlap.
+        constexpr int U = 8;
+        for (uint32_t lb = l0; lb < l1; lb += U) {
+            float light[U], wd[U];
+            bool ok[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t l = min(lb + uint32_t(u), l1 - 1);
+                const fbp_vec3 lp = a.laser[l];
+                float dist = rn_sub(rn_mul(distance3(lp.x, lp.y, lp.z, px, py, pz), 2.0f), a.t0);
+                if (a.discard == 0)
+                    dist = rn_add(dist,
+                                  rn_add(distance3(lp.x, lp.y, lp.z, a.lpos.x, a.lpos.y, a.lpos.z),
+                                         distance3(lp.x, lp.y, lp.z, a.spos.x, a.spos.y, a.spos.z)));
+                const float q = rn_div(dist, a.dt);
+                ok[u] = lb + uint32_t(u) < l1 && q > -1.0f && q < float(a.T);
+                light[u] = a.I[size_t(l) * a.T + uint32_t(ok[u] ? int(q) : 0)];
+                wd[u] = safe_rcp(rn_mul(dist, dist));
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (ok[u] && light[u] > kEps) acc += double(rn_mul(light[u], wd[u]));
+        }
         acc *= double(w.w[t]);
     }
     red[t] = acc;
@@ -178,75 +267,43 @@ __global__ __launch_bounds__(128) void calib_neigh_kernel(PointArgs a, Weights w
     if (t == 0) partial[size_t(ci) * gridDim.y + chunk] = red[0];
 }
 
-// Least-squares sums of reference (r) on volume (v): per block, in FP64:
-// sums[block][0..4] = (n, sum v, sum v^2, sum r, sum v r).
-constexpr int kFitBlocks = 256;
-__global__ __launch_bounds__(256) void calib_sums_kernel(const float* __restrict__ v,
-                                                         const float* __restrict__ r, size_t n,
-                                                         double* __restrict__ sums) {
-    __shared__ double red[256][5];
-    double a[5] = {0, 0, 0, 0, 0};
-    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-         i += size_t(gridDim.x) * blockDim.x) {
-        const double x = v[i], y = r[i];
-        a[0] += 1.0;
-        a[1] += x;
-        a[2] += x * x;
-        a[3] += y;
-        a[4] += x * y;
-    }
-    for (int q = 0; q < 5; ++q) red[threadIdx.x][q] = a[q];
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (int(threadIdx.x) < s)
-            for (int q = 0; q < 5; ++q) red[threadIdx.x][q] += red[threadIdx.x + s][q];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-        for (int q = 0; q < 5; ++q) sums[blockIdx.x * 5 + q] = red[0][q];
-}
-
-// One block: L ~ a v + b from the block sums, the exact LoG of each
-// candidate from its chunk partials, the true range over the candidates;
-// coef = (a / range, (b - min) / range) so that out = coef0 v + coef1.
-__global__ __launch_bounds__(256) void calib_fit_kernel(const double* __restrict__ sums,
-                                                        const double* __restrict__ partial,
-                                                        uint32_t chunks, uint32_t k,
-                                                        float* __restrict__ coef) {
-    __shared__ double red[256][5];
+// Every block solves the fit (the same values in every block): L ~ a v + b
+// from fit[0..4], the exact LoG of each candidate from its chunk partials,
+// the true range over the candidates; then out = coef0 v + coef1 with
+// coef = (a / range, (b - min) / range).
+__global__ __launch_bounds__(256) void calib_apply_kernel(float* __restrict__ v, size_t n,
+                                                          const double* __restrict__ fit,
+                                                          const double* __restrict__ partial,
+                                                          uint32_t k, uint32_t chunks) {
     __shared__ double lval[2 * kMaxExtremes];
+    __shared__ float coef[2];
     const uint32_t t = threadIdx.x;
-    for (int q = 0; q < 5; ++q) red[t][q] = t < uint32_t(kFitBlocks) ? sums[t * 5 + q] : 0.0;
     if (t < 2 * k) {
         double l = 0.0;
         for (uint32_t c = 0; c < chunks; ++c) l += partial[size_t(t) * chunks + c];
         lval[t] = l;
     }
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (int(t) < s)
-            for (int q = 0; q < 5; ++q) red[t][q] += red[t + s][q];
-        __syncthreads();
+    if (t == 0) {
+        double S[5];
+        for (int q = 0; q < 5; ++q) S[q] = fit[q];
+        const double det = S[0] * S[2] - S[1] * S[1];
+        const double av = det > 0.0 ? (S[0] * S[4] - S[1] * S[3]) / det : 0.0;
+        const double bv = S[0] > 0.0 ? (S[3] - av * S[1]) / S[0] : 0.0;
+        double mn = lval[0], mx = lval[k];
+        for (uint32_t i = 1; i < k; ++i) mn = fmin(mn, lval[i]);
+        for (uint32_t i = k + 1; i < 2 * k; ++i) mx = fmax(mx, lval[i]);
+        const double range = mx - mn;
+        // safeRCP of the range, as the min-max normalisation (FB/Laser.cu:317-339)
+        const double inv = (range > 1e-5 || range < -1e-5) ? 1.0 / range : 3.402823466e+38;
+        coef[0] = float(av * inv);
+        coef[1] = float((bv - mn) * inv);
     }
-    if (t != 0) return;
-    const double* S = red[0];
-    const double det = S[0] * S[2] - S[1] * S[1];
-    const double a = det > 0.0 ? (S[0] * S[4] - S[1] * S[3]) / det : 0.0;
-    const double b = S[0] > 0.0 ? (S[3] - a * S[1]) / S[0] : 0.0;
-    double mn = lval[0], mx = lval[k];
-    for (uint32_t i = 1; i < k; ++i) mn = fmin(mn, lval[i]);
-    for (uint32_t i = k + 1; i < 2 * k; ++i) mx = fmax(mx, lval[i]);
-    const double range = mx - mn;
-    // safeRCP of the range, as the min-max normalisation (FB/Laser.cu:317-339)
-    const double inv = (range > 1e-5 || range < -1e-5) ? 1.0 / range : 3.402823466e+38;
-    coef[0] = float(a * inv);
-    coef[1] = float((b - mn) * inv);
-}
-
-__global__ void calib_apply_kernel(float* __restrict__ v, size_t n, const float* __restrict__ coef) {
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    __syncthreads();
+    const size_t i = size_t(blockIdx.x) * blockDim.x + t;
     if (i < n) v[i] = fmaf(coef[0], v[i], coef[1]);
 }
+
 
 PointArgs point_args(const fbp_reconstruction_info& info, const float* I, const fbp_vec3* laser) {
     PointArgs a{};
@@ -309,11 +366,12 @@ extern "C" fbp_status fbp_backproject_confocal_points(fbp_context ctx,
     return FBP_OK;
 }
 
-extern "C" fbp_status fbp_learned_calibrate(fbp_context ctx, const fbp_reconstruction_info* info,
-                                            const fbp_transient* tr,
-                                            const fbp_vec3* laser_targets, float* volume,
-                                            const float* reference, int32_t kernel_size,
-                                            float sigma, uint32_t extremes, void* stream) {
+namespace fbp {
+
+fbp_status learned_calibrate_impl(fbp_context ctx, const fbp_reconstruction_info* info,
+                                  const fbp_transient* tr, const fbp_vec3* laser_targets,
+                                  float* volume, const float* reference, int32_t kernel_size,
+                                  float sigma, uint32_t extremes, hipStream_t s) {
     if (!ctx || !info || !tr || !tr->data || !laser_targets || !volume || !reference)
         return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "calibrate: null argument");
     FBP_CHECK(check_confocal_f32(ctx, info, tr, "calibrate"));
@@ -322,7 +380,6 @@ extern "C" fbp_status fbp_learned_calibrate(fbp_context ctx, const fbp_reconstru
     if (extremes == 0 || extremes > uint32_t(kMaxExtremes))
         return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "calibrate: 1 <= extremes <= %d", kMaxExtremes);
     FBP_CHECK(bind(ctx));
-    hipStream_t s = as_stream(stream);
 
     Weights w{};
     FBP_CHECK(fbp_log_kernel_host(kernel_size, sigma, w.w));
@@ -340,10 +397,14 @@ extern "C" fbp_status fbp_learned_calibrate(fbp_context ctx, const fbp_reconstru
     const uint32_t chunk_len = (L + chunks - 1) / chunks;
     chunks = (L + chunk_len - 1) / chunk_len;
 
-    const uint32_t w0 = uint32_t((n + 255) / 256);  // level-0 waves (4 entries per lane)
-    const size_t cand = size_t(w0) * k;
-    const size_t bytes = 4 * 2 * align256(cand * 4) + align256(size_t(2) * k * chunks * 8) +
-                         align256(size_t(kFitBlocks) * 5 * 8) + 256;
+    // Volume segments of the scan pass: at most kMaxScanBlocks blocks.
+    uint32_t nb = uint32_t(std::min<size_t>(kMaxScanBlocks, (n + kScanChunk - 1) / kScanChunk));
+    const uint32_t seg = uint32_t((n + nb - 1) / nb);
+    nb = uint32_t((n + seg - 1) / seg);
+
+    const size_t bytes = align256(size_t(2) * nb * k * 8) + align256(size_t(nb) * 5 * 8) +
+                         align256(size_t(2) * k * 4) + align256(5 * 8) +
+                         align256(size_t(2) * k * chunks * 8) + 256;
     FBP_CHECK(ensure_scratch(ctx, bytes));
     char* cur = static_cast<char*>(ctx->scratch);
     auto take = [&](size_t b) {
@@ -351,40 +412,32 @@ extern "C" fbp_status fbp_learned_calibrate(fbp_context ctx, const fbp_reconstru
         cur += align256(b);
         return p;
     };
-    float* lo_v[2] = {reinterpret_cast<float*>(take(cand * 4)), reinterpret_cast<float*>(take(cand * 4))};
-    uint32_t* lo_i[2] = {reinterpret_cast<uint32_t*>(take(cand * 4)),
-                         reinterpret_cast<uint32_t*>(take(cand * 4))};
-    float* hi_v[2] = {reinterpret_cast<float*>(take(cand * 4)), reinterpret_cast<float*>(take(cand * 4))};
-    uint32_t* hi_i[2] = {reinterpret_cast<uint32_t*>(take(cand * 4)),
-                         reinterpret_cast<uint32_t*>(take(cand * 4))};
+    uint64_t* cand = reinterpret_cast<uint64_t*>(take(size_t(2) * nb * k * 8));
+    double* sums = reinterpret_cast<double*>(take(size_t(nb) * 5 * 8));
+    uint32_t* sel = reinterpret_cast<uint32_t*>(take(size_t(2) * k * 4));
+    double* fit = reinterpret_cast<double*>(take(5 * 8));
     double* partial = reinterpret_cast<double*>(take(size_t(2) * k * chunks * 8));
-    double* sums = reinterpret_cast<double*>(take(size_t(kFitBlocks) * 5 * 8));
-    float* coef = reinterpret_cast<float*>(take(256));
 
-    // Extreme candidates: per-wave selection (4 entries per lane: short
-    // rounds, many waves), repeated until one wave holds the k of each side.
-    auto blocks = [](uint32_t waves) { return dim3((waves + 3) / 4, 2); };
-    hipLaunchKernelGGL(wave_extremes_kernel<4>, blocks(w0), dim3(256), 0, s, volume, nullptr,
-                       volume, nullptr, uint32_t(n), k, lo_v[0], lo_i[0], hi_v[0], hi_i[0]);
-    uint32_t m = w0 * k;
-    int cb = 0;
-    while (m > k) {
-        const uint32_t wv = (m + 255) / 256;
-        hipLaunchKernelGGL(wave_extremes_kernel<4>, blocks(wv), dim3(256), 0, s, lo_v[cb],
-                           lo_i[cb], hi_v[cb], hi_i[cb], m, k, lo_v[cb ^ 1], lo_i[cb ^ 1],
-                           hi_v[cb ^ 1], hi_i[cb ^ 1]);
-        m = wv * k;
-        cb ^= 1;
-    }
-
+    hipLaunchKernelGGL(calib_scan_kernel, dim3(nb), dim3(256), 0, s, volume, reference,
+                       uint32_t(n), seg, k, cand, sums);
+    hipLaunchKernelGGL(calib_select_kernel, dim3(1), dim3(256), 0, s, cand, sums, nb, seg, k, sel,
+                       fit);
     const PointArgs a = point_args(*info, static_cast<const float*>(tr->data), laser_targets);
-    hipLaunchKernelGGL(calib_neigh_kernel, dim3(2 * k, chunks), dim3(128), 0, s, a, w, lo_i[cb],
-                       hi_i[cb], k, chunk_len, partial);
-    hipLaunchKernelGGL(calib_sums_kernel, dim3(kFitBlocks), dim3(256), 0, s, volume, reference, n,
-                       sums);
-    hipLaunchKernelGGL(calib_fit_kernel, dim3(1), dim3(256), 0, s, sums, partial, chunks, k, coef);
+    hipLaunchKernelGGL(calib_neigh_kernel, dim3(2 * k, chunks), dim3(128), 0, s, a, w, sel, k,
+                       chunk_len, partial);
     hipLaunchKernelGGL(calib_apply_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, volume,
-                       n, coef);
+                       n, fit, partial, k, chunks);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     return FBP_OK;
+}
+
+}  // namespace fbp
+
+extern "C" fbp_status fbp_learned_calibrate(fbp_context ctx, const fbp_reconstruction_info* info,
+                                            const fbp_transient* tr,
+                                            const fbp_vec3* laser_targets, float* volume,
+                                            const float* reference, int32_t kernel_size,
+                                            float sigma, uint32_t extremes, void* stream) {
+    return fbp::learned_calibrate_impl(ctx, info, tr, laser_targets, volume, reference,
+                                       kernel_size, sigma, extremes, as_stream(stream));
 }

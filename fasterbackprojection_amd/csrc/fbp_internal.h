@@ -44,6 +44,7 @@ struct fbp_context_s {
     void* noalloc_scratch = nullptr;
 };
 
+
 namespace fbp {
 
 // Records a message on the context and returns the status.
@@ -51,6 +52,49 @@ fbp_status fail(fbp_context ctx, fbp_status st, const char* fmt, ...);
 
 // hipError -> fbp_status with a message naming the call.
 fbp_status check_hip(fbp_context ctx, hipError_t e, const char* what);
+
+// Per-block (min, max) pairs of a volume: part[2 i], part[2 i + 1], i < count.
+struct MinMaxPartials {
+    float* part = nullptr;
+    uint32_t count = 0;
+};
+
+// LoG (filters.hip); with mm, also the output's per-block min / max pairs
+// (in ctx->scratch: valid until the next use of the scratch arena).
+fbp_status log_impl(fbp_context ctx, const float* in, float* out, fbp_uvec3 size,
+                    int32_t kernel_size, float sigma, hipStream_t s, MinMaxPartials* mm);
+
+// Min and max over per-block pairs, reduced by the whole block (every
+// thread gets the result).  Order-independent, so every block of a grid
+// gets the same values.
+__device__ __forceinline__ void block_minmax(const float* __restrict__ part, uint32_t count,
+                                             float& mn, float& mx) {
+    __shared__ float smn[256], smx[256];
+    float a = 3.402823466e+38f, b = -3.402823466e+38f;
+    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) {
+        a = fminf(a, part[2 * i]);
+        b = fmaxf(b, part[2 * i + 1]);
+    }
+    smn[threadIdx.x] = a;
+    smx[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = int(blockDim.x) / 2; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) {
+            smn[threadIdx.x] = fminf(smn[threadIdx.x], smn[threadIdx.x + w]);
+            smx[threadIdx.x] = fmaxf(smx[threadIdx.x], smx[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    mn = smn[0];
+    mx = smx[0];
+    __syncthreads();
+}
+
+// fbp_learned_calibrate on a stream (calibrate.hip).
+fbp_status learned_calibrate_impl(fbp_context ctx, const fbp_reconstruction_info* info,
+                                  const fbp_transient* tr, const fbp_vec3* laser_targets,
+                                  float* volume, const float* reference, int32_t kernel_size,
+                                  float sigma, uint32_t extremes, hipStream_t s);
 
 // Ensures ctx->scratch holds at least `bytes` (device).  Synchronises the
 // device only when it has to grow.

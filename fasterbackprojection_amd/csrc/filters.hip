@@ -133,10 +133,14 @@ constexpr int kRedBlocks = 1024;
 // (plus halos from L2) and one write instead of the intermediate volumes.
 constexpr int kFT = 16;  // tile edge (x and y); 256 threads
 
-template <int H>
+// Optional epilogue of the fused LoG: the min and max of the output, one
+// pair per block in part[2 b], part[2 b + 1] (reduced by the consumer:
+// block_minmax in fbp_internal.h).
+template <int H, bool MM>
 __global__ __launch_bounds__(256) void log_fused_kernel(const float* __restrict__ in,
                                                         float* __restrict__ out, AxisInfo a, Taps t,
-                                                        float a4, float a2, float c, int zchunk) {
+                                                        float a4, float a2, float c, int zchunk,
+                                                        float* __restrict__ mm_part) {
     constexpr int K = 2 * H + 1, PW = kFT + 2 * H, PH = kFT + 2 * H;
     __shared__ float pl[PH][PW + 1];
     __shared__ float sg[PH][kFT], sh[PH][kFT], sb[PH][kFT];
@@ -150,6 +154,7 @@ __global__ __launch_bounds__(256) void log_fused_kernel(const float* __restrict_
         tg[d] = t.g[d];
         th[d] = t.h[d];
     }
+    float lmn = FLT_MAX, lmx = -FLT_MAX;
     float rgg[K], rhg[K], rgh[K], rbb[K];
 #pragma unroll
     for (int d = 0; d < K; ++d) rgg[d] = rhg[d] = rgh[d] = rbb[d] = 0.f;
@@ -224,9 +229,32 @@ __global__ __launch_bounds__(256) void log_fused_kernel(const float* __restrict_
                 s_g = fmaf(tg[d], rgg[d], s_g);
                 s_b += rbb[d];
             }
-            out[(size_t(zo) * Y + oy) * X + ox] = (s_r2 * a4 - s_g * a2) - c * s_b;
+            const float val = (s_r2 * a4 - s_g * a2) - c * s_b;
+            out[(size_t(zo) * Y + oy) * X + ox] = val;
+            if constexpr (MM) {
+                lmn = fminf(lmn, val);
+                lmx = fmaxf(lmx, val);
+            }
         }
         __syncthreads();
+    }
+    if constexpr (MM) {
+        __shared__ float smn[256], smx[256];
+        smn[tid] = lmn;
+        smx[tid] = lmx;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (tid < w) {
+                smn[tid] = fminf(smn[tid], smn[tid + w]);
+                smx[tid] = fmaxf(smx[tid], smx[tid + w]);
+            }
+            __syncthreads();
+        }
+        const uint32_t bid = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        if (tid == 0) {
+            mm_part[2 * bid] = smn[0];
+            mm_part[2 * bid + 1] = smx[0];
+        }
     }
 }
 
@@ -386,6 +414,103 @@ static fbp_status run_minmax(fbp_context ctx, const float* v, size_t n, hipStrea
 
 static dim3 grid_for(size_t n) { return dim3(unsigned((n + 255) / 256)); }
 
+// v = (v - min) * safeRCP(max - min) with min / max reduced from per-block
+// pairs (every block reduces them itself).
+__global__ __launch_bounds__(256) void normalize_partials_kernel(float* __restrict__ v, size_t n,
+                                                                 const float* __restrict__ part,
+                                                                 uint32_t count) {
+    float mn, mx;
+    block_minmax(part, count, mn, mx);
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = rn_mul(rn_sub(v[i], mn), safe_rcp(rn_sub(mx, mn)));
+}
+
+// LoG of in into out; with mm, also per-block min / max pairs of the output
+// (mm->part in ctx->scratch, mm->count pairs), from the fused kernel's
+// epilogue or, on the dense path, a min/max pass.
+fbp_status log_impl(fbp_context ctx, const float* in, float* out, fbp_uvec3 size,
+                    int32_t kernel_size, float sigma, hipStream_t s, MinMaxPartials* mm) {
+    if (!ctx || !in || !out || size.x == 0 || size.y == 0 || size.z == 0)
+        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "log: bad argument");
+    if (in == out) return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "log: in and out must differ");
+    if (kernel_size < 1 || kernel_size > kMaxTaps || !(sigma > 0.f))
+        return fail(ctx, FBP_ERR_UNSUPPORTED, "log: kernel_size in [1, %d], sigma > 0", kMaxTaps);
+    FBP_CHECK(bind(ctx));
+    const AxisInfo a{size.x, size.y, size.z};
+    const size_t n = size_t(size.x) * size.y * size.z;
+    const int k = kernel_size, half = (k - 1) / 2;
+    // Separable iff no Gaussian tap falls below FLT_EPSILON * max (the
+    // corner tap is the smallest), and the kernel is odd-sized.
+    const float std2 = sigma * sigma;
+    const float corner = float(half);
+    const float wc = expf(-(3.0f * corner * corner) / (2.0f * std2));
+    const bool separable = (k % 2 == 1) && !(wc < FLT_EPSILON * 1.0f);
+    if (separable) {
+        Taps t{};
+        t.half = half;
+        double e[kMaxTaps], se = 0.0;
+        for (int d = -half; d <= half; ++d) {
+            e[d + half] = std::exp(-double(d) * d / (2.0 * double(std2)));
+            se += e[d + half];
+        }
+        for (int d = -half; d <= half; ++d) {
+            t.g[d + half] = float(e[d + half] / se);
+            t.h[d + half] = float(double(d) * d * e[d + half] / se);
+        }
+        // c = mean of the un-centred kernel w1 = ghat^3 (r^2 - 3 s^2) / s^4
+        //   = 3 (m2 - s^2) / (s^4 k^3),  m2 = sum_x x^2 ghat(x).
+        double m2 = 0.0;
+        for (int d = -half; d <= half; ++d) m2 += double(d) * d * e[d + half] / se;
+        const double s2 = double(std2);
+        const float c = float(3.0 * (m2 - s2) / (s2 * s2 * double(k) * k * k));
+        const float a4 = float(1.0 / (double(std2) * std2));
+        const float a2 = float(3.0 / double(std2));
+        int zchunk = 0;
+        const dim3 grid = fused_grid(ctx, a, half, zchunk);
+        float* part = nullptr;
+        if (mm) {
+            mm->count = grid.x * grid.y * grid.z;
+            FBP_CHECK(ensure_scratch(ctx, size_t(2) * mm->count * sizeof(float)));
+            part = static_cast<float*>(ctx->scratch);
+            mm->part = part;
+        }
+        switch (half) {  // k <= kMaxTaps = 15: half <= 7
+#define FBP_LOG(HH)                                                                                \
+    case HH:                                                                                       \
+        if (mm)                                                                                    \
+            hipLaunchKernelGGL((log_fused_kernel<HH, true>), grid, dim3(256), 0, s, in, out, a, t, \
+                               a4, a2, c, zchunk, part);                                           \
+        else                                                                                       \
+            hipLaunchKernelGGL((log_fused_kernel<HH, false>), grid, dim3(256), 0, s, in, out, a,   \
+                               t, a4, a2, c, zchunk, part);                                        \
+        break;
+            FBP_LOG(0) FBP_LOG(1) FBP_LOG(2) FBP_LOG(3) FBP_LOG(4) FBP_LOG(5) FBP_LOG(6) FBP_LOG(7)
+#undef FBP_LOG
+        }
+    } else {
+        std::vector<float> w(size_t(k) * k * k);
+        fbp_log_kernel_host(k, sigma, w.data());
+        FBP_CHECK(ensure_scratch(ctx, w.size() * sizeof(float)));
+        float* dw = static_cast<float*>(ctx->scratch);
+        FBP_CHECK_HIP(ctx, hipMemcpyAsync(dw, w.data(), w.size() * sizeof(float),
+                                          hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(log_dense_kernel, grid_for(n), dim3(256), 0, s, in, out, dw, a, k, half);
+        // w is a host vector consumed by the async copy.
+        FBP_CHECK_HIP(ctx, hipStreamSynchronize(s));
+        if (mm) {
+            size_t blocks = (n + 255) / 256;
+            if (blocks > kRedBlocks) blocks = kRedBlocks;
+            FBP_CHECK(ensure_scratch(ctx, 2 * blocks * sizeof(float)));
+            mm->part = static_cast<float*>(ctx->scratch);
+            mm->count = uint32_t(blocks);
+            hipLaunchKernelGGL(minmax_partial_kernel, dim3(unsigned(blocks)), dim3(256), 0, s, out,
+                               n, mm->part);
+        }
+    }
+    FBP_CHECK_HIP(ctx, hipGetLastError());
+    return FBP_OK;
+}
+
 }  // namespace fbp
 
 using namespace fbp;
@@ -473,64 +598,21 @@ fbp_status fbp_filter_laplacian(fbp_context ctx, const float* in, float* out, fb
 
 fbp_status fbp_filter_log(fbp_context ctx, const float* in, float* out, fbp_uvec3 size,
                           int32_t kernel_size, float sigma, void* stream) {
-    if (!ctx || !in || !out || size.x == 0 || size.y == 0 || size.z == 0)
-        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "log: bad argument");
-    if (in == out) return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "log: in and out must differ");
-    if (kernel_size < 1 || kernel_size > kMaxTaps || !(sigma > 0.f))
-        return fail(ctx, FBP_ERR_UNSUPPORTED, "log: kernel_size in [1, %d], sigma > 0", kMaxTaps);
+    return log_impl(ctx, in, out, size, kernel_size, sigma, as_stream(stream), nullptr);
+}
+
+fbp_status fbp_filter_log_normalize(fbp_context ctx, const float* in, float* out,
+                                    fbp_uvec3 size, int32_t kernel_size, float sigma,
+                                    void* stream) {
+    if (!ctx || size.x == 0 || size.y == 0 || size.z == 0)
+        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "log_normalize: bad argument");
     FBP_CHECK(bind(ctx));
-    const AxisInfo a{size.x, size.y, size.z};
-    const size_t n = size_t(size.x) * size.y * size.z;
     hipStream_t s = as_stream(stream);
-    const int k = kernel_size, half = (k - 1) / 2;
-    // Separable iff no Gaussian tap falls below FLT_EPSILON * max (the
-    // corner tap is the smallest), and the kernel is odd-sized.
-    const float std2 = sigma * sigma;
-    const float corner = float(half);
-    const float wc = expf(-(3.0f * corner * corner) / (2.0f * std2));
-    const bool separable = (k % 2 == 1) && !(wc < FLT_EPSILON * 1.0f);
-    if (separable) {
-        Taps t{};
-        t.half = half;
-        double e[kMaxTaps], se = 0.0;
-        for (int d = -half; d <= half; ++d) {
-            e[d + half] = std::exp(-double(d) * d / (2.0 * double(std2)));
-            se += e[d + half];
-        }
-        for (int d = -half; d <= half; ++d) {
-            t.g[d + half] = float(e[d + half] / se);
-            t.h[d + half] = float(double(d) * d * e[d + half] / se);
-        }
-        // c = mean of the un-centred kernel w1 = ghat^3 (r^2 - 3 s^2) / s^4
-        //   = 3 (m2 - s^2) / (s^4 k^3),  m2 = sum_x x^2 ghat(x).
-        double m2 = 0.0;
-        for (int d = -half; d <= half; ++d) m2 += double(d) * d * e[d + half] / se;
-        const double s2 = double(std2);
-        const float c = float(3.0 * (m2 - s2) / (s2 * s2 * double(k) * k * k));
-        const float a4 = float(1.0 / (double(std2) * std2));
-        const float a2 = float(3.0 / double(std2));
-        int zchunk = 0;
-        const dim3 grid = fused_grid(ctx, a, half, zchunk);
-        switch (half) {  // k <= kMaxTaps = 15: half <= 7
-#define FBP_LOG(HH)                                                                              \
-    case HH:                                                                                     \
-        hipLaunchKernelGGL(log_fused_kernel<HH>, grid, dim3(256), 0, s, in, out, a, t, a4, a2, c, \
-                           zchunk);                                                              \
-        break;
-            FBP_LOG(0) FBP_LOG(1) FBP_LOG(2) FBP_LOG(3) FBP_LOG(4) FBP_LOG(5) FBP_LOG(6) FBP_LOG(7)
-#undef FBP_LOG
-        }
-    } else {
-        std::vector<float> w(size_t(k) * k * k);
-        fbp_log_kernel_host(k, sigma, w.data());
-        FBP_CHECK(ensure_scratch(ctx, w.size() * sizeof(float)));
-        float* dw = static_cast<float*>(ctx->scratch);
-        FBP_CHECK_HIP(ctx, hipMemcpyAsync(dw, w.data(), w.size() * sizeof(float),
-                                          hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(log_dense_kernel, grid_for(n), dim3(256), 0, s, in, out, dw, a, k, half);
-        // w is a host vector consumed by the async copy.
-        FBP_CHECK_HIP(ctx, hipStreamSynchronize(s));
-    }
+    const size_t n = size_t(size.x) * size.y * size.z;
+    MinMaxPartials mm{};
+    FBP_CHECK(log_impl(ctx, in, out, size, kernel_size, sigma, s, &mm));
+    hipLaunchKernelGGL(normalize_partials_kernel, grid_for(n), dim3(256), 0, s, out, n, mm.part,
+                       mm.count);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     return FBP_OK;
 }

@@ -15,12 +15,14 @@
 // Training (backward, L1 + SSIM loss, Adam) runs on FP32 VALU kernels and
 // keeps FP32 master weights.  Parameters are one flat FP32 array in the
 // order w1 b1 w2 b2 ... w6 b6, weights as [tap][Cout][Cin].
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <vector>
 
 #include "fbp_internal.h"
 #include "refiner_kernels.h"
+#include "refiner_fused.h"
 
 using namespace fbp::nn;
 
@@ -267,7 +269,7 @@ __global__ void absmax_partial_kernel(const float* __restrict__ v, size_t n,
 }
 
 __global__ void absmax_final_kernel(const float* __restrict__ part, int nparts,
-                                    float* __restrict__ mm) {
+                                    float* __restrict__ mm, XForm* __restrict__ xf) {
     __shared__ float s[256];
     float m = 0.f;
     for (int i = threadIdx.x; i < nparts; i += blockDim.x) m = fmaxf(m, part[i]);
@@ -280,6 +282,7 @@ __global__ void absmax_final_kernel(const float* __restrict__ part, int nparts,
     if (threadIdx.x == 0) {
         mm[0] = s[0];
         mm[1] = 0.f;
+        if (xf) *xf = XForm{0.f, 1.f, s[0], 0.f};  // x = in / max|in|
     }
 }
 
@@ -388,10 +391,24 @@ __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
     p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
+// The last layer's FP32 weights (k = tap * 16 + cin) as two BF16 rows, hi
+// and lo = bf16(w - hi), zero-padded to kD0K (the inference kernel's
+// weight operand).
+__global__ void split_last_layer(const float* __restrict__ w, bf16* __restrict__ ws) {
+    const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= kD0K) return;
+    const float v = k < 27 * 16 ? w[k] : 0.f;
+    const bf16 hi = f2bf(v);
+    ws[k] = hi;
+    ws[kD0K + k] = f2bf(v - bf2f(hi));
+}
+
 // Weights [tap][Cout][Cin] FP32 -> BF16 copies used by the MFMA kernels.
-void cast_weights(const float* p, bf16* wb, hipStream_t s) {
+void cast_weights(const float* p, bf16* wb, bf16* w5s, hipStream_t s) {
     const size_t n = param_count();
     hipLaunchKernelGGL(to_bf16, g1(n), dim3(256), 0, s, p, wb, n);
+    hipLaunchKernelGGL(split_last_layer, dim3((kD0K + 255) / 256), dim3(256), 0, s, p + woff(5),
+                       w5s);
 }
 
 }  // namespace
@@ -417,6 +434,9 @@ struct fbp_refiner_s {
     float *gd1 = nullptr, *gu1 = nullptr, *gd2 = nullptr, *gu2 = nullptr, *ge3 = nullptr;
     float *gp2 = nullptr, *ge2 = nullptr, *gp1 = nullptr, *ge1 = nullptr, *tmp = nullptr;
     float* ssim = nullptr;   // moments (5V), parts (3V), box buffers (5V x 2), scalars
+    float* c1 = nullptr;     // inference: relu(conv1(d2)) on the S/2 grid, FP32 [V/8][16]
+    fbp::nn::XForm* xf = nullptr; // inference: input transform
+    bf16* w5s = nullptr;     // inference: last layer's weights as BF16 hi and lo rows
     std::vector<void*> owned;
 };
 
@@ -466,6 +486,9 @@ fbp_status refiner_alloc(fbp_refiner_s* r) {
     FBP_CHECK(alloc(r, &r->ge1, V * 16));
     FBP_CHECK(alloc(r, &r->tmp, V * 32));
     FBP_CHECK(alloc(r, &r->ssim, V * 18 + 64 + kAbsBlocks));
+    FBP_CHECK(alloc(r, &r->c1, V2 * 16));
+    FBP_CHECK(alloc(r, &r->xf, 1));
+    FBP_CHECK(alloc(r, &r->w5s, 2 * kD0K));
     return FBP_OK;
 }
 
@@ -608,9 +631,67 @@ fbp_status normalize_input(fbp_refiner_s* r, const float* in, hipStream_t s) {
     float* mm = r->ssim + 18 * V + 8;
     float* part = r->ssim + 18 * V + 64;
     hipLaunchKernelGGL(absmax_partial_kernel, dim3(kAbsBlocks), dim3(256), 0, s, in, V, part);
-    hipLaunchKernelGGL(absmax_final_kernel, dim3(1), dim3(256), 0, s, part, kAbsBlocks, mm);
+    hipLaunchKernelGGL(absmax_final_kernel, dim3(1), dim3(256), 0, s, part, kAbsBlocks, mm,
+                       nullptr);
     hipLaunchKernelGGL(scale_f, g1(V), dim3(256), 0, s, in, r->x, V, mm);
     FBP_CHECK_HIP(r->ctx, hipGetLastError());
+    return FBP_OK;
+}
+
+template <int CIN, int COUT, bool UP, int NBZ, int NBY, int NBX, int NWT, int MODE>
+fbp_status launch_conv3(fbp_context ctx, const bf16* x, const bf16* w, const float* b,
+                        const bf16* skip, bf16* y, bf16* pool, const bf16* cw, const float* cb,
+                        float* cy, Dims d, hipStream_t s) {
+    using Cf = Conv3Cfg<CIN, COUT, UP, NBZ, NBY, NBX, NWT, MODE>;
+    if (d.D % Cf::TZ || d.H % Cf::TY || d.W % Cf::TX)
+        return fail(ctx, FBP_ERR_UNSUPPORTED, "refiner: grid %dx%dx%d is not a multiple of the tile",
+                    d.D, d.H, d.W);
+    if (Cf::LDS > size_t(ctx->max_lds_per_block))
+        return fail(ctx, FBP_ERR_UNSUPPORTED, "refiner: %zu B of LDS per block needed", Cf::LDS);
+    auto kern = conv3_mfma_kernel<CIN, COUT, UP, NBZ, NBY, NBX, NWT, MODE>;
+    // The attribute is set once per device (a host call on every launch
+    // would stall the enqueue of the following kernels).
+    static std::atomic<uint64_t> attr_set{0};
+    const uint64_t bit = uint64_t(1) << (ctx->device & 63);
+    if (!(attr_set.load(std::memory_order_acquire) & bit)) {
+        FBP_CHECK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               int(Cf::LDS)));
+        attr_set.fetch_or(bit, std::memory_order_release);
+    }
+    const unsigned blocks = unsigned(d.vox() / (size_t(Cf::TZ) * Cf::TY * Cf::TX));
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), Cf::LDS, s, x, w, b, skip, y, pool, cw, cb,
+                       cy, d);
+    FBP_CHECK_HIP(ctx, hipGetLastError());
+    return FBP_OK;
+}
+
+// Inference forward in five fused kernels (refiner_fused.h): in is the raw
+// input, xf its affine normalisation (device), or, when mm.part is set, the
+// min-max normalisation from in's per-block (min, max) pairs; out the
+// refined volume.
+fbp_status refiner_infer(fbp_refiner_s* r, const float* in, const XForm* xf,
+                         const MinMaxPartials& mm, float* out, hipStream_t s) {
+    const int S = r->S;
+    const Dims d1{S, S, S}, d2{S / 2, S / 2, S / 2}, d4{S / 4, S / 4, S / 4};
+    const float* P = r->params;
+    const bf16* Wb = r->wbf;
+    fbp_context ctx = r->ctx;
+    hipLaunchKernelGGL(enc0_kernel, dim3(unsigned(d1.vox() / (kE0Z * kE0Y * kE0X))), dim3(256), 0,
+                       s, in, xf, mm.part, mm.count, P + woff(0), P + boff(0), r->e1, r->p1, d1);
+    FBP_CHECK_HIP(ctx, hipGetLastError());
+    FBP_CHECK((launch_conv3<16, 32, false, 2, 2, 2, 2, kModePool>(
+        ctx, r->p1, Wb + woff(1), P + boff(1), nullptr, r->e2, r->p2, nullptr, nullptr, nullptr,
+        d2, s)));
+    FBP_CHECK((launch_conv3<32, 64, false, 1, 1, 1, 1, kModePlain>(
+        ctx, r->p2, Wb + woff(2), P + boff(2), nullptr, r->e3, nullptr, nullptr, nullptr, nullptr,
+        d4, s)));
+    FBP_CHECK((launch_conv3<64, 32, true, 2, 2, 2, 2, kModeChain>(
+        ctx, r->e3, Wb + woff(3), P + boff(3), r->e2, nullptr, nullptr, Wb + woff(4),
+        P + boff(4), r->c1, d2, s)));
+    hipLaunchKernelGGL(dec0_kernel, dim3(unsigned(d1.vox() / (kD0Z * kD0Y * kD0X))), dim3(256), 0,
+                       s, r->c1, r->e1, r->w5s, P + boff(5), in, xf, mm.part, mm.count, out, d1);
+    FBP_CHECK_HIP(ctx, hipGetLastError());
     return FBP_OK;
 }
 
@@ -679,7 +760,7 @@ fbp_status fbp_refiner_load(fbp_refiner r, const float* host_params, uint64_t co
     FBP_CHECK_HIP(r->ctx, hipMemset(r->adam_m, 0, count * 4));
     FBP_CHECK_HIP(r->ctx, hipMemset(r->adam_v, 0, count * 4));
     r->step = 0;
-    cast_weights(r->params, r->wbf, nullptr);
+    cast_weights(r->params, r->wbf, r->w5s, nullptr);
     FBP_CHECK_HIP(r->ctx, hipDeviceSynchronize());
     return FBP_OK;
 }
@@ -697,11 +778,43 @@ fbp_status fbp_refiner_forward(fbp_refiner r, const float* input, float* output,
     if (!r || !input || !output) return FBP_ERR_INVALID_ARGUMENT;
     FBP_CHECK(bind(r->ctx));
     hipStream_t s = as_stream(stream);
-    FBP_CHECK(normalize_input(r, input, s));
-    FBP_CHECK(refiner_forward_impl(r, s, false));
     const size_t V = size_t(r->S) * r->S * r->S;
-    FBP_CHECK_HIP(r->ctx, hipMemcpyAsync(output, r->y, V * 4, hipMemcpyDeviceToDevice, s));
-    return FBP_OK;
+    float* mm = r->ssim + 18 * V + 8;
+    float* part = r->ssim + 18 * V + 64;
+    hipLaunchKernelGGL(absmax_partial_kernel, dim3(kAbsBlocks), dim3(256), 0, s, input, V, part);
+    hipLaunchKernelGGL(absmax_final_kernel, dim3(1), dim3(256), 0, s, part, kAbsBlocks, mm, r->xf);
+    FBP_CHECK_HIP(r->ctx, hipGetLastError());
+    return refiner_infer(r, input, r->xf, MinMaxPartials{}, output, s);
+}
+
+fbp_status fbp_learned_reconstruct(fbp_refiner r, const fbp_reconstruction_info* info,
+                                   const fbp_transient* tr, const fbp_vec3* laser_targets,
+                                   uint32_t num_samples, uint64_t seed, int32_t kernel_size,
+                                   float sigma, uint32_t extremes, float* volume, void* stream) {
+    if (!r || !info || !tr || !laser_targets || !volume) return FBP_ERR_INVALID_ARGUMENT;
+    fbp_context ctx = r->ctx;
+    const uint32_t S = uint32_t(r->S);
+    if (info->voxel_resolution.x != S || info->voxel_resolution.y != S ||
+        info->voxel_resolution.z != S)
+        return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "learned: the voxel grid must be the refiner's %u^3",
+                    S);
+    if (kernel_size != 5)
+        return fail(ctx, FBP_ERR_UNSUPPORTED, "learned: LoG kernel size 5 only");
+    FBP_CHECK(bind(ctx));
+    hipStream_t s = as_stream(stream);
+    // The MIS volume and its LoG live in the refiner's training buffers x, y.
+    fbp_mis_params p{};
+    p.num_samples = num_samples;
+    p.mode = FBP_MIS_TILED;
+    p.noise = nullptr;
+    p.noise_size = S * S * S;
+    p.seed = seed;
+    FBP_CHECK(fbp_backproject_confocal_mis(ctx, info, tr, laser_targets, &p, r->x, stream));
+    MinMaxPartials mm{};
+    FBP_CHECK(log_impl(ctx, r->x, r->y, fbp_uvec3{S, S, S}, kernel_size, sigma, s, &mm));
+    FBP_CHECK(refiner_infer(r, r->y, nullptr, mm, volume, s));
+    return learned_calibrate_impl(ctx, info, tr, laser_targets, volume, r->y, kernel_size, sigma,
+                                  extremes, s);
 }
 
 fbp_status fbp_refiner_train_step(fbp_refiner r, const float* inputs, const float* targets,
@@ -731,7 +844,7 @@ fbp_status fbp_refiner_train_step(fbp_refiner r, const float* inputs, const floa
     const float c1 = 1.f - std::pow(b1, float(r->step)), c2 = 1.f - std::pow(b2, float(r->step));
     hipLaunchKernelGGL(adam_kernel, g1(P), dim3(256), 0, s, r->params, r->grad, r->adam_m, r->adam_v,
                        P, lr, b1, b2, c1, c2, 1e-8f, 1.f / float(batch));
-    cast_weights(r->params, r->wbf, s);
+    cast_weights(r->params, r->wbf, r->w5s, s);
     FBP_CHECK_HIP(r->ctx, hipGetLastError());
     if (host_loss) *host_loss = float(loss / batch);
     return FBP_OK;

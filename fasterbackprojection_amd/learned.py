@@ -14,14 +14,18 @@ starting point the training improves on.
 
 The refined volume matches the target up to a global affine map (the
 normalisation of the noisy input is set by its extremes, which the
-convolutions cannot see); fbp_learned_calibrate fixes that map: a second,
-independent MIS draw (+ LoG) regressed on the refined volume over all
-voxels gives the LoG in physical units, and exact LoG values at the refined
-volume's CALIB_EXTREMES lowest and highest voxels (exact backprojections of
-their 5^3 neighbourhoods) give the true range.
+convolutions cannot see); fbp_learned_calibrate fixes that map: the draw's
+own LoG in physical units regressed on the refined volume over all voxels
+gives the LoG's scale and offset (its noise is about 5 % of the signal, so
+the bias it puts into the fit is of order 0.3 %), and exact LoG values at
+the refined volume's CALIB_EXTREMES lowest and highest voxels (exact
+backprojections of their 5^3 neighbourhoods) give the true range.
 
-Every step runs in libfbp.so kernels; torch holds device memory, its RNG
-draws the Poisson counts and noise buffers.
+The learned call is one C-ABI call, fbp_learned_reconstruct (Refiner.
+reconstruct): seeded MIS draw, LoG with its min / max, the refiner's five
+fused kernels, calibration.  Training data use torch's RNG for the noise
+buffer (the same sampler fed from a buffer); every step runs in libfbp.so
+kernels, torch holds device memory and draws the Poisson counts.
 
     python -m fasterbackprojection_amd.learned train --out DIR [--scenes N --steps K]
     python -m fasterbackprojection_amd.learned calibrate [--weights FILE]
@@ -67,7 +71,6 @@ class Pipeline:
         self.vol = torch.empty((SIZE,) * 3, dtype=torch.float32, device=self.dev)
         self.filt = torch.empty_like(self.vol)
         self.mis = torch.empty_like(self.vol)
-        self.mis2 = torch.empty_like(self.vol)
         self.ref = torch.empty_like(self.vol)
         # residual check: every 4th scan point in x and z
         n = SCAN
@@ -88,36 +91,37 @@ class Pipeline:
         """Backprojection over all scan points + LoG + min-max normalisation."""
         self.ctx.backproject_confocal(self.info, I, self.laser, out=self.vol)
         out = self.filt if out is None else out
-        self.ctx.log(self.vol, 5, 1.0, out=out)
-        self.ctx.normalize(out)
-        return out
+        return self.ctx.log_normalize(self.vol, 5, 1.0, out=out)
 
     def mis_input(self, I: torch.Tensor, seed: int, out: torch.Tensor | None = None) -> torch.Tensor:
         """Refiner input: unbiased MIS backprojection + LoG + min-max
-        normalisation (the same filter chain as the plain path)."""
+        normalisation (the same filter chain as the plain path).  self.ref
+        keeps the draw's LoG in physical units (calibrate's reference)."""
         gen = torch.Generator(device=self.dev)
         gen.manual_seed(10_000_019 + seed)
         noise = torch.rand(SIZE ** 3, generator=gen, device=self.dev)
         self.ctx.backproject_confocal_mis(self.info, I, self.laser, noise,
                                           self.L // SAMPLE_FRACTION, mode=MIS_TILED,
                                           out=self.mis)
+        self.ctx.log(self.mis, 5, 1.0, out=self.ref)
         out = torch.empty_like(self.vol) if out is None else out
-        self.ctx.log(self.mis, 5, 1.0, out=out)
+        out.copy_(self.ref)
         self.ctx.normalize(out)
         return out
 
-    def calibrate(self, volume: torch.Tensor, I: torch.Tensor, seed: int) -> torch.Tensor:
+    def calibrate(self, volume: torch.Tensor, I: torch.Tensor) -> torch.Tensor:
         """Global calibration of a refined volume (fbp_learned_calibrate), in
-        place, against the LoG of a second MIS draw (independent noise)."""
-        gen = torch.Generator(device=self.dev)
-        gen.manual_seed(20_000_033 + seed)
-        noise = torch.rand(SIZE ** 3, generator=gen, device=self.dev)
-        self.ctx.backproject_confocal_mis(self.info, I, self.laser, noise,
-                                          self.L // SAMPLE_FRACTION, mode=MIS_TILED,
-                                          out=self.mis2)
-        self.ctx.log(self.mis2, 5, 1.0, out=self.ref)
+        place, against the LoG of the draw that made the refiner's input
+        (self.ref, set by mis_input)."""
         return self.ctx.learned_calibrate(self.info, I, self.laser, volume, self.ref,
                                           CALIB_EXTREMES, 5, 1.0)
+
+    def learned(self, net: Refiner, I: torch.Tensor, seed: int,
+                out: torch.Tensor | None = None) -> torch.Tensor:
+        """The learned form in one call (fbp_learned_reconstruct), MIS
+        uniforms from seed."""
+        return net.reconstruct(self.info, I, self.laser, self.L // SAMPLE_FRACTION,
+                               10_000_019 + seed, out=out, extremes=CALIB_EXTREMES)
 
     def residual(self, volume: torch.Tensor, I: torch.Tensor) -> float:
         """Forward-model residual on every 16th scan point (fbp_volume_residual):
@@ -190,10 +194,7 @@ def flat_noise_residual(p: Pipeline, net: Refiner) -> float:
     gen.manual_seed(5)
     I = torch.poisson(torch.full((SCAN * SCAN, BINS), 3.0, device=p.dev), generator=gen)
     I = I.contiguous()
-    y = torch.empty((SIZE,) * 3, dtype=torch.float32, device=p.dev)
-    net.forward(p.mis_input(I, 0), out=y)
-    p.calibrate(y, I, 0)
-    return p.residual(y, I)
+    return p.residual(p.learned(net, I, 0), I)
 
 
 def calibrate(weights: Path, held_out: int = 256, log=print) -> dict:
@@ -229,8 +230,7 @@ def evaluate(p: Pipeline, net: Refiner, seeds, X=None, Y=None) -> dict:
         else:
             I = p.render(int(s))
             x, y = X[i], Y[i]
-        net.forward(x, out=out)
-        p.calibrate(out, I, int(s))
+        p.learned(net, I, int(s), out=out)
         learned.append(rel_l2(out, y))
         baseline.append(rel_l2(x, y))
         res.append(p.residual(out, I))
@@ -259,13 +259,10 @@ class LearnedReconstructor:
         self.net = Refiner(ctx, SIZE)
         self.net.load(np.fromfile(weights, np.float32))
         self.out = torch.empty((SIZE,) * 3, dtype=torch.float32, device=self.p.dev)
-        self.x = torch.empty_like(self.out)
         self.fallbacks = 0
 
     def __call__(self, I: torch.Tensor, seed: int = 0, check: bool = True):
-        x = self.p.mis_input(I, seed, out=self.x)
-        self.net.forward(x, out=self.out)
-        self.p.calibrate(self.out, I, seed)
+        self.p.learned(self.net, I, seed, out=self.out)
         if check:
             r = self.p.residual(self.out, I)
             if r > self.threshold:

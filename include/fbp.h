@@ -250,6 +250,13 @@ typedef struct fbp_mis_params {
     uint32_t mode;          /* FBP_MIS_*                                       */
     const float* noise;     /* device float[noise_size] in [0,1)               */
     uint32_t noise_size;    /* reference: number of voxels                     */
+    /* FBP_MIS_TILED with noise == NULL: entry p of the noise array is
+     * (mix(p ^ key(seed)) >> 8) / 2^24 (no reference counterpart; mix is the
+     * "lowbias32" integer hash x ^= x >> 16, x *= 0x7feb352d, x ^= x >> 15,
+     * x *= 0x846ca68b, x ^= x >> 16, and key(seed) = mix(lo ^ mix(hi +
+     * 0x9e3779b9)) of the seed's 32-bit halves): the draw needs no noise
+     * buffer.                                                                 */
+    uint64_t seed;
 } fbp_mis_params;
 fbp_status fbp_backproject_confocal_mis(fbp_context ctx, const fbp_reconstruction_info* info,
                                         const fbp_transient* transient,
@@ -289,6 +296,13 @@ fbp_status fbp_filter_log_fft(fbp_context ctx, float* volume, fbp_uvec3 size, fl
 /* Laser::normalizeMatrix + normalizeReconstruction (FB/Laser.cu:317-339,
  * FB/transient_postprocessing.cuh:10-17): v = (v - min) * safeRCP(max - min). */
 fbp_status fbp_normalize_minmax(fbp_context ctx, float* v, uint64_t n, void* stream);
+/* fbp_filter_log followed by fbp_normalize_minmax on out, with the same
+ * results, in two launches: the LoG kernel reduces the min and max of its
+ * output itself (the reference runs LoGFilter, then normalizeMatrix's
+ * reductions and normalizeReconstruction: FB/Laser.cu:317-339). */
+fbp_status fbp_filter_log_normalize(fbp_context ctx, const float* in, float* out,
+                                    fbp_uvec3 size, int32_t kernel_size, float sigma,
+                                    void* stream);
 /* min and max of v written to host pointers (synchronises the stream). */
 fbp_status fbp_minmax(fbp_context ctx, const float* v, uint64_t n, float* host_min,
                       float* host_max, void* stream);
@@ -410,6 +424,19 @@ fbp_status fbp_refiner_destroy(fbp_refiner r);
 fbp_status fbp_refiner_load(fbp_refiner r, const float* host_params, uint64_t count);
 fbp_status fbp_refiner_save(fbp_refiner r, float* host_params, uint64_t count);
 fbp_status fbp_refiner_forward(fbp_refiner r, const float* input, float* output, void* stream);
+/* Learned form in one call (no reference counterpart): the tile-shared MIS
+ * backprojection (FBP_MIS_TILED, num_samples scan points per tile, uniforms
+ * from `seed` over a noise array of S^3 entries), its LoG (kernel_size 5,
+ * sigma), the refiner on the LoG's min-max normalisation (the min and max
+ * come from the LoG kernel itself), then fbp_learned_calibrate of the
+ * refined volume against that LoG with `extremes` voxels per side.  The
+ * voxel grid must be the refiner's S^3; F32 time-fastest transients.
+ * volume: the calibrated [Z][Y][X] result. */
+fbp_status fbp_learned_reconstruct(fbp_refiner r, const fbp_reconstruction_info* info,
+                                   const fbp_transient* transient,
+                                   const fbp_vec3* laser_targets, uint32_t num_samples,
+                                   uint64_t seed, int32_t kernel_size, float sigma,
+                                   uint32_t extremes, float* volume, void* stream);
 /* One Adam step over `batch` (input, target) volume pairs laid out back to
  * back; loss = mean|y - t| + ssim_weight * (1 - mean SSIM(y, t)).
  * host_loss (may be NULL, else synchronises) receives the batch mean. */

@@ -575,6 +575,41 @@ def test_mis_tiled_matches_oracle(ctx):
     assert_volume_close(out, ref)
 
 
+def _seeded_noise(seed, n):
+    """The seeded virtual noise array of fbp_mis_params (its header formula)."""
+    M = 0xFFFFFFFF
+
+    def mix(x):
+        x = np.asarray(x, np.uint64) & M
+        x ^= x >> 16
+        x = (x * 0x7FEB352D) & M
+        x ^= x >> 15
+        x = (x * 0x846CA68B) & M
+        x ^= x >> 16
+        return x
+
+    key = mix(np.uint64(seed & M) ^ mix((np.uint64(seed >> 32) + 0x9E3779B9) & M))
+    h = mix(np.arange(n, dtype=np.uint64) ^ key)
+    return ((h >> 8).astype(np.float64) / 2.0 ** 24).astype(np.float32)
+
+
+@pytest.mark.parametrize("seed", [0, 12345, 2 ** 40 + 77])
+def test_mis_tiled_seeded_matches_oracle(ctx, seed):
+    """noise=None: the same samples as the restatement fed the seeded noise
+    array (the hash restated above); exact bins."""
+    g = geom(n_scan=16, T=256, res=(32, 32, 20))
+    info = make_info(g)
+    rng = np.random.default_rng(43)
+    I = (10.0 ** rng.uniform(-2, 3, size=(256, 256))).astype(np.float32)
+    I[rng.random(I.shape) < 0.3] = 0.0
+    n = 32 * 32 * 20
+    ref = oracle.mis_tiled(info, I, g.laser_targets, _seeded_noise(seed, n), 48)
+    out = ctx.backproject_confocal_mis(info, cuda(I), cuda(g.laser_targets), None, 48,
+                                       mode=MIS_TILED, seed=seed)
+    torch.cuda.synchronize()
+    assert_volume_close(out, ref)
+
+
 def test_mis_tiled_is_unbiased(ctx):
     """The mean over independent noise tables converges to the plain
     backprojection: its error must be within 4 standard errors estimated

@@ -74,3 +74,23 @@ def test_normalize(ctx):
     np.testing.assert_allclose(t.cpu().numpy(), ref, rtol=1e-6, atol=1e-7)
     mn, mx = ctx.minmax(t)
     assert mn == 0.0 and abs(mx - 1.0) < 1e-6
+
+
+@pytest.mark.parametrize("shape,k,sigma", [((32, 32, 32), 5, 1.0), ((19, 21, 33), 5, 1.3),
+                                           ((128, 128, 128), 5, 1.0), ((20, 24, 28), 15, 0.6)])
+def test_log_normalize_equals_the_two_calls(ctx, shape, k, sigma):
+    """fbp_filter_log_normalize (the LoG kernel reducing its own min / max)
+    gives bit for bit what fbp_filter_log then fbp_normalize_minmax give;
+    the last case (sigma 0.6, k 15) takes the dense LoG path."""
+    rng = np.random.default_rng(sum(shape) + k)
+    v = torch.from_numpy(rng.standard_normal(shape).astype(np.float32)).cuda()
+    two = ctx.log(v, k, sigma)
+    ctx.normalize(two)
+    one = ctx.log_normalize(v, k, sigma)
+    torch.cuda.synchronize()
+    assert torch.equal(one, two)
+    # a second call reuses the arrival counter
+    one2 = ctx.log_normalize(v * 2.0, k, sigma)
+    two2 = ctx.log(v * 2.0, k, sigma)
+    ctx.normalize(two2)
+    assert torch.equal(one2, two2)

@@ -110,3 +110,76 @@ def test_learned_path_within_two_percent(ctx):
         out, _, _ = lrn(I, s, check=False)
         errs.append(rel_l2(out, y))
     assert np.median(errs) < 0.02 and np.mean(errs) < 0.03, errs
+
+
+@pytest.mark.parametrize("res,ties", [((16, 16, 16), False), ((128, 128, 40), False),
+                                      ((128, 128, 40), True)])
+def test_calibration_matches_cpu_selection(ctx, res, ties):
+    """fbp_learned_calibrate against a CPU restatement: least squares of the
+    reference on the volume (FP64), the k lowest and k highest voxels by
+    (value, index) order (np.lexsort), the exact LoG there from the FP64
+    oracle backprojection and LoG filter, and the affine map.  The volume is
+    noise independent of the scene, so the range depends on exactly which
+    voxels are selected; 128 x 128 x 40 = 655360 voxels makes every block of
+    the GPU's selection pass sort two 512-voxel chunks, and the tied case
+    (16 distinct values) checks the index order of ties."""
+    import oracle
+    from fasterbackprojection_amd.api import make_info
+    from fasterbackprojection_amd.scene import C_LIGHT, confocal_geometry
+
+    k = 16
+    g = confocal_geometry(4, 1.0, 96, C_LIGHT * 128e-12, (-0.4, 0.3, -0.4), (0.4, 0.9, 0.4), res)
+    info = make_info(g)
+    rng = np.random.default_rng(71)
+    I = (10.0 ** rng.uniform(-2, 1, size=(len(g.laser_targets), 96))).astype(np.float32)
+    bp = oracle.confocal_voxel(info, I, g.laser_targets)
+    w = np.empty(125, np.float32)
+    ctx.lib.fbp_log_kernel_host(5, 1.0, w.ctypes.data)
+    L = oracle.log_filter(bp, 5, 1.0, weights=w.astype(np.float64).reshape(5, 5, 5)).ravel()
+    if ties:
+        v = (np.floor(rng.random(L.size) * 16) / 16).astype(np.float32)
+    else:
+        v = rng.random(L.size).astype(np.float32)
+    r = (L + 0.1 * np.std(L) * rng.standard_normal(L.size)).astype(np.float32)
+    vd, rd = v.astype(np.float64), r.astype(np.float64)
+    n = float(v.size)
+    S = [n, vd.sum(), (vd * vd).sum(), rd.sum(), (vd * rd).sum()]
+    det = S[0] * S[2] - S[1] ** 2
+    a = (S[0] * S[4] - S[1] * S[3]) / det
+    b = (S[3] - a * S[1]) / S[0]
+    idx = np.arange(v.size)
+    lows = np.lexsort((idx, v))[:k]
+    highs = np.lexsort((idx, -v))[:k]
+    mn, mx = L[lows].min(), L[highs].max()
+    c0, c1 = np.float32(a / (mx - mn)), np.float32((b - mn) / (mx - mn))
+    want = (np.float64(c0) * vd + np.float64(c1)).astype(np.float32)
+    shape = (res[2], res[1], res[0])
+    d = torch.from_numpy(v.reshape(shape)).cuda()
+    ctx.learned_calibrate(info, torch.from_numpy(I).cuda(), torch.from_numpy(g.laser_targets).cuda(),
+                          d, torch.from_numpy(r.reshape(shape)).cuda(), k)
+    got = d.cpu().numpy().ravel()
+    scale = np.abs(want).max()
+    assert np.max(np.abs(got - want)) <= 2e-5 * scale, np.max(np.abs(got - want)) / scale
+
+
+@need_weights
+def test_one_call_equals_the_steps(ctx):
+    """fbp_learned_reconstruct gives what its steps give through the
+    separate entry points: the seeded MIS draw, LoG, min-max normalisation,
+    fbp_refiner_forward, fbp_learned_calibrate against the draw's LoG."""
+    from fasterbackprojection_amd.api import MIS_TILED
+    from fasterbackprojection_amd.learned import CALIB_EXTREMES, SAMPLE_FRACTION
+    lrn = LearnedReconstructor(ctx)
+    p = lrn.p
+    for s in (3_000_600, 3_000_601):
+        I = p.render(s)
+        one = p.learned(lrn.net, I, s).clone()
+        mis = ctx.backproject_confocal_mis(p.info, I, p.laser, None, p.L // SAMPLE_FRACTION,
+                                           mode=MIS_TILED, seed=10_000_019 + s)
+        ref = ctx.log(mis, 5, 1.0)
+        x = ref.clone()
+        ctx.normalize(x)
+        y = lrn.net.forward(x)
+        ctx.learned_calibrate(p.info, I, p.laser, y, ref, CALIB_EXTREMES, 5, 1.0)
+        torch.cuda.synchronize()
+        assert torch.equal(one, y), float((one - y).abs().max())
