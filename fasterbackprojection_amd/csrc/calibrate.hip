@@ -147,66 +147,57 @@ __global__ __launch_bounds__(256) void calib_scan_kernel(
         for (int q = 0; q < 5; ++q) sums[size_t(b) * 5 + q] = red[q][0];
 }
 
-// One block after the scan over nb blocks: the fit sums reduced in a fixed
-// order into fit[0..4], and the global k of each side: the k blocks with
-// the smallest first keys hold all of them (a block outside those has k
-// smaller keys before its first), so the k smallest of their k x k keys
-// are the answer.  sel[0..k): lowest voxels, sel[k..2k): highest.
+// Two blocks after the scan over nb blocks, one per side (blockIdx.x = 0:
+// lows, 1: highs): the global k of the side: the k blocks with the smallest
+// first keys hold all of them (a block outside those has k smaller keys
+// before its first), so the k smallest of their k x k keys are the answer;
+// sel[side * k + i], i < k.  Block 0 also reduces the fit sums in a fixed
+// order into fit[0..4].
 __global__ __launch_bounds__(256) void calib_select_kernel(const uint64_t* __restrict__ cand,
                                                            const double* __restrict__ sums,
                                                            uint32_t nb, uint32_t seg, uint32_t k,
                                                            uint32_t* __restrict__ sel,
                                                            double* __restrict__ fit) {
     __shared__ double red[5][256];
-    __shared__ uint64_t sl[kMaxExtremes], sh[kMaxExtremes];
-    __shared__ uint32_t blo[kMaxExtremes], bhi[kMaxExtremes];
-    const uint32_t t = threadIdx.x;
-    // Fit sums: thread t adds blocks t, t + 256, ... in order, then the tree.
-    double s5[5] = {0, 0, 0, 0, 0};
-    for (uint32_t i = t; i < nb; i += 256)
-        for (int q = 0; q < 5; ++q) s5[q] += sums[size_t(i) * 5 + q];
-    for (int q = 0; q < 5; ++q) red[q][t] = s5[q];
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (int(t) < w)
-            for (int q = 0; q < 5; ++q) red[q][t] += red[q][t + w];
-        __syncthreads();
-    }
-    if (t < 5) fit[t] = red[t][0];
+    __shared__ uint64_t sk[kMaxExtremes];
+    __shared__ uint32_t blk[kMaxExtremes];
+    const uint32_t t = threadIdx.x, side = blockIdx.x;
+    const uint64_t* c = cand + size_t(side) * nb * k;
     // The first key of every block (nb <= 1024: 4 per thread).
-    uint64_t a[4], c[4];
+    uint64_t a[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const uint32_t i = t + 256u * uint32_t(j);
-        a[j] = i < nb ? cand[size_t(i) * k] : kPadKey;
-        c[j] = i < nb ? cand[(size_t(nb) + i) * k] : kPadKey;
+        a[j] = i < nb ? c[size_t(i) * k] : kPadKey;
     }
-    block_topk<4, kMaxExtremes>(a, k, nullptr, false, sl);
-    block_topk<4, kMaxExtremes>(c, k, nullptr, false, sh);
+    if (side == 0) {
+        // Fit sums: thread t adds blocks t, t + 256, ... in order, then the tree.
+        double s5[5] = {0, 0, 0, 0, 0};
+        for (uint32_t i = t; i < nb; i += 256)
+            for (int q = 0; q < 5; ++q) s5[q] += sums[size_t(i) * 5 + q];
+        for (int q = 0; q < 5; ++q) red[q][t] = s5[q];
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (int(t) < w)
+                for (int q = 0; q < 5; ++q) red[q][t] += red[q][t + w];
+            __syncthreads();
+        }
+        if (t < 5) fit[t] = red[t][0];
+    }
+    block_topk<4, kMaxExtremes>(a, k, nullptr, false, sk);
     // (fewer than k blocks: the pads select no block)
     constexpr uint32_t kNoBlock = 0xffffffffu;
-    if (t < k) {
-        blo[t] = sl[t] == kPadKey ? kNoBlock : uint32_t(sl[t] & 0xffffffffu) / seg;
-        bhi[t] = sh[t] == kPadKey ? kNoBlock : uint32_t(sh[t] & 0xffffffffu) / seg;
-    }
+    if (t < k) blk[t] = sk[t] == kPadKey ? kNoBlock : uint32_t(sk[t] & 0xffffffffu) / seg;
     __syncthreads();
     // Their k x k keys (k <= 32: 4 per thread).
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const uint32_t i = t + 256u * uint32_t(j);
-        a[j] = c[j] = kPadKey;
-        if (i < k * k) {
-            const uint32_t bb = i / k, m = i % k;
-            if (blo[bb] != kNoBlock) a[j] = cand[size_t(blo[bb]) * k + m];
-            if (bhi[bb] != kNoBlock) c[j] = cand[(size_t(nb) + bhi[bb]) * k + m];
-        }
+        a[j] = kPadKey;
+        if (i < k * k && blk[i / k] != kNoBlock) a[j] = c[size_t(blk[i / k]) * k + i % k];
     }
-    block_topk<4, kMaxExtremes>(a, k, nullptr, false, sl);
-    block_topk<4, kMaxExtremes>(c, k, nullptr, false, sh);
-    if (t < k) {
-        sel[t] = uint32_t(sl[t] & 0xffffffffu);
-        sel[k + t] = uint32_t(sh[t] & 0xffffffffu);
-    }
+    block_topk<4, kMaxExtremes>(a, k, nullptr, false, sk);
+    if (t < k) sel[side * k + t] = uint32_t(sk[t] & 0xffffffffu);
 }
 
 // Exact LoG contribution of one chunk of scan points at candidate ci
@@ -420,7 +411,7 @@ fbp_status learned_calibrate_impl(fbp_context ctx, const fbp_reconstruction_info
 
     hipLaunchKernelGGL(calib_scan_kernel, dim3(nb), dim3(256), 0, s, volume, reference,
                        uint32_t(n), seg, k, cand, sums);
-    hipLaunchKernelGGL(calib_select_kernel, dim3(1), dim3(256), 0, s, cand, sums, nb, seg, k, sel,
+    hipLaunchKernelGGL(calib_select_kernel, dim3(2), dim3(256), 0, s, cand, sums, nb, seg, k, sel,
                        fit);
     const PointArgs a = point_args(*info, static_cast<const float*>(tr->data), laser_targets);
     hipLaunchKernelGGL(calib_neigh_kernel, dim3(2 * k, chunks), dim3(128), 0, s, a, w, sel, k,

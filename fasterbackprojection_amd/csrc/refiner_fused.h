@@ -105,10 +105,10 @@ __global__ __launch_bounds__(256) void enc0_kernel(const float* __restrict__ in,
         a[s] = w[min(tap, 26) * 16 + r16];
         if (tap >= 27) a[s] = 0.f;
     }
-    const XForm tf = load_xform(xf, mm_part, mm_count);
-    // Every load of the tile is issued before the first LDS store, so the
-    // block waits for memory once.  Branch-free: the address is clamped into
-    // the volume, the value masked.
+    // Every load of the tile is issued before the first LDS store (and
+    // before the input transform's own loads and reduction), so the block
+    // waits for memory once.  Branch-free: the address is clamped into the
+    // volume, the value masked.
     constexpr int NH = HZ * HY * HX, PH = (NH + 255) / 256;
     float hv[PH];
     bool hin[PH];
@@ -122,6 +122,7 @@ __global__ __launch_bounds__(256) void enc0_kernel(const float* __restrict__ in,
                   cx = min(max(gx, 0), d.W - 1);
         hv[j] = in[(size_t(cz) * d.H + cy) * d.W + cx];
     }
+    const XForm tf = load_xform(xf, mm_part, mm_count);
 #pragma unroll
     for (int j = 0; j < PH; ++j) {
         const int i = t + j * 256;
@@ -441,13 +442,17 @@ __global__ __launch_bounds__(256) void dec0_kernel(const float* __restrict__ c1,
     const int z0 = bz * kD0Z, y0 = by * kD0Y, x0 = bx * kD0X;
     const Dims dc{d.D / 2, d.H / 2, d.W / 2};
     constexpr int KS = kD0K / 32;
-    // Weight operands (rows 0 and 1 live), issued first.
-    bf16x8 a[KS];
+    __shared__ __align__(16) bf16 wl[2 * kD0K];
+    for (int i = t; i < 2 * kD0K / 8; i += 256)
+        reinterpret_cast<u32x4*>(wl)[i] = reinterpret_cast<const u32x4*>(ws)[i];
+    int vz[2], vy[2], vx[2];
 #pragma unroll
-    for (int s = 0; s < KS; ++s) {
-        a[s] = frag_at(ws + min(r16, 1) * kD0K + 32 * s + 8 * g);
-        if (r16 >= 2) a[s] = bf16x8{};
-    }
+    for (int i = 0; i < 2; ++i) brick_voxel<2, 2>(2 * wave + i, r16, vz[i], vy[i], vx[i]);
+    // The raw input at this lane's two voxels (for y = ... + x), issued early.
+    float xin[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+        xin[i] = in[(size_t(z0 + vz[i]) * d.H + y0 + vy[i]) * d.W + x0 + vx[i]];
     // Stage d1; all loads of the tile issued before the first LDS store.
     // Branch-free: the address is clamped into the volume, the value masked.
     constexpr int NS = HZ * HY * HX * 2, PS = (NS + 255) / 256;
@@ -488,30 +493,26 @@ __global__ __launch_bounds__(256) void dec0_kernel(const float* __restrict__ c1,
         *reinterpret_cast<u32x4*>(dt + (i >> 1) * CS + (i & 1) * 8) = pk;
     }
     __syncthreads();
-    int vz[2], vy[2], vx[2];
-    f4 acc[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        brick_voxel<2, 2>(2 * wave + i, r16, vz[i], vy[i], vx[i]);
-        acc[i] = f4{0.f, 0.f, 0.f, 0.f};
-    }
+    const XForm tf = load_xform(xf, mm_part, mm_count);
+    f4 acc[2] = {f4{0.f, 0.f, 0.f, 0.f}, f4{0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
         const int tap = min(2 * s + (g >> 1), 26), ci0 = (g & 1) * 8;
         const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+        bf16x8 a = frag_at(wl + min(r16, 1) * kD0K + 32 * s + 8 * g);
+        if (r16 >= 2) a = bf16x8{};
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
             const bf16x8 b =
                 frag_at(dt + (((vz[i] + dz) * HY + vy[i] + dy) * HX + vx[i] + dx) * CS + ci0);
-            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s], b, acc[i], 0, 0, 0);
+            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i], 0, 0, 0);
         }
     }
-    const XForm tf = load_xform(xf, mm_part, mm_count);
     if (g != 0) return;  // rows 0 and 1 live in lanes 0..15
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
         const size_t v = (size_t(z0 + vz[i]) * d.H + y0 + vy[i]) * d.W + x0 + vx[i];
-        out[v] = ((acc[i][0] + acc[i][1]) + bias[0]) + xform_apply(in[v], tf);
+        out[v] = ((acc[i][0] + acc[i][1]) + bias[0]) + xform_apply(xin[i], tf);
     }
 }
 
