@@ -128,11 +128,14 @@ __device__ __forceinline__ float box_max_dist(const float p[3], const float lo[3
 
 // Sums per-chunk partial tiles in FP64 (fixed order) and scatters them to
 // the [Z][Y][X] volume.  partial layout: [chunk][tile][slot][lane(64)].
-// corr (may be null) is a second buffer of the same layout added in.
+// corr (may be null) is a second buffer of the same layout added in, for
+// the (chunk, tile) blocks whose corr_used[chunk][tile] is set (every block
+// when corr_used is null).
 // skip (may be null): when *skip != 0 on the device the kernel returns.
 fbp_status launch_reduce(fbp_context ctx, const float* partial, const float* corr, float* volume,
                          const VolumeGeom& g, uint32_t num_chunks, double scale,
-                         hipStream_t stream, const int* skip = nullptr);
+                         hipStream_t stream, const int* skip = nullptr,
+                         const uint8_t* corr_used = nullptr);
 
 // Confocal voxel backprojection (bp_confocal.hip).  With samples != 0 every
 // 16^3 voxel tile visits its own scan-point list sample_l[tile][0..samples)

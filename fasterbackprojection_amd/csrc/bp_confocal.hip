@@ -42,6 +42,7 @@ struct ConfocalArgs {
     const float* thr;      // S_k rows, (T + 1) floats each
     uint32_t thr_rows;     // 1 (pure chain) or L
     float* corr;           // near-voxel corrections, same layout as partial
+    uint8_t* corr_used;    // [chunk][tile]: 1 when the block wrote its corr slice
     const int* skip;       // when non-null and *skip != 0 the aligned path ran: return
     // Tile-sampled mode (MIS): each voxel tile visits its own list of
     // `items` scan points sample_l[tile][i] with weights sample_w[tile][i];
@@ -148,10 +149,11 @@ __device__ __forceinline__ void confocal_voxel_body(const ConfocalArgs& a) {
     float acc[64], comp[64];
 #pragma unroll
     for (int i = 0; i < 64; ++i) acc[i] = comp[i] = 0.f;
-    // Near-voxel corrections (rare) go to a lane-owned slice of a.corr.
+    // Near-voxel corrections (rare) go to a lane-owned slice of a.corr,
+    // cleared on the block's first near scan point; corr_used tells the
+    // reduction whether to read it.
     float* corr = a.corr + (size_t(chunk) * a.g.num_tiles + tile) * 64 * 64;
-#pragma unroll 8
-    for (int s = 0; s < 64; ++s) corr[s * 64 + lane] = 0.f;
+    bool corr_live = false;
 
     const uint32_t l_begin = chunk * a.chunk_len;
     const uint32_t l_end = min(a.items, l_begin + a.chunk_len);
@@ -244,6 +246,11 @@ __device__ __forceinline__ void confocal_voxel_body(const ConfocalArgs& a) {
                 // Scan point (almost) on a voxel centre, or dist near zero:
                 // every voxel of the tile goes through the IEEE chain and
                 // lands in the lane-owned correction slots (acc untouched).
+                if (!corr_live) {
+#pragma unroll 8
+                    for (int s = 0; s < 64; ++s) corr[s * 64 + lane] = 0.f;
+                    corr_live = true;
+                }
                 const float off = rfl(P[P_OFF]);
                 for (int slot = 0; slot < 64; ++slot) {
                     const float s = rn_add(rn_add(sq(rn_sub(pick4(px, slot >> 4), lxp)),
@@ -271,6 +278,7 @@ __device__ __forceinline__ void confocal_voxel_body(const ConfocalArgs& a) {
         }
     }
 
+    if (lane == 0) a.corr_used[size_t(chunk) * a.g.num_tiles + tile] = corr_live ? 1 : 0;
     float* out = a.partial + (size_t(chunk) * a.g.num_tiles + tile) * 64 * 64;
 #pragma unroll
     for (int s = 0; s < 64; ++s) out[s * 64 + lane] = KAHAN ? acc[s] - comp[s] : acc[s];
@@ -327,8 +335,10 @@ __global__ void confocal_thresholds_kernel(const fbp_vec3* __restrict__ laser, u
 }
 
 __global__ void reduce_partials_kernel(const float* __restrict__ partial,
-                                       const float* __restrict__ corr, float* __restrict__ volume,
-                                       VolumeGeom g, uint32_t num_chunks, double scale,
+                                       const float* __restrict__ corr,
+                                       const uint8_t* __restrict__ corr_used,
+                                       float* __restrict__ volume, VolumeGeom g,
+                                       uint32_t num_chunks, double scale,
                                        const int* __restrict__ skip) {
     if (skip && *skip) return;
     // One thread per (tile, slot, lane) entry of a chunk's partial block, so
@@ -351,18 +361,19 @@ __global__ void reduce_partials_kernel(const float* __restrict__ partial,
     double sum = 0.0;
     for (uint32_t c = 0; c < num_chunks; ++c) {
         sum += double(partial[size_t(c) * per + i]);
-        if (corr) sum += double(corr[size_t(c) * per + i]);
+        if (corr && (!corr_used || corr_used[size_t(c) * g.num_tiles + tile]))
+            sum += double(corr[size_t(c) * per + i]);
     }
     volume[(size_t(z) * g.res[1] + vy) * g.res[0] + vx] = float(sum * scale);
 }
 
 fbp_status launch_reduce(fbp_context ctx, const float* partial, const float* corr, float* volume,
                          const VolumeGeom& g, uint32_t num_chunks, double scale,
-                         hipStream_t stream, const int* skip) {
+                         hipStream_t stream, const int* skip, const uint8_t* corr_used) {
     const size_t n = size_t(g.num_tiles) * g.slots * 64;
     const unsigned blocks = unsigned((n + 255) / 256);
     hipLaunchKernelGGL(reduce_partials_kernel, dim3(blocks), dim3(256), 0, stream, partial, corr,
-                       volume, g, num_chunks, scale, skip);
+                       corr_used, volume, g, num_chunks, scale, skip);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     return FBP_OK;
 }
@@ -471,7 +482,8 @@ fbp_status confocal_voxel_impl(fbp_context ctx, const fbp_reconstruction_info* i
     // one of the two paths writes, so they share the space.
     AlignedPlan ap = plan_aligned_confocal(ctx, *info, *tr);
     if (samples) ap.usable = false;
-    const size_t general_bytes = (2 * slab + thr_n) * sizeof(float) + 256;
+    const size_t flags_n = size_t(chunks) * a.g.num_tiles;
+    const size_t general_bytes = (2 * slab + thr_n) * sizeof(float) + flags_n + 512;
     const size_t body = std::max(general_bytes, ap.usable ? aligned_scratch_bytes(ap) : size_t(0));
     FBP_CHECK(ensure_scratch(ctx, body));
     if (ap.usable && !ctx->path_flag)
@@ -481,6 +493,7 @@ fbp_status confocal_voxel_impl(fbp_context ctx, const fbp_reconstruction_info* i
     a.partial = reinterpret_cast<float*>(base);
     a.corr = a.partial + slab;
     a.thr = a.corr + slab;
+    a.corr_used = reinterpret_cast<uint8_t*>(const_cast<float*>(a.thr) + thr_n);
     a.skip = ap.usable ? flag : nullptr;
 
     hipStream_t s = as_stream(stream);
@@ -504,7 +517,7 @@ fbp_status confocal_voxel_impl(fbp_context ctx, const fbp_reconstruction_info* i
         launch_confocal<FBP_STORAGE_BF16, FBP_LAYOUT_TIME_FASTEST>(a, pure, grid, shmem, s);
     }
     FBP_CHECK_HIP(ctx, hipGetLastError());
-    return launch_reduce(ctx, a.partial, a.corr, volume, a.g, chunks, 1.0, s, a.skip);
+    return launch_reduce(ctx, a.partial, a.corr, volume, a.g, chunks, 1.0, s, a.skip, a.corr_used);
 }
 
 }  // namespace fbp

@@ -558,11 +558,13 @@ def test_mis_reference_mode_matches_oracle(ctx):
     assert rel_l2(out.cpu().numpy(), ref) < 1e-4
 
 
-def test_mis_tiled_matches_oracle(ctx):
+@pytest.mark.parametrize("discard,t0", [(1, 0.0), (0, 0.3)])
+def test_mis_tiled_matches_oracle(ctx, discard, t0):
     """Tile-shared MIS: same samples and weights as the restatement, exact
     bins; values either 0 or >= 1e-2 so that weighting cannot cross the
-    1e-5 threshold."""
-    g = geom(n_scan=16, T=256, res=(32, 32, 20))
+    1e-5 threshold.  (1, 0.0): the pure chain's single-pass sub-tile kernel;
+    (0, 0.3): the tile kernel over chunks of the lists."""
+    g = geom(n_scan=16, T=256, res=(32, 32, 20), discard=discard, t0=t0)
     info = make_info(g)
     rng = np.random.default_rng(41)
     I = (10.0 ** rng.uniform(-2, 3, size=(256, 256))).astype(np.float32)
@@ -570,6 +572,24 @@ def test_mis_tiled_matches_oracle(ctx):
     noise = rng.random(4096).astype(np.float32)
     ref = oracle.mis_tiled(info, I, g.laser_targets, noise, 48)
     out = ctx.backproject_confocal_mis(info, cuda(I), cuda(g.laser_targets), cuda(noise), 48,
+                                       mode=MIS_TILED)
+    torch.cuda.synchronize()
+    assert_volume_close(out, ref)
+
+
+def test_mis_tiled_near_wall(ctx):
+    """Tile-shared MIS with voxel centres 2 mm in front of scan points (the
+    first depth slice, x and z on the scan grid; dist 4 mm keeps safeRCP of
+    dist^2 finite): those samples take the IEEE fallback of the single-pass
+    kernel."""
+    g = geom(n_scan=16, T=128, res=(16, 8, 16), hmin=(-0.5, -0.008, -0.5),
+             hmax=(0.5, 0.152, 0.5))
+    info = make_info(g)
+    rng = np.random.default_rng(47)
+    I = (10.0 ** rng.uniform(-2, 3, size=(256, 128))).astype(np.float32)
+    noise = rng.random(2048).astype(np.float32)
+    ref = oracle.mis_tiled(info, I, g.laser_targets, noise, 64)
+    out = ctx.backproject_confocal_mis(info, cuda(I), cuda(g.laser_targets), cuda(noise), 64,
                                        mode=MIS_TILED)
     torch.cuda.synchronize()
     assert_volume_close(out, ref)
