@@ -132,6 +132,13 @@ __device__ __forceinline__ float box_max_dist(const float p[3], const float lo[3
 // the (chunk, tile) blocks whose corr_used[chunk][tile] is set (every block
 // when corr_used is null).
 // skip (may be null): when *skip != 0 on the device the kernel returns.
+// The pure chain's threshold row S_k = min{s >= 0 : bin(s) >= k}, k = 0..T
+// (bp_confocal.hip), computed on stream s when (T, dt) differs from the
+// context's cached row; device pointer in *thr.  Calls on one context are
+// stream-ordered, so a later call on the same stream sees the row.
+fbp_status pure_threshold_row(fbp_context ctx, uint32_t T, float dt, hipStream_t s,
+                              const float** thr);
+
 fbp_status launch_reduce(fbp_context ctx, const float* partial, const float* corr, float* volume,
                          const VolumeGeom& g, uint32_t num_chunks, double scale,
                          hipStream_t stream, const int* skip = nullptr,

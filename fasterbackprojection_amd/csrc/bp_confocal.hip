@@ -149,10 +149,12 @@ __device__ __forceinline__ void confocal_voxel_body(const ConfocalArgs& a) {
     float acc[64], comp[64];
 #pragma unroll
     for (int i = 0; i < 64; ++i) acc[i] = comp[i] = 0.f;
-    // Near-voxel corrections (rare) go to a lane-owned slice of a.corr,
-    // cleared on the block's first near scan point; corr_used tells the
-    // reduction whether to read it.
+    // Near-voxel corrections (rare) go to a lane-owned slice of a.corr;
+    // corr_used tells the reduction whether any scan point of the block
+    // wrote it.
     float* corr = a.corr + (size_t(chunk) * a.g.num_tiles + tile) * 64 * 64;
+#pragma unroll 8
+    for (int s = 0; s < 64; ++s) corr[s * 64 + lane] = 0.f;
     bool corr_live = false;
 
     const uint32_t l_begin = chunk * a.chunk_len;
@@ -246,11 +248,7 @@ __device__ __forceinline__ void confocal_voxel_body(const ConfocalArgs& a) {
                 // Scan point (almost) on a voxel centre, or dist near zero:
                 // every voxel of the tile goes through the IEEE chain and
                 // lands in the lane-owned correction slots (acc untouched).
-                if (!corr_live) {
-#pragma unroll 8
-                    for (int s = 0; s < 64; ++s) corr[s * 64 + lane] = 0.f;
-                    corr_live = true;
-                }
+                corr_live = true;
                 const float off = rfl(P[P_OFF]);
                 for (int slot = 0; slot < 64; ++slot) {
                     const float s = rn_add(rn_add(sq(rn_sub(pick4(px, slot >> 4), lxp)),
@@ -365,6 +363,30 @@ __global__ void reduce_partials_kernel(const float* __restrict__ partial,
             sum += double(corr[size_t(c) * per + i]);
     }
     volume[(size_t(z) * g.res[1] + vy) * g.res[0] + vx] = float(sum * scale);
+}
+
+fbp_status pure_threshold_row(fbp_context ctx, uint32_t T, float dt, hipStream_t s,
+                              const float** thr) {
+    if (!ctx->pure_thr || ctx->pure_thr_T != T || ctx->pure_thr_dt != dt) {
+        const size_t bytes = (size_t(T) + 1) * sizeof(float);
+        if (bytes > ctx->pure_thr_cap) {
+            if (ctx->pure_thr) {
+                FBP_CHECK_HIP(ctx, hipStreamSynchronize(s));
+                FBP_CHECK_HIP(ctx, hipFree(ctx->pure_thr));
+                ctx->pure_thr = nullptr;
+            }
+            FBP_CHECK_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->pure_thr), bytes));
+            ctx->pure_thr_cap = bytes;
+        }
+        const fbp_vec3 zero{0.f, 0.f, 0.f};
+        hipLaunchKernelGGL(confocal_thresholds_kernel, dim3(unsigned((T + 1 + 255) / 256)), dim3(256),
+                           0, s, nullptr, 1u, T, 0.f, dt, 1u, zero, zero, ctx->pure_thr, nullptr);
+        FBP_CHECK_HIP(ctx, hipGetLastError());
+        ctx->pure_thr_T = T;
+        ctx->pure_thr_dt = dt;
+    }
+    *thr = ctx->pure_thr;
+    return FBP_OK;
 }
 
 fbp_status launch_reduce(fbp_context ctx, const float* partial, const float* corr, float* volume,
@@ -500,11 +522,15 @@ fbp_status confocal_voxel_impl(fbp_context ctx, const fbp_reconstruction_info* i
     ctx->path_flag_valid = ap.usable;
     if (ap.usable)
         FBP_CHECK(launch_aligned_confocal(ctx, ap, *info, *tr, laser_targets, volume, flag, base, s));
-    hipLaunchKernelGGL(confocal_thresholds_kernel, dim3(unsigned((thr_n + 255) / 256)), dim3(256),
-                       0, s, laser_targets, a.thr_rows, a.T, a.t0, a.dt, a.discard,
-                       info->laser_position, info->sensor_position, const_cast<float*>(a.thr),
-                       a.skip);
-    FBP_CHECK_HIP(ctx, hipGetLastError());
+    if (pure) {
+        FBP_CHECK(pure_threshold_row(ctx, a.T, a.dt, s, &a.thr));
+    } else {
+        hipLaunchKernelGGL(confocal_thresholds_kernel, dim3(unsigned((thr_n + 255) / 256)),
+                           dim3(256), 0, s, laser_targets, a.thr_rows, a.T, a.t0, a.dt, a.discard,
+                           info->laser_position, info->sensor_position, const_cast<float*>(a.thr),
+                           a.skip);
+        FBP_CHECK_HIP(ctx, hipGetLastError());
+    }
     const dim3 grid(chunks * a.g.num_tiles);
     const size_t shmem = size_t(a.batch) * (2 * a.W + kPrmWords) * sizeof(float);
     if (tr->layout == FBP_LAYOUT_TIME_MAJOR) {

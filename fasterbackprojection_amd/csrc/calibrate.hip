@@ -207,51 +207,117 @@ __global__ __launch_bounds__(256) void calib_select_kernel(const uint64_t* __res
 // the round issued first.  All threads read the same scan point's row at
 // nearby bins (coalesced).  partial[ci * chunks + chunk] = sum_t w[t] bp_t
 // (FP64, fixed order).
-__global__ __launch_bounds__(128) void calib_neigh_kernel(PointArgs a, Weights w,
+// pthr (pure chain, t0 = 0 and bounces discarded; else null): the
+// threshold row S_k, staged in LDS; each bin is then found as in the tile
+// kernels (a v_rsq estimate biased low and one threshold compare, exact),
+// with weight 1 / (2 r)^2 from the same estimate; samples within 4.5 mm of
+// dist 0 take point_sample's IEEE chain.  256 threads: tap t % 128 (< 125)
+// over half t / 128 of the chunk, kNeighU samples per round with every load
+// of the round issued first; the chunk's scan points staged in LDS.
+constexpr int kNeighU = 16;
+constexpr uint32_t kNeighMaxChunk = 128;
+
+// Backprojection of voxel (px, py, pz) over scan points [h0, h1) (positions
+// in slp, indexed from l0), FP64 sum.  No branch between the loads of a
+// round: a near sample (PURE) is masked there and redone afterwards.
+template <bool PURE>
+__device__ __forceinline__ double neigh_samples(const PointArgs& a, const float (*slp)[3],
+                                                const float* sthr, uint32_t l0, uint32_t h0,
+                                                uint32_t h1, float px, float py, float pz) {
+    const int T = int(a.T);
+    // Bias of the estimate: 2^-18 relative, far above its error.
+    const float cb = (1.0f / a.dt) * (1.0f - 3.814697265625e-06f);
+    double acc = 0.0;
+    bool any_near = false;
+    for (uint32_t lb = h0; lb < h1; lb += kNeighU) {
+        float light[kNeighU], wt[kNeighU];
+#pragma unroll
+        for (int u = 0; u < kNeighU; ++u) {
+            const uint32_t l = min(lb + uint32_t(u), h1 - 1);
+            const float lx = slp[l - l0][0], ly = slp[l - l0][1], lz = slp[l - l0][2];
+            const bool live = lb + uint32_t(u) < h1;
+            if constexpr (PURE) {
+                // 4 x the exact squared distance, glm::dot order.
+                const float A = rn_add(sq(rn_sub(px, lx)) * 4.0f, sq(rn_sub(py, ly)) * 4.0f);
+                const float s4 = A + sq(rn_sub(pz, lz)) * 4.0f;
+                const float rs = __builtin_amdgcn_rsqf(s4);
+                const int ka = int((s4 * rs) * cb);
+                const int kb = min(max(ka + 1, 0), T);
+                const float th = sthr[kb];
+                const int kk = ka + int((ka + 1 <= T) & (s4 >= th));
+                const bool near = s4 < 2.1e-5f;
+                any_near |= live && near;
+                const bool ok = live & !near & (kk >= 0) & (kk < T);
+                const float x = a.I[size_t(l) * a.T + uint32_t(ok ? kk : 0)];
+                light[u] = (ok & (x > kEps)) ? x : 0.f;
+                wt[u] = rs * rs;
+            } else {
+                // point_sample's arithmetic.
+                float dist = rn_sub(rn_mul(distance3(lx, ly, lz, px, py, pz), 2.0f), a.t0);
+                if (a.discard == 0)
+                    dist = rn_add(dist, rn_add(distance3(lx, ly, lz, a.lpos.x, a.lpos.y, a.lpos.z),
+                                               distance3(lx, ly, lz, a.spos.x, a.spos.y, a.spos.z)));
+                const float q = rn_div(dist, a.dt);
+                const bool ok = live & (q > -1.0f) & (q < float(a.T));
+                const float x = a.I[size_t(l) * a.T + uint32_t(ok ? int(q) : 0)];
+                light[u] = (ok & (x > kEps)) ? x : 0.f;
+                wt[u] = safe_rcp(rn_mul(dist, dist));
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kNeighU; ++u)
+            acc += PURE ? double(light[u] * wt[u]) : double(rn_mul(light[u], wt[u]));
+    }
+    if (PURE && any_near) {
+        // dist within 4.5 mm of 0: the IEEE chain (rare).
+        for (uint32_t l = h0; l < h1; ++l) {
+            const float lx = slp[l - l0][0], ly = slp[l - l0][1], lz = slp[l - l0][2];
+            const float A = rn_add(sq(rn_sub(px, lx)) * 4.0f, sq(rn_sub(py, ly)) * 4.0f);
+            if (A + sq(rn_sub(pz, lz)) * 4.0f < 2.1e-5f) acc += point_sample(a, l, px, py, pz);
+        }
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(256) void calib_neigh_kernel(PointArgs a, Weights w,
                                                           const uint32_t* __restrict__ sel,
                                                           uint32_t k, uint32_t chunk_len,
+                                                          const float* __restrict__ pthr,
                                                           double* __restrict__ partial) {
-    __shared__ double red[128];
+    __shared__ double red[256];
+    __shared__ float slp[kNeighMaxChunk][3];
+    extern __shared__ float sthr[];  // T + 1 thresholds x 4 (pure chain)
     const uint32_t ci = blockIdx.x, chunk = blockIdx.y, t = threadIdx.x;
+    const uint32_t tap = t & 127, half = t >> 7;
+    const uint32_t l0 = chunk * chunk_len, l1 = min(a.L, l0 + chunk_len);
+    for (uint32_t i = t; i < (l1 - l0) * 3; i += 256) {
+        const fbp_vec3 lp = a.laser[l0 + i / 3];
+        slp[i / 3][i % 3] = (i % 3 == 0) ? lp.x : (i % 3 == 1) ? lp.y : lp.z;
+    }
+    if (pthr)
+        for (uint32_t i = t; i <= a.T; i += 256) sthr[i] = pthr[i] * 4.0f;
+    __syncthreads();
     double acc = 0.0;
-    if (t < uint32_t(kTaps)) {
+    if (tap < uint32_t(kTaps)) {
         const uint32_t c = sel[ci];
         const int x = int(c % a.X), y = int((c / a.X) % a.Y), z = int(c / (a.X * a.Y));
-        const int nx = min(max(x + int(t % 5) - 2, 0), int(a.X) - 1);
-        const int ny = min(max(y + int((t / 5) % 5) - 2, 0), int(a.Y) - 1);
-        const int nz = min(max(z + int(t / 25) - 2, 0), int(a.Z) - 1);
+        const int nx = min(max(x + int(tap % 5) - 2, 0), int(a.X) - 1);
+        const int ny = min(max(y + int((tap / 5) % 5) - 2, 0), int(a.Y) - 1);
+        const int nz = min(max(z + int(tap / 25) - 2, 0), int(a.Z) - 1);
         const float px = voxel_center(a.hmin[0], a.vs[0], uint32_t(nx));
         const float py = voxel_center(a.hmin[1], a.vs[1], uint32_t(nz));
         const float pz = voxel_center(a.hmin[2], a.vs[2], uint32_t(ny));
-        const uint32_t l0 = chunk * chunk_len, l1 = min(a.L, l0 + chunk_len);
-        // point_sample's arithmetic, branch-free so the round's loads overlap.
-        constexpr int U = 8;
-        for (uint32_t lb = l0; lb < l1; lb += U) {
-            float light[U], wd[U];
-            bool ok[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint32_t l = min(lb + uint32_t(u), l1 - 1);
-                const fbp_vec3 lp = a.laser[l];
-                float dist = rn_sub(rn_mul(distance3(lp.x, lp.y, lp.z, px, py, pz), 2.0f), a.t0);
-                if (a.discard == 0)
-                    dist = rn_add(dist,
-                                  rn_add(distance3(lp.x, lp.y, lp.z, a.lpos.x, a.lpos.y, a.lpos.z),
-                                         distance3(lp.x, lp.y, lp.z, a.spos.x, a.spos.y, a.spos.z)));
-                const float q = rn_div(dist, a.dt);
-                ok[u] = lb + uint32_t(u) < l1 && q > -1.0f && q < float(a.T);
-                light[u] = a.I[size_t(l) * a.T + uint32_t(ok[u] ? int(q) : 0)];
-                wd[u] = safe_rcp(rn_mul(dist, dist));
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (ok[u] && light[u] > kEps) acc += double(rn_mul(light[u], wd[u]));
-        }
-        acc *= double(w.w[t]);
+        const uint32_t hl = (l1 - l0 + 1) / 2;
+        const uint32_t h0 = l0 + half * hl, h1 = min(l1, h0 + hl);
+        if (pthr)
+            acc = neigh_samples<true>(a, slp, sthr, l0, h0, h1, px, py, pz);
+        else
+            acc = neigh_samples<false>(a, slp, sthr, l0, h0, h1, px, py, pz);
+        acc *= double(w.w[tap]);
     }
     red[t] = acc;
     __syncthreads();
-    for (int s = 64; s > 0; s >>= 1) {
+    for (int s = 128; s > 0; s >>= 1) {
         if (int(t) < s) red[t] += red[t + s];
         __syncthreads();
     }
@@ -266,14 +332,21 @@ __global__ __launch_bounds__(256) void calib_apply_kernel(float* __restrict__ v,
                                                           const double* __restrict__ fit,
                                                           const double* __restrict__ partial,
                                                           uint32_t k, uint32_t chunks) {
+    __shared__ double lpart[2 * kMaxExtremes][4];
     __shared__ double lval[2 * kMaxExtremes];
     __shared__ float coef[2];
     const uint32_t t = threadIdx.x;
-    if (t < 2 * k) {
+    // Candidate c's chunk partials: four threads, each a fixed quarter in
+    // order, then the quarters in order (the same sum in every block).
+    for (uint32_t i = t; i < 2 * k * 4; i += 256) {
+        const uint32_t c = i / 4, q = i % 4;
+        const uint32_t per = (chunks + 3) / 4, c0 = q * per, c1 = min(chunks, c0 + per);
         double l = 0.0;
-        for (uint32_t c = 0; c < chunks; ++c) l += partial[size_t(t) * chunks + c];
-        lval[t] = l;
+        for (uint32_t j = c0; j < c1; ++j) l += partial[size_t(c) * chunks + j];
+        lpart[c][q] = l;
     }
+    __syncthreads();
+    if (t < 2 * k) lval[t] = ((lpart[t][0] + lpart[t][1]) + lpart[t][2]) + lpart[t][3];
     __syncthreads();
     if (t == 0) {
         double S[5];
@@ -381,10 +454,12 @@ fbp_status learned_calibrate_impl(fbp_context ctx, const fbp_reconstruction_info
         return fail(ctx, FBP_ERR_UNSUPPORTED, "calibrate: volume size out of range");
     const uint32_t k = extremes;
 
-    // Scan-point chunks per candidate: enough blocks to fill the device.
+    // Scan-point chunks per candidate: enough blocks to fill the device, at
+    // most kNeighMaxChunk scan points each.
     const uint32_t L = info->num_laser_targets;
     const uint32_t target = 8u * uint32_t(ctx->num_cus > 0 ? ctx->num_cus : 256);
     uint32_t chunks = std::max(1u, std::min(L, (target + 2 * k - 1) / (2 * k)));
+    chunks = std::max(chunks, (L + kNeighMaxChunk - 1) / kNeighMaxChunk);
     const uint32_t chunk_len = (L + chunks - 1) / chunks;
     chunks = (L + chunk_len - 1) / chunk_len;
 
@@ -414,8 +489,13 @@ fbp_status learned_calibrate_impl(fbp_context ctx, const fbp_reconstruction_info
     hipLaunchKernelGGL(calib_select_kernel, dim3(2), dim3(256), 0, s, cand, sums, nb, seg, k, sel,
                        fit);
     const PointArgs a = point_args(*info, static_cast<const float*>(tr->data), laser_targets);
-    hipLaunchKernelGGL(calib_neigh_kernel, dim3(2 * k, chunks), dim3(128), 0, s, a, w, sel, k,
-                       chunk_len, partial);
+    // Pure chain: thresholds from the context's cached row, staged in LDS.
+    const float* pthr = nullptr;
+    const size_t thr_lds = (size_t(a.T) + 1) * sizeof(float);
+    if (info->time_offset == 0.f && info->discard_first_last_bounces != 0 && thr_lds <= 49152)
+        FBP_CHECK(pure_threshold_row(ctx, a.T, a.dt, s, &pthr));
+    hipLaunchKernelGGL(calib_neigh_kernel, dim3(2 * k, chunks), dim3(256), pthr ? thr_lds : 0, s,
+                       a, w, sel, k, chunk_len, pthr, partial);
     hipLaunchKernelGGL(calib_apply_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, volume,
                        n, fit, partial, k, chunks);
     FBP_CHECK_HIP(ctx, hipGetLastError());

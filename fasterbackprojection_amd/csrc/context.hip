@@ -172,6 +172,7 @@ fbp_status fbp_context_destroy(fbp_context ctx) {
     if (ctx->path_flag) hipFree(ctx->path_flag);
     if (ctx->aligned_tables) hipFree(ctx->aligned_tables);
     if (ctx->pinned) hipHostFree(ctx->pinned);
+    if (ctx->pure_thr) hipFree(ctx->pure_thr);
     delete ctx;
     return FBP_OK;
 }

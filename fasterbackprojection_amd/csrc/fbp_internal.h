@@ -42,6 +42,13 @@ struct fbp_context_s {
     // sum array of fbp_backproject_exhaustive_voxel_noalloc (resolved at
     // context creation; nullptr when that failed).
     void* noalloc_scratch = nullptr;
+    // Threshold row of the pure confocal chain (t0 = 0, bounces discarded):
+    // it depends on (T, dt) only, so it is kept for the next call with the
+    // same pair (float[pure_thr_T + 1]).
+    float* pure_thr = nullptr;
+    uint32_t pure_thr_T = 0;
+    float pure_thr_dt = 0.f;
+    size_t pure_thr_cap = 0;
 };
 
 

@@ -112,9 +112,9 @@ def test_learned_path_within_two_percent(ctx):
     assert np.median(errs) < 0.02 and np.mean(errs) < 0.03, errs
 
 
-@pytest.mark.parametrize("res,ties", [((16, 16, 16), False), ((128, 128, 40), False),
-                                      ((128, 128, 40), True)])
-def test_calibration_matches_cpu_selection(ctx, res, ties):
+@pytest.mark.parametrize("res,ties,pure", [((16, 16, 16), False, True), ((128, 128, 40), False, True),
+                                           ((128, 128, 40), True, True), ((16, 16, 16), False, False)])
+def test_calibration_matches_cpu_selection(ctx, res, ties, pure):
     """fbp_learned_calibrate against a CPU restatement: least squares of the
     reference on the volume (FP64), the k lowest and k highest voxels by
     (value, index) order (np.lexsort), the exact LoG there from the FP64
@@ -122,13 +122,19 @@ def test_calibration_matches_cpu_selection(ctx, res, ties):
     noise independent of the scene, so the range depends on exactly which
     voxels are selected; 128 x 128 x 40 = 655360 voxels makes every block of
     the GPU's selection pass sort two 512-voxel chunks, and the tied case
-    (16 distinct values) checks the index order of ties."""
+    (16 distinct values) checks the index order of ties; pure=False (first
+    and last bounces kept, t0 = 0.05 m) takes the IEEE chain for the exact
+    neighbourhoods instead of the threshold row."""
+    import dataclasses
     import oracle
     from fasterbackprojection_amd.api import make_info
     from fasterbackprojection_amd.scene import C_LIGHT, confocal_geometry
 
     k = 16
     g = confocal_geometry(4, 1.0, 96, C_LIGHT * 128e-12, (-0.4, 0.3, -0.4), (0.4, 0.9, 0.4), res)
+    if not pure:
+        g = dataclasses.replace(g, discard_first_last_bounces=0, time_offset=0.05,
+                                laser_position=(0.1, -0.8, 0.2), sensor_position=(-0.2, -0.7, 0.1))
     info = make_info(g)
     rng = np.random.default_rng(71)
     I = (10.0 ** rng.uniform(-2, 1, size=(len(g.laser_targets), 96))).astype(np.float32)
