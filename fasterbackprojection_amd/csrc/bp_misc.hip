@@ -209,11 +209,41 @@ __device__ __forceinline__ float mis_uniform_or_seeded(const float* __restrict__
 }
 
 // Scan-point samples of every voxel tile for FBP_MIS_TILED.
-__global__ void mis_tile_samples_kernel(const double* __restrict__ sum, const double* __restrict__ cum,
-                                        uint32_t L, const float* __restrict__ noise,
-                                        uint32_t noise_size, uint64_t seed, uint32_t samples,
-                                        uint32_t tiles, uint32_t* __restrict__ sample_l,
-                                        float* __restrict__ sample_w) {
+// cum == nullptr: every block builds the inclusive FP64 prefix of sum in
+// LDS itself (thread t adds its contiguous range of rows in order, then a
+// scan over the 256 range totals), so no separate prefix launch is needed
+// (L <= kMisLdsRows).
+constexpr uint32_t kMisLdsRows = 8192;
+
+__global__ __launch_bounds__(256) void mis_tile_samples_kernel(
+    const double* __restrict__ sum, const double* __restrict__ cum_in, uint32_t L,
+    const float* __restrict__ noise, uint32_t noise_size, uint64_t seed, uint32_t samples,
+    uint32_t tiles, uint32_t* __restrict__ sample_l, float* __restrict__ sample_w) {
+    extern __shared__ double scum[];  // L (prefix built here)
+    __shared__ double part[256];
+    const double* cum = cum_in;
+    if (!cum_in) {
+        const uint32_t t = threadIdx.x;
+        const uint32_t per = (L + 255) / 256;
+        const uint32_t b = min(L, t * per), e = min(L, b + per);
+        double local = 0.0;
+        for (uint32_t r = b; r < e; ++r) local += sum[r];
+        part[t] = local;
+        __syncthreads();
+        for (uint32_t off = 1; off < 256; off <<= 1) {
+            const double v = t >= off ? part[t - off] : 0.0;
+            __syncthreads();
+            part[t] += v;
+            __syncthreads();
+        }
+        double run = part[t] - local;
+        for (uint32_t r = b; r < e; ++r) {
+            run += sum[r];
+            scum[r] = run;
+        }
+        __syncthreads();
+        cum = scum;
+    }
     const size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (id >= size_t(tiles) * samples) return;
     const uint32_t tile = uint32_t(id / samples), i = uint32_t(id % samples);
@@ -541,10 +571,14 @@ fbp_status fbp_backproject_confocal_mis(fbp_context ctx, const fbp_reconstructio
         float* d_w = reinterpret_cast<float*>(d_l + ns);
         hipLaunchKernelGGL(row_sum_kernel, dim3(L), dim3(256), 0, s,
                            static_cast<const float*>(tr->data), L, T, d_sum);
-        hipLaunchKernelGGL(row_prefix_kernel, dim3(1), dim3(1024), 0, s, d_sum, L, d_cum);
-        hipLaunchKernelGGL(mis_tile_samples_kernel, dim3(unsigned((ns + 255) / 256)), dim3(256), 0, s,
-                           d_sum, d_cum, L, params->noise, params->noise_size, params->seed,
-                           params->num_samples, tiles, d_l, d_w);
+        // Short row lists: every samples block builds the prefix in LDS.
+        const bool lds_prefix = L <= kMisLdsRows;
+        if (!lds_prefix)
+            hipLaunchKernelGGL(row_prefix_kernel, dim3(1), dim3(1024), 0, s, d_sum, L, d_cum);
+        hipLaunchKernelGGL(mis_tile_samples_kernel, dim3(unsigned((ns + 255) / 256)), dim3(256),
+                           lds_prefix ? size_t(L) * sizeof(double) : 0, s, d_sum,
+                           lds_prefix ? nullptr : d_cum, L, params->noise, params->noise_size,
+                           params->seed, params->num_samples, tiles, d_l, d_w);
         FBP_CHECK_HIP(ctx, hipGetLastError());
         return confocal_voxel_impl(ctx, info, tr, laser_targets, volume, stream, d_l, d_w,
                                    params->num_samples);

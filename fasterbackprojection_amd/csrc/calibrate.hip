@@ -96,46 +96,45 @@ __device__ __forceinline__ uint64_t key_high(float v, uint32_t i) {
     return (uint64_t(~ordered_bits(v)) << 32) | i;
 }
 
-// One pass over the volume.  Block b owns voxels [b seg, (b + 1) seg):
-// (1) least-squares sums of reference r on volume v in FP64,
-// sums[b][0..4] = (n, sum v, sum v^2, sum r, sum v r); (2) its k lowest and
-// k highest keys, cand[0 or 1][b][0..k), over 512-voxel chunks merged with
-// the best k so far.
+// One pass over the volume, blockIdx.y = side (0: lows, 1: highs).  Block
+// b owns voxels [b seg, (b + 1) seg): its k keys of the side,
+// cand[side][b][0..k), over 512-voxel chunks merged with the best k so far;
+// the lows' blocks also form the least-squares sums of reference r on
+// volume v in FP64, sums[b][0..4] = (n, sum v, sum v^2, sum r, sum v r).
 __global__ __launch_bounds__(256) void calib_scan_kernel(
     const float* __restrict__ v, const float* __restrict__ r, uint32_t n, uint32_t seg,
     uint32_t k, uint64_t* __restrict__ cand, double* __restrict__ sums) {
-    __shared__ uint64_t blo[kMaxExtremes], bhi[kMaxExtremes];
+    __shared__ uint64_t best[kMaxExtremes];
     __shared__ double red[5][256];
     const uint32_t t = threadIdx.x, b = blockIdx.x, nb = gridDim.x;
+    const bool high = blockIdx.y == 1;
     const uint32_t begin = b * seg, end = min(n, begin + seg);
     double acc[5] = {0, 0, 0, 0, 0};
     bool first = true;
     for (uint32_t c0 = begin; c0 < end; c0 += kScanChunk) {
-        uint64_t kl[2], kh[2];
+        uint64_t kk[2];
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             const uint32_t i = c0 + t + 256u * uint32_t(j);
-            kl[j] = kh[j] = kPadKey;
+            kk[j] = kPadKey;
             if (i < end) {
                 const float x = v[i];
-                const double xd = x, yd = r[i];
-                acc[0] += 1.0;
-                acc[1] += xd;
-                acc[2] += xd * xd;
-                acc[3] += yd;
-                acc[4] += xd * yd;
-                kl[j] = key_low(x, i);
-                kh[j] = key_high(x, i);
+                if (!high) {
+                    const double xd = x, yd = r[i];
+                    acc[0] += 1.0;
+                    acc[1] += xd;
+                    acc[2] += xd * xd;
+                    acc[3] += yd;
+                    acc[4] += xd * yd;
+                }
+                kk[j] = high ? key_high(x, i) : key_low(x, i);
             }
         }
-        block_topk<2, kMaxExtremes>(kl, k, blo, !first, blo);
-        block_topk<2, kMaxExtremes>(kh, k, bhi, !first, bhi);
+        block_topk<2, kMaxExtremes>(kk, k, best, !first, best);
         first = false;
     }
-    if (t < k) {
-        cand[size_t(b) * k + t] = blo[t];
-        cand[(size_t(nb) + b) * k + t] = bhi[t];
-    }
+    if (t < k) cand[(size_t(high ? nb : 0) + b) * k + t] = best[t];
+    if (high) return;
     for (int q = 0; q < 5; ++q) red[q][t] = acc[q];
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -484,7 +483,7 @@ fbp_status learned_calibrate_impl(fbp_context ctx, const fbp_reconstruction_info
     double* fit = reinterpret_cast<double*>(take(5 * 8));
     double* partial = reinterpret_cast<double*>(take(size_t(2) * k * chunks * 8));
 
-    hipLaunchKernelGGL(calib_scan_kernel, dim3(nb), dim3(256), 0, s, volume, reference,
+    hipLaunchKernelGGL(calib_scan_kernel, dim3(nb, 2), dim3(256), 0, s, volume, reference,
                        uint32_t(n), seg, k, cand, sums);
     hipLaunchKernelGGL(calib_select_kernel, dim3(2), dim3(256), 0, s, cand, sums, nb, seg, k, sel,
                        fit);
