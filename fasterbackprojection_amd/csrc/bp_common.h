@@ -129,8 +129,8 @@ __device__ __forceinline__ float box_max_dist(const float p[3], const float lo[3
 // Sums per-chunk partial tiles in FP64 (fixed order) and scatters them to
 // the [Z][Y][X] volume.  partial layout: [chunk][tile][slot][lane(64)].
 // corr (may be null) is a second buffer of the same layout added in, for
-// the (chunk, tile) blocks whose corr_used[chunk][tile] is set (every block
-// when corr_used is null).
+// the (chunk, tile) blocks whose corr_used[chunk][tile] is set (corr_used is
+// required with corr).
 // skip (may be null): when *skip != 0 on the device the kernel returns.
 // The pure chain's threshold row S_k = min{s >= 0 : bin(s) >= k}, k = 0..T
 // (bp_confocal.hip), computed on stream s when (T, dt) differs from the

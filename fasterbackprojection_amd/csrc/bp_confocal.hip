@@ -332,6 +332,7 @@ __global__ void confocal_thresholds_kernel(const fbp_vec3* __restrict__ laser, u
     thr[idx] = __uint_as_float(hi);
 }
 
+template <bool CORR>
 __global__ void reduce_partials_kernel(const float* __restrict__ partial,
                                        const float* __restrict__ corr,
                                        const uint8_t* __restrict__ corr_used,
@@ -356,11 +357,30 @@ __global__ void reduce_partials_kernel(const float* __restrict__ partial,
     const uint32_t oy = (lane >> 4) + 4u * (slot % g.lt[1]);
     const uint32_t vx = tx * g.tile[0] + ox, vy = ty * g.tile[1] + oy, z = tz * g.tile[2] + oz;
     if (vx >= g.res[0] || vy >= g.res[1] || z >= g.res[2]) return;
+    // Eight chunks per round, every load of the round issued before the sums
+    // (same order of additions: chunk by chunk, partial then corr).
     double sum = 0.0;
-    for (uint32_t c = 0; c < num_chunks; ++c) {
-        sum += double(partial[size_t(c) * per + i]);
-        if (corr && (!corr_used || corr_used[size_t(c) * g.num_tiles + tile]))
-            sum += double(corr[size_t(c) * per + i]);
+    constexpr uint32_t R = 8;
+    for (uint32_t c0 = 0; c0 < num_chunks; c0 += R) {
+        float p[R], q[R];
+        bool use[R];
+#pragma unroll
+        for (uint32_t u = 0; u < R; ++u) {
+            const uint32_t c = min(c0 + u, num_chunks - 1);
+            p[u] = partial[size_t(c) * per + i];
+            if constexpr (CORR) {
+                use[u] = corr_used[size_t(c) * g.num_tiles + tile] != 0;
+                q[u] = corr[size_t(c) * per + i];
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < R; ++u) {
+            if (c0 + u < num_chunks) {
+                sum += double(p[u]);
+                if constexpr (CORR)
+                    if (use[u]) sum += double(q[u]);
+            }
+        }
     }
     volume[(size_t(z) * g.res[1] + vy) * g.res[0] + vx] = float(sum * scale);
 }
@@ -394,8 +414,13 @@ fbp_status launch_reduce(fbp_context ctx, const float* partial, const float* cor
                          hipStream_t stream, const int* skip, const uint8_t* corr_used) {
     const size_t n = size_t(g.num_tiles) * g.slots * 64;
     const unsigned blocks = unsigned((n + 255) / 256);
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(blocks), dim3(256), 0, stream, partial, corr,
-                       corr_used, volume, g, num_chunks, scale, skip);
+    if (corr && !corr_used) return fail(ctx, FBP_ERR_INVALID_ARGUMENT, "reduce: corr without flags");
+    if (corr)
+        hipLaunchKernelGGL(reduce_partials_kernel<true>, dim3(blocks), dim3(256), 0, stream,
+                           partial, corr, corr_used, volume, g, num_chunks, scale, skip);
+    else
+        hipLaunchKernelGGL(reduce_partials_kernel<false>, dim3(blocks), dim3(256), 0, stream,
+                           partial, corr, corr_used, volume, g, num_chunks, scale, skip);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     return FBP_OK;
 }
