@@ -130,7 +130,10 @@ __global__ __launch_bounds__(256) void calib_scan_kernel(
                 kk[j] = high ? key_high(x, i) : key_low(x, i);
             }
         }
-        block_topk<2, kMaxExtremes>(kk, k, best, !first, best);
+        // This chunk's keys and the best k so far (one more key per thread).
+        uint64_t kc[3] = {kk[0], kk[1], (!first && t < k) ? best[t] : kPadKey};
+        __syncthreads();  // best is read above and rewritten below
+        block_select_k<3>(kc, k, best);
         first = false;
     }
     if (t < k) cand[(size_t(high ? nb : 0) + b) * k + t] = best[t];
@@ -183,7 +186,7 @@ __global__ __launch_bounds__(256) void calib_select_kernel(const uint64_t* __res
         }
         if (t < 5) fit[t] = red[t][0];
     }
-    block_topk<4, kMaxExtremes>(a, k, nullptr, false, sk);
+    block_select_k<4>(a, k, sk);
     // (fewer than k blocks: the pads select no block)
     constexpr uint32_t kNoBlock = 0xffffffffu;
     if (t < k) blk[t] = sk[t] == kPadKey ? kNoBlock : uint32_t(sk[t] & 0xffffffffu) / seg;
@@ -195,7 +198,7 @@ __global__ __launch_bounds__(256) void calib_select_kernel(const uint64_t* __res
         a[j] = kPadKey;
         if (i < k * k && blk[i / k] != kNoBlock) a[j] = c[size_t(blk[i / k]) * k + i % k];
     }
-    block_topk<4, kMaxExtremes>(a, k, nullptr, false, sk);
+    block_select_k<4>(a, k, sk);
     if (t < k) sel[side * k + t] = uint32_t(sk[t] & 0xffffffffu);
 }
 

@@ -1,7 +1,5 @@
-// k smallest of 64-bit keys held by a workgroup of 256 threads (4 waves),
-// by k rounds of wave-wide minima on DPP lane moves (no LDS round trips
-// inside a round), then one merge round in wave 0.  Keys are unique
-// (callers put an index in the low word), so every round has one winner.
+// k smallest of 64-bit keys held by a workgroup of 256 threads.  Keys are
+// unique (callers put an index in the low word) apart from pads.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -12,63 +10,89 @@ namespace fbp {
 
 constexpr uint64_t kTopkPad = ~uint64_t(0);
 
-// Minimum over the 64 lanes of a full wave (every lane gets it).  DPP:
-// quad_perm [1,0,3,2] and [2,3,0,1], row_half_mirror, row_mirror give the
-// minimum of each 16-lane row; the four rows are read out by lane index.
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
-    v = min(v, uint32_t(__builtin_amdgcn_mov_dpp(int(v), 0xB1, 0xF, 0xF, false)));
-    v = min(v, uint32_t(__builtin_amdgcn_mov_dpp(int(v), 0x4E, 0xF, 0xF, false)));
-    v = min(v, uint32_t(__builtin_amdgcn_mov_dpp(int(v), 0x141, 0xF, 0xF, false)));
-    v = min(v, uint32_t(__builtin_amdgcn_mov_dpp(int(v), 0x140, 0xF, 0xF, false)));
-    const uint32_t a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
-    const uint32_t c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
-    return min(min(a, b), min(c, d));
-}
-
-__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
-    const uint32_t hi = wave_min_u32(uint32_t(v >> 32));
-    const uint32_t lo = wave_min_u32(uint32_t(v >> 32) == hi ? uint32_t(v) : 0xffffffffu);
-    return (uint64_t(hi) << 32) | lo;
-}
-
-// The k smallest of the wave's 64 x E keys, ascending, into out[0..k)
-// (written by lane 0).  Called by a full wave.
+// Block-wide (256 threads): the k smallest of the block's 256 x E unique
+// keys (pads kTopkPad may repeat; they are chosen only when fewer than k
+// other keys exist), ascending, into res[0..k) (LDS), k <= 256.  Radix selection
+// of the k-th smallest key, 8 bits per pass from the top (a 256-bin LDS
+// histogram of the keys that match the digits chosen so far; wave 0 finds
+// the bin holding the rank), then the keys <= that key are gathered and
+// ranked.  No per-candidate rounds: 8 passes whatever k is.
 template <int E>
-__device__ __forceinline__ void wave_topk(uint64_t (&key)[E], uint32_t k, uint64_t* out) {
-    const uint32_t lane = threadIdx.x & 63;
-    for (uint32_t r = 0; r < k; ++r) {
-        uint64_t best = key[0];
-#pragma unroll
-        for (int j = 1; j < E; ++j) best = key[j] < best ? key[j] : best;
-        const uint64_t w = wave_min_u64(best);
-        if (lane == 0) out[r] = w;
+__device__ void block_select_k(const uint64_t (&key)[E], uint32_t k, uint64_t* res) {
+    __shared__ uint32_t hist[256];
+    __shared__ uint64_t s_pref;
+    __shared__ uint32_t s_rank, s_cnt;
+    __shared__ uint64_t gath[256];
+    const uint32_t t = threadIdx.x, lane = t & 63;
+    if (t == 0) {
+        s_pref = 0;
+        s_rank = k;  // 1-based rank of the wanted key among the matching keys
+        s_cnt = 0;
+    }
+    uint64_t mask = 0;
+    for (int shift = 56; shift >= 0; shift -= 8) {
+        hist[t] = 0;
+        __syncthreads();
+        const uint64_t pref = s_pref;
 #pragma unroll
         for (int j = 0; j < E; ++j)
-            if (key[j] == w) key[j] = kTopkPad;  // pads stay pads
-    }
-}
-
-// Block-wide: the k smallest (k <= KMAX) of the block's 256 x E keys and of
-// prev[0..k) when has_prev, ascending, into res[0..k) (LDS).  res may be
-// prev.  Called by all 256 threads.
-template <int E, int KMAX>
-__device__ void block_topk(uint64_t (&key)[E], uint32_t k, const uint64_t* prev, bool has_prev,
-                           uint64_t* res) {
-    __shared__ uint64_t part[4 * KMAX + KMAX];
-    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    wave_topk<E>(key, k, part + wave * KMAX);
-    if (wave == 0 && lane < k) part[4 * KMAX + lane] = has_prev ? prev[lane] : kTopkPad;
-    __syncthreads();
-    if (wave == 0) {
-        constexpr int M = (5 * KMAX + 63) / 64;
-        uint64_t m[M];
-#pragma unroll
-        for (int j = 0; j < M; ++j) {
-            const uint32_t i = lane + 64u * uint32_t(j);
-            const uint32_t src = i / KMAX, r = i % KMAX;
-            m[j] = (i < 5u * KMAX && r < k) ? part[src * KMAX + r] : kTopkPad;
+            if ((key[j] & mask) == pref) atomicAdd(&hist[uint32_t(key[j] >> shift) & 255u], 1u);
+        __syncthreads();
+        if (t < 64) {
+            // Lane owns bins 4 lane .. 4 lane + 3; inclusive scan of the lane
+            // totals across the wave.
+            const uint32_t b0 = hist[4 * lane], b1 = hist[4 * lane + 1], b2 = hist[4 * lane + 2],
+                           b3 = hist[4 * lane + 3];
+            const uint32_t tot = b0 + b1 + b2 + b3;
+            uint32_t inc = tot;
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t o = __shfl_up(inc, off, 64);
+                if (int(lane) >= off) inc += o;
+            }
+            const uint32_t exc = inc - tot, r = s_rank;
+            if (r > exc && r <= inc) {
+                uint32_t d = 4 * lane, before = exc;
+                const uint32_t bs[4] = {b0, b1, b2, b3};
+                for (int q = 0; q < 4; ++q) {
+                    if (r <= before + bs[q]) {
+                        d = 4 * lane + uint32_t(q);
+                        break;
+                    }
+                    before += bs[q];
+                }
+                s_pref = pref | (uint64_t(d) << shift);
+                s_rank = r - before;
+            }
         }
-        wave_topk<M>(m, k, res);
+        mask |= uint64_t(255) << shift;
+        __syncthreads();
+    }
+    // Keys below the k-th first (fewer than k), then copies of the k-th
+    // itself (one, or pads when fewer than k other keys exist).
+    const uint64_t kth = s_pref;
+#pragma unroll
+    for (int j = 0; j < E; ++j)
+        if (key[j] < kth) gath[atomicAdd(&s_cnt, 1u)] = key[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < E; ++j)
+        if (key[j] == kth) {
+            const uint32_t slot = atomicAdd(&s_cnt, 1u);
+            if (slot < k) gath[slot] = key[j];
+        }
+    __syncthreads();
+    const uint32_t got = min(s_cnt, k);
+    if (t < k) {
+        // Fewer than k keys at or below the k-th (only when pads fill in):
+        // the rest of res is pads.
+        const uint64_t v = t < got ? gath[t] : kTopkPad;
+        uint32_t rank = t;
+        if (t < got) {
+            rank = 0;
+            for (uint32_t j = 0; j < got; ++j)
+                rank += (gath[j] < v || (gath[j] == v && j < t)) ? 1u : 0u;
+        }
+        res[rank] = v;
     }
     __syncthreads();
 }
