@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) void bp_points_kernel(PointArgs a, const uint3
 
 // ---- extremes: (value, index) pairs as 64-bit keys whose ascending order
 // is value ascending (lows) or descending (highs), ties by smaller index.
-constexpr int kScanChunk = 512;     // voxels per block_topk round (2 per thread)
+constexpr int kScanChunk = 512;     // voxels per selection round (2 per thread)
 constexpr uint32_t kMaxScanBlocks = 1024;
 constexpr uint64_t kPadKey = kTopkPad;
 
