@@ -86,6 +86,7 @@ SIGNATURES = {
     "fbp_context_get_option": (_S, [_CTX, C.c_int32, C.POINTER(C.c_int32)]),
     "fbp_context_last_confocal_path": (_S, [_CTX, C.POINTER(C.c_int32)]),
     "fbp_backproject_confocal_voxel": (_S, [_CTX, C.POINTER(ReconstructionInfo), C.POINTER(Transient), _P, _P, _P]),
+    "fbp_backproject_confocal_splat": (_S, [_CTX, C.POINTER(ReconstructionInfo), C.POINTER(Transient), _P, _P, _P]),
     "fbp_backproject_exhaustive_voxel": (_S, [_CTX, C.POINTER(ReconstructionInfo), C.POINTER(Transient), _P, _P, _P, _P]),
     "fbp_backproject_exhaustive_voxel_noalloc": (_S, [_CTX, C.POINTER(ReconstructionInfo), C.POINTER(Transient), _P, _P, _P, _P]),
     "fbp_backproject_depth_confocal": (_S, [_CTX, C.POINTER(ReconstructionInfo), C.POINTER(Transient), _P, _P, C.c_uint32, _P, _P]),

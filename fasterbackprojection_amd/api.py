@@ -215,6 +215,20 @@ class Context:
             self.handle, C.byref(info), C.byref(tr), _ptr(laser_targets), _ptr(vol), _stream()))
         return vol
 
+    def backproject_confocal_splat(self, info, intensity, laser_targets, *,
+                                   layout=LAYOUT_TIME_FASTEST, out=None):
+        """Scatter form (fbp_backproject_confocal_splat): every lit sample
+        splatted into the voxels of its shell, in place of
+        DrawEllipsoids::solveBackprojection (FB/DrawEllipsoids.cpp:332-440).
+        Same volume as backproject_confocal; F32 storage."""
+        _need_transient(info, intensity, STORAGE_F32, None, exhaustive=False)
+        _need_points(laser_targets, "laser_targets", info.num_laser_targets)
+        vol = self._volume(info, out)
+        tr = self._transient(intensity, STORAGE_F32, layout, None)
+        self.check(self.lib.fbp_backproject_confocal_splat(
+            self.handle, C.byref(info), C.byref(tr), _ptr(laser_targets), _ptr(vol), _stream()))
+        return vol
+
     def backproject_exhaustive(self, info, intensity, laser_targets, sensor_targets, *,
                                storage=STORAGE_F32, scales=None, out=None):
         """backprojectExhaustiveVoxel (FB/transient_reconstruction.cuh:138-166)."""

@@ -139,6 +139,13 @@ __device__ __forceinline__ float box_max_dist(const float p[3], const float lo[3
 fbp_status pure_threshold_row(fbp_context ctx, uint32_t T, float dt, hipStream_t s,
                               const float** thr);
 
+// Threshold rows S_k of the confocal chain, one per scan point (rows = L,
+// laser required) or the pure chain's single row (rows = 1, laser unused):
+// thr[row * (T + 1) + k], k = 0..T.
+fbp_status launch_confocal_thresholds(fbp_context ctx, const fbp_vec3* laser, uint32_t rows,
+                                      uint32_t T, float t0, float dt, uint32_t discard,
+                                      fbp_vec3 lpos, fbp_vec3 spos, float* thr, hipStream_t s);
+
 fbp_status launch_reduce(fbp_context ctx, const float* partial, const float* corr, float* volume,
                          const VolumeGeom& g, uint32_t num_chunks, double scale,
                          hipStream_t stream, const int* skip = nullptr,

@@ -409,6 +409,16 @@ fbp_status pure_threshold_row(fbp_context ctx, uint32_t T, float dt, hipStream_t
     return FBP_OK;
 }
 
+fbp_status launch_confocal_thresholds(fbp_context ctx, const fbp_vec3* laser, uint32_t rows,
+                                      uint32_t T, float t0, float dt, uint32_t discard,
+                                      fbp_vec3 lpos, fbp_vec3 spos, float* thr, hipStream_t s) {
+    const size_t n = size_t(rows) * (T + 1);
+    hipLaunchKernelGGL(confocal_thresholds_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0,
+                       s, laser, rows, T, t0, dt, discard, lpos, spos, thr, nullptr);
+    FBP_CHECK_HIP(ctx, hipGetLastError());
+    return FBP_OK;
+}
+
 fbp_status launch_reduce(fbp_context ctx, const float* partial, const float* corr, float* volume,
                          const VolumeGeom& g, uint32_t num_chunks, double scale,
                          hipStream_t stream, const int* skip, const uint8_t* corr_used) {

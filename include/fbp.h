@@ -153,6 +153,24 @@ fbp_status fbp_backproject_confocal_voxel(fbp_context ctx, const fbp_reconstruct
                                           const fbp_vec3* laser_targets, float* volume,
                                           void* stream);
 
+/* Scatter form of the confocal backprojection; stands in for
+ * DrawEllipsoids::solveBackprojection (FB/DrawEllipsoids.cpp:332-440), which
+ * rasterises one half-ellipsoid instance per lit sample (l, t) (I > epsilon,
+ * FB/DrawEllipsoids.cpp:29-56) and adds its intensity into the voxel under
+ * each fragment (FB/assets/shaders/shading/backprojection-fs.glsl:24).  Here
+ * each lit sample (I > 1e-5) adds I * safeRCP(dist^2) into exactly the voxels
+ * whose bin for scan point l is t under backprojectConfocalVoxel's arithmetic,
+ * so the volume equals fbp_backproject_confocal_voxel's within FP32 summation
+ * order.  Work scales with the voxels that lie in lit shells, not with the
+ * full (voxel, scan point) product.  Deterministic (FP64 chunk reduction).
+ * Storage F32 (either layout); FBP_ERR_UNSUPPORTED for F16/BF16, with the
+ * voxel-scattering option set, when a 16 x 16 x 32 voxel tile spans more than
+ * 4095 bins, or for a negative depth voxel size. */
+fbp_status fbp_backproject_confocal_splat(fbp_context ctx, const fbp_reconstruction_info* info,
+                                          const fbp_transient* transient,
+                                          const fbp_vec3* laser_targets, float* volume,
+                                          void* stream);
+
 /* Replaces backprojectExhaustiveVoxel + Laser::reconstructAABBExhaustive
  * (FB/transient_reconstruction.cuh:138-166, FB/Laser.cu:500-516).  Visits every
  * (laser, sensor) pair with l = ls / S, s = ls % S (the reference splits by L

@@ -108,6 +108,7 @@ def test_entry_points_reject_null_context_without_gpu():
     assert lib.fbp_normalize_minmax(n, n, 16, n) == 1
     assert lib.fbp_lct(n, n, n, n, n) == 1
     assert lib.fbp_backproject_confocal_points(n, n, n, n, n, 1, n, n) == 1
+    assert lib.fbp_backproject_confocal_splat(n, n, n, n, n, n) == 1
     assert lib.fbp_render_confocal(n, n, 1, n, n, 1, 16, 0.01, n, n) == 1
 
 
