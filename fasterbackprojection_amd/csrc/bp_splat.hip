@@ -17,16 +17,24 @@
 //
 // Work split.  A workgroup owns a 16 x 16 x 32 voxel tile, one thread per
 // (vx, vy) column of 32 depths, and a chunk of scan points.  Per scan point
-// it stages the tile's window of bins in LDS: sample (0 unless I > 1e-5),
-// threshold, and a bitmap of lit bins, double-buffered so the next scan
-// point's loads are in flight during the walk.  Along a column s is monotone
-// in depth on each side of the scan point's depth, so each thread walks its
-// column once per scan point: it jumps to the first voxel of the next lit
-// shell (a square-root estimate corrected by exact compares), adds the sample
-// into every voxel of that shell, and moves on.  Unlit shells, and shells that
-// pass between two voxel centres, cost one bitmap or threshold lookup.  The
-// column is thread-private LDS (no atomics); per-chunk partial volumes are
-// summed in FP64 in chunk order, so results are deterministic.
+// it stages the tile's window of bins in LDS (sample, 0 unless I > 1e-5;
+// threshold; a bitmap and count of lit bins) while the next scan point's
+// loads are in flight.  A window with no lit bin is skipped by the whole
+// workgroup.  Otherwise each thread splats the lit shells into its column:
+//  * sparse windows (under 1/64 of the bins lit): along a column s is
+//    monotone in depth on each side of the scan point's depth, so the thread
+//    walks the column once, jumping (square-root estimate, checked exactly)
+//    to the first voxel of the next lit shell and adding the sample into
+//    every voxel of that shell;
+//  * denser windows: each voxel's shell is found directly (bin estimate
+//    biased low, two threshold compares), eight independent voxels per pass,
+//    unlit shells adding 0, the sums in registers;
+//  * a voxel within 4.5 mm of zero distance: the walk with the reference's
+//    IEEE weights (safeRCP's clamp included).
+// Columns are thread-private (LDS or registers, no atomics); per-chunk
+// partial volumes are summed in FP64 in chunk order, so results are
+// deterministic.  Chunks are sized for two full rounds of resident
+// workgroups.
 #include "bp_common.h"
 
 namespace fbp {
@@ -64,10 +72,10 @@ struct SplatScan {
     bool exact;   // near the scan point or zero distance: IEEE weights, exact bin search
 };
 
-// Walk of a column per scan point: kExact (robust search, IEEE weights; the
-// rare near case), kStep (every voxel in turn: dense windows, where the next
-// lit shell is rarely more than a voxel away), kJump (sparse windows: skip to
-// the next lit shell).
+// Treatment of a column per scan point: kExact (walk with exact bin search
+// and IEEE weights; a voxel near zero distance), kStep (splat_dense: every
+// voxel's shell found directly), kJump (walk that jumps from lit shell to lit
+// shell; sparse windows).
 enum SplatMode { kExact = 0, kStep = 1, kJump = 2 };
 
 __device__ __forceinline__ SplatScan splat_scan(const SplatArgs& a, uint32_t l, const float lo[3],

@@ -58,6 +58,13 @@ CASES = {
                                          hmin=(-0.4, -0.3, -0.45), hmax=(0.45, 0.9, 0.4)), 0.05),
     "thick_shells": (dict(n_scan=8, T=256, res=(33, 17, 70), hmin=(-0.1, 0.4, -0.1),
                           hmax=(0.1, 0.6, 0.1)), 0.5),
+    # under 1/64 of a window's bins lit: the jump walk
+    "pure_jump": (dict(n_scan=8, T=1024, res=(20, 18, 37)), 0.01),
+    "general_jump": (dict(n_scan=8, T=1024, res=(17, 9, 40), discard=0, t0=0.37), 0.008),
+    "pure_jump_crosses_wall": (dict(n_scan=6, T=1024, res=(9, 7, 40), hmin=(-0.4, -0.3, -0.45),
+                                    hmax=(0.45, 0.9, 0.4)), 0.01),
+    "thick_shells_jump": (dict(n_scan=8, T=1024, res=(33, 17, 70), hmin=(-0.1, 0.4, -0.1),
+                               hmax=(0.1, 0.6, 0.1)), 0.01),
 }
 
 
@@ -97,6 +104,31 @@ def test_splat_bins_exact(ctx, discard, t0):
         rec = recovered_bins(vol.cpu().numpy(), dist)
         bad = np.argwhere(rec != bins)
         assert bad.size == 0, (l, bad[:5].tolist())
+
+
+@pytest.mark.parametrize("discard,t0", [(1, 0.0), (0, 0.37)])
+def test_splat_bins_exact_sparse(ctx, discard, t0):
+    """Ramp on two bins per scan point only (about 1 % of the window lit: the
+    jump walk), chosen among the bins the voxels reach: voxels whose bin is
+    lit recover it exactly, the others hold 0."""
+    g = geom(T=1024, discard=discard, t0=t0)
+    info = make_info(g)
+    lasers = cuda(g.laser_targets)
+    rng = np.random.default_rng(11)
+    hits = 0
+    for l in range(1, 64, 5):
+        bins, dist = oracle.confocal_bins(info, g.laser_targets, l)
+        reach = np.unique(bins[bins >= 0])
+        lit = rng.choice(reach, size=2, replace=False)
+        I = np.zeros((64, 1024), np.float32)
+        I[l, lit] = lit.astype(np.float32) + 1.0
+        vol = ctx.backproject_confocal_splat(info, cuda(I), lasers)
+        want = np.where(np.isin(bins, lit), bins, -1)
+        rec = recovered_bins(vol.cpu().numpy(), dist)
+        bad = np.argwhere(rec != want)
+        assert bad.size == 0, (l, bad[:5].tolist())
+        hits += int((want >= 0).sum())
+    assert hits >= 26
 
 
 def test_splat_eps_threshold(ctx):
