@@ -78,6 +78,8 @@ def parse():
                     help="skip the LCT leg (128^2 x 1024 padded to 256x256x2048)")
     ap.add_argument("--no-depth", action="store_true",
                     help="skip the depth-plane mode + slice images leg")
+    ap.add_argument("--no-splat", action="store_true",
+                    help="skip the scatter-form (splat) backprojection leg")
     ap.add_argument("--no-mis", action="store_true",
                     help="skip the MIS measurement on the main capture")
     ap.add_argument("--no-filters", action="store_true",
@@ -184,6 +186,8 @@ def main():
     ctx.set_aligned_path(True)
     log(f"general kernel on the same input: {general_ms:.3f} ms")
     mis = None if args.no_mis else mis_measure(ctx, info, I, laser, vol, stream)
+    splat = None if args.no_splat else splat_measure(ctx, info, I, laser, vol, stream, general_ms,
+                                                     bp_avg)
     # End to end from host memory: pinned transient -> device, one step,
     # normalised volume -> pinned host buffer.
     I_host = torch.empty(I.shape, dtype=I.dtype, pin_memory=True)
@@ -300,9 +304,72 @@ def main():
         "depth_planes": depth,
         "filters": filters,
         "mis": mis,
+        "splat": splat,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def splat_measure(ctx, info, I, laser, vol, stream, general_ms, bp_ms, reps=3):
+    """Scatter form (fbp_backproject_confocal_splat, the mathematical form of
+    DrawEllipsoids::solveBackprojection) beside the gather paths: on the
+    main capture, on the same capture keeping only its largest 4 % of
+    samples (a sparse transient), and on the 32^2 x 256 capture of 200
+    points -> 32^3.  Event-timed medians; rel-L2 against the general gather
+    kernel's volume."""
+    import numpy as np
+    import torch
+
+    from fasterbackprojection_amd.api import make_info
+    from fasterbackprojection_amd.scene import small_confocal
+
+    def timed(fn):
+        fn()
+        ms = []
+        for _ in range(reps):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            b.record(stream)
+            b.synchronize()
+            ms.append(a.elapsed_time(b))
+        return float(np.median(ms))
+
+    def case(inf, tr, las, general=None):
+        r = inf.voxel_resolution
+        upd = r.x * r.y * r.z * inf.num_laser_targets
+        out = torch.empty((r.z, r.y, r.x), dtype=torch.float32, device=tr.device)
+        t_s = timed(lambda: ctx.backproject_confocal_splat(inf, tr, las, out=out))
+        ref = torch.empty_like(out)
+        ctx.set_aligned_path(False)
+        try:
+            t_g = general if general is not None else timed(
+                lambda: ctx.backproject_confocal(inf, tr, las, out=ref))
+            ctx.backproject_confocal(inf, tr, las, out=ref)
+        finally:
+            ctx.set_aligned_path(True)
+        rel = float((out - ref).norm() / ref.norm())
+        return {"splat_ms": t_s, "gather_general_ms": t_g,
+                "lit_fraction": float((tr > 1e-5).float().mean()),
+                "voxel_updates_per_s": upd / (t_s * 1e-3), "rel_l2_vs_gather": rel}
+
+    res = {"kernel": "splat_kernel (fbp_backproject_confocal_splat)"}
+    res["capture_128x128x1024_to_128cube"] = case(info, I, laser, general_ms)
+    res["capture_128x128x1024_to_128cube"]["gather_aligned_ms"] = bp_ms
+    thr = torch.quantile(I.flatten()[::97].float(), 0.96)
+    sparse = torch.where(I > thr, I, torch.zeros_like(I)).contiguous()
+    res["capture_128x128x1024_to_128cube_largest_4pct"] = case(info, sparse, laser)
+    cap = small_confocal(seed=0)
+    info1 = make_info(cap.geometry)
+    I1 = torch.from_numpy(cap.transient).to(I.device)
+    l1 = torch.from_numpy(cap.geometry.laser_targets).to(I.device)
+    res["capture_32x32x256_to_32cube"] = case(info1, I1, l1)
+    for k, v in res.items():
+        if isinstance(v, dict):
+            log(f"splat {k}: {v['splat_ms']:.3f} ms (gather general {v['gather_general_ms']:.3f} "
+                f"ms), lit {v['lit_fraction']:.3f}, rel-L2 {v['rel_l2_vs_gather']:.1e}")
+    return res
 
 
 def mis_measure(ctx, info, I, laser, vol, stream, reps=3):
