@@ -51,8 +51,16 @@ __global__ void depth_kernel(const float* __restrict__ I, const fbp_vec3* __rest
 
 // --------------------------------------------------------------------- MIS
 // Reference: backprojectConfocalVoxelMIS (FB/transient_reconstruction.cuh:194-260),
-// getUniformRandom / getMISIndexAlias (FB/transient_utils.cuh:18-53).  One
-// thread per voxel walks its samples s = 0..N-1 in order (the reference runs
+// getUniformRandom / getMISIndexAlias (FB/transient_utils.cuh:18-53).
+//
+// Sample s of voxel v (linear index z * X * Y + vy * X + vx) draws from the
+// noise state v + s, so its scan point and weight depend on that sum alone:
+// every state is shared by num_samples consecutive voxels.  mis_state_kernel
+// draws each state once (scan point, weight factor, path offset);
+// mis_walk_kernel gives each wave 64 consecutive voxels and walks the states
+// v0 .. v0 + 63 + N - 1 in order, so at every step the wave reads one
+// transient row (nearby bins for nearby voxels) instead of 64 random rows.
+// Each voxel still adds its samples s = 0..N-1 in order (the reference runs
 // one thread per (voxel, sample) with a racy +=).
 
 __device__ __forceinline__ float mis_uniform(const float* __restrict__ noise, uint32_t n,
@@ -61,104 +69,180 @@ __device__ __forceinline__ float mis_uniform(const float* __restrict__ noise, ui
     return noise[state % n];
 }
 
-__global__ void mis_kernel(const float* __restrict__ I, const fbp_vec3* __restrict__ laser,
-                           const float* __restrict__ cdf, const uint32_t* __restrict__ alias,
-                           const float* __restrict__ prob, const float* __restrict__ noise,
-                           uint32_t noise_size, uint32_t num_samples, uint32_t mode, VolumeGeom g,
-                           uint32_t L, uint32_t T, float t0, float dt, uint32_t discard,
-                           fbp_vec3 lpos, fbp_vec3 spos, uint32_t scatter, float ext,
-                           float* __restrict__ volume) {
-    const uint32_t slice = g.res[0] * g.res[1];
-    const size_t vid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (vid >= size_t(slice) * g.res[2]) return;
-    const uint32_t v = uint32_t(vid % slice), z = uint32_t(vid / slice);
-    const uint32_t vx = v % g.res[0], vy = v / g.res[0];
+// One noise state's draw.  coef: REFERENCE mode pdf * safeRCP(globalPDF);
+// UNBIASED mode the estimator's denominator (p_alias + 1/L) / 2 * N.  off:
+// |Lpos - l| + |Spos - l| (0 when the first/last bounces are discarded).
+struct MisState {
+    uint32_t l;
+    float coef;
+    float off;
+    float unused;
+};
+
+__global__ void mis_state_kernel(const float* __restrict__ cdf, const uint32_t* __restrict__ alias,
+                                 const float* __restrict__ prob, const float* __restrict__ noise,
+                                 uint32_t noise_size, uint32_t L, uint32_t num_samples,
+                                 uint32_t mode, uint32_t discard,
+                                 const fbp_vec3* __restrict__ laser, fbp_vec3 lpos, fbp_vec3 spos,
+                                 size_t count, MisState* __restrict__ out) {
+    const size_t sigma = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (sigma >= count) return;
+    const float invL = safe_rcp(static_cast<float>(L));
+    // Draws r0 (strategy), r1 (uniform index k), r2 (alias acceptance, used
+    // by the alias strategy only); l = alias-table pick or k.
+    uint32_t state = uint32_t(sigma);
+    const float r0 = mis_uniform(noise, noise_size, state);
+    const float r1 = mis_uniform(noise, noise_size, state);
+    const float r2 = mis_uniform(noise, noise_size, state);
+    uint32_t k = static_cast<uint32_t>(r1 * static_cast<float>(L));
+    if (k >= L) k = L - 1;
+    const bool by_alias = r0 < 0.5f;
+    const uint32_t l = by_alias ? ((r2 < prob[k]) ? k : alias[k]) : k;
+    const float cl = cdf[l], cm = cdf[l > 0 ? l - 1 : 0];
+    const float pl = l > 0 ? cl - cm : cl;
+    const float pdf = by_alias ? pl : 1.0f * invL;
+    const float gpdf = by_alias ? pdf + 1.0f * invL : pdf + pl;
+    MisState st;
+    st.l = l;
+    if (mode == FBP_MIS_REFERENCE) {
+        st.coef = rn_mul(pdf, safe_rcp(gpdf));
+    } else {
+        // One-sample estimator over the mixture p = (p_alias + 1/L) / 2
+        // (unbiased: E[f(l) / p(l)] / N = sum_l f(l) / N).
+        const float pmix = 0.5f * (pl + invL);
+        st.coef = pmix * float(num_samples);
+    }
+    st.off = 0.f;
+    if (discard == 0) {
+        const float lx = laser[l].x, ly = laser[l].y, lz = laser[l].z;
+        st.off = rn_add(distance3(lx, ly, lz, lpos.x, lpos.y, lpos.z),
+                        distance3(lx, ly, lz, spos.x, spos.y, spos.z));
+    }
+    st.unused = 0.f;
+    out[sigma] = st;
+}
+
+// PURE (t0 = 0, bounces discarded, no voxel scattering): the bin comes from
+// the exact squared distance s and the threshold row S_k = min{s : bin(s) >=
+// k} staged in LDS (an estimate biased low, one compare: the reference's bin
+// without its IEEE sqrt and division), the weight from a hardware
+// reciprocal of 4 s (FP32 tolerance), the reference's IEEE weight for the
+// rare pairs within 1.7 mm of their scan point (safeRCP's clamp).
+// Otherwise every sample takes the reference's IEEE chain.
+template <bool REF, bool SCATTER, bool PURE>
+__global__ __launch_bounds__(256) void mis_walk_kernel(const float* __restrict__ I,
+                                                       const fbp_vec3* __restrict__ laser,
+                                                       const MisState* __restrict__ states,
+                                                       uint32_t num_samples, VolumeGeom g,
+                                                       uint32_t T, float t0, float dt,
+                                                       uint32_t discard, float ext, uint32_t rows,
+                                                       const float* __restrict__ thr,
+                                                       float* __restrict__ volume) {
+    extern __shared__ float srow[];  // PURE: S_0 .. S_T
+    if constexpr (PURE) {
+        for (uint32_t k = threadIdx.x; k <= T; k += blockDim.x) srow[k] = thr[k];
+        __syncthreads();
+    }
+    // A wave owns a patch of `rows` voxel rows x (64 / rows) voxels along x
+    // of one depth slice; its voxels' states lie in [v0, send).
+    const uint32_t X = g.res[0], Y = g.res[1];
+    const uint32_t wx = 64u / rows;
+    const uint32_t pbx = (X + wx - 1) / wx, pby = (Y + rows - 1) / rows;
+    const size_t wave = size_t(blockIdx.x) * (blockDim.x >> 6) + uint32_t(rfl(int(threadIdx.x >> 6)));
+    if (wave >= size_t(pbx) * pby * g.res[2]) return;
+    const uint32_t z = uint32_t(wave / (size_t(pbx) * pby));
+    const uint32_t pr = uint32_t(wave % (size_t(pbx) * pby));
+    const uint32_t x0 = (pr % pbx) * wx, y0 = (pr / pbx) * rows;
+    const uint32_t j = threadIdx.x & 63;
+    const uint32_t vx = min(x0 + j % wx, X - 1), vy = min(y0 + j / wx, Y - 1);
+    const bool valid = x0 + j % wx < X && y0 + j / wx < Y;
+    const size_t v0 = (size_t(z) * Y + y0) * X + x0;
+    const size_t vid = (size_t(z) * Y + vy) * X + vx;
     const float px = voxel_center(g.hmin[0], g.vs[0], vx);
     const float py = voxel_center(g.hmin[1], g.vs[1], z);
     const float pz = voxel_center(g.hmin[2], g.vs[2], vy);
-    const float invL = safe_rcp(static_cast<float>(L));
-    float sum = 0.f;
-    // Weight of one sample of l: reference MIS weight, or the one-sample
-    // estimator over the mixture p = (p_alias + 1/L) / 2 (unbiased:
-    // E[f(l) / p(l)] / N = sum_l f(l) / N).
-    auto add = [&](float light, float wd, float pdf, float gpdf, float pa) {
-        if (mode == FBP_MIS_REFERENCE) {
-            const float mis_weight = rn_mul(pdf, safe_rcp(gpdf));
-            sum += rn_mul(rn_mul(mis_weight, light), wd);
-        } else {
-            const float pmix = 0.5f * (pa + invL);
-            sum += (light * wd) / (pmix * float(num_samples));
-        }
-    };
-    // Sample s: draws r0 (strategy), r1 (uniform index k), r2 (alias
-    // acceptance, used by the alias strategy only) from state
-    // slice * z + v + s; l = alias-table pick or k.
-    auto pick = [&](uint32_t s, uint32_t& l, float& pdf, float& gpdf, float& pa) {
-        uint32_t state = slice * z + v + s;
-        const float r0 = mis_uniform(noise, noise_size, state);
-        const float r1 = mis_uniform(noise, noise_size, state);
-        const float r2 = mis_uniform(noise, noise_size, state);
-        uint32_t k = static_cast<uint32_t>(r1 * static_cast<float>(L));
-        if (k >= L) k = L - 1;
-        const bool by_alias = r0 < 0.5f;
-        const float pk = prob[k];
-        const uint32_t ak = alias[k];
-        l = by_alias ? ((r2 < pk) ? k : ak) : k;
-        const float cl = cdf[l], cm = cdf[l > 0 ? l - 1 : 0];
-        const float pl = l > 0 ? cl - cm : cl;
-        pdf = by_alias ? pl : 1.0f * invL;
-        pa = pl;
-        gpdf = by_alias ? pdf + 1.0f * invL : pdf + pa;
-    };
-    auto path = [&](uint32_t l) {
-        const float lx = laser[l].x, ly = laser[l].y, lz = laser[l].z;
-        float dist = rn_sub(rn_mul(distance3(lx, ly, lz, px, py, pz), 2.0f), t0);
-        if (discard == 0)
-            dist = rn_add(dist, rn_add(distance3(lx, ly, lz, lpos.x, lpos.y, lpos.z),
-                                       distance3(lx, ly, lz, spos.x, spos.y, spos.z)));
+    auto path = [&](const MisState& st) {
+        const fbp_vec3 lp = laser[st.l];
+        float dist = rn_sub(rn_mul(distance3(lp.x, lp.y, lp.z, px, py, pz), 2.0f), t0);
+        if (discard == 0) dist = rn_add(dist, st.off);
         return dist;
     };
-    if (scatter) {
+    auto add = [&](float& sum, float light, float wd, float coef) {
+        if constexpr (REF)
+            sum += rn_mul(rn_mul(coef, light), wd);
+        else
+            sum += (light * wd) / coef;
+    };
+    float sum = 0.f;
+    const size_t send = v0 + size_t(rows - 1) * X + wx - 1 + num_samples;
+    if constexpr (SCATTER) {
         // Voxel-scattering walk (FB/transient_reconstruction.cuh:235-250).
-        for (uint32_t s = 0; s < num_samples; ++s) {
-            uint32_t l;
-            float pdf, gpdf, pa;
-            pick(s, l, pdf, gpdf, pa);
-            const float dist = path(l);
+        for (size_t sg = v0; sg < send; ++sg) {
+            const bool act = valid && sg >= vid && sg < vid + num_samples;
+            if (!act) continue;
+            const MisState st = states[sg];
+            const float dist = path(st);
             const float wd = safe_rcp(rn_mul(dist, dist));
+            const float* row = I + size_t(st.l) * T;
             scatter_walk(dist, ext, dt, T, [&](int bin) {
-                const float light = I[size_t(l) * T + uint32_t(bin)];
-                if (light > kEps) add(light, wd, pdf, gpdf, pa);
+                const float light = row[uint32_t(bin)];
+                if (light > kEps) add(sum, light, wd, st.coef);
             });
         }
-        volume[vid] = sum;
-        return;
-    }
-    // Samples in groups of kMisGroup: their loads are independent (each has
-    // its own noise state), so the group's chains overlap; every load is
-    // issued (indices clamped in range) and the contributions are added in
-    // sample order.
-    constexpr int kMisGroup = 8;
-    for (uint32_t s0 = 0; s0 < num_samples; s0 += kMisGroup) {
-        uint32_t lq[kMisGroup];
-        float pdf[kMisGroup], gpdf[kMisGroup], pa[kMisGroup], wd[kMisGroup], light[kMisGroup];
-        bool ok[kMisGroup];
+    } else {
+        // Eight states per pass: their loads are independent, so the chains
+        // overlap; every load is issued (indices clamped in range) and the
+        // contributions are added in state order.
+        constexpr int U = 8;
+        // Pure-chain estimate (s * rsq(s)) * cb ~ 2 r / dt, biased low by
+        // 2^-18 relative so the bin is ka or ka + 1.
+        const float cb = (2.0f / dt) * (1.0f - 3.814697265625e-06f);
+        for (size_t s0 = v0; s0 < send; s0 += U) {
+            float light[U], wd[U], coef[U], s2[U];
+            bool ok[U];
+            // State sg serves this voxel iff 0 <= sg - vid < N (32-bit wrap).
+            const uint32_t rel0 = uint32_t(s0 - vid);
 #pragma unroll
-        for (int u = 0; u < kMisGroup; ++u) pick(s0 + u, lq[u], pdf[u], gpdf[u], pa[u]);
+            for (int u = 0; u < U; ++u) {
+                const size_t sg = s0 + u;
+                const MisState st = states[sg < send ? sg : send - 1];
+                const bool act = valid && rel0 + uint32_t(u) < num_samples;
+                int bin;
+                if constexpr (PURE) {
+                    const fbp_vec3 lp = laser[st.l];
+                    s2[u] = dot3(sq(rn_sub(px, lp.x)), sq(rn_sub(py, lp.y)), sq(rn_sub(pz, lp.z)));
+                    const float est = fmaxf((s2[u] * __builtin_amdgcn_rsqf(s2[u])) * cb, 0.f);
+                    const int ka = min(int(est), int(T));
+                    const int k = ka + int(srow[min(ka + 1, int(T))] <= s2[u] && ka < int(T));
+                    ok[u] = act && k < int(T);
+                    bin = ok[u] ? k : 0;
+                    wd[u] = __builtin_amdgcn_rcpf(s2[u] * 4.0f);
+                } else {
+                    const float dist = path(st);
+                    wd[u] = safe_rcp(rn_mul(dist, dist));
+                    const float q = rn_div(dist, dt);
+                    ok[u] = act && (q > -1.0f) && (q < float(T));
+                    bin = ok[u] ? int(q) : 0;
+                }
+                light[u] = I[size_t(ok[u] ? st.l : 0) * T + uint32_t(bin)];
+                coef[u] = st.coef;
+            }
 #pragma unroll
-        for (int u = 0; u < kMisGroup; ++u) {
-            const float dist = path(lq[u]);
-            wd[u] = safe_rcp(rn_mul(dist, dist));
-            const float q = rn_div(dist, dt);
-            ok[u] = (s0 + u < num_samples) && (q > -1.0f) && (q < float(T));
-            const int bin = ok[u] ? int(q) : 0;
-            light[u] = I[size_t(ok[u] ? lq[u] : 0) * T + uint32_t(bin)];
+            for (int u = 0; u < U; ++u) {
+                if (ok[u] && light[u] > kEps) {
+                    float w = wd[u];
+                    if constexpr (PURE) {
+                        if (!(s2[u] > 3.0e-6f)) {  // dist^2 = 4 s near safeRCP's 1e-5
+                            const float d = rn_mul(rn_sqrt(s2[u]), 2.0f);
+                            w = safe_rcp(rn_mul(d, d));
+                        }
+                    }
+                    add(sum, light[u], w, coef[u]);
+                }
+            }
         }
-#pragma unroll
-        for (int u = 0; u < kMisGroup; ++u)
-            if (ok[u] && light[u] > kEps) add(light[u], wd[u], pdf[u], gpdf[u], pa[u]);
     }
-    volume[vid] = sum;
+    if (valid) volume[vid] = sum;
 }
 
 // Inclusive FP64 prefix of the row sums (one workgroup of 1024 threads).
@@ -619,16 +703,54 @@ fbp_status fbp_backproject_confocal_mis(fbp_context ctx, const fbp_reconstructio
     FBP_CHECK_HIP(ctx, hipMemcpyAsync(d_prob, prob.data(), size_t(L) * 4, hipMemcpyHostToDevice, s));
     VolumeGeom g = make_geom(*info, 1, 1, 1);
     const size_t nv = size_t(g.res[0]) * g.res[1] * g.res[2];
-    hipLaunchKernelGGL(mis_kernel, dim3(unsigned((nv + 255) / 256)), dim3(256), 0, s,
-                       static_cast<const float*>(tr->data), laser_targets, d_cdf, d_alias, d_prob,
-                       params->noise, params->noise_size, params->num_samples, params->mode, g, L,
-                       T, info->time_offset, info->time_step, info->discard_first_last_bounces,
-                       info->laser_position, info->sensor_position,
-                       uint32_t(ctx->voxel_scattering),
-                       voxel_extent(info->hidden_volume_voxel_size.x,
-                                    info->hidden_volume_voxel_size.y,
-                                    info->hidden_volume_voxel_size.z),
-                       volume);
+    // Wave patches: rows x (64 / rows) voxels, as square as the state window
+    // allows (each lane is active for N of the wave's N + (rows - 1) X +
+    // 64 / rows - 1 states; squarer patches read fewer transient lines).
+    uint32_t rows = 1;
+    while (rows < 8 && 4 * (2 * rows - 1) * size_t(g.res[0]) <= params->num_samples) rows *= 2;
+    // States 0 .. nv + N + rows X + 64 (a patch's walk may run past nv - 1).
+    const size_t nstate = nv + params->num_samples + size_t(rows) * g.res[0] + 64;
+    FBP_CHECK(ensure_scratch2(ctx, nstate * sizeof(MisState) + 256));
+    MisState* states = static_cast<MisState*>(ctx->scratch2);
+    hipLaunchKernelGGL(mis_state_kernel, dim3(unsigned((nstate + 255) / 256)), dim3(256), 0, s,
+                       d_cdf, d_alias, d_prob, params->noise, params->noise_size, L,
+                       params->num_samples, params->mode, info->discard_first_last_bounces,
+                       laser_targets, info->laser_position, info->sensor_position, nstate, states);
+    FBP_CHECK_HIP(ctx, hipGetLastError());
+    const size_t waves = size_t((g.res[0] + 64 / rows - 1) / (64 / rows)) *
+                         ((g.res[1] + rows - 1) / rows) * g.res[2];
+    const dim3 grid(unsigned((waves + 3) / 4));
+    const float ext = voxel_extent(info->hidden_volume_voxel_size.x,
+                                   info->hidden_volume_voxel_size.y,
+                                   info->hidden_volume_voxel_size.z);
+    const bool ref = params->mode == FBP_MIS_REFERENCE;
+    const float* I = static_cast<const float*>(tr->data);
+    const bool pure = info->time_offset == 0.f && info->discard_first_last_bounces != 0 &&
+                      !ctx->voxel_scattering && (size_t(T) + 1) * sizeof(float) <= 65536;
+    const float* thr = nullptr;
+    if (pure) FBP_CHECK(pure_threshold_row(ctx, T, info->time_step, s, &thr));
+    const size_t shmem = pure ? (size_t(T) + 1) * sizeof(float) : 0;
+#define FBP_MIS_WALK(REF_, SCAT_, PURE_)                                                          \
+    hipLaunchKernelGGL((mis_walk_kernel<REF_, SCAT_, PURE_>), grid, dim3(256), shmem, s, I,      \
+                       laser_targets, states, params->num_samples, g, T, info->time_offset,      \
+                       info->time_step, info->discard_first_last_bounces, ext, rows, thr, volume)
+    if (ctx->voxel_scattering) {
+        if (ref)
+            FBP_MIS_WALK(true, true, false);
+        else
+            FBP_MIS_WALK(false, true, false);
+    } else if (pure) {
+        if (ref)
+            FBP_MIS_WALK(true, false, true);
+        else
+            FBP_MIS_WALK(false, false, true);
+    } else {
+        if (ref)
+            FBP_MIS_WALK(true, false, false);
+        else
+            FBP_MIS_WALK(false, false, false);
+    }
+#undef FBP_MIS_WALK
     FBP_CHECK_HIP(ctx, hipGetLastError());
     // The host vectors above are consumed by the async copies: wait for them.
     FBP_CHECK_HIP(ctx, hipStreamSynchronize(s));

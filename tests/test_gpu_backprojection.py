@@ -558,6 +558,50 @@ def test_mis_reference_mode_matches_oracle(ctx):
     assert rel_l2(out.cpu().numpy(), ref) < 1e-4
 
 
+@pytest.mark.parametrize("mode,discard,t0", [(0, 0, 0.3), (1, 1, 0.0), (1, 0, 0.3)])
+def test_mis_per_voxel_modes_match_oracle(ctx, mode, discard, t0):
+    """REFERENCE and UNBIASED per-voxel sampling on a volume whose voxel count
+    is not a multiple of 64 (the last wave is partial), with the general
+    chain's offsets: sample s of voxel v draws from noise state v + s."""
+    g = geom(n_scan=12, T=256, res=(13, 11, 9), discard=discard, t0=t0)
+    info = make_info(g)
+    rng = np.random.default_rng(40 + mode)
+    I = rng.uniform(0.0, 1.0, (144, 256)).astype(np.float32)
+    I[rng.random(I.shape) < 0.5] = 0.0
+    cdf = oracle.mis_cdf(I)
+    alias, prob = oracle.alias_tables(cdf)
+    noise = rng.random(5000).astype(np.float32)
+    ref = oracle.mis(info, I, g.laser_targets, cdf, alias, prob, noise, 100, mode)
+    out = ctx.backproject_confocal_mis(info, cuda(I), cuda(g.laser_targets), cuda(noise), 100,
+                                       mode=mode)
+    torch.cuda.synchronize()
+    assert rel_l2(out.cpu().numpy(), ref) < 1e-5
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_mis_per_voxel_at_scan_points(ctx, mode):
+    """Voxel centres exactly on scan points (depth slice 0 on the wall): those
+    pairs have dist = 0 and safeRCP's FLT_MAX weight, which the pure-chain
+    walk takes from the reference's IEEE chain."""
+    g = geom(n_scan=8, T=256, res=(8, 8, 8), hmin=(-0.5, -0.025, -0.5),
+             hmax=(0.5, 0.375, 0.5))
+    info = make_info(g)
+    rng = np.random.default_rng(3 + mode)
+    I = rng.uniform(0.1, 1.0, (64, 256)).astype(np.float32)
+    I[:, 0] = rng.uniform(1e-4, 1e-3, 64).astype(np.float32)
+    cdf = oracle.mis_cdf(I)
+    alias, prob = oracle.alias_tables(cdf)
+    noise = rng.random(3000).astype(np.float32)
+    ref = oracle.mis(info, I, g.laser_targets, cdf, alias, prob, noise, 48, mode)
+    clamp = ref > 1e30
+    assert clamp.any()
+    out = ctx.backproject_confocal_mis(info, cuda(I), cuda(g.laser_targets), cuda(noise), 48,
+                                       mode=mode).cpu().numpy()
+    assert np.all(np.isfinite(out))
+    np.testing.assert_allclose(out[clamp], ref[clamp], rtol=1e-5)
+    assert rel_l2(out[~clamp], ref[~clamp]) < 1e-5
+
+
 @pytest.mark.parametrize("discard,t0", [(1, 0.0), (0, 0.3)])
 def test_mis_tiled_matches_oracle(ctx, discard, t0):
     """Tile-shared MIS: same samples and weights as the restatement, exact
