@@ -52,6 +52,8 @@ struct FftCache {
     // LCT z-pass twiddles exp(-2 pi i k / (2M)), k < M
     uint32_t tw_m = 0;
     float2* tw = nullptr;
+    // The inverse filter in the 2048-point z-pass's load order (invt_tile_kernel)
+    float2* invt = nullptr;
 };
 
 std::once_flag g_rocfft_once;
@@ -481,7 +483,7 @@ __device__ __forceinline__ float2 w2048(const float2* twl, uint32_t m) {
 }
 
 __global__ __launch_bounds__(kZ2048Threads) void lct_zpass2048_kernel(float2* __restrict__ S,
-                                                                    const float2* __restrict__ inv,
+                                                                    const float2* __restrict__ invt,
                                                                     const float2* __restrict__ tw,
                                                                     uint32_t N) {
     extern __shared__ float2 lds[];
@@ -494,17 +496,15 @@ __global__ __launch_bounds__(kZ2048Threads) void lct_zpass2048_kernel(float2* __
     const bool live = kx < H;
     auto at = [&](uint32_t pos) -> float2& { return buf[pos * kLctCols + col + 4u * (pos >> 3)]; };
     for (uint32_t k = tid; k < 1024; k += kZ2048Threads) twl[k] = tw[k];
-    // The filter values this thread applies in pass 3, loaded now so that
-    // their latency overlaps the forward passes.
+    // The filter values this thread applies in pass 3 (the tiled copy: one
+    // contiguous 8 KB block per (h, k2) for the workgroup), loaded now so
+    // that their latency overlaps the forward passes.
     float2 fil[256 / kZU][8];
+    const float2* ft = invt + (size_t(ky) * gridDim.x + blockIdx.x) * (256 / kZU) * 8 * kZ2048Threads;
 #pragma unroll
-    for (int h = 0; h < 256 / kZU; ++h) {
-        const uint32_t r = u + uint32_t(kZU) * h, kl = r / 16u + 16u * (r % 16u);
+    for (int h = 0; h < 256 / kZU; ++h)
 #pragma unroll
-        for (int k2 = 0; k2 < 8; ++k2)
-            fil[h][k2] = live ? inv[(size_t(kl + 256u * k2) * Y2 + ky) * H + kx]
-                              : make_float2(0.f, 0.f);
-    }
+        for (int k2 = 0; k2 < 8; ++k2) fil[h][k2] = ft[(h * 8 + k2) * kZ2048Threads + tid];
     __syncthreads();
 
     // Forward pass 1: a = u; x[a + 128 b] = 0 for b >= 8.
@@ -676,6 +676,29 @@ __global__ void invpsf_kernel(const double2* __restrict__ F, float2* __restrict_
     }
 }
 
+// The inverse filter (layout [kz][2N][N + 1]) rearranged so that the
+// 2048-point z-pass reads it contiguously: entry
+// (((ky * nbx + bx) * 2 + h) * 8 + k2) * 1024 + tid holds the value thread
+// tid of workgroup (bx, ky) applies as fil[h][k2] (column kx = 8 bx + tid % 8,
+// kz = kl + 256 k2, kl = r / 16 + 16 (r % 16), r = tid / 8 + 128 h); 0 past
+// the last column.
+__global__ void invt_tile_kernel(const float2* __restrict__ inv, float2* __restrict__ invt,
+                                 uint32_t N, uint32_t nbx, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t tid = uint32_t(i % kZ2048Threads);
+    size_t rest = i / kZ2048Threads;
+    const uint32_t k2 = uint32_t(rest % 8);
+    rest /= 8;
+    const uint32_t h = uint32_t(rest % 2);
+    rest /= 2;
+    const uint32_t bx = uint32_t(rest % nbx), ky = uint32_t(rest / nbx);
+    const uint32_t H = N + 1, Y2 = 2 * N;
+    const uint32_t kx = bx * kLctCols + tid % kLctCols;
+    const uint32_t r = tid / kLctCols + uint32_t(kZU) * h, kl = r / 16u + 16u * (r % 16u);
+    invt[i] = kx < H ? inv[(size_t(kl + 256u * k2) * Y2 + ky) * H + kx] : make_float2(0.f, 0.f);
+}
+
 // Closed form of python/lct.py:resampling_operator (:350-419):
 // mtx[r][j] = (1/M) sum_{x in [rM+1, (r+1)M], ceil(sqrt x) = j+1} 1/sqrt(x);
 // mtxi = mtx^T.  Built in FP64 on the host, stored as float CSR.
@@ -736,6 +759,7 @@ void fft_cache_destroy(fbp_context ctx) {
     if (c->work) hipFree(c->work);
     if (c->khat) hipFree(c->khat);
     if (c->invpsf) hipFree(c->invpsf);
+    if (c->invt) hipFree(c->invt);
     if (c->tw) hipFree(c->tw);
     free_csr(c->mtx);
     free_csr(c->mtxi);
@@ -912,6 +936,8 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
     if (c->inv_key != keybuf) {
         if (c->invpsf) FBP_CHECK_HIP(ctx, hipFree(c->invpsf));
         c->invpsf = nullptr;
+        if (c->invt) FBP_CHECK_HIP(ctx, hipFree(c->invt));
+        c->invt = nullptr;
         c->inv_key.clear();
         double* psf = nullptr;
         double2* F = nullptr;
@@ -985,8 +1011,15 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
             FBP_CHECK_HIP(ctx, hipFuncSetAttribute(
                                    reinterpret_cast<const void*>(lct_zpass2048_kernel),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(lds2048)));
+            if (!c->invt) {
+                const size_t nt = size_t(zgrid.x) * zgrid.y * (256 / kZU) * 8 * kZ2048Threads;
+                FBP_CHECK_HIP(ctx, hipMalloc((void**)&c->invt, nt * sizeof(float2)));
+                hipLaunchKernelGGL(invt_tile_kernel, grid1(nt), dim3(256), 0, s, c->invpsf,
+                                   c->invt, N, zgrid.x, nt);
+                FBP_CHECK_HIP(ctx, hipGetLastError());
+            }
             hipLaunchKernelGGL(lct_zpass2048_kernel, zgrid, dim3(kZ2048Threads), lds2048, s, spec,
-                               c->invpsf, c->tw, N);
+                               c->invt, c->tw, N);
         } else {
             hipLaunchKernelGGL(lct_zpass_kernel, zgrid, dim3(kLctThreads), zpass_lds, s, spec,
                                c->invpsf, c->tw, N, M, lg);
