@@ -39,7 +39,7 @@ VALU_LANE_SLOTS = 39.3e12      # 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (non-pac
 # (profiles/counters_confocal_exhaustive_lct_bench.md; 2.37 = measured
 # v_fma / v_rsq throughput ratio on this GPU).
 GENERAL_SLOTS = 14.9           # confocal_voxel_kernel
-EXH_SLOTS = 19.8               # exhaustive_voxel_kernel (linear-weight kernel)
+EXH_SLOTS = 11.7               # exhaustive_group_kernel (profiles/exhaustive_group_c3_bench_counters.md)
 DEPTH_EXH_SLOTS = 17.4         # depth_exhaustive_window_kernel fast path, ISA count
 TRAFFIC_PROFILE = ROOT / "profiles" / "confocal_128cube_traffic.json"
 
@@ -801,6 +801,16 @@ def exhaustive_leg(ctx, args, reps=3):
         return float(np.mean(ms))
 
     bp = timed(lambda: ctx.backproject_exhaustive(info, I, lasers, sensors, out=vol))
+    kernel = ("exhaustive_group_kernel (laser groups of 8)" if ctx.last_exhaustive_laser_groups()
+              else "exhaustive_voxel_kernel (per pair)")
+    # The per-pair kernel on the same capture, for comparison.
+    ctx.set_exhaustive_laser_groups(False)
+    vol_pp = torch.empty_like(vol)
+    bp_pair = timed(lambda: ctx.backproject_exhaustive(info, I, lasers, sensors, out=vol_pp))
+    ctx.set_exhaustive_laser_groups(True)
+    pair_rel = float(torch.linalg.vector_norm((vol_pp - vol).double())
+                     / torch.linalg.vector_norm(vol.double()))
+    del vol_pp
     lap = timed(lambda: ctx.laplacian(vol, 5, out=filt))
     # Depth-plane mode of the same capture (backprojectExhaustive): 200
     # planes at focus +- 0.1 over the laser-grid pixels, every pair.
@@ -848,13 +858,15 @@ def exhaustive_leg(ctx, args, reps=3):
                "kind": "port",
                "rel_l2_gpu_vs_cpu": float(np.linalg.norm(got - ref) / np.linalg.norm(ref))}
     tflops = 25 * updates / (bp * 1e-3) / 1e12
-    # Issue slots per pair-update measured by counters over the whole kernel
-    # (profiles/counters_confocal_exhaustive_lct_bench.md): 18.3 VALU
-    # instructions, 1.12 of them transcendental at 2.37 slots.
+    # Issue slots per pair-update of the laser-group kernel measured by
+    # counters over the whole kernel (profiles/exhaustive_group_c3_bench_counters.md):
+    # 11.5 VALU instructions, 0.19 of them transcendental at 2.37 slots.
     slot_bound = VALU_LANE_SLOTS / EXH_SLOTS
     out = {"workload": "exhaustive 16^2 lasers x 64^2 sensors x 1024 (16 ps, 4 GiB) -> 64^3 "
                        "+ Laplacian(k=5)",
-           "pair_updates": updates, "backprojection_ms": bp, "laplacian_ms": lap,
+           "pair_updates": updates, "kernel": kernel, "backprojection_ms": bp,
+           "per_pair_kernel": {"ms": bp_pair, "rel_l2_vs_group_kernel": pair_rel},
+           "laplacian_ms": lap,
            "pair_updates_per_s": updates / ((bp + lap) * 1e-3),
            "roofline": {"bound": "valu", "achieved": tflops, "peak": PEAK_FP32_TFLOPS,
                         "unit": "TFLOP/s", "frac": tflops / PEAK_FP32_TFLOPS,

@@ -25,6 +25,7 @@ LAYOUT_TIME_FASTEST, LAYOUT_TIME_MAJOR = 0, 1
 MIS_REFERENCE, MIS_UNBIASED, MIS_TILED = 0, 1, 2
 OPTION_CONFOCAL_ALIGNED = 0
 OPTION_VOXEL_SCATTERING = 1
+OPTION_EXHAUSTIVE_LASER_GROUPS = 2
 
 
 class Vec3(C.Structure):
@@ -85,6 +86,7 @@ SIGNATURES = {
     "fbp_context_set_option": (_S, [_CTX, C.c_int32, C.c_int32]),
     "fbp_context_get_option": (_S, [_CTX, C.c_int32, C.POINTER(C.c_int32)]),
     "fbp_context_last_confocal_path": (_S, [_CTX, C.POINTER(C.c_int32)]),
+    "fbp_context_last_exhaustive_kernel": (_S, [_CTX, C.POINTER(C.c_int32)]),
     "fbp_backproject_confocal_voxel": (_S, [_CTX, C.POINTER(ReconstructionInfo), C.POINTER(Transient), _P, _P, _P]),
     "fbp_backproject_confocal_splat": (_S, [_CTX, C.POINTER(ReconstructionInfo), C.POINTER(Transient), _P, _P, _P]),
     "fbp_backproject_exhaustive_voxel": (_S, [_CTX, C.POINTER(ReconstructionInfo), C.POINTER(Transient), _P, _P, _P, _P]),

@@ -174,6 +174,14 @@ class Context:
         self.check(self.lib.fbp_context_set_option(self.handle, _lib.OPTION_VOXEL_SCATTERING,
                                                    int(bool(enabled))))
 
+    def set_exhaustive_laser_groups(self, enabled: bool) -> None:
+        """Allow (default) or forbid the laser-group kernel of the exhaustive
+        voxel backprojection (sensor distances shared by eight lasers; taken
+        for linear-weight captures with at least 8 lasers and 64 sensors)."""
+        self.check(self.lib.fbp_context_set_option(self.handle,
+                                                   _lib.OPTION_EXHAUSTIVE_LASER_GROUPS,
+                                                   int(bool(enabled))))
+
     def option(self, option: int) -> int:
         v = C.c_int32()
         self.check(self.lib.fbp_context_get_option(self.handle, option, C.byref(v)))
@@ -184,6 +192,13 @@ class Context:
         aligned-grid path (synchronises)."""
         v = C.c_int32()
         self.check(self.lib.fbp_context_last_confocal_path(self.handle, C.byref(v)))
+        return bool(v.value)
+
+    def last_exhaustive_laser_groups(self) -> bool:
+        """True when the last exhaustive voxel backprojection ran the
+        laser-group kernel (host-side record)."""
+        v = C.c_int32()
+        self.check(self.lib.fbp_context_last_exhaustive_kernel(self.handle, C.byref(v)))
         return bool(v.value)
 
     # ------------------------------------------------------------ helpers

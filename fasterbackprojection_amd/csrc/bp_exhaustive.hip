@@ -443,6 +443,529 @@ __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs
                     acc[(i * 2 + j) * 4 + m] + corrl[((i * 2 + j) * 4 + m) * 256];
 }
 
+// ---------------------------------------------------------------------------
+// Laser-group kernel: linear-weight captures with many sensors.
+//
+// A workgroup of eight waves owns the same 16 (vx) x 16 (vy) x 32 (z) tile
+// with 16 voxels per lane (wave w holds z = Z0 + lz + 4w) and a range of
+// (laser group, sensor) batches; a laser group is kGroup = 8 consecutive
+// lasers.  Each lane keeps the group's eight laser terms of its 16 voxels in
+// registers while the group is unchanged and computes its 16 sensor terms
+// once per sensor, for the eight pairs (l, s) of the group, so an update
+// costs no square root:
+//   q = rl'[g][v] + rs'[v],  k = (int)q,  band <- fract(q) < 2 (e_l + e_s),
+//   acc[v] += A_k + B_k q.
+// rl' = |v - l| / dt + (off_l - t0) / dt + e_l and rs' = |v - s| / dt +
+// off_s / dt + e_s (rsq estimates), where e_l + e_s bounds the estimate's
+// error against fl(dist / dt) for the pair; the reference's quotient then
+// lies in [q - 2 (e_l + e_s), q], so its bin is k unless fract(q) is below
+// 2 (e_l + e_s): those (voxel, pair) updates are replaced by the IEEE chain
+// after the batch (deferred fix-ups into per-thread LDS slots).  The window
+// entry of bin k is the linear expansion of I / Q^2 about m = k + 1/2 as a
+// function of the shifted q: {I (3 r^2 + 2 e r^3), -2 I r^3}, r = 1/m,
+// e = e_l + e_s (bins >= kLinMinBin; other pairs take the exact chain).
+// One batch = one sensor and the group's eight pairs; wave g stages pair g's
+// window (from a 4-bin aligned start, 16-byte loads and stores), the next
+// batch's raw loads in flight while the current batch runs.
+constexpr int kGroup = 8;
+constexpr int kGThreads = 512;
+constexpr int kGVec = 2;           // 4-bin loads per lane per batch: window <= 4 * 64 * kGVec bins
+
+struct GroupArgs {
+    const void* data;
+    const float* scales;
+    const fbp_vec3* laser;
+    const fbp_vec3* sensor;
+    float* partial;
+    VolumeGeom g;
+    uint32_t L, S, T;
+    float t0, dt, inv_dt, inv_dt2, dt2;
+    uint32_t discard;
+    float lpos[3], spos[3];
+    int W;                    // window length in bins (multiple of 4): bins kmin4 .. kmin4 + W - 1
+    uint32_t groups;          // ceil(L / kGroup)
+    uint32_t groups_per_chunk;
+    uint32_t sensor_parts;    // chunks along the sensors
+    uint32_t sensors_per_part;
+    uint32_t num_blocks;      // grid size (XCD remap)
+};
+
+// Pair parameters of one batch slot, written by one thread and read by all.
+struct __align__(16) GroupPair {
+    int base;      // g * W - kmin4: index of bin k in the batch's window buffer
+    int kmin;      // first bin the tile can reach (- 3)
+    int kmin4;     // kmin rounded down to a multiple of 4 (window start)
+    int fast;      // 1: linear-weight fast body, 0: exact chain, -1: no pair (l >= L)
+    float lx, ly, lz, off;     // laser; path offset fl(|l - Lpos| + |s - Spos|) (0 with discard)
+    float sx, sy, sz, e;       // sensor; e = e_l + e_s
+    float cl, cs, pad[2];      // (off_l - t0) / dt + e_l, off_s / dt + e_s
+};
+
+// Laser-side parameters of the chunk's lasers against the tile (set-up).
+struct __align__(16) GroupLaser {
+    float x, y, z, offl;       // position, |l - Lpos| (0 with discard)
+    float rmin, rmax, cl, el;  // box distances, (offl - t0) / dt + e_l, e_l
+};
+
+// Error bound per unit of magnitude (bins per bin): with 1-ulp v_rsq the
+// laser term is within 2.5 ulp of |v - l| / dt, the sensor term (its square
+// from per-axis products scaled by 1/dt^2) within 3 ulp, their sum 0.5 ulp,
+// and the reference's fl(fl(rl + rs) / dt) 1.5 ulp: 5.5 * 2^-23 of the
+// magnitude; 6.5 leaves room for the offset terms.
+constexpr float kGroupErr = 6.5f * 1.1920929e-7f;
+
+// Laser term of one voxel, cl = (off_l - t0) / dt + e_l (identical operations
+// in the set-up and in the fix-ups).
+__device__ __forceinline__ float group_laser_term(float x2, float y2, float z2, float cl,
+                                                  float inv_dt) {
+    const float sl = dot3(x2, y2, z2);
+    return fmaf(sl * __builtin_amdgcn_rsqf(sl), inv_dt, cl);
+}
+
+// Sensor term from the per-axis squares scaled by 1 / dt^2 ((x + y), z),
+// cs = off_s / dt + e_s.
+__device__ __forceinline__ float group_sensor_term(float aq, float zq, float cs) {
+    const float s2 = aq + zq;
+    return fmaf(s2, __builtin_amdgcn_rsqf(s2), cs);
+}
+
+// band * 2 + (f < tw): the compare's lane bit enters as the carry of v_addc.
+__device__ __forceinline__ uint32_t band_push(uint32_t band, float f, float tw) {
+    uint32_t out;
+    asm("v_cmp_lt_f32_e32 vcc, %2, %3\n\t"
+        "v_addc_co_u32_e32 %0, vcc, %1, %1, vcc"
+        : "=v"(out)
+        : "v"(band), "v"(f), "v"(tw)
+        : "vcc");
+    return out;
+}
+
+__device__ __forceinline__ uint32_t group_fast(float (&acc)[16], const float (&rl)[16],
+                                               const float (&rs)[16], const float2* win, int base,
+                                               float tw) {
+    uint32_t band = 0;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const float q = rl[v] + rs[v];
+        const int k = static_cast<int>(q);
+        band = band_push(band, __builtin_amdgcn_fractf(q), tw);
+        const float2 e = win[base + k];
+        acc[v] += fmaf(q, e.y, e.x);
+    }
+    return band;
+}
+
+// Distance from p to the box [lo, hi] with the hardware square root (1 ulp):
+// bounds only (window start, error bound, near test), all with margins.
+__device__ __forceinline__ float box_dist_fast(const float p[3], const float lo[3],
+                                               const float hi[3], bool far) {
+    float s = 0.f;
+    for (int ax = 0; ax < 3; ++ax) {
+        const float d = far ? fmaxf(fabsf(lo[ax] - p[ax]), fabsf(hi[ax] - p[ax]))
+                            : fmaxf(fmaxf(lo[ax] - p[ax], p[ax] - hi[ax]), 0.f);
+        s += d * d;
+    }
+    return __builtin_amdgcn_sqrtf(s);
+}
+
+// Raw window samples of 4 bins (converted and thresholded when stored).
+template <int STORAGE>
+struct GroupRaw {
+    using type = float4;
+};
+template <>
+struct GroupRaw<FBP_STORAGE_F16> {
+    using type = uint2;
+};
+template <>
+struct GroupRaw<FBP_STORAGE_BF16> {
+    using type = uint2;
+};
+
+template <int STORAGE>
+__device__ __forceinline__ void group_convert(const typename GroupRaw<STORAGE>::type& r, float sc,
+                                              float (&v)[4]) {
+    if constexpr (STORAGE == FBP_STORAGE_F32) {
+        v[0] = r.x;
+        v[1] = r.y;
+        v[2] = r.z;
+        v[3] = r.w;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[c] = (v[c] > kEps) ? v[c] : 0.f;
+    } else {
+        const uint32_t w[2] = {r.x, r.y};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const uint16_t b = uint16_t(w[c >> 1] >> (16 * (c & 1)));
+            if constexpr (STORAGE == FBP_STORAGE_F16)
+                v[c] = __half2float(__ushort_as_half(b)) * sc;
+            else
+                v[c] = __uint_as_float(uint32_t(b) << 16) * sc;
+        }
+    }
+}
+
+template <int STORAGE>
+__global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArgs a) {
+    using Raw = typename GroupRaw<STORAGE>::type;
+    extern __shared__ float smem[];
+    // [2][kGroup][W] window entries, [3][kGroup] pair parameters, the chunk's
+    // lasers [groups_per_chunk * kGroup] and sensors [sensors_per_part],
+    // [T] weight factors, [16][512] correction slots.
+    float2* winbuf = reinterpret_cast<float2*>(smem);
+    GroupPair* prmbuf = reinterpret_cast<GroupPair*>(winbuf + 2 * kGroup * a.W);
+    GroupLaser* lasl = reinterpret_cast<GroupLaser*>(prmbuf + 3 * kGroup);
+    float4* senl = reinterpret_cast<float4*>(lasl + a.groups_per_chunk * kGroup);
+    float2* tab = reinterpret_cast<float2*>(senl + a.sensors_per_part);
+    float* corrl = reinterpret_cast<float*>(tab + a.T) + threadIdx.x;
+
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+    // Blocks are dealt to the XCDs round-robin; consecutive indices on one XCD
+    // take the tiles of one chunk, so its L2 serves their shared windows.
+    const uint32_t nb8 = a.num_blocks / 8u;
+    const uint32_t bid = blockIdx.x < nb8 * 8u ? (blockIdx.x % 8u) * nb8 + blockIdx.x / 8u
+                                               : blockIdx.x;
+    const uint32_t tile = bid % a.g.num_tiles;
+    const uint32_t chunk = bid / a.g.num_tiles;
+    const uint32_t tx = tile % a.g.tiles[0];
+    const uint32_t ty = (tile / a.g.tiles[0]) % a.g.tiles[1];
+    const uint32_t tz = tile / (a.g.tiles[0] * a.g.tiles[1]);
+    const uint32_t X0 = tx * a.g.tile[0], Y0 = ty * a.g.tile[1], Z0 = tz * a.g.tile[2];
+    const int lx = lane & 3, lz = (lane >> 2) & 3, ly = lane >> 4;
+
+    float px[4], pz[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        px[i] = voxel_center(a.g.hmin[0], a.g.vs[0], X0 + lx + 4 * i);
+        pz[i] = voxel_center(a.g.hmin[2], a.g.vs[2], Y0 + ly + 4 * i);
+    }
+    const float py = voxel_center(a.g.hmin[1], a.g.vs[1], Z0 + lz + 4 * uint32_t(wave));
+    float lo[3], hi[3];
+    tile_box(a.g, X0, Y0, Z0, lo, hi);
+
+    const uint32_t lgc = chunk / a.sensor_parts, sp = chunk % a.sensor_parts;
+    const uint32_t g_begin = lgc * a.groups_per_chunk;
+    const uint32_t g_end = min(a.groups, g_begin + a.groups_per_chunk);
+    const uint32_t s_begin = sp * a.sensors_per_part;
+    const uint32_t s_end = min(a.S, s_begin + a.sensors_per_part);
+    const uint32_t ns = s_end > s_begin ? s_end - s_begin : 0u;
+    const uint32_t nbatch = (g_end > g_begin ? g_end - g_begin : 0u) * ns;
+
+#pragma unroll
+    for (int v = 0; v < 16; ++v) corrl[v * kGThreads] = 0.f;
+    // Weight factors of bin k (m = k + 1/2): {3 r^2, -2 r^3}.
+    for (uint32_t k = tid; k < a.T; k += kGThreads) {
+        const double r = 1.0 / (double(k) + 0.5);
+        tab[k] = make_float2(float(3.0 * r * r), float(-2.0 * r * r * r));
+    }
+    // The chunk's lasers (with their bounds against the tile) and sensors
+    // (with |s - Spos|), so the per-batch parameters read LDS only.
+    const uint32_t l_begin = g_begin * kGroup;
+    const uint32_t nl = min(a.L, g_end * kGroup) - min(a.L, l_begin);
+    for (uint32_t i = tid; i < nl; i += kGThreads) {
+        const uint32_t l = l_begin + i;
+        GroupLaser G;
+        const float lp3[3] = {a.laser[l].x, a.laser[l].y, a.laser[l].z};
+        G.x = lp3[0];
+        G.y = lp3[1];
+        G.z = lp3[2];
+        G.offl = a.discard == 0
+                     ? distance3(lp3[0], lp3[1], lp3[2], a.lpos[0], a.lpos[1], a.lpos[2])
+                     : 0.f;
+        G.rmin = box_dist_fast(lp3, lo, hi, false);
+        G.rmax = box_dist_fast(lp3, lo, hi, true);
+        G.el = kGroupErr * (G.rmax + G.offl + fabsf(a.t0)) * a.inv_dt + 0.5e-6f;
+        G.cl = (G.offl - a.t0) * a.inv_dt + G.el;
+        lasl[i] = G;
+    }
+    for (uint32_t i = tid; i < ns; i += kGThreads) {
+        const uint32_t s = s_begin + i;
+        const float x = a.sensor[s].x, y = a.sensor[s].y, z = a.sensor[s].z;
+        const float offs =
+            a.discard == 0 ? distance3(x, y, z, a.spos[0], a.spos[1], a.spos[2]) : 0.f;
+        senl[i] = make_float4(x, y, z, offs);
+    }
+    __syncthreads();
+
+    // Parameters of batch b's pair g (computed by thread g of a batch's eight)
+    // from the staged lasers and sensors.
+    auto params = [&](uint32_t b, int g, GroupPair* out) {
+        GroupPair P;
+        const uint32_t li = (b / ns) * kGroup + uint32_t(g);
+        const float4 S4 = senl[b % ns];
+        const float sp3[3] = {S4.x, S4.y, S4.z};
+        P.sx = sp3[0];
+        P.sy = sp3[1];
+        P.sz = sp3[2];
+        const float offs = S4.w;
+        const float rsmin = box_dist_fast(sp3, lo, hi, false);
+        const float rsmax = box_dist_fast(sp3, lo, hi, true);
+        const float es = kGroupErr * (rsmax + offs) * a.inv_dt + 0.5e-6f;
+        P.cs = offs * a.inv_dt + es;
+        P.pad[0] = P.pad[1] = 0.f;
+        if (l_begin + li >= a.L) {
+            P.fast = -1;
+            P.base = P.kmin = P.kmin4 = 0;
+            P.lx = P.ly = P.lz = P.off = P.cl = 0.f;
+            P.e = es;
+            *out = P;
+            return;
+        }
+        const GroupLaser G = lasl[li];
+        P.lx = G.x;
+        P.ly = G.y;
+        P.lz = G.z;
+        P.off = a.discard == 0 ? rn_add(G.offl, offs) : 0.f;
+        P.cl = G.cl;
+        P.e = G.el + es;
+        const float dmin = G.rmin + rsmin - a.t0 + P.off;
+        const float dmax = G.rmax + rsmax - a.t0 + P.off;
+        P.kmin = int(floorf(dmin * a.inv_dt)) - 3;
+        P.kmin4 = P.kmin & ~3;
+        P.base = g * a.W - P.kmin4;
+        const bool near = (dmin < 0.0045f && dmax > -0.0045f) || G.rmin < 1.0e-3f || rsmin < 1.0e-3f;
+        P.fast = (!near && P.kmin >= kLinMinBin) ? 1 : 0;
+        *out = P;
+    };
+
+    // Raw loads of batch b: wave g fetches pair g's bins kmin4 + 4 j .. + 3,
+    // j = lane + 64 i.  Nothing reads them before the store, so they stay in
+    // flight across the batch; loads outside the pair read element 0.
+    auto fetch = [&](uint32_t b, const GroupPair* prm, Raw (&reg)[kGVec]) {
+        const int fast = rfl(prm[wave].fast);
+        const int kmin4 = rfl(prm[wave].kmin4);
+        const uint32_t grp = g_begin + b / ns;
+        const uint32_t s = s_begin + b % ns;
+        const uint64_t pair = uint64_t(grp * kGroup + uint32_t(wave)) * a.S + s;
+#pragma unroll
+        for (int i = 0; i < kGVec; ++i) {
+            const int j = lane + 64 * i;
+            const int k0 = kmin4 + 4 * j;
+            const bool ok = fast >= 0 && 4 * j < a.W && k0 >= 0 && k0 < int(a.T);
+            const uint64_t idx = ok ? pair * a.T + uint32_t(k0) : 0;
+            if constexpr (STORAGE == FBP_STORAGE_F32)
+                reg[i] = *reinterpret_cast<const float4*>(static_cast<const float*>(a.data) + idx);
+            else
+                reg[i] = *reinterpret_cast<const uint2*>(static_cast<const uint16_t*>(a.data) + idx);
+        }
+    };
+    auto store = [&](uint32_t b, float2* win, const GroupPair* prm, const Raw (&reg)[kGVec]) {
+        const int fast = rfl(prm[wave].fast);
+        if (fast < 0) return;
+        const int kmin4 = rfl(prm[wave].kmin4);
+        const float e = rfl(prm[wave].e);
+        float sc = 1.f;
+        if constexpr (STORAGE != FBP_STORAGE_F32) {
+            const uint32_t grp = g_begin + b / ns;
+            const uint32_t s = s_begin + b % ns;
+            sc = a.scales[uint64_t(grp * kGroup + uint32_t(wave)) * a.S + s];
+        }
+#pragma unroll
+        for (int i = 0; i < kGVec; ++i) {
+            const int j = lane + 64 * i;
+            if (4 * j < a.W) {
+                const int k0 = kmin4 + 4 * j;
+                const bool ok = k0 >= 0 && k0 < int(a.T);
+                float v[4];
+                group_convert<STORAGE>(reg[i], sc, v);
+                float4 e01 = make_float4(0.f, 0.f, 0.f, 0.f), e23 = e01;
+                if (ok) {
+                    if (fast == 1) {
+                        const float4 t01 = *reinterpret_cast<const float4*>(tab + k0);
+                        const float4 t23 = *reinterpret_cast<const float4*>(tab + k0 + 2);
+                        e01 = make_float4(v[0] * fmaf(-e, t01.y, t01.x), v[0] * t01.y,
+                                          v[1] * fmaf(-e, t01.w, t01.z), v[1] * t01.w);
+                        e23 = make_float4(v[2] * fmaf(-e, t23.y, t23.x), v[2] * t23.y,
+                                          v[3] * fmaf(-e, t23.w, t23.z), v[3] * t23.w);
+                    } else {
+                        e01 = make_float4(v[0], 0.f, v[1], 0.f);
+                        e23 = make_float4(v[2], 0.f, v[3], 0.f);
+                    }
+                }
+                float4* dst = reinterpret_cast<float4*>(win + wave * a.W + 4 * j);
+                dst[0] = e01;
+                dst[1] = e23;
+            }
+        }
+    };
+
+    float acc[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+    float rl[kGroup][16];
+#pragma unroll
+    for (int g = 0; g < kGroup; ++g)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) rl[g][v] = 0.f;
+
+    const int per = kGroup * a.W;
+    Raw reg[kGVec];
+    if (nbatch > 0) {
+        if (tid < kGroup) params(0, tid, prmbuf + tid);
+        if (tid >= kGroup && tid < 2 * kGroup && nbatch > 1)
+            params(1, tid - kGroup, prmbuf + tid);
+        __syncthreads();  // parameters and the weight table
+        fetch(0, prmbuf, reg);
+        store(0, winbuf, prmbuf, reg);
+        __syncthreads();
+    }
+    for (uint32_t b = 0; b < nbatch; ++b) {
+        const int cur = int(b & 1u);
+        const GroupPair* prm = prmbuf + (b % 3u) * kGroup;
+        const bool has_next = b + 1 < nbatch;
+        const GroupPair* prm_next = prmbuf + ((b + 1) % 3u) * kGroup;
+        if (has_next) fetch(b + 1, prm_next, reg);
+
+        if (b % ns == 0) {
+            // New laser group: the eight laser terms of the lane's voxels.
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) {
+                const float lxg = rfl(prm[g].lx), lyg = rfl(prm[g].ly), lzg = rfl(prm[g].lz);
+                const float cl = rfl(prm[g].cl);
+                float XL2[4], ZL2[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    XL2[i] = sq(rn_sub(px[i], lxg));
+                    ZL2[i] = sq(rn_sub(pz[i], lzg));
+                }
+                const float YL2 = sq(rn_sub(py, lyg));
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int m = 0; m < 4; ++m)
+                        rl[g][i * 4 + m] = group_laser_term(XL2[i], YL2, ZL2[m], cl, a.inv_dt);
+            }
+        }
+        // Sensor terms of the batch's sensor.
+        float rs[16];
+        const float sxp = rfl(prm[0].sx), syp = rfl(prm[0].sy), szp = rfl(prm[0].sz);
+        {
+            const float cs = rfl(prm[0].cs);
+            float AQ[4], ZQ[4];
+            const float YS2 = sq(rn_sub(py, syp));
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                AQ[i] = (sq(rn_sub(px[i], sxp)) + YS2) * a.inv_dt2;
+                ZQ[i] = sq(rn_sub(pz[i], szp)) * a.inv_dt2;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int m = 0; m < 4; ++m) rs[i * 4 + m] = group_sensor_term(AQ[i], ZQ[m], cs);
+        }
+        uint32_t mw[kGroup / 2];
+#pragma unroll
+        for (int w = 0; w < kGroup / 2; ++w) mw[w] = 0;
+        int fast_pairs = 0, exact_pairs = 0;
+#pragma unroll
+        for (int g = 0; g < kGroup; ++g) {
+            const int fg = rfl(prm[g].fast);
+            fast_pairs |= (fg == 1 ? 1 : 0) << g;
+            exact_pairs |= (fg == 0 ? 1 : 0) << g;
+        }
+        if (fast_pairs == (1 << kGroup) - 1) {
+            // Common case, straight-line: all eight pairs take the fast body.
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) {
+                const int base = rfl(prm[g].base) + cur * per;
+                const float tw = 2.f * rfl(prm[g].e);
+                mw[g >> 1] |= group_fast(acc, rl[g], rs, winbuf, base, tw) << (16 * (g & 1));
+            }
+        } else {
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) {
+                if (fast_pairs & (1 << g)) {
+                    const int base = rfl(prm[g].base) + cur * per;
+                    const float tw = 2.f * rfl(prm[g].e);
+                    mw[g >> 1] |= group_fast(acc, rl[g], rs, winbuf, base, tw) << (16 * (g & 1));
+                }
+            }
+        }
+        // Pairs near the wall or zero distance, or with bins below the linear
+        // range: the reference's chain for every voxel (entries {I, 0}).
+        while (exact_pairs) {
+            const int g = __builtin_ctz(exact_pairs);
+            exact_pairs &= exact_pairs - 1;
+            const GroupPair& P = prm[g];
+            const LaserPos lp{rfl(P.lx), rfl(P.ly), rfl(P.lz)};
+            const int base = rfl(P.base) + cur * per;
+            const float off = rfl(P.off);
+            ExhaustiveArgs ea{};
+            ea.t0 = a.t0;
+            ea.discard = a.discard;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const float de = exact_dist(px[i], py, pz[m], lp, sxp, syp, szp, ea, off);
+                    const int k = bin_of(de, a.dt);
+                    const float w = safe_rcp(rn_mul(de, de)) * a.dt2;
+                    acc[i * 4 + m] = fmaf(winbuf[base + k].x, w, acc[i * 4 + m]);
+                }
+        }
+        if (has_next) {
+            // The other window buffer was last read in batch b - 1.
+            store(b + 1, winbuf + (cur ^ 1) * per, prm_next, reg);
+            // Parameters of batch b + 2 go to the slot batch b - 1 used.
+            if (tid < kGroup && b + 2 < nbatch)
+                params(b + 2, tid, prmbuf + ((b + 2) % 3u) * kGroup + tid);
+        }
+        // Fix-ups of this batch's band voxels in a fixed order (pair 0's
+        // last voxel first): bits 31..16 of word w are pair 2w + 1, 15..0
+        // pair 2w; within a pair bit 15 - v is voxel v.
+        uint64_t blo = uint64_t(mw[0]) | (uint64_t(mw[1]) << 32);
+        uint64_t bhi = uint64_t(mw[2]) | (uint64_t(mw[3]) << 32);
+        while (blo | bhi) {
+            int bit;
+            if (blo) {
+                bit = __builtin_ctzll(blo);
+                blo &= blo - 1u;
+            } else {
+                bit = 64 + __builtin_ctzll(bhi);
+                bhi &= bhi - 1u;
+            }
+            const int g = bit >> 4;
+            const int v = 15 - (bit & 15);
+            const int i = v >> 2, m = v & 3;
+            const GroupPair& P = prm[g];
+            const float x = pick4f(px, i), z = pick4f(pz, m);
+            const float ql = group_laser_term(sq(rn_sub(x, P.lx)), sq(rn_sub(py, P.ly)),
+                                              sq(rn_sub(z, P.lz)), P.cl, a.inv_dt);
+            const float aq = (sq(rn_sub(x, P.sx)) + sq(rn_sub(py, P.sy))) * a.inv_dt2;
+            const float zq = sq(rn_sub(z, P.sz)) * a.inv_dt2;
+            const float q = ql + group_sensor_term(aq, zq, P.cs);
+            const int kf = static_cast<int>(q);
+            ExhaustiveArgs ea{};
+            ea.t0 = a.t0;
+            ea.discard = a.discard;
+            const float de =
+                exact_dist(x, py, z, LaserPos{P.lx, P.ly, P.lz}, P.sx, P.sy, P.sz, ea, P.off);
+            const int ke = bin_of(de, a.dt);
+            if (ke != kf) {
+                // ke is kf - 1 (the reference's quotient lies within 2 e
+                // below q); the clamp keeps the read inside the pair's window.
+                const int base = P.base + cur * per;
+                const int kc = min(max(ke, P.kmin), P.kmin + a.W - 5);
+                const float2 ef = winbuf[base + kf];
+                const float2 ee = winbuf[base + kc];
+                corrl[v * kGThreads] += fmaf(q, ee.y, ee.x) - fmaf(q, ef.y, ef.x);
+            }
+        }
+        __syncthreads();
+    }
+
+    // Slots of the workgroup tile (lt = 4 x 4 x 8): wave w holds oz >> 2 = w.
+    float* out = a.partial + (size_t(chunk) * a.g.num_tiles + tile) * a.g.slots * 64;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+            out[size_t((i * 8 + wave) * 4 + m) * 64 + lane] =
+                acc[i * 4 + m] + corrl[(i * 4 + m) * kGThreads];
+}
+
 namespace {
 constexpr int kMaxDevices = 64;
 std::mutex g_noalloc_mu;
@@ -509,6 +1032,100 @@ static bool linear_weights_apply(const fbp_reconstruction_info& in) {
     return kmin >= double(kLinMinBin);
 }
 
+// Laser-group kernel for linear-weight captures with at least kGroup lasers
+// and 64 sensors whose window fits the prefetch registers; *used = false
+// (nothing launched) otherwise.  Chunks are (laser-group range, sensor
+// range) blocks, reduced in a fixed order like the per-pair kernel's.
+static fbp_status exhaustive_group_launch(fbp_context ctx, const fbp_reconstruction_info* info,
+                                          const fbp_transient* tr, const fbp_vec3* laser_targets,
+                                          const fbp_vec3* sensor_targets, float* volume,
+                                          void* stream, float* fixed_partial, size_t fixed_floats,
+                                          bool* used) {
+    *used = false;
+    GroupArgs a{};
+    a.g = make_geom(*info, 4, 4, 8);
+    a.L = info->num_laser_targets;
+    a.S = info->num_sensor_targets;
+    a.T = info->num_time_bins;
+    // Window of the 4-bin aligned start: the per-pair window plus 4 bins.
+    a.W = window_bins(a.g, info->time_step, 2) + 4;
+    // 4-bin loads: T a multiple of 4 and the samples 16-byte (F32) or 8-byte
+    // (F16/BF16) aligned.
+    const uintptr_t align = tr->storage == FBP_STORAGE_F32 ? 16u : 8u;
+    if (a.L < uint32_t(kGroup) || a.S < 64u || a.T > 8192u || (a.T & 3u) != 0u ||
+        a.W > 4 * 64 * kGVec || (reinterpret_cast<uintptr_t>(tr->data) % align) != 0u)
+        return FBP_OK;
+
+    a.data = tr->data;
+    a.scales = tr->scales;
+    a.laser = laser_targets;
+    a.sensor = sensor_targets;
+    a.t0 = info->time_offset;
+    a.dt = info->time_step;
+    a.inv_dt = 1.0f / info->time_step;
+    a.inv_dt2 = a.inv_dt * a.inv_dt;
+    a.dt2 = info->time_step * info->time_step;
+    a.discard = info->discard_first_last_bounces;
+    a.lpos[0] = info->laser_position.x;
+    a.lpos[1] = info->laser_position.y;
+    a.lpos[2] = info->laser_position.z;
+    a.spos[0] = info->sensor_position.x;
+    a.spos[1] = info->sensor_position.y;
+    a.spos[2] = info->sensor_position.z;
+    a.groups = (a.L + kGroup - 1) / kGroup;
+
+    // About four workgroups (one per CU at a time) per CU over the launch:
+    // laser-group chunks first, then sensor parts of at least 64 sensors.
+    const uint64_t cus = uint64_t(ctx->num_cus > 0 ? ctx->num_cus : 256);
+    uint64_t want = (4 * cus + a.g.num_tiles - 1) / a.g.num_tiles;
+    const size_t chunk_floats = size_t(a.g.num_tiles) * a.g.slots * 64;
+    if (fixed_partial) {
+        const uint64_t fit = fixed_floats / chunk_floats;
+        if (fit == 0) return FBP_OK;  // the per-pair kernel reports the size error
+        want = std::min<uint64_t>(want, fit);
+    }
+    if (want < 1) want = 1;
+    uint64_t lgc = std::min<uint64_t>(a.groups, want);
+    a.groups_per_chunk = uint32_t((a.groups + lgc - 1) / lgc);
+    lgc = (a.groups + a.groups_per_chunk - 1) / a.groups_per_chunk;
+    uint64_t parts = std::max<uint64_t>(1, want / lgc);
+    parts = std::min<uint64_t>(parts, std::max<uint64_t>(1, a.S / 64));
+    a.sensors_per_part = uint32_t((a.S + parts - 1) / parts);
+    // LDS: windows, pair parameters, the chunk's lasers and sensors, weight
+    // factors, correction slots.  Sensor ranges shrink (more parts) to fit.
+    auto lds_bytes = [&](uint32_t spp) {
+        return size_t(2) * kGroup * a.W * sizeof(float2) + 3 * kGroup * sizeof(GroupPair) +
+               size_t(a.groups_per_chunk) * kGroup * sizeof(GroupLaser) + size_t(spp) * sizeof(float4) +
+               size_t(a.T) * sizeof(float2) + size_t(16) * kGThreads * sizeof(float);
+    };
+    while (a.sensors_per_part > 64 && lds_bytes(a.sensors_per_part) > size_t(ctx->max_lds_per_block))
+        a.sensors_per_part = (a.sensors_per_part + 1) / 2;
+    const size_t shmem = lds_bytes(a.sensors_per_part);
+    if (shmem > size_t(ctx->max_lds_per_block)) return FBP_OK;
+    a.sensor_parts = uint32_t((a.S + a.sensors_per_part - 1) / a.sensors_per_part);
+    const uint64_t chunks = lgc * a.sensor_parts;
+    if (fixed_partial) {
+        if (chunks * chunk_floats > fixed_floats) return FBP_OK;
+        a.partial = fixed_partial;
+    } else {
+        FBP_CHECK(ensure_scratch(ctx, size_t(chunks) * chunk_floats * sizeof(float)));
+        a.partial = static_cast<float*>(ctx->scratch);
+    }
+    a.num_blocks = uint32_t(chunks * a.g.num_tiles);
+
+    void (*kern)(GroupArgs) = tr->storage == FBP_STORAGE_F32   ? exhaustive_group_kernel<FBP_STORAGE_F32>
+                              : tr->storage == FBP_STORAGE_F16 ? exhaustive_group_kernel<FBP_STORAGE_F16>
+                                                               : exhaustive_group_kernel<FBP_STORAGE_BF16>;
+    hipStream_t s = as_stream(stream);
+    FBP_CHECK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
+    hipLaunchKernelGGL(kern, dim3(a.num_blocks), dim3(kGThreads), shmem, s, a);
+    FBP_CHECK_HIP(ctx, hipGetLastError());
+    *used = true;
+    return launch_reduce(ctx, a.partial, nullptr, volume, a.g, uint32_t(chunks),
+                         1.0 / (double(a.dt) * double(a.dt)), s);
+}
+
 // Shared body of the two exhaustive entry points.  fixed_partial == nullptr:
 // partial sums in the context's scratch arena (grown on demand).  Otherwise
 // they go to fixed_partial (fixed_floats floats) and the number of pair
@@ -532,6 +1149,19 @@ static fbp_status exhaustive_voxel_impl(fbp_context ctx, const fbp_reconstructio
     if (tr->layout != FBP_LAYOUT_TIME_FASTEST)
         return fail(ctx, FBP_ERR_UNSUPPORTED, "exhaustive: transient must be [l][s][t]");
     FBP_CHECK(bind(ctx));
+
+    const bool coff = !(info->time_offset == 0.f && info->discard_first_last_bounces != 0);
+    const bool link = linear_weights_apply(*info);
+    ctx->last_exhaustive_group = 0;
+    if (link && ctx->exhaustive_group_kernel) {
+        bool used = false;
+        FBP_CHECK(exhaustive_group_launch(ctx, info, tr, laser_targets, sensor_targets, volume,
+                                          stream, fixed_partial, fixed_floats, &used));
+        if (used) {
+            ctx->last_exhaustive_group = 1;
+            return FBP_OK;
+        }
+    }
 
     ExhaustiveArgs a{};
     a.data = tr->data;
@@ -603,8 +1233,6 @@ static fbp_status exhaustive_voxel_impl(fbp_context ctx, const fbp_reconstructio
     // parameters and masks): beyond the default 64 KiB for the longest windows.
     if (shmem > size_t(ctx->max_lds_per_block))
         return fail(ctx, FBP_ERR_UNSUPPORTED, "exhaustive: %zu B of LDS per workgroup needed", shmem);
-    const bool coff = !(a.t0 == 0.f && a.discard != 0);
-    const bool link = linear_weights_apply(*info);
     void (*kern)(ExhaustiveArgs) = pick_kernel(tr->storage, coff, link);
     FBP_CHECK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));

@@ -131,6 +131,11 @@ fbp_status fbp_context_set_option(fbp_context ctx, int32_t option, int32_t value
                 return fbp::fail(ctx, FBP_ERR_INVALID_ARGUMENT, "option value must be 0 or 1");
             ctx->voxel_scattering = value;
             return FBP_OK;
+        case FBP_OPTION_EXHAUSTIVE_LASER_GROUPS:
+            if (value != 0 && value != 1)
+                return fbp::fail(ctx, FBP_ERR_INVALID_ARGUMENT, "option value must be 0 or 1");
+            ctx->exhaustive_group_kernel = value;
+            return FBP_OK;
         default:
             return fbp::fail(ctx, FBP_ERR_INVALID_ARGUMENT, "unknown option %d", int(option));
     }
@@ -145,9 +150,18 @@ fbp_status fbp_context_get_option(fbp_context ctx, int32_t option, int32_t* valu
         case FBP_OPTION_VOXEL_SCATTERING:
             *value = ctx->voxel_scattering;
             return FBP_OK;
+        case FBP_OPTION_EXHAUSTIVE_LASER_GROUPS:
+            *value = ctx->exhaustive_group_kernel;
+            return FBP_OK;
         default:
             return fbp::fail(ctx, FBP_ERR_INVALID_ARGUMENT, "unknown option %d", int(option));
     }
+}
+
+fbp_status fbp_context_last_exhaustive_kernel(fbp_context ctx, int32_t* laser_groups) {
+    if (!ctx || !laser_groups) return FBP_ERR_INVALID_ARGUMENT;
+    *laser_groups = ctx->last_exhaustive_group;
+    return FBP_OK;
 }
 
 fbp_status fbp_context_last_confocal_path(fbp_context ctx, int32_t* aligned) {

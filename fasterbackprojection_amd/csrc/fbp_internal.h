@@ -19,6 +19,11 @@ struct fbp_context_s {
     // 1: confocal voxel and MIS backprojection take the voxel-scattering
     // walk over every bin a voxel reaches (reference compile-time switch).
     int voxel_scattering = 0;
+    // 1: the exhaustive voxel backprojection may take the laser-group kernel
+    // (bp_exhaustive.hip) when the capture qualifies; 0: per-pair kernel only.
+    int exhaustive_group_kernel = 1;
+    // Kernel of the last exhaustive voxel call: 1 = laser-group kernel.
+    int last_exhaustive_group = 0;
     // Device int: path flag of the last confocal voxel call (1 = aligned
     // path), valid when path_flag_valid is set.
     int* path_flag = nullptr;

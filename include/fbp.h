@@ -120,7 +120,12 @@ typedef enum fbp_option {
      * d < dist + e (e = half the voxel diagonal), each weighted by
      * safeRCP(dist^2); the walk ends at the first bin >= T.  Runs a
      * dedicated kernel (not the aligned-grid path). */
-    FBP_OPTION_VOXEL_SCATTERING = 1
+    FBP_OPTION_VOXEL_SCATTERING = 1,
+    /* 1 (default): fbp_backproject_exhaustive_voxel may run the laser-group
+     * kernel (sensor distances shared by groups of eight lasers) when the
+     * capture qualifies; 0: always the per-pair kernel.  Same bins; volumes
+     * agree to FP32 rounding of the weights. */
+    FBP_OPTION_EXHAUSTIVE_LASER_GROUPS = 2
 } fbp_option;
 fbp_status fbp_context_set_option(fbp_context ctx, int32_t option, int32_t value);
 fbp_status fbp_context_get_option(fbp_context ctx, int32_t option, int32_t* value);
@@ -130,6 +135,12 @@ fbp_status fbp_context_get_option(fbp_context ctx, int32_t option, int32_t* valu
  * the aligned-grid path, 0 for the general (gather) kernel.  Synchronises
  * the device. */
 fbp_status fbp_context_last_confocal_path(fbp_context ctx, int32_t* aligned);
+
+/* Which kernel the last fbp_backproject_exhaustive_voxel[_noalloc] call on
+ * this context launched: *laser_groups = 1 for the laser-group kernel, 0 for
+ * the per-pair kernel (host-side record, no synchronisation; no reference
+ * counterpart). */
+fbp_status fbp_context_last_exhaustive_kernel(fbp_context ctx, int32_t* laser_groups);
 
 /* --------------------------------------------------------- backprojection */
 

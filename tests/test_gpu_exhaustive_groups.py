@@ -1,0 +1,144 @@
+"""Laser-group kernel of the exhaustive voxel backprojection
+(backprojectExhaustiveVoxel, FB/transient_reconstruction.cuh:138-166): taken
+for linear-weight captures (every bin >= 160) with at least 8 lasers and 64
+sensors.  Sensor distances are shared by groups of eight lasers and the bin
+estimate's error bound is folded into the distance terms; voxels whose
+estimate lies within the bound of a bin boundary take the reference's IEEE
+chain after each batch.
+
+The captures here put the hidden box 0.5 m from the wall at 4.8 mm bins and
+use 9 or 144 lasers (a partial last group) and 81 or 144 sensors (one or two
+sensor parts per chunk).  Voxel pitches keep a 16 x 16 x 32 tile's bin
+window within the kernel's 512 bins."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from fasterbackprojection_amd.api import Geometry, make_info
+from fasterbackprojection_amd.scene import wall_grid
+
+pytestmark = pytest.mark.gpu
+
+
+def cuda(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def spiky(rng, shape):
+    """Adjacent bins differ by orders of magnitude: a wrong bin moves a voxel
+    far outside the tolerance."""
+    v = (10.0 ** rng.uniform(-3, 3, size=shape)).astype(np.float32)
+    v[rng.random(shape) < 0.2] = 0.0
+    return v
+
+
+def geometry(n_las, n_sen, T=1024, res=(32, 40, 32), discard=1, t0=0.0, depth0=0.5):
+    return Geometry(laser_targets=wall_grid(n_las, 1.0), sensor_targets=wall_grid(n_sen, 1.0),
+                    num_time_bins=T, time_step=0.0048,
+                    hidden_min=(-0.4, depth0, -0.45), hidden_max=(0.45, depth0 + 0.9, 0.4),
+                    voxel_resolution=res, confocal=False, discard_first_last_bounces=discard,
+                    time_offset=t0, laser_position=(0.1, -0.8, 0.2),
+                    sensor_position=(-0.2, -0.7, 0.1), wall_size=(1.0, 1.0))
+
+
+def run(ctx, info, I, g, groups=True, **kw):
+    ctx.set_exhaustive_laser_groups(groups)
+    try:
+        out = ctx.backproject_exhaustive(info, I, cuda(g.laser_targets), cuda(g.sensor_targets),
+                                         **kw)
+        used = ctx.last_exhaustive_laser_groups()
+    finally:
+        ctx.set_exhaustive_laser_groups(True)
+    return out, used
+
+
+@pytest.mark.parametrize("discard,t0", [(1, 0.0), (0, 0.1)])
+def test_group_kernel_matches_oracle(ctx, discard, t0):
+    g = geometry(3, 9, discard=discard, t0=t0)
+    info = make_info(g)
+    I = spiky(np.random.default_rng(21), (9, 81, 1024))
+    ref = oracle.exhaustive_voxel(info, I, g.laser_targets, g.sensor_targets)
+    out, used = run(ctx, info, cuda(I), g)
+    assert used
+    got = out.cpu().numpy().astype(np.float64)
+    assert rel_l2(got, ref) <= 1e-4
+    big = np.abs(ref) > 0.01 * np.abs(ref).max()
+    assert np.max(np.abs(got[big] - ref[big]) / np.abs(ref[big])) <= 5e-4
+    # The per-pair kernel on the same input: same bins, weights to FP32 rounding.
+    other, used2 = run(ctx, info, cuda(I), g, groups=False)
+    assert not used2
+    assert rel_l2(got, other.cpu().numpy()) <= 1e-5
+
+
+def test_group_kernel_needs_the_linear_range(ctx):
+    """Bins below 160 (box 0.3 m from the wall): the per-pair kernel runs."""
+    g = geometry(3, 9, depth0=0.3)
+    info = make_info(g)
+    I = spiky(np.random.default_rng(22), (9, 81, 1024))
+    out, used = run(ctx, info, cuda(I), g)
+    assert not used
+    ref = oracle.exhaustive_voxel(info, I, g.laser_targets, g.sensor_targets)
+    assert rel_l2(out.cpu().numpy(), ref) <= 1e-4
+
+
+@pytest.mark.parametrize("n_sen,discard,t0", [(9, 1, 0.0), (12, 0, 0.1), (12, 1, 0.0)])
+def test_group_kernel_bins_exact(ctx, n_sen, discard, t0):
+    """One lit (laser, sensor) pair per call, I[l][s][t] = t + 1: with the
+    linear weights vol * dist^2 = (bin + 1)(1 + e), |e| <= 3e-5, which rounds
+    to bin + 1 below 10^4.  Pairs from the full and the partial laser group
+    and from both sensor parts; 24 x 40 x 24 voxels per pair, so the band
+    (about one update in a thousand) is crossed hundreds of times."""
+    T = 2048
+    g = geometry(3, n_sen, T=T, res=(24, 40, 24), discard=discard, t0=t0)
+    info = make_info(g)
+    L, S = len(g.laser_targets), len(g.sensor_targets)
+    rng = np.random.default_rng(5)
+    pairs = [(0, 0), (7, S - 1), (8, S // 2), (3, 1), (5, S - 2), (8, 0)]
+    pairs += [(int(l), int(s)) for l, s in zip(rng.integers(0, L, 10), rng.integers(0, S, 10))]
+    cl, cs = cuda(g.laser_targets), cuda(g.sensor_targets)
+    vols = []
+    for l, s in pairs:
+        I = np.zeros((L, S, T), np.float32)
+        I[l, s] = np.arange(1, T + 1, dtype=np.float32)
+        vols.append(ctx.backproject_exhaustive(info, cuda(I), cl, cs))
+        assert ctx.last_exhaustive_laser_groups()
+    got = torch.stack(vols).cpu().numpy()
+    for i, (l, s) in enumerate(pairs):
+        bins, dist = oracle.exhaustive_bins(info, g.laser_targets, g.sensor_targets, l, s)
+        rec = np.rint(got[i].astype(np.float64) * dist.astype(np.float64) ** 2).astype(np.int64) - 1
+        bad = np.argwhere(rec != bins)
+        assert bad.size == 0, (l, s, bad[:5].tolist())
+        assert (bins >= 160).all()
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_group_kernel_quantized_storage(ctx, storage):
+    """F16/BF16 storage with per-series scales: the group kernel against the
+    per-pair kernel on the same quantised samples, and against the oracle on
+    the dequantised samples."""
+    from fasterbackprojection_amd._lib import STORAGE_BF16, STORAGE_F16
+    st = STORAGE_F16 if storage == "f16" else STORAGE_BF16
+    g = geometry(3, 9)
+    info = make_info(g)
+    I = spiky(np.random.default_rng(23), (9, 81, 1024))
+    q, sc = ctx.quantize(cuda(I), st)
+    out, used = run(ctx, info, q, g, storage=st, scales=sc)
+    assert used
+    other, _ = run(ctx, info, q, g, groups=False, storage=st, scales=sc)
+    got = out.cpu().numpy()
+    assert rel_l2(got, other.cpu().numpy()) <= 1e-5
+    qh = q.cpu().numpy().reshape(-1, 1024)
+    if st == STORAGE_F16:
+        v = qh.view(np.float16).astype(np.float32)
+    else:
+        v = (qh.view(np.uint16).astype(np.uint32) << 16).view(np.float32)
+    deq = (v * sc.cpu().numpy()[:, None]).astype(np.float32).reshape(I.shape)
+    ref = oracle.exhaustive_voxel(info, deq, g.laser_targets, g.sensor_targets)
+    assert rel_l2(got, ref) <= 1e-4

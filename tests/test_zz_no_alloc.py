@@ -90,3 +90,51 @@ def test_noalloc_refuses_volumes_beyond_its_array():
         assert ctx.scratch_bytes() == 0
     finally:
         ctx.close()
+
+
+def _linear_case(n_las, n_sen, res, seed):
+    """Box 0.5 m from the wall at 4.8 mm bins (every bin >= 160): the
+    laser-group kernel of the exhaustive backprojection runs."""
+    lasers, sensors = wall_grid(n_las, 1.0), wall_grid(n_sen, 1.0)
+    g = Geometry(laser_targets=lasers, sensor_targets=sensors, num_time_bins=1024,
+                 time_step=0.0048, hidden_min=(-0.45, 0.5, -0.45), hidden_max=(0.45, 1.4, 0.45),
+                 voxel_resolution=res, confocal=False, wall_size=(1.0, 1.0))
+    rng = np.random.default_rng(seed)
+    I = (10.0 ** rng.uniform(-3, 2, (len(lasers), len(sensors), 1024))).astype(np.float32)
+    I[rng.random(I.shape) < 0.3] = 0.0
+    return g, I
+
+
+def test_noalloc_laser_group_kernel():
+    """Both entry points on the laser-group kernel: 144 lasers (18 groups of
+    8) x 64 sensors -> 128 x 128 x 64 voxels, whose 4 MiB of partial sums per
+    chunk leave the 32 MiB array 8 chunks (three laser groups each); then a
+    smaller capture (9 lasers, a partial second group) checked against the
+    FP64 oracle."""
+    noalloc_ctx = Context(0)
+    alloc_ctx = Context(0)
+    try:
+        cases = [_linear_case(12, 8, (128, 128, 64), 4), _linear_case(3, 8, (24, 40, 24), 5)]
+        for i, (g, I) in enumerate(cases):
+            info = make_info(g)
+            r = info.voxel_resolution
+            It, lt, st = _cuda(I), _cuda(g.laser_targets), _cuda(g.sensor_targets)
+            out = torch.full((r.z, r.y, r.x), float("nan"), dtype=torch.float32, device="cuda")
+            torch.cuda.synchronize()
+            mem = torch.cuda.memory_allocated()
+            noalloc_ctx.backproject_exhaustive_noalloc(info, It, lt, st, out)
+            torch.cuda.synchronize()
+            assert torch.cuda.memory_allocated() == mem
+            assert noalloc_ctx.scratch_bytes() == 0
+            assert noalloc_ctx.last_exhaustive_laser_groups(), i
+            ref = alloc_ctx.backproject_exhaustive(info, It, lt, st).cpu().numpy()
+            assert alloc_ctx.last_exhaustive_laser_groups(), i
+            got = out.cpu().numpy()
+            assert np.all(np.isfinite(got)), i
+            assert rel_l2(got, ref) <= 1e-6, i
+            if i == 1:
+                exact = oracle.exhaustive_voxel(info, I, g.laser_targets, g.sensor_targets)
+                assert rel_l2(got, exact) <= 1e-4
+    finally:
+        noalloc_ctx.close()
+        alloc_ctx.close()
