@@ -449,24 +449,26 @@ __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs
 // A workgroup of eight waves owns the same 16 (vx) x 16 (vy) x 32 (z) tile
 // with 16 voxels per lane (wave w holds z = Z0 + lz + 4w) and a range of
 // (laser group, sensor) batches; a laser group is kGroup = 8 consecutive
-// lasers.  Each lane keeps the group's eight laser terms of its 16 voxels in
-// registers while the group is unchanged and computes its 16 sensor terms
-// once per sensor, for the eight pairs (l, s) of the group, so an update
-// costs no square root:
-//   q = rl'[g][v] + rs'[v],  k = (int)q,  band <- fract(q) < 2 (e_l + e_s),
-//   acc[v] += A_k + B_k q.
-// rl' = |v - l| / dt + (off_l - t0) / dt + e_l and rs' = |v - s| / dt +
-// off_s / dt + e_s (rsq estimates), where e_l + e_s bounds the estimate's
-// error against fl(dist / dt) for the pair; the reference's quotient then
-// lies in [q - 2 (e_l + e_s), q], so its bin is k unless fract(q) is below
-// 2 (e_l + e_s): those (voxel, pair) updates are replaced by the IEEE chain
-// after the batch (deferred fix-ups into per-thread LDS slots).  The window
-// entry of bin k is the linear expansion of I / Q^2 about m = k + 1/2 as a
-// function of the shifted q: {I (3 r^2 + 2 e r^3), -2 I r^3}, r = 1/m,
-// e = e_l + e_s (bins >= kLinMinBin; other pairs take the exact chain).
-// One batch = one sensor and the group's eight pairs; wave g stages pair g's
-// window (from a 4-bin aligned start, 16-byte loads and stores), the next
-// batch's raw loads in flight while the current batch runs.
+// lasers.  Each lane keeps the group's eight exact laser distances of its 16
+// voxels in registers while the group is unchanged, and computes its 16
+// exact sensor distances once per sensor for the eight pairs (l, s) of the
+// group.  Both are the reference's own IEEE values (correctly rounded square
+// roots of the same squared sums), so an update starts from the reference's
+// dist = fl(rl + rs) (- t0 and + off when the chain has them) and needs no
+// square root:
+//   q0 = fl(dist * c), c = fl(1 / dt);  q = fma(fma(-q0, dt, dist), c, q0);
+//   k = (int)q;  acc[v] += A_k + B_k q.
+// The corrected quotient q equals the reference's fl(dist / dt) wherever a
+// bin could differ: the host checks every float within 16 ulps of each bin
+// boundary k dt (the only places where two quotients within an ulp of
+// dist / dt can truncate differently) before it chooses this kernel, so the
+// bins are the reference's exactly.  The window entry of bin k is the linear
+// expansion of I / q^2 about m = k + 1/2: {3 I r^2, -2 I r^3}, r = 1/m, from a
+// per-bin table (bins >= kLinMinBin; other pairs take the reference's chain
+// for every voxel).  One batch = one sensor and the group's eight pairs;
+// wave g stages pair g's window (from a 4-bin aligned start, 16-byte loads
+// and stores), the next batch's raw loads in flight while the current batch
+// runs.
 constexpr int kGroup = 8;
 constexpr int kGThreads = 512;
 constexpr int kGVec = 2;           // 4-bin loads per lane per batch: window <= 4 * 64 * kGVec bins
@@ -479,7 +481,7 @@ struct GroupArgs {
     float* partial;
     VolumeGeom g;
     uint32_t L, S, T;
-    float t0, dt, inv_dt, inv_dt2, dt2;
+    float t0, dt, inv_dt, dt2;
     uint32_t discard;
     float lpos[3], spos[3];
     int W;                    // window length in bins (multiple of 4): bins kmin4 .. kmin4 + W - 1
@@ -497,66 +499,62 @@ struct __align__(16) GroupPair {
     int kmin4;     // kmin rounded down to a multiple of 4 (window start)
     int fast;      // 1: linear-weight fast body, 0: exact chain, -1: no pair (l >= L)
     float lx, ly, lz, off;     // laser; path offset fl(|l - Lpos| + |s - Spos|) (0 with discard)
-    float sx, sy, sz, e;       // sensor; e = e_l + e_s
-    float cl, cs, pad[2];      // (off_l - t0) / dt + e_l, off_s / dt + e_s
+    float sx, sy, sz, pad;     // sensor
 };
 
 // Laser-side parameters of the chunk's lasers against the tile (set-up).
 struct __align__(16) GroupLaser {
     float x, y, z, offl;       // position, |l - Lpos| (0 with discard)
-    float rmin, rmax, cl, el;  // box distances, (offl - t0) / dt + e_l, e_l
+    float rmin, rmax, pad[2];  // box distances
 };
 
-// Error bound per unit of magnitude (bins per bin): with 1-ulp v_rsq the
-// laser term is within 2.5 ulp of |v - l| / dt, the sensor term (its square
-// from per-axis products scaled by 1/dt^2) within 3 ulp, their sum 0.5 ulp,
-// and the reference's fl(fl(rl + rs) / dt) 1.5 ulp: 5.5 * 2^-23 of the
-// magnitude; 6.5 leaves room for the offset terms.
-constexpr float kGroupErr = 6.5f * 1.1920929e-7f;
-
-// Laser term of one voxel, cl = (off_l - t0) / dt + e_l (identical operations
-// in the set-up and in the fix-ups).
-__device__ __forceinline__ float group_laser_term(float x2, float y2, float z2, float cl,
-                                                  float inv_dt) {
-    const float sl = dot3(x2, y2, z2);
-    return fmaf(sl * __builtin_amdgcn_rsqf(sl), inv_dt, cl);
+// Correctly rounded square root of a normal x (here x >= 1e-6: squared
+// distances of pairs at least 1 mm apart): the hardware estimate s (within
+// an ulp) and its neighbours, chosen by the signs of the residuals
+// x - s' s computed exactly by FMA; the same selection as the compiler's
+// IEEE square root, without its rescaling of tiny inputs and its
+// special-value cases.
+__device__ __forceinline__ float sqrt_rn_normal(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = __int_as_float(__float_as_int(s) - 1);
+    const float su = __int_as_float(__float_as_int(s) + 1);
+    float r = (fmaf(-sd, s, x) <= 0.f) ? sd : s;
+    r = (fmaf(-su, s, x) > 0.f) ? su : r;
+    return r;
 }
 
-// Sensor term from the per-axis squares scaled by 1 / dt^2 ((x + y), z),
-// cs = off_s / dt + e_s.
-__device__ __forceinline__ float group_sensor_term(float aq, float zq, float cs) {
-    const float s2 = aq + zq;
-    return fmaf(s2, __builtin_amdgcn_rsqf(s2), cs);
+// The reference's bin of dist from the corrected product (exact wherever the
+// host's check passed): q0 = fl(dist c), r = dist - q0 dt (exact by FMA),
+// q = fl(q0 + r c).
+__device__ __forceinline__ float group_quotient(float dist, float c, float dt) {
+    const float q0 = dist * c;
+    const float r = fmaf(-q0, dt, dist);
+    return fmaf(r, c, q0);
 }
 
-// band * 2 + (f < tw): the compare's lane bit enters as the carry of v_addc.
-__device__ __forceinline__ uint32_t band_push(uint32_t band, float f, float tw) {
-    uint32_t out;
-    asm("v_cmp_lt_f32_e32 vcc, %2, %3\n\t"
-        "v_addc_co_u32_e32 %0, vcc, %1, %1, vcc"
-        : "=v"(out)
-        : "v"(band), "v"(f), "v"(tw)
-        : "vcc");
-    return out;
-}
-
-__device__ __forceinline__ uint32_t group_fast(float (&acc)[16], const float (&rl)[16],
-                                               const float (&rs)[16], const float2* win, int base,
-                                               float tw) {
-    uint32_t band = 0;
+template <bool COFF>
+__device__ __forceinline__ void group_fast(float (&acc)[16], const float (&rl)[16],
+                                           const float (&rs)[16], const float2* win, int base,
+                                           float c, float dt, float t0, float off) {
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-        const float q = rl[v] + rs[v];
-        const int k = static_cast<int>(q);
-        band = band_push(band, __builtin_amdgcn_fractf(q), tw);
-        const float2 e = win[base + k];
-        acc[v] += fmaf(q, e.y, e.x);
+    for (int h = 0; h < 16; h += 8) {
+        float q[8];
+        float2 e[8];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            float d = rl[h + v] + rs[h + v];
+            if constexpr (COFF) d = (d - t0) + off;
+            q[v] = group_quotient(d, c, dt);
+            e[v] = win[base + static_cast<int>(q[v])];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int v = 0; v < 8; ++v) acc[h + v] += fmaf(q[v], e[v].y, e[v].x);
     }
-    return band;
 }
 
 // Distance from p to the box [lo, hi] with the hardware square root (1 ulp):
-// bounds only (window start, error bound, near test), all with margins.
+// bounds only (window start, near test), all with margins.
 __device__ __forceinline__ float box_dist_fast(const float p[3], const float lo[3],
                                                const float hi[3], bool far) {
     float s = 0.f;
@@ -605,19 +603,18 @@ __device__ __forceinline__ void group_convert(const typename GroupRaw<STORAGE>::
     }
 }
 
-template <int STORAGE>
+template <int STORAGE, bool COFF>
 __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArgs a) {
     using Raw = typename GroupRaw<STORAGE>::type;
     extern __shared__ float smem[];
     // [2][kGroup][W] window entries, [3][kGroup] pair parameters, the chunk's
-    // lasers [groups_per_chunk * kGroup] and sensors [sensors_per_part],
-    // [T] weight factors, [16][512] correction slots.
+    // lasers [groups_per_chunk * kGroup] and sensors [sensors_per_part], [T]
+    // weight factors.
     float2* winbuf = reinterpret_cast<float2*>(smem);
     GroupPair* prmbuf = reinterpret_cast<GroupPair*>(winbuf + 2 * kGroup * a.W);
     GroupLaser* lasl = reinterpret_cast<GroupLaser*>(prmbuf + 3 * kGroup);
     float4* senl = reinterpret_cast<float4*>(lasl + a.groups_per_chunk * kGroup);
     float2* tab = reinterpret_cast<float2*>(senl + a.sensors_per_part);
-    float* corrl = reinterpret_cast<float*>(tab + a.T) + threadIdx.x;
 
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -652,9 +649,8 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
     const uint32_t s_end = min(a.S, s_begin + a.sensors_per_part);
     const uint32_t ns = s_end > s_begin ? s_end - s_begin : 0u;
     const uint32_t nbatch = (g_end > g_begin ? g_end - g_begin : 0u) * ns;
+    const float c = a.inv_dt;
 
-#pragma unroll
-    for (int v = 0; v < 16; ++v) corrl[v * kGThreads] = 0.f;
     // Weight factors of bin k (m = k + 1/2): {3 r^2, -2 r^3}.
     for (uint32_t k = tid; k < a.T; k += kGThreads) {
         const double r = 1.0 / (double(k) + 0.5);
@@ -676,8 +672,7 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
                      : 0.f;
         G.rmin = box_dist_fast(lp3, lo, hi, false);
         G.rmax = box_dist_fast(lp3, lo, hi, true);
-        G.el = kGroupErr * (G.rmax + G.offl + fabsf(a.t0)) * a.inv_dt + 0.5e-6f;
-        G.cl = (G.offl - a.t0) * a.inv_dt + G.el;
+        G.pad[0] = G.pad[1] = 0.f;
         lasl[i] = G;
     }
     for (uint32_t i = tid; i < ns; i += kGThreads) {
@@ -699,17 +694,14 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
         P.sx = sp3[0];
         P.sy = sp3[1];
         P.sz = sp3[2];
+        P.pad = 0.f;
         const float offs = S4.w;
         const float rsmin = box_dist_fast(sp3, lo, hi, false);
         const float rsmax = box_dist_fast(sp3, lo, hi, true);
-        const float es = kGroupErr * (rsmax + offs) * a.inv_dt + 0.5e-6f;
-        P.cs = offs * a.inv_dt + es;
-        P.pad[0] = P.pad[1] = 0.f;
         if (l_begin + li >= a.L) {
             P.fast = -1;
             P.base = P.kmin = P.kmin4 = 0;
-            P.lx = P.ly = P.lz = P.off = P.cl = 0.f;
-            P.e = es;
+            P.lx = P.ly = P.lz = P.off = 0.f;
             *out = P;
             return;
         }
@@ -718,8 +710,6 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
         P.ly = G.y;
         P.lz = G.z;
         P.off = a.discard == 0 ? rn_add(G.offl, offs) : 0.f;
-        P.cl = G.cl;
-        P.e = G.el + es;
         const float dmin = G.rmin + rsmin - a.t0 + P.off;
         const float dmax = G.rmax + rsmax - a.t0 + P.off;
         P.kmin = int(floorf(dmin * a.inv_dt)) - 3;
@@ -755,7 +745,6 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
         const int fast = rfl(prm[wave].fast);
         if (fast < 0) return;
         const int kmin4 = rfl(prm[wave].kmin4);
-        const float e = rfl(prm[wave].e);
         float sc = 1.f;
         if constexpr (STORAGE != FBP_STORAGE_F32) {
             const uint32_t grp = g_begin + b / ns;
@@ -775,10 +764,8 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
                     if (fast == 1) {
                         const float4 t01 = *reinterpret_cast<const float4*>(tab + k0);
                         const float4 t23 = *reinterpret_cast<const float4*>(tab + k0 + 2);
-                        e01 = make_float4(v[0] * fmaf(-e, t01.y, t01.x), v[0] * t01.y,
-                                          v[1] * fmaf(-e, t01.w, t01.z), v[1] * t01.w);
-                        e23 = make_float4(v[2] * fmaf(-e, t23.y, t23.x), v[2] * t23.y,
-                                          v[3] * fmaf(-e, t23.w, t23.z), v[3] * t23.w);
+                        e01 = make_float4(v[0] * t01.x, v[0] * t01.y, v[1] * t01.z, v[1] * t01.w);
+                        e23 = make_float4(v[2] * t23.x, v[2] * t23.y, v[3] * t23.z, v[3] * t23.w);
                     } else {
                         e01 = make_float4(v[0], 0.f, v[1], 0.f);
                         e23 = make_float4(v[2], 0.f, v[3], 0.f);
@@ -804,8 +791,7 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
     Raw reg[kGVec];
     if (nbatch > 0) {
         if (tid < kGroup) params(0, tid, prmbuf + tid);
-        if (tid >= kGroup && tid < 2 * kGroup && nbatch > 1)
-            params(1, tid - kGroup, prmbuf + tid);
+        if (tid >= kGroup && tid < 2 * kGroup && nbatch > 1) params(1, tid - kGroup, prmbuf + tid);
         __syncthreads();  // parameters and the weight table
         fetch(0, prmbuf, reg);
         store(0, winbuf, prmbuf, reg);
@@ -819,11 +805,10 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
         if (has_next) fetch(b + 1, prm_next, reg);
 
         if (b % ns == 0) {
-            // New laser group: the eight laser terms of the lane's voxels.
+            // New laser group: the eight exact laser distances of the lane's voxels.
 #pragma unroll
             for (int g = 0; g < kGroup; ++g) {
                 const float lxg = rfl(prm[g].lx), lyg = rfl(prm[g].ly), lzg = rfl(prm[g].lz);
-                const float cl = rfl(prm[g].cl);
                 float XL2[4], ZL2[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
@@ -835,29 +820,26 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
                     for (int m = 0; m < 4; ++m)
-                        rl[g][i * 4 + m] = group_laser_term(XL2[i], YL2, ZL2[m], cl, a.inv_dt);
+                        rl[g][i * 4 + m] = rn_sqrt(dot3(XL2[i], YL2, ZL2[m]));
             }
         }
-        // Sensor terms of the batch's sensor.
+        // Exact sensor distances of the batch's sensor.
         float rs[16];
         const float sxp = rfl(prm[0].sx), syp = rfl(prm[0].sy), szp = rfl(prm[0].sz);
         {
-            const float cs = rfl(prm[0].cs);
-            float AQ[4], ZQ[4];
+            float XS2[4], ZS2[4];
             const float YS2 = sq(rn_sub(py, syp));
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                AQ[i] = (sq(rn_sub(px[i], sxp)) + YS2) * a.inv_dt2;
-                ZQ[i] = sq(rn_sub(pz[i], szp)) * a.inv_dt2;
+                XS2[i] = sq(rn_sub(px[i], sxp));
+                ZS2[i] = sq(rn_sub(pz[i], szp));
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int m = 0; m < 4; ++m) rs[i * 4 + m] = group_sensor_term(AQ[i], ZQ[m], cs);
+                for (int m = 0; m < 4; ++m)
+                    rs[i * 4 + m] = sqrt_rn_normal(dot3(XS2[i], YS2, ZS2[m]));
         }
-        uint32_t mw[kGroup / 2];
-#pragma unroll
-        for (int w = 0; w < kGroup / 2; ++w) mw[w] = 0;
         int fast_pairs = 0, exact_pairs = 0;
 #pragma unroll
         for (int g = 0; g < kGroup; ++g) {
@@ -866,25 +848,29 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
             exact_pairs |= (fg == 0 ? 1 : 0) << g;
         }
         if (fast_pairs == (1 << kGroup) - 1) {
-            // Common case, straight-line: all eight pairs take the fast body.
+            // Common case, straight-line: all eight pairs take the fast body,
+            // their window offsets and path offsets read up front.
+            int bases[kGroup];
+            float offs[kGroup];
 #pragma unroll
             for (int g = 0; g < kGroup; ++g) {
-                const int base = rfl(prm[g].base) + cur * per;
-                const float tw = 2.f * rfl(prm[g].e);
-                mw[g >> 1] |= group_fast(acc, rl[g], rs, winbuf, base, tw) << (16 * (g & 1));
+                bases[g] = rfl(prm[g].base) + cur * per;
+                offs[g] = rfl(prm[g].off);
             }
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g)
+                group_fast<COFF>(acc, rl[g], rs, winbuf, bases[g], c, a.dt, a.t0, offs[g]);
         } else {
 #pragma unroll
             for (int g = 0; g < kGroup; ++g) {
-                if (fast_pairs & (1 << g)) {
-                    const int base = rfl(prm[g].base) + cur * per;
-                    const float tw = 2.f * rfl(prm[g].e);
-                    mw[g >> 1] |= group_fast(acc, rl[g], rs, winbuf, base, tw) << (16 * (g & 1));
-                }
+                if (fast_pairs & (1 << g))
+                    group_fast<COFF>(acc, rl[g], rs, winbuf, rfl(prm[g].base) + cur * per, c,
+                                     a.dt, a.t0, rfl(prm[g].off));
             }
         }
         // Pairs near the wall or zero distance, or with bins below the linear
-        // range: the reference's chain for every voxel (entries {I, 0}).
+        // range: the reference's chain and weight for every voxel (entries
+        // {I, 0}).
         while (exact_pairs) {
             const int g = __builtin_ctz(exact_pairs);
             exact_pairs &= exact_pairs - 1;
@@ -912,47 +898,6 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
             if (tid < kGroup && b + 2 < nbatch)
                 params(b + 2, tid, prmbuf + ((b + 2) % 3u) * kGroup + tid);
         }
-        // Fix-ups of this batch's band voxels in a fixed order (pair 0's
-        // last voxel first): bits 31..16 of word w are pair 2w + 1, 15..0
-        // pair 2w; within a pair bit 15 - v is voxel v.
-        uint64_t blo = uint64_t(mw[0]) | (uint64_t(mw[1]) << 32);
-        uint64_t bhi = uint64_t(mw[2]) | (uint64_t(mw[3]) << 32);
-        while (blo | bhi) {
-            int bit;
-            if (blo) {
-                bit = __builtin_ctzll(blo);
-                blo &= blo - 1u;
-            } else {
-                bit = 64 + __builtin_ctzll(bhi);
-                bhi &= bhi - 1u;
-            }
-            const int g = bit >> 4;
-            const int v = 15 - (bit & 15);
-            const int i = v >> 2, m = v & 3;
-            const GroupPair& P = prm[g];
-            const float x = pick4f(px, i), z = pick4f(pz, m);
-            const float ql = group_laser_term(sq(rn_sub(x, P.lx)), sq(rn_sub(py, P.ly)),
-                                              sq(rn_sub(z, P.lz)), P.cl, a.inv_dt);
-            const float aq = (sq(rn_sub(x, P.sx)) + sq(rn_sub(py, P.sy))) * a.inv_dt2;
-            const float zq = sq(rn_sub(z, P.sz)) * a.inv_dt2;
-            const float q = ql + group_sensor_term(aq, zq, P.cs);
-            const int kf = static_cast<int>(q);
-            ExhaustiveArgs ea{};
-            ea.t0 = a.t0;
-            ea.discard = a.discard;
-            const float de =
-                exact_dist(x, py, z, LaserPos{P.lx, P.ly, P.lz}, P.sx, P.sy, P.sz, ea, P.off);
-            const int ke = bin_of(de, a.dt);
-            if (ke != kf) {
-                // ke is kf - 1 (the reference's quotient lies within 2 e
-                // below q); the clamp keeps the read inside the pair's window.
-                const int base = P.base + cur * per;
-                const int kc = min(max(ke, P.kmin), P.kmin + a.W - 5);
-                const float2 ef = winbuf[base + kf];
-                const float2 ee = winbuf[base + kc];
-                corrl[v * kGThreads] += fmaf(q, ee.y, ee.x) - fmaf(q, ef.y, ef.x);
-            }
-        }
         __syncthreads();
     }
 
@@ -962,8 +907,7 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int m = 0; m < 4; ++m)
-            out[size_t((i * 8 + wave) * 4 + m) * 64 + lane] =
-                acc[i * 4 + m] + corrl[(i * 4 + m) * kGThreads];
+            out[size_t((i * 8 + wave) * 4 + m) * 64 + lane] = acc[i * 4 + m];
 }
 
 namespace {
@@ -1032,15 +976,42 @@ static bool linear_weights_apply(const fbp_reconstruction_info& in) {
     return kmin >= double(kLinMinBin);
 }
 
+// Whether the group kernel's corrected product truncates like the
+// reference's fl(dist / dt) for every dist the bins 1 .. T + 1 can meet.  Both
+// quotients lie within an ulp of dist / dt (the product q0 within about one,
+// the FMA-corrected one within about half), so they can only truncate
+// differently when dist / dt is within an ulp or so of an integer k, i.e.
+// dist within a few ulps of k dt; every float within 16 ulps of each k dt is
+// checked with the same float operations the kernel uses.
+static bool group_quotient_exact(float dt, uint32_t T) {
+    const float c = 1.0f / dt;
+    for (uint32_t k = 1; k <= T + 1; ++k) {
+        float x = float(double(k) * double(dt));
+        for (int i = 0; i < 16; ++i) x = std::nextafter(x, 0.0f);
+        for (int i = 0; i <= 32; ++i, x = std::nextafter(x, INFINITY)) {
+            const float q0 = x * c;
+            const float q = std::fma(std::fma(-q0, dt, x), c, q0);
+            if (static_cast<int>(q) != static_cast<int>(x / dt)) return false;
+        }
+    }
+    return true;
+}
+
+template <int STORAGE>
+static void (*pick_group_s(bool coff))(GroupArgs) {
+    return coff ? exhaustive_group_kernel<STORAGE, true> : exhaustive_group_kernel<STORAGE, false>;
+}
+
 // Laser-group kernel for linear-weight captures with at least kGroup lasers
-// and 64 sensors whose window fits the prefetch registers; *used = false
-// (nothing launched) otherwise.  Chunks are (laser-group range, sensor
-// range) blocks, reduced in a fixed order like the per-pair kernel's.
+// and 64 sensors whose window fits the staging registers and whose bin
+// quotient check passes; *used = false (nothing launched) otherwise.  Chunks
+// are (laser-group range, sensor range) blocks, reduced in a fixed order like
+// the per-pair kernel's.
 static fbp_status exhaustive_group_launch(fbp_context ctx, const fbp_reconstruction_info* info,
                                           const fbp_transient* tr, const fbp_vec3* laser_targets,
                                           const fbp_vec3* sensor_targets, float* volume,
                                           void* stream, float* fixed_partial, size_t fixed_floats,
-                                          bool* used) {
+                                          bool coff, bool* used) {
     *used = false;
     GroupArgs a{};
     a.g = make_geom(*info, 4, 4, 8);
@@ -1055,6 +1026,13 @@ static fbp_status exhaustive_group_launch(fbp_context ctx, const fbp_reconstruct
     if (a.L < uint32_t(kGroup) || a.S < 64u || a.T > 8192u || (a.T & 3u) != 0u ||
         a.W > 4 * 64 * kGVec || (reinterpret_cast<uintptr_t>(tr->data) % align) != 0u)
         return FBP_OK;
+    // The quotient check depends on (dt, T) only: kept for the next call.
+    if (ctx->group_quot_dt != info->time_step || ctx->group_quot_T != a.T) {
+        ctx->group_quot_ok = group_quotient_exact(info->time_step, a.T) ? 1 : 0;
+        ctx->group_quot_dt = info->time_step;
+        ctx->group_quot_T = a.T;
+    }
+    if (!ctx->group_quot_ok) return FBP_OK;
 
     a.data = tr->data;
     a.scales = tr->scales;
@@ -1063,7 +1041,6 @@ static fbp_status exhaustive_group_launch(fbp_context ctx, const fbp_reconstruct
     a.t0 = info->time_offset;
     a.dt = info->time_step;
     a.inv_dt = 1.0f / info->time_step;
-    a.inv_dt2 = a.inv_dt * a.inv_dt;
     a.dt2 = info->time_step * info->time_step;
     a.discard = info->discard_first_last_bounces;
     a.lpos[0] = info->laser_position.x;
@@ -1092,11 +1069,11 @@ static fbp_status exhaustive_group_launch(fbp_context ctx, const fbp_reconstruct
     parts = std::min<uint64_t>(parts, std::max<uint64_t>(1, a.S / 64));
     a.sensors_per_part = uint32_t((a.S + parts - 1) / parts);
     // LDS: windows, pair parameters, the chunk's lasers and sensors, weight
-    // factors, correction slots.  Sensor ranges shrink (more parts) to fit.
+    // factors.  Sensor ranges shrink (more parts) to fit.
     auto lds_bytes = [&](uint32_t spp) {
         return size_t(2) * kGroup * a.W * sizeof(float2) + 3 * kGroup * sizeof(GroupPair) +
                size_t(a.groups_per_chunk) * kGroup * sizeof(GroupLaser) + size_t(spp) * sizeof(float4) +
-               size_t(a.T) * sizeof(float2) + size_t(16) * kGThreads * sizeof(float);
+               size_t(a.T) * sizeof(float2);
     };
     while (a.sensors_per_part > 64 && lds_bytes(a.sensors_per_part) > size_t(ctx->max_lds_per_block))
         a.sensors_per_part = (a.sensors_per_part + 1) / 2;
@@ -1113,9 +1090,9 @@ static fbp_status exhaustive_group_launch(fbp_context ctx, const fbp_reconstruct
     }
     a.num_blocks = uint32_t(chunks * a.g.num_tiles);
 
-    void (*kern)(GroupArgs) = tr->storage == FBP_STORAGE_F32   ? exhaustive_group_kernel<FBP_STORAGE_F32>
-                              : tr->storage == FBP_STORAGE_F16 ? exhaustive_group_kernel<FBP_STORAGE_F16>
-                                                               : exhaustive_group_kernel<FBP_STORAGE_BF16>;
+    void (*kern)(GroupArgs) = tr->storage == FBP_STORAGE_F32   ? pick_group_s<FBP_STORAGE_F32>(coff)
+                              : tr->storage == FBP_STORAGE_F16 ? pick_group_s<FBP_STORAGE_F16>(coff)
+                                                               : pick_group_s<FBP_STORAGE_BF16>(coff);
     hipStream_t s = as_stream(stream);
     FBP_CHECK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
@@ -1156,7 +1133,7 @@ static fbp_status exhaustive_voxel_impl(fbp_context ctx, const fbp_reconstructio
     if (link && ctx->exhaustive_group_kernel) {
         bool used = false;
         FBP_CHECK(exhaustive_group_launch(ctx, info, tr, laser_targets, sensor_targets, volume,
-                                          stream, fixed_partial, fixed_floats, &used));
+                                          stream, fixed_partial, fixed_floats, coff, &used));
         if (used) {
             ctx->last_exhaustive_group = 1;
             return FBP_OK;
