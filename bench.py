@@ -40,7 +40,7 @@ VALU_LANE_SLOTS = 39.3e12      # 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (non-pac
 # v_fma / v_rsq throughput ratio on this GPU).
 GENERAL_SLOTS = 14.9           # confocal_voxel_kernel
 EXH_SLOTS = 10.65              # exhaustive_group_kernel (profiles/exhaustive_group_c3_bench_counters.md)
-DEPTH_EXH_SLOTS = 17.4         # depth_exhaustive_window_kernel fast path, ISA count
+DEPTH_EXH_SLOTS = 12.36        # depth_group_kernel (profiles/exhaustive_group_c3_bench_counters.md)
 TRAFFIC_PROFILE = ROOT / "profiles" / "confocal_128cube_traffic.json"
 
 
@@ -822,13 +822,23 @@ def exhaustive_leg(ctx, args, reps=3):
     act = torch.empty((L, 200), dtype=torch.float32, device=dev)
     t_depth = timed(lambda: ctx.backproject_depth(info, I, lasers, depths, sensor_targets=sensors,
                                                   out=act))
+    depth_kernel = ("depth_group_kernel (laser groups of 8)" if ctx.last_exhaustive_laser_groups()
+                    else "depth_exhaustive_window_kernel")
+    ctx.set_exhaustive_laser_groups(False)
+    act_pp = torch.empty_like(act)
+    t_depth_pair = timed(lambda: ctx.backproject_depth(info, I, lasers, depths,
+                                                       sensor_targets=sensors, out=act_pp))
+    ctx.set_exhaustive_laser_groups(True)
+    depth_pair_rel = float(torch.linalg.vector_norm((act_pp - act).double())
+                           / torch.linalg.vector_norm(act.double()))
+    del act_pp
     depth_updates = L * 200 * L * S
     # backprojectExhaustive per (pixel, depth, pair): |p - l| (3 sub, 3 mul,
     # 2 add, sqrt) + |p - s| (cached per sensor) - t0, / dt, + sample = 13.
     depth_tflops = 13 * depth_updates / (t_depth * 1e-3) / 1e12
-    # Fast path per update (counted in the kernel's ISA): 16 VALU + v_rsq
-    # (2.37 slots at the measured transcendental rate) = 17.4 issue slots;
-    # boundary points add the IEEE chain.
+    # Issue slots per counted pair-update of the depth-plane group kernel from
+    # its counters (12.1 VALU, 0.16 transcendental; 200 of its 256 depth slots
+    # per tile hold planes).
     depth_slot_bound = VALU_LANE_SLOTS / DEPTH_EXH_SLOTS
     depth_check = None
     cpu = None
@@ -875,7 +885,10 @@ def exhaustive_leg(ctx, args, reps=3):
                                      "frac": updates / (bp * 1e-3) / slot_bound},
            "depth_planes": {"workload": "200 depth planes at focus +- 0.1 over the 16x16 laser "
                                         "pixels, all 1,048,576 pairs",
-                            "ms": t_depth, "pair_updates": depth_updates,
+                            "kernel": depth_kernel, "ms": t_depth,
+                            "staged_window_kernel": {"ms": t_depth_pair,
+                                                     "rel_l2_vs_group_kernel": depth_pair_rel},
+                            "pair_updates": depth_updates,
                             "roofline": {"bound": "valu", "achieved": depth_tflops,
                                          "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                                          "frac": depth_tflops / PEAK_FP32_TFLOPS,

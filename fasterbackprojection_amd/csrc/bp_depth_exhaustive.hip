@@ -26,11 +26,13 @@
 // come within millimetres of the laser or sensor point, or of zero distance,
 // take the IEEE chain for every point.
 #include "bp_common.h"
+#include "group_common.h"
 
 namespace fbp {
 
 namespace {
 
+constexpr int kGVecD = 2;   // 4-bin loads per lane per batch in the group form
 constexpr int kDxThreads = 256;
 constexpr int kDxOut = 8;                         // outputs per thread (independent chains)
 constexpr int kDxBatch = 16;                      // pairs per staged batch (at most)
@@ -362,6 +364,500 @@ __global__ void depth_reduce_kernel(const float* __restrict__ partial, uint32_t 
     act[i] = float(s);
 }
 
+// ---------------------------------------------------------------------------
+// Laser-group form of the exhaustive depth planes (the laser-group voxel
+// kernel's scheme on the depth-plane points): for a wall normal along y the
+// point p = laser[c] + n * depth[d] keeps the pixel's x and z, so a lane holds
+// one pixel and 16 depths (x, z and 16 y values).  A workgroup of eight waves
+// owns a tile of 32 pixels (consecutive in a Morton order of the pixels on
+// the wall, so a tile is a compact block) x 256 depth slots and a range of
+// (group of 8 lasers, sensor) batches.  Each lane keeps the eight exact laser
+// distances of its 16 points while the group is unchanged and computes its
+// 16 exact sensor distances once per sensor; the update is the reference's
+// dist = fl(rl + rs) (- t0, + off), the corrected quotient (exact at every
+// bin boundary by the host's check) and one LDS read of the staged sample:
+// 7 VALU.  Pairs whose points may come within a millimetre of the laser or
+// sensor point, or of zero distance, take the reference's chain per point;
+// a tile whose bin window would exceed the staging capacity reads its samples
+// from the transient directly.
+constexpr int kDgPix = 32;                 // pixels per tile
+constexpr int kDgDep = 256;                // depth slots per tile (16 per lane)
+constexpr int kDgWin = 4 * 64 * kGVecD;    // staged window capacity (bins) per pair
+
+struct DepthGroupArgs {
+    const float* data;  // I[l][s][t], F32
+    const fbp_vec3* laser;
+    const fbp_vec3* sensor;
+    const double* depths;
+    const uint32_t* perm;  // pixels in Morton order on the wall
+    float* partial;        // [chunk][L * D]
+    uint32_t L, S, T, D;
+    float t0, dt, inv_dt, ny;
+    uint32_t discard;
+    float lpos[3], spos[3];
+    uint32_t groups, groups_per_chunk, sensor_parts, sensors_per_part;
+    uint32_t ptiles, num_tiles, num_blocks;
+};
+
+struct __align__(16) DgPair {
+    int base;      // g * kDgWin - kmin4
+    int kmin;
+    int kmin4;
+    int fast;      // 1: fast body, 0: the reference's chain per point, -1: no pair
+    float lx, ly, lz, off;
+    float sx, sy, sz, pad;
+};
+
+struct __align__(16) DgLaser {
+    float x, y, z, offl;
+    float rmin, rmax, pad[2];
+};
+
+// Pixels ordered along a Morton curve of their (x, z) position on the wall,
+// quantised to 16 bits over the pixels' bounding box; ties by index.  One
+// workgroup, L <= 4096.
+__global__ __launch_bounds__(1024) void depth_pixel_order_kernel(const fbp_vec3* laser, uint32_t L,
+                                                                 uint32_t* perm) {
+    __shared__ uint64_t key[4096];
+    __shared__ float red[4][16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    float b[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};
+    for (uint32_t i = tid; i < L; i += 1024) {
+        b[0] = fminf(b[0], laser[i].x);
+        b[1] = fmaxf(b[1], laser[i].x);
+        b[2] = fminf(b[2], laser[i].z);
+        b[3] = fmaxf(b[3], laser[i].z);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        b[0] = fminf(b[0], __shfl_xor(b[0], o));
+        b[1] = fmaxf(b[1], __shfl_xor(b[1], o));
+        b[2] = fminf(b[2], __shfl_xor(b[2], o));
+        b[3] = fmaxf(b[3], __shfl_xor(b[3], o));
+    }
+    if (lane == 0)
+        for (int k = 0; k < 4; ++k) red[k][wave] = b[k];
+    __syncthreads();
+    for (int w = 0; w < 16; ++w) {
+        b[0] = fminf(b[0], red[0][w]);
+        b[1] = fmaxf(b[1], red[1][w]);
+        b[2] = fminf(b[2], red[2][w]);
+        b[3] = fmaxf(b[3], red[3][w]);
+    }
+    const float sx = b[1] > b[0] ? 65535.f / (b[1] - b[0]) : 0.f;
+    const float sz = b[3] > b[2] ? 65535.f / (b[3] - b[2]) : 0.f;
+    uint32_t N = 1;
+    while (N < L) N <<= 1;
+    for (uint32_t i = tid; i < N; i += 1024) {
+        uint64_t k = ~0ull;
+        if (i < L) {
+            const uint32_t qx = uint32_t(fminf(fmaxf((laser[i].x - b[0]) * sx, 0.f), 65535.f));
+            const uint32_t qz = uint32_t(fminf(fmaxf((laser[i].z - b[2]) * sz, 0.f), 65535.f));
+            uint32_t code = 0;
+            for (int bit = 0; bit < 16; ++bit)
+                code |= (((qx >> bit) & 1u) << (2 * bit)) | (((qz >> bit) & 1u) << (2 * bit + 1));
+            k = (uint64_t(code) << 32) | i;
+        }
+        key[i] = k;
+    }
+    __syncthreads();
+    // Bitonic sort of key[0 .. N).
+    for (uint32_t size = 2; size <= N; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t i = tid; i < N; i += 1024) {
+                const uint32_t j = i ^ stride;
+                if (j > i) {
+                    const bool up = (i & size) == 0;
+                    const uint64_t x = key[i], y = key[j];
+                    if ((x > y) == up) {
+                        key[i] = y;
+                        key[j] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t i = tid; i < L; i += 1024) perm[i] = uint32_t(key[i]);
+}
+
+template <bool COFF>
+__global__ __launch_bounds__(512, 1) void depth_group_kernel(DepthGroupArgs a) {
+    extern __shared__ float smem[];
+    // [2][kGroup][kDgWin] staged samples, [3][kGroup] pair parameters, the
+    // chunk's lasers and sensors, box reduction scratch.
+    float* winbuf = smem;
+    DgPair* prmbuf = reinterpret_cast<DgPair*>(winbuf + 2 * kGroup * kDgWin);
+    DgLaser* lasl = reinterpret_cast<DgLaser*>(prmbuf + 3 * kGroup);
+    float4* senl = reinterpret_cast<float4*>(lasl + a.groups_per_chunk * kGroup);
+    float* red = reinterpret_cast<float*>(senl + a.sensors_per_part);  // [6][8]
+
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+    const uint32_t nb8 = a.num_blocks / 8u;
+    const uint32_t bid = blockIdx.x < nb8 * 8u ? (blockIdx.x % 8u) * nb8 + blockIdx.x / 8u
+                                               : blockIdx.x;
+    const uint32_t tile = bid % a.num_tiles, chunk = bid / a.num_tiles;
+    const uint32_t tp = tile % a.ptiles, td = tile / a.ptiles;
+
+    // The lane's pixel and 16 depths.
+    const uint32_t pi = tp * kDgPix + uint32_t(tid % kDgPix);
+    const bool pix_ok = pi < a.L;
+    const uint32_t c = a.perm[pix_ok ? pi : 0u];
+    const uint32_t d0 = td * kDgDep + uint32_t(tid / kDgPix) * 16u;
+    const float px = a.laser[c].x, pz = a.laser[c].z, lcy = a.laser[c].y;
+    float py[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const uint32_t d = min(d0 + uint32_t(v), a.D - 1);
+        py[v] = lcy + a.ny * static_cast<float>(a.depths[d]);
+    }
+    // Box of the tile's points.
+    float b[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
+    if (pix_ok && d0 < a.D) {
+        b[0] = b[1] = px;
+        b[4] = b[5] = pz;
+#pragma unroll
+        for (int v = 0; v < 16; ++v)
+            if (d0 + uint32_t(v) < a.D) {
+                b[2] = fminf(b[2], py[v]);
+                b[3] = fmaxf(b[3], py[v]);
+            }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int k = 0; k < 6; k += 2) {
+            b[k] = fminf(b[k], __shfl_xor(b[k], o));
+            b[k + 1] = fmaxf(b[k + 1], __shfl_xor(b[k + 1], o));
+        }
+    if (lane == 0)
+        for (int k = 0; k < 6; ++k) red[k * 8 + wave] = b[k];
+    __syncthreads();
+    float lo[3], hi[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        lo[ax] = INFINITY;
+        hi[ax] = -INFINITY;
+        for (int w = 0; w < 8; ++w) {
+            lo[ax] = fminf(lo[ax], red[(2 * ax) * 8 + w]);
+            hi[ax] = fmaxf(hi[ax], red[(2 * ax + 1) * 8 + w]);
+        }
+    }
+    const float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+    // Bins the tile can reach for one pair: both legs vary by at most the box
+    // diagonal; + 3 below kmin, aligned start and margin.
+    const float wt = ceilf(2.f * sqrtf(ex * ex + ey * ey + ez * ez) * a.inv_dt) + 16.f;
+    const bool fits = wt <= float(kDgWin);
+
+    const uint32_t lgc = chunk / a.sensor_parts, sp = chunk % a.sensor_parts;
+    const uint32_t g_begin = lgc * a.groups_per_chunk;
+    const uint32_t g_end = min(a.groups, g_begin + a.groups_per_chunk);
+    const uint32_t s_begin = sp * a.sensors_per_part;
+    const uint32_t s_end = min(a.S, s_begin + a.sensors_per_part);
+    const uint32_t ns = s_end > s_begin ? s_end - s_begin : 0u;
+    const uint32_t nbatch = (g_end > g_begin ? g_end - g_begin : 0u) * ns;
+    const float cq = a.inv_dt;
+
+    const uint32_t l_begin = g_begin * kGroup;
+    const uint32_t nl = min(a.L, g_end * kGroup) - min(a.L, l_begin);
+    for (uint32_t i = tid; i < nl; i += 512) {
+        const uint32_t l = l_begin + i;
+        DgLaser G;
+        const float lp3[3] = {a.laser[l].x, a.laser[l].y, a.laser[l].z};
+        G.x = lp3[0];
+        G.y = lp3[1];
+        G.z = lp3[2];
+        G.offl = a.discard == 0
+                     ? distance3(lp3[0], lp3[1], lp3[2], a.lpos[0], a.lpos[1], a.lpos[2])
+                     : 0.f;
+        G.rmin = box_dist_fast(lp3, lo, hi, false);
+        G.rmax = box_dist_fast(lp3, lo, hi, true);
+        G.pad[0] = G.pad[1] = 0.f;
+        lasl[i] = G;
+    }
+    for (uint32_t i = tid; i < ns; i += 512) {
+        const uint32_t s = s_begin + i;
+        const float x = a.sensor[s].x, y = a.sensor[s].y, z = a.sensor[s].z;
+        const float offs =
+            a.discard == 0 ? distance3(x, y, z, a.spos[0], a.spos[1], a.spos[2]) : 0.f;
+        senl[i] = make_float4(x, y, z, offs);
+    }
+    __syncthreads();
+
+    auto params = [&](uint32_t b, int g, DgPair* out) {
+        DgPair P;
+        const uint32_t li = (b / ns) * kGroup + uint32_t(g);
+        const float4 S4 = senl[b % ns];
+        const float sp3[3] = {S4.x, S4.y, S4.z};
+        P.sx = sp3[0];
+        P.sy = sp3[1];
+        P.sz = sp3[2];
+        P.pad = 0.f;
+        const float offs = S4.w;
+        const float rsmin = box_dist_fast(sp3, lo, hi, false);
+        const float rsmax = box_dist_fast(sp3, lo, hi, true);
+        if (l_begin + li >= a.L) {
+            P.fast = -1;
+            P.base = P.kmin = P.kmin4 = 0;
+            P.lx = P.ly = P.lz = P.off = 0.f;
+            *out = P;
+            return;
+        }
+        const DgLaser G = lasl[li];
+        P.lx = G.x;
+        P.ly = G.y;
+        P.lz = G.z;
+        P.off = a.discard == 0 ? rn_add(G.offl, offs) : 0.f;
+        const float dmin = G.rmin + rsmin - a.t0 + P.off;
+        const float dmax = G.rmax + rsmax - a.t0 + P.off;
+        P.kmin = int(floorf(dmin * a.inv_dt)) - 3;
+        P.kmin4 = P.kmin & ~3;
+        P.base = g * kDgWin - P.kmin4;
+        const bool near = dmin < 0.0045f || G.rmin < 1.0e-3f || rsmin < 1.0e-3f;
+        (void)dmax;
+        P.fast = (fits && !near) ? 1 : 0;
+        *out = P;
+    };
+    // Raw samples of batch b: wave g fetches pair g's bins kmin4 + 4 j .. + 3.
+    auto fetch = [&](uint32_t b, const DgPair* prm, float4 (&reg)[kGVecD]) {
+        const int fast = rfl(prm[wave].fast);
+        const int kmin4 = rfl(prm[wave].kmin4);
+        const uint32_t grp = g_begin + b / ns;
+        const uint32_t s = s_begin + b % ns;
+        const uint64_t pair = uint64_t(grp * kGroup + uint32_t(wave)) * a.S + s;
+#pragma unroll
+        for (int i = 0; i < kGVecD; ++i) {
+            const int j = lane + 64 * i;
+            const int k0 = kmin4 + 4 * j;
+            const bool ok = fits && fast >= 0 && k0 >= 0 && k0 < int(a.T);
+            reg[i] = *reinterpret_cast<const float4*>(a.data + (ok ? pair * a.T + uint32_t(k0) : 0));
+        }
+    };
+    auto store = [&](float* win, const DgPair* prm, const float4 (&reg)[kGVecD]) {
+        const int fast = rfl(prm[wave].fast);
+        if (!fits || fast < 0) return;
+        const int kmin4 = rfl(prm[wave].kmin4);
+#pragma unroll
+        for (int i = 0; i < kGVecD; ++i) {
+            const int j = lane + 64 * i;
+            const int k0 = kmin4 + 4 * j;
+            const bool ok = k0 >= 0 && k0 < int(a.T);
+            const float4 r = reg[i];
+            float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (ok)
+                e = make_float4(r.x > kEps ? r.x : 0.f, r.y > kEps ? r.y : 0.f,
+                                r.z > kEps ? r.z : 0.f, r.w > kEps ? r.w : 0.f);
+            *reinterpret_cast<float4*>(win + wave * kDgWin + 4 * j) = e;
+        }
+    };
+
+    float acc[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+    float rl[kGroup][16];
+#pragma unroll
+    for (int g = 0; g < kGroup; ++g)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) rl[g][v] = 0.f;
+
+    const int per = kGroup * kDgWin;
+    float4 reg[kGVecD];
+    if (nbatch > 0) {
+        if (tid < kGroup) params(0, tid, prmbuf + tid);
+        if (tid >= kGroup && tid < 2 * kGroup && nbatch > 1) params(1, tid - kGroup, prmbuf + tid);
+        __syncthreads();
+        fetch(0, prmbuf, reg);
+        store(winbuf, prmbuf, reg);
+        __syncthreads();
+    }
+    for (uint32_t b = 0; b < nbatch; ++b) {
+        const int cur = int(b & 1u);
+        const DgPair* prm = prmbuf + (b % 3u) * kGroup;
+        const bool has_next = b + 1 < nbatch;
+        const DgPair* prm_next = prmbuf + ((b + 1) % 3u) * kGroup;
+        if (has_next) fetch(b + 1, prm_next, reg);
+        if (b % ns == 0) {
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) {
+                const float XL2 = sq(rn_sub(px, rfl(prm[g].lx)));
+                const float ZL2 = sq(rn_sub(pz, rfl(prm[g].lz)));
+                const float lyg = rfl(prm[g].ly);
+#pragma unroll
+                for (int v = 0; v < 16; ++v)
+                    rl[g][v] = rn_sqrt(dot3(XL2, sq(rn_sub(py[v], lyg)), ZL2));
+            }
+        }
+        float rs[16];
+        const float sxp = rfl(prm[0].sx), syp = rfl(prm[0].sy), szp = rfl(prm[0].sz);
+        {
+            const float XS2 = sq(rn_sub(px, sxp)), ZS2 = sq(rn_sub(pz, szp));
+#pragma unroll
+            for (int v = 0; v < 16; ++v)
+                rs[v] = sqrt_rn_normal(dot3(XS2, sq(rn_sub(py[v], syp)), ZS2));
+        }
+        int fast_pairs = 0, exact_pairs = 0;
+#pragma unroll
+        for (int g = 0; g < kGroup; ++g) {
+            const int fg = rfl(prm[g].fast);
+            fast_pairs |= (fg == 1 ? 1 : 0) << g;
+            exact_pairs |= (fg == 0 ? 1 : 0) << g;
+        }
+        auto fast_body = [&](int g, int base, float off) {
+#pragma unroll
+            for (int h = 0; h < 16; h += 8) {
+                float e[8];
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+                    float d = rl[g][h + v] + rs[h + v];
+                    if constexpr (COFF) d = (d - a.t0) + off;
+                    e[v] = winbuf[base + static_cast<int>(group_quotient(d, cq, a.dt))];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int v = 0; v < 8; ++v) acc[h + v] += e[v];
+            }
+        };
+        if (fast_pairs == (1 << kGroup) - 1) {
+            int bases[kGroup];
+            float offs[kGroup];
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) {
+                bases[g] = rfl(prm[g].base) + cur * per;
+                offs[g] = rfl(prm[g].off);
+            }
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) fast_body(g, bases[g], offs[g]);
+        } else {
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g)
+                if (fast_pairs & (1 << g)) fast_body(g, rfl(prm[g].base) + cur * per, rfl(prm[g].off));
+        }
+        // The reference's chain per point (samples from the window when the
+        // tile's window fits, else from the transient).
+        while (exact_pairs) {
+            const int g = __builtin_ctz(exact_pairs);
+            exact_pairs &= exact_pairs - 1;
+            const DgPair& P = prm[g];
+            const float lx = rfl(P.lx), ly = rfl(P.ly), lz = rfl(P.lz), off = rfl(P.off);
+            const int base = rfl(P.base) + cur * per;
+            const uint64_t pair = uint64_t((g_begin + b / ns) * kGroup + uint32_t(g)) * a.S +
+                                  (s_begin + b % ns);
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const float sl = dot3(sq(rn_sub(px, lx)), sq(rn_sub(py[v], ly)), sq(rn_sub(pz, lz)));
+                const float ss =
+                    dot3(sq(rn_sub(px, sxp)), sq(rn_sub(py[v], syp)), sq(rn_sub(pz, szp)));
+                const float de = exhaustive_dist(sl, ss, a.t0, a.discard, off);
+                const int k = bin_of(de, a.dt);
+                if (k >= 0 && k < int(a.T)) {
+                    float I;
+                    if (fits) {
+                        I = winbuf[base + k];
+                    } else {
+                        I = a.data[pair * a.T + uint32_t(k)];
+                        I = I > kEps ? I : 0.f;
+                    }
+                    acc[v] += I;
+                }
+            }
+        }
+        if (has_next) {
+            store(winbuf + (cur ^ 1) * per, prm_next, reg);
+            if (tid < kGroup && b + 2 < nbatch)
+                params(b + 2, tid, prmbuf + ((b + 2) % 3u) * kGroup + tid);
+        }
+        __syncthreads();
+    }
+    if (pix_ok) {
+        float* out = a.partial + size_t(chunk) * a.L * a.D + size_t(c) * a.D;
+#pragma unroll
+        for (int v = 0; v < 16; ++v)
+            if (d0 + uint32_t(v) < a.D) out[d0 + v] = acc[v];
+    }
+}
+
+// Laser-group form when the capture qualifies (wall normal along y, at
+// least 8 lasers and 64 sensors, at most 4096 pixels, T a multiple of 4,
+// 16-byte aligned samples, the bin quotient check passed); *used = false
+// (nothing launched) otherwise.
+fbp_status depth_group_launch(fbp_context ctx, const fbp_reconstruction_info* info,
+                              const fbp_transient* tr, const fbp_vec3* laser_targets,
+                              const fbp_vec3* sensor_targets, const double* depths,
+                              uint32_t num_depths, float* activation, hipStream_t s, bool* used) {
+    *used = false;
+    DepthGroupArgs a{};
+    a.L = info->num_laser_targets;
+    a.S = info->num_sensor_targets;
+    a.T = info->num_time_bins;
+    a.D = num_depths;
+    if (info->relay_wall_normal.x != 0.f || info->relay_wall_normal.z != 0.f || a.L < uint32_t(kGroup) ||
+        a.L > 4096u || a.S < 64u || (a.T & 3u) != 0u ||
+        (reinterpret_cast<uintptr_t>(tr->data) % 16u) != 0u)
+        return FBP_OK;
+    if (!group_quotient_cached(ctx, info->time_step, a.T)) return FBP_OK;
+    a.data = static_cast<const float*>(tr->data);
+    a.laser = laser_targets;
+    a.sensor = sensor_targets;
+    a.depths = depths;
+    a.t0 = info->time_offset;
+    a.dt = info->time_step;
+    a.inv_dt = 1.0f / info->time_step;
+    a.ny = info->relay_wall_normal.y;
+    a.discard = info->discard_first_last_bounces;
+    a.lpos[0] = info->laser_position.x;
+    a.lpos[1] = info->laser_position.y;
+    a.lpos[2] = info->laser_position.z;
+    a.spos[0] = info->sensor_position.x;
+    a.spos[1] = info->sensor_position.y;
+    a.spos[2] = info->sensor_position.z;
+    a.groups = (a.L + kGroup - 1) / kGroup;
+    a.ptiles = (a.L + kDgPix - 1) / kDgPix;
+    a.num_tiles = a.ptiles * ((a.D + kDgDep - 1) / kDgDep);
+    const uint64_t cus = uint64_t(ctx->num_cus > 0 ? ctx->num_cus : 256);
+    const uint64_t want = std::max<uint64_t>(1, (4 * cus + a.num_tiles - 1) / a.num_tiles);
+    uint64_t lgc = std::min<uint64_t>(a.groups, want);
+    a.groups_per_chunk = uint32_t((a.groups + lgc - 1) / lgc);
+    lgc = (a.groups + a.groups_per_chunk - 1) / a.groups_per_chunk;
+    uint64_t parts = std::max<uint64_t>(1, want / lgc);
+    parts = std::min<uint64_t>(parts, std::max<uint64_t>(1, a.S / 64));
+    a.sensors_per_part = uint32_t((a.S + parts - 1) / parts);
+    auto lds_bytes = [&](uint32_t spp) {
+        return size_t(2) * kGroup * kDgWin * sizeof(float) + 3 * kGroup * sizeof(DgPair) +
+               size_t(a.groups_per_chunk) * kGroup * sizeof(DgLaser) + size_t(spp) * sizeof(float4) +
+               48 * sizeof(float);
+    };
+    while (a.sensors_per_part > 64 && lds_bytes(a.sensors_per_part) > size_t(ctx->max_lds_per_block))
+        a.sensors_per_part = (a.sensors_per_part + 1) / 2;
+    const size_t shmem = lds_bytes(a.sensors_per_part);
+    if (shmem > size_t(ctx->max_lds_per_block)) return FBP_OK;
+    a.sensor_parts = uint32_t((a.S + a.sensors_per_part - 1) / a.sensors_per_part);
+    const uint64_t chunks = lgc * a.sensor_parts;
+    const size_t n = size_t(a.L) * a.D;
+    if (chunks * a.num_tiles > 0x7fffffffull) return FBP_OK;
+    // Partial sums [chunk][L * D], then the pixel order.
+    const size_t part_bytes = (size_t(chunks) * n * sizeof(float) + 255) & ~size_t(255);
+    FBP_CHECK(ensure_scratch(ctx, part_bytes + size_t(a.L) * sizeof(uint32_t)));
+    a.partial = static_cast<float*>(ctx->scratch);
+    uint32_t* perm = reinterpret_cast<uint32_t*>(static_cast<char*>(ctx->scratch) + part_bytes);
+    a.perm = perm;
+    a.num_blocks = uint32_t(chunks * a.num_tiles);
+    hipLaunchKernelGGL(depth_pixel_order_kernel, dim3(1), dim3(1024), 0, s, laser_targets, a.L, perm);
+    FBP_CHECK_HIP(ctx, hipGetLastError());
+    const bool coff = !(a.t0 == 0.f && a.discard != 0);
+    void (*kern)(DepthGroupArgs) = coff ? depth_group_kernel<true> : depth_group_kernel<false>;
+    FBP_CHECK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
+    hipLaunchKernelGGL(kern, dim3(a.num_blocks), dim3(512), shmem, s, a);
+    FBP_CHECK_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL(depth_reduce_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s,
+                       a.partial, uint32_t(chunks), n, activation);
+    FBP_CHECK_HIP(ctx, hipGetLastError());
+    *used = true;
+    return FBP_OK;
+}
+
 }  // namespace
 
 }  // namespace fbp
@@ -386,6 +882,17 @@ extern "C" fbp_status fbp_backproject_depth_exhaustive(
         return fail(ctx, FBP_ERR_UNSUPPORTED, "depth exhaustive: more than 2^32 outputs or pairs");
     FBP_CHECK(bind(ctx));
     hipStream_t s = as_stream(stream);
+
+    ctx->last_exhaustive_group = 0;
+    if (ctx->exhaustive_group_kernel) {
+        bool used = false;
+        FBP_CHECK(depth_group_launch(ctx, info, tr, laser_targets, sensor_targets, depths,
+                                     num_depths, activation, s, &used));
+        if (used) {
+            ctx->last_exhaustive_group = 1;
+            return FBP_OK;
+        }
+    }
 
     DepthExArgs a{};
     a.data = static_cast<const float*>(tr->data);

@@ -26,6 +26,7 @@
 #include <mutex>
 
 #include "bp_common.h"
+#include "group_common.h"
 
 namespace fbp {
 
@@ -469,7 +470,6 @@ __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs
 // wave g stages pair g's window (from a 4-bin aligned start, 16-byte loads
 // and stores), the next batch's raw loads in flight while the current batch
 // runs.
-constexpr int kGroup = 8;
 constexpr int kGThreads = 512;
 constexpr int kGVec = 2;           // 4-bin loads per lane per batch: window <= 4 * 64 * kGVec bins
 
@@ -508,30 +508,6 @@ struct __align__(16) GroupLaser {
     float rmin, rmax, pad[2];  // box distances
 };
 
-// Correctly rounded square root of a normal x (here x >= 1e-6: squared
-// distances of pairs at least 1 mm apart): the hardware estimate s (within
-// an ulp) and its neighbours, chosen by the signs of the residuals
-// x - s' s computed exactly by FMA; the same selection as the compiler's
-// IEEE square root, without its rescaling of tiny inputs and its
-// special-value cases.
-__device__ __forceinline__ float sqrt_rn_normal(float x) {
-    const float s = __builtin_amdgcn_sqrtf(x);
-    const float sd = __int_as_float(__float_as_int(s) - 1);
-    const float su = __int_as_float(__float_as_int(s) + 1);
-    float r = (fmaf(-sd, s, x) <= 0.f) ? sd : s;
-    r = (fmaf(-su, s, x) > 0.f) ? su : r;
-    return r;
-}
-
-// The reference's bin of dist from the corrected product (exact wherever the
-// host's check passed): q0 = fl(dist c), r = dist - q0 dt (exact by FMA),
-// q = fl(q0 + r c).
-__device__ __forceinline__ float group_quotient(float dist, float c, float dt) {
-    const float q0 = dist * c;
-    const float r = fmaf(-q0, dt, dist);
-    return fmaf(r, c, q0);
-}
-
 template <bool COFF>
 __device__ __forceinline__ void group_fast(float (&acc)[16], const float (&rl)[16],
                                            const float (&rs)[16], const float2* win, int base,
@@ -551,19 +527,6 @@ __device__ __forceinline__ void group_fast(float (&acc)[16], const float (&rl)[1
 #pragma unroll
         for (int v = 0; v < 8; ++v) acc[h + v] += fmaf(q[v], e[v].y, e[v].x);
     }
-}
-
-// Distance from p to the box [lo, hi] with the hardware square root (1 ulp):
-// bounds only (window start, near test), all with margins.
-__device__ __forceinline__ float box_dist_fast(const float p[3], const float lo[3],
-                                               const float hi[3], bool far) {
-    float s = 0.f;
-    for (int ax = 0; ax < 3; ++ax) {
-        const float d = far ? fmaxf(fabsf(lo[ax] - p[ax]), fabsf(hi[ax] - p[ax]))
-                            : fmaxf(fmaxf(lo[ax] - p[ax], p[ax] - hi[ax]), 0.f);
-        s += d * d;
-    }
-    return __builtin_amdgcn_sqrtf(s);
 }
 
 // Raw window samples of 4 bins (converted and thresholded when stored).
@@ -976,27 +939,6 @@ static bool linear_weights_apply(const fbp_reconstruction_info& in) {
     return kmin >= double(kLinMinBin);
 }
 
-// Whether the group kernel's corrected product truncates like the
-// reference's fl(dist / dt) for every dist the bins 1 .. T + 1 can meet.  Both
-// quotients lie within an ulp of dist / dt (the product q0 within about one,
-// the FMA-corrected one within about half), so they can only truncate
-// differently when dist / dt is within an ulp or so of an integer k, i.e.
-// dist within a few ulps of k dt; every float within 16 ulps of each k dt is
-// checked with the same float operations the kernel uses.
-static bool group_quotient_exact(float dt, uint32_t T) {
-    const float c = 1.0f / dt;
-    for (uint32_t k = 1; k <= T + 1; ++k) {
-        float x = float(double(k) * double(dt));
-        for (int i = 0; i < 16; ++i) x = std::nextafter(x, 0.0f);
-        for (int i = 0; i <= 32; ++i, x = std::nextafter(x, INFINITY)) {
-            const float q0 = x * c;
-            const float q = std::fma(std::fma(-q0, dt, x), c, q0);
-            if (static_cast<int>(q) != static_cast<int>(x / dt)) return false;
-        }
-    }
-    return true;
-}
-
 template <int STORAGE>
 static void (*pick_group_s(bool coff))(GroupArgs) {
     return coff ? exhaustive_group_kernel<STORAGE, true> : exhaustive_group_kernel<STORAGE, false>;
@@ -1027,12 +969,7 @@ static fbp_status exhaustive_group_launch(fbp_context ctx, const fbp_reconstruct
         a.W > 4 * 64 * kGVec || (reinterpret_cast<uintptr_t>(tr->data) % align) != 0u)
         return FBP_OK;
     // The quotient check depends on (dt, T) only: kept for the next call.
-    if (ctx->group_quot_dt != info->time_step || ctx->group_quot_T != a.T) {
-        ctx->group_quot_ok = group_quotient_exact(info->time_step, a.T) ? 1 : 0;
-        ctx->group_quot_dt = info->time_step;
-        ctx->group_quot_T = a.T;
-    }
-    if (!ctx->group_quot_ok) return FBP_OK;
+    if (!group_quotient_cached(ctx, info->time_step, a.T)) return FBP_OK;
 
     a.data = tr->data;
     a.scales = tr->scales;

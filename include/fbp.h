@@ -121,10 +121,11 @@ typedef enum fbp_option {
      * safeRCP(dist^2); the walk ends at the first bin >= T.  Runs a
      * dedicated kernel (not the aligned-grid path). */
     FBP_OPTION_VOXEL_SCATTERING = 1,
-    /* 1 (default): fbp_backproject_exhaustive_voxel may run the laser-group
-     * kernel (sensor distances shared by groups of eight lasers) when the
-     * capture qualifies; 0: always the per-pair kernel.  Same bins; volumes
-     * agree to FP32 rounding of the weights. */
+    /* 1 (default): fbp_backproject_exhaustive_voxel and
+     * fbp_backproject_depth_exhaustive may run their laser-group kernels
+     * (sensor distances shared by groups of eight lasers) when the capture
+     * qualifies; 0: always the per-pair / staged-window kernels.  Same bins;
+     * results agree to FP32 rounding. */
     FBP_OPTION_EXHAUSTIVE_LASER_GROUPS = 2
 } fbp_option;
 fbp_status fbp_context_set_option(fbp_context ctx, int32_t option, int32_t value);
@@ -136,10 +137,11 @@ fbp_status fbp_context_get_option(fbp_context ctx, int32_t option, int32_t* valu
  * the device. */
 fbp_status fbp_context_last_confocal_path(fbp_context ctx, int32_t* aligned);
 
-/* Which kernel the last fbp_backproject_exhaustive_voxel[_noalloc] call on
- * this context launched: *laser_groups = 1 for the laser-group kernel, 0 for
- * the per-pair kernel (host-side record, no synchronisation; no reference
- * counterpart). */
+/* Which kernel the last fbp_backproject_exhaustive_voxel[_noalloc] or
+ * fbp_backproject_depth_exhaustive call on this context launched:
+ * *laser_groups = 1 for a laser-group kernel, 0 for the per-pair (voxel) or
+ * staged-window (depth) kernel (host-side record, no synchronisation; no
+ * reference counterpart). */
 fbp_status fbp_context_last_exhaustive_kernel(fbp_context ctx, int32_t* laser_groups);
 
 /* --------------------------------------------------------- backprojection */

@@ -142,3 +142,75 @@ def test_group_kernel_quantized_storage(ctx, storage):
     deq = (v * sc.cpu().numpy()[:, None]).astype(np.float32).reshape(I.shape)
     ref = oracle.exhaustive_voxel(info, deq, g.laser_targets, g.sensor_targets)
     assert rel_l2(got, ref) <= 1e-4
+
+
+# ------------------------------------------------------------ depth planes
+def depth_geometry(n_las, n_sen, T=1024, discard=1, t0=0.0):
+    return Geometry(laser_targets=wall_grid(n_las, 1.0), sensor_targets=wall_grid(n_sen, 1.0),
+                    num_time_bins=T, time_step=0.0048, hidden_min=(-0.5, 0.2, -0.5),
+                    hidden_max=(0.5, 1.0, 0.5), voxel_resolution=(8, 8, 8), confocal=False,
+                    discard_first_last_bounces=discard, time_offset=t0,
+                    laser_position=(0.1, -0.8, 0.2), sensor_position=(-0.2, -0.7, 0.1),
+                    wall_size=(1.0, 1.0))
+
+
+def run_depth(ctx, info, I, g, depths, groups=True):
+    ctx.set_exhaustive_laser_groups(groups)
+    try:
+        out = ctx.backproject_depth(info, I, cuda(g.laser_targets), cuda(depths),
+                                    sensor_targets=cuda(g.sensor_targets))
+        used = ctx.last_exhaustive_laser_groups()
+    finally:
+        ctx.set_exhaustive_laser_groups(True)
+    return out.cpu().numpy().astype(np.float64), used
+
+
+@pytest.mark.parametrize("discard,t0", [(1, 0.0), (0, 0.1)])
+def test_depth_group_matches_oracle_and_pair_kernel(ctx, discard, t0):
+    """Depth planes (backprojectExhaustive, FB/transient_reconstruction.cuh:66-91)
+    through the laser-group form: 36 pixels (a full tile of 32 and a partial
+    one; 5 laser groups, the last of 4), 81 sensors, 300 depths (two depth
+    tiles), against the oracle and the staged-window kernel."""
+    g = depth_geometry(6, 9, discard=discard, t0=t0)
+    info = make_info(g)
+    rng = np.random.default_rng(31)
+    I = spiky(rng, (36, 81, 1024))
+    depths = np.linspace(0.3, 0.9, 300, endpoint=False)
+    ref = oracle.depth(info, I, g.laser_targets, depths, sensor=g.sensor_targets, exhaustive=True)
+    out, used = run_depth(ctx, info, cuda(I), g, depths)
+    assert used
+    assert np.abs(ref).max() > 0
+    err = np.abs(out - ref) / np.maximum(np.abs(ref), 1e-3 * np.abs(ref).max())
+    assert err.max() <= 2e-5, np.unravel_index(np.argmax(err), err.shape)
+    other, used2 = run_depth(ctx, info, cuda(I), g, depths, groups=False)
+    assert not used2
+    assert rel_l2(out, other) <= 1e-6
+
+
+@pytest.mark.parametrize("discard,t0", [(1, 0.0), (0, 0.11)])
+def test_depth_group_bins_exact(ctx, discard, t0):
+    """One lit (laser, sensor) pair per call with I[l][s][t] = t + 1: each
+    activation is the single sample bin + 1 (exact in FP32), so every
+    (pixel, depth) bin is compared with the oracle's restatement of the
+    reference's IEEE chain; pairs from the full and the partial laser group."""
+    T = 1024
+    g = depth_geometry(3, 8, T=T, discard=discard, t0=t0)
+    info = make_info(g)
+    L, S = len(g.laser_targets), len(g.sensor_targets)
+    depths = np.linspace(0.2, 1.0, 37) + 0.0013
+    cl, cs, cd = cuda(g.laser_targets), cuda(g.sensor_targets), cuda(depths)
+    pairs = [(0, 0), (7, S - 1), (8, 5), (4, 33), (8, S - 1), (2, 17)]
+    acts = []
+    for l, s in pairs:
+        I = np.zeros((L, S, T), np.float32)
+        I[l, s] = np.arange(1, T + 1, dtype=np.float32)
+        acts.append(ctx.backproject_depth(info, cuda(I), cl, cd, sensor_targets=cs))
+        assert ctx.last_exhaustive_laser_groups()
+    got = torch.stack(acts).cpu().numpy()
+    for i, (l, s) in enumerate(pairs):
+        bins = oracle.depth_bins(info, g.laser_targets, depths, l, g.sensor_targets, s, True)
+        assert np.array_equal(got[i], np.rint(got[i])), (l, s)
+        rec = np.rint(got[i].astype(np.float64)).astype(np.int64) - 1
+        bad = np.argwhere(rec != bins)
+        assert bad.size == 0, (l, s, bad[:5].tolist())
+        assert (bins >= 0).mean() > 0.9
