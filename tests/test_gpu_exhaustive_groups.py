@@ -59,11 +59,11 @@ def run(ctx, info, I, g, groups=True, **kw):
     return out, used
 
 
-@pytest.mark.parametrize("discard,t0", [(1, 0.0), (0, 0.1)])
-def test_group_kernel_matches_oracle(ctx, discard, t0):
-    g = geometry(3, 9, discard=discard, t0=t0)
+@pytest.mark.parametrize("n_las,discard,t0", [(3, 1, 0.0), (3, 0, 0.1), (5, 1, 0.0), (5, 0, 0.1)])
+def test_group_kernel_matches_oracle(ctx, n_las, discard, t0):
+    g = geometry(n_las, 9, discard=discard, t0=t0)
     info = make_info(g)
-    I = spiky(np.random.default_rng(21), (9, 81, 1024))
+    I = spiky(np.random.default_rng(21), (n_las * n_las, 81, 1024))
     ref = oracle.exhaustive_voxel(info, I, g.laser_targets, g.sensor_targets)
     out, used = run(ctx, info, cuda(I), g)
     assert used
@@ -88,19 +88,22 @@ def test_group_kernel_needs_the_linear_range(ctx):
     assert rel_l2(out.cpu().numpy(), ref) <= 1e-4
 
 
-@pytest.mark.parametrize("n_sen,discard,t0", [(9, 1, 0.0), (12, 0, 0.1), (12, 1, 0.0)])
-def test_group_kernel_bins_exact(ctx, n_sen, discard, t0):
+@pytest.mark.parametrize("n_las,n_sen,discard,t0", [(3, 9, 1, 0.0), (3, 12, 0, 0.1), (3, 12, 1, 0.0),
+                                                   (5, 9, 1, 0.0), (5, 8, 0, 0.1)])
+def test_group_kernel_bins_exact(ctx, n_las, n_sen, discard, t0):
     """One lit (laser, sensor) pair per call, I[l][s][t] = t + 1: with the
     linear weights vol * dist^2 = (bin + 1)(1 + e), |e| <= 3e-5, which rounds
-    to bin + 1 below 10^4.  Pairs from the full and the partial laser group
-    and from both sensor parts; 24 x 40 x 24 voxels per pair, so the band
-    (about one update in a thousand) is crossed hundreds of times."""
+    to bin + 1 below 10^4.  9 lasers (one full group of 8 and a partial one)
+    or 25 (three full groups); pairs from full and partial laser groups and
+    from both sensor parts."""
     T = 2048
-    g = geometry(3, n_sen, T=T, res=(24, 40, 24), discard=discard, t0=t0)
+    g = geometry(n_las, n_sen, T=T, res=(24, 40, 24), discard=discard, t0=t0)
     info = make_info(g)
     L, S = len(g.laser_targets), len(g.sensor_targets)
     rng = np.random.default_rng(5)
-    pairs = [(0, 0), (7, S - 1), (8, S // 2), (3, 1), (5, S - 2), (8, 0)]
+    pairs = [(0, 0), (7, S - 1), (8, S // 2), (3, 1), (5, S - 2), (L - 1, 0)]
+    if L > 16:
+        pairs += [(15, 3), (16, S - 1), (L - 2, 7)]
     pairs += [(int(l), int(s)) for l, s in zip(rng.integers(0, L, 10), rng.integers(0, S, 10))]
     cl, cs = cuda(g.laser_targets), cuda(g.sensor_targets)
     vols = []
