@@ -374,9 +374,9 @@ __global__ void depth_reduce_kernel(const float* __restrict__ partial, uint32_t 
 // (group of 8 lasers, sensor) batches.  Each lane keeps the eight exact laser
 // distances of its 16 points while the group is unchanged and computes its
 // 16 exact sensor distances once per sensor; the update is the reference's
-// dist = fl(rl + rs) (- t0, + off), the corrected quotient (exact at every
+// dist = fl(rl + rs) (- t0, + off), the two-term quotient (exact at every
 // bin boundary by the host's check) and one LDS read of the staged sample:
-// 7 VALU.  Pairs whose points may come within a millimetre of the laser or
+// 6 VALU.  Pairs whose points may come within a millimetre of the laser or
 // sensor point, or of zero distance, take the reference's chain per point;
 // a tile whose bin window would exceed the staging capacity reads its samples
 // from the transient directly.
@@ -392,7 +392,7 @@ struct DepthGroupArgs {
     const uint32_t* perm;  // pixels in Morton order on the wall
     float* partial;        // [chunk][L * D]
     uint32_t L, S, T, D;
-    float t0, dt, inv_dt, ny;
+    float t0, dt, inv_dt, inv_dt_lo, ny;  // inv_dt + inv_dt_lo = 1 / dt
     uint32_t discard;
     float lpos[3], spos[3];
     uint32_t groups, groups_per_chunk, sensor_parts, sensors_per_part;
@@ -558,7 +558,7 @@ __global__ __launch_bounds__(512, 1) void depth_group_kernel(DepthGroupArgs a) {
     const uint32_t s_end = min(a.S, s_begin + a.sensors_per_part);
     const uint32_t ns = s_end > s_begin ? s_end - s_begin : 0u;
     const uint32_t nbatch = (g_end > g_begin ? g_end - g_begin : 0u) * ns;
-    const float cq = a.inv_dt;
+    const float cq = a.inv_dt, cql = a.inv_dt_lo;
 
     const uint32_t l_begin = g_begin * kGroup;
     const uint32_t nl = min(a.L, g_end * kGroup) - min(a.L, l_begin);
@@ -712,7 +712,7 @@ __global__ __launch_bounds__(512, 1) void depth_group_kernel(DepthGroupArgs a) {
                 for (int v = 0; v < 8; ++v) {
                     float d = rl[g][h + v] + rs[h + v];
                     if constexpr (COFF) d = (d - a.t0) + off;
-                    e[v] = winbuf[base + static_cast<int>(group_quotient(d, cq, a.dt))];
+                    e[v] = winbuf[base + static_cast<int>(group_quotient(d, cq, cql))];
                 }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -804,6 +804,7 @@ fbp_status depth_group_launch(fbp_context ctx, const fbp_reconstruction_info* in
     a.t0 = info->time_offset;
     a.dt = info->time_step;
     a.inv_dt = 1.0f / info->time_step;
+    a.inv_dt_lo = group_inv_dt_lo(info->time_step);
     a.ny = info->relay_wall_normal.y;
     a.discard = info->discard_first_last_bounces;
     a.lpos[0] = info->laser_position.x;

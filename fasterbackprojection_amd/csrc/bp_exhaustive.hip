@@ -457,13 +457,13 @@ __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs
 // roots of the same squared sums), so an update starts from the reference's
 // dist = fl(rl + rs) (- t0 and + off when the chain has them) and needs no
 // square root:
-//   q0 = fl(dist * c), c = fl(1 / dt);  q = fma(fma(-q0, dt, dist), c, q0);
+//   q = fma(dist, c_hi, fl(dist * c_lo)),  c_hi + c_lo = 1 / dt;
 //   k = (int)q;  acc[v] += A_k + B_k q.
-// The corrected quotient q equals the reference's fl(dist / dt) wherever a
-// bin could differ: the host checks every float within 16 ulps of each bin
-// boundary k dt (the only places where two quotients within an ulp of
-// dist / dt can truncate differently) before it chooses this kernel, so the
-// bins are the reference's exactly.  The window entry of bin k is the linear
+// The two-term quotient q truncates like the reference's fl(dist / dt): the
+// host checks every float within 16 ulps of each bin boundary k dt (the only
+// places where two quotients rounded from values within 2^-47 of dist / dt
+// can truncate differently) before it chooses this kernel, so the bins are
+// the reference's exactly.  The window entry of bin k is the linear
 // expansion of I / q^2 about m = k + 1/2: {3 I r^2, -2 I r^3}, r = 1/m, from a
 // per-bin table (bins >= kLinMinBin; other pairs take the reference's chain
 // for every voxel).  One batch = one sensor and the group's eight pairs;
@@ -472,6 +472,9 @@ __global__ __launch_bounds__(256, 2) void exhaustive_voxel_kernel(ExhaustiveArgs
 // runs.
 constexpr int kGThreads = 512;
 constexpr int kGVec = 2;           // 4-bin loads per lane per batch: window <= 4 * 64 * kGVec bins
+// Pair parameters are computed kGPrmBatches batches at a time, by the first
+// kGPrmBatches * kGroup threads, into one half of a two-block LDS ring.
+constexpr int kGPrmBatches = 16;
 
 struct GroupArgs {
     const void* data;
@@ -481,7 +484,7 @@ struct GroupArgs {
     float* partial;
     VolumeGeom g;
     uint32_t L, S, T;
-    float t0, dt, inv_dt, dt2;
+    float t0, dt, inv_dt, inv_dt_lo, dt2;  // inv_dt + inv_dt_lo = 1 / dt
     uint32_t discard;
     float lpos[3], spos[3];
     int W;                    // window length in bins (multiple of 4): bins kmin4 .. kmin4 + W - 1
@@ -511,7 +514,7 @@ struct __align__(16) GroupLaser {
 template <bool COFF>
 __device__ __forceinline__ void group_fast(float (&acc)[16], const float (&rl)[16],
                                            const float (&rs)[16], const float2* win, int base,
-                                           float c, float dt, float t0, float off) {
+                                           float c_hi, float c_lo, float t0, float off) {
 #pragma unroll
     for (int h = 0; h < 16; h += 8) {
         float q[8];
@@ -520,7 +523,7 @@ __device__ __forceinline__ void group_fast(float (&acc)[16], const float (&rl)[1
         for (int v = 0; v < 8; ++v) {
             float d = rl[h + v] + rs[h + v];
             if constexpr (COFF) d = (d - t0) + off;
-            q[v] = group_quotient(d, c, dt);
+            q[v] = group_quotient(d, c_hi, c_lo);
             e[v] = win[base + static_cast<int>(q[v])];
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -570,12 +573,12 @@ template <int STORAGE, bool COFF>
 __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArgs a) {
     using Raw = typename GroupRaw<STORAGE>::type;
     extern __shared__ float smem[];
-    // [2][kGroup][W] window entries, [3][kGroup] pair parameters, the chunk's
-    // lasers [groups_per_chunk * kGroup] and sensors [sensors_per_part], [T]
-    // weight factors.
+    // [2][kGroup][W] window entries, [2 * kGPrmBatches][kGroup] pair
+    // parameters, the chunk's lasers [groups_per_chunk * kGroup] and sensors
+    // [sensors_per_part], [T] weight factors.
     float2* winbuf = reinterpret_cast<float2*>(smem);
     GroupPair* prmbuf = reinterpret_cast<GroupPair*>(winbuf + 2 * kGroup * a.W);
-    GroupLaser* lasl = reinterpret_cast<GroupLaser*>(prmbuf + 3 * kGroup);
+    GroupLaser* lasl = reinterpret_cast<GroupLaser*>(prmbuf + 2 * kGPrmBatches * kGroup);
     float4* senl = reinterpret_cast<float4*>(lasl + a.groups_per_chunk * kGroup);
     float2* tab = reinterpret_cast<float2*>(senl + a.sensors_per_part);
 
@@ -612,7 +615,7 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
     const uint32_t s_end = min(a.S, s_begin + a.sensors_per_part);
     const uint32_t ns = s_end > s_begin ? s_end - s_begin : 0u;
     const uint32_t nbatch = (g_end > g_begin ? g_end - g_begin : 0u) * ns;
-    const float c = a.inv_dt;
+    const float c = a.inv_dt, cl = a.inv_dt_lo;
 
     // Weight factors of bin k (m = k + 1/2): {3 r^2, -2 r^3}.
     for (uint32_t k = tid; k < a.T; k += kGThreads) {
@@ -647,8 +650,7 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
     }
     __syncthreads();
 
-    // Parameters of batch b's pair g (computed by thread g of a batch's eight)
-    // from the staged lasers and sensors.
+    // Parameters of batch b's pair g from the staged lasers and sensors.
     auto params = [&](uint32_t b, int g, GroupPair* out) {
         GroupPair P;
         const uint32_t li = (b / ns) * kGroup + uint32_t(g);
@@ -685,23 +687,28 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
 
     // Raw loads of batch b: wave g fetches pair g's bins kmin4 + 4 j .. + 3,
     // j = lane + 64 i.  Nothing reads them before the store, so they stay in
-    // flight across the batch; loads outside the pair read element 0.
+    // flight across the batch.  The pair's row is a scalar address; lanes whose
+    // bins lie outside [0, T) read bins inside the row (the store zeroes
+    // them), and a batch slot without a pair (l >= L) reads row 0.
     auto fetch = [&](uint32_t b, const GroupPair* prm, Raw (&reg)[kGVec]) {
         const int fast = rfl(prm[wave].fast);
         const int kmin4 = rfl(prm[wave].kmin4);
         const uint32_t grp = g_begin + b / ns;
         const uint32_t s = s_begin + b % ns;
-        const uint64_t pair = uint64_t(grp * kGroup + uint32_t(wave)) * a.S + s;
+        const uint64_t pair = fast >= 0 ? uint64_t(grp * kGroup + uint32_t(wave)) * a.S + s : 0u;
 #pragma unroll
         for (int i = 0; i < kGVec; ++i) {
-            const int j = lane + 64 * i;
-            const int k0 = kmin4 + 4 * j;
-            const bool ok = fast >= 0 && 4 * j < a.W && k0 >= 0 && k0 < int(a.T);
-            const uint64_t idx = ok ? pair * a.T + uint32_t(k0) : 0;
-            if constexpr (STORAGE == FBP_STORAGE_F32)
-                reg[i] = *reinterpret_cast<const float4*>(static_cast<const float*>(a.data) + idx);
-            else
-                reg[i] = *reinterpret_cast<const uint2*>(static_cast<const uint16_t*>(a.data) + idx);
+            const int k0 = kmin4 + 4 * (lane + 64 * i);
+            // T <= 8192 and a multiple of 4 for this kernel.
+            const uint32_t kc = uint32_t(min(max(k0, 0), int(a.T) - 4));
+            __builtin_assume(kc < 8192u);
+            if constexpr (STORAGE == FBP_STORAGE_F32) {
+                const float* row = static_cast<const float*>(a.data) + pair * a.T;
+                reg[i] = *reinterpret_cast<const float4*>(row + kc);
+            } else {
+                const uint16_t* row = static_cast<const uint16_t*>(a.data) + pair * a.T;
+                reg[i] = *reinterpret_cast<const uint2*>(row + kc);
+            }
         }
     };
     auto store = [&](uint32_t b, float2* win, const GroupPair* prm, const Raw (&reg)[kGVec]) {
@@ -722,21 +729,30 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
                 const bool ok = k0 >= 0 && k0 < int(a.T);
                 float v[4];
                 group_convert<STORAGE>(reg[i], sc, v);
-                float4 e01 = make_float4(0.f, 0.f, 0.f, 0.f), e23 = e01;
-                if (ok) {
-                    if (fast == 1) {
-                        const float4 t01 = *reinterpret_cast<const float4*>(tab + k0);
-                        const float4 t23 = *reinterpret_cast<const float4*>(tab + k0 + 2);
-                        e01 = make_float4(v[0] * t01.x, v[0] * t01.y, v[1] * t01.z, v[1] * t01.w);
-                        e23 = make_float4(v[2] * t23.x, v[2] * t23.y, v[3] * t23.z, v[3] * t23.w);
-                    } else {
-                        e01 = make_float4(v[0], 0.f, v[1], 0.f);
-                        e23 = make_float4(v[2], 0.f, v[3], 0.f);
-                    }
+                // Bins outside [0, T) hold zero samples (their loads read
+                // element 0).
+#pragma unroll
+                for (int c = 0; c < 4; ++c) v[c] = ok ? v[c] : 0.f;
+                float4 e01, e23;
+                if (fast == 1) {
+                    // Factors of bins k0 .. k0 + 3, the index clamped into
+                    // the table (T is a multiple of 4) for the zero samples.
+                    const int kt = min(max(k0, 0), int(a.T) - 4);
+                    const float4 t01 = *reinterpret_cast<const float4*>(tab + kt);
+                    const float4 t23 = *reinterpret_cast<const float4*>(tab + kt + 2);
+                    e01 = make_float4(v[0] * t01.x, v[0] * t01.y, v[1] * t01.z, v[1] * t01.w);
+                    e23 = make_float4(v[2] * t23.x, v[2] * t23.y, v[3] * t23.z, v[3] * t23.w);
+                } else {
+                    e01 = make_float4(v[0], 0.f, v[1], 0.f);
+                    e23 = make_float4(v[2], 0.f, v[3], 0.f);
                 }
-                float4* dst = reinterpret_cast<float4*>(win + wave * a.W + 4 * j);
-                dst[0] = e01;
-                dst[1] = e23;
+                // Four 8-byte entries (two paired LDS writes: no register
+                // moves to make 16-byte sources).
+                float2* dst = win + wave * a.W + 4 * j;
+                dst[0] = make_float2(e01.x, e01.y);
+                dst[1] = make_float2(e01.z, e01.w);
+                dst[2] = make_float2(e23.x, e23.y);
+                dst[3] = make_float2(e23.z, e23.w);
             }
         }
     };
@@ -753,8 +769,9 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
     const int per = kGroup * a.W;
     Raw reg[kGVec];
     if (nbatch > 0) {
-        if (tid < kGroup) params(0, tid, prmbuf + tid);
-        if (tid >= kGroup && tid < 2 * kGroup && nbatch > 1) params(1, tid - kGroup, prmbuf + tid);
+        // Parameters of the first two blocks of batches.
+        if (tid < 2 * kGPrmBatches * kGroup && uint32_t(tid / kGroup) < nbatch)
+            params(uint32_t(tid / kGroup), tid % kGroup, prmbuf + tid);
         __syncthreads();  // parameters and the weight table
         fetch(0, prmbuf, reg);
         store(0, winbuf, prmbuf, reg);
@@ -762,9 +779,9 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
     }
     for (uint32_t b = 0; b < nbatch; ++b) {
         const int cur = int(b & 1u);
-        const GroupPair* prm = prmbuf + (b % 3u) * kGroup;
+        const GroupPair* prm = prmbuf + (b % (2u * kGPrmBatches)) * kGroup;
         const bool has_next = b + 1 < nbatch;
-        const GroupPair* prm_next = prmbuf + ((b + 1) % 3u) * kGroup;
+        const GroupPair* prm_next = prmbuf + ((b + 1) % (2u * kGPrmBatches)) * kGroup;
         if (has_next) fetch(b + 1, prm_next, reg);
 
         if (b % ns == 0) {
@@ -822,13 +839,13 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
             }
 #pragma unroll
             for (int g = 0; g < kGroup; ++g)
-                group_fast<COFF>(acc, rl[g], rs, winbuf, bases[g], c, a.dt, a.t0, offs[g]);
+                group_fast<COFF>(acc, rl[g], rs, winbuf, bases[g], c, cl, a.t0, offs[g]);
         } else {
 #pragma unroll
             for (int g = 0; g < kGroup; ++g) {
                 if (fast_pairs & (1 << g))
                     group_fast<COFF>(acc, rl[g], rs, winbuf, rfl(prm[g].base) + cur * per, c,
-                                     a.dt, a.t0, rfl(prm[g].off));
+                                     cl, a.t0, rfl(prm[g].off));
             }
         }
         // Pairs near the wall or zero distance, or with bins below the linear
@@ -857,9 +874,14 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
         if (has_next) {
             // The other window buffer was last read in batch b - 1.
             store(b + 1, winbuf + (cur ^ 1) * per, prm_next, reg);
-            // Parameters of batch b + 2 go to the slot batch b - 1 used.
-            if (tid < kGroup && b + 2 < nbatch)
-                params(b + 2, tid, prmbuf + ((b + 2) % 3u) * kGroup + tid);
+        }
+        // First batch of a block: the block after next goes to the slots of
+        // the block before, whose last batch ended with the previous barrier
+        // (and is first needed by the fetch at the end of the next block).
+        if (b > 0 && b % kGPrmBatches == 0 && tid < kGPrmBatches * kGroup) {
+            const uint32_t bn = b + kGPrmBatches + uint32_t(tid / kGroup);
+            if (bn < nbatch)
+                params(bn, tid % kGroup, prmbuf + (bn % (2u * kGPrmBatches)) * kGroup + tid % kGroup);
         }
         __syncthreads();
     }
@@ -978,6 +1000,7 @@ static fbp_status exhaustive_group_launch(fbp_context ctx, const fbp_reconstruct
     a.t0 = info->time_offset;
     a.dt = info->time_step;
     a.inv_dt = 1.0f / info->time_step;
+    a.inv_dt_lo = group_inv_dt_lo(info->time_step);
     a.dt2 = info->time_step * info->time_step;
     a.discard = info->discard_first_last_bounces;
     a.lpos[0] = info->laser_position.x;
@@ -1008,7 +1031,8 @@ static fbp_status exhaustive_group_launch(fbp_context ctx, const fbp_reconstruct
     // LDS: windows, pair parameters, the chunk's lasers and sensors, weight
     // factors.  Sensor ranges shrink (more parts) to fit.
     auto lds_bytes = [&](uint32_t spp) {
-        return size_t(2) * kGroup * a.W * sizeof(float2) + 3 * kGroup * sizeof(GroupPair) +
+        return size_t(2) * kGroup * a.W * sizeof(float2) +
+               size_t(2) * kGPrmBatches * kGroup * sizeof(GroupPair) +
                size_t(a.groups_per_chunk) * kGroup * sizeof(GroupLaser) + size_t(spp) * sizeof(float4) +
                size_t(a.T) * sizeof(float2);
     };

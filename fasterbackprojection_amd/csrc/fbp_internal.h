@@ -24,7 +24,7 @@ struct fbp_context_s {
     int exhaustive_group_kernel = 1;
     // Kernel of the last exhaustive voxel call: 1 = laser-group kernel.
     int last_exhaustive_group = 0;
-    // Whether the laser-group kernel's corrected product truncates like
+    // Whether the laser-group kernels' two-term quotient truncates like
     // fl(dist / dt) at every bin boundary, for the (dt, T) it was checked at.
     int group_quot_ok = 0;
     float group_quot_dt = 0.f;

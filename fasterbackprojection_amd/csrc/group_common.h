@@ -1,6 +1,6 @@
 // Pieces shared by the laser-group kernels (exhaustive voxel backprojection
 // and exhaustive depth planes): the exact sensor-distance square root, the
-// corrected bin quotient and the host check that makes it exact, and the
+// two-term bin quotient and the host check that makes it exact, and the
 // box bounds used for bin windows.
 #pragma once
 
@@ -30,13 +30,18 @@ __device__ __forceinline__ float sqrt_rn_normal(float x) {
     return r;
 }
 
-// The reference's bin of dist from the corrected product (exact wherever the
-// host's check passed): q0 = fl(dist c), r = dist - q0 dt (exact by FMA),
-// q = fl(q0 + r c).
-__device__ __forceinline__ float group_quotient(float dist, float c, float dt) {
-    const float q0 = dist * c;
-    const float r = fmaf(-q0, dt, dist);
-    return fmaf(r, c, q0);
+// The reference's bin of dist from a two-term product, exact wherever the
+// host's check passed: c_hi = fl(1 / dt), c_lo = fl(1 / dt - c_hi), so
+// c_hi + c_lo is 1 / dt to about 2^-48 and q = fl(dist c_hi + fl(dist c_lo))
+// is dist / dt rounded once from a value within about 2^-47 of it.  Host and
+// device evaluate it with the same float operations.
+__host__ __device__ inline float group_quotient(float dist, float c_hi, float c_lo) {
+    return fmaf(dist, c_hi, dist * c_lo);
+}
+
+// c_lo of group_quotient for dt.
+inline float group_inv_dt_lo(float dt) {
+    return static_cast<float>(1.0 / double(dt) - double(1.0f / dt));
 }
 
 // Distance from p to the box [lo, hi] with the hardware square root (1 ulp):
@@ -52,22 +57,20 @@ __device__ __forceinline__ float box_dist_fast(const float p[3], const float lo[
     return __builtin_amdgcn_sqrtf(s);
 }
 
-// Whether the group kernel's corrected product truncates like the
-// reference's fl(dist / dt) for every dist the bins 1 .. T + 1 can meet.  Both
-// quotients lie within an ulp of dist / dt (the product q0 within about one,
-// the FMA-corrected one within about half), so they can only truncate
-// differently when dist / dt is within an ulp or so of an integer k, i.e.
-// dist within a few ulps of k dt; every float within 16 ulps of each k dt is
-// checked with the same float operations the kernel uses.
+// Whether group_quotient truncates like the reference's fl(dist / dt) for
+// every dist the bins 1 .. T + 1 can meet.  Both quotients are dist / dt
+// rounded once (from values at most about 2^-47 apart), so they can only
+// truncate differently when dist / dt is within an ulp of an integer k, i.e.
+// dist within an ulp or so of k dt; every float within 16 ulps of each k dt
+// is checked.
 inline bool group_quotient_exact(float dt, uint32_t T) {
-    const float c = 1.0f / dt;
+    const float c_hi = 1.0f / dt, c_lo = group_inv_dt_lo(dt);
     for (uint32_t k = 1; k <= T + 1; ++k) {
         float x = float(double(k) * double(dt));
         for (int i = 0; i < 16; ++i) x = std::nextafter(x, 0.0f);
         for (int i = 0; i <= 32; ++i, x = std::nextafter(x, INFINITY)) {
-            const float q0 = x * c;
-            const float q = std::fma(std::fma(-q0, dt, x), c, q0);
-            if (static_cast<int>(q) != static_cast<int>(x / dt)) return false;
+            if (static_cast<int>(group_quotient(x, c_hi, c_lo)) != static_cast<int>(x / dt))
+                return false;
         }
     }
     return true;
