@@ -690,11 +690,11 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
     // flight across the batch.  The pair's row is a scalar address; lanes whose
     // bins lie outside [0, T) read bins inside the row (the store zeroes
     // them), and a batch slot without a pair (l >= L) reads row 0.
-    auto fetch = [&](uint32_t b, const GroupPair* prm, Raw (&reg)[kGVec]) {
+    auto fetch = [&](uint32_t gi, uint32_t si, const GroupPair* prm, Raw (&reg)[kGVec]) {
         const int fast = rfl(prm[wave].fast);
         const int kmin4 = rfl(prm[wave].kmin4);
-        const uint32_t grp = g_begin + b / ns;
-        const uint32_t s = s_begin + b % ns;
+        const uint32_t grp = g_begin + gi;
+        const uint32_t s = s_begin + si;
         const uint64_t pair = fast >= 0 ? uint64_t(grp * kGroup + uint32_t(wave)) * a.S + s : 0u;
 #pragma unroll
         for (int i = 0; i < kGVec; ++i) {
@@ -711,14 +711,15 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
             }
         }
     };
-    auto store = [&](uint32_t b, float2* win, const GroupPair* prm, const Raw (&reg)[kGVec]) {
+    auto store = [&](uint32_t gi, uint32_t si, float2* win, const GroupPair* prm,
+                     const Raw (&reg)[kGVec]) {
         const int fast = rfl(prm[wave].fast);
         if (fast < 0) return;
         const int kmin4 = rfl(prm[wave].kmin4);
         float sc = 1.f;
         if constexpr (STORAGE != FBP_STORAGE_F32) {
-            const uint32_t grp = g_begin + b / ns;
-            const uint32_t s = s_begin + b % ns;
+            const uint32_t grp = g_begin + gi;
+            const uint32_t s = s_begin + si;
             sc = a.scales[uint64_t(grp * kGroup + uint32_t(wave)) * a.S + s];
         }
 #pragma unroll
@@ -773,18 +774,26 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
         if (tid < 2 * kGPrmBatches * kGroup && uint32_t(tid / kGroup) < nbatch)
             params(uint32_t(tid / kGroup), tid % kGroup, prmbuf + tid);
         __syncthreads();  // parameters and the weight table
-        fetch(0, prmbuf, reg);
-        store(0, winbuf, prmbuf, reg);
+        fetch(0, 0, prmbuf, reg);
+        store(0, 0, winbuf, prmbuf, reg);
         __syncthreads();
     }
+    // Batch b = gi * ns + si (laser group gi of the chunk, sensor si of the
+    // part), kept as running counters.
+    uint32_t gi = 0, si = 0;
     for (uint32_t b = 0; b < nbatch; ++b) {
         const int cur = int(b & 1u);
+        uint32_t gn = gi, sn = si + 1;
+        if (sn == ns) {
+            sn = 0;
+            ++gn;
+        }
         const GroupPair* prm = prmbuf + (b % (2u * kGPrmBatches)) * kGroup;
         const bool has_next = b + 1 < nbatch;
         const GroupPair* prm_next = prmbuf + ((b + 1) % (2u * kGPrmBatches)) * kGroup;
-        if (has_next) fetch(b + 1, prm_next, reg);
+        if (has_next) fetch(gn, sn, prm_next, reg);
 
-        if (b % ns == 0) {
+        if (si == 0) {
             // New laser group: the eight exact laser distances of the lane's voxels.
 #pragma unroll
             for (int g = 0; g < kGroup; ++g) {
@@ -873,7 +882,7 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
         }
         if (has_next) {
             // The other window buffer was last read in batch b - 1.
-            store(b + 1, winbuf + (cur ^ 1) * per, prm_next, reg);
+            store(gn, sn, winbuf + (cur ^ 1) * per, prm_next, reg);
         }
         // First batch of a block: the block after next goes to the slots of
         // the block before, whose last batch ended with the previous barrier
@@ -884,6 +893,8 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
                 params(bn, tid % kGroup, prmbuf + (bn % (2u * kGPrmBatches)) * kGroup + tid % kGroup);
         }
         __syncthreads();
+        gi = gn;
+        si = sn;
     }
 
     // Slots of the workgroup tile (lt = 4 x 4 x 8): wave w holds oz >> 2 = w.
