@@ -578,7 +578,10 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
     // [sensors_per_part], [T] weight factors.
     float2* winbuf = reinterpret_cast<float2*>(smem);
     GroupPair* prmbuf = reinterpret_cast<GroupPair*>(winbuf + 2 * kGroup * a.W);
-    GroupLaser* lasl = reinterpret_cast<GroupLaser*>(prmbuf + 2 * kGPrmBatches * kGroup);
+    // Per batch slot: bits 0-7 pairs on the fast body, bits 8-15 pairs on
+    // the exact chain.
+    uint32_t* bmask = reinterpret_cast<uint32_t*>(prmbuf + 2 * kGPrmBatches * kGroup);
+    GroupLaser* lasl = reinterpret_cast<GroupLaser*>(bmask + 2 * kGPrmBatches);
     float4* senl = reinterpret_cast<float4*>(lasl + a.groups_per_chunk * kGroup);
     float2* tab = reinterpret_cast<float2*>(senl + a.sensors_per_part);
 
@@ -685,6 +688,23 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
         *out = P;
     };
 
+    // Parameters of batch bn's pair g for the threads of a block (lanes
+    // 8j .. 8j + 7 of a wave hold one batch; every lane of the wave takes
+    // part, with valid = false past the last batch) and the batch's masks.
+    auto params_block = [&](uint32_t bn, int g, bool valid) {
+        int fast = -1;
+        if (valid) {
+            GroupPair* o = prmbuf + (bn % (2u * kGPrmBatches)) * kGroup + g;
+            params(bn, g, o);
+            fast = o->fast;
+        }
+        const uint64_t fm = __ballot(fast == 1), em = __ballot(fast == 0);
+        const int sh = lane & ~(kGroup - 1);
+        if (valid && g == 0)
+            bmask[bn % (2u * kGPrmBatches)] =
+                uint32_t((fm >> sh) & 0xffu) | (uint32_t((em >> sh) & 0xffu) << 8);
+    };
+
     // Raw loads of batch b: wave g fetches pair g's bins kmin4 + 4 j .. + 3,
     // j = lane + 64 i.  Nothing reads them before the store, so they stay in
     // flight across the batch.  The pair's row is a scalar address; lanes whose
@@ -771,8 +791,8 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
     Raw reg[kGVec];
     if (nbatch > 0) {
         // Parameters of the first two blocks of batches.
-        if (tid < 2 * kGPrmBatches * kGroup && uint32_t(tid / kGroup) < nbatch)
-            params(uint32_t(tid / kGroup), tid % kGroup, prmbuf + tid);
+        if (tid < 2 * kGPrmBatches * kGroup)
+            params_block(uint32_t(tid / kGroup), tid % kGroup, uint32_t(tid / kGroup) < nbatch);
         __syncthreads();  // parameters and the weight table
         fetch(0, 0, prmbuf, reg);
         store(0, 0, winbuf, prmbuf, reg);
@@ -829,13 +849,9 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
                 for (int m = 0; m < 4; ++m)
                     rs[i * 4 + m] = sqrt_rn_normal(dot3(XS2[i], YS2, ZS2[m]));
         }
-        int fast_pairs = 0, exact_pairs = 0;
-#pragma unroll
-        for (int g = 0; g < kGroup; ++g) {
-            const int fg = rfl(prm[g].fast);
-            fast_pairs |= (fg == 1 ? 1 : 0) << g;
-            exact_pairs |= (fg == 0 ? 1 : 0) << g;
-        }
+        const uint32_t masks = uint32_t(rfl(int(bmask[b % (2u * kGPrmBatches)])));
+        const int fast_pairs = int(masks & 0xffu);
+        int exact_pairs = int(masks >> 8);
         if (fast_pairs == (1 << kGroup) - 1) {
             // Common case, straight-line: all eight pairs take the fast body,
             // their window offsets and path offsets read up front.
@@ -889,8 +905,7 @@ __global__ __launch_bounds__(kGThreads, 1) void exhaustive_group_kernel(GroupArg
         // (and is first needed by the fetch at the end of the next block).
         if (b > 0 && b % kGPrmBatches == 0 && tid < kGPrmBatches * kGroup) {
             const uint32_t bn = b + kGPrmBatches + uint32_t(tid / kGroup);
-            if (bn < nbatch)
-                params(bn, tid % kGroup, prmbuf + (bn % (2u * kGPrmBatches)) * kGroup + tid % kGroup);
+            params_block(bn, tid % kGroup, bn < nbatch);
         }
         __syncthreads();
         gi = gn;
@@ -1043,7 +1058,7 @@ static fbp_status exhaustive_group_launch(fbp_context ctx, const fbp_reconstruct
     // factors.  Sensor ranges shrink (more parts) to fit.
     auto lds_bytes = [&](uint32_t spp) {
         return size_t(2) * kGroup * a.W * sizeof(float2) +
-               size_t(2) * kGPrmBatches * kGroup * sizeof(GroupPair) +
+               size_t(2) * kGPrmBatches * (kGroup * sizeof(GroupPair) + sizeof(uint32_t)) +
                size_t(a.groups_per_chunk) * kGroup * sizeof(GroupLaser) + size_t(spp) * sizeof(float4) +
                size_t(a.T) * sizeof(float2);
     };
