@@ -383,6 +383,9 @@ __global__ void depth_reduce_kernel(const float* __restrict__ partial, uint32_t 
 constexpr int kDgPix = 32;                 // pixels per tile
 constexpr int kDgDep = 256;                // depth slots per tile (16 per lane)
 constexpr int kDgWin = 4 * 64 * kGVecD;    // staged window capacity (bins) per pair
+// Pair parameters are computed kDgPrmBatches batches at a time, by the first
+// kDgPrmBatches * kGroup threads, into one half of a two-block LDS ring.
+constexpr int kDgPrmBatches = 16;
 
 struct DepthGroupArgs {
     const float* data;  // I[l][s][t], F32
@@ -484,11 +487,15 @@ __global__ __launch_bounds__(1024) void depth_pixel_order_kernel(const fbp_vec3*
 template <bool COFF>
 __global__ __launch_bounds__(512, 1) void depth_group_kernel(DepthGroupArgs a) {
     extern __shared__ float smem[];
-    // [2][kGroup][kDgWin] staged samples, [3][kGroup] pair parameters, the
-    // chunk's lasers and sensors, box reduction scratch.
+    // [2][kGroup][kDgWin] staged samples, [2 * kDgPrmBatches][kGroup] pair
+    // parameters and per-batch masks, the chunk's lasers and sensors, box
+    // reduction scratch.
     float* winbuf = smem;
     DgPair* prmbuf = reinterpret_cast<DgPair*>(winbuf + 2 * kGroup * kDgWin);
-    DgLaser* lasl = reinterpret_cast<DgLaser*>(prmbuf + 3 * kGroup);
+    // Per batch slot: bits 0-7 pairs on the fast body, bits 8-15 pairs on
+    // the reference's chain.
+    uint32_t* bmask = reinterpret_cast<uint32_t*>(prmbuf + 2 * kDgPrmBatches * kGroup);
+    DgLaser* lasl = reinterpret_cast<DgLaser*>(bmask + 2 * kDgPrmBatches);
     float4* senl = reinterpret_cast<float4*>(lasl + a.groups_per_chunk * kGroup);
     float* red = reinterpret_cast<float*>(senl + a.sensors_per_part);  // [6][8]
 
@@ -620,19 +627,39 @@ __global__ __launch_bounds__(512, 1) void depth_group_kernel(DepthGroupArgs a) {
         P.fast = (fits && !near) ? 1 : 0;
         *out = P;
     };
-    // Raw samples of batch b: wave g fetches pair g's bins kmin4 + 4 j .. + 3.
-    auto fetch = [&](uint32_t b, const DgPair* prm, float4 (&reg)[kGVecD]) {
+    // Parameters of batch bn's pair g for the threads of a block (lanes
+    // 8j .. 8j + 7 of a wave hold one batch; every lane of the wave takes
+    // part, with valid = false past the last batch) and the batch's masks.
+    auto params_block = [&](uint32_t bn, int g, bool valid) {
+        int fast = -1;
+        if (valid) {
+            DgPair* o = prmbuf + (bn % (2u * kDgPrmBatches)) * kGroup + g;
+            params(bn, g, o);
+            fast = o->fast;
+        }
+        const uint64_t fm = __ballot(fast == 1), em = __ballot(fast == 0);
+        const int sh = lane & ~(kGroup - 1);
+        if (valid && g == 0)
+            bmask[bn % (2u * kDgPrmBatches)] =
+                uint32_t((fm >> sh) & 0xffu) | (uint32_t((em >> sh) & 0xffu) << 8);
+    };
+    // Raw samples of batch (gi, si): wave g fetches pair g's bins kmin4 + 4 j
+    // .. + 3 from the pair's row (a scalar address); lanes whose bins lie
+    // outside [0, T) read bins inside the row (the store zeroes them), and a
+    // slot without a staged pair reads row 0.
+    auto fetch = [&](uint32_t gi, uint32_t si, const DgPair* prm, float4 (&reg)[kGVecD]) {
         const int fast = rfl(prm[wave].fast);
         const int kmin4 = rfl(prm[wave].kmin4);
-        const uint32_t grp = g_begin + b / ns;
-        const uint32_t s = s_begin + b % ns;
-        const uint64_t pair = uint64_t(grp * kGroup + uint32_t(wave)) * a.S + s;
+        const uint32_t grp = g_begin + gi;
+        const uint32_t s = s_begin + si;
+        const uint64_t pair =
+            (fits && fast >= 0) ? uint64_t(grp * kGroup + uint32_t(wave)) * a.S + s : 0u;
+        const float* row = a.data + pair * a.T;
 #pragma unroll
         for (int i = 0; i < kGVecD; ++i) {
-            const int j = lane + 64 * i;
-            const int k0 = kmin4 + 4 * j;
-            const bool ok = fits && fast >= 0 && k0 >= 0 && k0 < int(a.T);
-            reg[i] = *reinterpret_cast<const float4*>(a.data + (ok ? pair * a.T + uint32_t(k0) : 0));
+            const int k0 = kmin4 + 4 * (lane + 64 * i);
+            const uint32_t kc = uint32_t(min(max(k0, 0), int(a.T) - 4));
+            reg[i] = *reinterpret_cast<const float4*>(row + kc);
         }
     };
     auto store = [&](float* win, const DgPair* prm, const float4 (&reg)[kGVecD]) {
@@ -645,10 +672,8 @@ __global__ __launch_bounds__(512, 1) void depth_group_kernel(DepthGroupArgs a) {
             const int k0 = kmin4 + 4 * j;
             const bool ok = k0 >= 0 && k0 < int(a.T);
             const float4 r = reg[i];
-            float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (ok)
-                e = make_float4(r.x > kEps ? r.x : 0.f, r.y > kEps ? r.y : 0.f,
-                                r.z > kEps ? r.z : 0.f, r.w > kEps ? r.w : 0.f);
+            const float4 e = make_float4(ok && r.x > kEps ? r.x : 0.f, ok && r.y > kEps ? r.y : 0.f,
+                                         ok && r.z > kEps ? r.z : 0.f, ok && r.w > kEps ? r.w : 0.f);
             *reinterpret_cast<float4*>(win + wave * kDgWin + 4 * j) = e;
         }
     };
@@ -665,20 +690,29 @@ __global__ __launch_bounds__(512, 1) void depth_group_kernel(DepthGroupArgs a) {
     const int per = kGroup * kDgWin;
     float4 reg[kGVecD];
     if (nbatch > 0) {
-        if (tid < kGroup) params(0, tid, prmbuf + tid);
-        if (tid >= kGroup && tid < 2 * kGroup && nbatch > 1) params(1, tid - kGroup, prmbuf + tid);
+        // Parameters of the first two blocks of batches.
+        if (tid < 2 * kDgPrmBatches * kGroup)
+            params_block(uint32_t(tid / kGroup), tid % kGroup, uint32_t(tid / kGroup) < nbatch);
         __syncthreads();
-        fetch(0, prmbuf, reg);
+        fetch(0, 0, prmbuf, reg);
         store(winbuf, prmbuf, reg);
         __syncthreads();
     }
+    // Batch b = gi * ns + si (laser group gi of the chunk, sensor si of the
+    // part), kept as running counters.
+    uint32_t gi = 0, si = 0;
     for (uint32_t b = 0; b < nbatch; ++b) {
         const int cur = int(b & 1u);
-        const DgPair* prm = prmbuf + (b % 3u) * kGroup;
+        const DgPair* prm = prmbuf + (b % (2u * kDgPrmBatches)) * kGroup;
         const bool has_next = b + 1 < nbatch;
-        const DgPair* prm_next = prmbuf + ((b + 1) % 3u) * kGroup;
-        if (has_next) fetch(b + 1, prm_next, reg);
-        if (b % ns == 0) {
+        const DgPair* prm_next = prmbuf + ((b + 1) % (2u * kDgPrmBatches)) * kGroup;
+        uint32_t gn = gi, sn = si + 1;
+        if (sn == ns) {
+            sn = 0;
+            ++gn;
+        }
+        if (has_next) fetch(gn, sn, prm_next, reg);
+        if (si == 0) {
 #pragma unroll
             for (int g = 0; g < kGroup; ++g) {
                 const float XL2 = sq(rn_sub(px, rfl(prm[g].lx)));
@@ -697,13 +731,9 @@ __global__ __launch_bounds__(512, 1) void depth_group_kernel(DepthGroupArgs a) {
             for (int v = 0; v < 16; ++v)
                 rs[v] = sqrt_rn_normal(dot3(XS2, sq(rn_sub(py[v], syp)), ZS2));
         }
-        int fast_pairs = 0, exact_pairs = 0;
-#pragma unroll
-        for (int g = 0; g < kGroup; ++g) {
-            const int fg = rfl(prm[g].fast);
-            fast_pairs |= (fg == 1 ? 1 : 0) << g;
-            exact_pairs |= (fg == 0 ? 1 : 0) << g;
-        }
+        const uint32_t masks = uint32_t(rfl(int(bmask[b % (2u * kDgPrmBatches)])));
+        const int fast_pairs = int(masks & 0xffu);
+        int exact_pairs = int(masks >> 8);
         auto fast_body = [&](int g, int base, float off) {
 #pragma unroll
             for (int h = 0; h < 16; h += 8) {
@@ -742,8 +772,8 @@ __global__ __launch_bounds__(512, 1) void depth_group_kernel(DepthGroupArgs a) {
             const DgPair& P = prm[g];
             const float lx = rfl(P.lx), ly = rfl(P.ly), lz = rfl(P.lz), off = rfl(P.off);
             const int base = rfl(P.base) + cur * per;
-            const uint64_t pair = uint64_t((g_begin + b / ns) * kGroup + uint32_t(g)) * a.S +
-                                  (s_begin + b % ns);
+            const uint64_t pair = uint64_t((g_begin + gi) * kGroup + uint32_t(g)) * a.S +
+                                  (s_begin + si);
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
                 const float sl = dot3(sq(rn_sub(px, lx)), sq(rn_sub(py[v], ly)), sq(rn_sub(pz, lz)));
@@ -763,12 +793,17 @@ __global__ __launch_bounds__(512, 1) void depth_group_kernel(DepthGroupArgs a) {
                 }
             }
         }
-        if (has_next) {
-            store(winbuf + (cur ^ 1) * per, prm_next, reg);
-            if (tid < kGroup && b + 2 < nbatch)
-                params(b + 2, tid, prmbuf + ((b + 2) % 3u) * kGroup + tid);
+        if (has_next) store(winbuf + (cur ^ 1) * per, prm_next, reg);
+        // First batch of a block: the block after next goes to the slots of
+        // the block before, whose last batch ended with the previous barrier
+        // (and is first needed by the fetch at the end of the next block).
+        if (b > 0 && b % kDgPrmBatches == 0 && tid < kDgPrmBatches * kGroup) {
+            const uint32_t bn = b + kDgPrmBatches + uint32_t(tid / kGroup);
+            params_block(bn, tid % kGroup, bn < nbatch);
         }
         __syncthreads();
+        gi = gn;
+        si = sn;
     }
     if (pix_ok) {
         float* out = a.partial + size_t(chunk) * a.L * a.D + size_t(c) * a.D;
@@ -793,7 +828,7 @@ fbp_status depth_group_launch(fbp_context ctx, const fbp_reconstruction_info* in
     a.T = info->num_time_bins;
     a.D = num_depths;
     if (info->relay_wall_normal.x != 0.f || info->relay_wall_normal.z != 0.f || a.L < uint32_t(kGroup) ||
-        a.L > 4096u || a.S < 64u || (a.T & 3u) != 0u ||
+        a.L > 4096u || a.S < 64u || a.T < 4u || (a.T & 3u) != 0u ||
         (reinterpret_cast<uintptr_t>(tr->data) % 16u) != 0u)
         return FBP_OK;
     if (!group_quotient_cached(ctx, info->time_step, a.T)) return FBP_OK;
@@ -825,7 +860,8 @@ fbp_status depth_group_launch(fbp_context ctx, const fbp_reconstruction_info* in
     parts = std::min<uint64_t>(parts, std::max<uint64_t>(1, a.S / 64));
     a.sensors_per_part = uint32_t((a.S + parts - 1) / parts);
     auto lds_bytes = [&](uint32_t spp) {
-        return size_t(2) * kGroup * kDgWin * sizeof(float) + 3 * kGroup * sizeof(DgPair) +
+        return size_t(2) * kGroup * kDgWin * sizeof(float) +
+               size_t(2) * kDgPrmBatches * (kGroup * sizeof(DgPair) + sizeof(uint32_t)) +
                size_t(a.groups_per_chunk) * kGroup * sizeof(DgLaser) + size_t(spp) * sizeof(float4) +
                48 * sizeof(float);
     };

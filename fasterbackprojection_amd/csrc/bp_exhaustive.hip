@@ -1013,7 +1013,7 @@ static fbp_status exhaustive_group_launch(fbp_context ctx, const fbp_reconstruct
     // 4-bin loads: T a multiple of 4 and the samples 16-byte (F32) or 8-byte
     // (F16/BF16) aligned.
     const uintptr_t align = tr->storage == FBP_STORAGE_F32 ? 16u : 8u;
-    if (a.L < uint32_t(kGroup) || a.S < 64u || a.T > 8192u || (a.T & 3u) != 0u ||
+    if (a.L < uint32_t(kGroup) || a.S < 64u || a.T < 4u || a.T > 8192u || (a.T & 3u) != 0u ||
         a.W > 4 * 64 * kGVec || (reinterpret_cast<uintptr_t>(tr->data) % align) != 0u)
         return FBP_OK;
     // The quotient check depends on (dt, T) only: kept for the next call.
