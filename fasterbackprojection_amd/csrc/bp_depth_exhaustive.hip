@@ -368,10 +368,11 @@ __global__ void depth_reduce_kernel(const float* __restrict__ partial, uint32_t 
 // Laser-group form of the exhaustive depth planes (the laser-group voxel
 // kernel's scheme on the depth-plane points): for a wall normal along y the
 // point p = laser[c] + n * depth[d] keeps the pixel's x and z, so a lane holds
-// one pixel and 16 depths (x, z and 16 y values).  A workgroup of eight waves
-// owns a tile of 32 pixels (consecutive in a Morton order of the pixels on
-// the wall, so a tile is a compact block) x 256 depth slots and a range of
-// (group of 8 lasers, sensor) batches.  Each lane keeps the eight exact laser
+// 4 pixels x 4 depths (4 x and z values, 16 y values; the 64 lanes of a wave
+// take 64 consecutive depths of one pixel per slot).  A workgroup of eight
+// waves owns a tile of 32 pixels (consecutive in a Morton order of the
+// pixels on the wall, so a tile is a compact block) x 256 depth slots and a
+// range of (group of 8 lasers, sensor) batches.  Each lane keeps the eight exact laser
 // distances of its 16 points while the group is unchanged and computes its
 // 16 exact sensor distances once per sensor; the update is the reference's
 // dist = fl(rl + rs) (- t0, + off), the two-term quotient (exact at every
@@ -508,30 +509,41 @@ __global__ __launch_bounds__(512, 1) void depth_group_kernel(DepthGroupArgs a) {
     const uint32_t tile = bid % a.num_tiles, chunk = bid / a.num_tiles;
     const uint32_t tp = tile % a.ptiles, td = tile / a.ptiles;
 
-    // The lane's pixel and 16 depths.
-    const uint32_t pi = tp * kDgPix + uint32_t(tid % kDgPix);
-    const bool pix_ok = pi < a.L;
-    const uint32_t c = a.perm[pix_ok ? pi : 0u];
-    const uint32_t d0 = td * kDgDep + uint32_t(tid / kDgPix) * 16u;
-    const float px = a.laser[c].x, pz = a.laser[c].z, lcy = a.laser[c].y;
-    float py[16];
+    // Wave w holds pixels 4w .. 4w + 3 of the tile; the lane's slot v is
+    // pixel j = v / 4 at depth dl + 64 (v % 4), so the 64 lanes of a wave
+    // take 64 consecutive depths of one pixel at a time (neighbouring bins:
+    // window reads with few bank conflicts).  Pixels past the last one
+    // repeat the tile's first pixel (inside the tile's box), depths past the
+    // last one repeat the last depth; neither is written out.
+    const uint32_t dl = td * kDgDep + uint32_t(lane);
+    uint32_t cj[4];
+    float px[4], pz[4], py[16];
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-        const uint32_t d = min(d0 + uint32_t(v), a.D - 1);
-        py[v] = lcy + a.ny * static_cast<float>(a.depths[d]);
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t pi = tp * kDgPix + 4u * uint32_t(wave) + uint32_t(j);
+        cj[j] = a.perm[pi < a.L ? pi : tp * kDgPix];
+        px[j] = a.laser[cj[j]].x;
+        pz[j] = a.laser[cj[j]].z;
+        const float lcy = a.laser[cj[j]].y;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t d = min(dl + 64u * uint32_t(u), a.D - 1);
+            py[4 * j + u] = lcy + a.ny * static_cast<float>(a.depths[d]);
+        }
     }
+    const uint32_t pix0 = tp * kDgPix + 4u * uint32_t(wave);
     // Box of the tile's points.
     float b[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
-    if (pix_ok && d0 < a.D) {
-        b[0] = b[1] = px;
-        b[4] = b[5] = pz;
 #pragma unroll
-        for (int v = 0; v < 16; ++v)
-            if (d0 + uint32_t(v) < a.D) {
-                b[2] = fminf(b[2], py[v]);
-                b[3] = fmaxf(b[3], py[v]);
-            }
-    }
+    for (int v = 0; v < 16; ++v)
+        if (pix0 + uint32_t(v / 4) < a.L && dl + 64u * uint32_t(v % 4) < a.D) {
+            b[0] = fminf(b[0], px[v / 4]);
+            b[1] = fmaxf(b[1], px[v / 4]);
+            b[2] = fminf(b[2], py[v]);
+            b[3] = fmaxf(b[3], py[v]);
+            b[4] = fminf(b[4], pz[v / 4]);
+            b[5] = fmaxf(b[5], pz[v / 4]);
+        }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1)
 #pragma unroll
@@ -715,21 +727,30 @@ __global__ __launch_bounds__(512, 1) void depth_group_kernel(DepthGroupArgs a) {
         if (si == 0) {
 #pragma unroll
             for (int g = 0; g < kGroup; ++g) {
-                const float XL2 = sq(rn_sub(px, rfl(prm[g].lx)));
-                const float ZL2 = sq(rn_sub(pz, rfl(prm[g].lz)));
-                const float lyg = rfl(prm[g].ly);
+                const float lxg = rfl(prm[g].lx), lyg = rfl(prm[g].ly), lzg = rfl(prm[g].lz);
+                float XL2[4], ZL2[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    XL2[j] = sq(rn_sub(px[j], lxg));
+                    ZL2[j] = sq(rn_sub(pz[j], lzg));
+                }
 #pragma unroll
                 for (int v = 0; v < 16; ++v)
-                    rl[g][v] = rn_sqrt(dot3(XL2, sq(rn_sub(py[v], lyg)), ZL2));
+                    rl[g][v] = rn_sqrt(dot3(XL2[v / 4], sq(rn_sub(py[v], lyg)), ZL2[v / 4]));
             }
         }
         float rs[16];
         const float sxp = rfl(prm[0].sx), syp = rfl(prm[0].sy), szp = rfl(prm[0].sz);
         {
-            const float XS2 = sq(rn_sub(px, sxp)), ZS2 = sq(rn_sub(pz, szp));
+            float XS2[4], ZS2[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                XS2[j] = sq(rn_sub(px[j], sxp));
+                ZS2[j] = sq(rn_sub(pz[j], szp));
+            }
 #pragma unroll
             for (int v = 0; v < 16; ++v)
-                rs[v] = sqrt_rn_normal(dot3(XS2, sq(rn_sub(py[v], syp)), ZS2));
+                rs[v] = sqrt_rn_normal(dot3(XS2[v / 4], sq(rn_sub(py[v], syp)), ZS2[v / 4]));
         }
         const uint32_t masks = uint32_t(rfl(int(bmask[b % (2u * kDgPrmBatches)])));
         const int fast_pairs = int(masks & 0xffu);
@@ -776,9 +797,10 @@ __global__ __launch_bounds__(512, 1) void depth_group_kernel(DepthGroupArgs a) {
                                   (s_begin + si);
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                const float sl = dot3(sq(rn_sub(px, lx)), sq(rn_sub(py[v], ly)), sq(rn_sub(pz, lz)));
+                const float qx = px[v / 4], qz = pz[v / 4];
+                const float sl = dot3(sq(rn_sub(qx, lx)), sq(rn_sub(py[v], ly)), sq(rn_sub(qz, lz)));
                 const float ss =
-                    dot3(sq(rn_sub(px, sxp)), sq(rn_sub(py[v], syp)), sq(rn_sub(pz, szp)));
+                    dot3(sq(rn_sub(qx, sxp)), sq(rn_sub(py[v], syp)), sq(rn_sub(qz, szp)));
                 const float de = exhaustive_dist(sl, ss, a.t0, a.discard, off);
                 const int k = bin_of(de, a.dt);
                 if (k >= 0 && k < int(a.T)) {
@@ -805,11 +827,11 @@ __global__ __launch_bounds__(512, 1) void depth_group_kernel(DepthGroupArgs a) {
         gi = gn;
         si = sn;
     }
-    if (pix_ok) {
-        float* out = a.partial + size_t(chunk) * a.L * a.D + size_t(c) * a.D;
+    float* out = a.partial + size_t(chunk) * a.L * a.D;
 #pragma unroll
-        for (int v = 0; v < 16; ++v)
-            if (d0 + uint32_t(v) < a.D) out[d0 + v] = acc[v];
+    for (int v = 0; v < 16; ++v) {
+        const uint32_t d = dl + 64u * uint32_t(v % 4);
+        if (pix0 + uint32_t(v / 4) < a.L && d < a.D) out[size_t(cj[v / 4]) * a.D + d] = acc[v];
     }
 }
 
