@@ -39,8 +39,8 @@ VALU_LANE_SLOTS = 39.3e12      # 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (non-pac
 # (profiles/counters_confocal_exhaustive_lct_bench.md; 2.37 = measured
 # v_fma / v_rsq throughput ratio on this GPU).
 GENERAL_SLOTS = 14.9           # confocal_voxel_kernel
-EXH_SLOTS = 10.65              # exhaustive_group_kernel (profiles/exhaustive_group_c3_bench_counters.md)
-DEPTH_EXH_SLOTS = 12.36        # depth_group_kernel (profiles/exhaustive_group_c3_bench_counters.md)
+EXH_SLOTS = 9.38               # exhaustive_group_kernel (profiles/exhaustive_group_c3_bench_counters.md)
+DEPTH_EXH_SLOTS = 10.91        # depth_group_kernel (profiles/exhaustive_group_c3_bench_counters.md)
 TRAFFIC_PROFILE = ROOT / "profiles" / "confocal_128cube_traffic.json"
 
 
@@ -837,7 +837,7 @@ def exhaustive_leg(ctx, args, reps=3):
     # 2 add, sqrt) + |p - s| (cached per sensor) - t0, / dt, + sample = 13.
     depth_tflops = 13 * depth_updates / (t_depth * 1e-3) / 1e12
     # Issue slots per counted pair-update of the depth-plane group kernel from
-    # its counters (12.1 VALU, 0.16 transcendental; 200 of its 256 depth slots
+    # its counters (10.7 VALU, 0.16 transcendental; 200 of its 256 depth slots
     # per tile hold planes).
     depth_slot_bound = VALU_LANE_SLOTS / DEPTH_EXH_SLOTS
     depth_check = None
@@ -870,7 +870,7 @@ def exhaustive_leg(ctx, args, reps=3):
     tflops = 25 * updates / (bp * 1e-3) / 1e12
     # Issue slots per pair-update of the laser-group kernel measured by
     # counters over the whole kernel (profiles/exhaustive_group_c3_bench_counters.md):
-    # 10.5 VALU instructions, 0.13 of them transcendental (the sensor square roots) at 2.37 slots.
+    # 9.2 VALU instructions, 0.13 of them transcendental (the sensor square roots) at 2.37 slots.
     slot_bound = VALU_LANE_SLOTS / EXH_SLOTS
     out = {"workload": "exhaustive 16^2 lasers x 64^2 sensors x 1024 (16 ps, 4 GiB) -> 64^3 "
                        "+ Laplacian(k=5)",
