@@ -331,6 +331,34 @@ __global__ void lct_resample_planes_kernel(const float* __restrict__ in, float* 
     pad[i] = acc;
 }
 
+// lct_resample_planes_kernel for N a multiple of 4: four consecutive x per
+// thread (16-byte loads and stores), plane z = blockIdx.y; the same sums in
+// the same order.
+__global__ __launch_bounds__(256) void lct_resample_planes4_kernel(
+    const float* __restrict__ in, float* __restrict__ pad, const int* __restrict__ rowptr,
+    const int* __restrict__ cols, const float* __restrict__ vals, uint32_t N) {
+    const uint32_t z = blockIdx.y;
+    const uint32_t w4 = N / 2;  // 16-byte groups per padded row of 2N
+    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+    if (q >= 2u * N * w4) return;
+    const uint32_t y = q / w4, x = (q % w4) * 4u;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    const size_t NN = size_t(N) * N;
+    if (y < N && x < N) {
+        const size_t p = size_t(y) * N + x;
+        const int e1 = rowptr[z + 1];
+        for (int e = rowptr[z]; e < e1; ++e) {
+            const float w = vals[e];
+            const float4 v = *reinterpret_cast<const float4*>(in + size_t(cols[e]) * NN + p);
+            acc.x = fmaf(w, v.x, acc.x);
+            acc.y = fmaf(w, v.y, acc.y);
+            acc.z = fmaf(w, v.z, acc.z);
+            acc.w = fmaf(w, v.w, acc.w);
+        }
+    }
+    *reinterpret_cast<float4*>(pad + size_t(z) * 4 * NN + size_t(q) * 4) = acc;
+}
+
 // ----------------------------------------------------------- LCT z-pass
 // For every (ky, kx) column of the half spectrum S[z][ky][kx] of the
 // planes z < M (2D transforms of the padded planes), in one workgroup:
@@ -485,14 +513,21 @@ __device__ __forceinline__ float2 w2048(const float2* twl, uint32_t m) {
 __global__ __launch_bounds__(kZ2048Threads) void lct_zpass2048_kernel(float2* __restrict__ S,
                                                                     const float2* __restrict__ invt,
                                                                     const float2* __restrict__ tw,
-                                                                    uint32_t N) {
+                                                                    uint32_t N, uint32_t gx) {
     extern __shared__ float2 lds[];
     float2* twl = lds;          // 1024 twiddles
     float2* buf = lds + 1024;   // [2048][kLctCols] + padding
     const uint32_t tid = threadIdx.x;
     const uint32_t col = tid % kLctCols, u = tid / kLctCols;  // u in [0, kZU)
     const uint32_t H = N + 1, Y2 = 2 * N;
-    const uint32_t kx0 = blockIdx.x * kLctCols, ky = blockIdx.y, kx = kx0 + col;
+    // Blocks are dealt to the XCDs round-robin; each XCD takes a contiguous
+    // range of (ky, column block) in order, so the two column blocks that
+    // share a 128-byte line of a plane run one after the other on one XCD.
+    const uint32_t nb = gridDim.x, nb8 = nb / 8u;
+    const uint32_t bid = blockIdx.x < nb8 * 8u ? (blockIdx.x % 8u) * nb8 + blockIdx.x / 8u
+                                               : blockIdx.x;
+    const uint32_t bx = bid % gx;
+    const uint32_t kx0 = bx * kLctCols, ky = bid / gx, kx = kx0 + col;
     const bool live = kx < H;
     auto at = [&](uint32_t pos) -> float2& { return buf[pos * kLctCols + col + 4u * (pos >> 3)]; };
     for (uint32_t k = tid; k < 1024; k += kZ2048Threads) twl[k] = tw[k];
@@ -500,7 +535,7 @@ __global__ __launch_bounds__(kZ2048Threads) void lct_zpass2048_kernel(float2* __
     // contiguous 8 KB block per (h, k2) for the workgroup), loaded now so
     // that their latency overlaps the forward passes.
     float2 fil[256 / kZU][8];
-    const float2* ft = invt + (size_t(ky) * gridDim.x + blockIdx.x) * (256 / kZU) * 8 * kZ2048Threads;
+    const float2* ft = invt + (size_t(ky) * gx + bx) * (256 / kZU) * 8 * kZ2048Threads;
 #pragma unroll
     for (int h = 0; h < 256 / kZU; ++h)
 #pragma unroll
@@ -620,6 +655,35 @@ __global__ void lct_final_kernel(const float* __restrict__ pad, float* __restric
     for (int e = rowptr[r]; e < rowptr[r + 1]; ++e)
         acc = fmaf(vals[e], pad[(size_t(cols[e]) * 2 * N + y) * 2 * N + x], acc);
     out[i] = fmaxf(acc * norm, 0.f);
+}
+
+// lct_final_kernel for N a multiple of 4: four consecutive x per thread,
+// output plane r = blockIdx.y; the same sums in the same order.
+__global__ __launch_bounds__(256) void lct_final4_kernel(const float* __restrict__ pad,
+                                                         float* __restrict__ out,
+                                                         const int* __restrict__ rowptr,
+                                                         const int* __restrict__ cols,
+                                                         const float* __restrict__ vals, uint32_t N,
+                                                         float norm) {
+    const uint32_t r = blockIdx.y;
+    const uint32_t n4 = N / 4;
+    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+    if (q >= N * n4) return;
+    const uint32_t y = q / n4, x = (q % n4) * 4u;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int e1 = rowptr[r + 1];
+    for (int e = rowptr[r]; e < e1; ++e) {
+        const float w = vals[e];
+        const float4 v =
+            *reinterpret_cast<const float4*>(pad + (size_t(cols[e]) * 2 * N + y) * 2 * N + x);
+        acc.x = fmaf(w, v.x, acc.x);
+        acc.y = fmaf(w, v.y, acc.y);
+        acc.z = fmaf(w, v.z, acc.z);
+        acc.w = fmaf(w, v.w, acc.w);
+    }
+    *reinterpret_cast<float4*>(out + size_t(r) * N * N + size_t(q) * 4) =
+        make_float4(fmaxf(acc.x * norm, 0.f), fmaxf(acc.y * norm, 0.f), fmaxf(acc.z * norm, 0.f),
+                    fmaxf(acc.w * norm, 0.f));
 }
 
 // psf (define_psf, python/lct.py:295-347) in FP64: for every (y, x) column the
@@ -743,9 +807,10 @@ fbp_status upload(fbp_context ctx, const std::vector<T>& v, T** dst, hipStream_t
 }
 
 void free_csr(Csr& c) {
-    if (c.rowptr) hipFree(c.rowptr);
-    if (c.cols) hipFree(c.cols);
-    if (c.vals) hipFree(c.vals);
+    // Teardown: nothing to report a failed free to.
+    if (c.rowptr) (void)hipFree(c.rowptr);
+    if (c.cols) (void)hipFree(c.cols);
+    if (c.vals) (void)hipFree(c.vals);
     c = Csr{};
 }
 
@@ -756,11 +821,11 @@ void fft_cache_destroy(fbp_context ctx) {
     FftCache* c = static_cast<FftCache*>(ctx->fft_cache);
     for (auto& kv : c->plans) rocfft_plan_destroy(kv.second.plan);
     if (c->info) rocfft_execution_info_destroy(c->info);
-    if (c->work) hipFree(c->work);
-    if (c->khat) hipFree(c->khat);
-    if (c->invpsf) hipFree(c->invpsf);
-    if (c->invt) hipFree(c->invt);
-    if (c->tw) hipFree(c->tw);
+    if (c->work) (void)hipFree(c->work);
+    if (c->khat) (void)hipFree(c->khat);
+    if (c->invpsf) (void)hipFree(c->invpsf);
+    if (c->invt) (void)hipFree(c->invt);
+    if (c->tw) (void)hipFree(c->tw);
     free_csr(c->mtx);
     free_csr(c->mtxi);
     delete c;
@@ -957,14 +1022,24 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
                            1.0 / double(p->snr), p->is_backprop);
         FBP_CHECK_HIP(ctx, hipGetLastError());
         FBP_CHECK_HIP(ctx, hipStreamSynchronize(s));
-        hipFree(psf);
-        hipFree(F);
-        hipFree(cnt);
+        FBP_CHECK_HIP(ctx, hipFree(psf));
+        FBP_CHECK_HIP(ctx, hipFree(F));
+        FBP_CHECK_HIP(ctx, hipFree(cnt));
         c->inv_key = keybuf;
     }
 
     const dim3 tgrid((M + 31) / 32, (N + 31) / 32, N);
     const float norm = float(1.0 / double(pad_n));
+    // Crop + inverse resampling + max(., 0) into out: 16-byte stores when
+    // the rows allow them and out is 16-byte aligned.
+    auto launch_final = [&](const float* padded) {
+        if (N % 4 == 0 && M <= 65535u && reinterpret_cast<uintptr_t>(out) % 16u == 0)
+            hipLaunchKernelGGL(lct_final4_kernel, dim3((N * N / 4 + 255) / 256, M), dim3(256), 0, s,
+                               padded, out, c->mtxi.rowptr, c->mtxi.cols, c->mtxi.vals, N, norm);
+        else
+            hipLaunchKernelGGL(lct_final_kernel, grid1(NN * M), dim3(256), 0, s, padded, out,
+                               c->mtxi.rowptr, c->mtxi.cols, c->mtxi.vals, N, M, norm);
+    };
     // Pruned path: the padded volume's upper M planes are zero and only its
     // first M output planes are kept, so the 2D transforms run on M planes
     // and one kernel does the z transforms and the filter in LDS.
@@ -993,8 +1068,12 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
         float2* spec = reinterpret_cast<float2*>(base + d_bytes + r_bytes);
         hipLaunchKernelGGL(lct_permute_kernel, tgrid, dim3(32, 8), 0, s, data, d, N, M, K, z_trim,
                            p->is_diffuse);
-        hipLaunchKernelGGL(lct_resample_planes_kernel, grid1(plane * M), dim3(256), 0, s, d, planes,
-                           c->mtx.rowptr, c->mtx.cols, c->mtx.vals, N, M);
+        if (N % 4 == 0)
+            hipLaunchKernelGGL(lct_resample_planes4_kernel, dim3((N * N + 255) / 256, M), dim3(256),
+                               0, s, d, planes, c->mtx.rowptr, c->mtx.cols, c->mtx.vals, N);
+        else
+            hipLaunchKernelGGL(lct_resample_planes_kernel, grid1(plane * M), dim3(256), 0, s, d,
+                               planes, c->mtx.rowptr, c->mtx.cols, c->mtx.vals, N, M);
         FBP_CHECK_HIP(ctx, hipGetLastError());
         FftPlan *pf = nullptr, *pi = nullptr;
         FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_forward, rocfft_precision_single,
@@ -1018,16 +1097,15 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
                                    c->invt, N, zgrid.x, nt);
                 FBP_CHECK_HIP(ctx, hipGetLastError());
             }
-            hipLaunchKernelGGL(lct_zpass2048_kernel, zgrid, dim3(kZ2048Threads), lds2048, s, spec,
-                               c->invt, c->tw, N);
+            hipLaunchKernelGGL(lct_zpass2048_kernel, dim3(zgrid.x * zgrid.y), dim3(kZ2048Threads),
+                               lds2048, s, spec, c->invt, c->tw, N, zgrid.x);
         } else {
             hipLaunchKernelGGL(lct_zpass_kernel, zgrid, dim3(kLctThreads), zpass_lds, s, spec,
                                c->invpsf, c->tw, N, M, lg);
         }
         FBP_CHECK_HIP(ctx, hipGetLastError());
         FBP_CHECK(run_plan(ctx, pi, spec, planes, s));
-        hipLaunchKernelGGL(lct_final_kernel, grid1(NN * M), dim3(256), 0, s, planes, out,
-                           c->mtxi.rowptr, c->mtxi.cols, c->mtxi.vals, N, M, norm);
+        launch_final(planes);
         FBP_CHECK_HIP(ctx, hipGetLastError());
         return FBP_OK;
     }
@@ -1055,8 +1133,7 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
     FBP_CHECK(run_plan(ctx, pf, padv, spec, s));
     hipLaunchKernelGGL(lct_filter_kernel, grid1(half_n), dim3(256), 0, s, spec, c->invpsf, half_n);
     FBP_CHECK(run_plan(ctx, pi, spec, padv, s));
-    hipLaunchKernelGGL(lct_final_kernel, grid1(NN * M), dim3(256), 0, s, padv, out, c->mtxi.rowptr,
-                       c->mtxi.cols, c->mtxi.vals, N, M, norm);
+    launch_final(padv);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     return FBP_OK;
 }
