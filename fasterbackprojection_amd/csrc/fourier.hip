@@ -54,6 +54,8 @@ struct FftCache {
     float2* tw = nullptr;
     // The inverse filter in the 2048-point z-pass's load order (invt_tile_kernel)
     float2* invt = nullptr;
+    // exp(-2 pi i k / 256), k < 256 (the 256-point y-pass)
+    float2* tw256 = nullptr;
 };
 
 std::once_flag g_rocfft_once;
@@ -257,8 +259,11 @@ __device__ inline float lct_tree_sum(const float* __restrict__ src, uint32_t K) 
 // Tiled transpose rect[x][y][t_in] -> d[t][y][x] (t_in = t * 2^K .. (t+1) *
 // 2^K - 1 summed by lct_tree_sum) with z-scaling and trimming; M is the
 // downsampled bin count.
+// inv_m1 = 1 / (M - 1) in double (the host's IEEE division, passed in so
+// that no thread divides).
 __global__ void lct_permute_kernel(const float* __restrict__ rect, float* __restrict__ d, uint32_t N,
-                                   uint32_t M, uint32_t K, uint32_t z_trim, int power4) {
+                                   uint32_t M, uint32_t K, uint32_t z_trim, int power4,
+                                   double inv_m1) {
     __shared__ float tile[32][33];
     const uint32_t y = blockIdx.z;
     const uint32_t t0 = blockIdx.x * 32, x0 = blockIdx.y * 32;
@@ -277,7 +282,7 @@ __global__ void lct_permute_kernel(const float* __restrict__ rect, float* __rest
             float v = tile[threadIdx.x][r];
             if (t < z_trim) v = 0.f;
             // grid_z = linspace(0, 1, M)[t]
-            const double gz = (M > 1) ? (t == M - 1 ? 1.0 : double(t) * (1.0 / double(M - 1))) : 0.0;
+            const double gz = (M > 1) ? (t == M - 1 ? 1.0 : double(t) * inv_m1) : 0.0;
             const double g2 = gz * gz;
             v = float(double(v) * (power4 ? g2 * g2 : g2));
             d[(size_t(t) * N + y) * N + x] = v;
@@ -632,6 +637,47 @@ __global__ __launch_bounds__(kZ2048Threads) void lct_zpass2048_kernel(float2* __
     }
 }
 
+// ----------------------------------------------------------- LCT y-pass
+// Length-256 DFT along ky of the half spectrum S[z][ky][kx] (row pitch H
+// complex, planes of 256 rows), in place, for 2N = 256: one workgroup per
+// (plane, 16 kx columns), 256 = 16 x 16 with x[n], n = 16 n1 + n2:
+//   A[n2][k1] = sum_n1 x[16 n1 + n2] W16^(n1 k1)     (thread n2)
+//   X[k1 + 16 k2] = sum_n2 A[n2][k1] W256^(n2 k1) W16^(n2 k2)   (thread k1)
+// W = exp(-2 pi i / n); INV: the conjugate, unnormalised (as rocFFT's
+// inverse).  The rows of 16 columns are 128-byte pieces of the planes.
+constexpr int kYCols = 16;
+template <bool INV>
+__global__ __launch_bounds__(256) void lct_ypass256_kernel(float2* __restrict__ S,
+                                                           const float2* __restrict__ tw256,
+                                                           uint32_t H) {
+    __shared__ float2 twl[256];
+    __shared__ float2 buf[16][16 * kYCols + 4];  // [n2][k1 * 16 + column], padded rows
+    const uint32_t t = threadIdx.x, col = t % kYCols, u = t / kYCols;
+    const uint32_t kx = blockIdx.x * kYCols + col;
+    const bool live = kx < H;
+    float2* P = S + size_t(blockIdx.y) * 256 * H;
+    twl[t] = tw256[t];
+    float2 v[16];
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1)
+        v[n1] = live ? P[size_t(16 * n1 + int(u)) * H + kx] : make_float2(0.f, 0.f);
+    dft_reg<16, INV>(v);
+    __syncthreads();  // twiddles
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) {
+        float2 w = twl[u * uint32_t(k1)];  // u k1 < 256
+        if (INV) w.y = -w.y;
+        buf[u][k1 * kYCols + col] = k1 == 0 ? v[0] : cmul(v[k1], w);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) v[n2] = buf[n2][u * kYCols + col];
+    dft_reg<16, INV>(v);
+    if (live)
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) P[size_t(u + 16u * uint32_t(k2)) * H + kx] = v[k2];
+}
+
 __global__ void lct_filter_kernel(float2* __restrict__ C, const float2* __restrict__ inv, size_t n) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -826,6 +872,7 @@ void fft_cache_destroy(fbp_context ctx) {
     if (c->invpsf) (void)hipFree(c->invpsf);
     if (c->invt) (void)hipFree(c->invt);
     if (c->tw) (void)hipFree(c->tw);
+    if (c->tw256) (void)hipFree(c->tw256);
     free_csr(c->mtx);
     free_csr(c->mtxi);
     delete c;
@@ -1029,6 +1076,7 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
     }
 
     const dim3 tgrid((M + 31) / 32, (N + 31) / 32, N);
+    const double inv_m1 = M > 1 ? 1.0 / double(M - 1) : 0.0;
     const float norm = float(1.0 / double(pad_n));
     // Crop + inverse resampling + max(., 0) into out: 16-byte stores when
     // the rows allow them and out is 16-byte aligned.
@@ -1067,7 +1115,7 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
         float* planes = reinterpret_cast<float*>(base + d_bytes);
         float2* spec = reinterpret_cast<float2*>(base + d_bytes + r_bytes);
         hipLaunchKernelGGL(lct_permute_kernel, tgrid, dim3(32, 8), 0, s, data, d, N, M, K, z_trim,
-                           p->is_diffuse);
+                           p->is_diffuse, inv_m1);
         if (N % 4 == 0)
             hipLaunchKernelGGL(lct_resample_planes4_kernel, dim3((N * N + 255) / 256, M), dim3(256),
                                0, s, d, planes, c->mtx.rowptr, c->mtx.cols, c->mtx.vals, N);
@@ -1075,12 +1123,36 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
             hipLaunchKernelGGL(lct_resample_planes_kernel, grid1(plane * M), dim3(256), 0, s, d,
                                planes, c->mtx.rowptr, c->mtx.cols, c->mtx.vals, N, M);
         FBP_CHECK_HIP(ctx, hipGetLastError());
+        // 2D transforms of the M planes: for 2N = 256, rocFFT's batched 1D
+        // R2C / C2R along x and the y-pass kernel along ky (its 128-byte
+        // pieces instead of the strided column kernel of the 2D plans).
+        const bool ysplit = 2 * N == 256;
+        const dim3 ygrid((N + 1 + kYCols - 1) / kYCols, M);
         FftPlan *pf = nullptr, *pi = nullptr;
-        FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_forward, rocfft_precision_single,
-                           {2 * N, 2 * N}, M, &pf));
-        FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_inverse, rocfft_precision_single,
-                           {2 * N, 2 * N}, M, &pi));
-        FBP_CHECK(run_plan(ctx, pf, planes, spec, s));
+        if (ysplit) {
+            if (!c->tw256) {
+                std::vector<float2> w(256);
+                for (int k = 0; k < 256; ++k) {
+                    const double a = -2.0 * 3.14159265358979323846 * double(k) / 256.0;
+                    w[k] = make_float2(float(std::cos(a)), float(std::sin(a)));
+                }
+                FBP_CHECK(upload(ctx, w, &c->tw256, s));
+            }
+            FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_forward, rocfft_precision_single,
+                               {2 * N}, size_t(2 * N) * M, &pf));
+            FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_inverse, rocfft_precision_single,
+                               {2 * N}, size_t(2 * N) * M, &pi));
+            FBP_CHECK(run_plan(ctx, pf, planes, spec, s));
+            hipLaunchKernelGGL(lct_ypass256_kernel<false>, ygrid, dim3(256), 0, s, spec, c->tw256,
+                               N + 1);
+            FBP_CHECK_HIP(ctx, hipGetLastError());
+        } else {
+            FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_forward, rocfft_precision_single,
+                               {2 * N, 2 * N}, M, &pf));
+            FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_inverse, rocfft_precision_single,
+                               {2 * N, 2 * N}, M, &pi));
+            FBP_CHECK(run_plan(ctx, pf, planes, spec, s));
+        }
         FBP_CHECK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(lct_zpass_kernel),
                                                hipFuncAttributeMaxDynamicSharedMemorySize,
                                                int(zpass_lds)));
@@ -1104,6 +1176,11 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
                                c->invpsf, c->tw, N, M, lg);
         }
         FBP_CHECK_HIP(ctx, hipGetLastError());
+        if (ysplit) {
+            hipLaunchKernelGGL(lct_ypass256_kernel<true>, ygrid, dim3(256), 0, s, spec, c->tw256,
+                               N + 1);
+            FBP_CHECK_HIP(ctx, hipGetLastError());
+        }
         FBP_CHECK(run_plan(ctx, pi, spec, planes, s));
         launch_final(planes);
         FBP_CHECK_HIP(ctx, hipGetLastError());
@@ -1120,7 +1197,7 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
     float2* spec = reinterpret_cast<float2*>(base + d_bytes + r_bytes);
 
     hipLaunchKernelGGL(lct_permute_kernel, tgrid, dim3(32, 8), 0, s, data, d, N, M, K, z_trim,
-                       p->is_diffuse);
+                       p->is_diffuse, inv_m1);
     hipLaunchKernelGGL(fill_kernel, grid1(pad_n), dim3(256), 0, s, padv, pad_n);
     hipLaunchKernelGGL(lct_resample_kernel, grid1(NN * M), dim3(256), 0, s, d, padv, c->mtx.rowptr,
                        c->mtx.cols, c->mtx.vals, N, M, 1);

@@ -52,6 +52,9 @@ def test_log_fft(ctx, shape, sigma):
                                                          (16, 64, True, False, 5),
                                                          (8, 32, False, True, 3),
                                                          (32, 128, False, False, 10),
+                                                         # 2N = 256: rocFFT 1D passes along x
+                                                         # and the y-pass kernel
+                                                         (128, 64, True, False, 3),
                                                          # 2M > 2048: the full 3D transform path
                                                          (4, 2048, False, False, 0)])
 def test_lct_random(ctx, N, M, diffuse, backprop, z_trim):
