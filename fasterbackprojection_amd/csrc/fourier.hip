@@ -70,14 +70,18 @@ fbp_status fft_fail(fbp_context ctx, rocfft_status st, const char* what) {
     return fail(ctx, FBP_ERR_FFT, "%s: rocfft status %d", what, int(st));
 }
 
+// hdist != 0 (1D real transforms only): the Hermitian side's batch distance
+// in complex elements (rows padded to hdist), the real side's distance the
+// length.
 fbp_status get_plan(fbp_context ctx, rocfft_transform_type type, rocfft_precision prec,
-                    std::vector<size_t> lengths, size_t batch, FftPlan** out) {
+                    std::vector<size_t> lengths, size_t batch, FftPlan** out, size_t hdist = 0) {
     std::call_once(g_rocfft_once, [] { g_rocfft_setup = rocfft_setup(); });
     if (g_rocfft_setup != rocfft_status_success) return fft_fail(ctx, g_rocfft_setup, "rocfft_setup");
     FftCache* c = cache_of(ctx);
     std::string key = std::to_string(int(type)) + "/" + std::to_string(int(prec)) + "/" +
                       std::to_string(batch);
     for (size_t l : lengths) key += "x" + std::to_string(l);
+    if (hdist) key += "/h" + std::to_string(hdist);
     auto it = c->plans.find(key);
     if (it == c->plans.end()) {
         if (c->plans.size() >= kMaxPlans) {
@@ -89,8 +93,26 @@ fbp_status get_plan(fbp_context ctx, rocfft_transform_type type, rocfft_precisio
             c->plans.erase(lru);
         }
         FftPlan p;
-        rocfft_status st = rocfft_plan_create(&p.plan, rocfft_placement_notinplace, type, prec,
-                                              lengths.size(), lengths.data(), batch, nullptr);
+        rocfft_plan_description desc = nullptr;
+        rocfft_status st = rocfft_status_success;
+        if (hdist) {
+            if (lengths.size() != 1) return fft_fail(ctx, rocfft_status_invalid_arg_value, "hdist");
+            const bool fwd = type == rocfft_transform_type_real_forward;
+            const size_t one = 1, rdist = lengths[0];
+            st = rocfft_plan_description_create(&desc);
+            if (st != rocfft_status_success) return fft_fail(ctx, st, "rocfft_plan_description_create");
+            st = rocfft_plan_description_set_data_layout(
+                desc, fwd ? rocfft_array_type_real : rocfft_array_type_hermitian_interleaved,
+                fwd ? rocfft_array_type_hermitian_interleaved : rocfft_array_type_real, nullptr,
+                nullptr, 1, &one, fwd ? rdist : hdist, 1, &one, fwd ? hdist : rdist);
+            if (st != rocfft_status_success) {
+                rocfft_plan_description_destroy(desc);
+                return fft_fail(ctx, st, "rocfft_plan_description_set_data_layout");
+            }
+        }
+        st = rocfft_plan_create(&p.plan, rocfft_placement_notinplace, type, prec, lengths.size(),
+                                lengths.data(), batch, desc);
+        if (desc) rocfft_plan_description_destroy(desc);
         if (st != rocfft_status_success) return fft_fail(ctx, st, "rocfft_plan_create");
         st = rocfft_plan_get_work_buffer_size(p.plan, &p.work);
         if (st != rocfft_status_success) return fft_fail(ctx, st, "rocfft_plan_get_work_buffer_size");
@@ -393,7 +415,7 @@ __global__ __launch_bounds__(kLctThreads) void lct_zpass_kernel(float2* __restri
                                                                 const float2* __restrict__ inv,
                                                                 const float2* __restrict__ tw,
                                                                 uint32_t N, uint32_t M,
-                                                                uint32_t lg) {
+                                                                uint32_t lg, uint32_t sp) {
     extern __shared__ float2 lds[];
     float2* twl = lds;           // M twiddles
     float2* buf = lds + M;       // [2M][kLctCols]
@@ -405,7 +427,7 @@ __global__ __launch_bounds__(kLctThreads) void lct_zpass_kernel(float2* __restri
         const uint32_t i = idx / kLctCols, c = idx % kLctCols;
         const uint32_t z = bitrev(i, lg), kx = kx0 + c;
         float2 v = make_float2(0.f, 0.f);
-        if (z < M && kx < H) v = S[(size_t(z) * Y2 + ky) * H + kx];
+        if (z < M && kx < H) v = S[(size_t(z) * Y2 + ky) * sp + kx];
         buf[idx] = v;
     }
     // Forward, decimation in time: butterflies of span h = 2^s.
@@ -445,7 +467,7 @@ __global__ __launch_bounds__(kLctThreads) void lct_zpass_kernel(float2* __restri
     __syncthreads();
     for (uint32_t idx = tid; idx < M * kLctCols; idx += kLctThreads) {
         const uint32_t z = idx / kLctCols, c = idx % kLctCols, kx = kx0 + c;
-        if (kx < H) S[(size_t(z) * Y2 + ky) * H + kx] = buf[bitrev(z, lg) * kLctCols + c];
+        if (kx < H) S[(size_t(z) * Y2 + ky) * sp + kx] = buf[bitrev(z, lg) * kLctCols + c];
     }
 }
 
@@ -518,7 +540,8 @@ __device__ __forceinline__ float2 w2048(const float2* twl, uint32_t m) {
 __global__ __launch_bounds__(kZ2048Threads) void lct_zpass2048_kernel(float2* __restrict__ S,
                                                                     const float2* __restrict__ invt,
                                                                     const float2* __restrict__ tw,
-                                                                    uint32_t N, uint32_t gx) {
+                                                                    uint32_t N, uint32_t gx,
+                                                                    uint32_t sp) {
     extern __shared__ float2 lds[];
     float2* twl = lds;          // 1024 twiddles
     float2* buf = lds + 1024;   // [2048][kLctCols] + padding
@@ -555,7 +578,7 @@ __global__ __launch_bounds__(kZ2048Threads) void lct_zpass2048_kernel(float2* __
 #pragma unroll
         for (int b = 0; b < 16; ++b) {
             v[b] = make_float2(0.f, 0.f);
-            if (b < 8 && live) v[b] = S[(size_t(a + 128u * b) * Y2 + ky) * H + kx];
+            if (b < 8 && live) v[b] = S[(size_t(a + 128u * b) * Y2 + ky) * sp + kx];
         }
         dft_reg<16, false>(v);
 #pragma unroll
@@ -633,13 +656,13 @@ __global__ __launch_bounds__(kZ2048Threads) void lct_zpass2048_kernel(float2* __
         if (live)
 #pragma unroll
             for (int z2 = 0; z2 < 8; ++z2)
-                S[(size_t(z0 + 8u * z1 + 128u * z2) * Y2 + ky) * H + kx] = r2[h][z2];
+                S[(size_t(z0 + 8u * z1 + 128u * z2) * Y2 + ky) * sp + kx] = r2[h][z2];
     }
 }
 
 // ----------------------------------------------------------- LCT y-pass
-// Length-256 DFT along ky of the half spectrum S[z][ky][kx] (row pitch H
-// complex, planes of 256 rows), in place, for 2N = 256: one workgroup per
+// Length-256 DFT along ky of the half spectrum S[z][ky][kx] (kx < H, row
+// pitch sp complex, planes of 256 rows), in place, for 2N = 256: one workgroup per
 // (plane, 16 kx columns), 256 = 16 x 16 with x[n], n = 16 n1 + n2:
 //   A[n2][k1] = sum_n1 x[16 n1 + n2] W16^(n1 k1)     (thread n2)
 //   X[k1 + 16 k2] = sum_n2 A[n2][k1] W256^(n2 k1) W16^(n2 k2)   (thread k1)
@@ -649,18 +672,18 @@ constexpr int kYCols = 16;
 template <bool INV>
 __global__ __launch_bounds__(256) void lct_ypass256_kernel(float2* __restrict__ S,
                                                            const float2* __restrict__ tw256,
-                                                           uint32_t H) {
+                                                           uint32_t H, uint32_t sp) {
     __shared__ float2 twl[256];
     __shared__ float2 buf[16][16 * kYCols + 4];  // [n2][k1 * 16 + column], padded rows
     const uint32_t t = threadIdx.x, col = t % kYCols, u = t / kYCols;
     const uint32_t kx = blockIdx.x * kYCols + col;
     const bool live = kx < H;
-    float2* P = S + size_t(blockIdx.y) * 256 * H;
+    float2* P = S + size_t(blockIdx.y) * 256 * sp;
     twl[t] = tw256[t];
     float2 v[16];
 #pragma unroll
     for (int n1 = 0; n1 < 16; ++n1)
-        v[n1] = live ? P[size_t(16 * n1 + int(u)) * H + kx] : make_float2(0.f, 0.f);
+        v[n1] = live ? P[size_t(16 * n1 + int(u)) * sp + kx] : make_float2(0.f, 0.f);
     dft_reg<16, INV>(v);
     __syncthreads();  // twiddles
 #pragma unroll
@@ -675,7 +698,7 @@ __global__ __launch_bounds__(256) void lct_ypass256_kernel(float2* __restrict__ 
     dft_reg<16, INV>(v);
     if (live)
 #pragma unroll
-        for (int k2 = 0; k2 < 16; ++k2) P[size_t(u + 16u * uint32_t(k2)) * H + kx] = v[k2];
+        for (int k2 = 0; k2 < 16; ++k2) P[size_t(u + 16u * uint32_t(k2)) * sp + kx] = v[k2];
 }
 
 __global__ void lct_filter_kernel(float2* __restrict__ C, const float2* __restrict__ inv, size_t n) {
@@ -1106,7 +1129,11 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
             FBP_CHECK(upload(ctx, tw, &c->tw, s));
             c->tw_m = M;
         }
-        const size_t plane = size_t(2 * N) * (2 * N), hplane = size_t(2 * N) * (N + 1);
+        // Spectrum rows of N + 1 complex; for 2N = 256 (the y-pass form)
+        // padded to a multiple of 16 (128-byte aligned rows).
+        const bool ysplit = 2 * N == 256;
+        const uint32_t sp = ysplit ? ((N + 1 + 15) / 16) * 16 : N + 1;
+        const size_t plane = size_t(2 * N) * (2 * N), hplane = size_t(2 * N) * sp;
         const size_t d_bytes = (NN * M * sizeof(float) + 255) & ~size_t(255);
         const size_t r_bytes = (plane * M * sizeof(float) + 255) & ~size_t(255);
         FBP_CHECK(ensure_scratch2(ctx, d_bytes + r_bytes + hplane * M * sizeof(float2)));
@@ -1126,7 +1153,6 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
         // 2D transforms of the M planes: for 2N = 256, rocFFT's batched 1D
         // R2C / C2R along x and the y-pass kernel along ky (its 128-byte
         // pieces instead of the strided column kernel of the 2D plans).
-        const bool ysplit = 2 * N == 256;
         const dim3 ygrid((N + 1 + kYCols - 1) / kYCols, M);
         FftPlan *pf = nullptr, *pi = nullptr;
         if (ysplit) {
@@ -1139,12 +1165,12 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
                 FBP_CHECK(upload(ctx, w, &c->tw256, s));
             }
             FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_forward, rocfft_precision_single,
-                               {2 * N}, size_t(2 * N) * M, &pf));
+                               {2 * N}, size_t(2 * N) * M, &pf, sp));
             FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_inverse, rocfft_precision_single,
-                               {2 * N}, size_t(2 * N) * M, &pi));
+                               {2 * N}, size_t(2 * N) * M, &pi, sp));
             FBP_CHECK(run_plan(ctx, pf, planes, spec, s));
             hipLaunchKernelGGL(lct_ypass256_kernel<false>, ygrid, dim3(256), 0, s, spec, c->tw256,
-                               N + 1);
+                               N + 1, sp);
             FBP_CHECK_HIP(ctx, hipGetLastError());
         } else {
             FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_forward, rocfft_precision_single,
@@ -1170,15 +1196,15 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
                 FBP_CHECK_HIP(ctx, hipGetLastError());
             }
             hipLaunchKernelGGL(lct_zpass2048_kernel, dim3(zgrid.x * zgrid.y), dim3(kZ2048Threads),
-                               lds2048, s, spec, c->invt, c->tw, N, zgrid.x);
+                               lds2048, s, spec, c->invt, c->tw, N, zgrid.x, sp);
         } else {
             hipLaunchKernelGGL(lct_zpass_kernel, zgrid, dim3(kLctThreads), zpass_lds, s, spec,
-                               c->invpsf, c->tw, N, M, lg);
+                               c->invpsf, c->tw, N, M, lg, sp);
         }
         FBP_CHECK_HIP(ctx, hipGetLastError());
         if (ysplit) {
             hipLaunchKernelGGL(lct_ypass256_kernel<true>, ygrid, dim3(256), 0, s, spec, c->tw256,
-                               N + 1);
+                               N + 1, sp);
             FBP_CHECK_HIP(ctx, hipGetLastError());
         }
         FBP_CHECK(run_plan(ctx, pi, spec, planes, s));
