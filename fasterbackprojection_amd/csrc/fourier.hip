@@ -70,18 +70,14 @@ fbp_status fft_fail(fbp_context ctx, rocfft_status st, const char* what) {
     return fail(ctx, FBP_ERR_FFT, "%s: rocfft status %d", what, int(st));
 }
 
-// hdist != 0 (1D real transforms only): the Hermitian side's batch distance
-// in complex elements (rows padded to hdist), the real side's distance the
-// length.
 fbp_status get_plan(fbp_context ctx, rocfft_transform_type type, rocfft_precision prec,
-                    std::vector<size_t> lengths, size_t batch, FftPlan** out, size_t hdist = 0) {
+                    std::vector<size_t> lengths, size_t batch, FftPlan** out) {
     std::call_once(g_rocfft_once, [] { g_rocfft_setup = rocfft_setup(); });
     if (g_rocfft_setup != rocfft_status_success) return fft_fail(ctx, g_rocfft_setup, "rocfft_setup");
     FftCache* c = cache_of(ctx);
     std::string key = std::to_string(int(type)) + "/" + std::to_string(int(prec)) + "/" +
                       std::to_string(batch);
     for (size_t l : lengths) key += "x" + std::to_string(l);
-    if (hdist) key += "/h" + std::to_string(hdist);
     auto it = c->plans.find(key);
     if (it == c->plans.end()) {
         if (c->plans.size() >= kMaxPlans) {
@@ -93,26 +89,8 @@ fbp_status get_plan(fbp_context ctx, rocfft_transform_type type, rocfft_precisio
             c->plans.erase(lru);
         }
         FftPlan p;
-        rocfft_plan_description desc = nullptr;
-        rocfft_status st = rocfft_status_success;
-        if (hdist) {
-            if (lengths.size() != 1) return fft_fail(ctx, rocfft_status_invalid_arg_value, "hdist");
-            const bool fwd = type == rocfft_transform_type_real_forward;
-            const size_t one = 1, rdist = lengths[0];
-            st = rocfft_plan_description_create(&desc);
-            if (st != rocfft_status_success) return fft_fail(ctx, st, "rocfft_plan_description_create");
-            st = rocfft_plan_description_set_data_layout(
-                desc, fwd ? rocfft_array_type_real : rocfft_array_type_hermitian_interleaved,
-                fwd ? rocfft_array_type_hermitian_interleaved : rocfft_array_type_real, nullptr,
-                nullptr, 1, &one, fwd ? rdist : hdist, 1, &one, fwd ? hdist : rdist);
-            if (st != rocfft_status_success) {
-                rocfft_plan_description_destroy(desc);
-                return fft_fail(ctx, st, "rocfft_plan_description_set_data_layout");
-            }
-        }
-        st = rocfft_plan_create(&p.plan, rocfft_placement_notinplace, type, prec, lengths.size(),
-                                lengths.data(), batch, desc);
-        if (desc) rocfft_plan_description_destroy(desc);
+        rocfft_status st = rocfft_plan_create(&p.plan, rocfft_placement_notinplace, type, prec,
+                                              lengths.size(), lengths.data(), batch, nullptr);
         if (st != rocfft_status_success) return fft_fail(ctx, st, "rocfft_plan_create");
         st = rocfft_plan_get_work_buffer_size(p.plan, &p.work);
         if (st != rocfft_status_success) return fft_fail(ctx, st, "rocfft_plan_get_work_buffer_size");
@@ -669,7 +647,9 @@ __global__ __launch_bounds__(kZ2048Threads) void lct_zpass2048_kernel(float2* __
 // W = exp(-2 pi i / n); INV: the conjugate, unnormalised (as rocFFT's
 // inverse).  The rows of 16 columns are 128-byte pieces of the planes.
 constexpr int kYCols = 16;
-template <bool INV>
+// PRUNE: forward, rows ky >= 128 of the input are zero (not read);
+// inverse, only output rows < 128 are written (the crop keeps y < N = 128).
+template <bool INV, bool PRUNE>
 __global__ __launch_bounds__(256) void lct_ypass256_kernel(float2* __restrict__ S,
                                                            const float2* __restrict__ tw256,
                                                            uint32_t H, uint32_t sp) {
@@ -683,7 +663,8 @@ __global__ __launch_bounds__(256) void lct_ypass256_kernel(float2* __restrict__ 
     float2 v[16];
 #pragma unroll
     for (int n1 = 0; n1 < 16; ++n1)
-        v[n1] = live ? P[size_t(16 * n1 + int(u)) * sp + kx] : make_float2(0.f, 0.f);
+        v[n1] = live && !(PRUNE && !INV && n1 >= 8) ? P[size_t(16 * n1 + int(u)) * sp + kx]
+                                                    : make_float2(0.f, 0.f);
     dft_reg<16, INV>(v);
     __syncthreads();  // twiddles
 #pragma unroll
@@ -698,7 +679,101 @@ __global__ __launch_bounds__(256) void lct_ypass256_kernel(float2* __restrict__ 
     dft_reg<16, INV>(v);
     if (live)
 #pragma unroll
-        for (int k2 = 0; k2 < 16; ++k2) P[size_t(u + 16u * uint32_t(k2)) * sp + kx] = v[k2];
+        for (int k2 = 0; k2 < (PRUNE && INV ? 8 : 16); ++k2)
+            P[size_t(u + 16u * uint32_t(k2)) * sp + kx] = v[k2];
+}
+
+// Forward transform along x of the resampled planes, fused with the
+// resampling, for N = 128 (2N = 256): row y < 128 of plane z is
+// x[n] = sum_e vals[e] in[cols[e]][y][n] for n < 128 (the resampling's sums
+// in its order) and 0 for 128 <= n < 256; its 256-point DFT X[k], k <= 128,
+// goes to spectrum row (z, y) (pitch sp).  Rows y >= 128 are zero and are
+// not written (the pruned y-pass does not read them).  Sixteen rows per
+// workgroup, sixteen threads per row: the y-pass's 16 x 16 scheme along x.
+__global__ __launch_bounds__(256) void lct_xr2c256_kernel(const float* __restrict__ in,
+                                                          float2* __restrict__ S,
+                                                          const int* __restrict__ rowptr,
+                                                          const int* __restrict__ cols,
+                                                          const float* __restrict__ vals,
+                                                          const float2* __restrict__ tw256,
+                                                          uint32_t sp) {
+    constexpr uint32_t N = 128;
+    __shared__ float2 twl[256];
+    __shared__ float2 buf[16][16 * 17];  // [row][n2 * 17 + k1]: conflict-free both ways
+    const uint32_t t = threadIdx.x, r = t / 16u, u = t % 16u;
+    const uint32_t y = blockIdx.x * 16u + r, z = blockIdx.y;
+    twl[t] = tw256[t];
+    const size_t NN = size_t(N) * N;
+    const float* row = in + size_t(y) * N;
+    float acc[8];
+#pragma unroll
+    for (int n1 = 0; n1 < 8; ++n1) acc[n1] = 0.f;
+    const int e1 = rowptr[z + 1];
+    for (int e = rowptr[z]; e < e1; ++e) {
+        const float w = vals[e];
+        const float* src = row + size_t(cols[e]) * NN + u;
+#pragma unroll
+        for (int n1 = 0; n1 < 8; ++n1) acc[n1] = fmaf(w, src[16 * n1], acc[n1]);
+    }
+    float2 v[16];
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1) v[n1] = make_float2(n1 < 8 ? acc[n1] : 0.f, 0.f);
+    dft_reg<16, false>(v);
+    __syncthreads();  // twiddles
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1)
+        buf[r][u * 17u + uint32_t(k1)] = k1 == 0 ? v[0] : cmul(v[k1], twl[u * uint32_t(k1)]);
+    __syncthreads();
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) v[n2] = buf[r][uint32_t(n2) * 17u + u];
+    dft_reg<16, false>(v);
+    float2* dst = S + (size_t(z) * 256 + y) * sp;
+#pragma unroll
+    for (int k2 = 0; k2 <= 8; ++k2)
+        if (u + 16u * uint32_t(k2) <= N) dst[u + 16u * uint32_t(k2)] = v[k2];
+}
+
+// Inverse transform along x for N = 128, cropped: spectrum row (z, y),
+// y < 128, holds X[k], k <= 128, of a real row; its unnormalised 256-point
+// inverse DFT (X[256 - k] = conj X[k]) at n < 128 goes to
+// out[z][y][n] (planes of 128 x 128).
+__global__ __launch_bounds__(256) void lct_xc2r256_kernel(const float2* __restrict__ S,
+                                                          float* __restrict__ out,
+                                                          const float2* __restrict__ tw256,
+                                                          uint32_t sp) {
+    constexpr uint32_t N = 128;
+    __shared__ float2 twl[256];
+    __shared__ float2 buf[16][16 * 17];  // [row][j2 * 17 + m1]
+    const uint32_t t = threadIdx.x, r = t / 16u, u = t % 16u;
+    const uint32_t y = blockIdx.x * 16u + r, z = blockIdx.y;
+    twl[t] = tw256[t];
+    const float2* src = S + (size_t(z) * 256 + y) * sp;
+    float2 v[16];
+#pragma unroll
+    for (int j1 = 0; j1 < 16; ++j1) {
+        const uint32_t k = 16u * uint32_t(j1) + u;
+        if (k <= N) {
+            v[j1] = src[k];
+        } else {
+            const float2 c = src[256u - k];
+            v[j1] = make_float2(c.x, -c.y);
+        }
+    }
+    dft_reg<16, true>(v);
+    __syncthreads();  // twiddles
+#pragma unroll
+    for (int m1 = 0; m1 < 16; ++m1) {
+        float2 w = twl[u * uint32_t(m1)];
+        w.y = -w.y;
+        buf[r][u * 17u + uint32_t(m1)] = m1 == 0 ? v[0] : cmul(v[m1], w);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j2 = 0; j2 < 16; ++j2) v[j2] = buf[r][uint32_t(j2) * 17u + u];
+    dft_reg<16, true>(v);
+    float* dst = out + (size_t(z) * N + y) * N;
+#pragma unroll
+    for (int m2 = 0; m2 < 8; ++m2) dst[u + 16u * uint32_t(m2)] = v[m2].x;
 }
 
 __global__ void lct_filter_kernel(float2* __restrict__ C, const float2* __restrict__ inv, size_t n) {
@@ -713,7 +788,7 @@ __global__ void lct_filter_kernel(float2* __restrict__ C, const float2* __restri
 __global__ void lct_final_kernel(const float* __restrict__ pad, float* __restrict__ out,
                                  const int* __restrict__ rowptr, const int* __restrict__ cols,
                                  const float* __restrict__ vals, uint32_t N, uint32_t M,
-                                 float norm) {
+                                 uint32_t pitch, float norm) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const size_t NN = size_t(N) * N;
     if (i >= NN * M) return;
@@ -722,18 +797,20 @@ __global__ void lct_final_kernel(const float* __restrict__ pad, float* __restric
     const size_t y = p / N, x = p % N;
     float acc = 0.f;
     for (int e = rowptr[r]; e < rowptr[r + 1]; ++e)
-        acc = fmaf(vals[e], pad[(size_t(cols[e]) * 2 * N + y) * 2 * N + x], acc);
+        acc = fmaf(vals[e], pad[(size_t(cols[e]) * pitch + y) * pitch + x], acc);
     out[i] = fmaxf(acc * norm, 0.f);
 }
 
 // lct_final_kernel for N a multiple of 4: four consecutive x per thread,
-// output plane r = blockIdx.y; the same sums in the same order.
+// output plane r = blockIdx.y, source planes of pitch x pitch (2N: the
+// padded planes; N: planes already cropped); the same sums in the same
+// order.
 __global__ __launch_bounds__(256) void lct_final4_kernel(const float* __restrict__ pad,
                                                          float* __restrict__ out,
                                                          const int* __restrict__ rowptr,
                                                          const int* __restrict__ cols,
                                                          const float* __restrict__ vals, uint32_t N,
-                                                         float norm) {
+                                                         uint32_t pitch, float norm) {
     const uint32_t r = blockIdx.y;
     const uint32_t n4 = N / 4;
     const uint32_t q = blockIdx.x * 256u + threadIdx.x;
@@ -744,7 +821,7 @@ __global__ __launch_bounds__(256) void lct_final4_kernel(const float* __restrict
     for (int e = rowptr[r]; e < e1; ++e) {
         const float w = vals[e];
         const float4 v =
-            *reinterpret_cast<const float4*>(pad + (size_t(cols[e]) * 2 * N + y) * 2 * N + x);
+            *reinterpret_cast<const float4*>(pad + (size_t(cols[e]) * pitch + y) * pitch + x);
         acc.x = fmaf(w, v.x, acc.x);
         acc.y = fmaf(w, v.y, acc.y);
         acc.z = fmaf(w, v.z, acc.z);
@@ -1103,13 +1180,15 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
     const float norm = float(1.0 / double(pad_n));
     // Crop + inverse resampling + max(., 0) into out: 16-byte stores when
     // the rows allow them and out is 16-byte aligned.
-    auto launch_final = [&](const float* padded) {
+    // pitch: row and plane edge of the source planes (2N padded, N cropped).
+    auto launch_final = [&](const float* src, uint32_t pitch) {
         if (N % 4 == 0 && M <= 65535u && reinterpret_cast<uintptr_t>(out) % 16u == 0)
             hipLaunchKernelGGL(lct_final4_kernel, dim3((N * N / 4 + 255) / 256, M), dim3(256), 0, s,
-                               padded, out, c->mtxi.rowptr, c->mtxi.cols, c->mtxi.vals, N, norm);
+                               src, out, c->mtxi.rowptr, c->mtxi.cols, c->mtxi.vals, N, pitch,
+                               norm);
         else
-            hipLaunchKernelGGL(lct_final_kernel, grid1(NN * M), dim3(256), 0, s, padded, out,
-                               c->mtxi.rowptr, c->mtxi.cols, c->mtxi.vals, N, M, norm);
+            hipLaunchKernelGGL(lct_final_kernel, grid1(NN * M), dim3(256), 0, s, src, out,
+                               c->mtxi.rowptr, c->mtxi.cols, c->mtxi.vals, N, M, pitch, norm);
     };
     // Pruned path: the padded volume's upper M planes are zero and only its
     // first M output planes are kept, so the 2D transforms run on M planes
@@ -1143,16 +1222,11 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
         float2* spec = reinterpret_cast<float2*>(base + d_bytes + r_bytes);
         hipLaunchKernelGGL(lct_permute_kernel, tgrid, dim3(32, 8), 0, s, data, d, N, M, K, z_trim,
                            p->is_diffuse, inv_m1);
-        if (N % 4 == 0)
-            hipLaunchKernelGGL(lct_resample_planes4_kernel, dim3((N * N + 255) / 256, M), dim3(256),
-                               0, s, d, planes, c->mtx.rowptr, c->mtx.cols, c->mtx.vals, N);
-        else
-            hipLaunchKernelGGL(lct_resample_planes_kernel, grid1(plane * M), dim3(256), 0, s, d,
-                               planes, c->mtx.rowptr, c->mtx.cols, c->mtx.vals, N, M);
-        FBP_CHECK_HIP(ctx, hipGetLastError());
-        // 2D transforms of the M planes: for 2N = 256, rocFFT's batched 1D
-        // R2C / C2R along x and the y-pass kernel along ky (its 128-byte
-        // pieces instead of the strided column kernel of the 2D plans).
+        // 2D transforms of the M planes.  For 2N = 256: the resampling fused
+        // with a 256-point x transform of the 128 non-zero rows of each plane
+        // (the padded planes are never written), and the y-pass kernel along
+        // ky on 128-byte row pieces, pruned to the non-zero input rows.
+        // Otherwise the resampled padded planes and rocFFT's 2D plans.
         const dim3 ygrid((N + 1 + kYCols - 1) / kYCols, M);
         FftPlan *pf = nullptr, *pi = nullptr;
         if (ysplit) {
@@ -1164,15 +1238,21 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
                 }
                 FBP_CHECK(upload(ctx, w, &c->tw256, s));
             }
-            FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_forward, rocfft_precision_single,
-                               {2 * N}, size_t(2 * N) * M, &pf, sp));
-            FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_inverse, rocfft_precision_single,
-                               {2 * N}, size_t(2 * N) * M, &pi, sp));
-            FBP_CHECK(run_plan(ctx, pf, planes, spec, s));
-            hipLaunchKernelGGL(lct_ypass256_kernel<false>, ygrid, dim3(256), 0, s, spec, c->tw256,
-                               N + 1, sp);
+            hipLaunchKernelGGL(lct_xr2c256_kernel, dim3(N / 16, M), dim3(256), 0, s, d, spec,
+                               c->mtx.rowptr, c->mtx.cols, c->mtx.vals, c->tw256, sp);
+            FBP_CHECK_HIP(ctx, hipGetLastError());
+            hipLaunchKernelGGL((lct_ypass256_kernel<false, true>), ygrid, dim3(256), 0, s, spec,
+                               c->tw256, N + 1, sp);
             FBP_CHECK_HIP(ctx, hipGetLastError());
         } else {
+            if (N % 4 == 0)
+                hipLaunchKernelGGL(lct_resample_planes4_kernel, dim3((N * N + 255) / 256, M),
+                                   dim3(256), 0, s, d, planes, c->mtx.rowptr, c->mtx.cols,
+                                   c->mtx.vals, N);
+            else
+                hipLaunchKernelGGL(lct_resample_planes_kernel, grid1(plane * M), dim3(256), 0, s,
+                                   d, planes, c->mtx.rowptr, c->mtx.cols, c->mtx.vals, N, M);
+            FBP_CHECK_HIP(ctx, hipGetLastError());
             FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_forward, rocfft_precision_single,
                                {2 * N, 2 * N}, M, &pf));
             FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_inverse, rocfft_precision_single,
@@ -1203,12 +1283,19 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
         }
         FBP_CHECK_HIP(ctx, hipGetLastError());
         if (ysplit) {
-            hipLaunchKernelGGL(lct_ypass256_kernel<true>, ygrid, dim3(256), 0, s, spec, c->tw256,
-                               N + 1, sp);
+            // Inverse y-pass keeping the rows y < N, the inverse x transform
+            // keeping x < N: cropped 128 x 128 planes.
+            hipLaunchKernelGGL((lct_ypass256_kernel<true, true>), ygrid, dim3(256), 0, s, spec,
+                               c->tw256, N + 1, sp);
             FBP_CHECK_HIP(ctx, hipGetLastError());
+            hipLaunchKernelGGL(lct_xc2r256_kernel, dim3(N / 16, M), dim3(256), 0, s, spec, planes,
+                               c->tw256, sp);
+            FBP_CHECK_HIP(ctx, hipGetLastError());
+            launch_final(planes, N);
+        } else {
+            FBP_CHECK(run_plan(ctx, pi, spec, planes, s));
+            launch_final(planes, 2 * N);
         }
-        FBP_CHECK(run_plan(ctx, pi, spec, planes, s));
-        launch_final(planes);
         FBP_CHECK_HIP(ctx, hipGetLastError());
         return FBP_OK;
     }
@@ -1236,7 +1323,7 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
     FBP_CHECK(run_plan(ctx, pf, padv, spec, s));
     hipLaunchKernelGGL(lct_filter_kernel, grid1(half_n), dim3(256), 0, s, spec, c->invpsf, half_n);
     FBP_CHECK(run_plan(ctx, pi, spec, padv, s));
-    launch_final(padv);
+    launch_final(padv, 2 * N);
     FBP_CHECK_HIP(ctx, hipGetLastError());
     return FBP_OK;
 }
