@@ -474,6 +474,19 @@ __device__ __forceinline__ float2 w16(int m) {
     return make_float2(c, INV ? sn : -sn);
 }
 
+// a W16^m (W16 = exp(-2 pi i / 16); INV: the conjugate); the quarter turns
+// m = 4, 8, 12 as the exact swaps and negations their products give.
+template <bool INV>
+__device__ __forceinline__ float2 mul_w16(float2 a, int m) {
+    switch (m & 15) {
+        case 0: return a;
+        case 4: return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+        case 8: return make_float2(-a.x, -a.y);
+        case 12: return INV ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+        default: return cmul(a, w16<INV>(m));
+    }
+}
+
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
@@ -498,7 +511,7 @@ __device__ __forceinline__ void dft_reg(float2 (&v)[N]) {
         for (int j = 0; j < N / 2; ++j) {
             const int pos = j & (h - 1);
             const int i0 = ((j >> s) << (s + 1)) | pos, i1 = i0 + h;
-            const float2 b = pos == 0 ? t[i1] : cmul(t[i1], w16<INV>(pos * (16 / (2 * h))));
+            const float2 b = mul_w16<INV>(t[i1], pos * (16 / (2 * h)));
             t[i1] = csub(t[i0], b);
             t[i0] = cadd(t[i0], b);
         }
