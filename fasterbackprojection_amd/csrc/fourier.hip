@@ -54,6 +54,10 @@ struct FftCache {
     float2* tw = nullptr;
     // The inverse filter in the 2048-point z-pass's load order (invt_tile_kernel)
     float2* invt = nullptr;
+    // The PSF is mirror-symmetric in y (psf[z][2N-1-y][x] == psf[z][y][x]
+    // bit for bit), so inv(kx, 2N - k) = exp(2 pi i k / 2N) inv(kx, k): the
+    // 2048-point z-pass keeps the tiles of rows ky <= N only.
+    int inv_sym = 0;
     // exp(-2 pi i k / 256), k < 256 (the 256-point y-pass)
     float2* tw256 = nullptr;
 };
@@ -532,7 +536,8 @@ __global__ __launch_bounds__(kZ2048Threads) void lct_zpass2048_kernel(float2* __
                                                                     const float2* __restrict__ invt,
                                                                     const float2* __restrict__ tw,
                                                                     uint32_t N, uint32_t gx,
-                                                                    uint32_t sp) {
+                                                                    uint32_t sp,
+                                                                    const float2* __restrict__ rotw) {
     extern __shared__ float2 lds[];
     float2* twl = lds;          // 1024 twiddles
     float2* buf = lds + 1024;   // [2048][kLctCols] + padding
@@ -545,8 +550,17 @@ __global__ __launch_bounds__(kZ2048Threads) void lct_zpass2048_kernel(float2* __
     const uint32_t nb = gridDim.x, nb8 = nb / 8u;
     const uint32_t bid = blockIdx.x < nb8 * 8u ? (blockIdx.x % 8u) * nb8 + blockIdx.x / 8u
                                                : blockIdx.x;
-    const uint32_t bx = bid % gx;
-    const uint32_t kx0 = bx * kLctCols, ky = bid / gx, kx = kx0 + col;
+    const uint32_t bx = bid % gx, seq = bid / gx;
+    // rotw != nullptr (2N = 256, mirror-symmetric PSF): rows ky and 2N - ky
+    // share the filter tile of row min(ky, 2N - ky) (the second times
+    // exp(2 pi i k / 2N), k = 2N - ky, = conj(rotw[k])); the sequence
+    // 0, N, 1, 2N - 1, 2, 2N - 2, ... puts each pair next to each other.
+    uint32_t ky = seq, kyt = seq;
+    if (rotw) {
+        ky = seq == 0 ? 0u : seq == 1 ? N : (seq % 2 == 0 ? seq / 2 : Y2 - (seq - 1) / 2);
+        kyt = ky <= N ? ky : Y2 - ky;
+    }
+    const uint32_t kx0 = bx * kLctCols, kx = kx0 + col;
     const bool live = kx < H;
     auto at = [&](uint32_t pos) -> float2& { return buf[pos * kLctCols + col + 4u * (pos >> 3)]; };
     for (uint32_t k = tid; k < 1024; k += kZ2048Threads) twl[k] = tw[k];
@@ -554,11 +568,19 @@ __global__ __launch_bounds__(kZ2048Threads) void lct_zpass2048_kernel(float2* __
     // contiguous 8 KB block per (h, k2) for the workgroup), loaded now so
     // that their latency overlaps the forward passes.
     float2 fil[256 / kZU][8];
-    const float2* ft = invt + (size_t(ky) * gx + bx) * (256 / kZU) * 8 * kZ2048Threads;
+    const float2* ft = invt + (size_t(kyt) * gx + bx) * (256 / kZU) * 8 * kZ2048Threads;
 #pragma unroll
     for (int h = 0; h < 256 / kZU; ++h)
 #pragma unroll
         for (int k2 = 0; k2 < 8; ++k2) fil[h][k2] = ft[(h * 8 + k2) * kZ2048Threads + tid];
+    if (ky != kyt) {
+        const float2 w = rotw[kyt];
+        const float2 rot = make_float2(w.x, -w.y);
+#pragma unroll
+        for (int h = 0; h < 256 / kZU; ++h)
+#pragma unroll
+            for (int k2 = 0; k2 < 8; ++k2) fil[h][k2] = cmul(fil[h][k2], rot);
+    }
     __syncthreads();
 
     // Forward pass 1: a = u; x[a + 128 b] = 0 for b >= 8.
@@ -885,6 +907,19 @@ __global__ void scale_d_kernel(double* __restrict__ p, size_t n,
     if (i < n) p[i] *= 1.0 / sqrt(double(*count));
 }
 
+// flag != 0 when the (rolled) PSF is not mirror-symmetric in y bit for bit:
+// psf[z][2U-1-y][x] != psf[z][y][x] for some element.
+__global__ void psf_sym_check_kernel(const double* __restrict__ psf, uint32_t U, size_t n,
+                                     int* __restrict__ flag) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;  // over [z][y < U][x]
+    if (i >= n) return;
+    const uint32_t n2u = 2 * U;
+    const size_t x = i % n2u, zy = i / n2u;
+    const size_t y = zy % U, z = zy / U;
+    const size_t a = (z * n2u + y) * n2u + x, b = (z * n2u + (n2u - 1 - y)) * n2u + x;
+    if (psf[a] != psf[b]) atomicOr(flag, 1);
+}
+
 // invpsf = conj(F) / (|F|^2 + 1/snr), or conj(F) (backprojection), FP64 -> c64.
 __global__ void invpsf_kernel(const double2* __restrict__ F, float2* __restrict__ inv, size_t n,
                               double inv_snr, int backprop) {
@@ -1170,9 +1205,15 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
         FBP_CHECK_HIP(ctx, hipMalloc((void**)&psf, pad_n * sizeof(double)));
         FBP_CHECK_HIP(ctx, hipMalloc((void**)&F, half_n * sizeof(double2)));
         FBP_CHECK_HIP(ctx, hipMalloc((void**)&cnt, sizeof(unsigned long long)));
+        int* symflag = nullptr;
+        FBP_CHECK_HIP(ctx, hipMalloc((void**)&symflag, sizeof(int)));
         FBP_CHECK_HIP(ctx, hipMemsetAsync(cnt, 0, sizeof(unsigned long long), s));
+        FBP_CHECK_HIP(ctx, hipMemsetAsync(symflag, 0, sizeof(int), s));
         hipLaunchKernelGGL(psf_kernel, grid1(size_t(2 * N) * 2 * N), dim3(256), 0, s, psf, N, M, slope, cnt);
         hipLaunchKernelGGL(scale_d_kernel, grid1(pad_n), dim3(256), 0, s, psf, pad_n, cnt);
+        hipLaunchKernelGGL(psf_sym_check_kernel, grid1(pad_n / 2), dim3(256), 0, s, psf, N,
+                           pad_n / 2, symflag);
+        int symhost = 1;
         FftPlan* pd = nullptr;
         FBP_CHECK(get_plan(ctx, rocfft_transform_type_real_forward, rocfft_precision_double,
                            {2 * N, 2 * N, 2 * M}, 1, &pd));
@@ -1181,10 +1222,13 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
         hipLaunchKernelGGL(invpsf_kernel, grid1(half_n), dim3(256), 0, s, F, c->invpsf, half_n,
                            1.0 / double(p->snr), p->is_backprop);
         FBP_CHECK_HIP(ctx, hipGetLastError());
+        FBP_CHECK_HIP(ctx, hipMemcpyAsync(&symhost, symflag, sizeof(int), hipMemcpyDeviceToHost, s));
         FBP_CHECK_HIP(ctx, hipStreamSynchronize(s));
         FBP_CHECK_HIP(ctx, hipFree(psf));
         FBP_CHECK_HIP(ctx, hipFree(F));
         FBP_CHECK_HIP(ctx, hipFree(cnt));
+        FBP_CHECK_HIP(ctx, hipFree(symflag));
+        c->inv_sym = symhost == 0 ? 1 : 0;
         c->inv_key = keybuf;
     }
 
@@ -1281,15 +1325,19 @@ fbp_status fbp_lct(fbp_context ctx, const float* data, const fbp_lct_params* p, 
             FBP_CHECK_HIP(ctx, hipFuncSetAttribute(
                                    reinterpret_cast<const void*>(lct_zpass2048_kernel),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(lds2048)));
+            // Mirror-symmetric PSF at 2N = 256: tiles of rows ky <= N only.
+            const bool zsym = ysplit && c->inv_sym;
             if (!c->invt) {
-                const size_t nt = size_t(zgrid.x) * zgrid.y * (256 / kZU) * 8 * kZ2048Threads;
+                const size_t nt =
+                    size_t(zgrid.x) * (zsym ? N + 1 : zgrid.y) * (256 / kZU) * 8 * kZ2048Threads;
                 FBP_CHECK_HIP(ctx, hipMalloc((void**)&c->invt, nt * sizeof(float2)));
                 hipLaunchKernelGGL(invt_tile_kernel, grid1(nt), dim3(256), 0, s, c->invpsf,
                                    c->invt, N, zgrid.x, nt);
                 FBP_CHECK_HIP(ctx, hipGetLastError());
             }
             hipLaunchKernelGGL(lct_zpass2048_kernel, dim3(zgrid.x * zgrid.y), dim3(kZ2048Threads),
-                               lds2048, s, spec, c->invt, c->tw, N, zgrid.x, sp);
+                               lds2048, s, spec, c->invt, c->tw, N, zgrid.x, sp,
+                               zsym ? c->tw256 : nullptr);
         } else {
             hipLaunchKernelGGL(lct_zpass_kernel, zgrid, dim3(kLctThreads), zpass_lds, s, spec,
                                c->invpsf, c->tw, N, M, lg, sp);
