@@ -197,44 +197,50 @@ __device__ __forceinline__ void confocal_voxel_body(const ConfocalArgs& a) {
             P[P_WL] = tile_l ? tile_w[lb + lane] : 1.f;
         }
         __syncthreads();
-        // Window staging: element f = jl * W + w of the batch.  Each lane
-        // issues the loads of kStage elements (the sample and the threshold
-        // of bin kmin + w; unused ones read element 0) before writing any.
-        const int total = nb * a.W;
-        for (int f0 = 0; f0 < total; f0 += 64 * kStage) {
+        // Window staging, one 64-bin block of one scan point per step: in
+        // step (jl, wb) lane w stages bin kmin + 64 wb + w of scan point jl
+        // (the sample and the threshold; lanes past the window, and bins
+        // outside [0, T), read element 0 of the row and store zeros).  The
+        // scan point's row, offsets and weight are uniform per step; each
+        // lane issues the loads of kStage steps before writing any.
+        const int nwb = (a.W + 63) / 64;
+        const int steps = nb * nwb;
+        for (int s0 = 0; s0 < steps; s0 += kStage) {
             float v[kStage], t[kStage];
+            int dst[kStage];
 #pragma unroll
             for (int u = 0; u < kStage; ++u) {
-                const int f = f0 + u * 64 + lane;
-                const bool in = f < total;
-                // f / W through the reciprocal: exact for f < 2^22.
-                const int jl = in ? int((float(f) + 0.5f) * a.inv_W) : 0;
-                const int w = f - jl * a.W;
+                const int st = s0 + u;
+                const bool live = st < steps;
+                const int jl = live ? st / nwb : 0;
+                const int w = (st - jl * nwb) * 64 + lane;
                 const float* P = prm + jl * kPrmWords;
-                const uint32_t l = __float_as_uint(P[P_L]);
-                const int k = jl * a.W - __float_as_int(P[P_BASE]) + w;
-                const bool ok = in && k >= 0 && k < int(a.T);
-                const size_t idx = !ok ? 0
-                                   : (LAYOUT == FBP_LAYOUT_TIME_FASTEST)
-                                       ? size_t(l) * a.T + uint32_t(k)
-                                       : size_t(uint32_t(k)) * a.L + l;
+                const uint32_t l = uint32_t(rfl(__float_as_int(P[P_L])));
+                const int kmin = jl * a.W - rfl(__float_as_int(P[P_BASE]));
+                const float wl = rfl(P[P_WL]);
+                const int k = kmin + w;
+                const bool inw = live && w < a.W;
+                const bool ok = inw && k >= 0 && k < int(a.T);
+                const uint32_t kc = ok ? uint32_t(k) : 0u;
+                const size_t idx = (LAYOUT == FBP_LAYOUT_TIME_FASTEST)
+                                       ? size_t(l) * a.T + kc
+                                       : size_t(kc) * a.L + l;
                 float sc = 1.f;
-                if constexpr (STORAGE != FBP_STORAGE_F32) sc = a.scales[ok ? l : 0];
+                if constexpr (STORAGE != FBP_STORAGE_F32) sc = a.scales[l];
                 float x = load_sample<STORAGE>(a.data, idx, sc);
                 // Quantised storage was thresholded before quantisation.
                 if constexpr (STORAGE == FBP_STORAGE_F32) x = (x > kEps) ? x : 0.f;
-                v[u] = ok ? x * P[P_WL] : 0.f;
+                v[u] = ok ? x * wl : 0.f;
                 // Threshold of bin k (only bins 0..T matter; outside, both
                 // candidate samples are zero).
                 const float* thr = a.thr + size_t(a.thr_rows > 1 ? l : 0) * (a.T + 1);
                 const float th = thr[min(max(k, 0), int(a.T))];
                 t[u] = (k < 0) ? 0.f : (k > int(a.T) ? __int_as_float(0x7f800000) : th);
+                dst[u] = inw ? jl * a.W + w : -1;
             }
 #pragma unroll
-            for (int u = 0; u < kStage; ++u) {
-                const int f = f0 + u * 64 + lane;
-                if (f < total) win[f] = make_float2(v[u], t[u] * tscale);
-            }
+            for (int u = 0; u < kStage; ++u)
+                if (dst[u] >= 0) win[dst[u]] = make_float2(v[u], t[u] * tscale);
         }
         __syncthreads();
         for (int jl = 0; jl < nb; ++jl) {
